@@ -1,0 +1,1144 @@
+/*
+ * flecsale_oracle.c -- CPU restatement of FleCSALE's two hydro time-step paths.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker (or as the timed CPU baseline), never as the product.
+ *
+ * It restates, loop by loop and in the same floating-point operation order,
+ *   apps/hydro/tasks.h            (Eulerian MUSCL step)
+ *   apps/maire_hydro/tasks.h      (cell-centred Lagrangian step)
+ *   flecsale/eqns/euler_eqns.h, flecsale/eqns/flux.h, flecsale/eqns/lagrange_eqns.h
+ *   flecsale/eos/ideal_gas.h, flecsale/linalg/qr.h + detail/qr_impl.h
+ * Every function cites the reference lines it follows.  Build with
+ * -ffp-contract=off and without -ffast-math: the reference's limiter and HLLE
+ * branches flip on one-ulp differences.
+ *
+ * The mesh layer (connectivity + geometry) replaces the un-vendored burton
+ * mesh; its conventions are build-defined where the reference is silent:
+ *   - faces are numbered in order of first visit, cell by cell, local edge k =
+ *     (v_k, v_{k+1}) in 2D, Exodus hex sides in 3D; face_cells[0] is the first
+ *     (lowest-id) cell, and the face normal points out of face_cells[0];
+ *   - cell_vertex_neighbors(c) lists the cells sharing a vertex with c in
+ *     ascending id, c excluded;
+ *   - 2D cell centroid/area: shoelace about the origin (pinned end-to-end by
+ *     apps/maire_hydro/2d/sedov_2d-solution_rank000000.txt.1procs.std);
+ *   - 3D hex: faces split in 4 triangles about the vertex average, cells in
+ *     24 tetrahedra about the vertex average;
+ *   - Maire corners ordered by ascending cell id around a vertex, wedges of a
+ *     corner ordered (edge v_{k-1}v_k, edge v_k v_{k+1}); wedge facet = half
+ *     edge: facet_area = |edge midpoint - vertex|, facet_normal = outward edge
+ *     unit normal (pinned end-to-end by the sedov_2d golden);
+ *   - min_length(c) = minimum distance between two vertices of c.
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+
+#define OM_OK 0
+#define OM_ERR_BAD_MESH 1
+#define OM_ERR_NEG_STATE 2      /* negative density or internal energy */
+#define OM_ERR_DT 3             /* non-positive inverse time step */
+#define OM_ERR_ALLOC 4
+
+/* std::min / std::max semantics (first argument wins ties / NaN order). */
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+/* ------------------------------------------------------------------------ */
+/* Mesh                                                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int dim, nc, nv, vpc, fpc, vpf, nf;
+  double *x;        /* [nv*dim] vertex coordinates                      */
+  int *cv;          /* [nc*vpc] cell vertices (Exodus order)            */
+  int *cf;          /* [nc*fpc] cell faces in local face order          */
+  int *fc;          /* [nf*2]   face cells, [1] = -1 on the boundary    */
+  int *fv;          /* [nf*vpf] face vertices, oriented out of fc[0]    */
+  int *nbr_off;     /* [nc+1]   cell-vertex neighbour CSR               */
+  int *nbr;
+  int *vc_off;      /* [nv+1]   vertex -> corner CSR (corner = c*vpc+k)  */
+  int *vc;
+  /* geometry */
+  double *ccen, *cvol, *cminlen;   /* [nc*dim], [nc], [nc] */
+  double *fcen, *fnrm, *farea;     /* [nf*dim], [nf*dim], [nf] */
+  /* boundary tags (Maire symmetry): bit0 = |n_x|~1, bit1 = |n_y|~1, bit2 = |n_z|~1 */
+  unsigned char *ftag;              /* [nf] */
+  unsigned char *vbnd;              /* [nv] vertex touches a boundary face */
+} om_mesh;
+
+/* Exodus hex8 sides, outward for a positively oriented hex. */
+static const int HEX_FACES[6][4] = {
+  {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+
+typedef struct { int k[4]; int face; } fkey;
+
+static int fkey_cmp(const void *a, const void *b) {
+  const fkey *x = (const fkey *)a, *y = (const fkey *)b;
+  for (int i = 0; i < 4; ++i) {
+    if (x->k[i] != y->k[i]) return x->k[i] < y->k[i] ? -1 : 1;
+  }
+  return 0;
+}
+
+static void sort_small(int *a, int n) {
+  for (int i = 1; i < n; ++i) {
+    int t = a[i], j = i - 1;
+    while (j >= 0 && a[j] > t) { a[j + 1] = a[j]; --j; }
+    a[j + 1] = t;
+  }
+}
+
+static int cmp_int(const void *a, const void *b) {
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+/* local face f of a cell -> local vertex list */
+static void local_face(const om_mesh *m, int lf, int *lv) {
+  if (m->dim == 2) { lv[0] = lf; lv[1] = (lf + 1) % m->vpc; }
+  else { for (int i = 0; i < 4; ++i) lv[i] = HEX_FACES[lf][i]; }
+}
+
+void om_mesh_destroy(om_mesh *m);
+
+/* 2D geometry: apps/hydro/tasks.h uses c->centroid()/volume(), f->centroid()/normal()/area().
+ * Cell centroid + area: shoelace about the origin (see header). */
+static void cell_geom_2d(const om_mesh *m, int c, double *cen, double *vol, double *minlen) {
+  const int n = m->vpc;
+  double a2 = 0.0, sx = 0.0, sy = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double *p = m->x + 2 * m->cv[c * n + i];
+    const double *q = m->x + 2 * m->cv[c * n + (i + 1) % n];
+    double cr = p[0] * q[1] - q[0] * p[1];
+    a2 += cr;
+    sx += (p[0] + q[0]) * cr;
+    sy += (p[1] + q[1]) * cr;
+  }
+  double area = 0.5 * a2;
+  cen[0] = sx / (6.0 * area);
+  cen[1] = sy / (6.0 * area);
+  *vol = area;
+  double ml = DBL_MAX;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      const double *p = m->x + 2 * m->cv[c * n + i];
+      const double *q = m->x + 2 * m->cv[c * n + j];
+      double dx = q[0] - p[0], dy = q[1] - p[1];
+      double l = sqrt(dx * dx + dy * dy);
+      ml = smin(ml, l);
+    }
+  *minlen = ml;
+}
+
+static void face_geom_2d(const om_mesh *m, int f) {
+  const double *a = m->x + 2 * m->fv[2 * f];
+  const double *b = m->x + 2 * m->fv[2 * f + 1];
+  double tx = b[0] - a[0], ty = b[1] - a[1];
+  double len = sqrt(tx * tx + ty * ty);
+  m->fnrm[2 * f] = ty / len;
+  m->fnrm[2 * f + 1] = -tx / len;
+  m->farea[f] = len;
+  m->fcen[2 * f] = 0.5 * (a[0] + b[0]);
+  m->fcen[2 * f + 1] = 0.5 * (a[1] + b[1]);
+}
+
+static inline void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* 3D quad face: area vector = sum of 4 triangle area vectors about the
+ * vertex average; centroid = area-weighted triangle centroids. */
+static void quad3_geom(const double *q[4], double *cen, double *nrm, double *area) {
+  double fm[3];
+  for (int d = 0; d < 3; ++d) fm[d] = (((q[0][d] + q[1][d]) + q[2][d]) + q[3][d]) * 0.25;
+  double S[3] = {0, 0, 0}, C[3] = {0, 0, 0}, A = 0.0;
+  for (int i = 0; i < 4; ++i) {
+    const double *p = q[i], *r = q[(i + 1) % 4];
+    double u[3], v[3], w[3];
+    for (int d = 0; d < 3; ++d) { u[d] = p[d] - fm[d]; v[d] = r[d] - fm[d]; }
+    cross3(u, v, w);
+    for (int d = 0; d < 3; ++d) w[d] = 0.5 * w[d];
+    double a = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int d = 0; d < 3; ++d) {
+      S[d] += w[d];
+      C[d] += a * ((fm[d] + p[d]) + r[d]);
+    }
+    A += a;
+  }
+  double s = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  for (int d = 0; d < 3; ++d) {
+    nrm[d] = S[d] / s;
+    cen[d] = C[d] / (3.0 * A);
+  }
+  *area = s;
+}
+
+static void cell_geom_3d(const om_mesh *m, int c, double *cen, double *vol, double *minlen) {
+  const int *cv = m->cv + 8 * c;
+  double cm[3] = {0, 0, 0};
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < 3; ++d) cm[d] += m->x[3 * cv[i] + d];
+  for (int d = 0; d < 3; ++d) cm[d] = cm[d] * 0.125;
+  double V = 0.0, C[3] = {0, 0, 0};
+  for (int lf = 0; lf < 6; ++lf) {
+    const double *q[4];
+    for (int i = 0; i < 4; ++i) q[i] = m->x + 3 * cv[HEX_FACES[lf][i]];
+    double fm[3];
+    for (int d = 0; d < 3; ++d) fm[d] = (((q[0][d] + q[1][d]) + q[2][d]) + q[3][d]) * 0.25;
+    for (int i = 0; i < 4; ++i) {
+      const double *p = q[i], *r = q[(i + 1) % 4];
+      double u[3], v[3], w[3], h[3];
+      for (int d = 0; d < 3; ++d) { u[d] = p[d] - fm[d]; v[d] = r[d] - fm[d]; h[d] = fm[d] - cm[d]; }
+      cross3(u, v, w);
+      double tv = (w[0] * h[0] + w[1] * h[1] + w[2] * h[2]) / 6.0;
+      V += tv;
+      for (int d = 0; d < 3; ++d) C[d] += tv * (((cm[d] + fm[d]) + p[d]) + r[d]);
+    }
+  }
+  for (int d = 0; d < 3; ++d) cen[d] = C[d] / (4.0 * V);
+  *vol = V;
+  double ml = DBL_MAX;
+  for (int i = 0; i < 8; ++i)
+    for (int j = i + 1; j < 8; ++j) {
+      const double *p = m->x + 3 * cv[i], *q = m->x + 3 * cv[j];
+      double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+      ml = smin(ml, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  *minlen = ml;
+}
+
+/* burton update_geometry() equivalent. */
+void om_mesh_update_geometry(om_mesh *m) {
+  #pragma omp parallel for schedule(static)
+  for (int c = 0; c < m->nc; ++c) {
+    if (m->dim == 2) cell_geom_2d(m, c, m->ccen + 2 * c, m->cvol + c, m->cminlen + c);
+    else cell_geom_3d(m, c, m->ccen + 3 * c, m->cvol + c, m->cminlen + c);
+  }
+  #pragma omp parallel for schedule(static)
+  for (int f = 0; f < m->nf; ++f) {
+    if (m->dim == 2) face_geom_2d(m, f);
+    else {
+      const double *q[4];
+      for (int i = 0; i < 4; ++i) q[i] = m->x + 3 * m->fv[4 * f + i];
+      quad3_geom(q, m->fcen + 3 * f, m->fnrm + 3 * f, m->farea + f);
+    }
+  }
+}
+
+om_mesh *om_mesh_create(int dim, int nc, int nv, int vpc, const double *coords, const int *cell_verts) {
+  if (!((dim == 2 && vpc >= 3) || (dim == 3 && vpc == 8)) || nc <= 0 || nv <= 0) return NULL;
+  om_mesh *m = (om_mesh *)calloc(1, sizeof(om_mesh));
+  m->dim = dim; m->nc = nc; m->nv = nv; m->vpc = vpc;
+  m->fpc = (dim == 2) ? vpc : 6;
+  m->vpf = (dim == 2) ? 2 : 4;
+  m->x = (double *)malloc(sizeof(double) * nv * dim);
+  memcpy(m->x, coords, sizeof(double) * nv * dim);
+  m->cv = (int *)malloc(sizeof(int) * nc * vpc);
+  memcpy(m->cv, cell_verts, sizeof(int) * nc * vpc);
+  for (int i = 0; i < nc * vpc; ++i)
+    if (m->cv[i] < 0 || m->cv[i] >= nv) { om_mesh_destroy(m); return NULL; }
+
+  /* faces: first visit creates the face */
+  const int nkeys = nc * m->fpc;
+  fkey *keys = (fkey *)malloc(sizeof(fkey) * nkeys);
+  for (int c = 0; c < nc; ++c)
+    for (int lf = 0; lf < m->fpc; ++lf) {
+      int lv[4], g[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+      local_face(m, lf, lv);
+      for (int i = 0; i < m->vpf; ++i) g[i] = m->cv[c * vpc + lv[i]];
+      sort_small(g, m->vpf);
+      fkey *k = keys + c * m->fpc + lf;
+      memcpy(k->k, g, sizeof(g));
+      k->face = c * m->fpc + lf;   /* slot id, ascending = visit order */
+    }
+  qsort(keys, nkeys, sizeof(fkey), fkey_cmp);
+  /* group identical keys; the group's first slot (lowest slot id) owns it */
+  int *slot_group_first = (int *)malloc(sizeof(int) * nkeys);
+  for (int i = 0; i < nkeys;) {
+    int j = i + 1;
+    int first = keys[i].face;
+    while (j < nkeys && fkey_cmp(keys + i, keys + j) == 0) { if (keys[j].face < first) first = keys[j].face; ++j; }
+    if (j - i > 2) { free(keys); free(slot_group_first); om_mesh_destroy(m); return NULL; }
+    for (int t = i; t < j; ++t) slot_group_first[keys[t].face] = first;
+    i = j;
+  }
+  free(keys);
+  int *slot_face = (int *)malloc(sizeof(int) * nkeys);
+  m->cf = (int *)malloc(sizeof(int) * nkeys);
+  int nf = 0;
+  for (int s = 0; s < nkeys; ++s) {
+    if (slot_group_first[s] == s) slot_face[s] = nf++;
+  }
+  m->nf = nf;
+  m->fc = (int *)malloc(sizeof(int) * nf * 2);
+  m->fv = (int *)malloc(sizeof(int) * nf * m->vpf);
+  for (int f = 0; f < nf; ++f) { m->fc[2 * f] = -1; m->fc[2 * f + 1] = -1; }
+  for (int s = 0; s < nkeys; ++s) {
+    int c = s / m->fpc, lf = s % m->fpc;
+    int f = slot_face[slot_group_first[s]];
+    m->cf[s] = f;
+    if (slot_group_first[s] == s) {
+      int lv[4];
+      local_face(m, lf, lv);
+      m->fc[2 * f] = c;
+      for (int i = 0; i < m->vpf; ++i) m->fv[m->vpf * f + i] = m->cv[c * vpc + lv[i]];
+    } else {
+      m->fc[2 * f + 1] = c;
+    }
+  }
+  free(slot_face);
+  free(slot_group_first);
+
+  /* vertex -> corner CSR (ascending cell id, local vertex order) */
+  m->vc_off = (int *)calloc(nv + 1, sizeof(int));
+  for (int i = 0; i < nc * vpc; ++i) m->vc_off[m->cv[i] + 1]++;
+  for (int v = 0; v < nv; ++v) m->vc_off[v + 1] += m->vc_off[v];
+  m->vc = (int *)malloc(sizeof(int) * nc * vpc);
+  int *fill = (int *)calloc(nv, sizeof(int));
+  for (int i = 0; i < nc * vpc; ++i) {
+    int v = m->cv[i];
+    m->vc[m->vc_off[v] + fill[v]++] = i;
+  }
+  free(fill);
+
+  /* cell-vertex neighbours: union of cells around the cell's vertices */
+  m->nbr_off = (int *)calloc(nc + 1, sizeof(int));
+  int cap = nc * 8 + 16, used = 0;
+  m->nbr = (int *)malloc(sizeof(int) * cap);
+  int *tmp = (int *)malloc(sizeof(int) * 512);
+  for (int c = 0; c < nc; ++c) {
+    int nt = 0;
+    for (int i = 0; i < vpc; ++i) {
+      int v = m->cv[c * vpc + i];
+      for (int k = m->vc_off[v]; k < m->vc_off[v + 1]; ++k) {
+        int c1 = m->vc[k] / vpc;
+        if (c1 != c && nt < 512) tmp[nt++] = c1;
+      }
+    }
+    qsort(tmp, nt, sizeof(int), cmp_int);
+    int nu = 0;
+    for (int i = 0; i < nt; ++i)
+      if (nu == 0 || tmp[nu - 1] != tmp[i]) tmp[nu++] = tmp[i];
+    if (used + nu > cap) { cap = 2 * (used + nu); m->nbr = (int *)realloc(m->nbr, sizeof(int) * cap); }
+    memcpy(m->nbr + used, tmp, sizeof(int) * nu);
+    used += nu;
+    m->nbr_off[c + 1] = used;
+  }
+  free(tmp);
+
+  m->ccen = (double *)malloc(sizeof(double) * nc * dim);
+  m->cvol = (double *)malloc(sizeof(double) * nc);
+  m->cminlen = (double *)malloc(sizeof(double) * nc);
+  m->fcen = (double *)malloc(sizeof(double) * nf * dim);
+  m->fnrm = (double *)malloc(sizeof(double) * nf * dim);
+  m->farea = (double *)malloc(sizeof(double) * nf);
+  om_mesh_update_geometry(m);
+
+  /* boundary tags from the initial normals:
+   * apps/maire_hydro/2d/inputs_regression.cc:134-162 (near(|n_d|, 1), tol 1e-14) */
+  m->ftag = (unsigned char *)calloc(nf, 1);
+  m->vbnd = (unsigned char *)calloc(nv, 1);
+  for (int f = 0; f < nf; ++f) {
+    if (m->fc[2 * f + 1] >= 0) continue;
+    unsigned char t = 0;
+    for (int d = 0; d < dim; ++d)
+      if (fabs(fabs(m->fnrm[dim * f + d]) - 1.0) <= 1e-14) t |= (unsigned char)(1u << d);
+    m->ftag[f] = t;
+    for (int i = 0; i < m->vpf; ++i) m->vbnd[m->fv[m->vpf * f + i]] = 1;
+  }
+  return m;
+}
+
+void om_mesh_destroy(om_mesh *m) {
+  if (!m) return;
+  free(m->x); free(m->cv); free(m->cf); free(m->fc); free(m->fv);
+  free(m->nbr_off); free(m->nbr); free(m->vc_off); free(m->vc);
+  free(m->ccen); free(m->cvol); free(m->cminlen); free(m->fcen); free(m->fnrm); free(m->farea);
+  free(m->ftag); free(m->vbnd);
+  free(m);
+}
+
+/* Introspection for tests. counts = {nf, nnbr, dim, nc, nv, vpc, fpc, vpf} */
+void om_mesh_counts(const om_mesh *m, int *counts) {
+  counts[0] = m->nf; counts[1] = m->nbr_off[m->nc]; counts[2] = m->dim; counts[3] = m->nc;
+  counts[4] = m->nv; counts[5] = m->vpc; counts[6] = m->fpc; counts[7] = m->vpf;
+}
+
+void om_mesh_topology(const om_mesh *m, int *cell_faces, int *face_cells, int *face_verts,
+                      int *nbr_off, int *nbr) {
+  if (cell_faces) memcpy(cell_faces, m->cf, sizeof(int) * m->nc * m->fpc);
+  if (face_cells) memcpy(face_cells, m->fc, sizeof(int) * m->nf * 2);
+  if (face_verts) memcpy(face_verts, m->fv, sizeof(int) * m->nf * m->vpf);
+  if (nbr_off) memcpy(nbr_off, m->nbr_off, sizeof(int) * (m->nc + 1));
+  if (nbr) memcpy(nbr, m->nbr, sizeof(int) * m->nbr_off[m->nc]);
+}
+
+void om_mesh_geometry(const om_mesh *m, double *ccen, double *cvol, double *cminlen,
+                      double *fcen, double *fnrm, double *farea, double *coords) {
+  const int d = m->dim;
+  if (ccen) memcpy(ccen, m->ccen, sizeof(double) * m->nc * d);
+  if (cvol) memcpy(cvol, m->cvol, sizeof(double) * m->nc);
+  if (cminlen) memcpy(cminlen, m->cminlen, sizeof(double) * m->nc);
+  if (fcen) memcpy(fcen, m->fcen, sizeof(double) * m->nf * d);
+  if (fnrm) memcpy(fnrm, m->fnrm, sizeof(double) * m->nf * d);
+  if (farea) memcpy(farea, m->farea, sizeof(double) * m->nf);
+  if (coords) memcpy(coords, m->x, sizeof(double) * m->nv * d);
+}
+
+void om_mesh_set_coords(om_mesh *m, const double *coords) {
+  memcpy(m->x, coords, sizeof(double) * m->nv * m->dim);
+  om_mesh_update_geometry(m);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ideal gas EOS: flecsale/eos/ideal_gas.h:178-226                           */
+/* ------------------------------------------------------------------------ */
+
+static inline double eos_e_dp(double g, double d, double p) { return p / (d * (g - 1.0)); }  /* :183 */
+static inline double eos_p_de(double g, double d, double e) { return (g - 1.0) * d * e; }    /* :198 */
+static inline double eos_a_de(double g, double e) { return sqrt(g * (g - 1.0) * e); }        /* :212 */
+static inline double eos_T_de(double cv, double e) { return e / cv; }                        /* :225 */
+
+void om_eos_eval(double g, double cv, int n, const double *d, const double *p, double *e_dp,
+                 double *p_de, double *a_de, double *T_de) {
+  for (int i = 0; i < n; ++i) {
+    e_dp[i] = eos_e_dp(g, d[i], p[i]);
+    p_de[i] = eos_p_de(g, d[i], e_dp[i]);
+    a_de[i] = eos_a_de(g, e_dp[i]);
+    T_de[i] = eos_T_de(cv, e_dp[i]);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Euler equations: flecsale/eqns/euler_eqns.h                               */
+/* ------------------------------------------------------------------------ */
+
+/* primitive face/cell state, tuple slot order (d, v, p, e, T, a):
+ * flecsale/eqns/euler_eqns.h:84-97 */
+typedef struct { double d, v[3], p, e, T, a; } ew;
+
+static inline double dot(const double *a, const double *b, int n) {
+  double s = a[0] * b[0];
+  for (int i = 1; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* total_energy :166-173 */
+static inline double e_total(const ew *u, int N) { return u->e + 0.5 * dot(u->v, u->v, N); }
+
+/* flux :274-304 */
+static void e_flux(const ew *u, const double *n, int N, double *f) {
+  double et = e_total(u, N);
+  double vn = dot(u->v, n, N);
+  double mf = u->d * vn;
+  f[0] = mf;
+  for (int i = 0; i < N; ++i) f[1 + i] = mf * u->v[i] + u->p * n[i];
+  f[N + 1] = mf * (et + u->p / u->d);
+}
+
+/* solution_delta :238-265 */
+static void e_delta(const ew *l, const ew *r, int N, double *du) {
+  double el = l->d * e_total(l, N);
+  double er = r->d * e_total(r, N);
+  du[0] = r->d - l->d;
+  for (int i = 0; i < N; ++i) du[1 + i] = r->d * r->v[i] - l->d * l->v[i];
+  du[N + 1] = er - el;
+}
+
+/* hlle_flux: flecsale/eqns/flux.h:80-106 (minmax_eigenvalues euler_eqns.h:222-230) */
+static void hlle_flux(const ew *wl, const ew *wr, const double *n, int N, double *f) {
+  double vnl = dot(wl->v, n, N), vnr = dot(wr->v, n, N);
+  double sl1 = vnl - wl->a, sl2 = vnl + wl->a;
+  double sr1 = vnr - wr->a, sr2 = vnr + wr->a;
+  double ll = smin(sl1, sr1);
+  double lr = smax(sl2, sr2);
+  if (ll >= 0) { e_flux(wl, n, N, f); return; }
+  if (lr <= 0) { e_flux(wr, n, N, f); return; }
+  double fl[5], fr[5], du[5];
+  e_flux(wl, n, N, fl);
+  e_flux(wr, n, N, fr);
+  double c1 = ll * lr, c2 = lr - ll, c2inv = 1 / c2;
+  e_delta(wl, wr, N, du);
+  for (int i = 0; i < N + 2; ++i) f[i] = c2inv * (lr * fl[i] - ll * fr[i] + c1 * du[i]);
+}
+
+/* rusanov_flux: flecsale/eqns/flux.h:52-65 (fastest_wavespeed euler_eqns.h:183-189) */
+static void rusanov_flux(const ew *wl, const ew *wr, const double *n, int N, double *f) {
+  double fl[5], fr[5], du[5];
+  e_flux(wl, n, N, fl);
+  e_flux(wr, n, N, fr);
+  double sl = wl->a + fabs(dot(wl->v, n, N));
+  double sr = wr->a + fabs(dot(wr->v, n, N));
+  double s = smax(sl, sr);
+  e_delta(wl, wr, N, du);
+  double hs = s / 2;
+  for (int i = 0; i < N + 2; ++i) f[i] = (fl[i] + fr[i]) / 2 - du[i] * hs;
+}
+
+/* HLLC (Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics",
+ * 3rd ed., sec. 10.4) with the HLLE (Davis) wave-speed bounds of flux.h:82-86.
+ * Not present in the reference: a build-side addition. */
+static void hllc_flux(const ew *wl, const ew *wr, const double *n, int N, double *f) {
+  double ul = dot(wl->v, n, N), ur = dot(wr->v, n, N);
+  double sL = smin(ul - wl->a, ur - wr->a);
+  double sR = smax(ul + wl->a, ur + wr->a);
+  if (sL >= 0) { e_flux(wl, n, N, f); return; }
+  if (sR <= 0) { e_flux(wr, n, N, f); return; }
+  double ml = wl->d * (sL - ul), mr = wr->d * (sR - ur);
+  double sM = (wr->p - wl->p + ml * ul - mr * ur) / (ml - mr);
+  const ew *w = (sM >= 0) ? wl : wr;
+  double s = (sM >= 0) ? sL : sR;
+  double un = (sM >= 0) ? ul : ur;
+  double fk[5];
+  e_flux(w, n, N, fk);
+  double Ek = w->d * e_total(w, N);
+  double coef = w->d * (s - un) / (s - sM);
+  double us[5];
+  us[0] = coef;
+  for (int i = 0; i < N; ++i) us[1 + i] = coef * (w->v[i] + (sM - un) * n[i]);
+  us[N + 1] = coef * (Ek / w->d + (sM - un) * (sM + w->p / (w->d * (s - un))));
+  double uk[5];
+  uk[0] = w->d;
+  for (int i = 0; i < N; ++i) uk[1 + i] = w->d * w->v[i];
+  uk[N + 1] = Ek;
+  for (int i = 0; i < N + 2; ++i) f[i] = fk[i] + s * (us[i] - uk[i]);
+}
+
+/* wall_flux :313-324 */
+static void wall_flux(const ew *u, const double *n, int N, double *f) {
+  f[0] = 0;
+  for (int i = 0; i < N; ++i) f[1 + i] = u->p * n[i];
+  f[N + 1] = 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Eulerian tasks: apps/hydro/tasks.h                                        */
+/* State layout: d[nc], v[nc*dim], e[nc], p[nc], T[nc], a[nc]                */
+/* Gradients: gd[nc*dim], gv[nc*dim*dim] (gv[c][d][i] = dv_d/dx_i),          */
+/*            ge, gp, ga [nc*dim]                                            */
+/* ------------------------------------------------------------------------ */
+
+/* initial_conditions :46-68 -> update_state_from_pressure euler_eqns.h:333-350 */
+void oe_state_from_pressure(int nc, double g, double cv, const double *d, const double *p,
+                            double *e, double *T, double *a) {
+  for (int c = 0; c < nc; ++c) {
+    double ie = eos_e_dp(g, d[c], p[c]);
+    e[c] = ie;
+    T[c] = eos_T_de(cv, ie);
+    a[c] = eos_a_de(g, ie);
+  }
+}
+
+/* evaluate_time_step :331-376 */
+int oe_time_step(const om_mesh *m, const double *v, const double *a, double cfl,
+                 double final_time, double soln_time, double *dt_out) {
+  const int N = m->dim;
+  double max_dt = final_time - soln_time;
+  double dt_inv = 0;
+  for (int c = 0; c < m->nc; ++c) {
+    for (int k = 0; k < m->fpc; ++k) {
+      int f = m->cf[c * m->fpc + k];
+      double delta_x = m->cvol[c] / m->farea[f];
+      double vn = dot(v + N * c, m->fnrm + N * f, N);
+      double dti = (a[c] + fabs(vn)) / delta_x;
+      dt_inv = smax(dti, dt_inv);
+    }
+  }
+  if (dt_inv <= 0) return OM_ERR_DT;
+  double ts = 1 / dt_inv;
+  ts *= cfl;
+  ts = smin(ts, max_dt);
+  *dt_out = ts;
+  return OM_OK;
+}
+
+/* evaluate_gradients :145-321 (gradient_2d :109-117, gradient_3d :119-143).
+ * first_order != 0 writes zero gradients (a build-side switch). */
+void oe_gradients(const om_mesh *m, const double *d, const double *v, const double *e,
+                  const double *p, const double *a, double *gd, double *gv, double *ge,
+                  double *gp, double *ga, int first_order) {
+  const int N = m->dim, NV = N + 4;
+  const double tol = 1.e-9;
+  #pragma omp parallel for schedule(static)
+  for (int c0 = 0; c0 < m->nc; ++c0) {
+    double u0[7], u1[7], umin[7], umax[7], phi[7], dudx[7][3], gu[7][3], dxdx[3][3];
+    const double *x0 = m->ccen + N * c0;
+    int i = 0;
+    u0[i++] = d[c0];
+    for (int k = 0; k < N; ++k) u0[i++] = v[N * c0 + k];
+    u0[i++] = e[c0];
+    u0[i++] = p[c0];
+    u0[i++] = a[c0];
+    for (int j = 0; j < NV; ++j) { umin[j] = u0[j]; umax[j] = u0[j]; }
+    memset(dxdx, 0, sizeof(dxdx));
+    memset(dudx, 0, sizeof(dudx));
+    for (int q = m->nbr_off[c0]; q < m->nbr_off[c0 + 1]; ++q) {
+      int c1 = m->nbr[q];
+      const double *x1 = m->ccen + N * c1;
+      i = 0;
+      u1[i++] = d[c1];
+      for (int k = 0; k < N; ++k) u1[i++] = v[N * c1 + k];
+      u1[i++] = e[c1];
+      u1[i++] = p[c1];
+      u1[i++] = a[c1];
+      for (int j = 0; j < NV; ++j) {
+        umin[j] = smin(u1[j], umin[j]);
+        umax[j] = smax(u1[j], umax[j]);
+      }
+      double dx[3] = {0, 0, 0};
+      for (int k = 0; k < N; ++k) dx[k] = x1[k] - x0[k];
+      if (N == 2) {
+        dxdx[0][0] += dx[0] * dx[0];
+        dxdx[0][1] += dx[0] * dx[1];
+        dxdx[1][1] += dx[1] * dx[1];
+      } else {
+        dxdx[0][0] += dx[0] * dx[0];
+        dxdx[0][1] += dx[0] * dx[1];
+        dxdx[0][2] += dx[0] * dx[2];
+        dxdx[1][1] += dx[1] * dx[1];
+        dxdx[1][2] += dx[1] * dx[2];
+        dxdx[2][2] += dx[2] * dx[2];
+      }
+      for (int j = 0; j < NV; ++j) {
+        double du = u1[j] - u0[j];
+        for (int k = 0; k < N; ++k) dudx[j][k] += du * dx[k];
+      }
+    }
+    if (N == 2) {
+      double denom = 1 / (dxdx[0][0] * dxdx[1][1] - dxdx[0][1] * dxdx[0][1]);
+      for (int j = 0; j < NV; ++j) {
+        gu[j][0] = (dudx[j][0] * dxdx[1][1] - dudx[j][1] * dxdx[0][1]) * denom;
+        gu[j][1] = (dudx[j][1] * dxdx[0][0] - dudx[j][0] * dxdx[0][1]) * denom;
+      }
+    } else {
+      double min1 = dxdx[1][1] * dxdx[2][2] - dxdx[1][2] * dxdx[1][2];
+      double min2 = dxdx[0][1] * dxdx[2][2] - dxdx[1][2] * dxdx[0][2];
+      double min3 = dxdx[0][1] * dxdx[1][2] - dxdx[1][1] * dxdx[0][2];
+      double denom = 1 / (dxdx[0][0] * min1 - dxdx[0][1] * min2 + dxdx[0][2] * min3);
+      for (int j = 0; j < NV; ++j) {
+        const double *u = dudx[j];
+        gu[j][0] = (u[0] * min1 - dxdx[0][1] * (u[1] * dxdx[2][2] - dxdx[1][2] * u[2]) +
+                    dxdx[0][2] * (u[1] * dxdx[1][2] - dxdx[1][1] * u[2])) * denom;
+        gu[j][1] = (dxdx[0][0] * (u[1] * dxdx[2][2] - dxdx[1][2] * u[2]) - u[0] * min2 +
+                    dxdx[0][2] * (u[2] * dxdx[0][1] - dxdx[0][2] * u[1])) * denom;
+        gu[j][2] = (dxdx[0][0] * (u[2] * dxdx[1][1] - dxdx[1][2] * u[1]) -
+                    dxdx[0][1] * (u[2] * dxdx[0][1] - dxdx[0][2] * u[1]) + u[0] * min3) * denom;
+      }
+    }
+    /* limiter :267-299 */
+    for (int j = 0; j < NV; ++j) phi[j] = 1;
+    for (int lv = 0; lv < m->vpc; ++lv) {
+      const double *xv = m->x + N * m->cv[c0 * m->vpc + lv];
+      for (int j = 0; j < NV; ++j) u1[j] = u0[j];
+      for (int k = 0; k < N; ++k) {
+        double dx = xv[k] - x0[k];
+        for (int j = 0; j < NV; ++j) u1[j] += dx * gu[j][k];
+      }
+      for (int j = 0; j < NV; ++j) {
+        double du = u1[j] - u0[j];
+        if (du > tol) phi[j] = smin(phi[j], (umax[j] - u0[j]) / du);
+        else if (du < -tol) phi[j] = smin(phi[j], (umin[j] - u0[j]) / du);
+      }
+    }
+    for (int j = 0; j < NV; ++j) {
+      phi[j] = smax(phi[j], 0);
+      for (int k = 0; k < N; ++k) gu[j][k] *= phi[j];
+      if (first_order)
+        for (int k = 0; k < N; ++k) gu[j][k] = 0.0;
+    }
+    for (int k = 0; k < N; ++k) {
+      gd[N * c0 + k] = gu[0][k];
+      for (int dd = 0; dd < N; ++dd) gv[N * N * c0 + N * dd + k] = gu[1 + dd][k];
+      ge[N * c0 + k] = gu[1 + N][k];
+      gp[N * c0 + k] = gu[2 + N][k];
+      ga[N * c0 + k] = gu[3 + N][k];
+    }
+  }
+}
+
+/* face-state reconstruction of evaluate_fluxes :418-432, with the reference's
+ * slot order: slot 2 (pressure) += dx.grad_e, slot 3 (energy) += dx.grad_p. */
+static void recon(const om_mesh *m, int c, const double *fx, const double *d, const double *v,
+                  const double *e, const double *p, const double *T, const double *a,
+                  const double *gd, const double *gv, const double *ge, const double *gp,
+                  const double *ga, ew *w) {
+  const int N = m->dim;
+  const double *cx = m->ccen + N * c;
+  double dx[3];
+  for (int i = 0; i < N; ++i) dx[i] = fx[i] - cx[i];
+  w->d = d[c];
+  for (int k = 0; k < N; ++k) w->v[k] = v[N * c + k];
+  w->p = p[c]; w->e = e[c]; w->T = T[c]; w->a = a[c];
+  for (int i = 0; i < N; ++i) {
+    w->d += dx[i] * gd[N * c + i];
+    for (int k = 0; k < N; ++k) w->v[k] += dx[i] * gv[N * N * c + N * k + i];
+    w->p += dx[i] * ge[N * c + i];
+    w->e += dx[i] * gp[N * c + i];
+    w->a += dx[i] * ga[N * c + i];
+  }
+}
+
+/* evaluate_fluxes :385-464. flux_kind: 0 HLLE (types.h:118-125), 1 Rusanov, 2 HLLC.
+ * F is [nf*(dim+2)] (mass, momentum[dim], energy), scaled by face area. */
+void oe_fluxes(const om_mesh *m, const double *d, const double *v, const double *e,
+               const double *p, const double *T, const double *a, const double *gd,
+               const double *gv, const double *ge, const double *gp, const double *ga,
+               int flux_kind, double *F) {
+  const int N = m->dim, NF = N + 2;
+  #pragma omp parallel for schedule(static)
+  for (int f = 0; f < m->nf; ++f) {
+    const double *fx = m->fcen + N * f;
+    const double *n = m->fnrm + N * f;
+    ew wl, wr;
+    double *out = F + NF * f;
+    recon(m, m->fc[2 * f], fx, d, v, e, p, T, a, gd, gv, ge, gp, ga, &wl);
+    int c1 = m->fc[2 * f + 1];
+    if (c1 >= 0) {
+      recon(m, c1, fx, d, v, e, p, T, a, gd, gv, ge, gp, ga, &wr);
+      if (flux_kind == 1) rusanov_flux(&wl, &wr, n, N, out);
+      else if (flux_kind == 2) hllc_flux(&wl, &wr, n, N, out);
+      else hlle_flux(&wl, &wr, n, N, out);
+    } else {
+      wall_flux(&wl, n, N, out);
+    }
+    for (int i = 0; i < NF; ++i) out[i] *= m->farea[f];
+  }
+}
+
+/* apply_update :589-659 with update_state_from_flux euler_eqns.h:383-410 and
+ * update_state_from_energy :359-375.  Returns OM_ERR_NEG_STATE on e<0||d<0;
+ * *bad_cell receives the first such cell. */
+int oe_apply_update(const om_mesh *m, double g, double cv, const double *F, double dt,
+                    double factor, double *d, double *v, double *e, double *p, double *T,
+                    double *a, int *bad_cell) {
+  const int N = m->dim, NF = N + 2;
+  double delta_t = dt;
+  delta_t *= factor;
+  int bad = INT32_MAX;
+  #pragma omp parallel for schedule(static) reduction(min:bad)
+  for (int c = 0; c < m->nc; ++c) {
+    double du[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < m->fpc; ++k) {
+      int f = m->cf[c * m->fpc + k];
+      const double *fl = F + NF * f;
+      if (m->fc[2 * f] == c) { for (int i = 0; i < NF; ++i) du[i] -= fl[i]; }
+      else { for (int i = 0; i < NF; ++i) du[i] += fl[i]; }
+    }
+    double s = delta_t / m->cvol[c];
+    for (int i = 0; i < NF; ++i) du[i] *= s;
+    /* update_state_from_flux */
+    double *vel = v + N * c;
+    double den0 = d[c];
+    double mass = den0 + du[0];
+    double et0 = e[c] + 0.5 * dot(vel, vel, N);
+    double ener = den0 * et0 + du[N + 1];
+    double inv_mass = 1 / mass;
+    for (int i = 0; i < N; ++i) {
+      double mom = den0 * vel[i] + du[1 + i];
+      vel[i] = mom * inv_mass;
+    }
+    d[c] = mass;
+    e[c] = ener * inv_mass - 0.5 * dot(vel, vel, N);
+    /* update_state_from_energy */
+    p[c] = eos_p_de(g, d[c], e[c]);
+    T[c] = eos_T_de(cv, e[c]);
+    a[c] = eos_a_de(g, e[c]);
+    if ((e[c] < 0 || d[c] < 0) && c < bad) bad = c;
+  }
+  if (bad == INT32_MAX) bad = -1;
+  if (bad_cell) *bad_cell = bad;
+  return bad >= 0 ? OM_ERR_NEG_STATE : OM_OK;
+}
+
+/* Eulerian driver: apps/hydro/driver.h:317-413.  One dt per step, then two
+ * [gradients -> fluxes -> update(0.5)] half-steps.  stop_at_final != 0 also
+ * ends the loop once t >= final_time (a build-side option; the reference loop
+ * runs exactly max_steps).  Work arrays are allocated here. */
+int oe_run(const om_mesh *m, double g, double cv, double cfl, double final_time, int max_steps,
+           int flux_kind, int first_order, int stop_at_final, double *d, double *v, double *e,
+           double *p, double *T, double *a, double *soln_time, int *steps_done) {
+  const int N = m->dim, nc = m->nc;
+  double *gd = (double *)malloc(sizeof(double) * nc * N);
+  double *gv = (double *)malloc(sizeof(double) * nc * N * N);
+  double *ge = (double *)malloc(sizeof(double) * nc * N);
+  double *gp = (double *)malloc(sizeof(double) * nc * N);
+  double *ga = (double *)malloc(sizeof(double) * nc * N);
+  double *F = (double *)malloc(sizeof(double) * m->nf * (N + 2));
+  int rc = OM_OK, n = 0;
+  double t = *soln_time;
+  for (n = 0; n < max_steps; ++n) {
+    if (stop_at_final && !(t < final_time)) break;
+    double dt;
+    rc = oe_time_step(m, v, a, cfl, final_time, t, &dt);
+    if (rc) break;
+    for (int half = 0; half < 2; ++half) {
+      oe_gradients(m, d, v, e, p, a, gd, gv, ge, gp, ga, first_order);
+      oe_fluxes(m, d, v, e, p, T, a, gd, gv, ge, gp, ga, flux_kind, F);
+      double dtf = dt;
+      dtf *= 0.5;
+      t = t + dtf;   /* update_soln_time :554 with initial_soln_time = 0 */
+      int bad;
+      rc = oe_apply_update(m, g, cv, F, dt, 0.5, d, v, e, p, T, a, &bad);
+      if (rc) break;
+    }
+    if (rc) { ++n; break; }
+  }
+  *soln_time = t;
+  *steps_done = n;
+  free(gd); free(gv); free(ge); free(gp); free(ga); free(F);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Householder QR with column pivoting: flecsale/linalg/qr.h:44-94,          */
+/* flecsale/linalg/detail/qr_impl.h:38-288.  A is row-major n x n (n <= 8).  */
+/* ------------------------------------------------------------------------ */
+#define QRMAX 8
+void om_qr_solve(int rows, double *A, double *B) {
+  const int cols = rows;
+  double v[QRMAX] = {0};
+  int jpvt[QRMAX];
+  for (int i = 0; i < cols; ++i) jpvt[i] = i;
+  for (int i = 0; i < cols; ++i) {
+    /* get_next_col :38-73 */
+    double cn[QRMAX];
+    for (int j = 0; j < cols; ++j) {
+      cn[j] = 0;
+      for (int r = i; r < rows; ++r) cn[j] += A[r * cols + jpvt[j]] * A[r * cols + jpvt[j]];
+    }
+    int max_loc = 0;
+    double mx = 0;
+    for (int j = 0; j < cols; ++j)
+      if (cn[j] > mx) { mx = cn[j]; max_loc = j; }
+    { int t = jpvt[i]; jpvt[i] = jpvt[max_loc]; jpvt[max_loc] = t; }
+    /* householder :96-132 */
+    const int col = jpvt[i];
+    do {
+      double norm = 0;
+      for (int r = i; r < rows; ++r) norm += A[r * cols + col] * A[r * cols + col];
+      if (norm == 0) break;
+      norm = sqrt(norm);
+      v[0] = A[i * cols + col] - norm;
+      for (int r = 1; r < rows - i; ++r) v[r] = A[(r + i) * cols + col];
+      norm = 0;
+      for (int r = 0; r < rows - i; ++r) norm += v[r] * v[r];
+      if (norm == 0) break;
+      norm = sqrt(norm);
+      for (int r = 0; r < rows - i; ++r) v[r] *= (1.0 / norm);
+    } while (0);
+    /* apply_householder :161-217 */
+    const int nn = rows - i;
+    double Ac[QRMAX * QRMAX], Bc[QRMAX], hh[QRMAX * QRMAX];
+    memcpy(Ac, A, sizeof(double) * rows * cols);
+    memcpy(Bc, B, sizeof(double) * rows);
+    for (int j = 0; j < nn; ++j)
+      for (int r = 0; r < nn; ++r)
+        hh[r + nn * j] = (r != j) ? -2 * v[j] * v[r] : 1 - 2 * v[j] * v[r];
+    for (int k = 0; k < cols; ++k)
+      for (int j = 0; j < nn; ++j) {
+        double temp = 0;
+        for (int r = 0; r < nn; ++r) temp += hh[r + nn * j] * Ac[(r + i) * cols + jpvt[k]];
+        A[(j + i) * cols + jpvt[k]] = temp;
+      }
+    for (int j = 0; j < nn; ++j) {
+      double sum = 0;
+      for (int r = 0; r < nn; ++r) sum += hh[r + nn * j] * Bc[r + i];
+      B[j + i] = sum;
+    }
+  }
+  /* back_solve :240-288 */
+  const double eps = DBL_EPSILON;
+  double Bc[QRMAX];
+  memcpy(Bc, B, sizeof(double) * rows);
+  int bottom = 0;
+  for (int i = rows; i-- > 0;) {
+    if (fabs(A[i * cols + jpvt[cols - 1]]) > eps) { bottom = i; break; }
+  }
+  if (bottom > cols - 1) bottom = cols - 1;
+  for (int i = bottom + 1; i-- > 0;) {
+    double sum = 0;
+    for (int j = cols; j-- > i + 1;) sum += B[jpvt[j]] * A[i * cols + jpvt[j]];
+    if (fabs(A[i * cols + jpvt[i]]) > eps) {
+      double temp = 1 / A[i * cols + jpvt[i]];
+      B[jpvt[i]] = (Bc[i] - sum) * temp;
+    } else {
+      B[jpvt[i]] = 0;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Maire cell-centred Lagrangian tasks (2D): apps/maire_hydro/tasks.h         */
+/* Cell state: V, M, v[nc*2], p, d, e, T, a (lagrange_eqns.h:91-106);        */
+/* per-cell gamma g[nc] (uniform in the reference; per-cell is an extension) */
+/* Corner arrays indexed by corner = c*vpc + k: npc[ncn*2], Fpc[ncn*2].      */
+/* ------------------------------------------------------------------------ */
+
+#define MIN_SOUND_SPEED 1.e-6   /* lagrange_eqns.h:54 */
+
+/* update_state_from_pressure lagrange_eqns.h:245-271 */
+static void ml_from_pressure(double g, double cv, double M, double V, double p, double *d,
+                             double *e, double *T, double *a) {
+  *d = M / V;
+  *e = eos_e_dp(g, *d, p);
+  *T = eos_T_de(cv, *e);
+  double ss = eos_a_de(g, *e);
+  *a = smax(ss, MIN_SOUND_SPEED);
+}
+
+/* initial_conditions tasks.h:71-101 given (den, v, p) per cell */
+void om_maire_init(const om_mesh *m, const double *g, double cv, const double *den,
+                   const double *p, double *V, double *M, double *d, double *e, double *T,
+                   double *a) {
+  for (int c = 0; c < m->nc; ++c) {
+    double vol = m->cvol[c];
+    M[c] = den[c] * vol;
+    V[c] = vol;
+    ml_from_pressure(g[c], cv, M[c], V[c], p[c], d + c, e + c, T + c, a + c);
+  }
+}
+
+/* wedge facets of corner (c, k): (edge v_{k-1}->v_k, edge v_k->v_{k+1}) in
+ * the cell's CCW order; outward unit normal and half edge length. */
+static void wedge_facets_2d(const om_mesh *m, int c, int k, double n[2][2], double l[2], int ledge[2]) {
+  const int nv = m->vpc;
+  int km = (k + nv - 1) % nv, kp = (k + 1) % nv;
+  const double *xm = m->x + 2 * m->cv[c * nv + km];
+  const double *xk = m->x + 2 * m->cv[c * nv + k];
+  const double *xp = m->x + 2 * m->cv[c * nv + kp];
+  const double *A[2] = {xm, xk}, *Bv[2] = {xk, xp};
+  ledge[0] = km; ledge[1] = k;
+  for (int w = 0; w < 2; ++w) {
+    double tx = Bv[w][0] - A[w][0], ty = Bv[w][1] - A[w][1];
+    double len = sqrt(tx * tx + ty * ty);
+    n[w][0] = ty / len;
+    n[w][1] = -tx / len;
+    /* facet_area = |edge midpoint - vertex|: pins sedov_2d to print precision */
+    double mx = 0.5 * (A[w][0] + Bv[w][0]) - xk[0];
+    double my = 0.5 * (A[w][1] + Bv[w][1]) - xk[1];
+    l[w] = sqrt(mx * mx + my * my);
+  }
+}
+
+/* evaluate_nodal_state tasks.h:221-453 (2D, symmetry boundary conditions of
+ * inputs_regression.cc:116-162: both tags symmetric).  un[nv*2]. */
+void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const double *dc,
+                    const double *ac, double *un, double *npc, double *Fpc) {
+  const int nvpc = m->vpc;
+  #pragma omp parallel for schedule(static)
+  for (int vt = 0; vt < m->nv; ++vt) {
+    double Mp[2][2] = {{0, 0}, {0, 0}}, rhs[2] = {0, 0};
+    int ncn = m->vc_off[vt + 1] - m->vc_off[vt];
+    double Mpc[16][2][2];
+    if (ncn > 16) ncn = 16;
+    for (int j = 0; j < ncn; ++j) {
+      int cn = m->vc[m->vc_off[vt] + j];
+      int cl = cn / nvpc, k = cn % nvpc;
+      double *F = Fpc + 2 * cn, *np = npc + 2 * cn;
+      F[0] = 0; F[1] = 0; np[0] = 0; np[1] = 0;
+      double zc = dc[cl] * ac[cl];
+      double n[2][2], l[2];
+      int le[2];
+      wedge_facets_2d(m, cl, k, n, l, le);
+      memset(Mpc[j], 0, sizeof(Mpc[j]));
+      for (int w = 0; w < 2; ++w) {
+        double fact = zc * l[w];
+        for (int r = 0; r < 2; ++r)
+          for (int s = 0; s < 2; ++s) Mpc[j][r][s] += fact * n[w][r] * n[w][s];
+        for (int dd = 0; dd < 2; ++dd) np[dd] += l[w] * n[w][dd];
+      }
+      for (int r = 0; r < 2; ++r) { Mp[r][0] += Mpc[j][r][0]; Mp[r][1] += Mpc[j][r][1]; }
+      const double *u = uc + 2 * cl;
+      for (int r = 0; r < 2; ++r) {
+        F[r] += Mpc[j][r][0] * u[0];
+        F[r] += Mpc[j][r][1] * u[1];
+      }
+      for (int dd = 0; dd < 2; ++dd) {
+        F[dd] += pc[cl] * np[dd];
+        rhs[dd] += F[dd];
+      }
+    }
+    double *u = un + 2 * vt;
+    int nsym = 0;
+    double symn[2][2];
+    int has[2] = {0, 0};
+    if (m->vbnd[vt]) {
+      /* wedges of the vertex: corners in order, wedges (minus, plus) */
+      for (int j = 0; j < ncn; ++j) {
+        int cn = m->vc[m->vc_off[vt] + j];
+        int cl = cn / nvpc, k = cn % nvpc;
+        double n[2][2], l[2];
+        int le[2];
+        wedge_facets_2d(m, cl, k, n, l, le);
+        for (int w = 0; w < 2; ++w) {
+          int f = m->cf[cl * m->fpc + le[w]];
+          if (m->fc[2 * f + 1] >= 0) continue;
+          for (int tag = 0; tag < 2; ++tag) {
+            if (!(m->ftag[f] & (1u << tag))) continue;
+            if (has[tag]) { for (int dd = 0; dd < 2; ++dd) symn[tag][dd] += l[w] * n[w][dd]; }
+            else { has[tag] = 1; for (int dd = 0; dd < 2; ++dd) symn[tag][dd] = l[w] * n[w][dd]; }
+          }
+        }
+      }
+      nsym = has[0] + has[1];
+    }
+    if (nsym == 0) {
+      /* ristra::math::solve (un-vendored): Cramer's rule, build-defined */
+      double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
+      u[0] = (rhs[0] * Mp[1][1] - Mp[0][1] * rhs[1]) / det;
+      u[1] = (Mp[0][0] * rhs[1] - rhs[0] * Mp[1][0]) / det;
+    } else {
+      int nr = 2 + nsym;
+      double A[16] = {0}, b[4] = {0};
+      b[0] = rhs[0]; b[1] = rhs[1];
+      for (int r = 0; r < 2; ++r)
+        for (int s = 0; s < 2; ++s) A[r * nr + s] = Mp[r][s];
+      int jj = 2;
+      for (int tag = 0; tag < 2; ++tag) {
+        if (!has[tag]) continue;
+        for (int r = 0; r < 2; ++r) A[r * nr + jj] = symn[tag][r];
+        for (int s = 0; s < 2; ++s) A[jj * nr + s] = symn[tag][s];
+        ++jj;
+      }
+      om_qr_solve(nr, A, b);
+      u[0] = b[0]; u[1] = b[1];
+    }
+    /* scatter: Fpc -= Mpc * un (matrix_vector(-1, Mpc, un, 1, Fpc)) */
+    for (int j = 0; j < ncn; ++j) {
+      int cn = m->vc[m->vc_off[vt] + j];
+      double *F = Fpc + 2 * cn;
+      for (int r = 0; r < 2; ++r) {
+        double s = Mpc[j][r][0] * u[0];
+        s += Mpc[j][r][1] * u[1];
+        F[r] = -1.0 * s + 1.0 * F[r];
+      }
+    }
+  }
+}
+
+/* evaluate_residual tasks.h:461-489 with compute_update lagrange_eqns.h:374-389.
+ * dudt[nc*4] = (mass, mom x, mom y, energy). */
+void om_maire_residual(const om_mesh *m, const double *un, const double *npc, const double *Fpc,
+                       double *dudt) {
+  const int nvpc = m->vpc;
+  #pragma omp parallel for schedule(static)
+  for (int c = 0; c < m->nc; ++c) {
+    double r[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nvpc; ++k) {
+      int cn = c * nvpc + k;
+      const double *u = un + 2 * m->cv[cn];
+      const double *F = Fpc + 2 * cn, *n = npc + 2 * cn;
+      for (int dd = 0; dd < 2; ++dd) {
+        r[0] += n[dd] * u[dd];
+        r[3] -= F[dd] * u[dd];
+        r[1 + dd] -= F[dd];
+      }
+    }
+    memcpy(dudt + 4 * c, r, sizeof(r));
+  }
+}
+
+/* evaluate_time_step tasks.h:142-190 */
+double om_maire_time_step(const om_mesh *m, double cfl_acc, double cfl_vol, double cfl_growth,
+                          double prev_dt, const double *a, const double *dudt) {
+  double acc_inv = 0, vol_inv = 0;
+  for (int c = 0; c < m->nc; ++c) {
+    double dti = a[c] / m->cminlen[c];
+    acc_inv = smax(dti, acc_inv);
+    double dVdt = dudt[4 * c];
+    dti = fabs(dVdt) / m->cvol[c];
+    vol_inv = smax(dti, vol_inv);
+  }
+  double dts[3] = {cfl_acc / acc_inv, cfl_vol / vol_inv, cfl_growth * prev_dt};
+  int it = 0;
+  for (int i = 1; i < 3; ++i)
+    if (dts[i] < dts[it]) it = i;
+  return dts[it];
+}
+
+/* apply_update tasks.h:497-548 with update_state_from_flux lagrange_eqns.h:310-343
+ * and update_volume :352-367.  x0 are the saved coordinates. */
+int om_maire_apply_update(om_mesh *m, double delta_t, const double *x0, const double *un,
+                          const double *dudt, double *V, const double *M, double *uc, double *d,
+                          double *e, int *bad_cell) {
+  if (delta_t > 1e-14) {   /* flecsale::config::test_tolerance */
+    for (int vt = 0; vt < m->nv; ++vt)
+      for (int dd = 0; dd < 2; ++dd) m->x[2 * vt + dd] = x0[2 * vt + dd] + delta_t * un[2 * vt + dd];
+    om_mesh_update_geometry(m);
+  }
+  int bad = -1;
+  for (int c = 0; c < m->nc; ++c) {
+    double inv_mass = 1 / M[c];
+    double *vel = uc + 2 * c;
+    const double *du = dudt + 4 * c;
+    double et = e[c] + dot(vel, vel, 2) / 2 + delta_t * inv_mass * du[3];
+    for (int i = 0; i < 2; ++i) vel[i] += delta_t * inv_mass * du[1 + i];
+    e[c] = et - dot(vel, vel, 2) / 2;
+    if (e[c] < 0 && bad < 0) bad = c;
+    V[c] = m->cvol[c];
+    d[c] = M[c] / m->cvol[c];
+    if (d[c] < 0 && bad < 0) bad = c;
+  }
+  if (bad_cell) *bad_cell = bad;
+  return bad >= 0 ? OM_ERR_NEG_STATE : OM_OK;
+}
+
+/* update_state_from_energy tasks.h:110-133 -> lagrange_eqns.h:280-302 */
+void om_maire_from_energy(int nc, const double *g, double cv, const double *d, const double *e,
+                          double *p, double *T, double *a) {
+  for (int c = 0; c < nc; ++c) {
+    p[c] = eos_p_de(g[c], d[c], e[c]);
+    T[c] = eos_T_de(cv, e[c]);
+    a[c] = smax(eos_a_de(g[c], e[c]), MIN_SOUND_SPEED);
+  }
+}
+
+/* Maire driver apps/maire_hydro/driver.h:330-538 (midpoint predictor-corrector).
+ * The mesh coordinates move; *dt_io holds the initial (in) / last (out) step. */
+int om_maire_run(om_mesh *m, const double *g, double cv, double cfl_acc, double cfl_vol,
+                 double cfl_growth, double final_time, int max_steps, double *dt_io, double *V,
+                 const double *M, double *uc, double *p, double *d, double *e, double *T,
+                 double *a, double *soln_time, int *steps_done) {
+  const int nc = m->nc, nv = m->nv, ncn = nc * m->vpc;
+  double *x0 = (double *)malloc(sizeof(double) * nv * 2);
+  double *uc0 = (double *)malloc(sizeof(double) * nc * 2);
+  double *ec0 = (double *)malloc(sizeof(double) * nc);
+  double *un = (double *)malloc(sizeof(double) * nv * 2);
+  double *npc = (double *)malloc(sizeof(double) * ncn * 2);
+  double *Fpc = (double *)malloc(sizeof(double) * ncn * 2);
+  double *dudt = (double *)malloc(sizeof(double) * nc * 4);
+  double t = *soln_time, ts = *dt_io;
+  int n, rc = OM_OK, bad;
+  for (n = 0; n < max_steps && t < final_time; ++n) {
+    memcpy(x0, m->x, sizeof(double) * nv * 2);
+    memcpy(uc0, uc, sizeof(double) * nc * 2);
+    memcpy(ec0, e, sizeof(double) * nc);
+    om_maire_nodal(m, uc, p, d, a, un, npc, Fpc);
+    om_maire_residual(m, un, npc, Fpc, dudt);
+    ts = om_maire_time_step(m, cfl_acc, cfl_vol, cfl_growth, ts, a, dudt);
+    ts = smin(ts, final_time - t);
+    rc = om_maire_apply_update(m, 0.5 * ts, x0, un, dudt, V, M, uc, d, e, &bad);
+    if (rc) break;
+    om_maire_from_energy(nc, g, cv, d, e, p, T, a);
+    om_maire_nodal(m, uc, p, d, a, un, npc, Fpc);
+    om_maire_residual(m, un, npc, Fpc, dudt);
+    memcpy(m->x, x0, sizeof(double) * nv * 2);
+    memcpy(uc, uc0, sizeof(double) * nc * 2);
+    memcpy(e, ec0, sizeof(double) * nc);
+    rc = om_maire_apply_update(m, ts, x0, un, dudt, V, M, uc, d, e, &bad);
+    if (rc) break;
+    om_maire_from_energy(nc, g, cv, d, e, p, T, a);
+    t += ts;
+  }
+  *soln_time = t;
+  *steps_done = n;
+  *dt_io = ts;
+  free(x0); free(uc0); free(ec0); free(un); free(npc); free(Fpc); free(dudt);
+  return rc;
+}
