@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the Eulerian MUSCL step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload muscl2d|muscl3d|maire2d]
+
+Default workload (the configuration the metric is quoted on): second-order
+MUSCL (LSQ gradients + vertex limiter, HLLC flux) four-quadrant Riemann
+problem on a 1024x1024 quad mesh of [0,1]^2 (1,048,576 cells), gamma 1.4,
+FP64, 200 steps.  A "step" is one full time step (dt reduction + two
+gradient/flux/update half steps) of the whole mesh; a zone-update is one cell
+advanced one step.  The state is resident in HBM before the timed region
+(inputs uploaded, warm-up steps run, state reset on the device).
+
+Prints ONE JSON line on stdout.  `roofline` is the dominant kernel: its
+algorithmic bytes per launch (task-dataflow compulsory bytes: each task reads
+every array it consumes once and writes each output once) over its average
+launch time, measured with HIP events on the context's stream in a profiled
+replay of the same K steps.  `cpu_baseline` times the CPU restatement
+(oracle/, OpenMP over the host cores) on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0   # MI355X HBM3E peak
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes_euler(nc, nf, nv, dim):
+    """Algorithmic bytes per launch of each Eulerian kernel (FP64 8 B, int32 4 B)."""
+    if dim == 2:
+        # dt: v, a, V, cell->face CSR | n, A ; gradients: 6 vars, centroid, neighbour CSR,
+        # cell->vertex CSR, 12 gradients | coords ; fluxes: 6 vars + 12 gradients + centroid |
+        # centroid, n, A, face->cell, 4 fluxes ; update: d, v, e in, 7 state out, V, CSR | flux, face->cell
+        return {0: 52 * nc + 24 * nf, 1: 216 * nc + 16 * nv, 2: 160 * nc + 80 * nf, 3: 116 * nc + 40 * nf}
+    return {0: 68 * nc + 32 * nf, 1: 392 * nc + 24 * nv, 2: 248 * nc + 104 * nf, 3: 140 * nc + 48 * nf}
+
+
+def kernel_bytes_maire(nc, nv, ncn):
+    """Algorithmic bytes per launch of each Lagrangian kernel.
+
+    nodal: cell d, a, p, v + cell->vertex | vertex->corner CSR, coords, u_p out | corner npc, Fpc out;
+    residual: cell->vertex, dU out | u_p | npc, Fpc; dt: a, min_length, dV, V;
+    move: x0, u_p in, x out; update: geometry (cell->vertex, coords, V/centroid/min_length out)
+    + state (M, v, e, dU, gamma in; v, e, V, d, p, T, a out); save/restore: x, v, e copies."""
+    return {4: 56 * nc + 36 * nv + 36 * ncn, 5: 48 * nc + 16 * nv + 32 * ncn, 6: 32 * nc,
+            7: 48 * nv, 8: 184 * nc + 16 * nv, 9: 48 * nc + 32 * nv}
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline(prob, kind, budget_s=12.0):
+    """Oracle (CPU restatement, OpenMP) on the same workload for a bounded number of steps."""
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    nc = len(prob["cells"])
+    if kind == "euler":
+        s = O.euler_initial_state(mesh, prob["eos"][0], prob["eos"][1], prob["d"], prob["v"], prob["p"])
+        steps, t = 0, 0.0
+        t0 = time.perf_counter()
+        while steps < 2 or (time.perf_counter() - t0 < budget_s and steps < prob["max_steps"]):
+            rc, t, n = O.euler_run(mesh, s, prob["eos"][0], prob["eos"][1], prob["cfl"], prob["final_time"], 1,
+                                   flux_kind=prob["flux"], soln_time=t)
+            steps += 1
+        el = time.perf_counter() - t0
+    else:
+        s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
+        steps, t, dt = 0, 0.0, prob["dt0"]
+        t0 = time.perf_counter()
+        while steps < 2 or (time.perf_counter() - t0 < budget_s and steps < prob["max_steps"]):
+            rc, t, n, dt = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], 1, dt, soln_time=t)
+            steps += 1
+        el = time.perf_counter() - t0
+    return dict(value=nc * steps / el, unit="zone-updates/s", cores=threads, kind="port",
+                sample=f"first {steps} steps of the same workload ({nc} cells), {el:.1f} s, "
+                       f"OMP_NUM_THREADS={threads}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d"])
+    ap.add_argument("--size", type=int, default=None, help="override mesh size (cells per side)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                                   "(domain decomposition + halo exchange), not a measured path",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import flecsale_amd as fa
+    from flecsale_amd import inputs
+
+    t_build = time.perf_counter()
+    if args.workload == "muscl2d":
+        n = args.size or 1024
+        K = args.steps or 200
+        prob = inputs.quadrants_2d(n, steps=K, flux=fa.FLUX_HLLC | fa.BOUNDARY_TRANSMISSIVE)
+        kind, dim = "euler", 2
+        config = {"workload": f"MUSCL2D four-quadrant Riemann problem, {n}x{n} quads of [0,1]^2, "
+                              f"LSQ gradients + Barth-Jespersen vertex limiter, HLLC, CFL 0.5, gamma 1.4",
+                  "cells": n * n, "steps_per_run": K}
+    elif args.workload == "muscl3d":
+        n = args.size or 128
+        K = args.steps or 100
+        prob = inputs.kelvin_helmholtz_3d(n, steps=K)
+        kind, dim = "euler", 3
+        config = {"workload": f"MUSCL3D Kelvin-Helmholtz, {n}^3 hexes jittered +-10%, HLLE, CFL 1/3",
+                  "cells": n ** 3, "steps_per_run": K}
+    else:
+        n = args.size or 512
+        K = args.steps or 500
+        prob = inputs.maire_bubble_2d(n, steps=K)
+        kind, dim = "maire", 2
+        config = {"workload": f"Maire cell-centred Lagrangian, {n}x{n} quads jittered +-15%, two ideal "
+                              f"gases (bubble gamma 1.67), left-driven shock, symmetry walls",
+                  "cells": n * n, "steps_per_run": K}
+    log(f"[bench] inputs {config['workload']} built in {time.perf_counter() - t_build:.1f}s")
+    W = args.warmup
+
+    t0 = time.perf_counter()
+    if kind == "euler":
+        ctx = fa.EulerContext(prob["coords"], prob["cells"])
+
+        def reset():
+            ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+
+        def run(k, graph=True):
+            ctx.advance(prob["cfl"], prob["final_time"], k, flux_kind=prob["flux"], use_graph=graph)
+    else:
+        ctx = fa.MaireContext(prob["coords"], prob["cells"])
+
+        def reset():
+            ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
+            ctx.set_time(0.0, prob["dt0"])
+
+        def run(k, graph=True):
+            ctx.advance(prob["cfl"], prob["final_time"], k, use_graph=graph)
+    log(f"[bench] context (connectivity + upload + geometry) {time.perf_counter() - t0:.1f}s")
+
+    reset()
+    if W:
+        run(W)
+    ctx.sync()
+    reset()
+    ctx.sync()
+    log(f"[bench] warm-up {W} steps done; timing {K} steps")
+    t0 = time.perf_counter()
+    run(K)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    nc = ctx.nc
+    value = nc * K / el
+    t_sim, dt_last, steps_done = ctx.time()
+    log(f"[bench] {K} steps in {el * 1e3:.2f} ms: {value:.4e} zone-updates/s (t = {t_sim:.6g})")
+
+    # byte model
+    if kind == "euler":
+        kb = kernel_bytes_euler(nc, ctx.nf, ctx.nv, dim)
+        launches_per_step = {0: 1, 1: 2, 2: 2, 3: 2}
+    else:
+        kb = kernel_bytes_maire(nc, ctx.nv, nc * 4)
+        launches_per_step = {4: 2, 5: 2, 6: 1, 7: 2, 8: 2, 9: 2}
+    bytes_per_step = sum(kb[k] * launches_per_step[k] for k in kb)
+
+    roofline = None
+    kernels = {}
+    if not args.no_profile:
+        reset()
+        ctx.reset_profile()
+        ctx.set_profiling(True)
+        run(K, graph=False)
+        ctx.sync()
+        ctx.set_profiling(False)
+        for k in kb:
+            tot, nl = ctx.profile(k)
+            if nl:
+                avg_s = tot / nl * 1e-3
+                kernels[fa.KERNEL_NAMES[k]] = {"avg_us": avg_s * 1e6, "launches": nl,
+                                               "alg_bytes": kb[k], "GBps": kb[k] / avg_s / 1e9}
+        dom = max(kernels, key=lambda name: kernels[name]["avg_us"] * kernels[name]["launches"])
+        d = kernels[dom]
+        traffic = load_traffic(args.workload)
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1), "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": round(d["GBps"] / PEAK_HBM_GBS, 4),
+                    "traffic": (traffic or {}).get(dom)}
+        log("[bench] per-kernel: " + json.dumps({k: {kk: round(vv, 2) for kk, vv in v.items()}
+                                                  for k, v in kernels.items()}))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU restatement on a bounded sample")
+        cpu = cpu_baseline(prob, kind)
+        log(f"[bench] cpu baseline {cpu['value']:.4e} zone-updates/s ({cpu['sample']})")
+
+    ctx.close()
+    out = {
+        "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": el * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deterministic generated mesh and initial data)",
+        "config": config,
+        "alg_bytes_per_step": bytes_per_step,
+        "alg_GBps": bytes_per_step * K / el / 1e9,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,326 @@
+"""flecsale_amd -- MI355X implementation of FleCSALE's hydro time-step paths.
+
+Python binding (ctypes) over the C ABI of libflecsale_amd.so
+(include/flecsale_amd.h).  The classes mirror the reference's task interface:
+
+  EulerContext   apps/hydro/tasks.h       initial_conditions, evaluate_time_step,
+                                          evaluate_gradients, evaluate_fluxes,
+                                          apply_update (+ the driver loop: advance)
+  MaireContext   apps/maire_hydro/tasks.h initial_conditions, evaluate_nodal_state,
+                                          evaluate_residual, evaluate_time_step,
+                                          apply_update, save/restore (+ advance)
+
+Errors raise FsaleError with the library's status (the reference throws
+THROW_RUNTIME_ERROR at the same points).  The library must be built
+(python -m flecsale_amd.build); there is no CPU fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflecsale_amd.so")
+
+FLUX_HLLE, FLUX_RUSANOV, FLUX_HLLC = 0, 1, 2
+BOUNDARY_WALL, BOUNDARY_TRANSMISSIVE = 0x00, 0x10   # OR-ed into flux_kind
+SOLVER_EULER, SOLVER_MAIRE = 0, 1
+
+(F_DENSITY, F_VELOCITY, F_INTERNAL_ENERGY, F_PRESSURE, F_TEMPERATURE, F_SOUND_SPEED,
+ F_GRAD_DENSITY, F_GRAD_VELOCITY, F_GRAD_ENERGY, F_GRAD_PRESSURE, F_GRAD_SOUND_SPEED,
+ F_FLUX, F_CELL_CENTROID, F_CELL_VOLUME, F_CELL_MIN_LENGTH, F_FACE_CENTROID, F_FACE_NORMAL,
+ F_FACE_AREA, F_COORDS, F_MASS, F_VOLUME, F_NODE_VELOCITY, F_CORNER_NORMAL, F_CORNER_FORCE,
+ F_RESIDUAL, F_GAMMA) = range(26)
+
+K_EUL_DT, K_EUL_GRAD, K_EUL_FLUX, K_EUL_UPDATE = 0, 1, 2, 3
+K_MAIRE_NODAL, K_MAIRE_RESIDUAL, K_MAIRE_DT, K_MAIRE_MOVE, K_MAIRE_UPDATE, K_MAIRE_COPY = 4, 5, 6, 7, 8, 9
+KERNEL_NAMES = ["eul_dt", "eul_grad", "eul_flux", "eul_update", "maire_nodal", "maire_residual",
+                "maire_dt", "maire_move", "maire_update", "maire_copy"]
+
+ERR_NEG_STATE = 5
+ERR_DT = 6
+ERR_NO_DEVICE = 8
+
+# every symbol include/flecsale_amd.h declares
+EXPORTS = [
+    "fsale_ctx_create", "fsale_ctx_destroy", "fsale_sync", "fsale_mesh_counts", "fsale_get_topology",
+    "fsale_mesh_connectivity", "fsale_get_field", "fsale_set_field", "fsale_get_time", "fsale_set_time",
+    "fsale_status_string", "fsale_last_message", "fsale_error_cell", "fsale_set_profiling",
+    "fsale_get_profile", "fsale_reset_profile",
+    "fsale_eul_initial_conditions", "fsale_eul_evaluate_time_step", "fsale_eul_evaluate_gradients",
+    "fsale_eul_evaluate_fluxes", "fsale_eul_apply_update", "fsale_eul_advance",
+    "fsale_maire_initial_conditions", "fsale_maire_evaluate_nodal_state", "fsale_maire_evaluate_residual",
+    "fsale_maire_evaluate_time_step", "fsale_maire_apply_update", "fsale_maire_save", "fsale_maire_restore",
+    "fsale_maire_advance",
+]
+
+
+class FsaleError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+
+
+class MeshDesc(ct.Structure):
+    _fields_ = [("dim", ct.c_int32), ("num_cells", ct.c_int32), ("num_vertices", ct.c_int32),
+                ("verts_per_cell", ct.c_int32), ("coords", ct.c_void_p), ("cells", ct.c_void_p)]
+
+
+class Eos(ct.Structure):
+    _fields_ = [("gamma", ct.c_double), ("cv", ct.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it was not built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run python -m flecsale_amd.build")
+        L = ct.CDLL(LIB_PATH)
+        vp, i32, dp = ct.c_void_p, ct.c_int32, ct.c_double
+        L.fsale_status_string.restype = ct.c_char_p
+        L.fsale_status_string.argtypes = [ct.c_int]
+        L.fsale_last_message.restype = ct.c_char_p
+        L.fsale_ctx_create.argtypes = [ct.POINTER(MeshDesc), ct.c_int, ct.POINTER(vp)]
+        L.fsale_ctx_destroy.argtypes = [vp]
+        L.fsale_sync.argtypes = [vp]
+        L.fsale_mesh_counts.argtypes = [vp, vp]
+        L.fsale_get_topology.argtypes = [vp, vp, vp, vp, vp]
+        L.fsale_mesh_connectivity.argtypes = [ct.POINTER(MeshDesc), vp, vp, vp, vp, vp]
+        L.fsale_get_field.argtypes = [vp, ct.c_int, vp]
+        L.fsale_set_field.argtypes = [vp, ct.c_int, vp]
+        L.fsale_get_time.argtypes = [vp, ct.POINTER(dp), ct.POINTER(dp), ct.POINTER(ct.c_int64)]
+        L.fsale_set_time.argtypes = [vp, dp, dp]
+        L.fsale_error_cell.argtypes = [vp, ct.POINTER(ct.c_int64)]
+        L.fsale_set_profiling.argtypes = [vp, ct.c_int]
+        L.fsale_get_profile.argtypes = [vp, ct.c_int, ct.POINTER(dp), ct.POINTER(ct.c_int64)]
+        L.fsale_reset_profile.argtypes = [vp]
+        L.fsale_eul_initial_conditions.argtypes = [vp, Eos, vp, vp, vp]
+        L.fsale_eul_evaluate_time_step.argtypes = [vp, dp, dp, ct.POINTER(dp)]
+        L.fsale_eul_evaluate_gradients.argtypes = [vp, ct.c_int]
+        L.fsale_eul_evaluate_fluxes.argtypes = [vp, ct.c_int]
+        L.fsale_eul_apply_update.argtypes = [vp, Eos, dp]
+        L.fsale_eul_advance.argtypes = [vp, Eos, dp, dp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]
+        L.fsale_maire_initial_conditions.argtypes = [vp, dp, vp, vp, vp, vp]
+        L.fsale_maire_evaluate_nodal_state.argtypes = [vp]
+        L.fsale_maire_evaluate_residual.argtypes = [vp]
+        L.fsale_maire_evaluate_time_step.argtypes = [vp, dp, dp, dp, dp, ct.POINTER(dp)]
+        L.fsale_maire_apply_update.argtypes = [vp, dp]
+        L.fsale_maire_save.argtypes = [vp]
+        L.fsale_maire_restore.argtypes = [vp]
+        L.fsale_maire_advance.argtypes = [vp, dp, dp, dp, dp, ct.c_int, ct.c_int]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if f.restype is ct.c_int or f.restype is None:
+                f.restype = ct.c_int
+        L.fsale_status_string.restype = ct.c_char_p
+        L.fsale_last_message.restype = ct.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        L = lib()
+        raise FsaleError(rc, (L.fsale_last_message() or L.fsale_status_string(rc)).decode())
+
+
+def _desc(coords, cells):
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    d = MeshDesc(coords.shape[1], cells.shape[0], coords.shape[0], cells.shape[1],
+                 coords.ctypes.data, cells.ctypes.data)
+    return d, (coords, cells)
+
+
+def connectivity(coords, cells):
+    """Host-only connectivity (fsale_mesh_connectivity): face_cells, cell_faces, nbr_off, nbr."""
+    d, keep = _desc(coords, cells)
+    counts = np.zeros(5, np.int32)
+    _check(lib().fsale_mesh_connectivity(ct.byref(d), counts.ctypes.data, None, None, None, None))
+    nc, nv, nf, nn, dim = [int(x) for x in counts]
+    fpc = cells.shape[1] if dim == 2 else 6
+    fc = np.zeros((nf, 2), np.int32)
+    cf = np.zeros((nc, fpc), np.int32)
+    off = np.zeros(nc + 1, np.int32)
+    nbr = np.zeros(max(nn, 1), np.int32)
+    _check(lib().fsale_mesh_connectivity(ct.byref(d), counts.ctypes.data, fc.ctypes.data, cf.ctypes.data,
+                                         off.ctypes.data, nbr.ctypes.data))
+    return dict(face_cells=fc, cell_faces=cf, nbr_off=off, nbr=nbr[:nn])
+
+
+class _Context:
+    solver = None
+
+    def __init__(self, coords, cells):
+        d, keep = _desc(coords, cells)
+        h = ct.c_void_p()
+        _check(lib().fsale_ctx_create(ct.byref(d), self.solver, ct.byref(h)))
+        self.h = h
+        self.dim = d.dim
+        counts = np.zeros(5, np.int32)
+        _check(lib().fsale_mesh_counts(self.h, counts.ctypes.data))
+        self.nc, self.nv, self.nf, self.nnbr, _ = [int(x) for x in counts]
+        self.vpc = d.verts_per_cell
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fsale_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    _shapes = None
+
+    def _shape(self, field):
+        N, nc, nf, nv = self.dim, self.nc, self.nf, self.nv
+        table = {
+            F_DENSITY: (nc,), F_VELOCITY: (nc, N), F_INTERNAL_ENERGY: (nc,), F_PRESSURE: (nc,),
+            F_TEMPERATURE: (nc,), F_SOUND_SPEED: (nc,), F_GRAD_DENSITY: (nc, N),
+            F_GRAD_VELOCITY: (nc, N, N), F_GRAD_ENERGY: (nc, N), F_GRAD_PRESSURE: (nc, N),
+            F_GRAD_SOUND_SPEED: (nc, N), F_FLUX: (nf, N + 2), F_CELL_CENTROID: (nc, N),
+            F_CELL_VOLUME: (nc,), F_CELL_MIN_LENGTH: (nc,), F_FACE_CENTROID: (nf, N),
+            F_FACE_NORMAL: (nf, N), F_FACE_AREA: (nf,), F_COORDS: (nv, N), F_MASS: (nc,),
+            F_VOLUME: (nc,), F_NODE_VELOCITY: (nv, N), F_CORNER_NORMAL: (nc * self.vpc, N),
+            F_CORNER_FORCE: (nc * self.vpc, N), F_RESIDUAL: (nc, 4), F_GAMMA: (nc,),
+        }
+        return table[field]
+
+    def get(self, field):
+        out = np.zeros(self._shape(field))
+        _check(lib().fsale_get_field(self.h, field, out.ctypes.data))
+        return out
+
+    def set(self, field, values):
+        a = np.ascontiguousarray(np.broadcast_to(values, self._shape(field)), dtype=np.float64)
+        _check(lib().fsale_set_field(self.h, field, a.ctypes.data))
+
+    def sync(self):
+        _check(lib().fsale_sync(self.h))
+
+    def time(self):
+        t, dt, n = ct.c_double(), ct.c_double(), ct.c_int64()
+        _check(lib().fsale_get_time(self.h, ct.byref(t), ct.byref(dt), ct.byref(n)))
+        return t.value, dt.value, n.value
+
+    def set_time(self, t, dt):
+        _check(lib().fsale_set_time(self.h, t, dt))
+
+    def error_cell(self):
+        c = ct.c_int64()
+        _check(lib().fsale_error_cell(self.h, ct.byref(c)))
+        return c.value
+
+    def set_profiling(self, enable):
+        _check(lib().fsale_set_profiling(self.h, int(enable)))
+
+    def profile(self, kernel):
+        """(total milliseconds, launches) of one kernel id since the last reset."""
+        t, n = ct.c_double(), ct.c_int64()
+        _check(lib().fsale_get_profile(self.h, int(kernel), ct.byref(t), ct.byref(n)))
+        return t.value, n.value
+
+    def reset_profile(self):
+        _check(lib().fsale_reset_profile(self.h))
+
+    def topology(self):
+        fpc = self.vpc if self.dim == 2 else 6
+        fc = np.zeros((self.nf, 2), np.int32)
+        cf = np.zeros((self.nc, fpc), np.int32)
+        off = np.zeros(self.nc + 1, np.int32)
+        nbr = np.zeros(max(self.nnbr, 1), np.int32)
+        _check(lib().fsale_get_topology(self.h, fc.ctypes.data, cf.ctypes.data, off.ctypes.data,
+                                        nbr.ctypes.data))
+        return dict(face_cells=fc, cell_faces=cf, nbr_off=off, nbr=nbr[:self.nnbr])
+
+    def state(self):
+        return dict(d=self.get(F_DENSITY), v=self.get(F_VELOCITY), e=self.get(F_INTERNAL_ENERGY),
+                    p=self.get(F_PRESSURE), T=self.get(F_TEMPERATURE), a=self.get(F_SOUND_SPEED))
+
+
+class EulerContext(_Context):
+    """Device-resident apps/hydro state over one mesh."""
+    solver = SOLVER_EULER
+
+    def initial_conditions(self, eos, d, v, p):
+        self.eos = Eos(*eos)
+        d = np.ascontiguousarray(np.broadcast_to(d, (self.nc,)), dtype=np.float64)
+        v = np.ascontiguousarray(np.broadcast_to(v, (self.nc, self.dim)), dtype=np.float64)
+        p = np.ascontiguousarray(np.broadcast_to(p, (self.nc,)), dtype=np.float64)
+        _check(lib().fsale_eul_initial_conditions(self.h, self.eos, d.ctypes.data, v.ctypes.data,
+                                                  p.ctypes.data))
+
+    def evaluate_time_step(self, cfl, final_time, sync=True):
+        dt = ct.c_double()
+        _check(lib().fsale_eul_evaluate_time_step(self.h, cfl, final_time, ct.byref(dt) if sync else None))
+        return dt.value if sync else None
+
+    def evaluate_gradients(self, first_order=False):
+        _check(lib().fsale_eul_evaluate_gradients(self.h, int(first_order)))
+
+    def evaluate_fluxes(self, flux_kind=FLUX_HLLE):
+        _check(lib().fsale_eul_evaluate_fluxes(self.h, int(flux_kind)))
+
+    def apply_update(self, factor):
+        _check(lib().fsale_eul_apply_update(self.h, self.eos, factor))
+
+    def advance(self, cfl, final_time, nsteps, flux_kind=FLUX_HLLE, first_order=False,
+                stop_at_final=False, use_graph=True):
+        _check(lib().fsale_eul_advance(self.h, self.eos, cfl, final_time, int(nsteps), int(flux_kind),
+                                       int(first_order), int(stop_at_final), int(use_graph)))
+
+
+class MaireContext(_Context):
+    """Device-resident apps/maire_hydro state over one 2D quad mesh."""
+    solver = SOLVER_MAIRE
+
+    def initial_conditions(self, cv, gamma, d, v, p):
+        g = np.ascontiguousarray(np.broadcast_to(gamma, (self.nc,)), dtype=np.float64)
+        d = np.ascontiguousarray(np.broadcast_to(d, (self.nc,)), dtype=np.float64)
+        v = np.ascontiguousarray(np.broadcast_to(v, (self.nc, 2)), dtype=np.float64)
+        p = np.ascontiguousarray(np.broadcast_to(p, (self.nc,)), dtype=np.float64)
+        _check(lib().fsale_maire_initial_conditions(self.h, cv, g.ctypes.data, d.ctypes.data,
+                                                    v.ctypes.data, p.ctypes.data))
+
+    def evaluate_nodal_state(self):
+        _check(lib().fsale_maire_evaluate_nodal_state(self.h))
+
+    def evaluate_residual(self):
+        _check(lib().fsale_maire_evaluate_residual(self.h))
+
+    def evaluate_time_step(self, cfl, final_time, sync=True):
+        dt = ct.c_double()
+        _check(lib().fsale_maire_evaluate_time_step(self.h, cfl[0], cfl[1], cfl[2], final_time,
+                                                    ct.byref(dt) if sync else None))
+        return dt.value if sync else None
+
+    def apply_update(self, factor):
+        _check(lib().fsale_maire_apply_update(self.h, factor))
+
+    def save(self):
+        _check(lib().fsale_maire_save(self.h))
+
+    def restore(self):
+        _check(lib().fsale_maire_restore(self.h))
+
+    def advance(self, cfl, final_time, nsteps, use_graph=True):
+        _check(lib().fsale_maire_advance(self.h, cfl[0], cfl[1], cfl[2], final_time, int(nsteps),
+                                         int(use_graph)))
+
+    def state(self):
+        s = super().state()
+        s.update(M=self.get(F_MASS), V=self.get(F_VOLUME))
+        return s

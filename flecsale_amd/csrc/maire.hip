@@ -1,0 +1,553 @@
+// Cell-centred Lagrangian (Maire) step of apps/maire_hydro on the device, 2D
+// quads.
+//
+//   k_maire_nodal     evaluate_nodal_state      apps/maire_hydro/tasks.h:221-453
+//   k_maire_residual  evaluate_residual         apps/maire_hydro/tasks.h:461-489
+//   k_maire_dt        evaluate_time_step        apps/maire_hydro/tasks.h:142-190
+//   k_maire_move      apply_update (mesh move)  apps/maire_hydro/tasks.h:517-534
+//   k_maire_update    apply_update (geometry + state) + update_state_from_energy
+//                     apps/maire_hydro/tasks.h:536-546, :110-133
+//   k_maire_copy      save_/restore_coordinates, save_/restore_solution :557-654
+//
+// Wedge facets (half edges, burton's wedge facet_normal/facet_area) are
+// recomputed from the vertex coordinates where they are needed instead of
+// being stored: the mesh moves every half step.  Corner arrays are k-major
+// [4][nc] so the residual gather of a cell's four corners is coalesced.
+#include "internal.h"
+#include "kernels_common.h"
+
+namespace fsale {
+namespace {
+
+constexpr double kMinSoundSpeed = 1.e-6;   // flecsale/eqns/lagrange_eqns.h:54
+
+// ------------------------------------------------------------ state init
+__global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  // initial_conditions apps/maire_hydro/tasks.h:86-100 (u.d holds rho on entry)
+  const double vol = m.cvol[c];
+  const double M = u.d[c] * vol;
+  u.M[c] = M;
+  u.V[c] = vol;
+  // update_state_from_pressure lagrange_eqns.h:245-271
+  const double d = M / vol;
+  const double g = u.g[c];
+  const double ie = eos_e_dp(g, d, u.p[c]);
+  u.d[c] = d;
+  u.e[c] = ie;
+  u.T[c] = eos_T_de(u.cv, ie);
+  u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
+}
+
+// wedge facets of corner (c, k): wedge 0 on edge v_{k-1} -> v_k, wedge 1 on
+// edge v_k -> v_{k+1}; outward unit edge normal, facet_area = |edge
+// midpoint - vertex| (burton wedge geometry; pinned by the sedov_2d golden).
+__device__ inline void wedge_facets(const DevMesh &m, int c, int k, double n[2][2], double l[2]) {
+  const int km = (k + 3) & 3, kp = (k + 1) & 3;
+  const int im = m.cv[(size_t)km * m.nc + c], ik = m.cv[(size_t)k * m.nc + c],
+            ip = m.cv[(size_t)kp * m.nc + c];
+  const double xm = m.x[0][im], ym = m.x[1][im];
+  const double xk = m.x[0][ik], yk = m.x[1][ik];
+  const double xp = m.x[0][ip], yp = m.x[1][ip];
+  const double ax[2] = {xm, xk}, ay[2] = {ym, yk}, bx[2] = {xk, xp}, by[2] = {yk, yp};
+#pragma unroll
+  for (int w = 0; w < 2; ++w) {
+    const double tx = bx[w] - ax[w], ty = by[w] - ay[w];
+    const double len = sqrt(tx * tx + ty * ty);
+    n[w][0] = ty / len;
+    n[w][1] = -tx / len;
+    const double mx = 0.5 * (ax[w] + bx[w]) - xk;
+    const double my = 0.5 * (ay[w] + by[w]) - yk;
+    l[w] = sqrt(mx * mx + my * my);
+  }
+}
+
+// corner matrix M_pc = sum_w z_c l_w n_w n_w^T (ristra outer_product with
+// factor z_c l_w) and (l n)_pc = sum_w l_w n_w.
+__device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int c, int k, double M[2][2],
+                                     double np[2]) {
+  double n[2][2], l[2];
+  wedge_facets(m, c, k, n, l);
+  const double zc = u.d[c] * u.a[c];
+  M[0][0] = M[0][1] = M[1][0] = M[1][1] = 0;
+  np[0] = np[1] = 0;
+#pragma unroll
+  for (int w = 0; w < 2; ++w) {
+    const double fact = zc * l[w];
+    for (int r = 0; r < 2; ++r)
+      for (int q = 0; q < 2; ++q) M[r][q] += fact * n[w][r] * n[w][q];
+    for (int d = 0; d < 2; ++d) np[d] += l[w] * n[w][d];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, const DevScalars *s) {
+  const int vt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (vt >= m.nv || !s->active) return;
+  const int q0 = m.vc_off[vt], q1 = m.vc_off[vt + 1];
+  double Mp[2][2] = {{0, 0}, {0, 0}}, rhs[2] = {0, 0};
+  for (int q = q0; q < q1; ++q) {
+    const int cn = m.vc[q];
+    const int c = cn >> 2, k = cn & 3;
+    double M[2][2], np[2];
+    corner_matrix(m, u, c, k, M, np);
+    for (int r = 0; r < 2; ++r) {
+      Mp[r][0] += M[r][0];
+      Mp[r][1] += M[r][1];
+    }
+    const double uc[2] = {u.v[0][c], u.v[1][c]};
+    const double pc = u.p[c];
+    double F[2];
+    for (int r = 0; r < 2; ++r) {
+      F[r] = 0;
+      F[r] += M[r][0] * uc[0];
+      F[r] += M[r][1] * uc[1];
+    }
+    for (int d = 0; d < 2; ++d) {
+      F[d] += pc * np[d];
+      rhs[d] += F[d];
+    }
+    const size_t ci = (size_t)k * m.nc + c;
+    u.npc[0][ci] = np[0];
+    u.npc[1][ci] = np[1];
+    u.Fpc[0][ci] = F[0];
+    u.Fpc[1][ci] = F[1];
+  }
+  double un[2];
+  int has[2] = {0, 0};
+  double symn[2][2] = {{0, 0}, {0, 0}};
+  if (m.vbnd[vt]) {
+    // symmetry normals, accumulated per tag over the vertex's boundary wedges
+    for (int q = q0; q < q1; ++q) {
+      const int cn = m.vc[q];
+      const int c = cn >> 2, k = cn & 3;
+      const int tags = m.cwtag[(size_t)k * m.nc + c];
+      if (!(tags & 0x44)) continue;
+      double n[2][2], l[2];
+      wedge_facets(m, c, k, n, l);
+      for (int w = 0; w < 2; ++w) {
+        const int t = (tags >> (4 * w)) & 0xf;
+        if (!(t & 4)) continue;
+        for (int tag = 0; tag < 2; ++tag) {
+          if (!(t & (1 << tag))) continue;
+          if (has[tag]) {
+            for (int d = 0; d < 2; ++d) symn[tag][d] += l[w] * n[w][d];
+          } else {
+            has[tag] = 1;
+            for (int d = 0; d < 2; ++d) symn[tag][d] = l[w] * n[w][d];
+          }
+        }
+      }
+    }
+  }
+  const int nsym = has[0] + has[1];
+  if (nsym == 0) {
+    // ristra::math::solve (un-vendored): Cramer's rule
+    const double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
+    un[0] = (rhs[0] * Mp[1][1] - Mp[0][1] * rhs[1]) / det;
+    un[1] = (Mp[0][0] * rhs[1] - rhs[0] * Mp[1][0]) / det;
+  } else {
+    // augmented system [M_p N^T; N 0] solved by flecsale::linalg::qr
+    // (apps/maire_hydro/tasks.h:383-418)
+    const int nr = 2 + nsym;
+    double A[16], b[4] = {rhs[0], rhs[1], 0, 0};
+    for (int i = 0; i < 16; ++i) A[i] = 0;
+    for (int r = 0; r < 2; ++r)
+      for (int q = 0; q < 2; ++q) A[r * nr + q] = Mp[r][q];
+    int jj = 2;
+    for (int tag = 0; tag < 2; ++tag) {
+      if (!has[tag]) continue;
+      for (int r = 0; r < 2; ++r) A[r * nr + jj] = symn[tag][r];
+      for (int q = 0; q < 2; ++q) A[jj * nr + q] = symn[tag][q];
+      ++jj;
+    }
+    qr_solve<4>(nr, A, b);
+    un[0] = b[0];
+    un[1] = b[1];
+  }
+  u.un[0][vt] = un[0];
+  u.un[1][vt] = un[1];
+  // F_pc -= M_pc u_p  (matrix_vector(-1, Mpc, un, 1, Fpc), tasks.h:435-448)
+  for (int q = q0; q < q1; ++q) {
+    const int cn = m.vc[q];
+    const int c = cn >> 2, k = cn & 3;
+    double M[2][2], np[2];
+    corner_matrix(m, u, c, k, M, np);
+    const size_t ci = (size_t)k * m.nc + c;
+    for (int r = 0; r < 2; ++r) {
+      double t = M[r][0] * un[0];
+      t += M[r][1] * un[1];
+      u.Fpc[r][ci] = -1.0 * t + 1.0 * u.Fpc[r][ci];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, const DevScalars *s) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc || !s->active) return;
+  double r[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t ci = (size_t)k * m.nc + c;
+    const int vt = m.cv[ci];
+    const double uv[2] = {u.un[0][vt], u.un[1][vt]};
+    const double F[2] = {u.Fpc[0][ci], u.Fpc[1][ci]};
+    const double n[2] = {u.npc[0][ci], u.npc[1][ci]};
+    // compute_update lagrange_eqns.h:374-389
+    for (int d = 0; d < 2; ++d) {
+      r[0] += n[d] * uv[d];
+      r[3] -= F[d] * uv[d];
+      r[1 + d] -= F[d];
+    }
+  }
+  for (int i = 0; i < 4; ++i) u.dudt[i][c] = r[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_maire_dt(DevMesh m, MaireFields u, DevScalars *s, double cfl_acc,
+                                                     double cfl_vol, double cfl_growth, double final_time) {
+  __shared__ double red[2][kBlock / 64];
+  __shared__ int last;
+  if (!s->active) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0, vol = 0;
+  if (c < m.nc) {
+    acc = u.a[c] / m.cminlen[c];
+    vol = fabs(u.dudt[0][c]) / m.cvol[c];
+  }
+  acc = wave_max(acc);
+  vol = wave_max(vol);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = acc;
+    red[1][threadIdx.x >> 6] = vol;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b0 = red[0][0], b1 = red[1][0];
+    for (int w = 1; w < kBlock / 64; ++w) {
+      b0 = smax(red[0][w], b0);
+      b1 = smax(red[1][w], b1);
+    }
+    atomic_max_nonneg(&s->red[0], b0);
+    atomic_max_nonneg(&s->red[1], b1);
+    __threadfence();
+    const unsigned prev = atomicAdd(&s->done, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    unsigned long long *r = reinterpret_cast<unsigned long long *>(s->red);
+    const double acc_inv = __longlong_as_double((long long)atomicAdd(&r[0], 0ull));
+    const double vol_inv = __longlong_as_double((long long)atomicAdd(&r[1], 0ull));
+    s->red[0] = 0;
+    s->red[1] = 0;
+    s->done = 0;
+    // evaluate_time_step :179-188: first minimum of the three limits
+    const double dts[3] = {cfl_acc / acc_inv, cfl_vol / vol_inv, cfl_growth * s->dt};
+    int it = 0;
+    for (int i = 1; i < 3; ++i)
+      if (dts[i] < dts[it]) it = i;
+    // driver.h:391
+    s->dt = smin(dts[it], final_time - s->t);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u, const DevScalars *s,
+                                                       double factor) {
+  const int vt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (vt >= m.nv || !s->active) return;
+  const double delta_t = factor * s->dt;
+  if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
+  for (int d = 0; d < 2; ++d) m.x[d][vt] = u.x0[d][vt] + delta_t * u.un[d][vt];
+}
+
+__global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, DevScalars *s,
+                                                         double factor) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc || !s->active) return;
+  const double delta_t = factor * s->dt;
+  double vol;
+  if (delta_t > 1e-14) {
+    // burton update_geometry() for the moved cell
+    double px[4], py[4];
+    for (int i = 0; i < 4; ++i) {
+      const int v = m.cv[(size_t)i * m.nc + c];
+      px[i] = m.x[0][v];
+      py[i] = m.x[1][v];
+    }
+    double cx, cy, ml;
+    cell_geom_2d(4, px, py, cx, cy, vol, ml);
+    m.ccen[0][c] = cx;
+    m.ccen[1][c] = cy;
+    m.cvol[c] = vol;
+    m.cminlen[c] = ml;
+  } else {
+    vol = m.cvol[c];
+  }
+  // update_state_from_flux lagrange_eqns.h:310-343
+  const double M = u.M[c];
+  const double inv_mass = 1 / M;
+  double vel[2] = {u.v[0][c], u.v[1][c]};
+  const double et = u.e[c] + dotN<2>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[3][c];
+  for (int i = 0; i < 2; ++i) vel[i] += delta_t * inv_mass * u.dudt[1 + i][c];
+  const double ie = et - dotN<2>(vel, vel) / 2;
+  // update_volume :352-367
+  const double d = M / vol;
+  u.v[0][c] = vel[0];
+  u.v[1][c] = vel[1];
+  u.e[c] = ie;
+  u.V[c] = vol;
+  u.d[c] = d;
+  // update_state_from_energy lagrange_eqns.h:280-302
+  const double g = u.g[c];
+  u.p[c] = eos_p_de(g, d, ie);
+  u.T[c] = eos_T_de(u.cv, ie);
+  u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
+  if (ie < 0 || d < 0) {
+    atomicCAS(&s->err_code, 0, (int)FSALE_ERR_NEG_STATE);
+    atomicMin(reinterpret_cast<unsigned long long *>(&s->err_cell), (unsigned long long)c);
+    s->active = 0;
+  }
+}
+
+// save (dir 0) / restore (dir 1) of coordinates, velocity and energy
+__global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u, const DevScalars *s, int dir) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!s->active) return;
+  if (i < m.nv) {
+    for (int d = 0; d < 2; ++d) {
+      if (dir == 0)
+        u.x0[d][i] = m.x[d][i];
+      else
+        m.x[d][i] = u.x0[d][i];
+    }
+  }
+  if (i < m.nc) {
+    if (dir == 0) {
+      u.v0[0][i] = u.v[0][i];
+      u.v0[1][i] = u.v[1][i];
+      u.e0[i] = u.e[i];
+    } else {
+      u.v[0][i] = u.v0[0][i];
+      u.v[1][i] = u.v0[1][i];
+      u.e[i] = u.e0[i];
+    }
+  }
+}
+
+// step bracket of the driver loop (apps/maire_hydro/driver.h:332,514-515)
+__global__ void k_maire_begin(DevScalars *s, double final_time) {
+  s->active = (s->err_code == 0) && (s->t < final_time);
+}
+__global__ void k_maire_end(DevScalars *s) {
+  if (!s->active) return;
+  s->t += s->dt;
+  s->steps = s->steps + 1;
+}
+
+inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
+
+bool check_ctx(fsale_ctx *ctx) {
+  if (!ctx || ctx->solver != FSALE_SOLVER_MAIRE) {
+    set_message("not a Lagrangian context");
+    return false;
+  }
+  return true;
+}
+
+int launch_nodal(fsale_ctx *ctx) {
+  ProfScope prof(ctx, K_MAIRE_NODAL);
+  hipLaunchKernelGGL(k_maire_nodal, dim3(grid(ctx->m.nv)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+int launch_residual(fsale_ctx *ctx) {
+  ProfScope prof(ctx, K_MAIRE_RESIDUAL);
+  hipLaunchKernelGGL(k_maire_residual, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
+                     ctx->s);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+int launch_dt(fsale_ctx *ctx, double ca, double cv, double cg, double tf) {
+  ProfScope prof(ctx, K_MAIRE_DT);
+  hipLaunchKernelGGL(k_maire_dt, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s, ca,
+                     cv, cg, tf);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+int launch_apply(fsale_ctx *ctx, double factor) {
+  {
+    ProfScope prof(ctx, K_MAIRE_MOVE);
+    hipLaunchKernelGGL(k_maire_move, dim3(grid(ctx->m.nv)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s,
+                       factor);
+    FSALE_HIP(hipGetLastError());
+  }
+  ProfScope prof(ctx, K_MAIRE_UPDATE);
+  hipLaunchKernelGGL(k_maire_update, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
+                     ctx->s, factor);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+int launch_copy(fsale_ctx *ctx, int dir) {
+  ProfScope prof(ctx, K_MAIRE_COPY);
+  const int n = ctx->m.nv > ctx->m.nc ? ctx->m.nv : ctx->m.nc;
+  hipLaunchKernelGGL(k_maire_copy, dim3(grid(n)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s, dir);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
+int launch_step(fsale_ctx *ctx, double ca, double cv, double cg, double tf) {
+  int rc;
+  hipLaunchKernelGGL(k_maire_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->s, tf);
+  FSALE_HIP(hipGetLastError());
+  if ((rc = launch_copy(ctx, 0))) return rc;
+  if ((rc = launch_nodal(ctx))) return rc;
+  if ((rc = launch_residual(ctx))) return rc;
+  if ((rc = launch_dt(ctx, ca, cv, cg, tf))) return rc;
+  if ((rc = launch_apply(ctx, 0.5))) return rc;
+  if ((rc = launch_nodal(ctx))) return rc;
+  if ((rc = launch_residual(ctx))) return rc;
+  if ((rc = launch_copy(ctx, 1))) return rc;
+  if ((rc = launch_apply(ctx, 1.0))) return rc;
+  hipLaunchKernelGGL(k_maire_end, dim3(1), dim3(1), 0, ctx->stream, ctx->s);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
+}  // namespace
+
+int maire_alloc(fsale_ctx *ctx) {
+  const DevMesh &m = ctx->m;
+  MaireFields &u = ctx->ma;
+  int rc;
+  double **cell1[] = {&u.V, &u.M, &u.p, &u.d, &u.e, &u.T, &u.a, &u.g, &u.e0};
+  for (double **p : cell1)
+    if ((rc = dev_alloc_n(ctx, p, m.nc))) return rc;
+  for (int d = 0; d < 2; ++d) {
+    if ((rc = dev_alloc_n(ctx, &u.v[d], m.nc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.v0[d], m.nc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.x0[d], m.nv))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.un[d], m.nv))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.npc[d], (size_t)m.nc * m.vpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.Fpc[d], (size_t)m.nc * m.vpc))) return rc;
+  }
+  for (int i = 0; i < 4; ++i)
+    if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.nc))) return rc;
+  return FSALE_OK;
+}
+
+}  // namespace fsale
+
+using namespace fsale;
+
+extern "C" {
+
+int fsale_maire_initial_conditions(fsale_ctx *ctx, double cv, const double *gamma, const double *d,
+                                   const double *v, const double *p) {
+  if (!check_ctx(ctx) || !gamma || !d || !v || !p || !(cv > 0)) return FSALE_ERR_ARG;
+  for (int c = 0; c < ctx->m.nc; ++c)
+    if (!(gamma[c] > 1)) return FSALE_ERR_ARG;
+  int rc;
+  if ((rc = fsale_set_field(ctx, FSALE_F_GAMMA, gamma))) return rc;
+  if ((rc = fsale_set_field(ctx, FSALE_F_DENSITY, d))) return rc;
+  if ((rc = fsale_set_field(ctx, FSALE_F_VELOCITY, v))) return rc;
+  if ((rc = fsale_set_field(ctx, FSALE_F_PRESSURE, p))) return rc;
+  ctx->ma.cv = cv;
+  hipLaunchKernelGGL(k_maire_ic, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma);
+  FSALE_HIP(hipGetLastError());
+  ctx->have_state = true;
+  if ((rc = read_scalars(ctx))) return rc;
+  DevScalars h = *ctx->h_s;
+  h.t = 0;
+  h.dt = 0;
+  h.steps = 0;
+  h.err_code = 0;
+  h.err_cell = INT64_MAX;
+  h.active = 1;
+  h.red[0] = h.red[1] = 0;
+  h.done = 0;
+  *ctx->h_s = h;
+  FSALE_HIP(hipMemcpyAsync(ctx->s, ctx->h_s, sizeof(DevScalars), hipMemcpyHostToDevice, ctx->stream));
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  return FSALE_OK;
+}
+
+int fsale_maire_evaluate_nodal_state(fsale_ctx *ctx) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  if (!ctx->have_state) return FSALE_ERR_STATE;
+  return launch_nodal(ctx);
+}
+
+int fsale_maire_evaluate_residual(fsale_ctx *ctx) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  if (!ctx->have_state) return FSALE_ERR_STATE;
+  return launch_residual(ctx);
+}
+
+int fsale_maire_evaluate_time_step(fsale_ctx *ctx, double cfl_accoustic, double cfl_volume, double cfl_growth,
+                                   double final_time, double *dt_out) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  if (!ctx->have_state) return FSALE_ERR_STATE;
+  int rc = launch_dt(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time);
+  if (rc || !dt_out) return rc;
+  if ((rc = check_device_error(ctx))) return rc;
+  *dt_out = ctx->h_s->dt;
+  return FSALE_OK;
+}
+
+int fsale_maire_apply_update(fsale_ctx *ctx, double factor) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  if (!ctx->have_state) return FSALE_ERR_STATE;
+  return launch_apply(ctx, factor);
+}
+
+int fsale_maire_save(fsale_ctx *ctx) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  return launch_copy(ctx, 0);
+}
+
+int fsale_maire_restore(fsale_ctx *ctx) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  return launch_copy(ctx, 1);
+}
+
+int fsale_maire_advance(fsale_ctx *ctx, double cfl_accoustic, double cfl_volume, double cfl_growth,
+                        double final_time, int nsteps, int use_graph) {
+  if (!check_ctx(ctx) || nsteps < 0) return FSALE_ERR_ARG;
+  if (!ctx->have_state) return FSALE_ERR_STATE;
+  int rc;
+  if (!use_graph || nsteps < 2 || ctx->profiling) {
+    for (int n = 0; n < nsteps; ++n)
+      if ((rc = launch_step(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time))) return rc;
+    return FSALE_OK;
+  }
+  const int chunk = nsteps < 10 ? nsteps : 10;
+  std::vector<double> key = {1.0, cfl_accoustic, cfl_volume, cfl_growth, final_time};
+  if (!ctx->graph.exec || ctx->graph.key != key || ctx->graph.steps != chunk) {
+    if (ctx->graph.exec) {
+      (void)hipGraphExecDestroy(ctx->graph.exec);
+      ctx->graph.exec = nullptr;
+    }
+    hipGraph_t graph;
+    FSALE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int n = 0; n < chunk; ++n) {
+      rc = launch_step(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time);
+      if (rc) {
+        (void)hipStreamEndCapture(ctx->stream, &graph);
+        return rc;
+      }
+    }
+    FSALE_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    FSALE_HIP(hipGraphInstantiate(&ctx->graph.exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    ctx->graph.key = key;
+    ctx->graph.steps = chunk;
+  }
+  int n = 0;
+  for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(ctx->graph.exec, ctx->stream));
+  for (; n < nsteps; ++n)
+    if ((rc = launch_step(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time))) return rc;
+  return FSALE_OK;
+}
+
+}  // extern "C"

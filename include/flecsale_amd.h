@@ -1,0 +1,195 @@
+/*
+ * flecsale_amd.h -- C ABI of the MI355X implementation of FleCSALE's two hydro
+ * time-step paths (apps/hydro Eulerian MUSCL, apps/maire_hydro cell-centred
+ * Lagrangian).
+ *
+ * Each entry point replaces one FleCSI task body of the reference; the task's
+ * accessor arguments become fields owned by an opaque context that lives in
+ * device memory (SoA, FP64; int32 connectivity).  Plain pointers and sizes
+ * only.  No entry point throws; every one returns an fsale_status.
+ *
+ * Threading: one host thread per context.  Work is queued on the context's
+ * HIP stream and is asynchronous until fsale_sync() or a host read
+ * (fsale_get_field / fsale_get_time).  Device-side failures (negative density
+ * or energy, non-positive inverse time step) are latched in a per-context
+ * error word and reported by the next synchronising call.
+ */
+#ifndef FLECSALE_AMD_H
+#define FLECSALE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef enum {
+  FSALE_OK = 0,
+  FSALE_ERR_ARG = 1,          /* bad argument / size                                */
+  FSALE_ERR_MESH = 2,         /* mesh is not a conforming quad or hex mesh          */
+  FSALE_ERR_HIP = 3,          /* HIP runtime failure (message: fsale_last_message)  */
+  FSALE_ERR_OOM = 4,          /* device allocation failed                           */
+  FSALE_ERR_NEG_STATE = 5,    /* "Negative density or internal energy encountered!"
+                                 apps/hydro/tasks.h:654-655,
+                                 flecsale/eqns/lagrange_eqns.h:337-341,361-365      */
+  FSALE_ERR_DT = 6,           /* "infinite delta t" apps/hydro/tasks.h:366-367      */
+  FSALE_ERR_STATE = 7,        /* call out of order (e.g. no initial conditions)     */
+  FSALE_ERR_NO_DEVICE = 8     /* no HIP device visible                              */
+} fsale_status;
+
+/* ------------------------------------------------------------------- types */
+typedef struct fsale_ctx fsale_ctx;
+
+/* A conforming mesh: quads (dim 2, CCW vertex order as in the Exodus files of
+ * data/meshes) or hexes (dim 3, Exodus hex8 order).  Replaces the burton mesh
+ * client handle client_handle_r<mesh_t> (apps/hydro/types.h:33,87-88). */
+typedef struct {
+  int32_t dim;               /* 2 or 3                                       */
+  int32_t num_cells;
+  int32_t num_vertices;
+  int32_t verts_per_cell;    /* 4 (dim 2) or 8 (dim 3)                       */
+  const double *coords;      /* [num_vertices * dim], interleaved            */
+  const int32_t *cells;      /* [num_cells * verts_per_cell], 0-based        */
+} fsale_mesh_desc;
+
+/* flecsale::eos::ideal_gas_t<double>(gamma, cv), passed by value like the
+ * reference's eos_t (flecsale/eos/ideal_gas.h:47-52, apps/hydro/tasks.h:48). */
+typedef struct { double gamma; double cv; } fsale_eos;
+
+/* Face flux functions: flux_function<eqns_t> (apps/hydro/types.h:118-125)
+ * selects hlle_flux; rusanov_flux is flecsale/eqns/flux.h:52-65; HLLC is a
+ * build-side addition (Toro ch. 10, same wave-speed bounds as HLLE). */
+enum { FSALE_FLUX_HLLE = 0, FSALE_FLUX_RUSANOV = 1, FSALE_FLUX_HLLC = 2 };
+/* Boundary flux, OR-ed into flux_kind: boundary_flux<eqns_t> = wall_flux
+ * (apps/hydro/types.h:132-136, euler_eqns.h:313-324) is the default; the
+ * transmissive variant (physical flux of the reconstructed interior state) is
+ * a build-side addition for outflow problems. */
+enum { FSALE_BOUNDARY_WALL = 0x00, FSALE_BOUNDARY_TRANSMISSIVE = 0x10 };
+
+enum { FSALE_SOLVER_EULER = 0, FSALE_SOLVER_MAIRE = 1 };
+
+/* Field ids for fsale_get_field / fsale_set_field.  Vectors are interleaved
+ * [n * dim] on the host side; gradients are [num_cells * dim] (grad_v is
+ * [num_cells * dim * dim], row d = dv_d/dx). */
+enum {
+  FSALE_F_DENSITY = 0, FSALE_F_VELOCITY, FSALE_F_INTERNAL_ENERGY, FSALE_F_PRESSURE,
+  FSALE_F_TEMPERATURE, FSALE_F_SOUND_SPEED,
+  FSALE_F_GRAD_DENSITY, FSALE_F_GRAD_VELOCITY, FSALE_F_GRAD_ENERGY, FSALE_F_GRAD_PRESSURE,
+  FSALE_F_GRAD_SOUND_SPEED,
+  FSALE_F_FLUX,              /* [num_faces * (dim+2)]: mass, momentum, energy  */
+  FSALE_F_CELL_CENTROID, FSALE_F_CELL_VOLUME, FSALE_F_CELL_MIN_LENGTH,
+  FSALE_F_FACE_CENTROID, FSALE_F_FACE_NORMAL, FSALE_F_FACE_AREA,
+  FSALE_F_COORDS,            /* vertex coordinates (moving for Maire)           */
+  FSALE_F_MASS, FSALE_F_VOLUME,         /* Maire cell mass and volume           */
+  FSALE_F_NODE_VELOCITY,     /* Maire u_p [num_vertices * dim]                  */
+  FSALE_F_CORNER_NORMAL,     /* Maire (l n)_pc [num_cells*4 * dim], corner c*4+k */
+  FSALE_F_CORNER_FORCE,      /* Maire F_pc, same layout                         */
+  FSALE_F_RESIDUAL,          /* Maire dU/dt [num_cells * 4]: dV, mom x, mom y, E */
+  FSALE_F_GAMMA,             /* per-cell ideal-gas gamma                        */
+  FSALE_F_COUNT
+};
+
+/* ------------------------------------------------------------ context life */
+/* Builds connectivity (faces, face->cell with cells[0] the lowest cell id and
+ * the normal out of it, cell-vertex neighbours, vertex->corner) on the host,
+ * uploads it with the coordinates and computes geometry on the device.
+ * Replaces specialization_tlt_init + the update_geometry task
+ * (apps/hydro/tasks.h:32-36, apps/maire_hydro/tasks.h:59-61). */
+int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out);
+int fsale_ctx_destroy(fsale_ctx *ctx);
+int fsale_sync(fsale_ctx *ctx);
+/* counts = {num_cells, num_vertices, num_faces, num_neighbour_entries, dim} */
+int fsale_mesh_counts(const fsale_ctx *ctx, int32_t *counts);
+/* face_cells [num_faces*2] (-1 on the boundary), cell_faces [num_cells*fpc],
+ * nbr_off [num_cells+1], nbr [counts[3]]; any pointer may be NULL. */
+int fsale_get_topology(fsale_ctx *ctx, int32_t *face_cells, int32_t *cell_faces,
+                       int32_t *nbr_off, int32_t *nbr);
+/* Host-only connectivity of a mesh, without a device (the reference's
+ * connectivity dump, the print task of apps/hydro/tasks.h:741-766).  Call
+ * with NULL arrays to get counts, then again with arrays of those sizes. */
+int fsale_mesh_connectivity(const fsale_mesh_desc *mesh, int32_t *counts, int32_t *face_cells,
+                            int32_t *cell_faces, int32_t *nbr_off, int32_t *nbr);
+int fsale_get_field(fsale_ctx *ctx, int field, double *host_out);
+int fsale_set_field(fsale_ctx *ctx, int field, const double *host_in);
+/* solution time, last time step and completed steps, as the reference's
+ * solution_time color handle (apps/hydro/driver.h:274) and time_cnt. */
+int fsale_get_time(fsale_ctx *ctx, double *soln_time, double *time_step, int64_t *steps);
+int fsale_set_time(fsale_ctx *ctx, double soln_time, double time_step);
+const char *fsale_status_string(int status);
+const char *fsale_last_message(void);
+/* first cell that failed (negative density/energy), or -1 */
+int fsale_error_cell(fsale_ctx *ctx, int64_t *cell);
+/* Per-kernel timing (the reference's only timer is the wall-clock print
+ * around the step loop, apps/hydro/driver.h:311,419).  While enabled, the
+ * step loops run without graphs and each launch is bracketed by HIP events on
+ * the context's stream.  Kernel ids: 0 dt, 1 gradients, 2 fluxes, 3 update
+ * (Eulerian); 4 nodal, 5 residual, 6 dt, 7 move, 8 update, 9 save/restore
+ * (Lagrangian).  total_ms / launches accumulate until fsale_reset_profile. */
+int fsale_set_profiling(fsale_ctx *ctx, int enable);
+int fsale_get_profile(fsale_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
+int fsale_reset_profile(fsale_ctx *ctx);
+
+/* ------------------------------------------------- Eulerian (apps/hydro) */
+/* initial_conditions task (apps/hydro/tasks.h:46-68): sets d, v, p from the
+ * host arrays (inputs_t::initial_conditions evaluated by the caller) and
+ * completes the state with euler_eqns_t::update_state_from_pressure
+ * (flecsale/eqns/euler_eqns.h:333-350).  v is [num_cells*dim]. */
+int fsale_eul_initial_conditions(fsale_ctx *ctx, fsale_eos eos, const double *d,
+                                 const double *v, const double *p);
+/* evaluate_time_step (apps/hydro/tasks.h:331-376), a min-reduction task:
+ * dt = min(CFL / max_{c,f} (a_c + |v_c.n_f|) A_f / V_c, t_final - t).  The
+ * result stays on the device for apply_update; dt_out may be NULL (no sync). */
+int fsale_eul_evaluate_time_step(fsale_ctx *ctx, double cfl, double final_time, double *dt_out);
+/* evaluate_gradients (apps/hydro/tasks.h:145-321): LSQ gradients with the
+ * vertex limiter.  first_order != 0 zeroes them (build-side switch). */
+int fsale_eul_evaluate_gradients(fsale_ctx *ctx, int first_order);
+/* evaluate_fluxes (apps/hydro/tasks.h:385-536) with flux_function =
+ * flux_kind and boundary_flux = wall_flux (apps/hydro/types.h:132-136). */
+int fsale_eul_evaluate_fluxes(fsale_ctx *ctx, int flux_kind);
+/* apply_update (apps/hydro/tasks.h:589-659): dt * factor from the device dt,
+ * advances the solution time like update_soln_time (:545-576). */
+int fsale_eul_apply_update(fsale_ctx *ctx, fsale_eos eos, double factor);
+/* The driver step loop (apps/hydro/driver.h:317-413): nsteps x { dt;
+ * 2 x [gradients, fluxes, apply_update(0.5)] }, all on the device, captured
+ * once as a hipGraph when use_graph != 0. stop_at_final != 0 turns steps after
+ * t >= final_time into no-ops on the device (the reference loop keeps
+ * stepping with dt = 0). */
+int fsale_eul_advance(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_time, int nsteps,
+                      int flux_kind, int first_order, int stop_at_final, int use_graph);
+
+/* --------------------------------------- Maire Lagrangian (apps/maire_hydro) */
+/* initial_conditions (apps/maire_hydro/tasks.h:71-101): M = rho V, V = cell
+ * volume, then lagrange_eqns_t::update_state_from_pressure
+ * (flecsale/eqns/lagrange_eqns.h:245-271).  gamma is per cell (the reference
+ * has one eos_t; a uniform array reproduces it). */
+int fsale_maire_initial_conditions(fsale_ctx *ctx, double cv, const double *gamma,
+                                   const double *d, const double *v, const double *p);
+/* evaluate_nodal_state (apps/maire_hydro/tasks.h:221-453) with symmetry
+ * conditions on every boundary face whose normal is an axis (tags of
+ * apps/maire_hydro/2d/inputs_regression.cc:134-162). */
+int fsale_maire_evaluate_nodal_state(fsale_ctx *ctx);
+/* evaluate_residual (apps/maire_hydro/tasks.h:461-489). */
+int fsale_maire_evaluate_residual(fsale_ctx *ctx);
+/* evaluate_time_step (apps/maire_hydro/tasks.h:142-190) followed by the
+ * driver's cap min(dt, t_final - t) (apps/maire_hydro/driver.h:391); the
+ * previous step is the device-held one.  dt_out may be NULL. */
+int fsale_maire_evaluate_time_step(fsale_ctx *ctx, double cfl_accoustic, double cfl_volume,
+                                   double cfl_growth, double final_time, double *dt_out);
+/* apply_update (apps/maire_hydro/tasks.h:497-548) with delta_t = factor * dt
+ * from the saved coordinates, then update_state_from_energy (:110-133). */
+int fsale_maire_apply_update(fsale_ctx *ctx, double factor);
+/* save_coordinates + save_solution / restore_coordinates + restore_solution
+ * (apps/maire_hydro/tasks.h:557-654). */
+int fsale_maire_save(fsale_ctx *ctx);
+int fsale_maire_restore(fsale_ctx *ctx);
+/* The driver loop (apps/maire_hydro/driver.h:330-538): midpoint predictor /
+ * corrector; stops when t >= final_time (checked on the device). */
+int fsale_maire_advance(fsale_ctx *ctx, double cfl_accoustic, double cfl_volume,
+                        double cfl_growth, double final_time, int nsteps, int use_graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLECSALE_AMD_H */

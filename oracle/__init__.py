@@ -55,6 +55,8 @@ def lib():
                              ct.c_int, ct.c_int, ct.c_int] + [_dp] * 6 + \
             [ct.POINTER(ct.c_double), ct.POINTER(ct.c_int)]
         L.om_qr_solve.argtypes = [ct.c_int, _dp, _dp]
+        L.om_set_variant.argtypes = [ct.c_int]
+        L.om_eos_eval.argtypes = [ct.c_double, ct.c_double, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.om_maire_init.argtypes = [_vp, _dp, ct.c_double, _dp, _dp] + [_dp] * 6
         L.om_maire_nodal.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.om_maire_residual.argtypes = [_vp, _dp, _dp, _dp, _dp]
@@ -201,6 +203,19 @@ def euler_run(mesh, s, gamma, cv, cfl, final_time, max_steps, flux_kind=0, first
                       int(stop_at_final), s.d, s.v.reshape(-1), s.e, s.p, s.T, s.a, ct.byref(t),
                       ct.byref(n))
     return rc, t.value, n.value
+
+
+def set_variant(v):
+    """Plant a known mistake in the oracle (tests only); 0 restores the reference arithmetic."""
+    lib().om_set_variant(int(v))
+
+
+def eos_eval(gamma, cv, d, p):
+    """ideal_gas_t: e(d,p), p(d,e), a(e), T(e) (flecsale/eos/ideal_gas.h:178-226)."""
+    d = _c(d); p = _c(p); n = len(d)
+    e, pp, a, T = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().om_eos_eval(gamma, cv, n, d, p, e, pp, a, T)
+    return e, pp, a, T
 
 
 # ---------------------------------------------------------------- QR

@@ -43,6 +43,14 @@
 #define OM_ERR_DT 3             /* non-positive inverse time step */
 #define OM_ERR_ALLOC 4
 
+/* Planted-mistake switch for the tests that show the goldens discriminate a
+ * convention (0 = the reference's arithmetic):
+ *   1 = reconstruct pressure with grad_p and energy with grad_e (no slot swap)
+ *   2 = wedge facet_area = 0.5 * edge length
+ *   3 = 2D cell centroid = vertex average */
+static int om_variant = 0;
+void om_set_variant(int v) { om_variant = v; }
+
 /* std::min / std::max semantics (first argument wins ties / NaN order). */
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static inline double smax(double a, double b) { return (a < b) ? b : a; }
@@ -121,6 +129,11 @@ static void cell_geom_2d(const om_mesh *m, int c, double *cen, double *vol, doub
   double area = 0.5 * a2;
   cen[0] = sx / (6.0 * area);
   cen[1] = sy / (6.0 * area);
+  if (om_variant == 3) {
+    double ax = 0, ay = 0;
+    for (int i = 0; i < n; ++i) { ax += m->x[2 * m->cv[c * n + i]]; ay += m->x[2 * m->cv[c * n + i] + 1]; }
+    cen[0] = ax / n; cen[1] = ay / n;
+  }
   *vol = area;
   double ml = DBL_MAX;
   for (int i = 0; i < n; ++i)
@@ -678,14 +691,22 @@ static void recon(const om_mesh *m, int c, const double *fx, const double *d, co
   for (int i = 0; i < N; ++i) {
     w->d += dx[i] * gd[N * c + i];
     for (int k = 0; k < N; ++k) w->v[k] += dx[i] * gv[N * N * c + N * k + i];
-    w->p += dx[i] * ge[N * c + i];
-    w->e += dx[i] * gp[N * c + i];
+    if (om_variant == 1) {
+      w->p += dx[i] * gp[N * c + i];
+      w->e += dx[i] * ge[N * c + i];
+    } else {
+      w->p += dx[i] * ge[N * c + i];
+      w->e += dx[i] * gp[N * c + i];
+    }
     w->a += dx[i] * ga[N * c + i];
   }
 }
 
-/* evaluate_fluxes :385-464. flux_kind: 0 HLLE (types.h:118-125), 1 Rusanov, 2 HLLC.
- * F is [nf*(dim+2)] (mass, momentum[dim], energy), scaled by face area. */
+/* evaluate_fluxes :385-464. flux_kind & 0xf: 0 HLLE (types.h:118-125), 1 Rusanov,
+ * 2 HLLC.  flux_kind >> 4 selects boundary_flux (types.h:132-136): 0 wall_flux
+ * (the reference), 1 transmissive = physical flux of the reconstructed interior
+ * state (build-side addition, the usual outflow condition of the 2D Riemann
+ * problems).  F is [nf*(dim+2)] (mass, momentum[dim], energy), scaled by area. */
 void oe_fluxes(const om_mesh *m, const double *d, const double *v, const double *e,
                const double *p, const double *T, const double *a, const double *gd,
                const double *gv, const double *ge, const double *gp, const double *ga,
@@ -701,9 +722,12 @@ void oe_fluxes(const om_mesh *m, const double *d, const double *v, const double 
     int c1 = m->fc[2 * f + 1];
     if (c1 >= 0) {
       recon(m, c1, fx, d, v, e, p, T, a, gd, gv, ge, gp, ga, &wr);
-      if (flux_kind == 1) rusanov_flux(&wl, &wr, n, N, out);
-      else if (flux_kind == 2) hllc_flux(&wl, &wr, n, N, out);
+      const int fk = flux_kind & 0xf;
+      if (fk == 1) rusanov_flux(&wl, &wr, n, N, out);
+      else if (fk == 2) hllc_flux(&wl, &wr, n, N, out);
       else hlle_flux(&wl, &wr, n, N, out);
+    } else if ((flux_kind >> 4) == 1) {
+      e_flux(&wl, n, N, out);
     } else {
       wall_flux(&wl, n, N, out);
     }
@@ -922,7 +946,7 @@ static void wedge_facets_2d(const om_mesh *m, int c, int k, double n[2][2], doub
     /* facet_area = |edge midpoint - vertex|: pins sedov_2d to print precision */
     double mx = 0.5 * (A[w][0] + Bv[w][0]) - xk[0];
     double my = 0.5 * (A[w][1] + Bv[w][1]) - xk[1];
-    l[w] = sqrt(mx * mx + my * my);
+    l[w] = (om_variant == 2) ? 0.5 * len : sqrt(mx * mx + my * my);
   }
 }
 
