@@ -35,7 +35,8 @@ SOLVER_EULER, SOLVER_MAIRE = 0, 1
 K_EUL_DT, K_EUL_GRAD, K_EUL_FLUX, K_EUL_UPDATE = 0, 1, 2, 3
 K_MAIRE_NODAL, K_MAIRE_RESIDUAL, K_MAIRE_DT, K_MAIRE_MOVE, K_MAIRE_UPDATE, K_MAIRE_COPY = 4, 5, 6, 7, 8, 9
 KERNEL_NAMES = ["eul_dt", "eul_grad", "eul_flux", "eul_update", "maire_nodal", "maire_residual",
-                "maire_dt", "maire_move", "maire_update", "maire_copy"]
+                "maire_dt", "maire_move", "maire_update", "maire_copy", "eul_fluxupd"]
+K_EUL_FLUXUPD = 10
 
 ERR_NEG_STATE = 5
 ERR_DT = 6

@@ -416,6 +416,7 @@ int fsale_set_field(fsale_ctx *ctx, int field, const double *host_in) {
                              ctx->stream));
     FSALE_HIP(hipStreamSynchronize(ctx->stream));
   }
+  if (ctx->solver == FSALE_SOLVER_EULER) ctx->eu.partials_valid = false;
   if (field == FSALE_F_COORDS) return launch_geometry(ctx, true, true);
   return FSALE_OK;
 }

@@ -1,24 +1,64 @@
 // Eulerian MUSCL finite-volume step of apps/hydro on the device.
 //
-//   k_eul_dt      evaluate_time_step  apps/hydro/tasks.h:331-376
-//   k_eul_grad    evaluate_gradients  apps/hydro/tasks.h:145-321
-//   k_eul_flux    evaluate_fluxes     apps/hydro/tasks.h:385-536
-//   k_eul_update  apply_update        apps/hydro/tasks.h:589-659
+//   k_eul_dt_partial + k_eul_dt_finalize   evaluate_time_step  apps/hydro/tasks.h:331-376
+//   k_eul_grad                             evaluate_gradients  apps/hydro/tasks.h:145-321
+//   k_eul_flux                             evaluate_fluxes     apps/hydro/tasks.h:385-536
+//   k_eul_update                           apply_update        apps/hydro/tasks.h:589-659
+//   k_eul_fluxupd   evaluate_fluxes + apply_update (+ the next step's dt^-1
+//                   partials), one work-item per cell: the driver loop's path
 //
-// One work-item per cell (dt, gradients, update) or per face (fluxes), SoA
-// FP64 state, k-major fixed-degree connectivity so neighbouring lanes read
-// neighbouring words.  The update gathers the face fluxes of its cell in the
-// reference's mesh.faces(c) order (no atomics).  All four kernels are
-// HBM-bound (no dense contraction): see the byte model in bench.py.
+// SoA FP64 state, k-major fixed-degree connectivity so neighbouring lanes read
+// neighbouring words, and an XCD-aware block order: the hardware deals blocks
+// round-robin to the 8 XCDs, so logical block L is launched as physical block
+// (L % per) * 8 + L / per, giving each XCD (and its L2) one contiguous range
+// of cells whose neighbour gathers then hit that L2.
+//
+// k_eul_fluxupd recomputes, for each cell, the numerical flux of each of its
+// faces exactly as k_eul_flux does (same left/right states, same operation
+// order), so a face shared by two cells gets the bit-identical flux on both
+// sides and the update gathers them in the reference's mesh.faces(c) order.
+// In 2D the face centroid / normal / length come from the cell's own
+// vertices (bit-identical to the stored face geometry: reversing an edge
+// negates it exactly).  The flux array never goes through HBM.
+//
+// The time step: each block writes the max of dt^-1 over its cells to a
+// partials array (no atomics); one block folds the partials into dt.  In the
+// driver loop the partials come from the second half-step's update, which
+// already holds the new state.
 #include "internal.h"
 #include "kernels_common.h"
 
 namespace fsale {
 namespace {
 
+// physical -> logical block so that every XCD owns one contiguous range
+__device__ inline int logical_block() {
+  const int per = gridDim.x >> 3;
+  return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// blocks launched for n entities: a multiple of 8 (one slice per XCD)
+inline int grid8(int n) {
+  const int b = (n + kBlock - 1) / kBlock;
+  return ((b + 7) / 8) * 8;
+}
+
+// block max of dt^-1 -> partials[L]
+__device__ inline void block_max_to(double v, double *partials, int L) {
+  __shared__ double red[kBlock / 64];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) b = smax(red[w], b);
+    partials[L] = b;
+  }
+}
+
 // ------------------------------------------------------------ state init
 __global__ void __launch_bounds__(kBlock) k_eul_ic(int nc, EulerFields u, double g, double cv) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = logical_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
   // update_state_from_pressure euler_eqns.h:333-350
   const double ie = eos_e_dp(g, u.d[c], u.p[c]);
@@ -28,15 +68,10 @@ __global__ void __launch_bounds__(kBlock) k_eul_ic(int nc, EulerFields u, double
 }
 
 // ----------------------------------------------------------- time step
-// Block-level max of dt^-1, combined across blocks with an atomic max on the
-// IEEE bit pattern (all values are >= 0); the last block to finish turns it
-// into dt = min(CFL/dt_inv, t_final - t) and re-arms the scratch.
 template <int N, int FPC>
-__global__ void __launch_bounds__(kBlock) k_eul_dt(DevMesh m, EulerFields u, DevScalars *s, double cfl,
-                                                   double final_time, int stop_at_final) {
-  __shared__ double red[kBlock / 64];
-  __shared__ int last;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerFields u, double *partials) {
+  const int L = logical_block();
+  const int c = L * kBlock + threadIdx.x;
   double dt_inv = 0;
   if (c < m.nc) {
     double vel[N];
@@ -48,46 +83,41 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt(DevMesh m, EulerFields u, Dev
       double n[N];
       for (int d = 0; d < N; ++d) n[d] = m.fnrm[d][f];
       const double delta_x = vol / m.farea[f];
-      const double vn = dotN<N>(vel, n);
-      const double dti = (a + fabs(vn)) / delta_x;
+      const double dti = (a + fabs(dotN<N>(vel, n))) / delta_x;
       dt_inv = smax(dti, dt_inv);
     }
   }
-  dt_inv = wave_max(dt_inv);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dt_inv;
+  block_max_to(dt_inv, partials, L);
+}
+
+// dt = min(CFL / max dt^-1, t_final - t); also opens or closes the step
+__global__ void __launch_bounds__(kBlock) k_eul_dt_finalize(const double *partials, int nparts, DevScalars *s,
+                                                           double cfl, double final_time, int stop_at_final) {
+  __shared__ double red[kBlock / 64];
+  double v = 0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) v = smax(partials[i], v);
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) b = smax(red[w], b);
-    atomic_max_nonneg(&s->red[0], b);
-    __threadfence();
-    const unsigned prev = atomicAdd(&s->done, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    const double mx = __longlong_as_double(
-        (long long)atomicAdd(reinterpret_cast<unsigned long long *>(&s->red[0]), 0ull));
-    s->red[0] = 0;
-    s->done = 0;
-    const double t = s->t;
-    int active = (s->err_code == 0);
-    if (stop_at_final && !(t < final_time)) active = 0;
-    if (active) {
-      if (mx <= 0) {
-        atomicCAS(&s->err_code, 0, (int)FSALE_ERR_DT);
-        active = 0;
-      } else {
-        double ts = 1 / mx;
-        ts *= cfl;
-        ts = smin(ts, final_time - t);
-        s->dt = ts;
-        s->steps = s->steps + 1;
-      }
+  if (threadIdx.x != 0) return;
+  double mx = red[0];
+  for (int w = 1; w < kBlock / 64; ++w) mx = smax(red[w], mx);
+  const double t = s->t;
+  int active = (s->err_code == 0);
+  if (stop_at_final && !(t < final_time)) active = 0;
+  if (active) {
+    if (mx <= 0) {   // apps/hydro/tasks.h:366-367
+      s->err_code = FSALE_ERR_DT;
+      active = 0;
+    } else {
+      double ts = 1 / mx;
+      ts *= cfl;
+      ts = smin(ts, final_time - t);
+      s->dt = ts;
+      s->steps = s->steps + 1;
     }
-    s->active = active;
   }
+  s->active = active;
 }
 
 // ----------------------------------------------------------- gradients
@@ -95,7 +125,7 @@ template <int N, int VPC>
 __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, const DevScalars *s,
                                                      int first_order) {
   constexpr int NV = N + 4;
-  const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c0 = logical_block() * kBlock + threadIdx.x;
   if (c0 >= m.nc || !s->active) return;
   const double tol = 1.e-9;
   double u0[NV], umin[NV], umax[NV], dudx[NV][N], dxdx[N][N], x0[N];
@@ -151,6 +181,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
   }
   double gu[NV][N];
   if constexpr (N == 2) {
+    // gradient_2d apps/hydro/tasks.h:109-117
     const double denom = 1 / (dxdx[0][0] * dxdx[1][1] - dxdx[0][1] * dxdx[0][1]);
     for (int j = 0; j < NV; ++j) {
       gu[j][0] = (dudx[j][0] * dxdx[1][1] - dudx[j][1] * dxdx[0][1]) * denom;
@@ -207,57 +238,125 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
 }
 
 // -------------------------------------------------------------- fluxes
+// Cell state + gradients + centroid, as gathered for a face reconstruction.
+template <int N>
+struct CellData {
+  double d, v[N], p, e, a, gd[N], gv[N * N], ge[N], gp[N], ga[N], cx[N];
+};
+
+template <int N>
+__device__ inline void load_cell(const DevMesh &m, const EulerFields &u, int c, CellData<N> &q) {
+  q.d = u.d[c];
+  for (int k = 0; k < N; ++k) q.v[k] = u.v[k][c];
+  q.p = u.p[c];
+  q.e = u.e[c];
+  q.a = u.a[c];
+  for (int i = 0; i < N; ++i) {
+    q.gd[i] = u.gd[i][c];
+    q.ge[i] = u.ge[i][c];
+    q.gp[i] = u.gp[i][c];
+    q.ga[i] = u.ga[i][c];
+    q.cx[i] = m.ccen[i][c];
+  }
+  for (int i = 0; i < N * N; ++i) q.gv[i] = u.gv[i][c];
+}
+
 // face state of apps/hydro/tasks.h:418-432: slot 2 (pressure) gets dx.grad_e
 // and slot 3 (energy) gets dx.grad_p, as the reference's tuple order does.
 template <int N>
-__device__ inline void reconstruct(const DevMesh &m, const EulerFields &u, int c, const double *fx,
-                                   EState<N> &w) {
+__device__ inline void reconstruct(const CellData<N> &q, const double *fx, EState<N> &w) {
   double dx[N];
-  for (int d = 0; d < N; ++d) dx[d] = fx[d] - m.ccen[d][c];
-  w.d = u.d[c];
-  for (int k = 0; k < N; ++k) w.v[k] = u.v[k][c];
-  w.p = u.p[c];
-  w.e = u.e[c];
-  w.a = u.a[c];
+  for (int d = 0; d < N; ++d) dx[d] = fx[d] - q.cx[d];
+  w.d = q.d;
+  for (int k = 0; k < N; ++k) w.v[k] = q.v[k];
+  w.p = q.p;
+  w.e = q.e;
+  w.a = q.a;
   for (int i = 0; i < N; ++i) {
-    w.d += dx[i] * u.gd[i][c];
-    for (int k = 0; k < N; ++k) w.v[k] += dx[i] * u.gv[k * N + i][c];
-    w.p += dx[i] * u.ge[i][c];
-    w.e += dx[i] * u.gp[i][c];
-    w.a += dx[i] * u.ga[i][c];
+    w.d += dx[i] * q.gd[i];
+    for (int k = 0; k < N; ++k) w.v[k] += dx[i] * q.gv[k * N + i];
+    w.p += dx[i] * q.ge[i];
+    w.e += dx[i] * q.gp[i];
+    w.a += dx[i] * q.ga[i];
   }
 }
 
+// numerical flux of one face times its area (flux(f) of evaluate_fluxes)
 template <int N>
-__global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, const DevScalars *s,
-                                                     int kind) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= m.nf || !s->active) return;
-  double fx[N], n[N], out[N + 2];
-  for (int d = 0; d < N; ++d) {
-    fx[d] = m.fcen[d][f];
-    n[d] = m.fnrm[d][f];
-  }
+__device__ inline void face_flux_area(int kind, const CellData<N> &q0, const CellData<N> *q1, const double *fx,
+                                      const double *n, double area, double *out) {
   EState<N> wl, wr;
-  reconstruct<N>(m, u, m.fc[f], fx, wl);
-  const int c1 = m.fc[(size_t)m.nf + f];
-  if (c1 >= 0) {
-    reconstruct<N>(m, u, c1, fx, wr);
+  reconstruct<N>(q0, fx, wl);
+  if (q1) {
+    reconstruct<N>(*q1, fx, wr);
     face_flux<N>(kind & 0xf, wl, wr, n, out);
   } else if ((kind >> 4) == 1) {
     e_flux<N>(wl, n, out);   // transmissive boundary
   } else {
     wall_flux<N>(wl, n, out);
   }
-  const double area = m.farea[f];
-  for (int i = 0; i < N + 2; ++i) u.F[i][f] = out[i] * area;
+  for (int i = 0; i < N + 2; ++i) out[i] *= area;
+}
+
+template <int N>
+__global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, const DevScalars *s, int kind) {
+  const int f = logical_block() * kBlock + threadIdx.x;
+  if (f >= m.nf || !s->active) return;
+  double fx[N], n[N], out[N + 2];
+  for (int d = 0; d < N; ++d) {
+    fx[d] = m.fcen[d][f];
+    n[d] = m.fnrm[d][f];
+  }
+  CellData<N> q0, q1;
+  load_cell<N>(m, u, m.fc[f], q0);
+  const int c1 = m.fc[(size_t)m.nf + f];
+  if (c1 >= 0) load_cell<N>(m, u, c1, q1);
+  face_flux_area<N>(kind, q0, c1 >= 0 ? &q1 : nullptr, fx, n, m.farea[f], out);
+  for (int i = 0; i < N + 2; ++i) u.F[i][f] = out[i];
 }
 
 // -------------------------------------------------------------- update
+// update_state_from_flux euler_eqns.h:383-410 + update_state_from_energy
+// :359-375 for one cell; returns false on negative density / energy.
+template <int N>
+__device__ inline bool update_cell(EulerFields &u, int c, double *du, double sc, double g, double cv,
+                                   double *vel_out, double &a_out) {
+  for (int i = 0; i < N + 2; ++i) du[i] *= sc;
+  double vel[N];
+  for (int d = 0; d < N; ++d) vel[d] = u.v[d][c];
+  const double den0 = u.d[c];
+  const double mass = den0 + du[0];
+  const double et0 = u.e[c] + 0.5 * dotN<N>(vel, vel);
+  const double ener = den0 * et0 + du[N + 1];
+  const double inv_mass = 1 / mass;
+  for (int d = 0; d < N; ++d) {
+    const double mom = den0 * vel[d] + du[1 + d];
+    vel[d] = mom * inv_mass;
+  }
+  const double ie = ener * inv_mass - 0.5 * dotN<N>(vel, vel);
+  u.d[c] = mass;
+  for (int d = 0; d < N; ++d) {
+    u.v[d][c] = vel[d];
+    vel_out[d] = vel[d];
+  }
+  u.e[c] = ie;
+  u.p[c] = eos_p_de(g, mass, ie);
+  u.T[c] = eos_T_de(cv, ie);
+  a_out = eos_a_de(g, ie);
+  u.a[c] = a_out;
+  return !(ie < 0 || mass < 0);   // apps/hydro/tasks.h:654-655
+}
+
+__device__ inline void flag_negative(DevScalars *s, int c) {
+  atomicCAS(&s->err_code, 0, (int)FSALE_ERR_NEG_STATE);
+  atomicMin(reinterpret_cast<unsigned long long *>(&s->err_cell), (unsigned long long)c);
+  s->active = 0;
+}
+
 template <int N, int FPC>
 __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u, DevScalars *s, double g,
                                                        double cv, double factor) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = logical_block() * kBlock + threadIdx.x;
   if (!s->active) return;
   double delta_t = s->dt;
   delta_t *= factor;
@@ -275,95 +374,165 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
       for (int i = 0; i < N + 2; ++i) du[i] -= u.F[i][f];
     }
   }
-  const double sc = delta_t / m.cvol[c];
-  for (int i = 0; i < N + 2; ++i) du[i] *= sc;
-  // update_state_from_flux euler_eqns.h:383-410
-  double vel[N];
-  for (int d = 0; d < N; ++d) vel[d] = u.v[d][c];
-  const double den0 = u.d[c];
-  const double mass = den0 + du[0];
-  const double et0 = u.e[c] + 0.5 * dotN<N>(vel, vel);
-  const double ener = den0 * et0 + du[N + 1];
-  const double inv_mass = 1 / mass;
-  for (int d = 0; d < N; ++d) {
-    const double mom = den0 * vel[d] + du[1 + d];
-    vel[d] = mom * inv_mass;
-  }
-  const double ie = ener * inv_mass - 0.5 * dotN<N>(vel, vel);
-  u.d[c] = mass;
-  for (int d = 0; d < N; ++d) u.v[d][c] = vel[d];
-  u.e[c] = ie;
-  // update_state_from_energy euler_eqns.h:359-375
-  u.p[c] = eos_p_de(g, mass, ie);
-  u.T[c] = eos_T_de(cv, ie);
-  u.a[c] = eos_a_de(g, ie);
-  if (ie < 0 || mass < 0) {   // apps/hydro/tasks.h:654-655
-    atomicCAS(&s->err_code, 0, (int)FSALE_ERR_NEG_STATE);
-    atomicMin(reinterpret_cast<unsigned long long *>(&s->err_cell), (unsigned long long)c);
-    s->active = 0;
-  }
+  double vel[N], a;
+  if (!update_cell<N>(u, c, du, delta_t / m.cvol[c], g, cv, vel, a)) flag_negative(s, c);
 }
 
-inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
+// fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
+template <int N, int FPC, int VPC>
+__global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, DevScalars *s, double g,
+                                                        double cv, double factor, int kind, double *partials) {
+  const int L = logical_block();
+  const int c = L * kBlock + threadIdx.x;
+  // one read of the step flag per block: another block may clear it (negative
+  // state) while this one runs, and the block reduction below needs every lane
+  __shared__ int blk_active;
+  if (threadIdx.x == 0) blk_active = s->active;
+  __syncthreads();
+  if (!blk_active) return;
+  double delta_t = s->dt;
+  delta_t *= factor;
+  if (c == 0) s->t = s->t + delta_t;
+  const bool live = c < m.nc;
+  double dt_inv = 0;
+  if (live) {
+    CellData<N> own;
+    load_cell<N>(m, u, c, own);
+    double px[VPC], py[VPC];
+    if constexpr (N == 2) {
+      for (int k = 0; k < VPC; ++k) {
+        const int v = m.cv[(size_t)k * m.nc + c];
+        px[k] = m.x[0][v];
+        py[k] = m.x[1][v];
+      }
+    }
+    double du[N + 2];
+    for (int i = 0; i < N + 2; ++i) du[i] = 0;
+    double nk[FPC][N], ak[FPC];
+#pragma unroll
+    for (int k = 0; k < FPC; ++k) {
+      const int code = m.cf[(size_t)k * m.nc + c];
+      const int f = code >> 1;
+      const bool second = code & 1;   // this cell is the face's cells[1]
+      double fx[N];
+      if constexpr (N == 2) {
+        const int k1 = (k + 1 == VPC) ? 0 : k + 1;
+        // the face runs v_k -> v_k1 out of cells[0], reversed for cells[1]
+        if (!second)
+          face_geom_2d(px[k], py[k], px[k1], py[k1], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
+        else
+          face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
+      } else {
+        for (int d = 0; d < N; ++d) {
+          fx[d] = m.fcen[d][f];
+          nk[k][d] = m.fnrm[d][f];
+        }
+        ak[k] = m.farea[f];
+      }
+      const int other = m.fc[(size_t)(second ? 0 : 1) * m.nf + f];
+      double out[N + 2];
+      if (other >= 0) {
+        CellData<N> nb;
+        load_cell<N>(m, u, other, nb);
+        if (!second)
+          face_flux_area<N>(kind, own, &nb, fx, nk[k], ak[k], out);
+        else
+          face_flux_area<N>(kind, nb, &own, fx, nk[k], ak[k], out);
+      } else {
+        face_flux_area<N>(kind, own, nullptr, fx, nk[k], ak[k], out);
+      }
+      if (second) {
+        for (int i = 0; i < N + 2; ++i) du[i] += out[i];
+      } else {
+        for (int i = 0; i < N + 2; ++i) du[i] -= out[i];
+      }
+    }
+    const double vol = m.cvol[c];
+    double vel[N], a;
+    if (!update_cell<N>(u, c, du, delta_t / vol, g, cv, vel, a)) flag_negative(s, c);
+    if (partials) {
+#pragma unroll
+      for (int k = 0; k < FPC; ++k) {
+        const double delta_x = vol / ak[k];
+        const double dti = (a + fabs(dotN<N>(vel, nk[k]))) / delta_x;
+        dt_inv = smax(dti, dt_inv);
+      }
+    }
+  }
+  if (partials) block_max_to(dt_inv, partials, L);
+}
+
+// ------------------------------------------------------------- launches
+template <class K, class... A>
+int launch(fsale_ctx *ctx, int kid, K kernel, int n, A... args) {
+  ProfScope prof(ctx, kid);
+  hipLaunchKernelGGL(kernel, dim3(grid8(n)), dim3(kBlock), 0, ctx->stream, args...);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
 
 int launch_dt(fsale_ctx *ctx, double cfl, double final_time, int stop_at_final) {
   const DevMesh &m = ctx->m;
-  ProfScope prof(ctx, K_EUL_DT);
+  int rc;
   if (m.dim == 2)
-    hipLaunchKernelGGL((k_eul_dt<2, 4>), dim3(grid(m.nc)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s,
-                       cfl, final_time, stop_at_final);
+    rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<2, 4>, m.nc, m, ctx->eu, ctx->eu.partials);
   else
-    hipLaunchKernelGGL((k_eul_dt<3, 6>), dim3(grid(m.nc)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s,
-                       cfl, final_time, stop_at_final);
+    rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<3, 6>, m.nc, m, ctx->eu, ctx->eu.partials);
+  if (rc) return rc;
+  ProfScope prof(ctx, K_EUL_DT);
+  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->eu.partials,
+                     grid8(m.nc), ctx->s, cfl, final_time, stop_at_final);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
+int launch_finalize(fsale_ctx *ctx, double cfl, double final_time, int stop_at_final) {
+  ProfScope prof(ctx, K_EUL_DT);
+  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->eu.partials,
+                     grid8(ctx->m.nc), ctx->s, cfl, final_time, stop_at_final);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
 
 int launch_grad(fsale_ctx *ctx, int first_order) {
   const DevMesh &m = ctx->m;
-  ProfScope prof(ctx, K_EUL_GRAD);
-  if (m.dim == 2)
-    hipLaunchKernelGGL((k_eul_grad<2, 4>), dim3(grid(m.nc)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s,
-                       first_order);
-  else
-    hipLaunchKernelGGL((k_eul_grad<3, 8>), dim3(grid(m.nc)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s,
-                       first_order);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
+  if (m.dim == 2) return launch(ctx, K_EUL_GRAD, k_eul_grad<2, 4>, m.nc, m, ctx->eu, (const DevScalars *)ctx->s, first_order);
+  return launch(ctx, K_EUL_GRAD, k_eul_grad<3, 8>, m.nc, m, ctx->eu, (const DevScalars *)ctx->s, first_order);
 }
 
 int launch_flux(fsale_ctx *ctx, int kind) {
   const DevMesh &m = ctx->m;
-  ProfScope prof(ctx, K_EUL_FLUX);
-  if (m.dim == 2)
-    hipLaunchKernelGGL((k_eul_flux<2>), dim3(grid(m.nf)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s, kind);
-  else
-    hipLaunchKernelGGL((k_eul_flux<3>), dim3(grid(m.nf)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s, kind);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
+  if (m.dim == 2) return launch(ctx, K_EUL_FLUX, k_eul_flux<2>, m.nf, m, ctx->eu, (const DevScalars *)ctx->s, kind);
+  return launch(ctx, K_EUL_FLUX, k_eul_flux<3>, m.nf, m, ctx->eu, (const DevScalars *)ctx->s, kind);
 }
 
 int launch_update(fsale_ctx *ctx, fsale_eos eos, double factor) {
   const DevMesh &m = ctx->m;
-  ProfScope prof(ctx, K_EUL_UPDATE);
   if (m.dim == 2)
-    hipLaunchKernelGGL((k_eul_update<2, 4>), dim3(grid(m.nc)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s,
-                       eos.gamma, eos.cv, factor);
-  else
-    hipLaunchKernelGGL((k_eul_update<3, 6>), dim3(grid(m.nc)), dim3(kBlock), 0, ctx->stream, m, ctx->eu, ctx->s,
-                       eos.gamma, eos.cv, factor);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
+    return launch(ctx, K_EUL_UPDATE, k_eul_update<2, 4>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor);
+  return launch(ctx, K_EUL_UPDATE, k_eul_update<3, 6>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor);
 }
 
+int launch_fluxupd(fsale_ctx *ctx, fsale_eos eos, double factor, int kind, bool want_dt) {
+  const DevMesh &m = ctx->m;
+  double *parts = want_dt ? ctx->eu.partials : nullptr;
+  if (m.dim == 2)
+    return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor,
+                  kind, parts);
+  return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor,
+                kind, parts);
+}
+
+// one driver step (apps/hydro/driver.h:317-413) with the fused kernels; the
+// dt^-1 partials of the current state must be valid on entry and are valid
+// again on exit
 int launch_step(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_time, int kind, int first_order,
                 int stop_at_final) {
   int rc;
-  if ((rc = launch_dt(ctx, cfl, final_time, stop_at_final))) return rc;
+  if ((rc = launch_finalize(ctx, cfl, final_time, stop_at_final))) return rc;
   for (int half = 0; half < 2; ++half) {
     if ((rc = launch_grad(ctx, first_order))) return rc;
-    if ((rc = launch_flux(ctx, kind))) return rc;
-    if ((rc = launch_update(ctx, eos, 0.5))) return rc;
+    if ((rc = launch_fluxupd(ctx, eos, 0.5, kind, half == 1))) return rc;
   }
   return FSALE_OK;
 }
@@ -375,6 +544,8 @@ bool check_ctx(fsale_ctx *ctx) {
   }
   return true;
 }
+
+bool valid_kind(int k) { return k >= 0 && (k & 0xf) <= 2 && (k >> 4) <= 1; }
 
 }  // namespace
 
@@ -397,6 +568,7 @@ int euler_alloc(fsale_ctx *ctx) {
     if ((rc = dev_alloc_n(ctx, &u.gv[d], m.nc))) return rc;
   for (int i = 0; i < N + 2; ++i)
     if ((rc = dev_alloc_n(ctx, &u.F[i], m.nf))) return rc;
+  if ((rc = dev_alloc_n(ctx, &u.partials, grid8(m.nc)))) return rc;
   return FSALE_OK;
 }
 
@@ -414,12 +586,13 @@ int fsale_eul_initial_conditions(fsale_ctx *ctx, fsale_eos eos, const double *d,
   if ((rc = fsale_set_field(ctx, FSALE_F_DENSITY, d))) return rc;
   if ((rc = fsale_set_field(ctx, FSALE_F_VELOCITY, v))) return rc;
   if ((rc = fsale_set_field(ctx, FSALE_F_PRESSURE, p))) return rc;
-  hipLaunchKernelGGL(k_eul_ic, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m.nc, ctx->eu,
+  hipLaunchKernelGGL(k_eul_ic, dim3(grid8(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m.nc, ctx->eu,
                      eos.gamma, eos.cv);
   FSALE_HIP(hipGetLastError());
   ctx->eu.gamma = eos.gamma;
   ctx->eu.cv = eos.cv;
   ctx->have_state = true;
+  ctx->eu.partials_valid = false;
   // fresh run: time 0, no error, active
   if ((rc = read_scalars(ctx))) return rc;
   DevScalars h = *ctx->h_s;
@@ -441,7 +614,9 @@ int fsale_eul_evaluate_time_step(fsale_ctx *ctx, double cfl, double final_time, 
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   if (!ctx->have_state) return FSALE_ERR_STATE;
   int rc = launch_dt(ctx, cfl, final_time, 0);
-  if (rc || !dt_out) return rc;
+  if (rc) return rc;
+  ctx->eu.partials_valid = true;
+  if (!dt_out) return FSALE_OK;
   if ((rc = check_device_error(ctx))) return rc;
   *dt_out = ctx->h_s->dt;
   return FSALE_OK;
@@ -453,8 +628,6 @@ int fsale_eul_evaluate_gradients(fsale_ctx *ctx, int first_order) {
   return launch_grad(ctx, first_order);
 }
 
-bool valid_kind(int k) { return k >= 0 && (k & 0xf) <= 2 && (k >> 4) <= 1; }
-
 int fsale_eul_evaluate_fluxes(fsale_ctx *ctx, int flux_kind) {
   if (!check_ctx(ctx) || !valid_kind(flux_kind)) return FSALE_ERR_ARG;
   if (!ctx->have_state) return FSALE_ERR_STATE;
@@ -464,6 +637,7 @@ int fsale_eul_evaluate_fluxes(fsale_ctx *ctx, int flux_kind) {
 int fsale_eul_apply_update(fsale_ctx *ctx, fsale_eos eos, double factor) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   if (!ctx->have_state) return FSALE_ERR_STATE;
+  ctx->eu.partials_valid = false;
   return launch_update(ctx, eos, factor);
 }
 
@@ -471,7 +645,17 @@ int fsale_eul_advance(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_ti
                       int first_order, int stop_at_final, int use_graph) {
   if (!check_ctx(ctx) || nsteps < 0 || !valid_kind(flux_kind)) return FSALE_ERR_ARG;
   if (!ctx->have_state) return FSALE_ERR_STATE;
+  if (nsteps == 0) return FSALE_OK;
   int rc;
+  if (!ctx->eu.partials_valid) {
+    const DevMesh &m = ctx->m;
+    if (m.dim == 2)
+      rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<2, 4>, m.nc, m, ctx->eu, ctx->eu.partials);
+    else
+      rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<3, 6>, m.nc, m, ctx->eu, ctx->eu.partials);
+    if (rc) return rc;
+    ctx->eu.partials_valid = true;
+  }
   if (!use_graph || nsteps < 2 || ctx->profiling) {
     for (int n = 0; n < nsteps; ++n)
       if ((rc = launch_step(ctx, eos, cfl, final_time, flux_kind, first_order, stop_at_final))) return rc;

@@ -85,6 +85,8 @@ struct EulerFields {
          *T = nullptr, *a = nullptr;
   double *gd[3] = {}, *gv[9] = {}, *ge[3] = {}, *gp[3] = {}, *ga[3] = {};
   double *F[5] = {};
+  double *partials = nullptr;    // per-block max of dt^-1 [grid8(nc)]
+  bool partials_valid = false;   // partials describe the current state
   double gamma = 0, cv = 0;
 };
 
@@ -109,6 +111,7 @@ struct GraphCache {
 enum KernelId {
   K_EUL_DT = 0, K_EUL_GRAD, K_EUL_FLUX, K_EUL_UPDATE,
   K_MAIRE_NODAL, K_MAIRE_RESIDUAL, K_MAIRE_DT, K_MAIRE_MOVE, K_MAIRE_UPDATE, K_MAIRE_COPY,
+  K_EUL_FLUXUPD,
   K_COUNT
 };
 

@@ -104,6 +104,14 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
           err = "inconsistent face matching";
           return FSALE_ERR_MESH;
         }
+        // 2D cells must share an edge in opposite directions (consistent
+        // orientation): the fused flux kernel rebuilds face geometry from the
+        // cell's own vertices and relies on it
+        if (t.dim == 2 && !(t.cv[(size_t)c * t.vpc + lf] == t.fv[2 * (size_t)f + 1] &&
+                            t.cv[(size_t)c * t.vpc + (lf + 1) % t.vpc] == t.fv[2 * (size_t)f])) {
+          err = "inconsistently oriented cells (2D cells must all be counter-clockwise)";
+          return FSALE_ERR_MESH;
+        }
         t.fc[2 * (size_t)f + 1] = c;
         t.cf[(size_t)c * t.fpc + lf] = f;
       } else {

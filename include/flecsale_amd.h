@@ -125,7 +125,8 @@ int fsale_error_cell(fsale_ctx *ctx, int64_t *cell);
  * step loops run without graphs and each launch is bracketed by HIP events on
  * the context's stream.  Kernel ids: 0 dt, 1 gradients, 2 fluxes, 3 update
  * (Eulerian); 4 nodal, 5 residual, 6 dt, 7 move, 8 update, 9 save/restore
- * (Lagrangian).  total_ms / launches accumulate until fsale_reset_profile. */
+ * (Lagrangian); 10 fused fluxes + update (the Eulerian step loop's kernel).
+ * total_ms / launches accumulate until fsale_reset_profile. */
 int fsale_set_profiling(fsale_ctx *ctx, int enable);
 int fsale_get_profile(fsale_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
 int fsale_reset_profile(fsale_ctx *ctx);
