@@ -43,8 +43,13 @@ def kernel_bytes_euler(nc, nf, nv, dim):
         # dt: v, a, V, cell->face CSR | n, A ; gradients: 6 vars, centroid, neighbour CSR,
         # cell->vertex CSR, 12 gradients | coords ; fluxes: 6 vars + 12 gradients + centroid |
         # centroid, n, A, face->cell, 4 fluxes ; update: d, v, e in, 7 state out, V, CSR | flux, face->cell
-        return {0: 52 * nc + 24 * nf, 1: 216 * nc + 16 * nv, 2: 160 * nc + 80 * nf, 3: 116 * nc + 40 * nf}
-    return {0: 68 * nc + 32 * nf, 1: 392 * nc + 24 * nv, 2: 248 * nc + 104 * nf, 3: 140 * nc + 48 * nf}
+        b = {0: 52 * nc + 24 * nf, 1: 216 * nc + 16 * nv, 2: 160 * nc + 80 * nf, 3: 116 * nc + 40 * nf}
+    else:
+        b = {0: 68 * nc + 32 * nf, 1: 392 * nc + 24 * nv, 2: 248 * nc + 104 * nf, 3: 140 * nc + 48 * nf}
+    # the step loop's fused kernel does the flux and update tasks, and every
+    # second launch also the dt task of the next step
+    b[10] = b[2] + b[3] + b[0] // 2
+    return b
 
 
 def kernel_bytes_maire(nc, nv, ncn):
@@ -180,12 +185,17 @@ def main():
 
     # byte model
     if kind == "euler":
-        kb = kernel_bytes_euler(nc, ctx.nf, ctx.nv, dim)
-        launches_per_step = {0: 1, 1: 2, 2: 2, 3: 2}
+        tasks = kernel_bytes_euler(nc, ctx.nf, ctx.nv, dim)
+        # task model: one dt, two (gradients, fluxes, update) per step
+        bytes_per_step = tasks[0] + 2 * (tasks[1] + tasks[2] + tasks[3])
+        # kernels the step loop launches: dt finalize (reads the block partials),
+        # gradients, fused fluxes + update (+ next dt)
+        nparts = 8 * ((nc + 256 * 8 - 1) // (256 * 8))
+        kb = {0: 8 * nparts, 1: tasks[1], 10: tasks[10]}
     else:
         kb = kernel_bytes_maire(nc, ctx.nv, nc * 4)
         launches_per_step = {4: 2, 5: 2, 6: 1, 7: 2, 8: 2, 9: 2}
-    bytes_per_step = sum(kb[k] * launches_per_step[k] for k in kb)
+        bytes_per_step = sum(kb[k] * launches_per_step[k] for k in kb)
 
     roofline = None
     kernels = {}

@@ -319,14 +319,14 @@ __global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, c
 // update_state_from_flux euler_eqns.h:383-410 + update_state_from_energy
 // :359-375 for one cell; returns false on negative density / energy.
 template <int N>
-__device__ inline bool update_cell(EulerFields &u, int c, double *du, double sc, double g, double cv,
-                                   double *vel_out, double &a_out) {
+__device__ inline bool update_cell(const EulerFields &in, EulerFields &u, int c, double *du, double sc, double g,
+                                   double cv, double *vel_out, double &a_out) {
   for (int i = 0; i < N + 2; ++i) du[i] *= sc;
   double vel[N];
-  for (int d = 0; d < N; ++d) vel[d] = u.v[d][c];
-  const double den0 = u.d[c];
+  for (int d = 0; d < N; ++d) vel[d] = in.v[d][c];
+  const double den0 = in.d[c];
   const double mass = den0 + du[0];
-  const double et0 = u.e[c] + 0.5 * dotN<N>(vel, vel);
+  const double et0 = in.e[c] + 0.5 * dotN<N>(vel, vel);
   const double ener = den0 * et0 + du[N + 1];
   const double inv_mass = 1 / mass;
   for (int d = 0; d < N; ++d) {
@@ -375,13 +375,14 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
     }
   }
   double vel[N], a;
-  if (!update_cell<N>(u, c, du, delta_t / m.cvol[c], g, cv, vel, a)) flag_negative(s, c);
+  if (!update_cell<N>(u, u, c, du, delta_t / m.cvol[c], g, cv, vel, a)) flag_negative(s, c);
 }
 
 // fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
 template <int N, int FPC, int VPC>
-__global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, DevScalars *s, double g,
-                                                        double cv, double factor, int kind, double *partials) {
+__global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, DevScalars *s,
+                                                        double g, double cv, double factor, int kind,
+                                                        double *partials) {
   const int L = logical_block();
   const int c = L * kBlock + threadIdx.x;
   // one read of the step flag per block: another block may clear it (negative
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
     }
     const double vol = m.cvol[c];
     double vel[N], a;
-    if (!update_cell<N>(u, c, du, delta_t / vol, g, cv, vel, a)) flag_negative(s, c);
+    if (!update_cell<N>(u, out, c, du, delta_t / vol, g, cv, vel, a)) flag_negative(s, c);
     if (partials) {
 #pragma unroll
       for (int k = 0; k < FPC; ++k) {
@@ -494,11 +495,33 @@ int launch_finalize(fsale_ctx *ctx, double cfl, double final_time, int stop_at_f
   return FSALE_OK;
 }
 
-int launch_grad(fsale_ctx *ctx, int first_order) {
-  const DevMesh &m = ctx->m;
-  if (m.dim == 2) return launch(ctx, K_EUL_GRAD, k_eul_grad<2, 4>, m.nc, m, ctx->eu, (const DevScalars *)ctx->s, first_order);
-  return launch(ctx, K_EUL_GRAD, k_eul_grad<3, 8>, m.nc, m, ctx->eu, (const DevScalars *)ctx->s, first_order);
+// the same fields with the two state buffers exchanged
+EulerFields swapped(const EulerFields &a) {
+  EulerFields b = a;
+  b.d = a.alt_d;
+  b.e = a.alt_e;
+  b.p = a.alt_p;
+  b.T = a.alt_T;
+  b.a = a.alt_a;
+  b.alt_d = a.d;
+  b.alt_e = a.e;
+  b.alt_p = a.p;
+  b.alt_T = a.T;
+  b.alt_a = a.a;
+  for (int d = 0; d < 3; ++d) {
+    b.v[d] = a.alt_v[d];
+    b.alt_v[d] = a.v[d];
+  }
+  return b;
 }
+
+int launch_grad(fsale_ctx *ctx, int first_order, const EulerFields &u) {
+  const DevMesh &m = ctx->m;
+  if (m.dim == 2) return launch(ctx, K_EUL_GRAD, k_eul_grad<2, 4>, m.nc, m, u, (const DevScalars *)ctx->s, first_order);
+  return launch(ctx, K_EUL_GRAD, k_eul_grad<3, 8>, m.nc, m, u, (const DevScalars *)ctx->s, first_order);
+}
+
+int launch_grad(fsale_ctx *ctx, int first_order) { return launch_grad(ctx, first_order, ctx->eu); }
 
 int launch_flux(fsale_ctx *ctx, int kind) {
   const DevMesh &m = ctx->m;
@@ -513,27 +536,31 @@ int launch_update(fsale_ctx *ctx, fsale_eos eos, double factor) {
   return launch(ctx, K_EUL_UPDATE, k_eul_update<3, 6>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor);
 }
 
-int launch_fluxupd(fsale_ctx *ctx, fsale_eos eos, double factor, int kind, bool want_dt) {
+int launch_fluxupd(fsale_ctx *ctx, fsale_eos eos, double factor, int kind, bool want_dt, const EulerFields &in,
+                   const EulerFields &out) {
   const DevMesh &m = ctx->m;
   double *parts = want_dt ? ctx->eu.partials : nullptr;
   if (m.dim == 2)
-    return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor,
+    return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, m.nc, m, in, out, ctx->s, eos.gamma, eos.cv, factor,
                   kind, parts);
-  return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor,
+  return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, m.nc, m, in, out, ctx->s, eos.gamma, eos.cv, factor,
                 kind, parts);
 }
 
-// one driver step (apps/hydro/driver.h:317-413) with the fused kernels; the
-// dt^-1 partials of the current state must be valid on entry and are valid
-// again on exit
+// one driver step (apps/hydro/driver.h:317-413) with the fused kernels: the
+// first half step reads the primary state buffer and writes the second one,
+// the second half step writes back, so the state is in the primary buffer at
+// every step boundary.  The dt^-1 partials of the current state must be valid
+// on entry and are valid again on exit.
 int launch_step(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_time, int kind, int first_order,
                 int stop_at_final) {
   int rc;
+  const EulerFields a = ctx->eu, b = swapped(ctx->eu);
   if ((rc = launch_finalize(ctx, cfl, final_time, stop_at_final))) return rc;
-  for (int half = 0; half < 2; ++half) {
-    if ((rc = launch_grad(ctx, first_order))) return rc;
-    if ((rc = launch_fluxupd(ctx, eos, 0.5, kind, half == 1))) return rc;
-  }
+  if ((rc = launch_grad(ctx, first_order, a))) return rc;
+  if ((rc = launch_fluxupd(ctx, eos, 0.5, kind, false, a, b))) return rc;
+  if ((rc = launch_grad(ctx, first_order, b))) return rc;
+  if ((rc = launch_fluxupd(ctx, eos, 0.5, kind, true, b, a))) return rc;
   return FSALE_OK;
 }
 
@@ -554,11 +581,12 @@ int euler_alloc(fsale_ctx *ctx) {
   EulerFields &u = ctx->eu;
   const int N = m.dim;
   int rc;
-  double **cell1[] = {&u.d, &u.e, &u.p, &u.T, &u.a};
+  double **cell1[] = {&u.d, &u.e, &u.p, &u.T, &u.a, &u.alt_d, &u.alt_e, &u.alt_p, &u.alt_T, &u.alt_a};
   for (double **p : cell1)
     if ((rc = dev_alloc_n(ctx, p, m.nc))) return rc;
   for (int d = 0; d < N; ++d) {
     if ((rc = dev_alloc_n(ctx, &u.v[d], m.nc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.alt_v[d], m.nc))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.gd[d], m.nc))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.ge[d], m.nc))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.gp[d], m.nc))) return rc;

@@ -83,6 +83,10 @@ struct DevScalars {
 struct EulerFields {
   double *d = nullptr, *v[3] = {nullptr, nullptr, nullptr}, *e = nullptr, *p = nullptr,
          *T = nullptr, *a = nullptr;
+  // second state buffer: the fused flux+update kernel reads one and writes
+  // the other (its neighbours must see the old state); a step flips twice
+  double *alt_d = nullptr, *alt_v[3] = {nullptr, nullptr, nullptr}, *alt_e = nullptr, *alt_p = nullptr,
+         *alt_T = nullptr, *alt_a = nullptr;
   double *gd[3] = {}, *gv[9] = {}, *ge[3] = {}, *gp[3] = {}, *ga[3] = {};
   double *F[5] = {};
   double *partials = nullptr;    // per-block max of dt^-1 [grid8(nc)]
