@@ -158,8 +158,8 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
       u1[i++] = u.a[c1];
     }
     for (int j = 0; j < NV; ++j) {
-      umin[j] = smin(u1[j], umin[j]);
-      umax[j] = smax(u1[j], umax[j]);
+      umin[j] = fmin(u1[j], umin[j]);
+      umax[j] = fmax(u1[j], umax[j]);
     }
     for (int d = 0; d < N; ++d) dx[d] = m.ccen[d][c1] - x0[d];
     if constexpr (N == 2) {
@@ -202,9 +202,18 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
       gu[j][2] = (a00 * (g2 * a11 - a12 * g1) - a01 * (g2 * a01 - a02 * g1) + g0 * min3) * denom;
     }
   }
-  // limiter apps/hydro/tasks.h:267-299
-  double phi[NV];
-  for (int j = 0; j < NV; ++j) phi[j] = 1;
+  // limiter apps/hydro/tasks.h:267-299.  The reference takes, over the cell's
+  // vertices, phi = min(1, (umax-u0)/du for du > tol, (umin-u0)/du for
+  // du < -tol).  Correctly rounded division is monotone in the divisor, so
+  // the minimum over the first kind is (umax-u0)/max(du) and over the second
+  // (umin-u0)/min(du): two divisions per variable instead of one per vertex,
+  // with the same phi (up to the sign of a zero phi, which only flips the
+  // sign of a zero gradient and never changes a reconstructed value).
+  double dpos[NV], dneg[NV];
+  for (int j = 0; j < NV; ++j) {
+    dpos[j] = 0;
+    dneg[j] = 0;
+  }
 #pragma unroll
   for (int lv = 0; lv < VPC; ++lv) {
     const int vid = m.cv[(size_t)lv * m.nc + c0];
@@ -219,10 +228,16 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
     for (int j = 0; j < NV; ++j) {
       const double du = u1[j] - u0[j];
       if (du > tol)
-        phi[j] = smin(phi[j], (umax[j] - u0[j]) / du);
+        dpos[j] = fmax(dpos[j], du);
       else if (du < -tol)
-        phi[j] = smin(phi[j], (umin[j] - u0[j]) / du);
+        dneg[j] = fmin(dneg[j], du);
     }
+  }
+  double phi[NV];
+  for (int j = 0; j < NV; ++j) {
+    phi[j] = 1;
+    if (dpos[j] > tol) phi[j] = smin(phi[j], (umax[j] - u0[j]) / dpos[j]);
+    if (dneg[j] < -tol) phi[j] = smin(phi[j], (umin[j] - u0[j]) / dneg[j]);
   }
   for (int j = 0; j < NV; ++j) {
     phi[j] = smax(phi[j], 0);
@@ -283,12 +298,12 @@ __device__ inline void reconstruct(const CellData<N> &q, const double *fx, EStat
 
 // numerical flux of one face times its area (flux(f) of evaluate_fluxes)
 template <int N>
-__device__ inline void face_flux_area(int kind, const CellData<N> &q0, const CellData<N> *q1, const double *fx,
-                                      const double *n, double area, double *out) {
+__device__ inline void face_flux_area(int kind, const CellData<N> &q0, const CellData<N> &q1, bool interior,
+                                      const double *fx, const double *n, double area, double *out) {
   EState<N> wl, wr;
   reconstruct<N>(q0, fx, wl);
-  if (q1) {
-    reconstruct<N>(*q1, fx, wr);
+  if (interior) {
+    reconstruct<N>(q1, fx, wr);
     face_flux<N>(kind & 0xf, wl, wr, n, out);
   } else if ((kind >> 4) == 1) {
     e_flux<N>(wl, n, out);   // transmissive boundary
@@ -310,8 +325,8 @@ __global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, c
   CellData<N> q0, q1;
   load_cell<N>(m, u, m.fc[f], q0);
   const int c1 = m.fc[(size_t)m.nf + f];
-  if (c1 >= 0) load_cell<N>(m, u, c1, q1);
-  face_flux_area<N>(kind, q0, c1 >= 0 ? &q1 : nullptr, fx, n, m.farea[f], out);
+  load_cell<N>(m, u, c1 >= 0 ? c1 : m.fc[f], q1);
+  face_flux_area<N>(kind, q0, q1, c1 >= 0, fx, n, m.farea[f], out);
   for (int i = 0; i < N + 2; ++i) u.F[i][f] = out[i];
 }
 
@@ -431,21 +446,21 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
         ak[k] = m.farea[f];
       }
       const int other = m.fc[(size_t)(second ? 0 : 1) * m.nf + f];
-      double out[N + 2];
+      double fl[N + 2];
       if (other >= 0) {
         CellData<N> nb;
         load_cell<N>(m, u, other, nb);
         if (!second)
-          face_flux_area<N>(kind, own, &nb, fx, nk[k], ak[k], out);
+          face_flux_area<N>(kind, own, nb, true, fx, nk[k], ak[k], fl);
         else
-          face_flux_area<N>(kind, nb, &own, fx, nk[k], ak[k], out);
+          face_flux_area<N>(kind, nb, own, true, fx, nk[k], ak[k], fl);
       } else {
-        face_flux_area<N>(kind, own, nullptr, fx, nk[k], ak[k], out);
+        face_flux_area<N>(kind, own, own, false, fx, nk[k], ak[k], fl);
       }
       if (second) {
-        for (int i = 0; i < N + 2; ++i) du[i] += out[i];
+        for (int i = 0; i < N + 2; ++i) du[i] += fl[i];
       } else {
-        for (int i = 0; i < N + 2; ++i) du[i] -= out[i];
+        for (int i = 0; i < N + 2; ++i) du[i] -= fl[i];
       }
     }
     const double vol = m.cvol[c];

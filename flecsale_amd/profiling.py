@@ -1,0 +1,142 @@
+"""HBM-traffic measurement for the step kernels with rocprofv3 hardware counters.
+
+Two passes, each under its own rocprofv3 run (the counters do not fit one pass):
+
+    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m flecsale_amd.profiling run
+    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m flecsale_amd.profiling run
+    python3 -m flecsale_amd.profiling summarize gpurun_out/pmc_fetch/run_results.db \\
+        gpurun_out/pmc_write/run_results.db --workload muscl2d
+
+`run` first launches a plain device-to-device copy of a known size (512 MiB,
+larger than the 256 MiB Infinity Cache) so each counter can be calibrated
+against bytes that are known exactly, then a few eager steps of the workload.
+`summarize` reads both databases, scales each kernel's per-dispatch counter by
+(known copy bytes / counted copy bytes) and writes profiles/<workload>_traffic.json
+(HBM bytes per launch, by kernel) and a markdown table.
+"""
+import argparse
+import json
+import os
+import re
+import sqlite3
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+COPY_BYTES = 512 << 20   # per direction
+
+
+def _copy_calibration():
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    n = COPY_BYTES
+    a, b = ct.c_void_p(), ct.c_void_p()
+    assert hip.hipMalloc(ct.byref(a), ct.c_size_t(n)) == 0
+    assert hip.hipMalloc(ct.byref(b), ct.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ct.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    for _ in range(3):   # hipMemcpyDeviceToDevice = 3
+        assert hip.hipMemcpy(b, a, ct.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def run(workload, steps, size):
+    sys.path.insert(0, ROOT)
+    import flecsale_amd as fa
+    from flecsale_amd import inputs
+    print("[profiling] copy calibration", flush=True)
+    _copy_calibration()
+    if workload == "muscl2d":
+        prob = inputs.quadrants_2d(size or 1024, steps=steps, flux=fa.FLUX_HLLC | fa.BOUNDARY_TRANSMISSIVE)
+        ctx = fa.EulerContext(prob["coords"], prob["cells"])
+        ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+        ctx.advance(prob["cfl"], prob["final_time"], steps, flux_kind=prob["flux"], use_graph=False)
+    elif workload == "muscl3d":
+        prob = inputs.kelvin_helmholtz_3d(size or 128, steps=steps)
+        ctx = fa.EulerContext(prob["coords"], prob["cells"])
+        ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+        ctx.advance(prob["cfl"], prob["final_time"], steps, flux_kind=prob["flux"], use_graph=False)
+    else:
+        prob = inputs.maire_bubble_2d(size or 512, steps=steps)
+        ctx = fa.MaireContext(prob["coords"], prob["cells"])
+        ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
+        ctx.set_time(0.0, prob["dt0"])
+        ctx.advance(prob["cfl"], prob["final_time"], steps, use_graph=False)
+    ctx.sync()
+    print(f"[profiling] {workload}: {steps} steps done", flush=True)
+    ctx.close()
+
+
+def _per_kernel(db_path):
+    """{kernel short name: (dispatches, mean counter value)} from a rocpd database."""
+    db = sqlite3.connect(db_path)
+    cur = db.cursor()
+    rows = cur.execute("select * from counters_collection").fetchall()
+    cols = [d[0] for d in cur.description]
+    name_col = "kernel_name" if "kernel_name" in cols else "name"
+    ic = cols.index(name_col)
+    iv = cols.index("value")
+    idisp = cols.index("dispatch_id") if "dispatch_id" in cols else None
+    acc = {}
+    for r in rows:
+        name = re.sub(r"\(.*", "", str(r[ic]).replace("fsale::(anonymous namespace)::", "").replace("void ", ""))
+        disp = r[idisp] if idisp is not None else None
+        acc.setdefault(name, {}).setdefault(disp, 0.0)
+        acc[name][disp] += float(r[iv])
+    return {k: (len(v), sum(v.values()) / len(v)) for k, v in acc.items()}
+
+
+def summarize(fetch_db, write_db, workload):
+    fetch = _per_kernel(fetch_db)
+    write = _per_kernel(write_db)
+    copy = [k for k in fetch if "copy" in k.lower()]
+    if not copy:
+        raise SystemExit("no copy kernel found for calibration")
+    ck = max(copy, key=lambda k: fetch[k][1])
+    f_scale = COPY_BYTES / (fetch[ck][1] * 1024.0)
+    w_scale = COPY_BYTES / (write[ck][1] * 1024.0)
+    table = {}
+    for k in fetch:
+        if k not in write or k in copy:
+            continue
+        table[k] = {"dispatches": fetch[k][0], "fetch_KiB": fetch[k][1], "write_KiB": write[k][1],
+                    "hbm_bytes": fetch[k][1] * 1024.0 * f_scale + write[k][1] * 1024.0 * w_scale}
+    out = {"copy_kernel": ck, "copy_bytes_per_direction": COPY_BYTES, "fetch_scale": f_scale,
+           "write_scale": w_scale, "kernels": table}
+    short = {}
+    for k, v in table.items():
+        m = re.match(r"k_(\w+?)(<|$)", k)
+        if m:
+            short[m.group(1)] = v["hbm_bytes"]
+    out.update(short)
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"copy kernel {ck}: fetch scale {f_scale:.4f}, write scale {w_scale:.4f}")
+    print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes (calibrated) |")
+    print("|---|---|---|---|---|")
+    for k, v in sorted(table.items(), key=lambda kv: -kv[1]["hbm_bytes"]):
+        print(f"| {k} | {v['dispatches']} | {v['fetch_KiB']:.0f} | {v['write_KiB']:.0f} | {v['hbm_bytes']:.4e} |")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("run")
+    r.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d"])
+    r.add_argument("--steps", type=int, default=3)
+    r.add_argument("--size", type=int, default=None)
+    s = sub.add_parser("summarize")
+    s.add_argument("fetch_db")
+    s.add_argument("write_db")
+    s.add_argument("--workload", default="muscl2d")
+    a = ap.parse_args()
+    if a.cmd == "run":
+        run(a.workload, a.steps, a.size)
+    else:
+        summarize(a.fetch_db, a.write_db, a.workload)
+
+
+if __name__ == "__main__":
+    main()
