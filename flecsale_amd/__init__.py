@@ -53,6 +53,9 @@ EXPORTS = [
     "fsale_maire_initial_conditions", "fsale_maire_evaluate_nodal_state", "fsale_maire_evaluate_residual",
     "fsale_maire_evaluate_time_step", "fsale_maire_apply_update", "fsale_maire_save", "fsale_maire_restore",
     "fsale_maire_advance",
+    "fsale_batch_create", "fsale_batch_add_euler", "fsale_batch_add_maire", "fsale_batch_remove",
+    "fsale_batch_step", "fsale_batch_status", "fsale_batch_get_field", "fsale_batch_capacity",
+    "fsale_launch_shape",
 ]
 
 
@@ -69,6 +72,21 @@ class MeshDesc(ct.Structure):
 
 class Eos(ct.Structure):
     _fields_ = [("gamma", ct.c_double), ("cv", ct.c_double)]
+
+
+class EulerParams(ct.Structure):
+    _fields_ = [("cfl", ct.c_double), ("final_time", ct.c_double), ("flux_kind", ct.c_int32),
+                ("first_order", ct.c_int32), ("stop_at_final", ct.c_int32), ("max_steps", ct.c_int64)]
+
+
+class MaireParams(ct.Structure):
+    _fields_ = [("cfl_accoustic", ct.c_double), ("cfl_volume", ct.c_double), ("cfl_growth", ct.c_double),
+                ("final_time", ct.c_double), ("initial_time_step", ct.c_double), ("max_steps", ct.c_int64)]
+
+
+class ProblemStatus(ct.Structure):
+    _fields_ = [("problem", ct.c_int32), ("error", ct.c_int32), ("steps", ct.c_int64), ("time", ct.c_double),
+                ("time_step", ct.c_double), ("done", ct.c_int32)]
 
 
 _lib = None
@@ -113,6 +131,19 @@ def lib():
         L.fsale_maire_save.argtypes = [vp]
         L.fsale_maire_restore.argtypes = [vp]
         L.fsale_maire_advance.argtypes = [vp, dp, dp, dp, dp, ct.c_int, ct.c_int]
+        i64 = ct.c_int64
+        L.fsale_batch_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, i32, i64, i64, i64, i64, ct.POINTER(vp)]
+        L.fsale_batch_add_euler.argtypes = [vp, ct.POINTER(MeshDesc), Eos, ct.POINTER(EulerParams), vp, vp, vp,
+                                            ct.POINTER(i32)]
+        L.fsale_batch_add_maire.argtypes = [vp, ct.POINTER(MeshDesc), dp, vp, ct.POINTER(MaireParams), vp, vp, vp,
+                                            ct.POINTER(i32)]
+        L.fsale_batch_remove.argtypes = [vp, i32]
+        L.fsale_batch_step.argtypes = [vp, ct.c_int, ct.c_int]
+        L.fsale_batch_status.argtypes = [vp, i32, ct.POINTER(ProblemStatus), ct.POINTER(i32)]
+        L.fsale_batch_get_field.argtypes = [vp, i32, ct.c_int, vp]
+        L.fsale_batch_capacity.argtypes = [vp, ct.POINTER(i64), ct.POINTER(i64), ct.POINTER(i64),
+                                           ct.POINTER(i64), ct.POINTER(i32)]
+        L.fsale_launch_shape.argtypes = [vp, i32]
         for name in EXPORTS:
             f = getattr(L, name)
             if f.restype is ct.c_int or f.restype is None:
@@ -325,3 +356,111 @@ class MaireContext(_Context):
         s = super().state()
         s.update(M=self.get(F_MASS), V=self.get(F_VOLUME))
         return s
+
+
+class Batch:
+    """A ragged batch of independent problems of one solver (fsale_batch_*).
+
+    Each resident problem runs the reference's own step loop with its own
+    mesh, EOS, CFL and clock; problems are admitted (add_euler / add_maire)
+    and retired (remove) between steps.
+    """
+
+    def __init__(self, solver, dim, verts_per_cell, max_problems, cells, faces, vertices, neighbours):
+        h = ct.c_void_p()
+        _check(lib().fsale_batch_create(int(solver), int(dim), int(verts_per_cell), int(max_problems),
+                                        int(cells), int(faces), int(vertices), int(neighbours), ct.byref(h)))
+        self.h = h
+        self.solver = solver
+        self.dim = dim
+        self.vpc = verts_per_cell
+        self.meshes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fsale_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_euler(self, coords, cells, eos, d, v, p, cfl, final_time, max_steps, flux_kind=FLUX_HLLE,
+                  first_order=False, stop_at_final=False):
+        desc, keep = _desc(coords, cells)
+        nc = desc.num_cells
+        d = np.ascontiguousarray(np.broadcast_to(d, (nc,)), dtype=np.float64)
+        v = np.ascontiguousarray(np.broadcast_to(v, (nc, self.dim)), dtype=np.float64)
+        p = np.ascontiguousarray(np.broadcast_to(p, (nc,)), dtype=np.float64)
+        prm = EulerParams(cfl, final_time, int(flux_kind), int(first_order), int(stop_at_final), int(max_steps))
+        out = ct.c_int32()
+        _check(lib().fsale_batch_add_euler(self.h, ct.byref(desc), Eos(*eos), ct.byref(prm), d.ctypes.data,
+                                           v.ctypes.data, p.ctypes.data, ct.byref(out)))
+        self.meshes[out.value] = keep
+        return out.value
+
+    def add_maire(self, coords, cells, cv, gamma, d, v, p, cfl, final_time, max_steps, initial_time_step):
+        desc, keep = _desc(coords, cells)
+        nc = desc.num_cells
+        g = np.ascontiguousarray(np.broadcast_to(gamma, (nc,)), dtype=np.float64)
+        d = np.ascontiguousarray(np.broadcast_to(d, (nc,)), dtype=np.float64)
+        v = np.ascontiguousarray(np.broadcast_to(v, (nc, 2)), dtype=np.float64)
+        p = np.ascontiguousarray(np.broadcast_to(p, (nc,)), dtype=np.float64)
+        prm = MaireParams(cfl[0], cfl[1], cfl[2], final_time, initial_time_step, int(max_steps))
+        out = ct.c_int32()
+        _check(lib().fsale_batch_add_maire(self.h, ct.byref(desc), cv, g.ctypes.data, ct.byref(prm),
+                                           d.ctypes.data, v.ctypes.data, p.ctypes.data, ct.byref(out)))
+        self.meshes[out.value] = keep
+        return out.value
+
+    def remove(self, problem):
+        _check(lib().fsale_batch_remove(self.h, int(problem)))
+        self.meshes.pop(problem, None)
+
+    def step(self, nsteps, use_graph=True):
+        _check(lib().fsale_batch_step(self.h, int(nsteps), int(use_graph)))
+
+    def status(self):
+        """{problem: ProblemStatus} of every resident problem."""
+        n = max(len(self.meshes), 1)
+        arr = (ProblemStatus * n)()
+        cnt = ct.c_int32()
+        _check(lib().fsale_batch_status(self.h, n, arr, ct.byref(cnt)))
+        return {arr[i].problem: arr[i] for i in range(cnt.value)}
+
+    def get(self, problem, field):
+        coords, cells = self.meshes[problem]
+        nc, nv, N = cells.shape[0], coords.shape[0], self.dim
+        shapes = {F_DENSITY: (nc,), F_VELOCITY: (nc, N), F_INTERNAL_ENERGY: (nc,), F_PRESSURE: (nc,),
+                  F_TEMPERATURE: (nc,), F_SOUND_SPEED: (nc,), F_CELL_CENTROID: (nc, N), F_CELL_VOLUME: (nc,),
+                  F_COORDS: (nv, N), F_MASS: (nc,), F_VOLUME: (nc,), F_NODE_VELOCITY: (nv, N),
+                  F_GAMMA: (nc,)}
+        out = np.zeros(shapes[field])
+        _check(lib().fsale_batch_get_field(self.h, int(problem), int(field), out.ctypes.data))
+        return out
+
+    def state(self, problem):
+        s = dict(d=self.get(problem, F_DENSITY), v=self.get(problem, F_VELOCITY),
+                 e=self.get(problem, F_INTERNAL_ENERGY), p=self.get(problem, F_PRESSURE),
+                 T=self.get(problem, F_TEMPERATURE), a=self.get(problem, F_SOUND_SPEED))
+        if self.solver == SOLVER_MAIRE:
+            s["x"] = self.get(problem, F_COORDS)
+        return s
+
+    def capacity(self):
+        c, f, v, n = ct.c_int64(), ct.c_int64(), ct.c_int64(), ct.c_int64()
+        p = ct.c_int32()
+        _check(lib().fsale_batch_capacity(self.h, ct.byref(c), ct.byref(f), ct.byref(v), ct.byref(n),
+                                          ct.byref(p)))
+        return dict(cells=c.value, faces=f.value, vertices=v.value, neighbours=n.value, problems=p.value)
+
+    def launch_shape(self, blocks_per_cu):
+        _check(lib().fsale_launch_shape(self.h, int(blocks_per_cu)))

@@ -1,10 +1,11 @@
-// Context life cycle, field I/O and error reporting of the C ABI
-// (include/flecsale_amd.h).  The context replaces the FleCSI runtime's field
-// storage and client handles (apps/hydro/driver.h:35-158,215-232;
-// apps/maire_hydro/driver.h:36-166,237-258): all state lives in device
-// memory as SoA FP64 arrays.
+// Contexts, problems, field I/O and error reporting of the C ABI
+// (include/flecsale_amd.h).  A context replaces the FleCSI runtime's field
+// storage and mesh client handles (apps/hydro/driver.h:35-158,215-232;
+// apps/maire_hydro/driver.h:36-166,237-258): every field of every resident
+// problem lives in device memory as SoA FP64 arrays of capacity size.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
-#include <mutex>
 
 #include "internal.h"
 #include "kernels_common.h"
@@ -13,6 +14,42 @@ namespace fsale {
 
 static thread_local std::string g_message;
 void set_message(const std::string &msg) { g_message = msg; }
+
+int RangeAlloc::alloc(int n) {
+  if (n <= 0) return 0;
+  for (auto it = free.begin(); it != free.end(); ++it) {
+    if (it->second >= n) {
+      const int start = it->first, len = it->second;
+      free.erase(it);
+      if (len > n) free[start + n] = len - n;
+      return start;
+    }
+  }
+  return -1;
+}
+
+void RangeAlloc::release(int start, int n) {
+  if (n <= 0) return;
+  auto it = free.emplace(start, n).first;
+  auto next = std::next(it);
+  if (next != free.end() && it->first + it->second == next->first) {
+    it->second += next->second;
+    free.erase(next);
+  }
+  if (it != free.begin()) {
+    auto prev = std::prev(it);
+    if (prev->first + prev->second == it->first) {
+      prev->second += it->second;
+      free.erase(it);
+    }
+  }
+}
+
+long long RangeAlloc::available() const {
+  long long s = 0;
+  for (auto &kv : free) s += kv.second;
+  return s;
+}
 
 int dev_alloc(fsale_ctx *ctx, void **p, size_t bytes) {
   *p = nullptr;
@@ -46,226 +83,579 @@ ProfScope::~ProfScope() {
 }
 
 int read_scalars(fsale_ctx *ctx) {
-  FSALE_HIP(hipMemcpyAsync(ctx->h_s, ctx->s, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+  FSALE_HIP(hipMemcpyAsync(ctx->h_s, ctx->s, sizeof(DevScalars) * ctx->max_problems, hipMemcpyDeviceToHost,
+                           ctx->stream));
   FSALE_HIP(hipStreamSynchronize(ctx->stream));
   return FSALE_OK;
 }
 
-int check_device_error(fsale_ctx *ctx) {
+int check_device_error(fsale_ctx *ctx, int prob) {
   int rc = read_scalars(ctx);
   if (rc) return rc;
-  if (ctx->h_s->err_code) {
-    if (ctx->h_s->err_code == FSALE_ERR_NEG_STATE)
-      set_message("Negative density or internal energy encountered! (cell " +
-                  std::to_string(ctx->h_s->err_cell) + ")");
-    else if (ctx->h_s->err_code == FSALE_ERR_DT)
+  const DevScalars &S = ctx->h_s[prob];
+  if (S.err_code) {
+    if (S.err_code == FSALE_ERR_NEG_STATE)
+      set_message("Negative density or internal energy encountered! (cell " + std::to_string(S.err_cell) + ")");
+    else if (S.err_code == FSALE_ERR_DT)
       set_message("infinite delta t");
-    return ctx->h_s->err_code;
+    return S.err_code;
   }
   return FSALE_OK;
+}
+
+int upload_params(fsale_ctx *ctx, int prob) {
+  FSALE_HIP(hipMemcpyAsync(ctx->d_params + prob, &ctx->probs[prob].params, sizeof(ProbParams),
+                           hipMemcpyHostToDevice, ctx->stream));
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  return FSALE_OK;
+}
+
+int reset_scalars(fsale_ctx *ctx, int prob, double t, double dt) {
+  DevScalars h{};
+  h.t = t;
+  h.dt = dt;
+  h.active = 1;
+  h.err_code = 0;
+  h.err_cell = INT64_MAX;
+  h.steps = 0;
+  ctx->h_s[prob] = h;
+  FSALE_HIP(hipMemcpyAsync(ctx->s + prob, &ctx->h_s[prob], sizeof(DevScalars), hipMemcpyHostToDevice,
+                           ctx->stream));
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  return FSALE_OK;
+}
+
+// Tile lists of the live problems, problems in id order, each problem's
+// tiles contiguous.  Uploaded only when the set of problems changed.
+int sync_tiles(fsale_ctx *ctx) {
+  if (!ctx->tiles_dirty) return FSALE_OK;
+  ctx->h_ctiles.clear();
+  ctx->h_vtiles.clear();
+  for (int p = 0; p < (int)ctx->probs.size(); ++p) {
+    const Problem &P = ctx->probs[p];
+    if (!P.live) continue;
+    for (int i = 0; i < P.nct; ++i)
+      ctx->h_ctiles.push_back(Tile{p, P.c0 + i * kBlock, std::min(kBlock, P.nc - i * kBlock), P.ct0 + i});
+    for (int i = 0; i < P.nvt; ++i)
+      ctx->h_vtiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nv - i * kBlock), P.vt0 + i});
+  }
+  const int counts[2] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size()};
+  if (!ctx->h_ctiles.empty())
+    FSALE_HIP(hipMemcpyAsync(ctx->d_ctiles, ctx->h_ctiles.data(), sizeof(Tile) * ctx->h_ctiles.size(),
+                             hipMemcpyHostToDevice, ctx->stream));
+  if (!ctx->h_vtiles.empty())
+    FSALE_HIP(hipMemcpyAsync(ctx->d_vtiles, ctx->h_vtiles.data(), sizeof(Tile) * ctx->h_vtiles.size(),
+                             hipMemcpyHostToDevice, ctx->stream));
+  FSALE_HIP(hipMemcpyAsync(ctx->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice, ctx->stream));
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->tiles_dirty = false;
+  return FSALE_OK;
+}
+
+Batch batch_args(const fsale_ctx *ctx) {
+  Batch b;
+  b.ct.tiles = ctx->d_ctiles;
+  b.ct.n = ctx->d_counts;
+  b.vt.tiles = ctx->d_vtiles;
+  b.vt.n = ctx->d_counts + 1;
+  b.pp = ctx->d_params;
+  b.s = ctx->s;
+  b.cap_c_tiles = ctx->ra_ct.capacity;
+  return b;
+}
+
+// Persistent grid: CUs x blocks_per_cu blocks, a multiple of 8 (one slice
+// per XCD), never more than the tile capacity can use.
+int persistent_grid(const fsale_ctx *ctx) {
+  int g = ctx->num_cus * ctx->shape.blocks_per_cu;
+  const int tiles = std::max(ctx->ra_ct.capacity, ctx->ra_vt.capacity);
+  const int cap = ((tiles + 7) / 8) * 8;
+  if (cap > 0 && g > cap) g = cap;
+  return std::max(8, (g / 8) * 8);
 }
 
 namespace {
 
-template <class T>
-int upload_kmajor(fsale_ctx *ctx, T **dst, const std::vector<T> &src, int n, int k) {
-  // src is entity-major [n][k]; device layout is k-major [k][n]
-  std::vector<T> tmp((size_t)n * k);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < k; ++j) tmp[(size_t)j * n + i] = src[(size_t)i * k + j];
-  int rc = dev_alloc_n(ctx, dst, tmp.size());
-  if (rc) return rc;
-  FSALE_HIP(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
-  return FSALE_OK;
+std::string mesh_key(const fsale_mesh_desc &m) {
+  // connectivity depends on the cell table only
+  std::string k(reinterpret_cast<const char *>(&m.dim), sizeof(int32_t) * 4);
+  k.append(reinterpret_cast<const char *>(m.cells), sizeof(int32_t) * (size_t)m.num_cells * m.verts_per_cell);
+  return k;
 }
 
 template <class T>
-int upload(fsale_ctx *ctx, T **dst, const std::vector<T> &src) {
-  int rc = dev_alloc_n(ctx, dst, src.size() + 1);
-  if (rc) return rc;
-  if (!src.empty()) FSALE_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-  return FSALE_OK;
-}
-
-int create_impl(const fsale_mesh_desc *mesh, int solver, fsale_ctx *ctx) {
-  std::string err;
-  int rc = build_topology(*mesh, ctx->topo, err);
-  if (rc) {
-    set_message(err);
-    return rc;
+int copy_h2d(fsale_ctx *ctx, T *dst, const T *src, size_t n) {
+  if (n) {
+    FSALE_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    FSALE_HIP(hipStreamSynchronize(ctx->stream));
   }
-  const HostTopology &t = ctx->topo;
-  if (solver == FSALE_SOLVER_MAIRE && (t.dim != 2 || t.vpc != 4)) {
+  return FSALE_OK;
+}
+
+int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, long long cap_c, long long cap_f,
+                long long cap_v, long long cap_n) {
+  if (!((dim == 2 && vpc >= 3 && vpc <= 8) || (dim == 3 && vpc == 8))) {
+    set_message("unsupported cell type (quads/polygons in 2D, hexes in 3D)");
+    return FSALE_ERR_MESH;
+  }
+  if (solver == FSALE_SOLVER_MAIRE && (dim != 2 || vpc != 4)) {
     set_message("the Lagrangian solver supports 2D quad meshes");
     return FSALE_ERR_MESH;
   }
+  if (max_problems <= 0 || cap_c <= 0 || cap_f <= 0 || cap_v <= 0 || cap_n < 0 || cap_c > INT32_MAX / 8 ||
+      cap_f > INT32_MAX / 2 || cap_v > INT32_MAX / 2 || cap_n > INT32_MAX / 2) {
+    set_message("bad batch capacity");
+    return FSALE_ERR_ARG;
+  }
+  ctx->solver = solver;
+  ctx->max_problems = max_problems;
+  DevMesh &m = ctx->m;
+  m.dim = dim;
+  m.vpc = vpc;
+  m.fpc = (dim == 2) ? vpc : 6;
+  m.vpf = (dim == 2) ? 2 : 4;
+  m.cap_c = (int)cap_c;
+  m.cap_f = (int)cap_f;
+  m.cap_v = (int)cap_v;
+  ctx->ra_c.init((int)cap_c);
+  ctx->ra_f.init((int)cap_f);
+  ctx->ra_v.init((int)cap_v);
+  ctx->ra_n.init((int)cap_n);
+  const int cap_ct = (int)((cap_c + kBlock - 1) / kBlock) + max_problems;
+  const int cap_vt = (int)((cap_v + kBlock - 1) / kBlock) + max_problems;
+  ctx->ra_ct.init(cap_ct);
+  ctx->ra_vt.init(cap_vt);
+  int rc;
+  for (int d = 0; d < dim; ++d) {
+    if ((rc = dev_alloc_n(ctx, &m.x[d], cap_v))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.ccen[d], cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.fcen[d], cap_f))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.fnrm[d], cap_f))) return rc;
+  }
+  if ((rc = dev_alloc_n(ctx, &m.cv, (size_t)cap_c * vpc))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.cf, (size_t)cap_c * m.fpc))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.fc, (size_t)cap_f * 2))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.fv, (size_t)cap_f * m.vpf))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.nbr_start, cap_c))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.nbr_cnt, cap_c))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.nbr, cap_n + 1))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.cvol, cap_c))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.cminlen, cap_c))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.farea, cap_f))) return rc;
+  if (solver == FSALE_SOLVER_MAIRE) {
+    if ((rc = dev_alloc_n(ctx, &m.vc_start, cap_v))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.vc_cnt, cap_v))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.vc, (size_t)cap_c * vpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.cwtag, (size_t)cap_c * vpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.vbnd, cap_v))) return rc;
+  }
+  if ((rc = dev_alloc_n(ctx, &ctx->d_ctiles, cap_ct))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->d_vtiles, cap_vt))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 2))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->d_params, max_problems))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->s, max_problems))) return rc;
+  FSALE_HIP(hipHostMalloc((void **)&ctx->h_s, sizeof(DevScalars) * max_problems, hipHostMallocDefault));
+  for (int p = 0; p < max_problems; ++p) {
+    ctx->h_s[p] = DevScalars{};
+    ctx->h_s[p].err_cell = INT64_MAX;
+  }
+  FSALE_HIP(hipMemcpy(ctx->s, ctx->h_s, sizeof(DevScalars) * max_problems, hipMemcpyHostToDevice));
+  std::vector<ProbParams> zero(max_problems);
+  std::memset(zero.data(), 0, sizeof(ProbParams) * max_problems);
+  FSALE_HIP(hipMemcpy(ctx->d_params, zero.data(), sizeof(ProbParams) * max_problems, hipMemcpyHostToDevice));
+  const int counts[2] = {0, 0};
+  FSALE_HIP(hipMemcpy(ctx->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice));
+  ctx->probs.assign(max_problems, Problem{});
+  if (solver == FSALE_SOLVER_MAIRE) return maire_alloc(ctx);
+  return euler_alloc(ctx);
+}
+
+// places one mesh in the batch: ranges, connectivity (offset to batch ids),
+// coordinates, geometry, boundary tags
+int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob) {
+  DevMesh &m = ctx->m;
+  if (mesh.dim != m.dim || mesh.verts_per_cell != m.vpc) {
+    set_message("mesh type differs from the batch's");
+    return FSALE_ERR_MESH;
+  }
+  if (!mesh.coords || !mesh.cells) return FSALE_ERR_ARG;
+  int prob = -1;
+  for (int p = 0; p < ctx->max_problems; ++p)
+    if (!ctx->probs[p].live) {
+      prob = p;
+      break;
+    }
+  if (prob < 0) {
+    set_message("no free problem slot");
+    return FSALE_ERR_OOM;
+  }
+  // connectivity, cached by cell table
+  std::shared_ptr<const HostTopology> topo;
+  const std::string key = mesh_key(mesh);
+  auto it = ctx->topo_cache.find(key);
+  if (it != ctx->topo_cache.end()) {
+    topo = it->second;
+  } else {
+    auto t = std::make_shared<HostTopology>();
+    std::string err;
+    int rc = build_topology(mesh, *t, err);
+    if (rc) {
+      set_message(err);
+      return rc;
+    }
+    if (ctx->topo_cache.size() > 1024) ctx->topo_cache.clear();
+    topo = t;
+    ctx->topo_cache[key] = t;
+  }
+  const HostTopology &t = *topo;
   for (int v = 0; v < t.nv; ++v)
     if (t.vc_off[v + 1] - t.vc_off[v] > kMaxCornersPerVertex) {
       set_message("more than 8 cells around a vertex");
       return FSALE_ERR_MESH;
     }
-  DevMesh &m = ctx->m;
-  m.dim = t.dim;
-  m.nc = t.nc;
-  m.nv = t.nv;
-  m.nf = t.nf;
-  m.vpc = t.vpc;
-  m.fpc = t.fpc;
-  m.vpf = t.vpf;
-  m.nnbr = (int)t.nbr.size();
-  for (int d = 0; d < t.dim; ++d) {
-    std::vector<double> xd(t.nv);
-    for (int v = 0; v < t.nv; ++v) xd[v] = mesh->coords[(size_t)v * t.dim + d];
-    if ((rc = upload(ctx, &m.x[d], xd))) return rc;
+  Problem P;
+  P.nc = t.nc;
+  P.nf = t.nf;
+  P.nv = t.nv;
+  P.nn = (int)t.nbr.size();
+  P.nct = (t.nc + kBlock - 1) / kBlock;
+  P.nvt = (t.nv + kBlock - 1) / kBlock;
+  P.c0 = ctx->ra_c.alloc(P.nc);
+  P.f0 = ctx->ra_f.alloc(P.nf);
+  P.v0 = ctx->ra_v.alloc(P.nv);
+  P.n0 = ctx->ra_n.alloc(P.nn);
+  P.ct0 = ctx->ra_ct.alloc(P.nct);
+  P.vt0 = ctx->ra_vt.alloc(P.nvt);
+  if (P.c0 < 0 || P.f0 < 0 || P.v0 < 0 || P.n0 < 0 || P.ct0 < 0 || P.vt0 < 0) {
+    if (P.c0 >= 0) ctx->ra_c.release(P.c0, P.nc);
+    if (P.f0 >= 0) ctx->ra_f.release(P.f0, P.nf);
+    if (P.v0 >= 0) ctx->ra_v.release(P.v0, P.nv);
+    if (P.n0 >= 0) ctx->ra_n.release(P.n0, P.nn);
+    if (P.ct0 >= 0) ctx->ra_ct.release(P.ct0, P.nct);
+    if (P.vt0 >= 0) ctx->ra_vt.release(P.vt0, P.nvt);
+    set_message("batch capacity exhausted");
+    return FSALE_ERR_OOM;
   }
-  if ((rc = upload_kmajor(ctx, &m.cv, t.cv, t.nc, t.vpc))) return rc;
+  P.topo = topo;
+  // mark the slot taken before any failure so that the cleanup releases it
+  P.live = true;
+  P.params = ProbParams{};
+  P.params.cell_first = P.c0;
+  P.params.cell_count = P.nc;
+  P.params.tile_first = P.ct0;
+  P.params.tile_count = P.nct;
+  ctx->probs[prob] = P;
+  ctx->tiles_dirty = true;
+  *out_prob = prob;
+  // connectivity with batch ids, one k-row segment at a time
+  const int nc = t.nc, nf = t.nf, nv = t.nv;
+  std::vector<int32_t> buf;
+  int rc;
+  for (int k = 0; k < t.vpc; ++k) {
+    buf.resize(nc);
+    for (int c = 0; c < nc; ++c) buf[c] = t.cv[(size_t)c * t.vpc + k] + P.v0;
+    if ((rc = copy_h2d(ctx, m.cv + (size_t)k * m.cap_c + P.c0, buf.data(), nc))) return rc;
+  }
+  for (int k = 0; k < t.fpc; ++k) {
+    buf.resize(nc);
+    for (int c = 0; c < nc; ++c) {
+      const int f = t.cf[(size_t)c * t.fpc + k];
+      buf[c] = 2 * (f + P.f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
+    }
+    if ((rc = copy_h2d(ctx, m.cf + (size_t)k * m.cap_c + P.c0, buf.data(), nc))) return rc;
+  }
+  for (int k = 0; k < 2; ++k) {
+    buf.resize(nf);
+    for (int f = 0; f < nf; ++f) {
+      const int c = t.fc[2 * (size_t)f + k];
+      buf[f] = c >= 0 ? c + P.c0 : -1;
+    }
+    if ((rc = copy_h2d(ctx, m.fc + (size_t)k * m.cap_f + P.f0, buf.data(), nf))) return rc;
+  }
+  for (int k = 0; k < t.vpf; ++k) {
+    buf.resize(nf);
+    for (int f = 0; f < nf; ++f) buf[f] = t.fv[(size_t)f * t.vpf + k] + P.v0;
+    if ((rc = copy_h2d(ctx, m.fv + (size_t)k * m.cap_f + P.f0, buf.data(), nf))) return rc;
+  }
   {
-    std::vector<int32_t> cf(t.cf.size());
-    for (int c = 0; c < t.nc; ++c)
-      for (int k = 0; k < t.fpc; ++k) {
-        const int f = t.cf[(size_t)c * t.fpc + k];
-        cf[(size_t)c * t.fpc + k] = 2 * f + (t.fc[2 * (size_t)f] == c ? 0 : 1);
-      }
-    if ((rc = upload_kmajor(ctx, &m.cf, cf, t.nc, t.fpc))) return rc;
+    std::vector<int32_t> st(nc), cnt(nc), nb(P.nn);
+    for (int c = 0; c < nc; ++c) {
+      st[c] = t.nbr_off[c] + P.n0;
+      cnt[c] = t.nbr_off[c + 1] - t.nbr_off[c];
+    }
+    for (int i = 0; i < P.nn; ++i) nb[i] = t.nbr[i] + P.c0;
+    if ((rc = copy_h2d(ctx, m.nbr_start + P.c0, st.data(), nc))) return rc;
+    if ((rc = copy_h2d(ctx, m.nbr_cnt + P.c0, cnt.data(), nc))) return rc;
+    if ((rc = copy_h2d(ctx, m.nbr + P.n0, nb.data(), P.nn))) return rc;
   }
-  if ((rc = upload_kmajor(ctx, &m.fc, t.fc, t.nf, 2))) return rc;
-  if ((rc = upload_kmajor(ctx, &m.fv, t.fv, t.nf, t.vpf))) return rc;
-  if ((rc = upload(ctx, &m.nbr_off, t.nbr_off))) return rc;
-  if ((rc = upload(ctx, &m.nbr, t.nbr))) return rc;
-  if ((rc = upload(ctx, &m.vc_off, t.vc_off))) return rc;
-  if ((rc = upload(ctx, &m.vc, t.vc))) return rc;
-  for (int d = 0; d < t.dim; ++d) {
-    if ((rc = dev_alloc_n(ctx, &m.ccen[d], t.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &m.fcen[d], t.nf))) return rc;
-    if ((rc = dev_alloc_n(ctx, &m.fnrm[d], t.nf))) return rc;
+  for (int d = 0; d < m.dim; ++d) {
+    std::vector<double> xd(nv);
+    for (int v = 0; v < nv; ++v) xd[v] = mesh.coords[(size_t)v * m.dim + d];
+    if ((rc = copy_h2d(ctx, m.x[d] + P.v0, xd.data(), nv))) return rc;
   }
-  if ((rc = dev_alloc_n(ctx, &m.cvol, t.nc))) return rc;
-  if ((rc = dev_alloc_n(ctx, &m.cminlen, t.nc))) return rc;
-  if ((rc = dev_alloc_n(ctx, &m.farea, t.nf))) return rc;
-  if ((rc = dev_alloc_n(ctx, &ctx->s, 1))) return rc;
-  FSALE_HIP(hipHostMalloc((void **)&ctx->h_s, sizeof(DevScalars), hipHostMallocDefault));
-  std::memset(ctx->h_s, 0, sizeof(DevScalars));
-  FSALE_HIP(hipMemcpy(ctx->s, ctx->h_s, sizeof(DevScalars), hipMemcpyHostToDevice));
-  if ((rc = launch_geometry(ctx, true, true))) return rc;
-
-  if (solver == FSALE_SOLVER_MAIRE) {
-    // boundary tags from the initial face normals: tag d when |n_d| is 1
-    // within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162)
+  if ((rc = launch_geometry(ctx, prob, true, true))) return rc;
+  if (ctx->solver == FSALE_SOLVER_MAIRE) {
+    // vertex -> corner lists and boundary tags from the initial face normals:
+    // tag d when |n_d| is 1 within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162)
+    std::vector<int32_t> st(nv), cnt(nv), vc(t.vc.size());
+    for (int v = 0; v < nv; ++v) {
+      st[v] = t.vc_off[v] + P.c0 * t.vpc;
+      cnt[v] = t.vc_off[v + 1] - t.vc_off[v];
+    }
+    for (size_t i = 0; i < t.vc.size(); ++i) vc[i] = t.vc[i] + P.c0 * t.vpc;
+    if ((rc = copy_h2d(ctx, m.vc_start + P.v0, st.data(), nv))) return rc;
+    if ((rc = copy_h2d(ctx, m.vc_cnt + P.v0, cnt.data(), nv))) return rc;
+    if ((rc = copy_h2d(ctx, m.vc + (size_t)P.c0 * t.vpc, vc.data(), vc.size()))) return rc;
     std::vector<double> fn[2];
     for (int d = 0; d < 2; ++d) {
-      fn[d].resize(t.nf);
-      FSALE_HIP(hipMemcpyAsync(fn[d].data(), m.fnrm[d], t.nf * sizeof(double), hipMemcpyDeviceToHost,
+      fn[d].resize(nf);
+      FSALE_HIP(hipMemcpyAsync(fn[d].data(), m.fnrm[d] + P.f0, nf * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream));
     }
     FSALE_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<uint8_t> ftag(t.nf, 0), vbnd(t.nv, 0);
-    for (int f = 0; f < t.nf; ++f) {
+    std::vector<uint8_t> ftag(nf, 0), vbnd(nv, 0);
+    for (int f = 0; f < nf; ++f) {
       if (t.fc[2 * (size_t)f + 1] >= 0) continue;
-      uint8_t tag = 0x4;  // bit 2: boundary face
+      uint8_t tag = 0x4;   // bit 2: boundary face
       for (int d = 0; d < 2; ++d)
         if (std::fabs(std::fabs(fn[d][f]) - 1.0) <= 1e-14) tag |= (uint8_t)(1u << d);
       ftag[f] = tag;
       for (int i = 0; i < t.vpf; ++i) vbnd[t.fv[(size_t)f * t.vpf + i]] = 1;
     }
     // corner (c,k): low nibble = face of edge (k-1 -> k), high nibble = edge (k -> k+1)
-    std::vector<uint8_t> cw((size_t)t.nc * t.vpc);
-    for (int c = 0; c < t.nc; ++c)
-      for (int k = 0; k < t.vpc; ++k) {
+    for (int k = 0; k < t.vpc; ++k) {
+      std::vector<uint8_t> cw(nc);
+      for (int c = 0; c < nc; ++c) {
         const int km = (k + t.vpc - 1) % t.vpc;
         const int fm = t.cf[(size_t)c * t.fpc + km], fp = t.cf[(size_t)c * t.fpc + k];
-        cw[(size_t)c * t.vpc + k] = (uint8_t)(ftag[fm] | (ftag[fp] << 4));
+        cw[c] = (uint8_t)(ftag[fm] | (ftag[fp] << 4));
       }
-    if ((rc = upload_kmajor(ctx, &m.cwtag, cw, t.nc, t.vpc))) return rc;
-    if ((rc = upload(ctx, &m.vbnd, vbnd))) return rc;
-    if ((rc = maire_alloc(ctx))) return rc;
-  } else {
-    if ((rc = euler_alloc(ctx))) return rc;
+      if ((rc = copy_h2d(ctx, m.cwtag + (size_t)k * m.cap_c + P.c0, cw.data(), nc))) return rc;
+    }
+    if ((rc = copy_h2d(ctx, m.vbnd + P.v0, vbnd.data(), nv))) return rc;
   }
-  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  if ((rc = upload_params(ctx, prob))) return rc;
+  if ((rc = reset_scalars(ctx, prob, 0.0, 0.0))) return rc;
+  *out_prob = prob;
   return FSALE_OK;
 }
 
-// host <-> device field conversions
+// host <-> device field conversions for one problem
 struct FieldView {
-  double *const *ptrs = nullptr;   // component arrays on the device
-  int ncomp = 0;                   // interleaved components on the host
-  int n = 0;                       // entities
-  bool corner = false;             // host corner id c*4+k, device [k][nc]
+  double *ptrs[9] = {};   // component arrays on the device (batch ids)
+  int ncomp = 0;          // interleaved components on the host
+  int n = 0;              // entities of the problem
+  int first = 0;          // first batch id
+  bool corner = false;    // host corner id c*4+k, device [k][cap_c] + c0
   bool writable = true;
 };
 
-int field_view(fsale_ctx *ctx, int field, FieldView &fv, double *tmp[9]) {
+int field_view(fsale_ctx *ctx, int prob, int field, FieldView &fv) {
   DevMesh &m = ctx->m;
+  const Problem &P = ctx->probs[prob];
   const int N = m.dim;
-  auto one = [&](double *p, int n) {
-    tmp[0] = p;
-    fv.ptrs = tmp;
+  auto one = [&](double *p, int n, int first) {
+    fv.ptrs[0] = p;
     fv.ncomp = 1;
     fv.n = n;
+    fv.first = first;
   };
-  auto vec = [&](double *const *p, int ncomp, int n) {
-    for (int i = 0; i < ncomp; ++i) tmp[i] = p[i];
-    fv.ptrs = tmp;
+  auto vec = [&](double *const *p, int ncomp, int n, int first) {
+    for (int i = 0; i < ncomp; ++i) fv.ptrs[i] = p[i];
     fv.ncomp = ncomp;
     fv.n = n;
+    fv.first = first;
   };
   if (ctx->solver == FSALE_SOLVER_EULER) {
     EulerFields &u = ctx->eu;
     switch (field) {
-      case FSALE_F_DENSITY: one(u.d, m.nc); return FSALE_OK;
-      case FSALE_F_VELOCITY: vec(u.v, N, m.nc); return FSALE_OK;
-      case FSALE_F_INTERNAL_ENERGY: one(u.e, m.nc); return FSALE_OK;
-      case FSALE_F_PRESSURE: one(u.p, m.nc); return FSALE_OK;
-      case FSALE_F_TEMPERATURE: one(u.T, m.nc); return FSALE_OK;
-      case FSALE_F_SOUND_SPEED: one(u.a, m.nc); return FSALE_OK;
-      case FSALE_F_GRAD_DENSITY: vec(u.gd, N, m.nc); return FSALE_OK;
-      case FSALE_F_GRAD_VELOCITY: vec(u.gv, N * N, m.nc); return FSALE_OK;
-      case FSALE_F_GRAD_ENERGY: vec(u.ge, N, m.nc); return FSALE_OK;
-      case FSALE_F_GRAD_PRESSURE: vec(u.gp, N, m.nc); return FSALE_OK;
-      case FSALE_F_GRAD_SOUND_SPEED: vec(u.ga, N, m.nc); return FSALE_OK;
-      case FSALE_F_FLUX: vec(u.F, N + 2, m.nf); return FSALE_OK;
+      case FSALE_F_DENSITY: one(u.d, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_VELOCITY: vec(u.v, N, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_INTERNAL_ENERGY: one(u.e, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_PRESSURE: one(u.p, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_TEMPERATURE: one(u.T, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_SOUND_SPEED: one(u.a, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_GRAD_DENSITY: vec(u.gd, N, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_GRAD_VELOCITY: vec(u.gv, N * N, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_GRAD_ENERGY: vec(u.ge, N, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_GRAD_PRESSURE: vec(u.gp, N, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_GRAD_SOUND_SPEED: vec(u.ga, N, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_FLUX:
+        if (!u.F[0]) break;
+        vec(u.F, N + 2, P.nf, P.f0);
+        return FSALE_OK;
       default: break;
     }
   } else {
     MaireFields &u = ctx->ma;
     switch (field) {
-      case FSALE_F_DENSITY: one(u.d, m.nc); return FSALE_OK;
-      case FSALE_F_VELOCITY: vec(u.v, 2, m.nc); return FSALE_OK;
-      case FSALE_F_INTERNAL_ENERGY: one(u.e, m.nc); return FSALE_OK;
-      case FSALE_F_PRESSURE: one(u.p, m.nc); return FSALE_OK;
-      case FSALE_F_TEMPERATURE: one(u.T, m.nc); return FSALE_OK;
-      case FSALE_F_SOUND_SPEED: one(u.a, m.nc); return FSALE_OK;
-      case FSALE_F_MASS: one(u.M, m.nc); return FSALE_OK;
-      case FSALE_F_VOLUME: one(u.V, m.nc); return FSALE_OK;
-      case FSALE_F_GAMMA: one(u.g, m.nc); return FSALE_OK;
-      case FSALE_F_NODE_VELOCITY: vec(u.un, 2, m.nv); return FSALE_OK;
-      case FSALE_F_CORNER_NORMAL: vec(u.npc, 2, m.nc * m.vpc); fv.corner = true; return FSALE_OK;
-      case FSALE_F_CORNER_FORCE: vec(u.Fpc, 2, m.nc * m.vpc); fv.corner = true; return FSALE_OK;
-      case FSALE_F_RESIDUAL: vec(u.dudt, 4, m.nc); return FSALE_OK;
+      case FSALE_F_DENSITY: one(u.d, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_VELOCITY: vec(u.v, 2, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_INTERNAL_ENERGY: one(u.e, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_PRESSURE: one(u.p, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_TEMPERATURE: one(u.T, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_SOUND_SPEED: one(u.a, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_MASS: one(u.M, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_VOLUME: one(u.V, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_GAMMA: one(u.g, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_NODE_VELOCITY: vec(u.un, 2, P.nv, P.v0); return FSALE_OK;
+      case FSALE_F_CORNER_NORMAL:
+        vec(u.npc, 2, P.nc * m.vpc, P.c0);
+        fv.corner = true;
+        return FSALE_OK;
+      case FSALE_F_CORNER_FORCE:
+        vec(u.Fpc, 2, P.nc * m.vpc, P.c0);
+        fv.corner = true;
+        return FSALE_OK;
+      case FSALE_F_RESIDUAL: vec(u.dudt, 4, P.nc, P.c0); return FSALE_OK;
       default: break;
     }
   }
   switch (field) {
-    case FSALE_F_CELL_CENTROID: vec(m.ccen, N, m.nc); fv.writable = false; return FSALE_OK;
-    case FSALE_F_CELL_VOLUME: one(m.cvol, m.nc); fv.writable = false; return FSALE_OK;
-    case FSALE_F_CELL_MIN_LENGTH: one(m.cminlen, m.nc); fv.writable = false; return FSALE_OK;
-    case FSALE_F_FACE_CENTROID: vec(m.fcen, N, m.nf); fv.writable = false; return FSALE_OK;
-    case FSALE_F_FACE_NORMAL: vec(m.fnrm, N, m.nf); fv.writable = false; return FSALE_OK;
-    case FSALE_F_FACE_AREA: one(m.farea, m.nf); fv.writable = false; return FSALE_OK;
-    case FSALE_F_COORDS: vec(m.x, N, m.nv); return FSALE_OK;
+    case FSALE_F_CELL_CENTROID: vec(m.ccen, N, P.nc, P.c0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_CELL_VOLUME: one(m.cvol, P.nc, P.c0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_CELL_MIN_LENGTH: one(m.cminlen, P.nc, P.c0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_FACE_CENTROID: vec(m.fcen, N, P.nf, P.f0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_FACE_NORMAL: vec(m.fnrm, N, P.nf, P.f0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_FACE_AREA: one(m.farea, P.nf, P.f0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_COORDS: vec(m.x, N, P.nv, P.v0); return FSALE_OK;
     default: break;
   }
   set_message("field not available for this solver");
   return FSALE_ERR_ARG;
 }
 
-// corner reorder: host index c*vpc+k  <->  device index k*nc+c
-inline size_t corner_dev_index(const DevMesh &m, size_t host_i) {
-  const size_t c = host_i / m.vpc, k = host_i % m.vpc;
-  return k * m.nc + c;
+bool valid_prob(const fsale_ctx *ctx, int prob) {
+  return ctx && prob >= 0 && prob < ctx->max_problems && ctx->probs[prob].live;
 }
 
 }  // namespace
+
+int get_problem_field(fsale_ctx *ctx, int prob, int field, double *host_out) {
+  if (!valid_prob(ctx, prob) || !host_out) return FSALE_ERR_ARG;
+  FieldView fv;
+  int rc = field_view(ctx, prob, field, fv);
+  if (rc) return rc;
+  const DevMesh &m = ctx->m;
+  std::vector<double> buf;
+  for (int k = 0; k < fv.ncomp; ++k) {
+    if (fv.corner) {
+      // corner fields: device [kk][cap_c] rows, host c*vpc + kk
+      const int nc = fv.n / m.vpc;
+      buf.resize(nc);
+      for (int kk = 0; kk < m.vpc; ++kk) {
+        FSALE_HIP(hipMemcpyAsync(buf.data(), fv.ptrs[k] + (size_t)kk * m.cap_c + fv.first, nc * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        FSALE_HIP(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < nc; ++c) host_out[((size_t)c * m.vpc + kk) * fv.ncomp + k] = buf[c];
+      }
+    } else {
+      buf.resize(fv.n);
+      FSALE_HIP(hipMemcpyAsync(buf.data(), fv.ptrs[k] + fv.first, fv.n * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+      FSALE_HIP(hipStreamSynchronize(ctx->stream));
+      for (int i = 0; i < fv.n; ++i) host_out[(size_t)i * fv.ncomp + k] = buf[i];
+    }
+  }
+  return check_device_error(ctx, prob);
+}
+
+int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in) {
+  if (!valid_prob(ctx, prob) || !host_in) return FSALE_ERR_ARG;
+  FieldView fv;
+  int rc = field_view(ctx, prob, field, fv);
+  if (rc) return rc;
+  if (!fv.writable) {
+    set_message("geometry fields are derived from the coordinates");
+    return FSALE_ERR_ARG;
+  }
+  const DevMesh &m = ctx->m;
+  std::vector<double> buf;
+  for (int k = 0; k < fv.ncomp; ++k) {
+    if (fv.corner) {
+      const int nc = fv.n / m.vpc;
+      buf.resize(nc);
+      for (int kk = 0; kk < m.vpc; ++kk) {
+        for (int c = 0; c < nc; ++c) buf[c] = host_in[((size_t)c * m.vpc + kk) * fv.ncomp + k];
+        FSALE_HIP(hipMemcpyAsync(fv.ptrs[k] + (size_t)kk * m.cap_c + fv.first, buf.data(), nc * sizeof(double),
+                                 hipMemcpyHostToDevice, ctx->stream));
+        FSALE_HIP(hipStreamSynchronize(ctx->stream));
+      }
+    } else {
+      buf.resize(fv.n);
+      for (int i = 0; i < fv.n; ++i) buf[i] = host_in[(size_t)i * fv.ncomp + k];
+      FSALE_HIP(hipMemcpyAsync(fv.ptrs[k] + fv.first, buf.data(), fv.n * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+      FSALE_HIP(hipStreamSynchronize(ctx->stream));
+    }
+  }
+  ctx->probs[prob].partials_valid = false;
+  if (field == FSALE_F_COORDS) return launch_geometry(ctx, prob, true, true);
+  return FSALE_OK;
+}
+
+// d, v, p of one problem from host arrays (v interleaved)
+int set_problem_state(fsale_ctx *ctx, int prob, const double *d, const double *v, const double *p) {
+  int rc;
+  if ((rc = set_problem_field(ctx, prob, FSALE_F_DENSITY, d))) return rc;
+  if ((rc = set_problem_field(ctx, prob, FSALE_F_VELOCITY, v))) return rc;
+  return set_problem_field(ctx, prob, FSALE_F_PRESSURE, p);
+}
+
+int euler_initial_state(fsale_ctx *ctx, int prob);
+int maire_initial_state(fsale_ctx *ctx, int prob);
+
+// Driver steps of every live problem; chunks of up to 10 steps are replayed
+// from a captured graph (the graph reads tiles, parameters and scalars from
+// device memory, so it stays valid while problems come and go).
+int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
+  int rc;
+  if ((rc = sync_tiles(ctx))) return rc;
+  if (ctx->h_ctiles.empty() || nsteps <= 0) return FSALE_OK;
+  if (ctx->solver == FSALE_SOLVER_EULER) {
+    bool need = false;
+    for (auto &P : ctx->probs)
+      if (P.live && !P.partials_valid) need = true;
+    if (need) {
+      if ((rc = euler_dt_partials(ctx))) return rc;
+      for (auto &P : ctx->probs)
+        if (P.live) P.partials_valid = true;
+    }
+  }
+  auto step = [&]() { return ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx); };
+  if (!use_graph || nsteps < 2 || ctx->profiling) {
+    for (int n = 0; n < nsteps; ++n)
+      if ((rc = step())) return rc;
+    return FSALE_OK;
+  }
+  const int chunk = nsteps < 10 ? nsteps : 10;
+  const std::vector<double> key = {(double)ctx->solver, (double)chunk, (double)persistent_grid(ctx)};
+  if (!ctx->graph.exec || ctx->graph.key != key) {
+    if (ctx->graph.exec) {
+      (void)hipGraphExecDestroy(ctx->graph.exec);
+      ctx->graph.exec = nullptr;
+    }
+    hipGraph_t graph;
+    FSALE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int n = 0; n < chunk; ++n) {
+      rc = step();
+      if (rc) {
+        (void)hipStreamEndCapture(ctx->stream, &graph);
+        return rc;
+      }
+    }
+    FSALE_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    FSALE_HIP(hipGraphInstantiate(&ctx->graph.exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    ctx->graph.key = key;
+  }
+  int n = 0;
+  for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(ctx->graph.exec, ctx->stream));
+  for (; n < nsteps; ++n)
+    if ((rc = step())) return rc;
+  return FSALE_OK;
+}
+
 }  // namespace fsale
 
 using namespace fsale;
@@ -278,7 +668,7 @@ const char *fsale_status_string(int status) {
     case FSALE_ERR_ARG: return "bad argument";
     case FSALE_ERR_MESH: return "unsupported or invalid mesh";
     case FSALE_ERR_HIP: return "HIP runtime error";
-    case FSALE_ERR_OOM: return "out of device memory";
+    case FSALE_ERR_OOM: return "out of device memory or batch capacity";
     case FSALE_ERR_NEG_STATE: return "negative density or internal energy";
     case FSALE_ERR_DT: return "infinite delta t";
     case FSALE_ERR_STATE: return "call out of order";
@@ -289,8 +679,10 @@ const char *fsale_status_string(int status) {
 
 const char *fsale_last_message(void) { return g_message.c_str(); }
 
-int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out) {
-  if (!mesh || !out || (solver != FSALE_SOLVER_EULER && solver != FSALE_SOLVER_MAIRE)) return FSALE_ERR_ARG;
+int fsale_batch_create(int solver, int dim, int verts_per_cell, int32_t max_problems, int64_t cell_capacity,
+                       int64_t face_capacity, int64_t vertex_capacity, int64_t neighbour_capacity,
+                       fsale_ctx **out) {
+  if (!out || (solver != FSALE_SOLVER_EULER && solver != FSALE_SOLVER_MAIRE)) return FSALE_ERR_ARG;
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -298,15 +690,51 @@ int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out) {
     return FSALE_ERR_NO_DEVICE;
   }
   fsale_ctx *ctx = new fsale_ctx();
-  ctx->solver = solver;
   (void)hipGetDevice(&ctx->device);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     set_message("hipStreamCreate failed");
     return FSALE_ERR_HIP;
   }
-  int rc = create_impl(mesh, solver, ctx);
+  int rc = alloc_batch(ctx, solver, dim, verts_per_cell, max_problems, cell_capacity, face_capacity,
+                       vertex_capacity, neighbour_capacity);
   if (rc) {
+    fsale_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return FSALE_OK;
+}
+
+int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out) {
+  if (!mesh || !out) return FSALE_ERR_ARG;
+  *out = nullptr;
+  // exact capacity from the mesh's own connectivity
+  auto t = std::make_shared<HostTopology>();
+  std::string err;
+  int rc = build_topology(*mesh, *t, err);
+  if (rc) {
+    set_message(err);
+    return rc;
+  }
+  fsale_ctx *ctx = nullptr;
+  if ((rc = fsale_batch_create(solver, mesh->dim, mesh->verts_per_cell, 1, t->nc, t->nf, t->nv,
+                               (int64_t)t->nbr.size(), &ctx)))
+    return rc;
+  ctx->single = true;
+  if (solver == FSALE_SOLVER_EULER) {
+    // the task-level evaluate_fluxes keeps a face flux array
+    for (int i = 0; i < mesh->dim + 2; ++i)
+      if ((rc = dev_alloc_n(ctx, &ctx->eu.F[i], t->nf))) {
+        fsale_ctx_destroy(ctx);
+        return rc;
+      }
+  }
+  ctx->topo_cache[mesh_key(*mesh)] = t;
+  int prob;
+  if ((rc = place_mesh(ctx, *mesh, &prob))) {
     fsale_ctx_destroy(ctx);
     return rc;
   }
@@ -331,23 +759,28 @@ int fsale_ctx_destroy(fsale_ctx *ctx) {
 
 int fsale_sync(fsale_ctx *ctx) {
   if (!ctx) return FSALE_ERR_ARG;
-  return check_device_error(ctx);
+  int rc = read_scalars(ctx);
+  if (rc) return rc;
+  for (int p = 0; p < ctx->max_problems; ++p)
+    if (ctx->probs[p].live && ctx->h_s[p].err_code) return check_device_error(ctx, p);
+  return FSALE_OK;
 }
 
 int fsale_mesh_counts(const fsale_ctx *ctx, int32_t *counts) {
-  if (!ctx || !counts) return FSALE_ERR_ARG;
-  counts[0] = ctx->m.nc;
-  counts[1] = ctx->m.nv;
-  counts[2] = ctx->m.nf;
-  counts[3] = ctx->m.nnbr;
+  if (!valid_prob(ctx, 0) || !counts) return FSALE_ERR_ARG;
+  const Problem &P = ctx->probs[0];
+  counts[0] = P.nc;
+  counts[1] = P.nv;
+  counts[2] = P.nf;
+  counts[3] = P.nn;
   counts[4] = ctx->m.dim;
   return FSALE_OK;
 }
 
 int fsale_get_topology(fsale_ctx *ctx, int32_t *face_cells, int32_t *cell_faces, int32_t *nbr_off,
                        int32_t *nbr) {
-  if (!ctx) return FSALE_ERR_ARG;
-  const HostTopology &t = ctx->topo;
+  if (!valid_prob(ctx, 0)) return FSALE_ERR_ARG;
+  const HostTopology &t = *ctx->probs[0].topo;
   if (face_cells) std::memcpy(face_cells, t.fc.data(), t.fc.size() * sizeof(int32_t));
   if (cell_faces) std::memcpy(cell_faces, t.cf.data(), t.cf.size() * sizeof(int32_t));
   if (nbr_off) std::memcpy(nbr_off, t.nbr_off.data(), t.nbr_off.size() * sizeof(int32_t));
@@ -378,67 +811,39 @@ int fsale_mesh_connectivity(const fsale_mesh_desc *mesh, int32_t *counts, int32_
 }
 
 int fsale_get_field(fsale_ctx *ctx, int field, double *host_out) {
-  if (!ctx || !host_out) return FSALE_ERR_ARG;
-  FieldView fv;
-  double *tmp[9];
-  int rc = field_view(ctx, field, fv, tmp);
-  if (rc) return rc;
-  std::vector<double> buf((size_t)fv.n);
-  for (int k = 0; k < fv.ncomp; ++k) {
-    FSALE_HIP(hipMemcpyAsync(buf.data(), fv.ptrs[k], buf.size() * sizeof(double), hipMemcpyDeviceToHost,
-                             ctx->stream));
-    FSALE_HIP(hipStreamSynchronize(ctx->stream));
-    for (size_t i = 0; i < buf.size(); ++i) {
-      const size_t src = fv.corner ? corner_dev_index(ctx->m, i) : i;
-      host_out[i * fv.ncomp + k] = buf[src];
-    }
-  }
-  return check_device_error(ctx);
+  return get_problem_field(ctx, 0, field, host_out);
 }
 
 int fsale_set_field(fsale_ctx *ctx, int field, const double *host_in) {
-  if (!ctx || !host_in) return FSALE_ERR_ARG;
-  FieldView fv;
-  double *tmp[9];
-  int rc = field_view(ctx, field, fv, tmp);
-  if (rc) return rc;
-  if (!fv.writable) {
-    set_message("geometry fields are derived from the coordinates");
-    return FSALE_ERR_ARG;
-  }
-  std::vector<double> buf((size_t)fv.n);
-  for (int k = 0; k < fv.ncomp; ++k) {
-    for (size_t i = 0; i < buf.size(); ++i) {
-      const size_t dst = fv.corner ? corner_dev_index(ctx->m, i) : i;
-      buf[dst] = host_in[i * fv.ncomp + k];
-    }
-    FSALE_HIP(hipMemcpyAsync(fv.ptrs[k], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice,
-                             ctx->stream));
-    FSALE_HIP(hipStreamSynchronize(ctx->stream));
-  }
-  if (ctx->solver == FSALE_SOLVER_EULER) ctx->eu.partials_valid = false;
-  if (field == FSALE_F_COORDS) return launch_geometry(ctx, true, true);
-  return FSALE_OK;
+  return set_problem_field(ctx, 0, field, host_in);
 }
 
 int fsale_get_time(fsale_ctx *ctx, double *soln_time, double *time_step, int64_t *steps) {
-  if (!ctx) return FSALE_ERR_ARG;
+  if (!valid_prob(ctx, 0)) return FSALE_ERR_ARG;
   int rc = read_scalars(ctx);
   if (rc) return rc;
-  if (soln_time) *soln_time = ctx->h_s->t;
-  if (time_step) *time_step = ctx->h_s->dt;
-  if (steps) *steps = ctx->h_s->steps;
+  if (soln_time) *soln_time = ctx->h_s[0].t;
+  if (time_step) *time_step = ctx->h_s[0].dt;
+  if (steps) *steps = ctx->h_s[0].steps;
   return FSALE_OK;
 }
 
 int fsale_set_time(fsale_ctx *ctx, double soln_time, double time_step) {
-  if (!ctx) return FSALE_ERR_ARG;
+  if (!valid_prob(ctx, 0)) return FSALE_ERR_ARG;
   int rc = read_scalars(ctx);
   if (rc) return rc;
-  ctx->h_s->t = soln_time;
-  ctx->h_s->dt = time_step;
+  ctx->h_s[0].t = soln_time;
+  ctx->h_s[0].dt = time_step;
   FSALE_HIP(hipMemcpyAsync(ctx->s, ctx->h_s, sizeof(DevScalars), hipMemcpyHostToDevice, ctx->stream));
   FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  return FSALE_OK;
+}
+
+int fsale_error_cell(fsale_ctx *ctx, int64_t *cell) {
+  if (!valid_prob(ctx, 0) || !cell) return FSALE_ERR_ARG;
+  int rc = read_scalars(ctx);
+  if (rc) return rc;
+  *cell = ctx->h_s[0].err_code ? ctx->h_s[0].err_cell : -1;
   return FSALE_OK;
 }
 
@@ -482,11 +887,146 @@ int fsale_reset_profile(fsale_ctx *ctx) {
   return FSALE_OK;
 }
 
-int fsale_error_cell(fsale_ctx *ctx, int64_t *cell) {
-  if (!ctx || !cell) return FSALE_ERR_ARG;
+// ----------------------------------------------------------------- batches
+
+int fsale_batch_add_euler(fsale_ctx *ctx, const fsale_mesh_desc *mesh, fsale_eos eos,
+                          const fsale_euler_params *prm, const double *d, const double *v, const double *p,
+                          int32_t *problem) {
+  if (!ctx || ctx->single || ctx->solver != FSALE_SOLVER_EULER || !mesh || !prm || !d || !v || !p || !problem)
+    return FSALE_ERR_ARG;
+  if (!(eos.gamma > 1) || !(eos.cv > 0) || prm->flux_kind < 0 || (prm->flux_kind & 0xf) > 2 ||
+      (prm->flux_kind >> 4) > 1)
+    return FSALE_ERR_ARG;
+  int prob = -1, rc;
+  if ((rc = place_mesh(ctx, *mesh, &prob))) {
+    if (prob >= 0 && prob < ctx->max_problems && ctx->probs[prob].live) fsale_batch_remove(ctx, prob);
+    return rc;
+  }
+  Problem &P = ctx->probs[prob];
+  P.params.gamma = eos.gamma;
+  P.params.cv = eos.cv;
+  P.params.cfl = prm->cfl;
+  P.params.final_time = prm->final_time;
+  P.params.kind = prm->flux_kind;
+  P.params.first_order = prm->first_order != 0;
+  P.params.stop_at_final = prm->stop_at_final != 0;
+  P.params.max_steps = prm->max_steps;
+  if ((rc = upload_params(ctx, prob)) || (rc = set_problem_state(ctx, prob, d, v, p)) ||
+      (rc = euler_initial_state(ctx, prob)) || (rc = reset_scalars(ctx, prob, 0.0, 0.0))) {
+    fsale_batch_remove(ctx, prob);
+    return rc;
+  }
+  P.have_state = true;
+  P.partials_valid = false;
+  *problem = prob;
+  return FSALE_OK;
+}
+
+int fsale_batch_add_maire(fsale_ctx *ctx, const fsale_mesh_desc *mesh, double cv, const double *gamma,
+                          const fsale_maire_params *prm, const double *d, const double *v, const double *p,
+                          int32_t *problem) {
+  if (!ctx || ctx->single || ctx->solver != FSALE_SOLVER_MAIRE || !mesh || !prm || !gamma || !d || !v || !p ||
+      !problem || !(cv > 0))
+    return FSALE_ERR_ARG;
+  for (int c = 0; c < mesh->num_cells; ++c)
+    if (!(gamma[c] > 1)) return FSALE_ERR_ARG;
+  int prob = -1, rc;
+  if ((rc = place_mesh(ctx, *mesh, &prob))) {
+    if (prob >= 0 && prob < ctx->max_problems && ctx->probs[prob].live) fsale_batch_remove(ctx, prob);
+    return rc;
+  }
+  Problem &P = ctx->probs[prob];
+  P.params.cv = cv;
+  P.params.cfl_acc = prm->cfl_accoustic;
+  P.params.cfl_vol = prm->cfl_volume;
+  P.params.cfl_growth = prm->cfl_growth;
+  P.params.final_time = prm->final_time;
+  P.params.max_steps = prm->max_steps;
+  if ((rc = upload_params(ctx, prob)) || (rc = set_problem_field(ctx, prob, FSALE_F_GAMMA, gamma)) ||
+      (rc = set_problem_state(ctx, prob, d, v, p)) || (rc = maire_initial_state(ctx, prob)) ||
+      (rc = reset_scalars(ctx, prob, 0.0, prm->initial_time_step))) {
+    fsale_batch_remove(ctx, prob);
+    return rc;
+  }
+  P.have_state = true;
+  *problem = prob;
+  return FSALE_OK;
+}
+
+int fsale_batch_remove(fsale_ctx *ctx, int32_t problem) {
+  if (!ctx || ctx->single || !valid_prob(ctx, problem)) return FSALE_ERR_ARG;
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  Problem &P = ctx->probs[problem];
+  ctx->ra_c.release(P.c0, P.nc);
+  ctx->ra_f.release(P.f0, P.nf);
+  ctx->ra_v.release(P.v0, P.nv);
+  ctx->ra_n.release(P.n0, P.nn);
+  ctx->ra_ct.release(P.ct0, P.nct);
+  ctx->ra_vt.release(P.vt0, P.nvt);
+  P = Problem{};
+  ctx->tiles_dirty = true;
+  // a dead problem owns no cells: the per-problem kernels skip it
+  ProbParams zero;
+  std::memset(&zero, 0, sizeof(zero));
+  FSALE_HIP(hipMemcpyAsync(ctx->d_params + problem, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  return FSALE_OK;
+}
+
+int fsale_batch_step(fsale_ctx *ctx, int nsteps, int use_graph) {
+  if (!ctx || nsteps < 0) return FSALE_ERR_ARG;
+  for (auto &P : ctx->probs)
+    if (P.live && !P.have_state) return FSALE_ERR_STATE;
+  return batch_steps(ctx, nsteps, use_graph);
+}
+
+int fsale_batch_status(fsale_ctx *ctx, int32_t max_out, fsale_problem_status *out, int32_t *count) {
+  if (!ctx || !out || !count || max_out < 0) return FSALE_ERR_ARG;
   int rc = read_scalars(ctx);
   if (rc) return rc;
-  *cell = ctx->h_s->err_code ? ctx->h_s->err_cell : -1;
+  int n = 0;
+  for (int p = 0; p < ctx->max_problems && n < max_out; ++p) {
+    const Problem &P = ctx->probs[p];
+    if (!P.live) continue;
+    const DevScalars &S = ctx->h_s[p];
+    fsale_problem_status st;
+    st.problem = p;
+    st.error = S.err_code;
+    st.steps = S.steps;
+    st.time = S.t;
+    st.time_step = S.dt;
+    bool done = S.err_code != 0 || S.steps >= P.params.max_steps;
+    if (ctx->solver == FSALE_SOLVER_MAIRE || P.params.stop_at_final) done = done || !(S.t < P.params.final_time);
+    st.done = done ? 1 : 0;
+    out[n++] = st;
+  }
+  *count = n;
+  return FSALE_OK;
+}
+
+int fsale_batch_get_field(fsale_ctx *ctx, int32_t problem, int field, double *host_out) {
+  return get_problem_field(ctx, problem, field, host_out);
+}
+
+int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_faces, int64_t *free_vertices,
+                         int64_t *free_neighbours, int32_t *free_problems) {
+  if (!ctx) return FSALE_ERR_ARG;
+  if (free_cells) *free_cells = ctx->ra_c.available();
+  if (free_faces) *free_faces = ctx->ra_f.available();
+  if (free_vertices) *free_vertices = ctx->ra_v.available();
+  if (free_neighbours) *free_neighbours = ctx->ra_n.available();
+  if (free_problems) {
+    int n = 0;
+    for (auto &P : ctx->probs)
+      if (!P.live) ++n;
+    *free_problems = n;
+  }
+  return FSALE_OK;
+}
+
+int fsale_launch_shape(fsale_ctx *ctx, int32_t blocks_per_cu) {
+  if (!ctx || blocks_per_cu < 1 || blocks_per_cu > 32) return FSALE_ERR_ARG;
+  ctx->shape.blocks_per_cu = blocks_per_cu;
   return FSALE_OK;
 }
 

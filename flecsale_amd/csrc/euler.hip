@@ -7,11 +7,11 @@
 //   k_eul_fluxupd   evaluate_fluxes + apply_update (+ the next step's dt^-1
 //                   partials), one work-item per cell: the driver loop's path
 //
-// SoA FP64 state, k-major fixed-degree connectivity so neighbouring lanes read
-// neighbouring words, and an XCD-aware block order: the hardware deals blocks
-// round-robin to the 8 XCDs, so logical block L is launched as physical block
-// (L % per) * 8 + L / per, giving each XCD (and its L2) one contiguous range
-// of cells whose neighbour gathers then hit that L2.
+// Every kernel but the task-level face kernel is a persistent loop over the
+// batch's tile list (batch_dev.h), so one launch advances every resident
+// problem; problem scalars (EOS, CFL, dt, time, flags) come from per-problem
+// records.  SoA FP64 state; k-major fixed-degree connectivity so
+// neighbouring lanes read neighbouring words.
 //
 // k_eul_fluxupd recomputes, for each cell, the numerical flux of each of its
 // faces exactly as k_eul_flux does (same left/right states, same operation
@@ -19,114 +19,95 @@
 // sides and the update gathers them in the reference's mesh.faces(c) order.
 // In 2D the face centroid / normal / length come from the cell's own
 // vertices (bit-identical to the stored face geometry: reversing an edge
-// negates it exactly).  The flux array never goes through HBM.
+// negates it exactly).  The flux array never goes through HBM.  Neighbours
+// must read the old state, so the state ping-pongs between two buffers.
 //
-// The time step: each block writes the max of dt^-1 over its cells to a
-// partials array (no atomics); one block folds the partials into dt.  In the
-// driver loop the partials come from the second half-step's update, which
-// already holds the new state.
-#include "internal.h"
-#include "kernels_common.h"
+// The time step: each tile writes the max of dt^-1 over its cells to a
+// partials slot (no atomics); one block per problem folds them into dt.  In
+// the driver loop the partials come from the second half-step's update,
+// which already holds the new state.
+#include "batch_dev.h"
 
 namespace fsale {
 namespace {
 
-// physical -> logical block so that every XCD owns one contiguous range
-__device__ inline int logical_block() {
-  const int per = gridDim.x >> 3;
-  return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-}
-
-// blocks launched for n entities: a multiple of 8 (one slice per XCD)
-inline int grid8(int n) {
-  const int b = (n + kBlock - 1) / kBlock;
-  return ((b + 7) / 8) * 8;
-}
-
-// block max of dt^-1 -> partials[L]
-__device__ inline void block_max_to(double v, double *partials, int L) {
-  __shared__ double red[kBlock / 64];
-  v = wave_max(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) b = smax(red[w], b);
-    partials[L] = b;
-  }
-}
-
 // ------------------------------------------------------------ state init
-__global__ void __launch_bounds__(kBlock) k_eul_ic(int nc, EulerFields u, double g, double cv) {
-  const int c = logical_block() * kBlock + threadIdx.x;
-  if (c >= nc) return;
+__global__ void __launch_bounds__(kBlock) k_eul_ic(EulerFields u, const ProbParams *pp, int prob) {
+  const ProbParams &P = pp[prob];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.cell_count) return;
+  const int c = P.cell_first + i;
   // update_state_from_pressure euler_eqns.h:333-350
-  const double ie = eos_e_dp(g, u.d[c], u.p[c]);
+  const double ie = eos_e_dp(P.gamma, u.d[c], u.p[c]);
   u.e[c] = ie;
-  u.T[c] = eos_T_de(cv, ie);
-  u.a[c] = eos_a_de(g, ie);
+  u.T[c] = eos_T_de(P.cv, ie);
+  u.a[c] = eos_a_de(P.gamma, ie);
 }
 
 // ----------------------------------------------------------- time step
 template <int N, int FPC>
-__global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerFields u, double *partials) {
-  const int L = logical_block();
-  const int c = L * kBlock + threadIdx.x;
-  double dt_inv = 0;
-  if (c < m.nc) {
-    double vel[N];
-    for (int d = 0; d < N; ++d) vel[d] = u.v[d][c];
-    const double a = u.a[c], vol = m.cvol[c];
+__global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerFields u, Batch b) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    double dt_inv = 0;
+    if ((int)threadIdx.x < tile.count) {
+      const int c = tile.first + threadIdx.x;
+      double vel[N];
+      for (int d = 0; d < N; ++d) vel[d] = u.v[d][c];
+      const double a = u.a[c], vol = m.cvol[c];
 #pragma unroll
-    for (int k = 0; k < FPC; ++k) {
-      const int f = m.cf[(size_t)k * m.nc + c] >> 1;
-      double n[N];
-      for (int d = 0; d < N; ++d) n[d] = m.fnrm[d][f];
-      const double delta_x = vol / m.farea[f];
-      const double dti = (a + fabs(dotN<N>(vel, n))) / delta_x;
-      dt_inv = smax(dti, dt_inv);
+      for (int k = 0; k < FPC; ++k) {
+        const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
+        double n[N];
+        for (int d = 0; d < N; ++d) n[d] = m.fnrm[d][f];
+        const double delta_x = vol / m.farea[f];
+        const double dti = (a + fabs(dotN<N>(vel, n))) / delta_x;
+        dt_inv = smax(dti, dt_inv);
+      }
     }
-  }
-  block_max_to(dt_inv, partials, L);
+    const double bm = block_max(dt_inv);
+    if (threadIdx.x == 0) u.partials[tile.slot] = bm;
+  });
 }
 
-// dt = min(CFL / max dt^-1, t_final - t); also opens or closes the step
-__global__ void __launch_bounds__(kBlock) k_eul_dt_finalize(const double *partials, int nparts, DevScalars *s,
-                                                           double cfl, double final_time, int stop_at_final) {
-  __shared__ double red[kBlock / 64];
-  double v = 0;
-  for (int i = threadIdx.x; i < nparts; i += kBlock) v = smax(partials[i], v);
-  v = wave_max(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double mx = red[0];
-  for (int w = 1; w < kBlock / 64; ++w) mx = smax(red[w], mx);
-  const double t = s->t;
-  int active = (s->err_code == 0);
-  if (stop_at_final && !(t < final_time)) active = 0;
-  if (active) {
-    if (mx <= 0) {   // apps/hydro/tasks.h:366-367
-      s->err_code = FSALE_ERR_DT;
-      active = 0;
-    } else {
-      double ts = 1 / mx;
-      ts *= cfl;
-      ts = smin(ts, final_time - t);
-      s->dt = ts;
-      s->steps = s->steps + 1;
+// Per problem: dt = min(CFL / max dt^-1, t_final - t) and the step flag.
+// driver != 0: the step loop (step budget, optional stop at t_final);
+// driver == 0: the task-level evaluate_time_step.
+__global__ void __launch_bounds__(kBlock) k_eul_dt_finalize(Batch b, const double *partials, int nprob,
+                                                           int driver) {
+  for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
+    const ProbParams &P = b.pp[p];
+    if (P.cell_count == 0) continue;
+    double v = 0;
+    for (int i = threadIdx.x; i < P.tile_count; i += kBlock) v = smax(partials[P.tile_first + i], v);
+    const double mx = block_max(v);
+    if (threadIdx.x != 0) continue;
+    DevScalars &S = b.s[p];
+    const double t = S.t;
+    int active = (S.err_code == 0);
+    if (driver) {
+      if (!(S.steps < P.max_steps)) active = 0;
+      if (P.stop_at_final && !(t < P.final_time)) active = 0;
     }
+    if (active) {
+      if (mx <= 0) {   // apps/hydro/tasks.h:366-367
+        S.err_code = FSALE_ERR_DT;
+        active = 0;
+      } else {
+        double ts = 1 / mx;
+        ts *= P.cfl;
+        ts = smin(ts, P.final_time - t);
+        S.dt = ts;
+        if (driver) S.steps = S.steps + 1;
+      }
+    }
+    S.active = active;
   }
-  s->active = active;
 }
 
 // ----------------------------------------------------------- gradients
 template <int N, int VPC>
-__global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, const DevScalars *s,
-                                                     int first_order) {
+__device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0, int first_order) {
   constexpr int NV = N + 4;
-  const int c0 = logical_block() * kBlock + threadIdx.x;
-  if (c0 >= m.nc || !s->active) return;
   const double tol = 1.e-9;
   double u0[NV], umin[NV], umax[NV], dudx[NV][N], dxdx[N][N], x0[N];
   for (int d = 0; d < N; ++d) x0[d] = m.ccen[d][c0];
@@ -145,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
   }
   for (int r = 0; r < N; ++r)
     for (int q = 0; q < N; ++q) dxdx[r][q] = 0;
-  const int q0 = m.nbr_off[c0], q1 = m.nbr_off[c0 + 1];
+  const int q0 = m.nbr_start[c0], q1 = q0 + m.nbr_cnt[c0];
   for (int q = q0; q < q1; ++q) {
     const int c1 = m.nbr[q];
     double u1[NV], dx[N];
@@ -216,7 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
   }
 #pragma unroll
   for (int lv = 0; lv < VPC; ++lv) {
-    const int vid = m.cv[(size_t)lv * m.nc + c0];
+    const int vid = m.cv[(size_t)lv * m.cap_c + c0];
     double xv[N];
     for (int d = 0; d < N; ++d) xv[d] = m.x[d][vid];
     double u1[NV];
@@ -233,15 +214,12 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
         dneg[j] = fmin(dneg[j], du);
     }
   }
-  double phi[NV];
   for (int j = 0; j < NV; ++j) {
-    phi[j] = 1;
-    if (dpos[j] > tol) phi[j] = smin(phi[j], (umax[j] - u0[j]) / dpos[j]);
-    if (dneg[j] < -tol) phi[j] = smin(phi[j], (umin[j] - u0[j]) / dneg[j]);
-  }
-  for (int j = 0; j < NV; ++j) {
-    phi[j] = smax(phi[j], 0);
-    for (int d = 0; d < N; ++d) gu[j][d] = first_order ? 0.0 : gu[j][d] * phi[j];
+    double phi = 1;
+    if (dpos[j] > tol) phi = smin(phi, (umax[j] - u0[j]) / dpos[j]);
+    if (dneg[j] < -tol) phi = smin(phi, (umin[j] - u0[j]) / dneg[j]);
+    phi = smax(phi, 0);
+    for (int d = 0; d < N; ++d) gu[j][d] = first_order ? 0.0 : gu[j][d] * phi;
   }
   for (int d = 0; d < N; ++d) {
     u.gd[d][c0] = gu[0][d];
@@ -250,6 +228,14 @@ __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, c
     u.gp[d][c0] = gu[2 + N][d];
     u.ga[d][c0] = gu[3 + N][d];
   }
+}
+
+template <int N, int VPC>
+__global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, Batch b) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    if ((int)threadIdx.x < tile.count) grad_cell<N, VPC>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
+  });
 }
 
 // -------------------------------------------------------------- fluxes
@@ -313,10 +299,14 @@ __device__ inline void face_flux_area(int kind, const CellData<N> &q0, const Cel
   for (int i = 0; i < N + 2; ++i) out[i] *= area;
 }
 
+// task-level evaluate_fluxes over the faces of one problem
 template <int N>
-__global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, const DevScalars *s, int kind) {
-  const int f = logical_block() * kBlock + threadIdx.x;
-  if (f >= m.nf || !s->active) return;
+__global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, Batch b, int prob, int f0,
+                                                     int nf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf || !b.s[prob].active) return;
+  const int f = f0 + i;
+  const int kind = b.pp[prob].kind;
   double fx[N], n[N], out[N + 2];
   for (int d = 0; d < N; ++d) {
     fx[d] = m.fcen[d][f];
@@ -324,18 +314,18 @@ __global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, c
   }
   CellData<N> q0, q1;
   load_cell<N>(m, u, m.fc[f], q0);
-  const int c1 = m.fc[(size_t)m.nf + f];
+  const int c1 = m.fc[(size_t)m.cap_f + f];
   load_cell<N>(m, u, c1 >= 0 ? c1 : m.fc[f], q1);
   face_flux_area<N>(kind, q0, q1, c1 >= 0, fx, n, m.farea[f], out);
-  for (int i = 0; i < N + 2; ++i) u.F[i][f] = out[i];
+  for (int j = 0; j < N + 2; ++j) u.F[j][f] = out[j];
 }
 
 // -------------------------------------------------------------- update
 // update_state_from_flux euler_eqns.h:383-410 + update_state_from_energy
 // :359-375 for one cell; returns false on negative density / energy.
 template <int N>
-__device__ inline bool update_cell(const EulerFields &in, EulerFields &u, int c, double *du, double sc, double g,
-                                   double cv, double *vel_out, double &a_out) {
+__device__ inline bool update_cell(const EulerFields &in, const EulerFields &out, int c, double *du, double sc,
+                                   double g, double cv, double *vel_out, double &a_out) {
   for (int i = 0; i < N + 2; ++i) du[i] *= sc;
   double vel[N];
   for (int d = 0; d < N; ++d) vel[d] = in.v[d][c];
@@ -349,163 +339,138 @@ __device__ inline bool update_cell(const EulerFields &in, EulerFields &u, int c,
     vel[d] = mom * inv_mass;
   }
   const double ie = ener * inv_mass - 0.5 * dotN<N>(vel, vel);
-  u.d[c] = mass;
+  out.d[c] = mass;
   for (int d = 0; d < N; ++d) {
-    u.v[d][c] = vel[d];
+    out.v[d][c] = vel[d];
     vel_out[d] = vel[d];
   }
-  u.e[c] = ie;
-  u.p[c] = eos_p_de(g, mass, ie);
-  u.T[c] = eos_T_de(cv, ie);
+  out.e[c] = ie;
+  out.p[c] = eos_p_de(g, mass, ie);
+  out.T[c] = eos_T_de(cv, ie);
   a_out = eos_a_de(g, ie);
-  u.a[c] = a_out;
+  out.a[c] = a_out;
   return !(ie < 0 || mass < 0);   // apps/hydro/tasks.h:654-655
 }
 
-__device__ inline void flag_negative(DevScalars *s, int c) {
-  atomicCAS(&s->err_code, 0, (int)FSALE_ERR_NEG_STATE);
-  atomicMin(reinterpret_cast<unsigned long long *>(&s->err_cell), (unsigned long long)c);
-  s->active = 0;
-}
-
+// task-level apply_update gathering the stored face fluxes
 template <int N, int FPC>
-__global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u, DevScalars *s, double g,
-                                                       double cv, double factor) {
-  const int c = logical_block() * kBlock + threadIdx.x;
-  if (!s->active) return;
-  double delta_t = s->dt;
-  delta_t *= factor;
-  if (c == 0) s->t = s->t + delta_t;   // update_soln_time :554 (initial time 0)
-  if (c >= m.nc) return;
-  double du[N + 2];
-  for (int i = 0; i < N + 2; ++i) du[i] = 0;
+__global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u, Batch b, double factor) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    const ProbParams &P = b.pp[tile.prob];
+    double delta_t = b.s[tile.prob].dt;
+    delta_t *= factor;
+    const int c = tile.first + threadIdx.x;
+    if (c == P.cell_first) b.s[tile.prob].t = b.s[tile.prob].t + delta_t;   // update_soln_time :554
+    if ((int)threadIdx.x >= tile.count) return;
+    double du[N + 2];
+    for (int i = 0; i < N + 2; ++i) du[i] = 0;
 #pragma unroll
-  for (int k = 0; k < FPC; ++k) {
-    const int code = m.cf[(size_t)k * m.nc + c];
-    const int f = code >> 1;
-    if (code & 1) {
-      for (int i = 0; i < N + 2; ++i) du[i] += u.F[i][f];
-    } else {
-      for (int i = 0; i < N + 2; ++i) du[i] -= u.F[i][f];
+    for (int k = 0; k < FPC; ++k) {
+      const int code = m.cf[(size_t)k * m.cap_c + c];
+      const int f = code >> 1;
+      if (code & 1) {
+        for (int i = 0; i < N + 2; ++i) du[i] += u.F[i][f];
+      } else {
+        for (int i = 0; i < N + 2; ++i) du[i] -= u.F[i][f];
+      }
     }
-  }
-  double vel[N], a;
-  if (!update_cell<N>(u, u, c, du, delta_t / m.cvol[c], g, cv, vel, a)) flag_negative(s, c);
+    double vel[N], a;
+    if (!update_cell<N>(u, u, c, du, delta_t / m.cvol[c], P.gamma, P.cv, vel, a))
+      flag_negative(b.s, tile.prob, c - P.cell_first);
+  });
 }
 
 // fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
 template <int N, int FPC, int VPC>
-__global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, DevScalars *s,
-                                                        double g, double cv, double factor, int kind,
-                                                        double *partials) {
-  const int L = logical_block();
-  const int c = L * kBlock + threadIdx.x;
-  // one read of the step flag per block: another block may clear it (negative
-  // state) while this one runs, and the block reduction below needs every lane
-  __shared__ int blk_active;
-  if (threadIdx.x == 0) blk_active = s->active;
-  __syncthreads();
-  if (!blk_active) return;
-  double delta_t = s->dt;
-  delta_t *= factor;
-  if (c == 0) s->t = s->t + delta_t;
-  const bool live = c < m.nc;
-  double dt_inv = 0;
-  if (live) {
-    CellData<N> own;
-    load_cell<N>(m, u, c, own);
-    double px[VPC], py[VPC];
-    if constexpr (N == 2) {
-      for (int k = 0; k < VPC; ++k) {
-        const int v = m.cv[(size_t)k * m.nc + c];
-        px[k] = m.x[0][v];
-        py[k] = m.x[1][v];
-      }
-    }
-    double du[N + 2];
-    for (int i = 0; i < N + 2; ++i) du[i] = 0;
-    double nk[FPC][N], ak[FPC];
-#pragma unroll
-    for (int k = 0; k < FPC; ++k) {
-      const int code = m.cf[(size_t)k * m.nc + c];
-      const int f = code >> 1;
-      const bool second = code & 1;   // this cell is the face's cells[1]
-      double fx[N];
+__global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, Batch b,
+                                                        double factor, int want_dt) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    const ProbParams &P = b.pp[tile.prob];
+    const int kind = P.kind;
+    double delta_t = b.s[tile.prob].dt;
+    delta_t *= factor;
+    const int c = tile.first + threadIdx.x;
+    if (c == P.cell_first) b.s[tile.prob].t = b.s[tile.prob].t + delta_t;   // update_soln_time :554
+    double dt_inv = 0;
+    if ((int)threadIdx.x < tile.count) {
+      CellData<N> own;
+      load_cell<N>(m, u, c, own);
+      double px[VPC], py[VPC];
       if constexpr (N == 2) {
-        const int k1 = (k + 1 == VPC) ? 0 : k + 1;
-        // the face runs v_k -> v_k1 out of cells[0], reversed for cells[1]
-        if (!second)
-          face_geom_2d(px[k], py[k], px[k1], py[k1], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
-        else
-          face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
-      } else {
-        for (int d = 0; d < N; ++d) {
-          fx[d] = m.fcen[d][f];
-          nk[k][d] = m.fnrm[d][f];
+        for (int k = 0; k < VPC; ++k) {
+          const int v = m.cv[(size_t)k * m.cap_c + c];
+          px[k] = m.x[0][v];
+          py[k] = m.x[1][v];
         }
-        ak[k] = m.farea[f];
       }
-      const int other = m.fc[(size_t)(second ? 0 : 1) * m.nf + f];
-      double fl[N + 2];
-      if (other >= 0) {
-        CellData<N> nb;
-        load_cell<N>(m, u, other, nb);
-        if (!second)
-          face_flux_area<N>(kind, own, nb, true, fx, nk[k], ak[k], fl);
-        else
-          face_flux_area<N>(kind, nb, own, true, fx, nk[k], ak[k], fl);
-      } else {
-        face_flux_area<N>(kind, own, own, false, fx, nk[k], ak[k], fl);
-      }
-      if (second) {
-        for (int i = 0; i < N + 2; ++i) du[i] += fl[i];
-      } else {
-        for (int i = 0; i < N + 2; ++i) du[i] -= fl[i];
-      }
-    }
-    const double vol = m.cvol[c];
-    double vel[N], a;
-    if (!update_cell<N>(u, out, c, du, delta_t / vol, g, cv, vel, a)) flag_negative(s, c);
-    if (partials) {
+      double du[N + 2];
+      for (int i = 0; i < N + 2; ++i) du[i] = 0;
+      double nk[FPC][N], ak[FPC];
 #pragma unroll
       for (int k = 0; k < FPC; ++k) {
-        const double delta_x = vol / ak[k];
-        const double dti = (a + fabs(dotN<N>(vel, nk[k]))) / delta_x;
-        dt_inv = smax(dti, dt_inv);
+        const int code = m.cf[(size_t)k * m.cap_c + c];
+        const int f = code >> 1;
+        const bool second = code & 1;   // this cell is the face's cells[1]
+        double fx[N];
+        if constexpr (N == 2) {
+          const int k1 = (k + 1 == VPC) ? 0 : k + 1;
+          // the face runs v_k -> v_k1 out of cells[0], reversed for cells[1]
+          if (!second)
+            face_geom_2d(px[k], py[k], px[k1], py[k1], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
+          else
+            face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
+        } else {
+          for (int d = 0; d < N; ++d) {
+            fx[d] = m.fcen[d][f];
+            nk[k][d] = m.fnrm[d][f];
+          }
+          ak[k] = m.farea[f];
+        }
+        const int other = m.fc[(size_t)(second ? 0 : 1) * m.cap_f + f];
+        double fl[N + 2];
+        if (other >= 0) {
+          CellData<N> nb;
+          load_cell<N>(m, u, other, nb);
+          if (!second)
+            face_flux_area<N>(kind, own, nb, true, fx, nk[k], ak[k], fl);
+          else
+            face_flux_area<N>(kind, nb, own, true, fx, nk[k], ak[k], fl);
+        } else {
+          face_flux_area<N>(kind, own, own, false, fx, nk[k], ak[k], fl);
+        }
+        if (second) {
+          for (int i = 0; i < N + 2; ++i) du[i] += fl[i];
+        } else {
+          for (int i = 0; i < N + 2; ++i) du[i] -= fl[i];
+        }
+      }
+      const double vol = m.cvol[c];
+      double vel[N], a;
+      if (!update_cell<N>(u, out, c, du, delta_t / vol, P.gamma, P.cv, vel, a))
+        flag_negative(b.s, tile.prob, c - P.cell_first);
+      if (want_dt) {
+#pragma unroll
+        for (int k = 0; k < FPC; ++k) {
+          const double delta_x = vol / ak[k];
+          const double dti = (a + fabs(dotN<N>(vel, nk[k]))) / delta_x;
+          dt_inv = smax(dti, dt_inv);
+        }
       }
     }
-  }
-  if (partials) block_max_to(dt_inv, partials, L);
+    if (want_dt) {
+      const double bm = block_max(dt_inv);
+      if (threadIdx.x == 0) u.partials[tile.slot] = bm;
+    }
+  });
 }
 
 // ------------------------------------------------------------- launches
 template <class K, class... A>
-int launch(fsale_ctx *ctx, int kid, K kernel, int n, A... args) {
+int launch_persistent(fsale_ctx *ctx, int kid, K kernel, A... args) {
   ProfScope prof(ctx, kid);
-  hipLaunchKernelGGL(kernel, dim3(grid8(n)), dim3(kBlock), 0, ctx->stream, args...);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
-}
-
-int launch_dt(fsale_ctx *ctx, double cfl, double final_time, int stop_at_final) {
-  const DevMesh &m = ctx->m;
-  int rc;
-  if (m.dim == 2)
-    rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<2, 4>, m.nc, m, ctx->eu, ctx->eu.partials);
-  else
-    rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<3, 6>, m.nc, m, ctx->eu, ctx->eu.partials);
-  if (rc) return rc;
-  ProfScope prof(ctx, K_EUL_DT);
-  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->eu.partials,
-                     grid8(m.nc), ctx->s, cfl, final_time, stop_at_final);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
-}
-
-int launch_finalize(fsale_ctx *ctx, double cfl, double final_time, int stop_at_final) {
-  ProfScope prof(ctx, K_EUL_DT);
-  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->eu.partials,
-                     grid8(ctx->m.nc), ctx->s, cfl, final_time, stop_at_final);
+  hipLaunchKernelGGL(kernel, dim3(persistent_grid(ctx)), dim3(kBlock), 0, ctx->stream, args...);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -530,57 +495,37 @@ EulerFields swapped(const EulerFields &a) {
   return b;
 }
 
-int launch_grad(fsale_ctx *ctx, int first_order, const EulerFields &u) {
-  const DevMesh &m = ctx->m;
-  if (m.dim == 2) return launch(ctx, K_EUL_GRAD, k_eul_grad<2, 4>, m.nc, m, u, (const DevScalars *)ctx->s, first_order);
-  return launch(ctx, K_EUL_GRAD, k_eul_grad<3, 8>, m.nc, m, u, (const DevScalars *)ctx->s, first_order);
+int launch_dt_partial(fsale_ctx *ctx) {
+  const Batch b = batch_args(ctx);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_DT, k_eul_dt_partial<2, 4>, ctx->m, ctx->eu, b);
+  return launch_persistent(ctx, K_EUL_DT, k_eul_dt_partial<3, 6>, ctx->m, ctx->eu, b);
 }
 
-int launch_grad(fsale_ctx *ctx, int first_order) { return launch_grad(ctx, first_order, ctx->eu); }
-
-int launch_flux(fsale_ctx *ctx, int kind) {
-  const DevMesh &m = ctx->m;
-  if (m.dim == 2) return launch(ctx, K_EUL_FLUX, k_eul_flux<2>, m.nf, m, ctx->eu, (const DevScalars *)ctx->s, kind);
-  return launch(ctx, K_EUL_FLUX, k_eul_flux<3>, m.nf, m, ctx->eu, (const DevScalars *)ctx->s, kind);
-}
-
-int launch_update(fsale_ctx *ctx, fsale_eos eos, double factor) {
-  const DevMesh &m = ctx->m;
-  if (m.dim == 2)
-    return launch(ctx, K_EUL_UPDATE, k_eul_update<2, 4>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor);
-  return launch(ctx, K_EUL_UPDATE, k_eul_update<3, 6>, m.nc, m, ctx->eu, ctx->s, eos.gamma, eos.cv, factor);
-}
-
-int launch_fluxupd(fsale_ctx *ctx, fsale_eos eos, double factor, int kind, bool want_dt, const EulerFields &in,
-                   const EulerFields &out) {
-  const DevMesh &m = ctx->m;
-  double *parts = want_dt ? ctx->eu.partials : nullptr;
-  if (m.dim == 2)
-    return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, m.nc, m, in, out, ctx->s, eos.gamma, eos.cv, factor,
-                  kind, parts);
-  return launch(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, m.nc, m, in, out, ctx->s, eos.gamma, eos.cv, factor,
-                kind, parts);
-}
-
-// one driver step (apps/hydro/driver.h:317-413) with the fused kernels: the
-// first half step reads the primary state buffer and writes the second one,
-// the second half step writes back, so the state is in the primary buffer at
-// every step boundary.  The dt^-1 partials of the current state must be valid
-// on entry and are valid again on exit.
-int launch_step(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_time, int kind, int first_order,
-                int stop_at_final) {
-  int rc;
-  const EulerFields a = ctx->eu, b = swapped(ctx->eu);
-  if ((rc = launch_finalize(ctx, cfl, final_time, stop_at_final))) return rc;
-  if ((rc = launch_grad(ctx, first_order, a))) return rc;
-  if ((rc = launch_fluxupd(ctx, eos, 0.5, kind, false, a, b))) return rc;
-  if ((rc = launch_grad(ctx, first_order, b))) return rc;
-  if ((rc = launch_fluxupd(ctx, eos, 0.5, kind, true, b, a))) return rc;
+int launch_finalize(fsale_ctx *ctx, int driver) {
+  ProfScope prof(ctx, K_EUL_DT);
+  const int np = (int)ctx->probs.size();
+  const int grid = np < 1024 ? np : 1024;
+  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(grid > 0 ? grid : 1), dim3(kBlock), 0, ctx->stream,
+                     batch_args(ctx), (const double *)ctx->eu.partials, np, driver);
+  FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
 
+int launch_grad(fsale_ctx *ctx, const EulerFields &u) {
+  const Batch b = batch_args(ctx);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4>, ctx->m, u, b);
+  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8>, ctx->m, u, b);
+}
+
+int launch_fluxupd(fsale_ctx *ctx, double factor, bool want_dt, const EulerFields &in, const EulerFields &out) {
+  const Batch b = batch_args(ctx);
+  if (ctx->m.dim == 2)
+    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, ctx->m, in, out, b, factor, (int)want_dt);
+  return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, ctx->m, in, out, b, factor, (int)want_dt);
+}
+
 bool check_ctx(fsale_ctx *ctx) {
-  if (!ctx || ctx->solver != FSALE_SOLVER_EULER) {
+  if (!ctx || ctx->solver != FSALE_SOLVER_EULER || !ctx->single || ctx->probs.empty() || !ctx->probs[0].live) {
     set_message("not an Eulerian context");
     return false;
   }
@@ -591,6 +536,24 @@ bool valid_kind(int k) { return k >= 0 && (k & 0xf) <= 2 && (k >> 4) <= 1; }
 
 }  // namespace
 
+// One driver step (apps/hydro/driver.h:317-413) of every live problem with
+// the fused kernels: the first half step reads the primary state buffer and
+// writes the second one, the second half step writes back, so the state is
+// in the primary buffer at every step boundary.  The dt^-1 partials of the
+// current state must be valid on entry and are valid again on exit.
+int euler_step(fsale_ctx *ctx) {
+  int rc;
+  const EulerFields a = ctx->eu, b = swapped(ctx->eu);
+  if ((rc = launch_finalize(ctx, 1))) return rc;
+  if ((rc = launch_grad(ctx, a))) return rc;
+  if ((rc = launch_fluxupd(ctx, 0.5, false, a, b))) return rc;
+  if ((rc = launch_grad(ctx, b))) return rc;
+  if ((rc = launch_fluxupd(ctx, 0.5, true, b, a))) return rc;
+  return FSALE_OK;
+}
+
+int euler_dt_partials(fsale_ctx *ctx) { return launch_dt_partial(ctx); }
+
 int euler_alloc(fsale_ctx *ctx) {
   const DevMesh &m = ctx->m;
   EulerFields &u = ctx->eu;
@@ -598,20 +561,31 @@ int euler_alloc(fsale_ctx *ctx) {
   int rc;
   double **cell1[] = {&u.d, &u.e, &u.p, &u.T, &u.a, &u.alt_d, &u.alt_e, &u.alt_p, &u.alt_T, &u.alt_a};
   for (double **p : cell1)
-    if ((rc = dev_alloc_n(ctx, p, m.nc))) return rc;
+    if ((rc = dev_alloc_n(ctx, p, m.cap_c))) return rc;
   for (int d = 0; d < N; ++d) {
-    if ((rc = dev_alloc_n(ctx, &u.v[d], m.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.alt_v[d], m.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.gd[d], m.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.ge[d], m.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.gp[d], m.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.ga[d], m.nc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.v[d], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.alt_v[d], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.gd[d], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.ge[d], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.gp[d], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.ga[d], m.cap_c))) return rc;
   }
   for (int d = 0; d < N * N; ++d)
-    if ((rc = dev_alloc_n(ctx, &u.gv[d], m.nc))) return rc;
-  for (int i = 0; i < N + 2; ++i)
-    if ((rc = dev_alloc_n(ctx, &u.F[i], m.nf))) return rc;
-  if ((rc = dev_alloc_n(ctx, &u.partials, grid8(m.nc)))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.gv[d], m.cap_c))) return rc;
+  if (ctx->single)   // the face flux array serves the task-level evaluate_fluxes only
+    for (int i = 0; i < N + 2; ++i)
+      if ((rc = dev_alloc_n(ctx, &u.F[i], m.cap_f))) return rc;
+  if ((rc = dev_alloc_n(ctx, &u.partials, ctx->ra_ct.capacity + 1))) return rc;
+  return FSALE_OK;
+}
+
+// initial conditions of one problem: d, v, p already in place
+int euler_initial_state(fsale_ctx *ctx, int prob) {
+  const Problem &P = ctx->probs[prob];
+  const int g = (P.nc + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_eul_ic, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->eu, (const ProbParams *)ctx->d_params,
+                     prob);
+  FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
 
@@ -619,120 +593,111 @@ int euler_alloc(fsale_ctx *ctx) {
 
 using namespace fsale;
 
+namespace fsale {
+int set_problem_state(fsale_ctx *ctx, int prob, const double *d, const double *v, const double *p);
+int reset_scalars(fsale_ctx *ctx, int prob, double t, double dt);
+int get_problem_field(fsale_ctx *ctx, int prob, int field, double *out);
+}  // namespace fsale
+
 extern "C" {
 
 int fsale_eul_initial_conditions(fsale_ctx *ctx, fsale_eos eos, const double *d, const double *v,
                                  const double *p) {
   if (!check_ctx(ctx) || !d || !v || !p) return FSALE_ERR_ARG;
   if (!(eos.gamma > 1) || !(eos.cv > 0)) return FSALE_ERR_ARG;
+  Problem &P = ctx->probs[0];
+  P.params.gamma = eos.gamma;
+  P.params.cv = eos.cv;
   int rc;
-  if ((rc = fsale_set_field(ctx, FSALE_F_DENSITY, d))) return rc;
-  if ((rc = fsale_set_field(ctx, FSALE_F_VELOCITY, v))) return rc;
-  if ((rc = fsale_set_field(ctx, FSALE_F_PRESSURE, p))) return rc;
-  hipLaunchKernelGGL(k_eul_ic, dim3(grid8(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m.nc, ctx->eu,
-                     eos.gamma, eos.cv);
-  FSALE_HIP(hipGetLastError());
-  ctx->eu.gamma = eos.gamma;
-  ctx->eu.cv = eos.cv;
-  ctx->have_state = true;
-  ctx->eu.partials_valid = false;
-  // fresh run: time 0, no error, active
-  if ((rc = read_scalars(ctx))) return rc;
-  DevScalars h = *ctx->h_s;
-  h.t = 0;
-  h.dt = 0;
-  h.steps = 0;
-  h.err_code = 0;
-  h.err_cell = INT64_MAX;
-  h.active = 1;
-  h.red[0] = h.red[1] = 0;
-  h.done = 0;
-  *ctx->h_s = h;
-  FSALE_HIP(hipMemcpyAsync(ctx->s, ctx->h_s, sizeof(DevScalars), hipMemcpyHostToDevice, ctx->stream));
-  FSALE_HIP(hipStreamSynchronize(ctx->stream));
-  return FSALE_OK;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  if ((rc = set_problem_state(ctx, 0, d, v, p))) return rc;
+  if ((rc = euler_initial_state(ctx, 0))) return rc;
+  P.have_state = true;
+  P.partials_valid = false;
+  return reset_scalars(ctx, 0, 0.0, 0.0);
 }
 
 int fsale_eul_evaluate_time_step(fsale_ctx *ctx, double cfl, double final_time, double *dt_out) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  int rc = launch_dt(ctx, cfl, final_time, 0);
-  if (rc) return rc;
-  ctx->eu.partials_valid = true;
+  Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  P.params.cfl = cfl;
+  P.params.final_time = final_time;
+  int rc;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  if ((rc = sync_tiles(ctx))) return rc;
+  if ((rc = launch_dt_partial(ctx))) return rc;
+  if ((rc = launch_finalize(ctx, 0))) return rc;
+  P.partials_valid = true;
   if (!dt_out) return FSALE_OK;
-  if ((rc = check_device_error(ctx))) return rc;
-  *dt_out = ctx->h_s->dt;
+  if ((rc = check_device_error(ctx, 0))) return rc;
+  *dt_out = ctx->h_s[0].dt;
   return FSALE_OK;
 }
 
 int fsale_eul_evaluate_gradients(fsale_ctx *ctx, int first_order) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  return launch_grad(ctx, first_order);
+  Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  int rc;
+  P.params.first_order = first_order != 0;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  if ((rc = sync_tiles(ctx))) return rc;
+  return launch_grad(ctx, ctx->eu);
 }
 
 int fsale_eul_evaluate_fluxes(fsale_ctx *ctx, int flux_kind) {
   if (!check_ctx(ctx) || !valid_kind(flux_kind)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  return launch_flux(ctx, flux_kind);
+  Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  P.params.kind = flux_kind;
+  int rc;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  ProfScope prof(ctx, K_EUL_FLUX);
+  const int g = (P.nf + kBlock - 1) / kBlock;
+  if (ctx->m.dim == 2)
+    hipLaunchKernelGGL(k_eul_flux<2>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->eu, batch_args(ctx), 0,
+                       P.f0, P.nf);
+  else
+    hipLaunchKernelGGL(k_eul_flux<3>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->eu, batch_args(ctx), 0,
+                       P.f0, P.nf);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
 }
 
 int fsale_eul_apply_update(fsale_ctx *ctx, fsale_eos eos, double factor) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  ctx->eu.partials_valid = false;
-  return launch_update(ctx, eos, factor);
+  Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  P.params.gamma = eos.gamma;
+  P.params.cv = eos.cv;
+  P.partials_valid = false;
+  int rc;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  if ((rc = sync_tiles(ctx))) return rc;
+  const Batch b = batch_args(ctx);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_UPDATE, k_eul_update<2, 4>, ctx->m, ctx->eu, b, factor);
+  return launch_persistent(ctx, K_EUL_UPDATE, k_eul_update<3, 6>, ctx->m, ctx->eu, b, factor);
 }
 
 int fsale_eul_advance(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_time, int nsteps, int flux_kind,
                       int first_order, int stop_at_final, int use_graph) {
   if (!check_ctx(ctx) || nsteps < 0 || !valid_kind(flux_kind)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
+  Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
   if (nsteps == 0) return FSALE_OK;
   int rc;
-  if (!ctx->eu.partials_valid) {
-    const DevMesh &m = ctx->m;
-    if (m.dim == 2)
-      rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<2, 4>, m.nc, m, ctx->eu, ctx->eu.partials);
-    else
-      rc = launch(ctx, K_EUL_DT, k_eul_dt_partial<3, 6>, m.nc, m, ctx->eu, ctx->eu.partials);
-    if (rc) return rc;
-    ctx->eu.partials_valid = true;
-  }
-  if (!use_graph || nsteps < 2 || ctx->profiling) {
-    for (int n = 0; n < nsteps; ++n)
-      if ((rc = launch_step(ctx, eos, cfl, final_time, flux_kind, first_order, stop_at_final))) return rc;
-    return FSALE_OK;
-  }
-  // capture a chunk of steps once, replay it; the key holds every baked-in argument
-  const int chunk = nsteps < 10 ? nsteps : 10;
-  std::vector<double> key = {0.0, eos.gamma, eos.cv, cfl, final_time, (double)flux_kind, (double)first_order,
-                             (double)stop_at_final};
-  if (!ctx->graph.exec || ctx->graph.key != key || ctx->graph.steps != chunk) {
-    if (ctx->graph.exec) {
-      (void)hipGraphExecDestroy(ctx->graph.exec);
-      ctx->graph.exec = nullptr;
-    }
-    hipGraph_t graph;
-    FSALE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int n = 0; n < chunk; ++n) {
-      rc = launch_step(ctx, eos, cfl, final_time, flux_kind, first_order, stop_at_final);
-      if (rc) {
-        (void)hipStreamEndCapture(ctx->stream, &graph);
-        return rc;
-      }
-    }
-    FSALE_HIP(hipStreamEndCapture(ctx->stream, &graph));
-    FSALE_HIP(hipGraphInstantiate(&ctx->graph.exec, graph, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(graph);
-    ctx->graph.key = key;
-    ctx->graph.steps = chunk;
-  }
-  int n = 0;
-  for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(ctx->graph.exec, ctx->stream));
-  for (; n < nsteps; ++n)
-    if ((rc = launch_step(ctx, eos, cfl, final_time, flux_kind, first_order, stop_at_final))) return rc;
-  return FSALE_OK;
+  if ((rc = read_scalars(ctx))) return rc;
+  P.params.gamma = eos.gamma;
+  P.params.cv = eos.cv;
+  P.params.cfl = cfl;
+  P.params.final_time = final_time;
+  P.params.kind = flux_kind;
+  P.params.first_order = first_order != 0;
+  P.params.stop_at_final = stop_at_final != 0;
+  P.params.max_steps = ctx->h_s[0].steps + nsteps;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  return fsale_batch_step(ctx, nsteps, use_graph);
 }
 
 }  // extern "C"

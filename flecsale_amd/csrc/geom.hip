@@ -5,34 +5,37 @@
 // The burton sources are not vendored; the 2D formulas below reproduce the
 // reference goldens (shoelace centroid about the origin), the 3D hex formulas
 // are build-defined (triangles / tetrahedra about vertex averages).
+// Each launch covers one problem's cell range [c0, c0+n) or face range.
 #include "internal.h"
 #include "kernels_common.h"
 
 namespace fsale {
 namespace {
 
-__global__ void __launch_bounds__(kBlock) k_cell_geom_2d(DevMesh m) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.nc) return;
-  const int n = m.vpc;
+__global__ void __launch_bounds__(kBlock) k_cell_geom_2d(DevMesh m, int c0, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = c0 + i;
+  const int nv = m.vpc;
   double px[8], py[8];
-  for (int i = 0; i < n; ++i) {
-    const int v = m.cv[(size_t)i * m.nc + c];
-    px[i] = m.x[0][v];
-    py[i] = m.x[1][v];
+  for (int k = 0; k < nv; ++k) {
+    const int v = m.cv[(size_t)k * m.cap_c + c];
+    px[k] = m.x[0][v];
+    py[k] = m.x[1][v];
   }
   double cx, cy, vol, ml;
-  cell_geom_2d(n, px, py, cx, cy, vol, ml);
+  cell_geom_2d(nv, px, py, cx, cy, vol, ml);
   m.ccen[0][c] = cx;
   m.ccen[1][c] = cy;
   m.cvol[c] = vol;
   m.cminlen[c] = ml;
 }
 
-__global__ void __launch_bounds__(kBlock) k_face_geom_2d(DevMesh m) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= m.nf) return;
-  const int a = m.fv[f], b = m.fv[(size_t)m.nf + f];
+__global__ void __launch_bounds__(kBlock) k_face_geom_2d(DevMesh m, int f0, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = f0 + i;
+  const int a = m.fv[f], b = m.fv[(size_t)m.cap_f + f];
   double cx, cy, nx, ny, area;
   face_geom_2d(m.x[0][a], m.x[1][a], m.x[0][b], m.x[1][b], cx, cy, nx, ny, area);
   m.fcen[0][f] = cx;
@@ -42,15 +45,16 @@ __global__ void __launch_bounds__(kBlock) k_face_geom_2d(DevMesh m) {
   m.farea[f] = area;
 }
 
-__global__ void __launch_bounds__(kBlock) k_cell_geom_3d(DevMesh m) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.nc) return;
+__global__ void __launch_bounds__(kBlock) k_cell_geom_3d(DevMesh m, int c0, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = c0 + i;
   double p[8][3];
-  for (int i = 0; i < 8; ++i) {
-    const int v = m.cv[(size_t)i * m.nc + c];
-    p[i][0] = m.x[0][v];
-    p[i][1] = m.x[1][v];
-    p[i][2] = m.x[2][v];
+  for (int k = 0; k < 8; ++k) {
+    const int v = m.cv[(size_t)k * m.cap_c + c];
+    p[k][0] = m.x[0][v];
+    p[k][1] = m.x[1][v];
+    p[k][2] = m.x[2][v];
   }
   double cen[3], vol, ml;
   hex_geom(p, cen, vol, ml);
@@ -59,15 +63,16 @@ __global__ void __launch_bounds__(kBlock) k_cell_geom_3d(DevMesh m) {
   m.cminlen[c] = ml;
 }
 
-__global__ void __launch_bounds__(kBlock) k_face_geom_3d(DevMesh m) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= m.nf) return;
+__global__ void __launch_bounds__(kBlock) k_face_geom_3d(DevMesh m, int f0, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = f0 + i;
   double q[4][3];
-  for (int i = 0; i < 4; ++i) {
-    const int v = m.fv[(size_t)i * m.nf + f];
-    q[i][0] = m.x[0][v];
-    q[i][1] = m.x[1][v];
-    q[i][2] = m.x[2][v];
+  for (int k = 0; k < 4; ++k) {
+    const int v = m.fv[(size_t)k * m.cap_f + f];
+    q[k][0] = m.x[0][v];
+    q[k][1] = m.x[1][v];
+    q[k][2] = m.x[2][v];
   }
   double cen[3], nrm[3], area;
   quad3_geom(q, cen, nrm, area);
@@ -80,22 +85,23 @@ __global__ void __launch_bounds__(kBlock) k_face_geom_3d(DevMesh m) {
 
 }  // namespace
 
-int launch_geometry(fsale_ctx *ctx, bool cells, bool faces) {
+int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces) {
   const DevMesh &m = ctx->m;
-  if (cells) {
-    const int g = (m.nc + kBlock - 1) / kBlock;
+  const Problem &P = ctx->probs[prob];
+  if (cells && P.nc > 0) {
+    const int g = (P.nc + kBlock - 1) / kBlock;
     if (m.dim == 2)
-      hipLaunchKernelGGL(k_cell_geom_2d, dim3(g), dim3(kBlock), 0, ctx->stream, m);
+      hipLaunchKernelGGL(k_cell_geom_2d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.c0, P.nc);
     else
-      hipLaunchKernelGGL(k_cell_geom_3d, dim3(g), dim3(kBlock), 0, ctx->stream, m);
+      hipLaunchKernelGGL(k_cell_geom_3d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.c0, P.nc);
     FSALE_HIP(hipGetLastError());
   }
-  if (faces) {
-    const int g = (m.nf + kBlock - 1) / kBlock;
+  if (faces && P.nf > 0) {
+    const int g = (P.nf + kBlock - 1) / kBlock;
     if (m.dim == 2)
-      hipLaunchKernelGGL(k_face_geom_2d, dim3(g), dim3(kBlock), 0, ctx->stream, m);
+      hipLaunchKernelGGL(k_face_geom_2d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.f0, P.nf);
     else
-      hipLaunchKernelGGL(k_face_geom_3d, dim3(g), dim3(kBlock), 0, ctx->stream, m);
+      hipLaunchKernelGGL(k_face_geom_3d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.f0, P.nf);
     FSALE_HIP(hipGetLastError());
   }
   return FSALE_OK;

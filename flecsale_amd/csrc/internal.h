@@ -1,10 +1,20 @@
 // Internal declarations shared by the host and device translation units of
 // libflecsale_amd.  Not part of the public ABI (include/flecsale_amd.h).
+//
+// Every context is a ragged batch: problems (one mesh + state each) occupy
+// disjoint ranges of capacity-sized device arrays, and every kernel is a
+// persistent loop over a device-side list of tiles (a tile = up to kBlock
+// consecutive cells, or vertices, of one problem).  The plain drop-in API is
+// a batch holding exactly one problem.  Problem-specific scalars (EOS, CFL,
+// time, dt, step budget, error word) live in per-problem device records, so a
+// problem's arithmetic is the same alone and in any mix.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -12,7 +22,7 @@
 
 namespace fsale {
 
-constexpr int kBlock = 256;     // threads per block for the entity kernels
+constexpr int kBlock = 256;     // threads per block = entities per tile
 constexpr int kMaxCornersPerVertex = 8;
 
 void set_message(const std::string &msg);
@@ -26,7 +36,7 @@ void set_message(const std::string &msg);
     }                                                                          \
   } while (0)
 
-// Host-side connectivity, built once per mesh (mesh_host.cpp).  Conventions
+// Host-side connectivity of one mesh (mesh_host.cpp), local ids.  Conventions
 // (the reference's burton mesh is not vendored; these reproduce its goldens):
 //  - faces numbered in order of first visit, cell by cell in local face order
 //    (2D: edge k = (v_k, v_{k+1}); 3D: Exodus hex sides), fc[0] = first
@@ -47,18 +57,24 @@ struct HostTopology {
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
 
-// Device-side mesh: SoA geometry, k-major fixed-degree connectivity.
+// Device-side mesh storage of a batch: SoA geometry, k-major fixed-degree
+// connectivity with row stride = capacity, global (batch) entity ids.
 struct DevMesh {
-  int dim = 0, nc = 0, nv = 0, nf = 0, vpc = 0, fpc = 0, vpf = 0, nnbr = 0;
-  double *x[3] = {nullptr, nullptr, nullptr};      // vertex coords [nv]
-  int32_t *cv = nullptr;     // [vpc][nc]
-  int32_t *cf = nullptr;     // [fpc][nc]: face*2 + (cell is fc[1] of the face)
-  int32_t *fc = nullptr;     // [2][nf]
-  int32_t *fv = nullptr;     // [vpf][nf]
-  int32_t *nbr_off = nullptr, *nbr = nullptr;
-  int32_t *vc_off = nullptr, *vc = nullptr;        // vertex -> corner (c*vpc+k)
-  uint8_t *cwtag = nullptr;  // [vpc][nc] Maire wedge boundary tags
-  uint8_t *vbnd = nullptr;   // [nv] vertex on a boundary face
+  int dim = 0, vpc = 0, fpc = 0, vpf = 0;
+  int cap_c = 0, cap_f = 0, cap_v = 0;           // row strides
+  double *x[3] = {nullptr, nullptr, nullptr};    // vertex coords [cap_v]
+  int32_t *cv = nullptr;        // [vpc][cap_c]
+  int32_t *cf = nullptr;        // [fpc][cap_c]: face*2 + (cell is fc[1] of the face)
+  int32_t *fc = nullptr;        // [2][cap_f]
+  int32_t *fv = nullptr;        // [vpf][cap_f]
+  int32_t *nbr_start = nullptr; // [cap_c] cell-vertex neighbour list start
+  int32_t *nbr_cnt = nullptr;   // [cap_c]
+  int32_t *nbr = nullptr;       // [cap_n]
+  int32_t *vc_start = nullptr;  // [cap_v] vertex -> corner list start
+  int32_t *vc_cnt = nullptr;    // [cap_v]
+  int32_t *vc = nullptr;        // corner = c*vpc + k (global cell id) [cap_c*vpc]
+  uint8_t *cwtag = nullptr;     // [vpc][cap_c] Maire wedge boundary tags
+  uint8_t *vbnd = nullptr;      // [cap_v] vertex on a boundary face
   double *ccen[3] = {nullptr, nullptr, nullptr};
   double *cvol = nullptr, *cminlen = nullptr;
   double *fcen[3] = {nullptr, nullptr, nullptr};
@@ -66,17 +82,37 @@ struct DevMesh {
   double *farea = nullptr;
 };
 
-// Scalars that stay on the device between tasks (dt future, solution time,
-// error word, reduction scratch).
+// A tile: `count` (<= kBlock) consecutive entities of problem `prob`
+// starting at global id `first`; `slot` indexes per-tile scratch (dt partials).
+struct Tile {
+  int prob, first, count, slot;
+};
+
+// Device tile list: tiles[0..*n)
+struct TileList {
+  const Tile *tiles = nullptr;
+  const int *n = nullptr;
+};
+
+// Per-problem parameters (read-only in a step).
+struct ProbParams {
+  double gamma, cv;                        // Eulerian EOS (Maire: per-cell gamma, cv here)
+  double cfl, final_time;                  // Eulerian CFL / final time
+  double cfl_acc, cfl_vol, cfl_growth;     // Maire time_constants_t
+  long long max_steps;                     // step budget: active while steps < max_steps
+  int kind, first_order, stop_at_final, pad;
+  int tile_first, tile_count;              // the problem's cell tiles (dt partials)
+  int cell_first, cell_count;
+};
+
+// Per-problem scalars that stay on the device between tasks (the dt future,
+// the solution time, the error word).
 struct DevScalars {
   double dt;            // current time step (the reference's dt future)
   double t;             // solution time
-  double red[2];        // max reductions as uint64 bit patterns (>= 0 values)
-  unsigned int done;    // finished-block counter of the reduction kernels
-  int active;           // step is live (t < t_final, no error)
+  int active;           // the current step is live
   int err_code;         // first fsale_status raised on the device
-  int pad;
-  long long err_cell;   // lowest failing cell
+  long long err_cell;   // lowest failing cell (local id of the problem)
   long long steps;      // completed steps
 };
 
@@ -89,9 +125,7 @@ struct EulerFields {
          *alt_T = nullptr, *alt_a = nullptr;
   double *gd[3] = {}, *gv[9] = {}, *ge[3] = {}, *gp[3] = {}, *ga[3] = {};
   double *F[5] = {};
-  double *partials = nullptr;    // per-block max of dt^-1 [grid8(nc)]
-  bool partials_valid = false;   // partials describe the current state
-  double gamma = 0, cv = 0;
+  double *partials = nullptr;    // per cell tile: max of dt^-1
 };
 
 struct MaireFields {
@@ -99,9 +133,18 @@ struct MaireFields {
          *T = nullptr, *a = nullptr, *g = nullptr;
   double *v0[2] = {}, *e0 = nullptr, *x0[2] = {};
   double *un[2] = {};
-  double *npc[2] = {}, *Fpc[2] = {};   // corner arrays [vpc][nc]
+  double *npc[2] = {}, *Fpc[2] = {};   // corner arrays [vpc][cap_c]
   double *dudt[4] = {};
-  double cv = 1.0;
+  double *partials = nullptr;          // per cell tile: max a/min_length, max |dV/dt|/V
+};
+
+// Kernel arguments common to all step kernels
+struct Batch {
+  TileList ct;                // cell tiles
+  TileList vt;                // vertex tiles (Maire)
+  const ProbParams *pp = nullptr;
+  DevScalars *s = nullptr;
+  int cap_c_tiles = 0;        // partials stride
 };
 
 struct GraphCache {
@@ -125,20 +168,61 @@ struct ProfSlot {
   long long launches = 0;
 };
 
+// Host record of one resident problem
+struct Problem {
+  bool live = false;
+  int c0 = 0, nc = 0, f0 = 0, nf = 0, v0 = 0, nv = 0, n0 = 0, nn = 0;   // ranges
+  int ct0 = 0, nct = 0, vt0 = 0, nvt = 0;                               // tile ranges
+  std::shared_ptr<const HostTopology> topo;
+  ProbParams params{};
+  bool have_state = false;
+  bool partials_valid = false;
+};
+
+// first-fit range allocator over [0, capacity)
+struct RangeAlloc {
+  int capacity = 0;
+  std::map<int, int> free;   // start -> length
+  void init(int cap) {
+    capacity = cap;
+    free.clear();
+    if (cap > 0) free[0] = cap;
+  }
+  int alloc(int n);          // -1 if no room
+  void release(int start, int n);
+  long long available() const;
+};
+
+// launch-shape choice (autotuned per context)
+struct LaunchShape {
+  int blocks_per_cu = 4;     // persistent grid = CUs * blocks_per_cu (multiple of 8)
+};
+
 }  // namespace fsale
 
 struct fsale_ctx {
   int solver = 0;
   int device = 0;
+  int num_cus = 256;
   hipStream_t stream = nullptr;
-  fsale::HostTopology topo;
+  bool single = false;                 // plain drop-in context (one problem)
   fsale::DevMesh m;
-  fsale::DevScalars *s = nullptr;      // device
-  fsale::DevScalars *h_s = nullptr;    // pinned host mirror
   fsale::EulerFields eu;
   fsale::MaireFields ma;
-  bool have_state = false;
+  // problems, tiles and their device mirrors
+  std::vector<fsale::Problem> probs;   // index = problem id
+  int max_problems = 0;
+  fsale::RangeAlloc ra_c, ra_f, ra_v, ra_n, ra_ct, ra_vt;
+  fsale::Tile *d_ctiles = nullptr, *d_vtiles = nullptr;
+  int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles
+  fsale::ProbParams *d_params = nullptr;
+  fsale::DevScalars *s = nullptr;      // [max_problems] device
+  fsale::DevScalars *h_s = nullptr;    // pinned host mirror
+  std::vector<fsale::Tile> h_ctiles, h_vtiles;
+  bool tiles_dirty = true;
+  std::map<std::string, std::shared_ptr<const fsale::HostTopology>> topo_cache;
   fsale::GraphCache graph;
+  fsale::LaunchShape shape;
   std::vector<void *> allocations;
   bool profiling = false;
   fsale::ProfSlot prof[fsale::K_COUNT];
@@ -160,12 +244,19 @@ int dev_alloc_n(fsale_ctx *ctx, T **p, size_t n) {
   return dev_alloc(ctx, reinterpret_cast<void **>(p), n * sizeof(T) + 16);
 }
 int read_scalars(fsale_ctx *ctx);      // sync + copy DevScalars to h_s
-int check_device_error(fsale_ctx *ctx);
+int check_device_error(fsale_ctx *ctx, int prob);
+int sync_tiles(fsale_ctx *ctx);        // upload the tile lists if they changed
+int upload_params(fsale_ctx *ctx, int prob);
+Batch batch_args(const fsale_ctx *ctx);
+int persistent_grid(const fsale_ctx *ctx);
 
-// geometry (geom.hip)
-int launch_geometry(fsale_ctx *ctx, bool cells, bool faces);
+// geometry (geom.hip): cells/faces of one problem
+int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces);
 // Euler (euler.hip)
 int euler_alloc(fsale_ctx *ctx);
+int euler_dt_partials(fsale_ctx *ctx);
+int euler_step(fsale_ctx *ctx);       // one driver step over every live problem
 // Maire (maire.hip)
 int maire_alloc(fsale_ctx *ctx);
+int maire_step(fsale_ctx *ctx);
 }  // namespace fsale

@@ -3,18 +3,19 @@
 //
 //   k_maire_nodal     evaluate_nodal_state      apps/maire_hydro/tasks.h:221-453
 //   k_maire_residual  evaluate_residual         apps/maire_hydro/tasks.h:461-489
-//   k_maire_dt        evaluate_time_step        apps/maire_hydro/tasks.h:142-190
+//   k_maire_dt_*      evaluate_time_step        apps/maire_hydro/tasks.h:142-190
 //   k_maire_move      apply_update (mesh move)  apps/maire_hydro/tasks.h:517-534
 //   k_maire_update    apply_update (geometry + state) + update_state_from_energy
 //                     apps/maire_hydro/tasks.h:536-546, :110-133
 //   k_maire_copy      save_/restore_coordinates, save_/restore_solution :557-654
 //
-// Wedge facets (half edges, burton's wedge facet_normal/facet_area) are
-// recomputed from the vertex coordinates where they are needed instead of
-// being stored: the mesh moves every half step.  Corner arrays are k-major
-// [4][nc] so the residual gather of a cell's four corners is coalesced.
-#include "internal.h"
-#include "kernels_common.h"
+// Persistent loops over the batch's vertex tiles (nodal, move) and cell tiles
+// (the rest) with per-problem scalars.  Wedge facets (half edges: burton's
+// wedge facet_normal / facet_area) are recomputed from the vertex coordinates
+// where needed instead of being stored: the mesh moves every half step.
+// Corner arrays are k-major [4][cap_c] so the residual gather of a cell's
+// four corners is coalesced.
+#include "batch_dev.h"
 
 namespace fsale {
 namespace {
@@ -22,9 +23,11 @@ namespace {
 constexpr double kMinSoundSpeed = 1.e-6;   // flecsale/eqns/lagrange_eqns.h:54
 
 // ------------------------------------------------------------ state init
-__global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.nc) return;
+__global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u, const ProbParams *pp, int prob) {
+  const ProbParams &P = pp[prob];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.cell_count) return;
+  const int c = P.cell_first + i;
   // initial_conditions apps/maire_hydro/tasks.h:86-100 (u.d holds rho on entry)
   const double vol = m.cvol[c];
   const double M = u.d[c] * vol;
@@ -36,7 +39,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u) {
   const double ie = eos_e_dp(g, d, u.p[c]);
   u.d[c] = d;
   u.e[c] = ie;
-  u.T[c] = eos_T_de(u.cv, ie);
+  u.T[c] = eos_T_de(P.cv, ie);
   u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
 }
 
@@ -45,8 +48,8 @@ __global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u) {
 // midpoint - vertex| (burton wedge geometry; pinned by the sedov_2d golden).
 __device__ inline void wedge_facets(const DevMesh &m, int c, int k, double n[2][2], double l[2]) {
   const int km = (k + 3) & 3, kp = (k + 1) & 3;
-  const int im = m.cv[(size_t)km * m.nc + c], ik = m.cv[(size_t)k * m.nc + c],
-            ip = m.cv[(size_t)kp * m.nc + c];
+  const int im = m.cv[(size_t)km * m.cap_c + c], ik = m.cv[(size_t)k * m.cap_c + c],
+            ip = m.cv[(size_t)kp * m.cap_c + c];
   const double xm = m.x[0][im], ym = m.x[1][im];
   const double xk = m.x[0][ik], yk = m.x[1][ik];
   const double xp = m.x[0][ip], yp = m.x[1][ip];
@@ -81,10 +84,9 @@ __device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, const DevScalars *s) {
-  const int vt = blockIdx.x * blockDim.x + threadIdx.x;
-  if (vt >= m.nv || !s->active) return;
-  const int q0 = m.vc_off[vt], q1 = m.vc_off[vt + 1];
+// nodal solve of one vertex (batch id vt)
+__device__ inline void nodal_vertex(const DevMesh &m, const MaireFields &u, int vt) {
+  const int q0 = m.vc_start[vt], q1 = q0 + m.vc_cnt[vt];
   double Mp[2][2] = {{0, 0}, {0, 0}}, rhs[2] = {0, 0};
   for (int q = q0; q < q1; ++q) {
     const int cn = m.vc[q];
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u
       F[d] += pc * np[d];
       rhs[d] += F[d];
     }
-    const size_t ci = (size_t)k * m.nc + c;
+    const size_t ci = (size_t)k * m.cap_c + c;
     u.npc[0][ci] = np[0];
     u.npc[1][ci] = np[1];
     u.Fpc[0][ci] = F[0];
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u
     for (int q = q0; q < q1; ++q) {
       const int cn = m.vc[q];
       const int c = cn >> 2, k = cn & 3;
-      const int tags = m.cwtag[(size_t)k * m.nc + c];
+      const int tags = m.cwtag[(size_t)k * m.cap_c + c];
       if (!(tags & 0x44)) continue;
       double n[2][2], l[2];
       wedge_facets(m, c, k, n, l);
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u
     const int c = cn >> 2, k = cn & 3;
     double M[2][2], np[2];
     corner_matrix(m, u, c, k, M, np);
-    const size_t ci = (size_t)k * m.nc + c;
+    const size_t ci = (size_t)k * m.cap_c + c;
     for (int r = 0; r < 2; ++r) {
       double t = M[r][0] * un[0];
       t += M[r][1] * un[1];
@@ -182,147 +184,156 @@ __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, const DevScalars *s) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.nc || !s->active) return;
-  double r[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const size_t ci = (size_t)k * m.nc + c;
-    const int vt = m.cv[ci];
-    const double uv[2] = {u.un[0][vt], u.un[1][vt]};
-    const double F[2] = {u.Fpc[0][ci], u.Fpc[1][ci]};
-    const double n[2] = {u.npc[0][ci], u.npc[1][ci]};
-    // compute_update lagrange_eqns.h:374-389
-    for (int d = 0; d < 2; ++d) {
-      r[0] += n[d] * uv[d];
-      r[3] -= F[d] * uv[d];
-      r[1 + d] -= F[d];
-    }
-  }
-  for (int i = 0; i < 4; ++i) u.dudt[i][c] = r[i];
+__global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
+  for_each_tile(b.vt, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    if ((int)threadIdx.x < tile.count) nodal_vertex(m, u, tile.first + threadIdx.x);
+  });
 }
 
-__global__ void __launch_bounds__(kBlock) k_maire_dt(DevMesh m, MaireFields u, DevScalars *s, double cfl_acc,
-                                                     double cfl_vol, double cfl_growth, double final_time) {
-  __shared__ double red[2][kBlock / 64];
-  __shared__ int last;
-  if (!s->active) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0, vol = 0;
-  if (c < m.nc) {
-    acc = u.a[c] / m.cminlen[c];
-    vol = fabs(u.dudt[0][c]) / m.cvol[c];
-  }
-  acc = wave_max(acc);
-  vol = wave_max(vol);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = acc;
-    red[1][threadIdx.x >> 6] = vol;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b0 = red[0][0], b1 = red[1][0];
-    for (int w = 1; w < kBlock / 64; ++w) {
-      b0 = smax(red[0][w], b0);
-      b1 = smax(red[1][w], b1);
+__global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, Batch b) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const int c = tile.first + threadIdx.x;
+    double r[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t ci = (size_t)k * m.cap_c + c;
+      const int vt = m.cv[ci];
+      const double uv[2] = {u.un[0][vt], u.un[1][vt]};
+      const double F[2] = {u.Fpc[0][ci], u.Fpc[1][ci]};
+      const double n[2] = {u.npc[0][ci], u.npc[1][ci]};
+      // compute_update lagrange_eqns.h:374-389
+      for (int d = 0; d < 2; ++d) {
+        r[0] += n[d] * uv[d];
+        r[3] -= F[d] * uv[d];
+        r[1 + d] -= F[d];
+      }
     }
-    atomic_max_nonneg(&s->red[0], b0);
-    atomic_max_nonneg(&s->red[1], b1);
-    __threadfence();
-    const unsigned prev = atomicAdd(&s->done, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    unsigned long long *r = reinterpret_cast<unsigned long long *>(s->red);
-    const double acc_inv = __longlong_as_double((long long)atomicAdd(&r[0], 0ull));
-    const double vol_inv = __longlong_as_double((long long)atomicAdd(&r[1], 0ull));
-    s->red[0] = 0;
-    s->red[1] = 0;
-    s->done = 0;
-    // evaluate_time_step :179-188: first minimum of the three limits
-    const double dts[3] = {cfl_acc / acc_inv, cfl_vol / vol_inv, cfl_growth * s->dt};
+    for (int i = 0; i < 4; ++i) u.dudt[i][c] = r[i];
+  });
+}
+
+// per cell tile: max a/min_length and max |dV/dt|/V
+__global__ void __launch_bounds__(kBlock) k_maire_dt_partial(DevMesh m, MaireFields u, Batch b) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    double acc = 0, vol = 0;
+    if ((int)threadIdx.x < tile.count) {
+      const int c = tile.first + threadIdx.x;
+      acc = u.a[c] / m.cminlen[c];
+      vol = fabs(u.dudt[0][c]) / m.cvol[c];
+    }
+    const double bacc = block_max(acc);
+    const double bvol = block_max(vol);
+    if (threadIdx.x == 0) {
+      u.partials[2 * tile.slot] = bacc;
+      u.partials[2 * tile.slot + 1] = bvol;
+    }
+  });
+}
+
+// evaluate_time_step :179-188 (first minimum of the three limits) and the
+// driver's cap min(dt, t_final - t) (driver.h:391), one block per problem
+__global__ void __launch_bounds__(kBlock) k_maire_dt_finalize(Batch b, const double *partials, int nprob) {
+  for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
+    const ProbParams &P = b.pp[p];
+    if (P.cell_count == 0) continue;
+    if (!block_active(b.s, p)) continue;
+    double a0 = 0, a1 = 0;
+    for (int i = threadIdx.x; i < P.tile_count; i += kBlock) {
+      a0 = smax(partials[2 * (P.tile_first + i)], a0);
+      a1 = smax(partials[2 * (P.tile_first + i) + 1], a1);
+    }
+    const double acc_inv = block_max(a0);
+    const double vol_inv = block_max(a1);
+    if (threadIdx.x != 0) continue;
+    DevScalars &S = b.s[p];
+    const double dts[3] = {P.cfl_acc / acc_inv, P.cfl_vol / vol_inv, P.cfl_growth * S.dt};
     int it = 0;
     for (int i = 1; i < 3; ++i)
       if (dts[i] < dts[it]) it = i;
-    // driver.h:391
-    s->dt = smin(dts[it], final_time - s->t);
+    S.dt = smin(dts[it], P.final_time - S.t);
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u, const DevScalars *s,
-                                                       double factor) {
-  const int vt = blockIdx.x * blockDim.x + threadIdx.x;
-  if (vt >= m.nv || !s->active) return;
-  const double delta_t = factor * s->dt;
-  if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
-  for (int d = 0; d < 2; ++d) m.x[d][vt] = u.x0[d][vt] + delta_t * u.un[d][vt];
+__global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u, Batch b, double factor) {
+  for_each_tile(b.vt, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const double delta_t = factor * b.s[tile.prob].dt;
+    if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
+    const int vt = tile.first + threadIdx.x;
+    for (int d = 0; d < 2; ++d) m.x[d][vt] = u.x0[d][vt] + delta_t * u.un[d][vt];
+  });
 }
 
-__global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, DevScalars *s,
-                                                         double factor) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.nc || !s->active) return;
-  const double delta_t = factor * s->dt;
-  double vol;
-  if (delta_t > 1e-14) {
-    // burton update_geometry() for the moved cell
-    double px[4], py[4];
-    for (int i = 0; i < 4; ++i) {
-      const int v = m.cv[(size_t)i * m.nc + c];
-      px[i] = m.x[0][v];
-      py[i] = m.x[1][v];
+__global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, Batch b, double factor) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const ProbParams &P = b.pp[tile.prob];
+    const int c = tile.first + threadIdx.x;
+    const double delta_t = factor * b.s[tile.prob].dt;
+    double vol;
+    if (delta_t > 1e-14) {
+      // burton update_geometry() for the moved cell
+      double px[4], py[4];
+      for (int i = 0; i < 4; ++i) {
+        const int v = m.cv[(size_t)i * m.cap_c + c];
+        px[i] = m.x[0][v];
+        py[i] = m.x[1][v];
+      }
+      double cx, cy, ml;
+      cell_geom_2d(4, px, py, cx, cy, vol, ml);
+      m.ccen[0][c] = cx;
+      m.ccen[1][c] = cy;
+      m.cvol[c] = vol;
+      m.cminlen[c] = ml;
+    } else {
+      vol = m.cvol[c];
     }
-    double cx, cy, ml;
-    cell_geom_2d(4, px, py, cx, cy, vol, ml);
-    m.ccen[0][c] = cx;
-    m.ccen[1][c] = cy;
-    m.cvol[c] = vol;
-    m.cminlen[c] = ml;
-  } else {
-    vol = m.cvol[c];
-  }
-  // update_state_from_flux lagrange_eqns.h:310-343
-  const double M = u.M[c];
-  const double inv_mass = 1 / M;
-  double vel[2] = {u.v[0][c], u.v[1][c]};
-  const double et = u.e[c] + dotN<2>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[3][c];
-  for (int i = 0; i < 2; ++i) vel[i] += delta_t * inv_mass * u.dudt[1 + i][c];
-  const double ie = et - dotN<2>(vel, vel) / 2;
-  // update_volume :352-367
-  const double d = M / vol;
-  u.v[0][c] = vel[0];
-  u.v[1][c] = vel[1];
-  u.e[c] = ie;
-  u.V[c] = vol;
-  u.d[c] = d;
-  // update_state_from_energy lagrange_eqns.h:280-302
-  const double g = u.g[c];
-  u.p[c] = eos_p_de(g, d, ie);
-  u.T[c] = eos_T_de(u.cv, ie);
-  u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
-  if (ie < 0 || d < 0) {
-    atomicCAS(&s->err_code, 0, (int)FSALE_ERR_NEG_STATE);
-    atomicMin(reinterpret_cast<unsigned long long *>(&s->err_cell), (unsigned long long)c);
-    s->active = 0;
-  }
+    // update_state_from_flux lagrange_eqns.h:310-343
+    const double M = u.M[c];
+    const double inv_mass = 1 / M;
+    double vel[2] = {u.v[0][c], u.v[1][c]};
+    const double et = u.e[c] + dotN<2>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[3][c];
+    for (int i = 0; i < 2; ++i) vel[i] += delta_t * inv_mass * u.dudt[1 + i][c];
+    const double ie = et - dotN<2>(vel, vel) / 2;
+    // update_volume :352-367
+    const double d = M / vol;
+    u.v[0][c] = vel[0];
+    u.v[1][c] = vel[1];
+    u.e[c] = ie;
+    u.V[c] = vol;
+    u.d[c] = d;
+    // update_state_from_energy lagrange_eqns.h:280-302
+    const double g = u.g[c];
+    u.p[c] = eos_p_de(g, d, ie);
+    u.T[c] = eos_T_de(P.cv, ie);
+    u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
+    if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, c - P.cell_first);
+  });
 }
 
 // save (dir 0) / restore (dir 1) of coordinates, velocity and energy
-__global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u, const DevScalars *s, int dir) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (!s->active) return;
-  if (i < m.nv) {
+__global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u, Batch b, int dir) {
+  for_each_tile(b.vt, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const int i = tile.first + threadIdx.x;
     for (int d = 0; d < 2; ++d) {
       if (dir == 0)
         u.x0[d][i] = m.x[d][i];
       else
         m.x[d][i] = u.x0[d][i];
     }
-  }
-  if (i < m.nc) {
+  });
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!block_active(b.s, tile.prob)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const int i = tile.first + threadIdx.x;
     if (dir == 0) {
       u.v0[0][i] = u.v[0][i];
       u.v0[1][i] = u.v[1][i];
@@ -332,23 +343,43 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
       u.v[1][i] = u.v0[1][i];
       u.e[i] = u.e0[i];
     }
-  }
+  });
 }
 
 // step bracket of the driver loop (apps/maire_hydro/driver.h:332,514-515)
-__global__ void k_maire_begin(DevScalars *s, double final_time) {
-  s->active = (s->err_code == 0) && (s->t < final_time);
+__global__ void k_maire_begin(Batch b, int nprob) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
+    const ProbParams &P = b.pp[p];
+    if (P.cell_count == 0) continue;
+    DevScalars &S = b.s[p];
+    S.active = (S.err_code == 0) && (S.steps < P.max_steps) && (S.t < P.final_time);
+  }
 }
-__global__ void k_maire_end(DevScalars *s) {
-  if (!s->active) return;
-  s->t += s->dt;
-  s->steps = s->steps + 1;
+__global__ void k_maire_end(Batch b, int nprob) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
+    if (b.pp[p].cell_count == 0) continue;
+    DevScalars &S = b.s[p];
+    if (!S.active) continue;
+    S.t += S.dt;
+    S.steps = S.steps + 1;
+  }
 }
 
-inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
+template <class K, class... A>
+int launch_persistent(fsale_ctx *ctx, int kid, K kernel, A... args) {
+  ProfScope prof(ctx, kid);
+  hipLaunchKernelGGL(kernel, dim3(persistent_grid(ctx)), dim3(kBlock), 0, ctx->stream, args...);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
+int small_grid(int n) {
+  const int g = (n + kBlock - 1) / kBlock;
+  return g < 1 ? 1 : (g > 1024 ? 1024 : g);
+}
 
 bool check_ctx(fsale_ctx *ctx) {
-  if (!ctx || ctx->solver != FSALE_SOLVER_MAIRE) {
+  if (!ctx || ctx->solver != FSALE_SOLVER_MAIRE || !ctx->single || ctx->probs.empty() || !ctx->probs[0].live) {
     set_message("not a Lagrangian context");
     return false;
   }
@@ -356,65 +387,52 @@ bool check_ctx(fsale_ctx *ctx) {
 }
 
 int launch_nodal(fsale_ctx *ctx) {
-  ProfScope prof(ctx, K_MAIRE_NODAL);
-  hipLaunchKernelGGL(k_maire_nodal, dim3(grid(ctx->m.nv)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
+  return launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal, ctx->m, ctx->ma, batch_args(ctx));
 }
 int launch_residual(fsale_ctx *ctx) {
-  ProfScope prof(ctx, K_MAIRE_RESIDUAL);
-  hipLaunchKernelGGL(k_maire_residual, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
-                     ctx->s);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
+  return launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual, ctx->m, ctx->ma, batch_args(ctx));
 }
-int launch_dt(fsale_ctx *ctx, double ca, double cv, double cg, double tf) {
+int launch_dt(fsale_ctx *ctx) {
+  int rc = launch_persistent(ctx, K_MAIRE_DT, k_maire_dt_partial, ctx->m, ctx->ma, batch_args(ctx));
+  if (rc) return rc;
   ProfScope prof(ctx, K_MAIRE_DT);
-  hipLaunchKernelGGL(k_maire_dt, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s, ca,
-                     cv, cg, tf);
+  const int np = (int)ctx->probs.size();
+  hipLaunchKernelGGL(k_maire_dt_finalize, dim3(np < 1024 ? np : 1024), dim3(kBlock), 0, ctx->stream,
+                     batch_args(ctx), (const double *)ctx->ma.partials, np);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
 int launch_apply(fsale_ctx *ctx, double factor) {
-  {
-    ProfScope prof(ctx, K_MAIRE_MOVE);
-    hipLaunchKernelGGL(k_maire_move, dim3(grid(ctx->m.nv)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s,
-                       factor);
-    FSALE_HIP(hipGetLastError());
-  }
-  ProfScope prof(ctx, K_MAIRE_UPDATE);
-  hipLaunchKernelGGL(k_maire_update, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
-                     ctx->s, factor);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
+  int rc = launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move, ctx->m, ctx->ma, batch_args(ctx), factor);
+  if (rc) return rc;
+  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update, ctx->m, ctx->ma, batch_args(ctx), factor);
 }
 int launch_copy(fsale_ctx *ctx, int dir) {
-  ProfScope prof(ctx, K_MAIRE_COPY);
-  const int n = ctx->m.nv > ctx->m.nc ? ctx->m.nv : ctx->m.nc;
-  hipLaunchKernelGGL(k_maire_copy, dim3(grid(n)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, ctx->s, dir);
-  FSALE_HIP(hipGetLastError());
-  return FSALE_OK;
+  return launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy, ctx->m, ctx->ma, batch_args(ctx), dir);
 }
 
-int launch_step(fsale_ctx *ctx, double ca, double cv, double cg, double tf) {
+}  // namespace
+
+// One driver step (apps/maire_hydro/driver.h:330-538) of every live problem:
+// predictor half step, then the corrector from the saved state.
+int maire_step(fsale_ctx *ctx) {
   int rc;
-  hipLaunchKernelGGL(k_maire_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->s, tf);
+  const int np = (int)ctx->probs.size();
+  hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx), np);
   FSALE_HIP(hipGetLastError());
   if ((rc = launch_copy(ctx, 0))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx))) return rc;
-  if ((rc = launch_dt(ctx, ca, cv, cg, tf))) return rc;
+  if ((rc = launch_dt(ctx))) return rc;
   if ((rc = launch_apply(ctx, 0.5))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx))) return rc;
   if ((rc = launch_copy(ctx, 1))) return rc;
   if ((rc = launch_apply(ctx, 1.0))) return rc;
-  hipLaunchKernelGGL(k_maire_end, dim3(1), dim3(1), 0, ctx->stream, ctx->s);
+  hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx), np);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
-
-}  // namespace
 
 int maire_alloc(fsale_ctx *ctx) {
   const DevMesh &m = ctx->m;
@@ -422,19 +440,35 @@ int maire_alloc(fsale_ctx *ctx) {
   int rc;
   double **cell1[] = {&u.V, &u.M, &u.p, &u.d, &u.e, &u.T, &u.a, &u.g, &u.e0};
   for (double **p : cell1)
-    if ((rc = dev_alloc_n(ctx, p, m.nc))) return rc;
+    if ((rc = dev_alloc_n(ctx, p, m.cap_c))) return rc;
   for (int d = 0; d < 2; ++d) {
-    if ((rc = dev_alloc_n(ctx, &u.v[d], m.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.v0[d], m.nc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.x0[d], m.nv))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.un[d], m.nv))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.npc[d], (size_t)m.nc * m.vpc))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.Fpc[d], (size_t)m.nc * m.vpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.v[d], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.v0[d], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.x0[d], m.cap_v))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.un[d], m.cap_v))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.npc[d], (size_t)m.cap_c * m.vpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.Fpc[d], (size_t)m.cap_c * m.vpc))) return rc;
   }
   for (int i = 0; i < 4; ++i)
-    if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.nc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.cap_c))) return rc;
+  if ((rc = dev_alloc_n(ctx, &u.partials, 2 * ((size_t)ctx->ra_ct.capacity + 1)))) return rc;
   return FSALE_OK;
 }
+
+int maire_initial_state(fsale_ctx *ctx, int prob) {
+  const Problem &P = ctx->probs[prob];
+  const int g = (P.nc + kBlock - 1) / kBlock;
+  if (g == 0) return FSALE_OK;
+  hipLaunchKernelGGL(k_maire_ic, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
+                     (const ProbParams *)ctx->d_params, prob);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
+int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in);
+int set_problem_state(fsale_ctx *ctx, int prob, const double *d, const double *v, const double *p);
+int reset_scalars(fsale_ctx *ctx, int prob, double t, double dt);
+int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph);
 
 }  // namespace fsale
 
@@ -445,109 +479,84 @@ extern "C" {
 int fsale_maire_initial_conditions(fsale_ctx *ctx, double cv, const double *gamma, const double *d,
                                    const double *v, const double *p) {
   if (!check_ctx(ctx) || !gamma || !d || !v || !p || !(cv > 0)) return FSALE_ERR_ARG;
-  for (int c = 0; c < ctx->m.nc; ++c)
+  Problem &P = ctx->probs[0];
+  for (int c = 0; c < P.nc; ++c)
     if (!(gamma[c] > 1)) return FSALE_ERR_ARG;
+  P.params.cv = cv;
   int rc;
-  if ((rc = fsale_set_field(ctx, FSALE_F_GAMMA, gamma))) return rc;
-  if ((rc = fsale_set_field(ctx, FSALE_F_DENSITY, d))) return rc;
-  if ((rc = fsale_set_field(ctx, FSALE_F_VELOCITY, v))) return rc;
-  if ((rc = fsale_set_field(ctx, FSALE_F_PRESSURE, p))) return rc;
-  ctx->ma.cv = cv;
-  hipLaunchKernelGGL(k_maire_ic, dim3(grid(ctx->m.nc)), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma);
-  FSALE_HIP(hipGetLastError());
-  ctx->have_state = true;
-  if ((rc = read_scalars(ctx))) return rc;
-  DevScalars h = *ctx->h_s;
-  h.t = 0;
-  h.dt = 0;
-  h.steps = 0;
-  h.err_code = 0;
-  h.err_cell = INT64_MAX;
-  h.active = 1;
-  h.red[0] = h.red[1] = 0;
-  h.done = 0;
-  *ctx->h_s = h;
-  FSALE_HIP(hipMemcpyAsync(ctx->s, ctx->h_s, sizeof(DevScalars), hipMemcpyHostToDevice, ctx->stream));
-  FSALE_HIP(hipStreamSynchronize(ctx->stream));
-  return FSALE_OK;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  if ((rc = set_problem_field(ctx, 0, FSALE_F_GAMMA, gamma))) return rc;
+  if ((rc = set_problem_state(ctx, 0, d, v, p))) return rc;
+  if ((rc = maire_initial_state(ctx, 0))) return rc;
+  P.have_state = true;
+  return reset_scalars(ctx, 0, 0.0, 0.0);
 }
 
 int fsale_maire_evaluate_nodal_state(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  return launch_nodal(ctx);
+  if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
+  int rc = sync_tiles(ctx);
+  return rc ? rc : launch_nodal(ctx);
 }
 
 int fsale_maire_evaluate_residual(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  return launch_residual(ctx);
+  if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
+  int rc = sync_tiles(ctx);
+  return rc ? rc : launch_residual(ctx);
 }
 
 int fsale_maire_evaluate_time_step(fsale_ctx *ctx, double cfl_accoustic, double cfl_volume, double cfl_growth,
                                    double final_time, double *dt_out) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  int rc = launch_dt(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time);
-  if (rc || !dt_out) return rc;
-  if ((rc = check_device_error(ctx))) return rc;
-  *dt_out = ctx->h_s->dt;
+  Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  P.params.cfl_acc = cfl_accoustic;
+  P.params.cfl_vol = cfl_volume;
+  P.params.cfl_growth = cfl_growth;
+  P.params.final_time = final_time;
+  int rc;
+  if ((rc = upload_params(ctx, 0)) || (rc = sync_tiles(ctx)) || (rc = launch_dt(ctx))) return rc;
+  if (!dt_out) return FSALE_OK;
+  if ((rc = check_device_error(ctx, 0))) return rc;
+  *dt_out = ctx->h_s[0].dt;
   return FSALE_OK;
 }
 
 int fsale_maire_apply_update(fsale_ctx *ctx, double factor) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
-  return launch_apply(ctx, factor);
+  if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
+  int rc = sync_tiles(ctx);
+  return rc ? rc : launch_apply(ctx, factor);
 }
 
 int fsale_maire_save(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  return launch_copy(ctx, 0);
+  int rc = sync_tiles(ctx);
+  return rc ? rc : launch_copy(ctx, 0);
 }
 
 int fsale_maire_restore(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
-  return launch_copy(ctx, 1);
+  int rc = sync_tiles(ctx);
+  return rc ? rc : launch_copy(ctx, 1);
 }
 
 int fsale_maire_advance(fsale_ctx *ctx, double cfl_accoustic, double cfl_volume, double cfl_growth,
                         double final_time, int nsteps, int use_graph) {
   if (!check_ctx(ctx) || nsteps < 0) return FSALE_ERR_ARG;
-  if (!ctx->have_state) return FSALE_ERR_STATE;
+  Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  if (nsteps == 0) return FSALE_OK;
   int rc;
-  if (!use_graph || nsteps < 2 || ctx->profiling) {
-    for (int n = 0; n < nsteps; ++n)
-      if ((rc = launch_step(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time))) return rc;
-    return FSALE_OK;
-  }
-  const int chunk = nsteps < 10 ? nsteps : 10;
-  std::vector<double> key = {1.0, cfl_accoustic, cfl_volume, cfl_growth, final_time};
-  if (!ctx->graph.exec || ctx->graph.key != key || ctx->graph.steps != chunk) {
-    if (ctx->graph.exec) {
-      (void)hipGraphExecDestroy(ctx->graph.exec);
-      ctx->graph.exec = nullptr;
-    }
-    hipGraph_t graph;
-    FSALE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int n = 0; n < chunk; ++n) {
-      rc = launch_step(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time);
-      if (rc) {
-        (void)hipStreamEndCapture(ctx->stream, &graph);
-        return rc;
-      }
-    }
-    FSALE_HIP(hipStreamEndCapture(ctx->stream, &graph));
-    FSALE_HIP(hipGraphInstantiate(&ctx->graph.exec, graph, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(graph);
-    ctx->graph.key = key;
-    ctx->graph.steps = chunk;
-  }
-  int n = 0;
-  for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(ctx->graph.exec, ctx->stream));
-  for (; n < nsteps; ++n)
-    if ((rc = launch_step(ctx, cfl_accoustic, cfl_volume, cfl_growth, final_time))) return rc;
-  return FSALE_OK;
+  if ((rc = read_scalars(ctx))) return rc;
+  P.params.cfl_acc = cfl_accoustic;
+  P.params.cfl_vol = cfl_volume;
+  P.params.cfl_growth = cfl_growth;
+  P.params.final_time = final_time;
+  P.params.max_steps = ctx->h_s[0].steps + nsteps;
+  if ((rc = upload_params(ctx, 0))) return rc;
+  return batch_steps(ctx, nsteps, use_graph);
 }
 
 }  // extern "C"

@@ -208,6 +208,49 @@ def maire_bubble_2d(n=512, steps=500, seed=MASTER_SEED):
                 final_time=1.0, max_steps=steps, dt0=1.e-5)
 
 
+def riemann_2d(side, solver, seed=MASTER_SEED, index=0, steps=100):
+    """Two-state Riemann problem on a side x side quad mesh of [0,1]^2.
+
+    The split line passes through the box centre at angle theta ~ U[0, pi);
+    each side draws (rho, p) ~ U[0.1, 1] and a velocity u ~ U[-1, 1] along the
+    split normal; gamma is 1.4 or 1.67.  Draws come from substream
+    1000 + index, so problem `index` of a stream is reproducible alone.
+    solver: "euler" (MUSCL, HLLE, walls, CFL 1/2) or "maire" (CFL
+    {0.25, 0.1, 1.01}, dt0 1e-5, symmetry boundaries)."""
+    r = uniform(seed, 1000 + index, 8)
+    theta = math.pi * r[0]
+    n = np.array([math.cos(theta), math.sin(theta)])
+    rho = 0.1 + 0.9 * r[1:3]
+    pre = 0.1 + 0.9 * r[3:5]
+    un = -1.0 + 2.0 * r[5:7]
+    gamma = 1.4 if r[7] < 0.5 else 1.67
+    coords, cells = box_quads(side, side)
+    x = quad_centroids(coords, cells)
+    right = ((x[:, 0] - 0.5) * n[0] + (x[:, 1] - 0.5) * n[1]) >= 0
+    k = right.astype(np.int64)
+    d = rho[k]
+    p = pre[k]
+    v = un[k][:, None] * n[None, :]
+    nc = len(cells)
+    if solver == "euler":
+        return dict(coords=coords, cells=cells, eos=(gamma, 1.0), d=d, v=v, p=p, cfl=0.5, final_time=10.0,
+                    max_steps=steps, flux=0)
+    return dict(coords=coords, cells=cells, cv=1.0, gamma=np.full(nc, gamma), d=d, v=v, p=p,
+                cfl=(0.25, 0.1, 1.01), final_time=10.0, max_steps=steps, dt0=1.e-5)
+
+
+def stream_sides(count=4096, seed=MASTER_SEED, lo=32, hi=512):
+    """Mesh sides of the mixed-size request stream: round(exp(U[ln lo, ln hi]))."""
+    u = uniform(seed, 999, count)
+    return np.rint(np.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * u)).astype(np.int64)
+
+
+def stream_arrivals(count, rate, seed=MASTER_SEED):
+    """Poisson arrival times (s) at `rate` requests/s: exponential gaps."""
+    u = uniform(seed, 998, count)
+    return np.cumsum(-np.log1p(-u) / rate)
+
+
 # ------------------------------------------------------------ byte models
 def euler_bytes_per_step(nc, nf, nv, dim):
     """Algorithmic (task-dataflow compulsory) HBM bytes of one Eulerian step."""

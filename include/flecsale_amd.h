@@ -189,6 +189,77 @@ int fsale_maire_restore(fsale_ctx *ctx);
 int fsale_maire_advance(fsale_ctx *ctx, double cfl_accoustic, double cfl_volume,
                         double cfl_growth, double final_time, int nsteps, int use_graph);
 
+/* ------------------------------------------------------- ragged batches */
+/* Many independent problems (each its own mesh, EOS, CFL and clock) resident
+ * in one context and stepped together by the same persistent kernels.  The
+ * reference runs one problem per MPI job (apps/hydro/driver.h:237-420,
+ * apps/maire_hydro/driver.h:330-538 are its step loops); a batch runs each
+ * resident problem through exactly that loop, with the same arithmetic as a
+ * context of its own: a problem's results do not depend on what else is
+ * resident.  All problems of a batch share the solver, dimension and cell
+ * type.  Problems are admitted and retired between steps; the capacities fix
+ * the device footprint at creation. */
+
+/* Per-problem driver parameters of the Eulerian loop (apps/hydro/driver.h
+ * inputs: CFL, final_time, max_steps; flux_kind as fsale_eul_advance). */
+typedef struct {
+  double cfl;
+  double final_time;
+  int32_t flux_kind;
+  int32_t first_order;
+  int32_t stop_at_final;
+  int64_t max_steps;
+} fsale_euler_params;
+
+/* Per-problem parameters of the Lagrangian loop (time_constants_t of
+ * apps/maire_hydro/tasks.h:142-190, initial_time_step and final_time of
+ * apps/maire_hydro/driver.h:330-338). */
+typedef struct {
+  double cfl_accoustic;
+  double cfl_volume;
+  double cfl_growth;
+  double final_time;
+  double initial_time_step;
+  int64_t max_steps;
+} fsale_maire_params;
+
+typedef struct {
+  int32_t problem;
+  int32_t error;             /* fsale_status latched on the device             */
+  int64_t steps;             /* completed steps                                */
+  double time;               /* solution time                                  */
+  double time_step;          /* last time step                                 */
+  int32_t done;              /* max_steps reached, final time reached (Lagrangian,
+                                or stop_at_final) or error                     */
+} fsale_problem_status;
+
+/* An empty batch.  Capacities bound the sum over resident problems of cells,
+ * faces, vertices and cell-vertex neighbour entries. */
+int fsale_batch_create(int solver, int dim, int verts_per_cell, int32_t max_problems,
+                       int64_t cell_capacity, int64_t face_capacity, int64_t vertex_capacity,
+                       int64_t neighbour_capacity, fsale_ctx **out);
+/* Admit a problem: connectivity, geometry and the initial_conditions task of
+ * the solver; *problem receives its id.  FSALE_ERR_OOM when it does not fit. */
+int fsale_batch_add_euler(fsale_ctx *ctx, const fsale_mesh_desc *mesh, fsale_eos eos,
+                          const fsale_euler_params *params, const double *d, const double *v,
+                          const double *p, int32_t *problem);
+int fsale_batch_add_maire(fsale_ctx *ctx, const fsale_mesh_desc *mesh, double cv, const double *gamma,
+                          const fsale_maire_params *params, const double *d, const double *v,
+                          const double *p, int32_t *problem);
+/* Retire a problem and release its ranges (synchronises the stream). */
+int fsale_batch_remove(fsale_ctx *ctx, int32_t problem);
+/* nsteps driver steps of every resident problem that is not done; captured
+ * as hipGraph chunks when use_graph != 0. */
+int fsale_batch_step(fsale_ctx *ctx, int nsteps, int use_graph);
+/* Status of up to max_out resident problems in id order. */
+int fsale_batch_status(fsale_ctx *ctx, int32_t max_out, fsale_problem_status *out, int32_t *count);
+int fsale_batch_get_field(fsale_ctx *ctx, int32_t problem, int field, double *host_out);
+/* Free capacity (sums; a range still has to fit one contiguous gap). */
+int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_faces,
+                         int64_t *free_vertices, int64_t *free_neighbours, int32_t *free_problems);
+/* Persistent-kernel launch shape: blocks per compute unit (1-32). */
+int fsale_launch_shape(fsale_ctx *ctx, int32_t blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif
