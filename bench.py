@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--size", type=int, default=None, help="override mesh size (cells per side)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--shape", type=int, default=None,
+                    help="blocks per CU for every kernel (0 = one block per tile); default: autotune")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -168,6 +170,15 @@ def main():
     log(f"[bench] context (connectivity + upload + geometry) {time.perf_counter() - t0:.1f}s")
 
     reset()
+    if args.shape is None:
+        run(1, graph=False)      # sets the step parameters the tuner runs with
+        t0 = time.perf_counter()
+        tuned_ms = ctx.autotune(3)
+        log(f"[bench] autotuned launch shapes in {time.perf_counter() - t0:.1f}s: "
+            f"{json.dumps(ctx.get_launch_shape())} (kernel time {tuned_ms:.3f} ms/step)")
+        reset()
+    else:
+        ctx.launch_shape(args.shape)
     if W:
         run(W)
     ctx.sync()
@@ -227,6 +238,7 @@ def main():
         cpu = cpu_baseline(prob, kind)
         log(f"[bench] cpu baseline {cpu['value']:.4e} zone-updates/s ({cpu['sample']})")
 
+    ctx_shape = ctx.get_launch_shape()
     ctx.close()
     out = {
         "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
@@ -247,6 +259,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "launch_shape": ctx_shape,
     }
     print(json.dumps(out), flush=True)
 

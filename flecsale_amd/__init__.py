@@ -55,7 +55,7 @@ EXPORTS = [
     "fsale_maire_advance",
     "fsale_batch_create", "fsale_batch_add_euler", "fsale_batch_add_maire", "fsale_batch_remove",
     "fsale_batch_step", "fsale_batch_status", "fsale_batch_get_field", "fsale_batch_capacity",
-    "fsale_launch_shape",
+    "fsale_launch_shape", "fsale_get_launch_shape", "fsale_autotune",
 ]
 
 
@@ -143,7 +143,9 @@ def lib():
         L.fsale_batch_get_field.argtypes = [vp, i32, ct.c_int, vp]
         L.fsale_batch_capacity.argtypes = [vp, ct.POINTER(i64), ct.POINTER(i64), ct.POINTER(i64),
                                            ct.POINTER(i64), ct.POINTER(i32)]
-        L.fsale_launch_shape.argtypes = [vp, i32]
+        L.fsale_launch_shape.argtypes = [vp, i32, i32]
+        L.fsale_get_launch_shape.argtypes = [vp, vp]
+        L.fsale_autotune.argtypes = [vp, i32, ct.POINTER(dp)]
         for name in EXPORTS:
             f = getattr(L, name)
             if f.restype is ct.c_int or f.restype is None:
@@ -267,6 +269,21 @@ class _Context:
 
     def reset_profile(self):
         _check(lib().fsale_reset_profile(self.h))
+
+    def launch_shape(self, blocks_per_cu, kernel=-1):
+        """Blocks per CU of one kernel id (-1: all); 0 = one block per tile."""
+        _check(lib().fsale_launch_shape(self.h, int(kernel), int(blocks_per_cu)))
+
+    def get_launch_shape(self):
+        out = np.zeros(len(KERNEL_NAMES), np.int32)
+        _check(lib().fsale_get_launch_shape(self.h, out.ctypes.data))
+        return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
+
+    def autotune(self, nsteps=3):
+        """Pick each kernel's launch shape by timing real steps; returns the tuned step's kernel ms."""
+        ms = ct.c_double()
+        _check(lib().fsale_autotune(self.h, int(nsteps), ct.byref(ms)))
+        return ms.value
 
     def topology(self):
         fpc = self.vpc if self.dim == 2 else 6
@@ -462,5 +479,18 @@ class Batch:
                                           ct.byref(p)))
         return dict(cells=c.value, faces=f.value, vertices=v.value, neighbours=n.value, problems=p.value)
 
-    def launch_shape(self, blocks_per_cu):
-        _check(lib().fsale_launch_shape(self.h, int(blocks_per_cu)))
+
+    def launch_shape(self, blocks_per_cu, kernel=-1):
+        """Blocks per CU of one kernel id (-1: all); 0 = one block per tile."""
+        _check(lib().fsale_launch_shape(self.h, int(kernel), int(blocks_per_cu)))
+
+    def get_launch_shape(self):
+        out = np.zeros(len(KERNEL_NAMES), np.int32)
+        _check(lib().fsale_get_launch_shape(self.h, out.ctypes.data))
+        return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
+
+    def autotune(self, nsteps=3):
+        """Pick each kernel's launch shape by timing real steps; returns the tuned step's kernel ms."""
+        ms = ct.c_double()
+        _check(lib().fsale_autotune(self.h, int(nsteps), ct.byref(ms)))
+        return ms.value

@@ -118,6 +118,7 @@ int reset_scalars(fsale_ctx *ctx, int prob, double t, double dt) {
   h.err_code = 0;
   h.err_cell = INT64_MAX;
   h.steps = 0;
+  h.err_token = 0;
   ctx->h_s[prob] = h;
   FSALE_HIP(hipMemcpyAsync(ctx->s + prob, &ctx->h_s[prob], sizeof(DevScalars), hipMemcpyHostToDevice,
                            ctx->stream));
@@ -125,8 +126,38 @@ int reset_scalars(fsale_ctx *ctx, int prob, double t, double dt) {
   return FSALE_OK;
 }
 
+namespace {
+
+// Grid of a one-block-per-tile launch over `live` tiles: a power of two (so a
+// captured graph stays valid while the batch changes within a factor of
+// two), at least 8, at most the tile capacity rounded up to 8.
+int direct_grid(int live, int cap_tiles) {
+  long long g = 8;
+  while (g < live) g *= 2;
+  const long long cap = ((long long)(cap_tiles + 7) / 8) * 8;
+  if (g > cap) g = cap;
+  return (int)std::max(8LL, g);
+}
+
+// Dispatch list of a one-block-per-tile launch with grid G: block b runs on
+// XCD b % 8 and gets tile (b % 8) * chunk + b / 8, so each XCD works on one
+// contiguous range of the list; blocks without a tile get count 0.
+void build_direct(const std::vector<Tile> &tiles, int G, std::vector<Tile> &out) {
+  const int T = (int)tiles.size();
+  const int chunk = (T + 7) / 8;
+  out.assign(G, Tile{0, 0, 0, 0});
+  for (int b = 0; b < G; ++b) {
+    const int x = b & 7, i = b >> 3;
+    const int t = x * chunk + i;
+    if (i < chunk && t < T) out[b] = tiles[t];
+  }
+}
+
+}  // namespace
+
 // Tile lists of the live problems, problems in id order, each problem's
-// tiles contiguous.  Uploaded only when the set of problems changed.
+// tiles contiguous, plus their one-block-per-tile dispatch lists.  Uploaded
+// only when the set of problems changed.
 int sync_tiles(fsale_ctx *ctx) {
   if (!ctx->tiles_dirty) return FSALE_OK;
   ctx->h_ctiles.clear();
@@ -147,31 +178,59 @@ int sync_tiles(fsale_ctx *ctx) {
     FSALE_HIP(hipMemcpyAsync(ctx->d_vtiles, ctx->h_vtiles.data(), sizeof(Tile) * ctx->h_vtiles.size(),
                              hipMemcpyHostToDevice, ctx->stream));
   FSALE_HIP(hipMemcpyAsync(ctx->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<Tile> dc, dv;
+  ctx->n_cdirect = direct_grid(counts[0], ctx->ra_ct.capacity);
+  build_direct(ctx->h_ctiles, ctx->n_cdirect, dc);
+  FSALE_HIP(hipMemcpyAsync(ctx->d_cdirect, dc.data(), sizeof(Tile) * dc.size(), hipMemcpyHostToDevice, ctx->stream));
+  if (ctx->d_vdirect) {
+    ctx->n_vdirect = direct_grid(counts[1], ctx->ra_vt.capacity);
+    build_direct(ctx->h_vtiles, ctx->n_vdirect, dv);
+    FSALE_HIP(hipMemcpyAsync(ctx->d_vdirect, dv.data(), sizeof(Tile) * dv.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+  }
   FSALE_HIP(hipStreamSynchronize(ctx->stream));
   ctx->tiles_dirty = false;
   return FSALE_OK;
 }
 
-Batch batch_args(const fsale_ctx *ctx) {
+// Arguments of a launch of kernel `kid`: one-block-per-tile kernels (shape
+// 0) read the dispatch lists, persistent ones loop over the tile lists.
+Batch batch_args(const fsale_ctx *ctx, int kid) {
   Batch b;
   b.ct.tiles = ctx->d_ctiles;
   b.ct.n = ctx->d_counts;
   b.vt.tiles = ctx->d_vtiles;
   b.vt.n = ctx->d_counts + 1;
+  if (kid >= 0 && kid < K_COUNT && ctx->shape.blocks_per_cu[kid] <= 0) {
+    b.ct.direct = ctx->d_cdirect;
+    b.ct.ndirect = ctx->n_cdirect;
+    b.vt.direct = ctx->d_vdirect;
+    b.vt.ndirect = ctx->d_vdirect ? ctx->n_vdirect : 0;
+  }
   b.pp = ctx->d_params;
   b.s = ctx->s;
   b.cap_c_tiles = ctx->ra_ct.capacity;
   return b;
 }
 
-// Persistent grid: CUs x blocks_per_cu blocks, a multiple of 8 (one slice
-// per XCD), never more than the tile capacity can use.
-int persistent_grid(const fsale_ctx *ctx) {
-  int g = ctx->num_cus * ctx->shape.blocks_per_cu;
-  const int tiles = std::max(ctx->ra_ct.capacity, ctx->ra_vt.capacity);
-  const int cap = ((tiles + 7) / 8) * 8;
-  if (cap > 0 && g > cap) g = cap;
-  return std::max(8, (g / 8) * 8);
+// Grid of a tile-list kernel, a multiple of 8 (one slice per XCD): shape 0
+// is one block per tile (the dispatch lists' grid), shape n a persistent
+// grid of CUs x n blocks, never more blocks than the capacity has tiles.
+int persistent_grid(const fsale_ctx *ctx, int kid) {
+  const bool vertex_list = kid == K_MAIRE_NODAL || kid == K_MAIRE_MOVE;
+  const bool both = kid == K_MAIRE_COPY;
+  const int bpc = (kid >= 0 && kid < K_COUNT) ? ctx->shape.blocks_per_cu[kid] : 0;
+  if (bpc <= 0) {
+    const int g = vertex_list ? ctx->n_vdirect : both ? std::max(ctx->n_cdirect, ctx->n_vdirect) : ctx->n_cdirect;
+    return std::max(8, g);
+  }
+  const int cap_tiles = vertex_list ? ctx->ra_vt.capacity
+                        : both      ? std::max(ctx->ra_ct.capacity, ctx->ra_vt.capacity)
+                                    : ctx->ra_ct.capacity;
+  long long g = (long long)ctx->num_cus * bpc;
+  const long long cap = ((long long)(cap_tiles + 7) / 8) * 8;
+  if (g > cap) g = cap;
+  return (int)std::max(8LL, (g / 8) * 8);
 }
 
 namespace {
@@ -239,6 +298,9 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if ((rc = dev_alloc_n(ctx, &m.nbr_start, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.nbr_cnt, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.nbr, cap_n + 1))) return rc;
+  m.nbk_deg = dim == 2 ? kNbk2 : kNbk3;
+  if ((rc = dev_alloc_n(ctx, &m.nbk, (size_t)cap_c * m.nbk_deg))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.cfn, (size_t)cap_c * m.fpc))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cvol, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cminlen, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.farea, cap_f))) return rc;
@@ -251,6 +313,9 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   }
   if ((rc = dev_alloc_n(ctx, &ctx->d_ctiles, cap_ct))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_vtiles, cap_vt))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->d_cdirect, ((size_t)cap_ct + 7) / 8 * 8))) return rc;
+  if (solver == FSALE_SOLVER_MAIRE)
+    if ((rc = dev_alloc_n(ctx, &ctx->d_vdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 2))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_params, max_problems))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->s, max_problems))) return rc;
@@ -363,6 +428,13 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob) {
       buf[c] = 2 * (f + P.f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
     }
     if ((rc = copy_h2d(ctx, m.cf + (size_t)k * m.cap_c + P.c0, buf.data(), nc))) return rc;
+    for (int c = 0; c < nc; ++c) {
+      const int f = t.cf[(size_t)c * t.fpc + k];
+      const bool second = t.fc[2 * (size_t)f] != c;
+      const int other = second ? t.fc[2 * (size_t)f] : t.fc[2 * (size_t)f + 1];
+      buf[c] = (other >= 0 ? other + P.c0 + 1 : 0) * 2 + (second ? 1 : 0);
+    }
+    if ((rc = copy_h2d(ctx, m.cfn + (size_t)k * m.cap_c + P.c0, buf.data(), nc))) return rc;
   }
   for (int k = 0; k < 2; ++k) {
     buf.resize(nf);
@@ -387,6 +459,10 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob) {
     if ((rc = copy_h2d(ctx, m.nbr_start + P.c0, st.data(), nc))) return rc;
     if ((rc = copy_h2d(ctx, m.nbr_cnt + P.c0, cnt.data(), nc))) return rc;
     if ((rc = copy_h2d(ctx, m.nbr + P.n0, nb.data(), P.nn))) return rc;
+    for (int k = 0; k < m.nbk_deg; ++k) {
+      for (int c = 0; c < nc; ++c) st[c] = k < cnt[c] ? nb[t.nbr_off[c] + k] : -1;
+      if ((rc = copy_h2d(ctx, m.nbk + (size_t)k * m.cap_c + P.c0, st.data(), nc))) return rc;
+    }
   }
   for (int d = 0; d < m.dim; ++d) {
     std::vector<double> xd(nv);
@@ -629,7 +705,8 @@ int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
     return FSALE_OK;
   }
   const int chunk = nsteps < 10 ? nsteps : 10;
-  const std::vector<double> key = {(double)ctx->solver, (double)chunk, (double)persistent_grid(ctx)};
+  std::vector<double> key = {(double)ctx->solver, (double)chunk};
+  for (int k = 0; k < K_COUNT; ++k) key.push_back((double)persistent_grid(ctx, k));
   if (!ctx->graph.exec || ctx->graph.key != key) {
     if (ctx->graph.exec) {
       (void)hipGraphExecDestroy(ctx->graph.exec);
@@ -653,6 +730,127 @@ int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(ctx->graph.exec, ctx->stream));
   for (; n < nsteps; ++n)
     if ((rc = step())) return rc;
+  return FSALE_OK;
+}
+
+namespace {
+
+// Device arrays that carry a batch from one step to the next (everything
+// else a step writes is rewritten before it is read): (pointer, bytes).
+void step_state(fsale_ctx *ctx, std::vector<std::pair<void *, size_t>> &out) {
+  const DevMesh &m = ctx->m;
+  const size_t C = (size_t)m.cap_c * sizeof(double), V = (size_t)m.cap_v * sizeof(double);
+  if (ctx->solver == FSALE_SOLVER_EULER) {
+    EulerFields &u = ctx->eu;
+    for (double *p : {u.d, u.e, u.p, u.T, u.a}) out.push_back({p, C});
+    for (int d = 0; d < m.dim; ++d) out.push_back({u.v[d], C});
+    out.push_back({u.partials, sizeof(double) * kWaves * ((size_t)ctx->ra_ct.capacity + 1)});
+  } else {
+    MaireFields &u = ctx->ma;
+    for (double *p : {u.V, u.p, u.d, u.e, u.T, u.a, m.cvol, m.cminlen}) out.push_back({p, C});
+    for (int d = 0; d < 2; ++d) {
+      out.push_back({u.v[d], C});
+      out.push_back({m.ccen[d], C});
+      out.push_back({m.x[d], V});
+    }
+  }
+  out.push_back({ctx->s, sizeof(DevScalars) * ctx->max_problems});
+}
+
+struct Snapshot {
+  std::vector<std::pair<void *, size_t>> live;
+  std::vector<void *> copy;
+  ~Snapshot() {
+    for (void *p : copy)
+      if (p) (void)hipFree(p);
+  }
+  int take(fsale_ctx *ctx) {
+    step_state(ctx, live);
+    copy.assign(live.size(), nullptr);
+    for (size_t i = 0; i < live.size(); ++i) {
+      FSALE_HIP(hipMalloc(&copy[i], live[i].second));
+      FSALE_HIP(hipMemcpyAsync(copy[i], live[i].first, live[i].second, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return FSALE_OK;
+  }
+  int restore(fsale_ctx *ctx) {
+    for (size_t i = 0; i < live.size(); ++i)
+      FSALE_HIP(hipMemcpyAsync(live[i].first, copy[i], live[i].second, hipMemcpyDeviceToDevice, ctx->stream));
+    return FSALE_OK;
+  }
+};
+
+}  // namespace
+
+// Launch-shape autotuner: every candidate shape runs `nsteps` real eager
+// steps of the resident batch from the same snapshot, each launch timed
+// with HIP events; each kernel keeps its fastest shape.  The batch's state,
+// step budgets and profiling counters are left as they were.
+int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
+  int rc;
+  if ((rc = sync_tiles(ctx))) return rc;
+  if (ctx->h_ctiles.empty()) return FSALE_OK;
+  if (ctx->solver == FSALE_SOLVER_EULER) {
+    if ((rc = euler_dt_partials(ctx))) return rc;
+    for (auto &P : ctx->probs)
+      if (P.live) P.partials_valid = true;
+  }
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  Snapshot snap;
+  if ((rc = snap.take(ctx))) return rc;
+  // no problem runs out of step budget while the candidates are timed
+  for (int p = 0; p < ctx->max_problems; ++p)
+    if (ctx->probs[p].live) {
+      ProbParams pp = ctx->probs[p].params;
+      pp.max_steps = INT64_MAX;
+      FSALE_HIP(hipMemcpyAsync(ctx->d_params + p, &pp, sizeof(pp), hipMemcpyHostToDevice, ctx->stream));
+      FSALE_HIP(hipStreamSynchronize(ctx->stream));
+    }
+  ProfSlot saved[K_COUNT];
+  for (int k = 0; k < K_COUNT; ++k) std::swap(saved[k], ctx->prof[k]);
+  const bool was_profiling = ctx->profiling;
+  ctx->profiling = true;
+  const int cands[] = {0, 1, 2, 3, 4, 6, 8, 12, 16};
+  double best[K_COUNT];
+  long long per_step[K_COUNT] = {};
+  LaunchShape choice = ctx->shape;
+  for (int k = 0; k < K_COUNT; ++k) best[k] = 1e300;
+  rc = FSALE_OK;
+  for (int c : cands) {
+    for (int k = 0; k < K_COUNT; ++k) ctx->shape.blocks_per_cu[k] = c;
+    if ((rc = snap.restore(ctx))) break;
+    for (int n = 0; n < nsteps && !rc; ++n)
+      rc = ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx);
+    if (rc) break;
+    for (int k = 0; k < K_COUNT; ++k) {
+      double tot = 0;
+      int64_t nl = 0;
+      if ((rc = fsale_get_profile(ctx, k, &tot, &nl))) break;
+      if (nl > 0 && tot / nl < best[k]) {
+        best[k] = tot / nl;
+        per_step[k] = nl / nsteps;
+        choice.blocks_per_cu[k] = c;
+      }
+      ctx->prof[k].total_ms = 0;
+      ctx->prof[k].launches = 0;
+    }
+    if (rc) break;
+  }
+  ctx->profiling = was_profiling;
+  for (int k = 0; k < K_COUNT; ++k) std::swap(saved[k], ctx->prof[k]);
+  int rc2 = snap.restore(ctx);
+  for (int p = 0; p < ctx->max_problems && !rc2; ++p)
+    if (ctx->probs[p].live) rc2 = upload_params(ctx, p);
+  if (rc2 == FSALE_OK) rc2 = read_scalars(ctx);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  ctx->shape = choice;
+  if (best_ms) {
+    double t = 0;
+    for (int k = 0; k < K_COUNT; ++k)
+      if (per_step[k]) t += best[k] * per_step[k];
+    *best_ms = t;
+  }
   return FSALE_OK;
 }
 
@@ -1024,10 +1222,24 @@ int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_face
   return FSALE_OK;
 }
 
-int fsale_launch_shape(fsale_ctx *ctx, int32_t blocks_per_cu) {
-  if (!ctx || blocks_per_cu < 1 || blocks_per_cu > 32) return FSALE_ERR_ARG;
-  ctx->shape.blocks_per_cu = blocks_per_cu;
+int fsale_launch_shape(fsale_ctx *ctx, int32_t kernel, int32_t blocks_per_cu) {
+  if (!ctx || kernel < -1 || kernel >= K_COUNT || blocks_per_cu < 0 || blocks_per_cu > 32) return FSALE_ERR_ARG;
+  for (int k = 0; k < K_COUNT; ++k)
+    if (kernel < 0 || k == kernel) ctx->shape.blocks_per_cu[k] = blocks_per_cu;
   return FSALE_OK;
+}
+
+int fsale_get_launch_shape(fsale_ctx *ctx, int32_t *blocks_per_cu) {
+  if (!ctx || !blocks_per_cu) return FSALE_ERR_ARG;
+  for (int k = 0; k < K_COUNT; ++k) blocks_per_cu[k] = ctx->shape.blocks_per_cu[k];
+  return FSALE_OK;
+}
+
+int fsale_autotune(fsale_ctx *ctx, int32_t nsteps, double *best_ms) {
+  if (!ctx || nsteps < 1 || nsteps > 1000) return FSALE_ERR_ARG;
+  for (auto &P : ctx->probs)
+    if (P.live && !P.have_state) return FSALE_ERR_STATE;
+  return autotune(ctx, nsteps, best_ms);
 }
 
 }  // extern "C"

@@ -1,4 +1,4 @@
-// Device helpers for the persistent, tile-driven kernels of a ragged batch.
+// Device helpers for the tile-driven kernels of a ragged batch.
 #pragma once
 
 #include "internal.h"
@@ -6,13 +6,22 @@
 
 namespace fsale {
 
-// Persistent loop over a device tile list.  The hardware deals blocks
-// round-robin to the 8 XCDs; XCD x = blockIdx % 8 takes tiles
-// [x*chunk, (x+1)*chunk) so each XCD (and its own L2) works on one contiguous
-// range of the batch and neighbour gathers stay in that L2.  `body(tile)` is
-// called uniformly by the whole block (it may use __syncthreads).
+// Iterates a block over its tiles.  A direct list (one block per tile)
+// holds the tile of every block of the grid in launch order, arranged on the
+// host so that XCD x = blockIdx % 8 (the hardware deals blocks round-robin
+// to the 8 XCDs) gets one contiguous range of the batch and neighbour
+// gathers stay in that XCD's L2; entries past the live tiles have count 0.
+// Otherwise the block loops: XCD x takes tiles [x*chunk, (x+1)*chunk) in
+// steps of gridDim/8.  `body(tile)` is called uniformly by the whole block.
 template <class F>
 __device__ inline void for_each_tile(const TileList tl, F &&body) {
+  if (tl.direct) {
+    if ((int)blockIdx.x < tl.ndirect) {
+      const Tile t = tl.direct[blockIdx.x];
+      if (t.count > 0) body(t);
+    }
+    return;
+  }
   const int T = *tl.n;
   const int chunk = (T + 7) >> 3;
   const int x = blockIdx.x & 7, b = blockIdx.x >> 3, G8 = gridDim.x >> 3;
@@ -23,8 +32,21 @@ __device__ inline void for_each_tile(const TileList tl, F &&body) {
   }
 }
 
-// A problem's step flag, read once per block (another block may clear it
-// while this one runs, and the block reductions need every lane).
+// Whether a wave works on problem `prob` in a launch identified by `token`:
+// the step is live and no earlier launch of it raised an error (launches
+// that can raise the negative-state error pass their own non-zero token, so
+// all of their waves finish even after one of them failed, and the reported
+// cell is the lowest failing one of that launch, as in the reference's
+// serial loop).  Each wave decides for itself: the kernels on the step path
+// take no block-wide barrier, so waves of one block may disagree.
+__device__ inline bool wave_live(const DevScalars *s, int prob, int token) {
+  const DevScalars &S = s[prob];
+  const int et = S.err_token;
+  return S.active && (et == 0 || et == token);
+}
+
+// A problem's step flag, agreed on by the whole block (for the one-block-
+// per-problem finalize kernels, which use block reductions).
 __device__ inline bool block_active(const DevScalars *s, int prob) {
   __shared__ int sh;
   __syncthreads();
@@ -33,22 +55,47 @@ __device__ inline bool block_active(const DevScalars *s, int prob) {
   return sh != 0;
 }
 
-// block max of a non-negative value, result valid in thread 0
-__device__ inline double block_max(double v) {
-  __shared__ double red[kBlock / 64];
-  v = wave_max(v);
+// Max over the time-step partials of one problem, computed by one block of
+// kFinalBlock threads: n entries of S interleaved non-negative values; four
+// independent loads in flight per thread.  Max is exact, so the order does
+// not matter.  Result (per component) valid in every thread.
+constexpr int kFinalBlock = 1024;
+
+template <int S>
+__device__ inline void final_max(const double *a, int n, double (&r)[S]) {
+  __shared__ double red[S][kFinalBlock / 64];
+  double acc[4][S];
+  for (int k = 0; k < 4; ++k)
+    for (int j = 0; j < S; ++j) acc[k][j] = 0;
+  int i = threadIdx.x;
+  for (; i + 3 * kFinalBlock < n; i += 4 * kFinalBlock)
+    for (int k = 0; k < 4; ++k)
+      for (int j = 0; j < S; ++j) acc[k][j] = smax(a[(size_t)(i + k * kFinalBlock) * S + j], acc[k][j]);
+  for (; i < n; i += kFinalBlock)
+    for (int j = 0; j < S; ++j) acc[0][j] = smax(a[(size_t)i * S + j], acc[0][j]);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  for (int j = 0; j < S; ++j) {
+    double v = smax(smax(acc[0][j], acc[1][j]), smax(acc[2][j], acc[3][j]));
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = v;
+  }
   __syncthreads();
-  double b = red[0];
-  for (int w = 1; w < kBlock / 64; ++w) b = smax(red[w], b);
-  return b;
+  for (int j = 0; j < S; ++j) {
+    double b = red[j][0];
+    for (int w = 1; w < kFinalBlock / 64; ++w) b = smax(red[j][w], b);
+    r[j] = b;
+  }
 }
 
-__device__ inline void flag_negative(DevScalars *s, int prob, int local_cell) {
-  atomicCAS(&s[prob].err_code, 0, (int)FSALE_ERR_NEG_STATE);
-  atomicMin(reinterpret_cast<unsigned long long *>(&s[prob].err_cell), (unsigned long long)local_cell);
-  s[prob].active = 0;
+// negative density or internal energy in cell `local_cell` of `prob`,
+// raised by launch `token`
+__device__ inline void flag_negative(DevScalars *s, int prob, int local_cell, int token) {
+  DevScalars &S = s[prob];
+  const int old = atomicCAS(&S.err_token, 0, token);
+  if (old == 0 || old == token) {
+    atomicCAS(&S.err_code, 0, (int)FSALE_ERR_NEG_STATE);
+    atomicMin(reinterpret_cast<unsigned long long *>(&S.err_cell), (unsigned long long)local_cell);
+  }
 }
 
 }  // namespace fsale

@@ -7,11 +7,11 @@
 //   k_eul_fluxupd   evaluate_fluxes + apply_update (+ the next step's dt^-1
 //                   partials), one work-item per cell: the driver loop's path
 //
-// Every kernel but the task-level face kernel is a persistent loop over the
-// batch's tile list (batch_dev.h), so one launch advances every resident
-// problem; problem scalars (EOS, CFL, dt, time, flags) come from per-problem
-// records.  SoA FP64 state; k-major fixed-degree connectivity so
-// neighbouring lanes read neighbouring words.
+// Every kernel but the task-level face kernel walks the batch's tile list
+// (batch_dev.h: one block per tile, or a persistent grid), so one launch
+// advances every resident problem; problem scalars (EOS, CFL, dt, time,
+// flags) come from per-problem records.  SoA FP64 state; k-major
+// fixed-degree connectivity so neighbouring lanes read neighbouring words.
 //
 // k_eul_fluxupd recomputes, for each cell, the numerical flux of each of its
 // faces exactly as k_eul_flux does (same left/right states, same operation
@@ -22,10 +22,10 @@
 // negates it exactly).  The flux array never goes through HBM.  Neighbours
 // must read the old state, so the state ping-pongs between two buffers.
 //
-// The time step: each tile writes the max of dt^-1 over its cells to a
-// partials slot (no atomics); one block per problem folds them into dt.  In
-// the driver loop the partials come from the second half-step's update,
-// which already holds the new state.
+// The time step: each wave writes the max of dt^-1 over its cells to a
+// partials slot (no atomics, no block barrier); one block per problem folds
+// them into dt.  In the driver loop the partials come from the second
+// half-step's update, which already holds the new state.
 #include "batch_dev.h"
 
 namespace fsale {
@@ -64,22 +64,21 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerField
         dt_inv = smax(dti, dt_inv);
       }
     }
-    const double bm = block_max(dt_inv);
-    if (threadIdx.x == 0) u.partials[tile.slot] = bm;
+    dt_inv = wave_max(dt_inv);
+    if ((threadIdx.x & 63) == 0) u.partials[tile.slot * kWaves + (threadIdx.x >> 6)] = dt_inv;
   });
 }
 
 // Per problem: dt = min(CFL / max dt^-1, t_final - t) and the step flag.
 // driver != 0: the step loop (step budget, optional stop at t_final);
 // driver == 0: the task-level evaluate_time_step.
-__global__ void __launch_bounds__(kBlock) k_eul_dt_finalize(Batch b, const double *partials, int nprob,
-                                                           int driver) {
+__global__ void __launch_bounds__(kFinalBlock) k_eul_dt_finalize(Batch b, const double *partials, int nprob,
+                                                                int driver) {
   for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
-    double v = 0;
-    for (int i = threadIdx.x; i < P.tile_count; i += kBlock) v = smax(partials[P.tile_first + i], v);
-    const double mx = block_max(v);
+    double mx[1];
+    final_max<1>(partials + (size_t)P.tile_first * kWaves, P.tile_count * kWaves, mx);
     if (threadIdx.x != 0) continue;
     DevScalars &S = b.s[p];
     const double t = S.t;
@@ -89,11 +88,12 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_finalize(Batch b, const doubl
       if (P.stop_at_final && !(t < P.final_time)) active = 0;
     }
     if (active) {
-      if (mx <= 0) {   // apps/hydro/tasks.h:366-367
+      if (mx[0] <= 0) {   // apps/hydro/tasks.h:366-367
         S.err_code = FSALE_ERR_DT;
+        S.err_token = -1;
         active = 0;
       } else {
-        double ts = 1 / mx;
+        double ts = 1 / mx[0];
         ts *= P.cfl;
         ts = smin(ts, P.final_time - t);
         S.dt = ts;
@@ -126,9 +126,9 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   }
   for (int r = 0; r < N; ++r)
     for (int q = 0; q < N; ++q) dxdx[r][q] = 0;
-  const int q0 = m.nbr_start[c0], q1 = q0 + m.nbr_cnt[c0];
-  for (int q = q0; q < q1; ++q) {
-    const int c1 = m.nbr[q];
+  // cell_vertex_neighbors in ascending id order: the k-major head of the
+  // list (loads independent of each other), then the CSR tail if any
+  auto neighbour = [&](int c1) {
     double u1[NV], dx[N];
     {
       int i = 0;
@@ -159,6 +159,17 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
       const double du = u1[j] - u0[j];
       for (int d = 0; d < N; ++d) dudx[j][d] += du * dx[d];
     }
+  };
+  constexpr int NBK = (N == 2) ? kNbk2 : kNbk3;
+#pragma unroll
+  for (int k = 0; k < NBK; ++k) {
+    const int c1 = m.nbk[(size_t)k * m.cap_c + c0];
+    if (c1 >= 0) neighbour(c1);
+  }
+  const int cnt = m.nbr_cnt[c0];
+  if (cnt > NBK) {
+    const int q0 = m.nbr_start[c0];
+    for (int q = q0 + NBK; q < q0 + cnt; ++q) neighbour(m.nbr[q]);
   }
   double gu[NV][N];
   if constexpr (N == 2) {
@@ -233,7 +244,7 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
 template <int N, int VPC>
 __global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x < tile.count) grad_cell<N, VPC>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
   });
 }
@@ -304,7 +315,7 @@ template <int N>
 __global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, Batch b, int prob, int f0,
                                                      int nf) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nf || !b.s[prob].active) return;
+  if (i >= nf || !wave_live(b.s, prob, 0)) return;
   const int f = f0 + i;
   const int kind = b.pp[prob].kind;
   double fx[N], n[N], out[N + 2];
@@ -354,9 +365,10 @@ __device__ inline bool update_cell(const EulerFields &in, const EulerFields &out
 
 // task-level apply_update gathering the stored face fluxes
 template <int N, int FPC>
-__global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u, Batch b, double factor) {
+__global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u, Batch b, double factor,
+                                                       int token) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, token)) return;
     const ProbParams &P = b.pp[tile.prob];
     double delta_t = b.s[tile.prob].dt;
     delta_t *= factor;
@@ -377,16 +389,16 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
     }
     double vel[N], a;
     if (!update_cell<N>(u, u, c, du, delta_t / m.cvol[c], P.gamma, P.cv, vel, a))
-      flag_negative(b.s, tile.prob, c - P.cell_first);
+      flag_negative(b.s, tile.prob, c - P.cell_first, token);
   });
 }
 
 // fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
 template <int N, int FPC, int VPC>
 __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, Batch b,
-                                                        double factor, int want_dt) {
+                                                        double factor, int want_dt, int token) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, token)) return;
     const ProbParams &P = b.pp[tile.prob];
     const int kind = P.kind;
     double delta_t = b.s[tile.prob].dt;
@@ -410,9 +422,9 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
       double nk[FPC][N], ak[FPC];
 #pragma unroll
       for (int k = 0; k < FPC; ++k) {
-        const int code = m.cf[(size_t)k * m.cap_c + c];
-        const int f = code >> 1;
-        const bool second = code & 1;   // this cell is the face's cells[1]
+        const int cn = m.cfn[(size_t)k * m.cap_c + c];
+        const bool second = cn & 1;     // this cell is the face's cells[1]
+        const int other = (cn >> 1) - 1;
         double fx[N];
         if constexpr (N == 2) {
           const int k1 = (k + 1 == VPC) ? 0 : k + 1;
@@ -422,24 +434,22 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
           else
             face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
         } else {
+          const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
           for (int d = 0; d < N; ++d) {
             fx[d] = m.fcen[d][f];
             nk[k][d] = m.fnrm[d][f];
           }
           ak[k] = m.farea[f];
         }
-        const int other = m.fc[(size_t)(second ? 0 : 1) * m.cap_f + f];
+        // the neighbour's loads do not wait on a branch (a boundary face
+        // loads the cell's own data again and ignores it)
+        CellData<N> nb;
+        load_cell<N>(m, u, other >= 0 ? other : c, nb);
         double fl[N + 2];
-        if (other >= 0) {
-          CellData<N> nb;
-          load_cell<N>(m, u, other, nb);
-          if (!second)
-            face_flux_area<N>(kind, own, nb, true, fx, nk[k], ak[k], fl);
-          else
-            face_flux_area<N>(kind, nb, own, true, fx, nk[k], ak[k], fl);
-        } else {
-          face_flux_area<N>(kind, own, own, false, fx, nk[k], ak[k], fl);
-        }
+        if (!second)
+          face_flux_area<N>(kind, own, nb, other >= 0, fx, nk[k], ak[k], fl);
+        else
+          face_flux_area<N>(kind, nb, own, true, fx, nk[k], ak[k], fl);
         if (second) {
           for (int i = 0; i < N + 2; ++i) du[i] += fl[i];
         } else {
@@ -449,7 +459,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
       const double vol = m.cvol[c];
       double vel[N], a;
       if (!update_cell<N>(u, out, c, du, delta_t / vol, P.gamma, P.cv, vel, a))
-        flag_negative(b.s, tile.prob, c - P.cell_first);
+        flag_negative(b.s, tile.prob, c - P.cell_first, token);
       if (want_dt) {
 #pragma unroll
         for (int k = 0; k < FPC; ++k) {
@@ -460,8 +470,8 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
       }
     }
     if (want_dt) {
-      const double bm = block_max(dt_inv);
-      if (threadIdx.x == 0) u.partials[tile.slot] = bm;
+      dt_inv = wave_max(dt_inv);
+      if ((threadIdx.x & 63) == 0) u.partials[tile.slot * kWaves + (threadIdx.x >> 6)] = dt_inv;
     }
   });
 }
@@ -470,7 +480,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
 template <class K, class... A>
 int launch_persistent(fsale_ctx *ctx, int kid, K kernel, A... args) {
   ProfScope prof(ctx, kid);
-  hipLaunchKernelGGL(kernel, dim3(persistent_grid(ctx)), dim3(kBlock), 0, ctx->stream, args...);
+  hipLaunchKernelGGL(kernel, dim3(persistent_grid(ctx, kid)), dim3(kBlock), 0, ctx->stream, args...);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -496,7 +506,7 @@ EulerFields swapped(const EulerFields &a) {
 }
 
 int launch_dt_partial(fsale_ctx *ctx) {
-  const Batch b = batch_args(ctx);
+  const Batch b = batch_args(ctx, K_EUL_DT);
   if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_DT, k_eul_dt_partial<2, 4>, ctx->m, ctx->eu, b);
   return launch_persistent(ctx, K_EUL_DT, k_eul_dt_partial<3, 6>, ctx->m, ctx->eu, b);
 }
@@ -505,23 +515,27 @@ int launch_finalize(fsale_ctx *ctx, int driver) {
   ProfScope prof(ctx, K_EUL_DT);
   const int np = (int)ctx->probs.size();
   const int grid = np < 1024 ? np : 1024;
-  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(grid > 0 ? grid : 1), dim3(kBlock), 0, ctx->stream,
-                     batch_args(ctx), (const double *)ctx->eu.partials, np, driver);
+  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(grid > 0 ? grid : 1), dim3(kFinalBlock), 0, ctx->stream,
+                     batch_args(ctx, -1), (const double *)ctx->eu.partials, np, driver);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
 
 int launch_grad(fsale_ctx *ctx, const EulerFields &u) {
-  const Batch b = batch_args(ctx);
+  const Batch b = batch_args(ctx, K_EUL_GRAD);
   if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4>, ctx->m, u, b);
   return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8>, ctx->m, u, b);
 }
 
-int launch_fluxupd(fsale_ctx *ctx, double factor, bool want_dt, const EulerFields &in, const EulerFields &out) {
-  const Batch b = batch_args(ctx);
+// token: the launch's id for error attribution (first half step 1, second 2)
+int launch_fluxupd(fsale_ctx *ctx, double factor, bool want_dt, int token, const EulerFields &in,
+                   const EulerFields &out) {
+  const Batch b = batch_args(ctx, K_EUL_FLUXUPD);
   if (ctx->m.dim == 2)
-    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, ctx->m, in, out, b, factor, (int)want_dt);
-  return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, ctx->m, in, out, b, factor, (int)want_dt);
+    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, ctx->m, in, out, b, factor, (int)want_dt,
+                             token);
+  return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, ctx->m, in, out, b, factor, (int)want_dt,
+                           token);
 }
 
 bool check_ctx(fsale_ctx *ctx) {
@@ -546,9 +560,9 @@ int euler_step(fsale_ctx *ctx) {
   const EulerFields a = ctx->eu, b = swapped(ctx->eu);
   if ((rc = launch_finalize(ctx, 1))) return rc;
   if ((rc = launch_grad(ctx, a))) return rc;
-  if ((rc = launch_fluxupd(ctx, 0.5, false, a, b))) return rc;
+  if ((rc = launch_fluxupd(ctx, 0.5, false, 1, a, b))) return rc;
   if ((rc = launch_grad(ctx, b))) return rc;
-  if ((rc = launch_fluxupd(ctx, 0.5, true, b, a))) return rc;
+  if ((rc = launch_fluxupd(ctx, 0.5, true, 2, b, a))) return rc;
   return FSALE_OK;
 }
 
@@ -575,7 +589,7 @@ int euler_alloc(fsale_ctx *ctx) {
   if (ctx->single)   // the face flux array serves the task-level evaluate_fluxes only
     for (int i = 0; i < N + 2; ++i)
       if ((rc = dev_alloc_n(ctx, &u.F[i], m.cap_f))) return rc;
-  if ((rc = dev_alloc_n(ctx, &u.partials, ctx->ra_ct.capacity + 1))) return rc;
+  if ((rc = dev_alloc_n(ctx, &u.partials, (size_t)kWaves * (ctx->ra_ct.capacity + 1)))) return rc;
   return FSALE_OK;
 }
 
@@ -656,10 +670,10 @@ int fsale_eul_evaluate_fluxes(fsale_ctx *ctx, int flux_kind) {
   ProfScope prof(ctx, K_EUL_FLUX);
   const int g = (P.nf + kBlock - 1) / kBlock;
   if (ctx->m.dim == 2)
-    hipLaunchKernelGGL(k_eul_flux<2>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->eu, batch_args(ctx), 0,
+    hipLaunchKernelGGL(k_eul_flux<2>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->eu, batch_args(ctx, -1), 0,
                        P.f0, P.nf);
   else
-    hipLaunchKernelGGL(k_eul_flux<3>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->eu, batch_args(ctx), 0,
+    hipLaunchKernelGGL(k_eul_flux<3>, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->eu, batch_args(ctx, -1), 0,
                        P.f0, P.nf);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
@@ -675,9 +689,10 @@ int fsale_eul_apply_update(fsale_ctx *ctx, fsale_eos eos, double factor) {
   int rc;
   if ((rc = upload_params(ctx, 0))) return rc;
   if ((rc = sync_tiles(ctx))) return rc;
-  const Batch b = batch_args(ctx);
-  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_UPDATE, k_eul_update<2, 4>, ctx->m, ctx->eu, b, factor);
-  return launch_persistent(ctx, K_EUL_UPDATE, k_eul_update<3, 6>, ctx->m, ctx->eu, b, factor);
+  const Batch b = batch_args(ctx, K_EUL_UPDATE);
+  if (ctx->m.dim == 2)
+    return launch_persistent(ctx, K_EUL_UPDATE, k_eul_update<2, 4>, ctx->m, ctx->eu, b, factor, 3);
+  return launch_persistent(ctx, K_EUL_UPDATE, k_eul_update<3, 6>, ctx->m, ctx->eu, b, factor, 3);
 }
 
 int fsale_eul_advance(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_time, int nsteps, int flux_kind,

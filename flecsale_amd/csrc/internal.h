@@ -2,9 +2,9 @@
 // libflecsale_amd.  Not part of the public ABI (include/flecsale_amd.h).
 //
 // Every context is a ragged batch: problems (one mesh + state each) occupy
-// disjoint ranges of capacity-sized device arrays, and every kernel is a
-// persistent loop over a device-side list of tiles (a tile = up to kBlock
-// consecutive cells, or vertices, of one problem).  The plain drop-in API is
+// disjoint ranges of capacity-sized device arrays, and every step kernel
+// walks a device-side list of tiles (a tile = up to kBlock consecutive cells,
+// or vertices, of one problem), one block per tile or in a persistent loop.  The plain drop-in API is
 // a batch holding exactly one problem.  Problem-specific scalars (EOS, CFL,
 // time, dt, step budget, error word) live in per-problem device records, so a
 // problem's arithmetic is the same alone and in any mix.
@@ -23,7 +23,11 @@
 namespace fsale {
 
 constexpr int kBlock = 256;     // threads per block = entities per tile
+constexpr int kWaves = kBlock / 64;   // waves per block = dt partials per tile
 constexpr int kMaxCornersPerVertex = 8;
+// neighbours held k-major per cell: all cell-vertex neighbours of an interior
+// cell of a structured quad (8) or hex (26) mesh
+constexpr int kNbk2 = 8, kNbk3 = 26;
 
 void set_message(const std::string &msg);
 
@@ -70,6 +74,14 @@ struct DevMesh {
   int32_t *nbr_start = nullptr; // [cap_c] cell-vertex neighbour list start
   int32_t *nbr_cnt = nullptr;   // [cap_c]
   int32_t *nbr = nullptr;       // [cap_n]
+  // the first nbk_deg neighbours of each cell again, k-major and -1 padded, so
+  // a gradient work-item issues all its neighbour loads at once (cells with
+  // more neighbours continue in the CSR list)
+  int nbk_deg = 0;
+  int32_t *nbk = nullptr;       // [nbk_deg][cap_c]
+  // cell across local face k: (cell + 1) * 2 + (this cell is the face's
+  // cells[1]); boundary faces hold 0
+  int32_t *cfn = nullptr;       // [fpc][cap_c]
   int32_t *vc_start = nullptr;  // [cap_v] vertex -> corner list start
   int32_t *vc_cnt = nullptr;    // [cap_v]
   int32_t *vc = nullptr;        // corner = c*vpc + k (global cell id) [cap_c*vpc]
@@ -88,10 +100,13 @@ struct Tile {
   int prob, first, count, slot;
 };
 
-// Device tile list: tiles[0..*n)
+// Device tile list: tiles[0..*n); or, for one-block-per-tile launches,
+// direct[blockIdx] for blocks below ndirect (see for_each_tile).
 struct TileList {
   const Tile *tiles = nullptr;
   const int *n = nullptr;
+  const Tile *direct = nullptr;
+  int ndirect = 0;
 };
 
 // Per-problem parameters (read-only in a step).
@@ -110,10 +125,12 @@ struct ProbParams {
 struct DevScalars {
   double dt;            // current time step (the reference's dt future)
   double t;             // solution time
-  int active;           // the current step is live
+  int active;           // the current step is live (written between launches only)
   int err_code;         // first fsale_status raised on the device
   long long err_cell;   // lowest failing cell (local id of the problem)
   long long steps;      // completed steps
+  int err_token;        // launch that raised err_code (0: none)
+  int pad;
 };
 
 struct EulerFields {
@@ -125,7 +142,7 @@ struct EulerFields {
          *alt_T = nullptr, *alt_a = nullptr;
   double *gd[3] = {}, *gv[9] = {}, *ge[3] = {}, *gp[3] = {}, *ga[3] = {};
   double *F[5] = {};
-  double *partials = nullptr;    // per cell tile: max of dt^-1
+  double *partials = nullptr;    // per cell tile and wave: max of dt^-1
 };
 
 struct MaireFields {
@@ -135,7 +152,7 @@ struct MaireFields {
   double *un[2] = {};
   double *npc[2] = {}, *Fpc[2] = {};   // corner arrays [vpc][cap_c]
   double *dudt[4] = {};
-  double *partials = nullptr;          // per cell tile: max a/min_length, max |dV/dt|/V
+  double *partials = nullptr;          // per cell tile and wave: max a/min_length, max |dV/dt|/V
 };
 
 // Kernel arguments common to all step kernels
@@ -193,9 +210,12 @@ struct RangeAlloc {
   long long available() const;
 };
 
-// launch-shape choice (autotuned per context)
+// Launch shape of each persistent kernel (autotuned per context):
+// blocks_per_cu[k] = 0: one block per live tile (grid = live tiles rounded
+// up to a power of two, so graphs survive most admissions); n > 0: a
+// persistent grid of CUs * n blocks that loops over the tiles.
 struct LaunchShape {
-  int blocks_per_cu = 4;     // persistent grid = CUs * blocks_per_cu (multiple of 8)
+  int blocks_per_cu[K_COUNT] = {};
 };
 
 }  // namespace fsale
@@ -214,6 +234,8 @@ struct fsale_ctx {
   int max_problems = 0;
   fsale::RangeAlloc ra_c, ra_f, ra_v, ra_n, ra_ct, ra_vt;
   fsale::Tile *d_ctiles = nullptr, *d_vtiles = nullptr;
+  fsale::Tile *d_cdirect = nullptr, *d_vdirect = nullptr;   // one-block-per-tile dispatch lists
+  int n_cdirect = 0, n_vdirect = 0;
   int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles
   fsale::ProbParams *d_params = nullptr;
   fsale::DevScalars *s = nullptr;      // [max_problems] device
@@ -247,8 +269,8 @@ int read_scalars(fsale_ctx *ctx);      // sync + copy DevScalars to h_s
 int check_device_error(fsale_ctx *ctx, int prob);
 int sync_tiles(fsale_ctx *ctx);        // upload the tile lists if they changed
 int upload_params(fsale_ctx *ctx, int prob);
-Batch batch_args(const fsale_ctx *ctx);
-int persistent_grid(const fsale_ctx *ctx);
+Batch batch_args(const fsale_ctx *ctx, int kid);
+int persistent_grid(const fsale_ctx *ctx, int kid);
 
 // geometry (geom.hip): cells/faces of one problem
 int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces);

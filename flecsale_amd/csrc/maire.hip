@@ -9,8 +9,8 @@
 //                     apps/maire_hydro/tasks.h:536-546, :110-133
 //   k_maire_copy      save_/restore_coordinates, save_/restore_solution :557-654
 //
-// Persistent loops over the batch's vertex tiles (nodal, move) and cell tiles
-// (the rest) with per-problem scalars.  Wedge facets (half edges: burton's
+// Loops over the batch's vertex tiles (nodal, move) and cell tiles (the
+// rest; batch_dev.h) with per-problem scalars.  Wedge facets (half edges: burton's
 // wedge facet_normal / facet_area) are recomputed from the vertex coordinates
 // where needed instead of being stored: the mesh moves every half step.
 // Corner arrays are k-major [4][cap_c] so the residual gather of a cell's
@@ -186,14 +186,14 @@ __device__ inline void nodal_vertex(const DevMesh &m, const MaireFields &u, int 
 
 __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
   for_each_tile(b.vt, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x < tile.count) nodal_vertex(m, u, tile.first + threadIdx.x);
   });
 }
 
 __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const int c = tile.first + threadIdx.x;
     double r[4] = {0, 0, 0, 0};
@@ -218,36 +218,34 @@ __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireField
 // per cell tile: max a/min_length and max |dV/dt|/V
 __global__ void __launch_bounds__(kBlock) k_maire_dt_partial(DevMesh m, MaireFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, 0)) return;
     double acc = 0, vol = 0;
     if ((int)threadIdx.x < tile.count) {
       const int c = tile.first + threadIdx.x;
       acc = u.a[c] / m.cminlen[c];
       vol = fabs(u.dudt[0][c]) / m.cvol[c];
     }
-    const double bacc = block_max(acc);
-    const double bvol = block_max(vol);
-    if (threadIdx.x == 0) {
-      u.partials[2 * tile.slot] = bacc;
-      u.partials[2 * tile.slot + 1] = bvol;
+    acc = wave_max(acc);
+    vol = wave_max(vol);
+    if ((threadIdx.x & 63) == 0) {
+      const size_t i = 2 * ((size_t)tile.slot * kWaves + (threadIdx.x >> 6));
+      u.partials[i] = acc;
+      u.partials[i + 1] = vol;
     }
   });
 }
 
 // evaluate_time_step :179-188 (first minimum of the three limits) and the
 // driver's cap min(dt, t_final - t) (driver.h:391), one block per problem
-__global__ void __launch_bounds__(kBlock) k_maire_dt_finalize(Batch b, const double *partials, int nprob) {
+__global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, const double *partials, int nprob) {
   for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
     if (!block_active(b.s, p)) continue;
-    double a0 = 0, a1 = 0;
-    for (int i = threadIdx.x; i < P.tile_count; i += kBlock) {
-      a0 = smax(partials[2 * (P.tile_first + i)], a0);
-      a1 = smax(partials[2 * (P.tile_first + i) + 1], a1);
-    }
-    const double acc_inv = block_max(a0);
-    const double vol_inv = block_max(a1);
+    double mx[2];
+    final_max<2>(partials + 2 * (size_t)P.tile_first * kWaves, P.tile_count * kWaves, mx);
+    const double acc_inv = mx[0];
+    const double vol_inv = mx[1];
     if (threadIdx.x != 0) continue;
     DevScalars &S = b.s[p];
     const double dts[3] = {P.cfl_acc / acc_inv, P.cfl_vol / vol_inv, P.cfl_growth * S.dt};
@@ -260,7 +258,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_dt_finalize(Batch b, const dou
 
 __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u, Batch b, double factor) {
   for_each_tile(b.vt, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const double delta_t = factor * b.s[tile.prob].dt;
     if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
@@ -269,9 +267,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
   });
 }
 
-__global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, Batch b, double factor) {
+__global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, Batch b, double factor,
+                                                         int token) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, token)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const ProbParams &P = b.pp[tile.prob];
     const int c = tile.first + threadIdx.x;
@@ -313,14 +312,14 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
     u.p[c] = eos_p_de(g, d, ie);
     u.T[c] = eos_T_de(P.cv, ie);
     u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
-    if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, c - P.cell_first);
+    if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, c - P.cell_first, token);
   });
 }
 
 // save (dir 0) / restore (dir 1) of coordinates, velocity and energy
 __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u, Batch b, int dir) {
   for_each_tile(b.vt, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const int i = tile.first + threadIdx.x;
     for (int d = 0; d < 2; ++d) {
@@ -331,7 +330,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
     }
   });
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!block_active(b.s, tile.prob)) return;
+    if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const int i = tile.first + threadIdx.x;
     if (dir == 0) {
@@ -359,7 +358,7 @@ __global__ void k_maire_end(Batch b, int nprob) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
     if (b.pp[p].cell_count == 0) continue;
     DevScalars &S = b.s[p];
-    if (!S.active) continue;
+    if (!S.active || S.err_code) continue;
     S.t += S.dt;
     S.steps = S.steps + 1;
   }
@@ -368,7 +367,7 @@ __global__ void k_maire_end(Batch b, int nprob) {
 template <class K, class... A>
 int launch_persistent(fsale_ctx *ctx, int kid, K kernel, A... args) {
   ProfScope prof(ctx, kid);
-  hipLaunchKernelGGL(kernel, dim3(persistent_grid(ctx)), dim3(kBlock), 0, ctx->stream, args...);
+  hipLaunchKernelGGL(kernel, dim3(persistent_grid(ctx, kid)), dim3(kBlock), 0, ctx->stream, args...);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -387,28 +386,32 @@ bool check_ctx(fsale_ctx *ctx) {
 }
 
 int launch_nodal(fsale_ctx *ctx) {
-  return launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal, ctx->m, ctx->ma, batch_args(ctx));
+  return launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_NODAL));
 }
 int launch_residual(fsale_ctx *ctx) {
-  return launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual, ctx->m, ctx->ma, batch_args(ctx));
+  return launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual, ctx->m, ctx->ma,
+                           batch_args(ctx, K_MAIRE_RESIDUAL));
 }
 int launch_dt(fsale_ctx *ctx) {
-  int rc = launch_persistent(ctx, K_MAIRE_DT, k_maire_dt_partial, ctx->m, ctx->ma, batch_args(ctx));
+  int rc = launch_persistent(ctx, K_MAIRE_DT, k_maire_dt_partial, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_DT));
   if (rc) return rc;
   ProfScope prof(ctx, K_MAIRE_DT);
   const int np = (int)ctx->probs.size();
-  hipLaunchKernelGGL(k_maire_dt_finalize, dim3(np < 1024 ? np : 1024), dim3(kBlock), 0, ctx->stream,
-                     batch_args(ctx), (const double *)ctx->ma.partials, np);
+  hipLaunchKernelGGL(k_maire_dt_finalize, dim3(np < 1024 ? np : 1024), dim3(kFinalBlock), 0, ctx->stream,
+                     batch_args(ctx, -1), (const double *)ctx->ma.partials, np);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
-int launch_apply(fsale_ctx *ctx, double factor) {
-  int rc = launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move, ctx->m, ctx->ma, batch_args(ctx), factor);
+// token: the update launch's id for error attribution (predictor 1,
+// corrector 2, task-level apply_update 3)
+int launch_apply(fsale_ctx *ctx, double factor, int token) {
+  int rc = launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_MOVE), factor);
   if (rc) return rc;
-  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update, ctx->m, ctx->ma, batch_args(ctx), factor);
+  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_UPDATE),
+                           factor, token);
 }
 int launch_copy(fsale_ctx *ctx, int dir) {
-  return launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy, ctx->m, ctx->ma, batch_args(ctx), dir);
+  return launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_COPY), dir);
 }
 
 }  // namespace
@@ -418,18 +421,18 @@ int launch_copy(fsale_ctx *ctx, int dir) {
 int maire_step(fsale_ctx *ctx) {
   int rc;
   const int np = (int)ctx->probs.size();
-  hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx), np);
+  hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1), np);
   FSALE_HIP(hipGetLastError());
   if ((rc = launch_copy(ctx, 0))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx))) return rc;
   if ((rc = launch_dt(ctx))) return rc;
-  if ((rc = launch_apply(ctx, 0.5))) return rc;
+  if ((rc = launch_apply(ctx, 0.5, 1))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx))) return rc;
   if ((rc = launch_copy(ctx, 1))) return rc;
-  if ((rc = launch_apply(ctx, 1.0))) return rc;
-  hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx), np);
+  if ((rc = launch_apply(ctx, 1.0, 2))) return rc;
+  hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1), np);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -451,7 +454,7 @@ int maire_alloc(fsale_ctx *ctx) {
   }
   for (int i = 0; i < 4; ++i)
     if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.cap_c))) return rc;
-  if ((rc = dev_alloc_n(ctx, &u.partials, 2 * ((size_t)ctx->ra_ct.capacity + 1)))) return rc;
+  if ((rc = dev_alloc_n(ctx, &u.partials, 2 * (size_t)kWaves * (ctx->ra_ct.capacity + 1)))) return rc;
   return FSALE_OK;
 }
 
@@ -527,7 +530,7 @@ int fsale_maire_apply_update(fsale_ctx *ctx, double factor) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
   int rc = sync_tiles(ctx);
-  return rc ? rc : launch_apply(ctx, factor);
+  return rc ? rc : launch_apply(ctx, factor, 3);
 }
 
 int fsale_maire_save(fsale_ctx *ctx) {

@@ -2,10 +2,11 @@
 
 Two passes, each under its own rocprofv3 run (the counters do not fit one pass):
 
-    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m flecsale_amd.profiling run
-    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m flecsale_amd.profiling run
-    python3 -m flecsale_amd.profiling summarize gpurun_out/pmc_fetch/run_results.db \\
-        gpurun_out/pmc_write/run_results.db --workload muscl2d
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- \\
+        python3 -m flecsale_amd.profiling run
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- \\
+        python3 -m flecsale_amd.profiling run
+    python3 -m flecsale_amd.profiling summarize gpurun_out/pmc_fetch gpurun_out/pmc_write --workload muscl2d
 
 `run` first launches a plain device-to-device copy of a known size (512 MiB,
 larger than the 256 MiB Infinity Cache) so each counter can be calibrated
@@ -68,22 +69,44 @@ def run(workload, steps, size):
     ctx.close()
 
 
-def _per_kernel(db_path):
-    """{kernel short name: (dispatches, mean counter value)} from a rocpd database."""
-    db = sqlite3.connect(db_path)
-    cur = db.cursor()
-    rows = cur.execute("select * from counters_collection").fetchall()
-    cols = [d[0] for d in cur.description]
-    name_col = "kernel_name" if "kernel_name" in cols else "name"
-    ic = cols.index(name_col)
-    iv = cols.index("value")
-    idisp = cols.index("dispatch_id") if "dispatch_id" in cols else None
+def _short(name):
+    return re.sub(r"\(.*", "", str(name).replace("fsale::(anonymous namespace)::", "").replace("void ", ""))
+
+
+def _per_kernel(path, counter=None):
+    """{kernel short name: (dispatches, mean counter value per dispatch)} from
+    rocprofv3 output: a counter_collection.csv (or a directory holding one),
+    or a rocpd SQLite database."""
     acc = {}
-    for r in rows:
-        name = re.sub(r"\(.*", "", str(r[ic]).replace("fsale::(anonymous namespace)::", "").replace("void ", ""))
-        disp = r[idisp] if idisp is not None else None
-        acc.setdefault(name, {}).setdefault(disp, 0.0)
-        acc[name][disp] += float(r[iv])
+    if os.path.isdir(path):
+        hits = [os.path.join(dp, f) for dp, _, fs in os.walk(path) for f in fs
+                if f.endswith("counter_collection.csv")]
+        if not hits:
+            raise SystemExit(f"no counter_collection.csv under {path}")
+        path = hits[0]
+    if path.endswith(".csv"):
+        import csv
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                if counter and r.get("Counter_Name") != counter:
+                    continue
+                name = _short(r.get("Kernel_Name", ""))
+                disp = r.get("Dispatch_Id")
+                acc.setdefault(name, {}).setdefault(disp, 0.0)
+                acc[name][disp] += float(r["Counter_Value"])
+    else:
+        db = sqlite3.connect(path)
+        cur = db.cursor()
+        rows = cur.execute("select * from counters_collection").fetchall()
+        cols = [d[0] for d in cur.description]
+        ic = cols.index("kernel_name" if "kernel_name" in cols else "name")
+        iv = cols.index("value")
+        idisp = cols.index("dispatch_id") if "dispatch_id" in cols else None
+        for r in rows:
+            name = _short(r[ic])
+            disp = r[idisp] if idisp is not None else None
+            acc.setdefault(name, {}).setdefault(disp, 0.0)
+            acc[name][disp] += float(r[iv])
     return {k: (len(v), sum(v.values()) / len(v)) for k, v in acc.items()}
 
 

@@ -257,8 +257,20 @@ int fsale_batch_get_field(fsale_ctx *ctx, int32_t problem, int field, double *ho
 /* Free capacity (sums; a range still has to fit one contiguous gap). */
 int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_faces,
                          int64_t *free_vertices, int64_t *free_neighbours, int32_t *free_problems);
-/* Persistent-kernel launch shape: blocks per compute unit (1-32). */
-int fsale_launch_shape(fsale_ctx *ctx, int32_t blocks_per_cu);
+/* Launch shape of the tile-list kernels (kernel ids as fsale_get_profile;
+ * kernel -1 sets all): blocks_per_cu 0 launches one block per resident
+ * tile of 256 entities, 1-32 a persistent grid of (compute units x
+ * blocks_per_cu) blocks looping over the tiles.  Shapes change timing only,
+ * never results.  Any context (a plain one is a batch of one). */
+int fsale_launch_shape(fsale_ctx *ctx, int32_t kernel, int32_t blocks_per_cu);
+/* blocks_per_cu[11] receives the current shape of every kernel id. */
+int fsale_get_launch_shape(fsale_ctx *ctx, int32_t *blocks_per_cu);
+/* Autotuner: times nsteps eager steps of the resident problems under each
+ * candidate shape (HIP events per launch) from one snapshot of the state,
+ * keeps the fastest shape per kernel and restores the state; *best_ms (may be
+ * NULL) receives the summed kernel time of one step at the chosen shapes.
+ * Allocates a transient snapshot of the state arrays. */
+int fsale_autotune(fsale_ctx *ctx, int32_t nsteps, double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -214,3 +214,30 @@ def test_failure_isolation():
         assert st[ib].error == fa.ERR_NEG_STATE and st[ib].done == 1
         assert st[ig].error == 0 and st[ig].steps == 6
         _same(b.state(ig), want, "healthy problem")
+
+
+@pytest.mark.parametrize("solver", ["euler", "maire"])
+def test_launch_shape_never_changes_results(solver):
+    """Persistent grids of any size and one-block-per-tile launches give
+    bit-identical results; the autotuner restores the state it timed."""
+    mix = _euler_mix() if solver == "euler" else _maire_mix()
+    make, add = (_euler_batch, _add_euler) if solver == "euler" else (_maire_batch, _add_maire)
+    results = []
+    for shape in [0, 1, 3, 16, "tuned"]:
+        with make([p for p, _ in mix]) as b:
+            ids = [add(b, p, n) for p, n in mix]
+            if shape == "tuned":
+                before = {pid: b.state(pid) for pid in ids}
+                ms = b.autotune(2)
+                assert ms > 0
+                for pid in ids:
+                    _same(b.state(pid), before[pid], "state after autotune")
+                assert all(s.steps == 0 for s in b.status().values())
+            else:
+                b.launch_shape(shape)
+            b.step(7)
+            b.step(6, use_graph=False)
+            results.append({pid: b.state(pid) for pid in ids})
+    for r in results[1:]:
+        for pid in r:
+            _same(r[pid], results[0][pid], f"problem {pid}")

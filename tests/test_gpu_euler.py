@@ -223,17 +223,34 @@ def test_conservation_in_walled_box():
         assert abs(e1 - e0) <= 1e-13 * e0
 
 
-def test_negative_state_is_reported():
+def test_negative_state_is_reported(oracle_lib):
     """A CFL of 50 drives the Sod state negative: the library reports the
-    reference's 'Negative density or internal energy' error (apps/hydro/tasks.h:654)."""
+    reference's 'Negative density or internal energy' error (apps/hydro/tasks.h:654)
+    for the same cell the reference's serial loop stops at: the lowest failing
+    cell of the first failing apply_update."""
+    O = oracle_lib
     prob = inputs.sod_100x10()
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    s = O.euler_initial_state(mesh, prob["eos"][0], prob["eos"][1], prob["d"], prob["v"], prob["p"])
+    bad, t = -1, 0.0
+    for _ in range(20):
+        dt = O.euler_time_step(mesh, s, 50.0, 0.2, t)
+        for _half in range(2):
+            F = O.euler_fluxes(mesh, s, O.euler_gradients(mesh, s, first_order=True), 0)
+            rc, bad = O.euler_apply_update(mesh, s, prob["eos"][0], prob["eos"][1], F, dt, 0.5)
+            if rc:
+                break
+            t += 0.5 * dt
+        if bad >= 0:
+            break
+    assert bad >= 0
     with fa.EulerContext(prob["coords"], prob["cells"]) as ctx:
         ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
         with pytest.raises(fa.FsaleError) as ei:
             ctx.advance(50.0, 0.2, 20, first_order=True)
             ctx.sync()
         assert ei.value.status == fa.ERR_NEG_STATE
-        assert ctx.error_cell() >= 0
+        assert ctx.error_cell() == bad
 
 
 def test_zero_wavespeed_reports_infinite_dt():
