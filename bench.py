@@ -101,12 +101,101 @@ def cpu_baseline(prob, kind, budget_s=12.0):
                        f"OMP_NUM_THREADS={threads}")
 
 
+# ------------------------------------------------------------------ serving
+SERVE_KINDS = ("euler", "maire")
+
+
+def serve_requests(inputs, n, steps):
+    """The mixed-size stream's problems, built before any timing (the load
+    generator's own work stays out of the served loop)."""
+    sides = inputs.stream_sides(n)
+    return [(SERVE_KINDS[i % 2], inputs.riemann_2d(int(sides[i]), SERVE_KINDS[i % 2], index=i, steps=steps))
+            for i in range(n)]
+
+
+def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=256):
+    """Serves `requests` (solver kind, problem) through fsale_server in `mode`
+    (0 batched, 1 one plain context per request).  arrivals: seconds after
+    the start (Poisson trace), or None to keep up to `window` requests queued
+    (saturated).  Requests are submitted when they arrive, while the previous
+    scheduling round runs."""
+    n = len(requests)
+    solver = {"euler": fa.SOLVER_EULER, "maire": fa.SOLVER_MAIRE}
+    # resident capacity per batch: ~170 average requests (47K cells each)
+    cap = capacity or (8 << 20, 17 << 20, 9 << 20, 66 << 20)
+    with fa.Server(mode=mode, max_problems=1024, euler=cap, maire=cap, chunk_steps=10, lookahead=64,
+                   frac_hbm=0.5) as srv:
+        t0 = srv.clock()
+        ids, cells = [], []
+        i = done = pending = 0
+        while done < n:
+            now = srv.clock() - t0
+            while i < n and ((arrivals is not None and arrivals[i] <= now) or
+                             (arrivals is None and pending < window)):
+                kind, prob = requests[i]
+                at = t0 + (arrivals[i] if arrivals is not None else now)
+                ids.append(srv.submit(prob, solver[kind], arrival=at))
+                cells.append(len(prob["cells"]))
+                i += 1
+                pending += 1
+            c, pending = srv.poll()
+            done += c
+            if pending == 0 and i < n and arrivals is not None:
+                time.sleep(max(0.0, min(0.005, arrivals[i] - (srv.clock() - t0))))
+        makespan = srv.clock() - t0
+        host = srv.stats() if mode == 0 else None
+        lat, zu, errors = [], 0, 0
+        for rid, nc in zip(ids, cells):
+            a, _, f = srv.times(rid)
+            st, _ = srv.result(rid, fields=False)
+            lat.append(f - a)
+            zu += nc * st.steps
+            errors += int(st.error != 0)
+    lat = np.array(lat)
+    return {"requests": n, "makespan_s": makespan, "zone_updates": zu, "zu_per_s": zu / makespan,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "errors": errors, "host": host}
+
+
+def serve_stream(fa, inputs, n, steps=100, calib=24):
+    """The mixed-size request stream: one-at-a-time service rate mu from the
+    first `calib` requests; a Poisson trace at 2 mu served both ways (latency
+    under the same saturated trace); and both ways with the queue kept full
+    (throughput capacity)."""
+    from flecsale_amd.inputs import stream_arrivals
+    t_gen = time.perf_counter()
+    requests = serve_requests(inputs, n, steps)
+    log(f"[serve] {n} requests built in {time.perf_counter() - t_gen:.1f}s")
+    cal = serve_trace(fa, requests[:calib], 1)
+    mu = calib / cal["makespan_s"]
+    rate = 2 * mu
+    arrivals = stream_arrivals(n, rate)
+    log(f"[serve] one-at-a-time service rate {mu:.1f} req/s; Poisson trace at {rate:.1f} req/s")
+    out = {"requests": n, "steps_per_request": steps, "arrival_rate_per_s": rate,
+           "stream": "two-state Riemann problems, side round(exp(U[ln 32, ln 512])), alternating MUSCL "
+                     "(HLLE, walls) / Maire (symmetry walls), gamma 1.4 or 1.67"}
+    for name, mode in (("batched", 0), ("one_at_a_time", 1)):
+        r = serve_trace(fa, requests, mode, arrivals=arrivals)
+        log(f"[serve] {name} trace: {r['zu_per_s']:.3e} ZU/s, p50 {r['p50_s']*1e3:.1f} ms, "
+            f"p99 {r['p99_s']*1e3:.1f} ms, makespan {r['makespan_s']:.2f} s")
+        out[name + "_trace"] = r
+        r = serve_trace(fa, requests, mode)
+        log(f"[serve] {name} saturated: {r['zu_per_s']:.3e} ZU/s, makespan {r['makespan_s']:.2f} s"
+            + (f", host {json.dumps({k: round(v, 3) for k, v in r['host'].items()})}" if r["host"] else ""))
+        out[name + "_saturated"] = r
+    out["throughput_ratio"] = out["batched_saturated"]["zu_per_s"] / out["one_at_a_time_saturated"]["zu_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d"])
+    ap.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d", "serve"])
+    ap.add_argument("--requests", type=int, default=None,
+                    help="requests of the served stream (default 256 beside a step benchmark, 4096 for serve)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--size", type=int, default=None, help="override mesh size (cells per side)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -121,6 +210,10 @@ def main():
 
     import flecsale_amd as fa
     from flecsale_amd import inputs
+
+    if args.workload == "serve":
+        main_serve(args, fa, inputs)
+        return
 
     t_build = time.perf_counter()
     if args.workload == "muscl2d":
@@ -240,6 +333,11 @@ def main():
 
     ctx_shape = ctx.get_launch_shape()
     ctx.close()
+    serving = None
+    if not args.no_serve:
+        n_req = args.requests or 256
+        log(f"[bench] served stream sample: {n_req} requests")
+        serving = serve_stream(fa, inputs, n_req)
     out = {
         "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
         "value": value,
@@ -260,8 +358,65 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "launch_shape": ctx_shape,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def stream_cpu_baseline(inputs, sides, steps, budget_s=12.0):
+    """CPU restatement on the first requests of the stream, one after another."""
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    zu = k = 0
+    while k < 2 or (time.perf_counter() - t0 < budget_s and k < len(sides)):
+        kind = SERVE_KINDS[k % 2]
+        prob = inputs.riemann_2d(int(sides[k]), kind, index=k, steps=steps)
+        mesh = O.Mesh(prob["coords"], prob["cells"])
+        if kind == "euler":
+            s = O.euler_initial_state(mesh, prob["eos"][0], prob["eos"][1], prob["d"], prob["v"], prob["p"])
+            rc, t, n = O.euler_run(mesh, s, prob["eos"][0], prob["eos"][1], prob["cfl"], prob["final_time"], steps,
+                                   flux_kind=prob["flux"])
+        else:
+            s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
+            rc, t, n, dt = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], steps, prob["dt0"])
+        zu += len(prob["cells"]) * n
+        k += 1
+    el = time.perf_counter() - t0
+    return dict(value=zu / el, unit="zone-updates/s", cores=threads, kind="port",
+                sample=f"first {k} requests of the stream served one after another, {el:.1f} s, "
+                       f"OMP_NUM_THREADS={threads}")
+
+
+def main_serve(args, fa, inputs):
+    n = args.requests or 4096
+    steps = args.steps or 100
+    out = serve_stream(fa, inputs, n, steps)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU restatement on the first requests of the stream")
+        cpu = stream_cpu_baseline(inputs, inputs.stream_sides(n), steps)
+        log(f"[bench] cpu baseline {cpu['value']:.4e} zone-updates/s ({cpu['sample']})")
+    b = out["batched_saturated"]
+    line = {
+        "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
+        "value": b["zu_per_s"],
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded request stream)",
+        "config": {"workload": f"served stream of {n} two-state Riemann problems, side log-uniform in "
+                               f"[32, 512], alternating MUSCL / Maire, {steps} steps each"},
+        "serving": out,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -56,6 +56,8 @@ EXPORTS = [
     "fsale_batch_create", "fsale_batch_add_euler", "fsale_batch_add_maire", "fsale_batch_remove",
     "fsale_batch_step", "fsale_batch_status", "fsale_batch_get_field", "fsale_batch_capacity",
     "fsale_launch_shape", "fsale_get_launch_shape", "fsale_autotune",
+    "fsale_server_create", "fsale_server_destroy", "fsale_server_clock", "fsale_server_submit",
+    "fsale_server_poll", "fsale_server_times", "fsale_server_result", "fsale_server_stats",
 ]
 
 
@@ -87,6 +89,22 @@ class MaireParams(ct.Structure):
 class ProblemStatus(ct.Structure):
     _fields_ = [("problem", ct.c_int32), ("error", ct.c_int32), ("steps", ct.c_int64), ("time", ct.c_double),
                 ("time_step", ct.c_double), ("done", ct.c_int32)]
+
+
+class ServerConfig(ct.Structure):
+    _fields_ = [("mode", ct.c_int32), ("dim", ct.c_int32), ("verts_per_cell", ct.c_int32),
+                ("max_problems", ct.c_int32),
+                ("euler_cells", ct.c_int64), ("euler_faces", ct.c_int64), ("euler_vertices", ct.c_int64),
+                ("euler_neighbours", ct.c_int64),
+                ("maire_cells", ct.c_int64), ("maire_faces", ct.c_int64), ("maire_vertices", ct.c_int64),
+                ("maire_neighbours", ct.c_int64),
+                ("chunk_steps", ct.c_int32), ("lookahead", ct.c_int32)]
+
+
+class Request(ct.Structure):
+    _fields_ = [("solver", ct.c_int32), ("mesh", MeshDesc), ("eos", Eos), ("euler", EulerParams),
+                ("cv", ct.c_double), ("gamma", ct.c_void_p), ("maire", MaireParams),
+                ("d", ct.c_void_p), ("v", ct.c_void_p), ("p", ct.c_void_p)]
 
 
 _lib = None
@@ -146,10 +164,20 @@ def lib():
         L.fsale_launch_shape.argtypes = [vp, i32, i32]
         L.fsale_get_launch_shape.argtypes = [vp, vp]
         L.fsale_autotune.argtypes = [vp, i32, ct.POINTER(dp)]
+        L.fsale_server_create.argtypes = [ct.POINTER(ServerConfig), dp, ct.POINTER(vp)]
+        L.fsale_server_destroy.argtypes = [vp]
+        L.fsale_server_clock.argtypes = []
+        L.fsale_server_clock.restype = dp
+        L.fsale_server_submit.argtypes = [vp, ct.POINTER(Request), dp, ct.POINTER(i64)]
+        L.fsale_server_poll.argtypes = [vp, ct.POINTER(i32), ct.POINTER(i32)]
+        L.fsale_server_times.argtypes = [vp, i64, ct.POINTER(dp), ct.POINTER(dp), ct.POINTER(dp)]
+        L.fsale_server_stats.argtypes = [vp, vp]
+        L.fsale_server_result.argtypes = [vp, i64, ct.c_int, ct.POINTER(ProblemStatus), vp, vp, vp, vp, vp]
         for name in EXPORTS:
             f = getattr(L, name)
             if f.restype is ct.c_int or f.restype is None:
                 f.restype = ct.c_int
+        L.fsale_server_clock.restype = dp
         L.fsale_status_string.restype = ct.c_char_p
         L.fsale_last_message.restype = ct.c_char_p
         _lib = L
@@ -494,3 +522,102 @@ class Batch:
         ms = ct.c_double()
         _check(lib().fsale_autotune(self.h, int(nsteps), ct.byref(ms)))
         return ms.value
+
+
+class Server:
+    """Serving loop (fsale_server_*): requests are admitted into resident
+    batches while capacity allows and advance together (mode 0), or run one
+    at a time in plain contexts (mode 1)."""
+
+    def __init__(self, mode=0, dim=2, verts_per_cell=4, max_problems=1024, euler=(0, 0, 0, 0), maire=(0, 0, 0, 0),
+                 chunk_steps=10, lookahead=64, frac_hbm=0.5):
+        cfg = ServerConfig(int(mode), int(dim), int(verts_per_cell), int(max_problems), *[int(x) for x in euler],
+                           *[int(x) for x in maire], int(chunk_steps), int(lookahead))
+        h = ct.c_void_p()
+        _check(lib().fsale_server_create(ct.byref(cfg), float(frac_hbm), ct.byref(h)))
+        self.h = h
+        self.dim = dim
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fsale_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def clock():
+        return lib().fsale_server_clock()
+
+    def submit(self, prob, solver, arrival=None):
+        """Queue one problem dict (inputs.riemann_2d and the like); returns its id."""
+        desc, keep = _desc(prob["coords"], prob["cells"])
+        nc = desc.num_cells
+        d = np.ascontiguousarray(np.broadcast_to(prob["d"], (nc,)), dtype=np.float64)
+        v = np.ascontiguousarray(np.broadcast_to(prob["v"], (nc, self.dim)), dtype=np.float64)
+        p = np.ascontiguousarray(np.broadcast_to(prob["p"], (nc,)), dtype=np.float64)
+        q = Request()
+        q.solver = int(solver)
+        q.mesh = desc
+        q.d, q.v, q.p = d.ctypes.data, v.ctypes.data, p.ctypes.data
+        if solver == SOLVER_EULER:
+            q.eos = Eos(*prob["eos"])
+            q.euler = EulerParams(prob["cfl"], prob["final_time"], int(prob["flux"]),
+                                  int(prob.get("first_order", False)), int(prob.get("stop_at_final", False)),
+                                  int(prob["max_steps"]))
+        else:
+            g = np.ascontiguousarray(np.broadcast_to(prob["gamma"], (nc,)), dtype=np.float64)
+            q.cv = prob["cv"]
+            q.gamma = g.ctypes.data
+            c = prob["cfl"]
+            q.maire = MaireParams(c[0], c[1], c[2], prob["final_time"], prob["dt0"], int(prob["max_steps"]))
+        out = ct.c_int64()
+        t = self.clock() if arrival is None else float(arrival)
+        _check(lib().fsale_server_submit(self.h, ct.byref(q), t, ct.byref(out)))
+        self.shapes[out.value] = (nc, desc.num_vertices, solver)
+        return out.value
+
+    def poll(self):
+        """One scheduling round: (requests completed in it, requests queued or running)."""
+        c, p = ct.c_int32(), ct.c_int32()
+        _check(lib().fsale_server_poll(self.h, ct.byref(c), ct.byref(p)))
+        return c.value, p.value
+
+    def stats(self):
+        """Host seconds per phase of the batched loop, rounds and admissions."""
+        out = np.zeros(6)
+        _check(lib().fsale_server_stats(self.h, out.ctypes.data))
+        return dict(admit_s=out[0], wait_s=out[1], retire_s=out[2], launch_s=out[3], rounds=int(out[4]),
+                    admissions=int(out[5]))
+
+    def times(self, rid):
+        a, b, c = ct.c_double(), ct.c_double(), ct.c_double()
+        _check(lib().fsale_server_times(self.h, int(rid), ct.byref(a), ct.byref(b), ct.byref(c)))
+        return a.value, b.value, c.value
+
+    def result(self, rid, keep=False, fields=True):
+        """(ProblemStatus, {d, v, e, p[, x]}) of a finished request."""
+        nc, nv, solver = self.shapes[rid]
+        st = ProblemStatus()
+        out = {}
+        ptrs = [None] * 5
+        if fields:
+            out = dict(d=np.zeros(nc), v=np.zeros((nc, self.dim)), e=np.zeros(nc), p=np.zeros(nc))
+            if solver == SOLVER_MAIRE:
+                out["x"] = np.zeros((nv, self.dim))
+            ptrs = [out[k].ctypes.data if k in out else None for k in ("d", "v", "e", "p", "x")]
+        _check(lib().fsale_server_result(self.h, int(rid), int(keep), ct.byref(st), *ptrs))
+        if not keep:
+            self.shapes.pop(rid, None)
+        return st, out

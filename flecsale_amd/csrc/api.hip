@@ -170,7 +170,16 @@ int sync_tiles(fsale_ctx *ctx) {
     for (int i = 0; i < P.nvt; ++i)
       ctx->h_vtiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nv - i * kBlock), P.vt0 + i});
   }
-  const int counts[2] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size()};
+  std::vector<int> plist;
+  for (int p = 0; p < (int)ctx->probs.size(); ++p)
+    if (ctx->probs[p].live) plist.push_back(p);
+  const int counts[3] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size()};
+  if (!plist.empty())
+    FSALE_HIP(hipMemcpyAsync(ctx->d_plist, plist.data(), sizeof(int) * plist.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+  int pg = 1;
+  while (pg < (int)plist.size() && pg < 1024) pg *= 2;
+  ctx->pgrid = pg;
   if (!ctx->h_ctiles.empty())
     FSALE_HIP(hipMemcpyAsync(ctx->d_ctiles, ctx->h_ctiles.data(), sizeof(Tile) * ctx->h_ctiles.size(),
                              hipMemcpyHostToDevice, ctx->stream));
@@ -210,6 +219,8 @@ Batch batch_args(const fsale_ctx *ctx, int kid) {
   b.pp = ctx->d_params;
   b.s = ctx->s;
   b.cap_c_tiles = ctx->ra_ct.capacity;
+  b.plist = ctx->d_plist;
+  b.np = ctx->d_counts + 2;
   return b;
 }
 
@@ -235,21 +246,58 @@ int persistent_grid(const fsale_ctx *ctx, int kid) {
 
 namespace {
 
-std::string mesh_key(const fsale_mesh_desc &m) {
-  // connectivity depends on the cell table only
-  std::string k(reinterpret_cast<const char *>(&m.dim), sizeof(int32_t) * 4);
-  k.append(reinterpret_cast<const char *>(m.cells), sizeof(int32_t) * (size_t)m.num_cells * m.verts_per_cell);
-  return k;
-}
-
-template <class T>
-int copy_h2d(fsale_ctx *ctx, T *dst, const T *src, size_t n) {
-  if (n) {
-    FSALE_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-    FSALE_HIP(hipStreamSynchronize(ctx->stream));
+// Uploads of one admission: every array is copied into a region of the
+// context's pinned staging buffer and its transfer is queued at once on the
+// upload stream (stream order keeps it ahead of the geometry and initial-state
+// kernels queued there), so an admission overlaps a running step round; one
+// synchronisation of the upload stream at the end makes the new problem's
+// data visible to later work and releases the buffer.  A running round
+// cannot see the new problem: its tiles and its entry in the problem list are
+// uploaded by the next round's launch.
+struct Stager {
+  fsale_ctx *ctx;
+  size_t used = 0;
+  explicit Stager(fsale_ctx *c) : ctx(c) {}
+  int reserve(size_t bytes) {
+    bytes += 64 * 64;
+    if (bytes <= ctx->stage_bytes) return FSALE_OK;
+    FSALE_HIP(hipStreamSynchronize(ctx->upload));
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_bytes = 0;
+    FSALE_HIP(hipHostMalloc(&ctx->stage, bytes, hipHostMallocDefault));
+    ctx->stage_bytes = bytes;
+    return FSALE_OK;
   }
-  return FSALE_OK;
-}
+  // a region of n elements, to be filled by the caller before put()
+  template <class T>
+  T *slot(size_t n) {
+    const size_t off = (used + 63) & ~size_t(63);
+    if (off + n * sizeof(T) > ctx->stage_bytes) return nullptr;
+    used = off + n * sizeof(T);
+    return reinterpret_cast<T *>(static_cast<char *>(ctx->stage) + off);
+  }
+  template <class T>
+  int put(T *dst, const T *region, size_t n) {
+    if (n) FSALE_HIP(hipMemcpyAsync(dst, region, n * sizeof(T), hipMemcpyHostToDevice, ctx->upload));
+    return FSALE_OK;
+  }
+  template <class T>
+  int copy(T *dst, const T *src, size_t n) {
+    T *r = slot<T>(n);
+    if (!r) {
+      set_message("staging buffer overflow");
+      return FSALE_ERR_ARG;
+    }
+    std::memcpy(r, src, n * sizeof(T));
+    return put(dst, r, n);
+  }
+  int finish() {
+    FSALE_HIP(hipStreamSynchronize(ctx->upload));
+    used = 0;
+    return FSALE_OK;
+  }
+};
 
 int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, long long cap_c, long long cap_f,
                 long long cap_v, long long cap_n) {
@@ -305,9 +353,8 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if ((rc = dev_alloc_n(ctx, &m.cminlen, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.farea, cap_f))) return rc;
   if (solver == FSALE_SOLVER_MAIRE) {
-    if ((rc = dev_alloc_n(ctx, &m.vc_start, cap_v))) return rc;
     if ((rc = dev_alloc_n(ctx, &m.vc_cnt, cap_v))) return rc;
-    if ((rc = dev_alloc_n(ctx, &m.vc, (size_t)cap_c * vpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.vck, (size_t)cap_v * kMaxCornersPerVertex))) return rc;
     if ((rc = dev_alloc_n(ctx, &m.cwtag, (size_t)cap_c * vpc))) return rc;
     if ((rc = dev_alloc_n(ctx, &m.vbnd, cap_v))) return rc;
   }
@@ -316,7 +363,8 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if ((rc = dev_alloc_n(ctx, &ctx->d_cdirect, ((size_t)cap_ct + 7) / 8 * 8))) return rc;
   if (solver == FSALE_SOLVER_MAIRE)
     if ((rc = dev_alloc_n(ctx, &ctx->d_vdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
-  if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 2))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 3))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->d_plist, max_problems))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_params, max_problems))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->s, max_problems))) return rc;
   FSALE_HIP(hipHostMalloc((void **)&ctx->h_s, sizeof(DevScalars) * max_problems, hipHostMallocDefault));
@@ -328,7 +376,7 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   std::vector<ProbParams> zero(max_problems);
   std::memset(zero.data(), 0, sizeof(ProbParams) * max_problems);
   FSALE_HIP(hipMemcpy(ctx->d_params, zero.data(), sizeof(ProbParams) * max_problems, hipMemcpyHostToDevice));
-  const int counts[2] = {0, 0};
+  const int counts[3] = {0, 0, 0};
   FSALE_HIP(hipMemcpy(ctx->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice));
   ctx->probs.assign(max_problems, Problem{});
   if (solver == FSALE_SOLVER_MAIRE) return maire_alloc(ctx);
@@ -336,8 +384,9 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
 }
 
 // places one mesh in the batch: ranges, connectivity (offset to batch ids),
-// coordinates, geometry, boundary tags
-int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob) {
+// coordinates, geometry, boundary tags, parameters and scalars, all through
+// `st` (the caller finishes it)
+int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stager &st) {
   DevMesh &m = ctx->m;
   if (mesh.dim != m.dim || mesh.verts_per_cell != m.vpc) {
     set_message("mesh type differs from the batch's");
@@ -354,23 +403,12 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob) {
     set_message("no free problem slot");
     return FSALE_ERR_OOM;
   }
-  // connectivity, cached by cell table
-  std::shared_ptr<const HostTopology> topo;
-  const std::string key = mesh_key(mesh);
-  auto it = ctx->topo_cache.find(key);
-  if (it != ctx->topo_cache.end()) {
-    topo = it->second;
-  } else {
-    auto t = std::make_shared<HostTopology>();
-    std::string err;
-    int rc = build_topology(mesh, *t, err);
-    if (rc) {
-      set_message(err);
-      return rc;
-    }
-    if (ctx->topo_cache.size() > 1024) ctx->topo_cache.clear();
-    topo = t;
-    ctx->topo_cache[key] = t;
+  int rc;
+  std::string err;
+  std::shared_ptr<const HostTopology> topo = cached_topology(mesh, rc, err);
+  if (rc) {
+    set_message(err);
+    return rc;
   }
   const HostTopology &t = *topo;
   for (int v = 0; v < t.nv; ++v)
@@ -412,107 +450,136 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob) {
   ctx->probs[prob] = P;
   ctx->tiles_dirty = true;
   *out_prob = prob;
-  // connectivity with batch ids, one k-row segment at a time
   const int nc = t.nc, nf = t.nf, nv = t.nv;
-  std::vector<int32_t> buf;
-  int rc;
-  for (int k = 0; k < t.vpc; ++k) {
-    buf.resize(nc);
-    for (int c = 0; c < nc; ++c) buf[c] = t.cv[(size_t)c * t.vpc + k] + P.v0;
-    if ((rc = copy_h2d(ctx, m.cv + (size_t)k * m.cap_c + P.c0, buf.data(), nc))) return rc;
-  }
-  for (int k = 0; k < t.fpc; ++k) {
-    buf.resize(nc);
-    for (int c = 0; c < nc; ++c) {
-      const int f = t.cf[(size_t)c * t.fpc + k];
-      buf[c] = 2 * (f + P.f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
+  const bool maire = ctx->solver == FSALE_SOLVER_MAIRE;
+  // staging: connectivity rows, coordinates, Lagrangian lists and tags, and
+  // room for the caller's state arrays, parameters and scalars
+  size_t bytes = 4 * ((size_t)nc * (t.vpc + 2 * t.fpc + m.nbk_deg + 2) + (size_t)nf * (2 + t.vpf) + P.nn) +
+                 8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
+  if (maire) bytes += 4 * (size_t)nv * (1 + kMaxCornersPerVertex) + (size_t)nc * t.vpc + nv;
+  if ((rc = st.reserve(bytes + 64 * 64))) return rc;
+  auto rows = [&](int32_t *dst_base, int rows_n, size_t stride, int n, auto &&value) -> int {
+    for (int k = 0; k < rows_n; ++k) {
+      int32_t *r = st.slot<int32_t>(n);
+      if (!r) {
+        set_message("staging buffer overflow");
+        return FSALE_ERR_ARG;
+      }
+      for (int i = 0; i < n; ++i) r[i] = value(k, i);
+      int rc2 = st.put(dst_base + (size_t)k * stride + 0, r, n);
+      if (rc2) return rc2;
     }
-    if ((rc = copy_h2d(ctx, m.cf + (size_t)k * m.cap_c + P.c0, buf.data(), nc))) return rc;
-    for (int c = 0; c < nc; ++c) {
-      const int f = t.cf[(size_t)c * t.fpc + k];
-      const bool second = t.fc[2 * (size_t)f] != c;
-      const int other = second ? t.fc[2 * (size_t)f] : t.fc[2 * (size_t)f + 1];
-      buf[c] = (other >= 0 ? other + P.c0 + 1 : 0) * 2 + (second ? 1 : 0);
-    }
-    if ((rc = copy_h2d(ctx, m.cfn + (size_t)k * m.cap_c + P.c0, buf.data(), nc))) return rc;
-  }
-  for (int k = 0; k < 2; ++k) {
-    buf.resize(nf);
-    for (int f = 0; f < nf; ++f) {
-      const int c = t.fc[2 * (size_t)f + k];
-      buf[f] = c >= 0 ? c + P.c0 : -1;
-    }
-    if ((rc = copy_h2d(ctx, m.fc + (size_t)k * m.cap_f + P.f0, buf.data(), nf))) return rc;
-  }
-  for (int k = 0; k < t.vpf; ++k) {
-    buf.resize(nf);
-    for (int f = 0; f < nf; ++f) buf[f] = t.fv[(size_t)f * t.vpf + k] + P.v0;
-    if ((rc = copy_h2d(ctx, m.fv + (size_t)k * m.cap_f + P.f0, buf.data(), nf))) return rc;
-  }
-  {
-    std::vector<int32_t> st(nc), cnt(nc), nb(P.nn);
-    for (int c = 0; c < nc; ++c) {
-      st[c] = t.nbr_off[c] + P.n0;
-      cnt[c] = t.nbr_off[c + 1] - t.nbr_off[c];
-    }
-    for (int i = 0; i < P.nn; ++i) nb[i] = t.nbr[i] + P.c0;
-    if ((rc = copy_h2d(ctx, m.nbr_start + P.c0, st.data(), nc))) return rc;
-    if ((rc = copy_h2d(ctx, m.nbr_cnt + P.c0, cnt.data(), nc))) return rc;
-    if ((rc = copy_h2d(ctx, m.nbr + P.n0, nb.data(), P.nn))) return rc;
-    for (int k = 0; k < m.nbk_deg; ++k) {
-      for (int c = 0; c < nc; ++c) st[c] = k < cnt[c] ? nb[t.nbr_off[c] + k] : -1;
-      if ((rc = copy_h2d(ctx, m.nbk + (size_t)k * m.cap_c + P.c0, st.data(), nc))) return rc;
-    }
-  }
+    return FSALE_OK;
+  };
+  // connectivity with batch ids, one k-row segment at a time
+  if ((rc = rows(m.cv + P.c0, t.vpc, m.cap_c, nc, [&](int k, int c) { return t.cv[(size_t)c * t.vpc + k] + P.v0; })))
+    return rc;
+  if ((rc = rows(m.cf + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
+         const int f = t.cf[(size_t)c * t.fpc + k];
+         return 2 * (f + P.f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
+       })))
+    return rc;
+  if ((rc = rows(m.cfn + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
+         const int f = t.cf[(size_t)c * t.fpc + k];
+         const bool second = t.fc[2 * (size_t)f] != c;
+         const int other = second ? t.fc[2 * (size_t)f] : t.fc[2 * (size_t)f + 1];
+         return (other >= 0 ? other + P.c0 + 1 : 0) * 2 + (second ? 1 : 0);
+       })))
+    return rc;
+  if ((rc = rows(m.fc + P.f0, 2, m.cap_f, nf, [&](int k, int f) {
+         const int c = t.fc[2 * (size_t)f + k];
+         return c >= 0 ? c + P.c0 : -1;
+       })))
+    return rc;
+  if ((rc = rows(m.fv + P.f0, t.vpf, m.cap_f, nf, [&](int k, int f) { return t.fv[(size_t)f * t.vpf + k] + P.v0; })))
+    return rc;
+  if ((rc = rows(m.nbr_start + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c] + P.n0; }))) return rc;
+  if ((rc = rows(m.nbr_cnt + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c + 1] - t.nbr_off[c]; }))) return rc;
+  if ((rc = rows(m.nbr + P.n0, 1, 0, P.nn, [&](int, int i) { return t.nbr[i] + P.c0; }))) return rc;
+  if ((rc = rows(m.nbk + P.c0, m.nbk_deg, m.cap_c, nc, [&](int k, int c) {
+         const int cnt = t.nbr_off[c + 1] - t.nbr_off[c];
+         return k < cnt ? t.nbr[t.nbr_off[c] + k] + P.c0 : -1;
+       })))
+    return rc;
   for (int d = 0; d < m.dim; ++d) {
-    std::vector<double> xd(nv);
-    for (int v = 0; v < nv; ++v) xd[v] = mesh.coords[(size_t)v * m.dim + d];
-    if ((rc = copy_h2d(ctx, m.x[d] + P.v0, xd.data(), nv))) return rc;
+    double *x = st.slot<double>(nv);
+    if (!x) return FSALE_ERR_ARG;
+    for (int v = 0; v < nv; ++v) x[v] = mesh.coords[(size_t)v * m.dim + d];
+    if ((rc = st.put(m.x[d] + P.v0, x, nv))) return rc;
   }
-  if ((rc = launch_geometry(ctx, prob, true, true))) return rc;
-  if (ctx->solver == FSALE_SOLVER_MAIRE) {
+  if ((rc = launch_geometry(ctx, prob, true, true, ctx->upload))) return rc;
+  if (maire) {
     // vertex -> corner lists and boundary tags from the initial face normals:
-    // tag d when |n_d| is 1 within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162)
-    std::vector<int32_t> st(nv), cnt(nv), vc(t.vc.size());
-    for (int v = 0; v < nv; ++v) {
-      st[v] = t.vc_off[v] + P.c0 * t.vpc;
-      cnt[v] = t.vc_off[v + 1] - t.vc_off[v];
-    }
-    for (size_t i = 0; i < t.vc.size(); ++i) vc[i] = t.vc[i] + P.c0 * t.vpc;
-    if ((rc = copy_h2d(ctx, m.vc_start + P.v0, st.data(), nv))) return rc;
-    if ((rc = copy_h2d(ctx, m.vc_cnt + P.v0, cnt.data(), nv))) return rc;
-    if ((rc = copy_h2d(ctx, m.vc + (size_t)P.c0 * t.vpc, vc.data(), vc.size()))) return rc;
-    std::vector<double> fn[2];
-    for (int d = 0; d < 2; ++d) {
-      fn[d].resize(nf);
-      FSALE_HIP(hipMemcpyAsync(fn[d].data(), m.fnrm[d] + P.f0, nf * sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    }
-    FSALE_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<uint8_t> ftag(nf, 0), vbnd(nv, 0);
+    // tag d when |n_d| is 1 within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162);
+    // the normals are evaluated here with the device's formula
+    if ((rc = rows(m.vc_cnt + P.v0, 1, 0, nv, [&](int, int v) { return t.vc_off[v + 1] - t.vc_off[v]; }))) return rc;
+    if ((rc = rows(m.vck + P.v0, kMaxCornersPerVertex, m.cap_v, nv, [&](int j, int v) {
+           return j < t.vc_off[v + 1] - t.vc_off[v] ? t.vc[t.vc_off[v] + j] + P.c0 * t.vpc : -1;
+         })))
+      return rc;
+    std::vector<uint8_t> ftag(nf, 0);
+    uint8_t *vbnd = st.slot<uint8_t>(nv);
+    if (!vbnd) return FSALE_ERR_ARG;
+    std::memset(vbnd, 0, nv);
     for (int f = 0; f < nf; ++f) {
       if (t.fc[2 * (size_t)f + 1] >= 0) continue;
+      const int a = t.fv[2 * (size_t)f], b = t.fv[2 * (size_t)f + 1];
+      double cx, cy, nx, ny, area;
+      face_geom_2d(mesh.coords[2 * (size_t)a], mesh.coords[2 * (size_t)a + 1], mesh.coords[2 * (size_t)b],
+                   mesh.coords[2 * (size_t)b + 1], cx, cy, nx, ny, area);
+      const double n[2] = {nx, ny};
       uint8_t tag = 0x4;   // bit 2: boundary face
       for (int d = 0; d < 2; ++d)
-        if (std::fabs(std::fabs(fn[d][f]) - 1.0) <= 1e-14) tag |= (uint8_t)(1u << d);
+        if (std::fabs(std::fabs(n[d]) - 1.0) <= 1e-14) tag |= (uint8_t)(1u << d);
       ftag[f] = tag;
       for (int i = 0; i < t.vpf; ++i) vbnd[t.fv[(size_t)f * t.vpf + i]] = 1;
     }
     // corner (c,k): low nibble = face of edge (k-1 -> k), high nibble = edge (k -> k+1)
     for (int k = 0; k < t.vpc; ++k) {
-      std::vector<uint8_t> cw(nc);
+      uint8_t *cw = st.slot<uint8_t>(nc);
+      if (!cw) return FSALE_ERR_ARG;
+      const int km = (k + t.vpc - 1) % t.vpc;
       for (int c = 0; c < nc; ++c) {
-        const int km = (k + t.vpc - 1) % t.vpc;
         const int fm = t.cf[(size_t)c * t.fpc + km], fp = t.cf[(size_t)c * t.fpc + k];
         cw[c] = (uint8_t)(ftag[fm] | (ftag[fp] << 4));
       }
-      if ((rc = copy_h2d(ctx, m.cwtag + (size_t)k * m.cap_c + P.c0, cw.data(), nc))) return rc;
+      if ((rc = st.put(m.cwtag + (size_t)k * m.cap_c + P.c0, cw, nc))) return rc;
     }
-    if ((rc = copy_h2d(ctx, m.vbnd + P.v0, vbnd.data(), nv))) return rc;
+    if ((rc = st.put(m.vbnd + P.v0, vbnd, nv))) return rc;
   }
-  if ((rc = upload_params(ctx, prob))) return rc;
-  if ((rc = reset_scalars(ctx, prob, 0.0, 0.0))) return rc;
-  *out_prob = prob;
+  return FSALE_OK;
+}
+
+// A problem's parameters and fresh scalars (time t, time step dt), staged.
+int stage_scalars(fsale_ctx *ctx, int prob, Stager &st, double t, double dt) {
+  DevScalars h{};
+  h.t = t;
+  h.dt = dt;
+  h.active = 1;
+  h.err_cell = INT64_MAX;
+  ctx->h_s[prob] = h;
+  int rc = st.copy(ctx->d_params + prob, &ctx->probs[prob].params, 1);
+  return rc ? rc : st.copy(ctx->s + prob, &ctx->h_s[prob], 1);
+}
+
+// Initial density, velocity (interleaved [nc*dim]), pressure and, for the
+// Lagrangian solver, gamma of one problem, staged.
+int stage_state(fsale_ctx *ctx, int prob, Stager &st, const double *d, const double *v, const double *p,
+                const double *gamma) {
+  const Problem &P = ctx->probs[prob];
+  const int N = ctx->m.dim, nc = P.nc;
+  const bool maire = ctx->solver == FSALE_SOLVER_MAIRE;
+  double *dd = maire ? ctx->ma.d : ctx->eu.d;
+  double *pp = maire ? ctx->ma.p : ctx->eu.p;
+  int rc;
+  if ((rc = st.copy(dd + P.c0, d, nc)) || (rc = st.copy(pp + P.c0, p, nc))) return rc;
+  for (int k = 0; k < N; ++k) {
+    double *r = st.slot<double>(nc);
+    if (!r) return FSALE_ERR_ARG;
+    for (int c = 0; c < nc; ++c) r[c] = v[(size_t)c * N + k];
+    if ((rc = st.put((maire ? ctx->ma.v[k] : ctx->eu.v[k]) + P.c0, r, nc))) return rc;
+  }
+  if (maire && (rc = st.copy(ctx->ma.g + P.c0, gamma, nc))) return rc;
   return FSALE_OK;
 }
 
@@ -666,7 +733,7 @@ int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in
     }
   }
   ctx->probs[prob].partials_valid = false;
-  if (field == FSALE_F_COORDS) return launch_geometry(ctx, prob, true, true);
+  if (field == FSALE_F_COORDS) return launch_geometry(ctx, prob, true, true, ctx->stream);
   return FSALE_OK;
 }
 
@@ -678,8 +745,6 @@ int set_problem_state(fsale_ctx *ctx, int prob, const double *d, const double *v
   return set_problem_field(ctx, prob, FSALE_F_PRESSURE, p);
 }
 
-int euler_initial_state(fsale_ctx *ctx, int prob);
-int maire_initial_state(fsale_ctx *ctx, int prob);
 
 // Driver steps of every live problem; chunks of up to 10 steps are replayed
 // from a captured graph (the graph reads tiles, parameters and scalars from
@@ -707,6 +772,7 @@ int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   const int chunk = nsteps < 10 ? nsteps : 10;
   std::vector<double> key = {(double)ctx->solver, (double)chunk};
   for (int k = 0; k < K_COUNT; ++k) key.push_back((double)persistent_grid(ctx, k));
+  key.push_back((double)ctx->pgrid);
   if (!ctx->graph.exec || ctx->graph.key != key) {
     if (ctx->graph.exec) {
       (void)hipGraphExecDestroy(ctx->graph.exec);
@@ -891,7 +957,9 @@ int fsale_batch_create(int solver, int dim, int verts_per_cell, int32_t max_prob
   (void)hipGetDevice(&ctx->device);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->upload, hipStreamNonBlocking) != hipSuccess) {
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     set_message("hipStreamCreate failed");
     return FSALE_ERR_HIP;
@@ -910,9 +978,9 @@ int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out) {
   if (!mesh || !out) return FSALE_ERR_ARG;
   *out = nullptr;
   // exact capacity from the mesh's own connectivity
-  auto t = std::make_shared<HostTopology>();
   std::string err;
-  int rc = build_topology(*mesh, *t, err);
+  int rc;
+  std::shared_ptr<const HostTopology> t = cached_topology(*mesh, rc, err);
   if (rc) {
     set_message(err);
     return rc;
@@ -930,9 +998,10 @@ int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out) {
         return rc;
       }
   }
-  ctx->topo_cache[mesh_key(*mesh)] = t;
   int prob;
-  if ((rc = place_mesh(ctx, *mesh, &prob))) {
+  Stager st(ctx);
+  if ((rc = place_mesh(ctx, *mesh, &prob, st)) || (rc = stage_scalars(ctx, prob, st, 0.0, 0.0)) ||
+      (rc = st.finish())) {
     fsale_ctx_destroy(ctx);
     return rc;
   }
@@ -943,6 +1012,7 @@ int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out) {
 int fsale_ctx_destroy(fsale_ctx *ctx) {
   if (!ctx) return FSALE_ERR_ARG;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->upload) (void)hipStreamSynchronize(ctx->upload);
   for (auto &p : ctx->prof) {
     for (hipEvent_t e : p.start) (void)hipEventDestroy(e);
     for (hipEvent_t e : p.stop) (void)hipEventDestroy(e);
@@ -950,7 +1020,9 @@ int fsale_ctx_destroy(fsale_ctx *ctx) {
   if (ctx->graph.exec) (void)hipGraphExecDestroy(ctx->graph.exec);
   for (void *p : ctx->allocations) (void)hipFree(p);
   if (ctx->h_s) (void)hipHostFree(ctx->h_s);
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->upload) (void)hipStreamDestroy(ctx->upload);
   delete ctx;
   return FSALE_OK;
 }
@@ -1096,7 +1168,9 @@ int fsale_batch_add_euler(fsale_ctx *ctx, const fsale_mesh_desc *mesh, fsale_eos
       (prm->flux_kind >> 4) > 1)
     return FSALE_ERR_ARG;
   int prob = -1, rc;
-  if ((rc = place_mesh(ctx, *mesh, &prob))) {
+  Stager st(ctx);
+  if ((rc = place_mesh(ctx, *mesh, &prob, st))) {
+    (void)st.finish();
     if (prob >= 0 && prob < ctx->max_problems && ctx->probs[prob].live) fsale_batch_remove(ctx, prob);
     return rc;
   }
@@ -1109,8 +1183,9 @@ int fsale_batch_add_euler(fsale_ctx *ctx, const fsale_mesh_desc *mesh, fsale_eos
   P.params.first_order = prm->first_order != 0;
   P.params.stop_at_final = prm->stop_at_final != 0;
   P.params.max_steps = prm->max_steps;
-  if ((rc = upload_params(ctx, prob)) || (rc = set_problem_state(ctx, prob, d, v, p)) ||
-      (rc = euler_initial_state(ctx, prob)) || (rc = reset_scalars(ctx, prob, 0.0, 0.0))) {
+  if ((rc = stage_scalars(ctx, prob, st, 0.0, 0.0)) || (rc = stage_state(ctx, prob, st, d, v, p, nullptr)) ||
+      (rc = euler_initial_state(ctx, prob, ctx->upload)) || (rc = st.finish())) {
+    (void)st.finish();
     fsale_batch_remove(ctx, prob);
     return rc;
   }
@@ -1129,7 +1204,9 @@ int fsale_batch_add_maire(fsale_ctx *ctx, const fsale_mesh_desc *mesh, double cv
   for (int c = 0; c < mesh->num_cells; ++c)
     if (!(gamma[c] > 1)) return FSALE_ERR_ARG;
   int prob = -1, rc;
-  if ((rc = place_mesh(ctx, *mesh, &prob))) {
+  Stager st(ctx);
+  if ((rc = place_mesh(ctx, *mesh, &prob, st))) {
+    (void)st.finish();
     if (prob >= 0 && prob < ctx->max_problems && ctx->probs[prob].live) fsale_batch_remove(ctx, prob);
     return rc;
   }
@@ -1140,9 +1217,10 @@ int fsale_batch_add_maire(fsale_ctx *ctx, const fsale_mesh_desc *mesh, double cv
   P.params.cfl_growth = prm->cfl_growth;
   P.params.final_time = prm->final_time;
   P.params.max_steps = prm->max_steps;
-  if ((rc = upload_params(ctx, prob)) || (rc = set_problem_field(ctx, prob, FSALE_F_GAMMA, gamma)) ||
-      (rc = set_problem_state(ctx, prob, d, v, p)) || (rc = maire_initial_state(ctx, prob)) ||
-      (rc = reset_scalars(ctx, prob, 0.0, prm->initial_time_step))) {
+  if ((rc = stage_scalars(ctx, prob, st, 0.0, prm->initial_time_step)) ||
+      (rc = stage_state(ctx, prob, st, d, v, p, gamma)) || (rc = maire_initial_state(ctx, prob, ctx->upload)) ||
+      (rc = st.finish())) {
+    (void)st.finish();
     fsale_batch_remove(ctx, prob);
     return rc;
   }

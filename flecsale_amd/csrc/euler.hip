@@ -72,9 +72,10 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerField
 // Per problem: dt = min(CFL / max dt^-1, t_final - t) and the step flag.
 // driver != 0: the step loop (step budget, optional stop at t_final);
 // driver == 0: the task-level evaluate_time_step.
-__global__ void __launch_bounds__(kFinalBlock) k_eul_dt_finalize(Batch b, const double *partials, int nprob,
-                                                                int driver) {
-  for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
+__global__ void __launch_bounds__(kFinalBlock) k_eul_dt_finalize(Batch b, const double *partials, int driver) {
+  const int nprob = *b.np;
+  for (int ip = blockIdx.x; ip < nprob; ip += gridDim.x) {
+    const int p = b.plist[ip];
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
     double mx[1];
@@ -513,10 +514,8 @@ int launch_dt_partial(fsale_ctx *ctx) {
 
 int launch_finalize(fsale_ctx *ctx, int driver) {
   ProfScope prof(ctx, K_EUL_DT);
-  const int np = (int)ctx->probs.size();
-  const int grid = np < 1024 ? np : 1024;
-  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(grid > 0 ? grid : 1), dim3(kFinalBlock), 0, ctx->stream,
-                     batch_args(ctx, -1), (const double *)ctx->eu.partials, np, driver);
+  hipLaunchKernelGGL(k_eul_dt_finalize, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream, batch_args(ctx, -1),
+                     (const double *)ctx->eu.partials, driver);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -594,11 +593,11 @@ int euler_alloc(fsale_ctx *ctx) {
 }
 
 // initial conditions of one problem: d, v, p already in place
-int euler_initial_state(fsale_ctx *ctx, int prob) {
+int euler_initial_state(fsale_ctx *ctx, int prob, hipStream_t stream) {
   const Problem &P = ctx->probs[prob];
   const int g = (P.nc + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_eul_ic, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->eu, (const ProbParams *)ctx->d_params,
-                     prob);
+  if (g == 0) return FSALE_OK;
+  hipLaunchKernelGGL(k_eul_ic, dim3(g), dim3(kBlock), 0, stream, ctx->eu, (const ProbParams *)ctx->d_params, prob);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -625,7 +624,7 @@ int fsale_eul_initial_conditions(fsale_ctx *ctx, fsale_eos eos, const double *d,
   int rc;
   if ((rc = upload_params(ctx, 0))) return rc;
   if ((rc = set_problem_state(ctx, 0, d, v, p))) return rc;
-  if ((rc = euler_initial_state(ctx, 0))) return rc;
+  if ((rc = euler_initial_state(ctx, 0, ctx->stream))) return rc;
   P.have_state = true;
   P.partials_valid = false;
   return reset_scalars(ctx, 0, 0.0, 0.0);

@@ -85,23 +85,23 @@ __global__ void __launch_bounds__(kBlock) k_face_geom_3d(DevMesh m, int f0, int 
 
 }  // namespace
 
-int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces) {
+int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces, hipStream_t stream) {
   const DevMesh &m = ctx->m;
   const Problem &P = ctx->probs[prob];
   if (cells && P.nc > 0) {
     const int g = (P.nc + kBlock - 1) / kBlock;
     if (m.dim == 2)
-      hipLaunchKernelGGL(k_cell_geom_2d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.c0, P.nc);
+      hipLaunchKernelGGL(k_cell_geom_2d, dim3(g), dim3(kBlock), 0, stream, m, P.c0, P.nc);
     else
-      hipLaunchKernelGGL(k_cell_geom_3d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.c0, P.nc);
+      hipLaunchKernelGGL(k_cell_geom_3d, dim3(g), dim3(kBlock), 0, stream, m, P.c0, P.nc);
     FSALE_HIP(hipGetLastError());
   }
   if (faces && P.nf > 0) {
     const int g = (P.nf + kBlock - 1) / kBlock;
     if (m.dim == 2)
-      hipLaunchKernelGGL(k_face_geom_2d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.f0, P.nf);
+      hipLaunchKernelGGL(k_face_geom_2d, dim3(g), dim3(kBlock), 0, stream, m, P.f0, P.nf);
     else
-      hipLaunchKernelGGL(k_face_geom_3d, dim3(g), dim3(kBlock), 0, ctx->stream, m, P.f0, P.nf);
+      hipLaunchKernelGGL(k_face_geom_3d, dim3(g), dim3(kBlock), 0, stream, m, P.f0, P.nf);
     FSALE_HIP(hipGetLastError());
   }
   return FSALE_OK;

@@ -60,6 +60,8 @@ struct HostTopology {
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
+// connectivity of a mesh from the process-wide cache (built on a miss)
+std::shared_ptr<const HostTopology> cached_topology(const fsale_mesh_desc &mesh, int &rc, std::string &err);
 
 // Device-side mesh storage of a batch: SoA geometry, k-major fixed-degree
 // connectivity with row stride = capacity, global (batch) entity ids.
@@ -82,9 +84,9 @@ struct DevMesh {
   // cell across local face k: (cell + 1) * 2 + (this cell is the face's
   // cells[1]); boundary faces hold 0
   int32_t *cfn = nullptr;       // [fpc][cap_c]
-  int32_t *vc_start = nullptr;  // [cap_v] vertex -> corner list start
-  int32_t *vc_cnt = nullptr;    // [cap_v]
-  int32_t *vc = nullptr;        // corner = c*vpc + k (global cell id) [cap_c*vpc]
+  int32_t *vc_cnt = nullptr;    // [cap_v] corners of each vertex
+  int32_t *vck = nullptr;       // [kMaxCornersPerVertex][cap_v] corner c*vpc + k (batch cell
+                                // id), ascending cell id, -1 padded
   uint8_t *cwtag = nullptr;     // [vpc][cap_c] Maire wedge boundary tags
   uint8_t *vbnd = nullptr;      // [cap_v] vertex on a boundary face
   double *ccen[3] = {nullptr, nullptr, nullptr};
@@ -162,6 +164,8 @@ struct Batch {
   const ProbParams *pp = nullptr;
   DevScalars *s = nullptr;
   int cap_c_tiles = 0;        // partials stride
+  const int *plist = nullptr; // problems of the tile lists, in id order
+  const int *np = nullptr;    // their count
 };
 
 struct GraphCache {
@@ -236,13 +240,17 @@ struct fsale_ctx {
   fsale::Tile *d_ctiles = nullptr, *d_vtiles = nullptr;
   fsale::Tile *d_cdirect = nullptr, *d_vdirect = nullptr;   // one-block-per-tile dispatch lists
   int n_cdirect = 0, n_vdirect = 0;
-  int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles
+  int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles, [2] problems
+  int *d_plist = nullptr;              // problems with tiles in the lists
+  int pgrid = 1;                       // blocks of the per-problem kernels
+  hipStream_t upload = nullptr;        // admissions: uploads, geometry, initial state
   fsale::ProbParams *d_params = nullptr;
   fsale::DevScalars *s = nullptr;      // [max_problems] device
   fsale::DevScalars *h_s = nullptr;    // pinned host mirror
   std::vector<fsale::Tile> h_ctiles, h_vtiles;
   bool tiles_dirty = true;
-  std::map<std::string, std::shared_ptr<const fsale::HostTopology>> topo_cache;
+  void *stage = nullptr;               // pinned staging buffer for uploads
+  size_t stage_bytes = 0;
   fsale::GraphCache graph;
   fsale::LaunchShape shape;
   std::vector<void *> allocations;
@@ -272,8 +280,10 @@ int upload_params(fsale_ctx *ctx, int prob);
 Batch batch_args(const fsale_ctx *ctx, int kid);
 int persistent_grid(const fsale_ctx *ctx, int kid);
 
-// geometry (geom.hip): cells/faces of one problem
-int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces);
+// geometry (geom.hip): cells/faces of one problem, on `stream`
+int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces, hipStream_t stream);
+int euler_initial_state(fsale_ctx *ctx, int prob, hipStream_t stream);
+int maire_initial_state(fsale_ctx *ctx, int prob, hipStream_t stream);
 // Euler (euler.hip)
 int euler_alloc(fsale_ctx *ctx);
 int euler_dt_partials(fsale_ctx *ctx);

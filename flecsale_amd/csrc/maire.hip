@@ -84,110 +84,151 @@ __device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int
   }
 }
 
-// nodal solve of one vertex (batch id vt)
-__device__ inline void nodal_vertex(const DevMesh &m, const MaireFields &u, int vt) {
-  const int q0 = m.vc_start[vt], q1 = q0 + m.vc_cnt[vt];
-  double Mp[2][2] = {{0, 0}, {0, 0}}, rhs[2] = {0, 0};
-  for (int q = q0; q < q1; ++q) {
-    const int cn = m.vc[q];
-    const int c = cn >> 2, k = cn & 3;
-    double M[2][2], np[2];
-    corner_matrix(m, u, c, k, M, np);
-    for (int r = 0; r < 2; ++r) {
-      Mp[r][0] += M[r][0];
-      Mp[r][1] += M[r][1];
-    }
-    const double uc[2] = {u.v[0][c], u.v[1][c]};
-    const double pc = u.p[c];
-    double F[2];
-    for (int r = 0; r < 2; ++r) {
-      F[r] = 0;
-      F[r] += M[r][0] * uc[0];
-      F[r] += M[r][1] * uc[1];
-    }
-    for (int d = 0; d < 2; ++d) {
-      F[d] += pc * np[d];
-      rhs[d] += F[d];
-    }
-    const size_t ci = (size_t)k * m.cap_c + c;
-    u.npc[0][ci] = np[0];
-    u.npc[1][ci] = np[1];
-    u.Fpc[0][ci] = F[0];
-    u.Fpc[1][ci] = F[1];
-  }
-  double un[2];
+// Symmetry-constrained nodal velocity of boundary vertex vt
+// (apps/maire_hydro/tasks.h:317-420): symmetry normals accumulated per tag
+// over the vertex's boundary wedges in corner order, then the augmented system
+// [M_p N^T; N 0] solved by flecsale::linalg::qr (:383-418); Cramer's rule
+// when no wedge carries a symmetry tag.
+__device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const double Mp[2][2], const double rhs[2],
+                                      double un[2]) {
   int has[2] = {0, 0};
   double symn[2][2] = {{0, 0}, {0, 0}};
-  if (m.vbnd[vt]) {
-    // symmetry normals, accumulated per tag over the vertex's boundary wedges
-    for (int q = q0; q < q1; ++q) {
-      const int cn = m.vc[q];
-      const int c = cn >> 2, k = cn & 3;
-      const int tags = m.cwtag[(size_t)k * m.cap_c + c];
-      if (!(tags & 0x44)) continue;
-      double n[2][2], l[2];
-      wedge_facets(m, c, k, n, l);
-      for (int w = 0; w < 2; ++w) {
-        const int t = (tags >> (4 * w)) & 0xf;
-        if (!(t & 4)) continue;
-        for (int tag = 0; tag < 2; ++tag) {
-          if (!(t & (1 << tag))) continue;
-          if (has[tag]) {
-            for (int d = 0; d < 2; ++d) symn[tag][d] += l[w] * n[w][d];
-          } else {
-            has[tag] = 1;
-            for (int d = 0; d < 2; ++d) symn[tag][d] = l[w] * n[w][d];
-          }
+  for (int q = 0; q < cnt; ++q) {
+    const int cn = m.vck[(size_t)q * m.cap_v + vt];
+    const int c = cn >> 2, k = cn & 3;
+    const int tags = m.cwtag[(size_t)k * m.cap_c + c];
+    if (!(tags & 0x44)) continue;
+    double n[2][2], l[2];
+    wedge_facets(m, c, k, n, l);
+    for (int w = 0; w < 2; ++w) {
+      const int t = (tags >> (4 * w)) & 0xf;
+      if (!(t & 4)) continue;
+      for (int tag = 0; tag < 2; ++tag) {
+        if (!(t & (1 << tag))) continue;
+        if (has[tag]) {
+          for (int d = 0; d < 2; ++d) symn[tag][d] += l[w] * n[w][d];
+        } else {
+          has[tag] = 1;
+          for (int d = 0; d < 2; ++d) symn[tag][d] = l[w] * n[w][d];
         }
       }
     }
   }
   const int nsym = has[0] + has[1];
   if (nsym == 0) {
-    // ristra::math::solve (un-vendored): Cramer's rule
     const double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
     un[0] = (rhs[0] * Mp[1][1] - Mp[0][1] * rhs[1]) / det;
     un[1] = (Mp[0][0] * rhs[1] - rhs[0] * Mp[1][0]) / det;
-  } else {
-    // augmented system [M_p N^T; N 0] solved by flecsale::linalg::qr
-    // (apps/maire_hydro/tasks.h:383-418)
-    const int nr = 2 + nsym;
-    double A[16], b[4] = {rhs[0], rhs[1], 0, 0};
-    for (int i = 0; i < 16; ++i) A[i] = 0;
-    for (int r = 0; r < 2; ++r)
-      for (int q = 0; q < 2; ++q) A[r * nr + q] = Mp[r][q];
-    int jj = 2;
-    for (int tag = 0; tag < 2; ++tag) {
-      if (!has[tag]) continue;
-      for (int r = 0; r < 2; ++r) A[r * nr + jj] = symn[tag][r];
-      for (int q = 0; q < 2; ++q) A[jj * nr + q] = symn[tag][q];
-      ++jj;
-    }
-    qr_solve<4>(nr, A, b);
-    un[0] = b[0];
-    un[1] = b[1];
+    return;
   }
-  u.un[0][vt] = un[0];
-  u.un[1][vt] = un[1];
-  // F_pc -= M_pc u_p  (matrix_vector(-1, Mpc, un, 1, Fpc), tasks.h:435-448)
-  for (int q = q0; q < q1; ++q) {
-    const int cn = m.vc[q];
-    const int c = cn >> 2, k = cn & 3;
-    double M[2][2], np[2];
-    corner_matrix(m, u, c, k, M, np);
-    const size_t ci = (size_t)k * m.cap_c + c;
-    for (int r = 0; r < 2; ++r) {
-      double t = M[r][0] * un[0];
-      t += M[r][1] * un[1];
-      u.Fpc[r][ci] = -1.0 * t + 1.0 * u.Fpc[r][ci];
-    }
+  const int nr = 2 + nsym;
+  double A[16], bb[4] = {rhs[0], rhs[1], 0, 0};
+  for (int i = 0; i < 16; ++i) A[i] = 0;
+  for (int r = 0; r < 2; ++r)
+    for (int q = 0; q < 2; ++q) A[r * nr + q] = Mp[r][q];
+  int jj = 2;
+  for (int tag = 0; tag < 2; ++tag) {
+    if (!has[tag]) continue;
+    for (int r = 0; r < 2; ++r) A[r * nr + jj] = symn[tag][r];
+    for (int q = 0; q < 2; ++q) A[jj * nr + q] = symn[tag][q];
+    ++jj;
   }
+  qr_solve<4>(nr, A, bb);
+  un[0] = bb[0];
+  un[1] = bb[1];
+}
+
+// evaluate_nodal_state, one work-item per vertex.  The vertex's corners
+// (ascending cell id, k-major table) are visited in order; the first four
+// corner matrices stay in registers for the final F_pc -= M_pc u_p (a vertex
+// with more corners recomputes the rest), and the first four corners' loads
+// do not depend on each other.
+constexpr int kNodalKeep = 4;
+
+__device__ inline void corner_force(const MaireFields &u, int c, const double M[2][2], const double np[2],
+                                    double F[2]) {
+  const double uc[2] = {u.v[0][c], u.v[1][c]};
+  const double pc = u.p[c];
+  for (int r = 0; r < 2; ++r) {
+    F[r] = 0;
+    F[r] += M[r][0] * uc[0];
+    F[r] += M[r][1] * uc[1];
+  }
+  for (int d = 0; d < 2; ++d) F[d] += pc * np[d];
 }
 
 __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
   for_each_tile(b.vt, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
-    if ((int)threadIdx.x < tile.count) nodal_vertex(m, u, tile.first + threadIdx.x);
+    if ((int)threadIdx.x >= tile.count) return;
+    const int vt = tile.first + threadIdx.x;
+    const int cnt = m.vc_cnt[vt];
+    double M[kNodalKeep][2][2], np[kNodalKeep][2], F[kNodalKeep][2];
+    int cc[kNodalKeep], kk[kNodalKeep];
+#pragma unroll
+    for (int q = 0; q < kNodalKeep; ++q) {
+      const int cn = q < cnt ? m.vck[(size_t)q * m.cap_v + vt] : -1;
+      cc[q] = cn >> 2;
+      kk[q] = cn & 3;
+      if (cn >= 0) {
+        corner_matrix(m, u, cc[q], kk[q], M[q], np[q]);
+        corner_force(u, cc[q], M[q], np[q], F[q]);
+      }
+    }
+    // M_p = sum_c M_pc and rhs = sum_c F_pc in corner order
+    double Mp[2][2] = {{0, 0}, {0, 0}}, rhs[2] = {0, 0};
+#pragma unroll
+    for (int q = 0; q < kNodalKeep; ++q) {
+      if (q >= cnt) break;
+      for (int r = 0; r < 2; ++r) {
+        Mp[r][0] += M[q][r][0];
+        Mp[r][1] += M[q][r][1];
+        rhs[r] += F[q][r];
+      }
+    }
+    for (int q = kNodalKeep; q < cnt; ++q) {
+      const int cn = m.vck[(size_t)q * m.cap_v + vt];
+      double Mq[2][2], npq[2], Fq[2];
+      corner_matrix(m, u, cn >> 2, cn & 3, Mq, npq);
+      corner_force(u, cn >> 2, Mq, npq, Fq);
+      for (int r = 0; r < 2; ++r) {
+        Mp[r][0] += Mq[r][0];
+        Mp[r][1] += Mq[r][1];
+        rhs[r] += Fq[r];
+      }
+    }
+    double un[2];
+    if (m.vbnd[vt]) {
+      boundary_solve(m, vt, cnt, Mp, rhs, un);
+    } else {
+      // ristra::math::solve (un-vendored): Cramer's rule
+      const double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
+      un[0] = (rhs[0] * Mp[1][1] - Mp[0][1] * rhs[1]) / det;
+      un[1] = (Mp[0][0] * rhs[1] - rhs[0] * Mp[1][0]) / det;
+    }
+    u.un[0][vt] = un[0];
+    u.un[1][vt] = un[1];
+    // F_pc -= M_pc u_p  (matrix_vector(-1, Mpc, un, 1, Fpc), tasks.h:435-448)
+    auto finish = [&](int c, int k, const double Mq[2][2], const double npq[2], const double Fq[2]) {
+      const size_t ci = (size_t)k * m.cap_c + c;
+      u.npc[0][ci] = npq[0];
+      u.npc[1][ci] = npq[1];
+      for (int r = 0; r < 2; ++r) {
+        double t = Mq[r][0] * un[0];
+        t += Mq[r][1] * un[1];
+        u.Fpc[r][ci] = -1.0 * t + 1.0 * Fq[r];
+      }
+    };
+#pragma unroll
+    for (int q = 0; q < kNodalKeep; ++q)
+      if (q < cnt) finish(cc[q], kk[q], M[q], np[q], F[q]);
+    for (int q = kNodalKeep; q < cnt; ++q) {
+      const int cn = m.vck[(size_t)q * m.cap_v + vt];
+      double Mq[2][2], npq[2], Fq[2];
+      corner_matrix(m, u, cn >> 2, cn & 3, Mq, npq);
+      corner_force(u, cn >> 2, Mq, npq, Fq);
+      finish(cn >> 2, cn & 3, Mq, npq, Fq);
+    }
   });
 }
 
@@ -237,8 +278,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_dt_partial(DevMesh m, MaireFie
 
 // evaluate_time_step :179-188 (first minimum of the three limits) and the
 // driver's cap min(dt, t_final - t) (driver.h:391), one block per problem
-__global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, const double *partials, int nprob) {
-  for (int p = blockIdx.x; p < nprob; p += gridDim.x) {
+__global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, const double *partials) {
+  const int nprob = *b.np;
+  for (int ip = blockIdx.x; ip < nprob; ip += gridDim.x) {
+    const int p = b.plist[ip];
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
     if (!block_active(b.s, p)) continue;
@@ -346,16 +389,20 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
 }
 
 // step bracket of the driver loop (apps/maire_hydro/driver.h:332,514-515)
-__global__ void k_maire_begin(Batch b, int nprob) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
+__global__ void k_maire_begin(Batch b) {
+  const int nprob = *b.np;
+  for (int ip = blockIdx.x * blockDim.x + threadIdx.x; ip < nprob; ip += gridDim.x * blockDim.x) {
+    const int p = b.plist[ip];
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
     DevScalars &S = b.s[p];
     S.active = (S.err_code == 0) && (S.steps < P.max_steps) && (S.t < P.final_time);
   }
 }
-__global__ void k_maire_end(Batch b, int nprob) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
+__global__ void k_maire_end(Batch b) {
+  const int nprob = *b.np;
+  for (int ip = blockIdx.x * blockDim.x + threadIdx.x; ip < nprob; ip += gridDim.x * blockDim.x) {
+    const int p = b.plist[ip];
     if (b.pp[p].cell_count == 0) continue;
     DevScalars &S = b.s[p];
     if (!S.active || S.err_code) continue;
@@ -396,9 +443,8 @@ int launch_dt(fsale_ctx *ctx) {
   int rc = launch_persistent(ctx, K_MAIRE_DT, k_maire_dt_partial, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_DT));
   if (rc) return rc;
   ProfScope prof(ctx, K_MAIRE_DT);
-  const int np = (int)ctx->probs.size();
-  hipLaunchKernelGGL(k_maire_dt_finalize, dim3(np < 1024 ? np : 1024), dim3(kFinalBlock), 0, ctx->stream,
-                     batch_args(ctx, -1), (const double *)ctx->ma.partials, np);
+  hipLaunchKernelGGL(k_maire_dt_finalize, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream, batch_args(ctx, -1),
+                     (const double *)ctx->ma.partials);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -421,7 +467,7 @@ int launch_copy(fsale_ctx *ctx, int dir) {
 int maire_step(fsale_ctx *ctx) {
   int rc;
   const int np = (int)ctx->probs.size();
-  hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1), np);
+  hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
   FSALE_HIP(hipGetLastError());
   if ((rc = launch_copy(ctx, 0))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
@@ -432,7 +478,7 @@ int maire_step(fsale_ctx *ctx) {
   if ((rc = launch_residual(ctx))) return rc;
   if ((rc = launch_copy(ctx, 1))) return rc;
   if ((rc = launch_apply(ctx, 1.0, 2))) return rc;
-  hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1), np);
+  hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -458,12 +504,12 @@ int maire_alloc(fsale_ctx *ctx) {
   return FSALE_OK;
 }
 
-int maire_initial_state(fsale_ctx *ctx, int prob) {
+int maire_initial_state(fsale_ctx *ctx, int prob, hipStream_t stream) {
   const Problem &P = ctx->probs[prob];
   const int g = (P.nc + kBlock - 1) / kBlock;
   if (g == 0) return FSALE_OK;
-  hipLaunchKernelGGL(k_maire_ic, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
-                     (const ProbParams *)ctx->d_params, prob);
+  hipLaunchKernelGGL(k_maire_ic, dim3(g), dim3(kBlock), 0, stream, ctx->m, ctx->ma, (const ProbParams *)ctx->d_params,
+                     prob);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -490,7 +536,7 @@ int fsale_maire_initial_conditions(fsale_ctx *ctx, double cv, const double *gamm
   if ((rc = upload_params(ctx, 0))) return rc;
   if ((rc = set_problem_field(ctx, 0, FSALE_F_GAMMA, gamma))) return rc;
   if ((rc = set_problem_state(ctx, 0, d, v, p))) return rc;
-  if ((rc = maire_initial_state(ctx, 0))) return rc;
+  if ((rc = maire_initial_state(ctx, 0, ctx->stream))) return rc;
   P.have_state = true;
   return reset_scalars(ctx, 0, 0.0, 0.0);
 }

@@ -6,6 +6,10 @@
 // through the cells around its first vertex.
 #include <algorithm>
 #include <array>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <unordered_map>
 
 #include "internal.h"
 
@@ -23,6 +27,16 @@ inline void local_face(int dim, int vpc, int lf, int *lv) {
   } else {
     for (int i = 0; i < 4; ++i) lv[i] = kHexFaces[lf][i];
   }
+}
+
+// local faces holding local vertex kk: 2D edges (kk-1 -> kk) and (kk -> kk+1);
+// hex: the three sides of kHexFaces that list kk
+constexpr int kHexVertexFaces[8][3] = {{0, 3, 4}, {0, 1, 4}, {1, 2, 4}, {2, 3, 4},
+                                       {0, 3, 5}, {0, 1, 5}, {1, 2, 5}, {2, 3, 5}};
+
+inline int vertex_faces(const HostTopology &t, int kk, int j) {
+  if (t.dim == 2) return j == 0 ? kk : (kk + t.vpc - 1) % t.vpc;
+  return kHexVertexFaces[kk][j];
 }
 
 inline std::array<int32_t, 4> face_key(const HostTopology &t, int c, int lf) {
@@ -70,23 +84,32 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
     for (size_t i = 0; i < nslots; ++i) t.vc[fill[t.cv[i]]++] = (int32_t)i;
   }
 
-  // faces by first visit
-  t.cf.assign((size_t)t.nc * t.fpc, -1);
+  // faces by first visit; each face is matched among the faces (with the
+  // same sorted vertex key) of the cells around its lowest vertex
+  const size_t nfk = (size_t)t.nc * t.fpc;
+  std::vector<std::array<int32_t, 4>> keys(nfk);
+  for (int c = 0; c < t.nc; ++c)
+    for (int lf = 0; lf < t.fpc; ++lf) keys[(size_t)c * t.fpc + lf] = face_key(t, c, lf);
+  t.cf.assign(nfk, -1);
   t.fc.clear();
   t.fv.clear();
-  t.fc.reserve((size_t)t.nc * t.fpc);
-  t.fv.reserve((size_t)t.nc * t.fpc * t.vpf / 2 + 16);
+  t.fc.reserve(nfk);
+  t.fv.reserve(nfk * t.vpf / 2 + 16);
   int nf = 0;
+  const int nvf = t.dim == 2 ? 2 : 3;   // faces of a cell holding one of its vertices
   for (int c = 0; c < t.nc; ++c) {
     for (int lf = 0; lf < t.fpc; ++lf) {
-      const auto key = face_key(t, c, lf);
+      const auto &key = keys[(size_t)c * t.fpc + lf];
       int other = -1, other_lf = -1, matches = 0;
       const int v0 = key[0];
       for (int q = t.vc_off[v0]; q < t.vc_off[v0 + 1]; ++q) {
-        const int c2 = t.vc[q] / t.vpc;
+        const int c2 = t.vc[q] / t.vpc, kk = t.vc[q] % t.vpc;
         if (c2 == c) continue;
-        for (int lf2 = 0; lf2 < t.fpc; ++lf2) {
-          if (face_key(t, c2, lf2) == key) {
+        // only the faces of c2 that hold its local vertex kk can match
+        const auto *k2 = &keys[(size_t)c2 * t.fpc];
+        for (int j = 0; j < nvf; ++j) {
+          const int lf2 = vertex_faces(t, kk, j);
+          if (k2[lf2] == key) {
             if (other != c2) ++matches;
             other = c2;
             other_lf = lf2;
@@ -132,6 +155,7 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
   t.nbr.clear();
   t.nbr.reserve((size_t)t.nc * (t.dim == 2 ? 8 : 26));
   std::vector<int32_t> tmp;
+  tmp.reserve(64);
   for (int c = 0; c < t.nc; ++c) {
     tmp.clear();
     for (int k = 0; k < t.vpc; ++k) {
@@ -147,6 +171,83 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
     t.nbr_off[c + 1] = (int32_t)t.nbr.size();
   }
   return FSALE_OK;
+}
+
+// Process-wide connectivity cache: meshes are identified by their cell table
+// (a hash to find candidates, the table itself to confirm), so every context
+// and batch that sees the same mesh again skips the build.  Oldest entries
+// are dropped past kTopologyCacheBytes.
+namespace {
+constexpr size_t kTopologyCacheBytes = size_t(4) << 30;
+
+struct TopologyCache {
+  std::mutex mu;
+  std::unordered_multimap<uint64_t, std::shared_ptr<const HostTopology>> map;
+  std::deque<std::pair<uint64_t, const HostTopology *>> order;
+  size_t bytes = 0;
+};
+
+TopologyCache &topology_cache() {
+  static TopologyCache cache;
+  return cache;
+}
+
+size_t topology_bytes(const HostTopology &t) {
+  return sizeof(int32_t) * (t.cv.size() + t.cf.size() + t.fc.size() + t.fv.size() + t.nbr_off.size() +
+                            t.nbr.size() + t.vc_off.size() + t.vc.size());
+}
+
+uint64_t mesh_hash(const fsale_mesh_desc &m) {
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ ((uint64_t)m.dim << 48) ^ ((uint64_t)m.verts_per_cell << 40) ^
+               ((uint64_t)(uint32_t)m.num_vertices << 8) ^ (uint64_t)(uint32_t)m.num_cells;
+  const size_t n = (size_t)m.num_cells * m.verts_per_cell;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= (uint64_t)(uint32_t)m.cells[i] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  }
+  return h;
+}
+}  // namespace
+
+std::shared_ptr<const HostTopology> cached_topology(const fsale_mesh_desc &mesh, int &rc, std::string &err) {
+  rc = FSALE_OK;
+  if (!mesh.cells || mesh.num_cells <= 0 || mesh.verts_per_cell <= 0) {
+    rc = FSALE_ERR_ARG;
+    err = "empty mesh";
+    return nullptr;
+  }
+  TopologyCache &C = topology_cache();
+  const uint64_t h = mesh_hash(mesh);
+  const size_t n = (size_t)mesh.num_cells * mesh.verts_per_cell;
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    auto range = C.map.equal_range(h);
+    for (auto it = range.first; it != range.second; ++it) {
+      const HostTopology &t = *it->second;
+      if (t.dim == mesh.dim && t.nc == mesh.num_cells && t.nv == mesh.num_vertices && t.vpc == mesh.verts_per_cell &&
+          std::memcmp(t.cv.data(), mesh.cells, n * sizeof(int32_t)) == 0)
+        return it->second;
+    }
+  }
+  auto t = std::make_shared<HostTopology>();
+  rc = build_topology(mesh, *t, err);
+  if (rc) return nullptr;
+  std::lock_guard<std::mutex> lock(C.mu);
+  const size_t b = topology_bytes(*t);
+  C.map.emplace(h, t);
+  C.order.emplace_back(h, t.get());
+  C.bytes += b;
+  while (C.bytes > kTopologyCacheBytes && C.order.size() > 1) {
+    const auto victim = C.order.front();
+    C.order.pop_front();
+    auto range = C.map.equal_range(victim.first);
+    for (auto it = range.first; it != range.second; ++it)
+      if (it->second.get() == victim.second) {
+        C.bytes -= topology_bytes(*it->second);
+        C.map.erase(it);
+        break;
+      }
+  }
+  return t;
 }
 
 }  // namespace fsale

@@ -208,6 +208,22 @@ def maire_bubble_2d(n=512, steps=500, seed=MASTER_SEED):
                 final_time=1.0, max_steps=steps, dt0=1.e-5)
 
 
+_SQUARES = {}
+
+
+def _unit_square(side):
+    """side x side quad mesh of [0,1]^2 and its centroids, built once per side
+    (a request stream reuses a few hundred sizes; the arrays are read-only)."""
+    m = _SQUARES.get(side)
+    if m is None:
+        coords, cells = box_quads(side, side)
+        x = quad_centroids(coords, cells)
+        for a in (coords, cells, x):
+            a.setflags(write=False)
+        m = _SQUARES[side] = (coords, cells, x)
+    return m
+
+
 def riemann_2d(side, solver, seed=MASTER_SEED, index=0, steps=100):
     """Two-state Riemann problem on a side x side quad mesh of [0,1]^2.
 
@@ -224,8 +240,7 @@ def riemann_2d(side, solver, seed=MASTER_SEED, index=0, steps=100):
     pre = 0.1 + 0.9 * r[3:5]
     un = -1.0 + 2.0 * r[5:7]
     gamma = 1.4 if r[7] < 0.5 else 1.67
-    coords, cells = box_quads(side, side)
-    x = quad_centroids(coords, cells)
+    coords, cells, x = _unit_square(side)
     right = ((x[:, 0] - 0.5) * n[0] + (x[:, 1] - 0.5) * n[1]) >= 0
     k = right.astype(np.int64)
     d = rho[k]

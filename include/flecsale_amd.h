@@ -272,6 +272,69 @@ int fsale_get_launch_shape(fsale_ctx *ctx, int32_t *blocks_per_cu);
  * Allocates a transient snapshot of the state arrays. */
 int fsale_autotune(fsale_ctx *ctx, int32_t nsteps, double *best_ms);
 
+/* ------------------------------------------------------------ serving */
+/* Hydro solves arriving as requests (no counterpart in the reference, which
+ * runs one problem per job; each request runs the reference's step loop of
+ * its solver).  A server holds one Eulerian and one Lagrangian batch, each on
+ * its own HIP stream, so the two solvers' work overlaps on the device.  Every
+ * scheduling round (fsale_server_poll) admits queued requests in arrival
+ * order while they fit the batches' free capacity (continuous batching: a
+ * request that does not fit waits; up to `lookahead` later ones may pass it),
+ * launches chunk_steps steps of both batches, and retires finished requests
+ * after copying their results to host memory.  Mode 1 serves the same
+ * requests one at a time, each in a plain context of its own (the
+ * per-problem GPU path, for comparison). */
+typedef struct fsale_server fsale_server;
+
+typedef struct {
+  int32_t mode;              /* 0 continuous batching, 1 one plain context per request  */
+  int32_t dim;               /* mesh type of every request                               */
+  int32_t verts_per_cell;
+  int32_t max_problems;      /* per batch                                                */
+  int64_t euler_cells, euler_faces, euler_vertices, euler_neighbours;   /* batch capacity */
+  int64_t maire_cells, maire_faces, maire_vertices, maire_neighbours;
+  int32_t chunk_steps;       /* steps per scheduling round                               */
+  int32_t lookahead;         /* queued requests that may pass a blocked one              */
+} fsale_server_config;
+
+typedef struct {
+  int32_t solver;            /* FSALE_SOLVER_EULER or FSALE_SOLVER_MAIRE                 */
+  fsale_mesh_desc mesh;
+  fsale_eos eos;             /* Eulerian                                                 */
+  fsale_euler_params euler;
+  double cv;                 /* Lagrangian: cv and per-cell gamma                        */
+  const double *gamma;
+  fsale_maire_params maire;
+  const double *d, *v, *p;   /* initial density, velocity [num_cells*dim], pressure     */
+} fsale_request;
+
+/* Capacities of 0 are sized from the free device memory: frac_hbm of it,
+ * split between the two batches in proportion to their bytes per cell. */
+int fsale_server_create(const fsale_server_config *config, double frac_hbm, fsale_server **out);
+int fsale_server_destroy(fsale_server *server);
+/* Seconds on the server's monotonic clock (arrival and completion times). */
+double fsale_server_clock(void);
+/* Queue a request (inputs are copied); arrival on fsale_server_clock. */
+int fsale_server_submit(fsale_server *server, const fsale_request *request, double arrival, int64_t *id);
+/* One scheduling round: retire the requests that finished in the round the
+ * previous call launched (waits for it), admit queued requests, launch the
+ * next round and return while it runs.  *completed receives the number of
+ * requests retired, *pending the number queued or running afterwards.
+ * Mode 1 serves the oldest queued request start to finish instead. */
+int fsale_server_poll(fsale_server *server, int32_t *completed, int32_t *pending);
+/* Host-side accounting of a batched server: stats[0..5] = seconds spent
+ * admitting, waiting for rounds, retiring (result download + removal) and
+ * launching rounds; rounds run; requests admitted. */
+int fsale_server_stats(fsale_server *server, double *stats);
+/* Timing of a finished request: arrival, admission and completion times. */
+int fsale_server_times(fsale_server *server, int64_t id, double *arrival, double *admitted,
+                       double *finished);
+/* Results of a finished request (status, then d, v, e, p and, Lagrangian,
+ * the vertex coordinates; any array may be NULL); the server forgets the
+ * request afterwards unless keep != 0. */
+int fsale_server_result(fsale_server *server, int64_t id, int keep, fsale_problem_status *status,
+                        double *d, double *v, double *e, double *p, double *coords);
+
 #ifdef __cplusplus
 }
 #endif
