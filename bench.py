@@ -312,6 +312,10 @@ def main():
         ctx.set_profiling(False)
         for k in kb:
             tot, nl = ctx.profile(k)
+            if k == fa.K_MAIRE_NODAL:
+                # the nodal task is two launches: interior vertices and the
+                # boundary vertices' symmetry solves
+                tot += ctx.profile(fa.K_MAIRE_NODAL_BND)[0]
             if nl:
                 avg_s = tot / nl * 1e-3
                 kernels[fa.KERNEL_NAMES[k]] = {"avg_us": avg_s * 1e6, "launches": nl,

@@ -35,8 +35,9 @@ SOLVER_EULER, SOLVER_MAIRE = 0, 1
 K_EUL_DT, K_EUL_GRAD, K_EUL_FLUX, K_EUL_UPDATE = 0, 1, 2, 3
 K_MAIRE_NODAL, K_MAIRE_RESIDUAL, K_MAIRE_DT, K_MAIRE_MOVE, K_MAIRE_UPDATE, K_MAIRE_COPY = 4, 5, 6, 7, 8, 9
 KERNEL_NAMES = ["eul_dt", "eul_grad", "eul_flux", "eul_update", "maire_nodal", "maire_residual",
-                "maire_dt", "maire_move", "maire_update", "maire_copy", "eul_fluxupd"]
+                "maire_dt", "maire_move", "maire_update", "maire_copy", "eul_fluxupd", "maire_nodal_bnd"]
 K_EUL_FLUXUPD = 10
+K_MAIRE_NODAL_BND = 11
 
 ERR_NEG_STATE = 5
 ERR_DT = 6
@@ -46,7 +47,7 @@ ERR_NO_DEVICE = 8
 EXPORTS = [
     "fsale_ctx_create", "fsale_ctx_destroy", "fsale_sync", "fsale_mesh_counts", "fsale_get_topology",
     "fsale_mesh_connectivity", "fsale_get_field", "fsale_set_field", "fsale_get_time", "fsale_set_time",
-    "fsale_status_string", "fsale_last_message", "fsale_error_cell", "fsale_set_profiling",
+    "fsale_status_string", "fsale_last_message", "fsale_qr_solve", "fsale_error_cell", "fsale_set_profiling",
     "fsale_get_profile", "fsale_reset_profile",
     "fsale_eul_initial_conditions", "fsale_eul_evaluate_time_step", "fsale_eul_evaluate_gradients",
     "fsale_eul_evaluate_fluxes", "fsale_eul_apply_update", "fsale_eul_advance",
@@ -121,6 +122,7 @@ def lib():
         L.fsale_status_string.restype = ct.c_char_p
         L.fsale_status_string.argtypes = [ct.c_int]
         L.fsale_last_message.restype = ct.c_char_p
+        L.fsale_qr_solve.argtypes = [i32, vp, vp, vp]
         L.fsale_ctx_create.argtypes = [ct.POINTER(MeshDesc), ct.c_int, ct.POINTER(vp)]
         L.fsale_ctx_destroy.argtypes = [vp]
         L.fsale_sync.argtypes = [vp]
@@ -196,6 +198,15 @@ def _desc(coords, cells):
     d = MeshDesc(coords.shape[1], cells.shape[0], coords.shape[0], cells.shape[1],
                  coords.ctypes.data, cells.ctypes.data)
     return d, (coords, cells)
+
+
+def qr_solve(A, b):
+    """flecsale::linalg::qr solve of A x = b (n <= 4) with the nodal kernel's arithmetic, on the host."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b))
+    _check(lib().fsale_qr_solve(len(b), A.ctypes.data, b.ctypes.data, x.ctypes.data))
+    return x
 
 
 def connectivity(coords, cells):

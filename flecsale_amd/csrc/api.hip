@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <type_traits>
+
 #include "internal.h"
 #include "kernels_common.h"
 
@@ -162,6 +164,7 @@ int sync_tiles(fsale_ctx *ctx) {
   if (!ctx->tiles_dirty) return FSALE_OK;
   ctx->h_ctiles.clear();
   ctx->h_vtiles.clear();
+  ctx->h_btiles.clear();
   for (int p = 0; p < (int)ctx->probs.size(); ++p) {
     const Problem &P = ctx->probs[p];
     if (!P.live) continue;
@@ -169,11 +172,14 @@ int sync_tiles(fsale_ctx *ctx) {
       ctx->h_ctiles.push_back(Tile{p, P.c0 + i * kBlock, std::min(kBlock, P.nc - i * kBlock), P.ct0 + i});
     for (int i = 0; i < P.nvt; ++i)
       ctx->h_vtiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nv - i * kBlock), P.vt0 + i});
+    for (int i = 0; i * kBlock < P.nb; ++i)
+      ctx->h_btiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nb - i * kBlock), P.vt0 + i});
   }
   std::vector<int> plist;
   for (int p = 0; p < (int)ctx->probs.size(); ++p)
     if (ctx->probs[p].live) plist.push_back(p);
-  const int counts[3] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size()};
+  const int counts[4] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size(),
+                         (int)ctx->h_btiles.size()};
   if (!plist.empty())
     FSALE_HIP(hipMemcpyAsync(ctx->d_plist, plist.data(), sizeof(int) * plist.size(), hipMemcpyHostToDevice,
                              ctx->stream));
@@ -191,10 +197,18 @@ int sync_tiles(fsale_ctx *ctx) {
   ctx->n_cdirect = direct_grid(counts[0], ctx->ra_ct.capacity);
   build_direct(ctx->h_ctiles, ctx->n_cdirect, dc);
   FSALE_HIP(hipMemcpyAsync(ctx->d_cdirect, dc.data(), sizeof(Tile) * dc.size(), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<Tile> db;
   if (ctx->d_vdirect) {
     ctx->n_vdirect = direct_grid(counts[1], ctx->ra_vt.capacity);
     build_direct(ctx->h_vtiles, ctx->n_vdirect, dv);
     FSALE_HIP(hipMemcpyAsync(ctx->d_vdirect, dv.data(), sizeof(Tile) * dv.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+    if (!ctx->h_btiles.empty())
+      FSALE_HIP(hipMemcpyAsync(ctx->d_btiles, ctx->h_btiles.data(), sizeof(Tile) * ctx->h_btiles.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    ctx->n_bdirect = direct_grid(counts[3], ctx->ra_vt.capacity);
+    build_direct(ctx->h_btiles, ctx->n_bdirect, db);
+    FSALE_HIP(hipMemcpyAsync(ctx->d_bdirect, db.data(), sizeof(Tile) * db.size(), hipMemcpyHostToDevice,
                              ctx->stream));
   }
   FSALE_HIP(hipStreamSynchronize(ctx->stream));
@@ -210,11 +224,15 @@ Batch batch_args(const fsale_ctx *ctx, int kid) {
   b.ct.n = ctx->d_counts;
   b.vt.tiles = ctx->d_vtiles;
   b.vt.n = ctx->d_counts + 1;
+  b.bt.tiles = ctx->d_btiles;
+  b.bt.n = ctx->d_counts + 3;
   if (kid >= 0 && kid < K_COUNT && ctx->shape.blocks_per_cu[kid] <= 0) {
     b.ct.direct = ctx->d_cdirect;
     b.ct.ndirect = ctx->n_cdirect;
     b.vt.direct = ctx->d_vdirect;
     b.vt.ndirect = ctx->d_vdirect ? ctx->n_vdirect : 0;
+    b.bt.direct = ctx->d_bdirect;
+    b.bt.ndirect = ctx->d_bdirect ? ctx->n_bdirect : 0;
   }
   b.pp = ctx->d_params;
   b.s = ctx->s;
@@ -228,11 +246,14 @@ Batch batch_args(const fsale_ctx *ctx, int kid) {
 // is one block per tile (the dispatch lists' grid), shape n a persistent
 // grid of CUs x n blocks, never more blocks than the capacity has tiles.
 int persistent_grid(const fsale_ctx *ctx, int kid) {
-  const bool vertex_list = kid == K_MAIRE_NODAL || kid == K_MAIRE_MOVE;
+  const bool vertex_list = kid == K_MAIRE_NODAL || kid == K_MAIRE_MOVE || kid == K_MAIRE_NODAL_BND;
   const bool both = kid == K_MAIRE_COPY;
   const int bpc = (kid >= 0 && kid < K_COUNT) ? ctx->shape.blocks_per_cu[kid] : 0;
   if (bpc <= 0) {
-    const int g = vertex_list ? ctx->n_vdirect : both ? std::max(ctx->n_cdirect, ctx->n_vdirect) : ctx->n_cdirect;
+    const int g = kid == K_MAIRE_NODAL_BND ? ctx->n_bdirect
+                  : vertex_list           ? ctx->n_vdirect
+                  : both                  ? std::max(ctx->n_cdirect, ctx->n_vdirect)
+                                          : ctx->n_cdirect;
     return std::max(8, g);
   }
   const int cap_tiles = vertex_list ? ctx->ra_vt.capacity
@@ -357,13 +378,17 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
     if ((rc = dev_alloc_n(ctx, &m.vck, (size_t)cap_v * kMaxCornersPerVertex))) return rc;
     if ((rc = dev_alloc_n(ctx, &m.cwtag, (size_t)cap_c * vpc))) return rc;
     if ((rc = dev_alloc_n(ctx, &m.vbnd, cap_v))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.bv, cap_v))) return rc;
   }
   if ((rc = dev_alloc_n(ctx, &ctx->d_ctiles, cap_ct))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_vtiles, cap_vt))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_cdirect, ((size_t)cap_ct + 7) / 8 * 8))) return rc;
-  if (solver == FSALE_SOLVER_MAIRE)
+  if (solver == FSALE_SOLVER_MAIRE) {
     if ((rc = dev_alloc_n(ctx, &ctx->d_vdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
-  if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 3))) return rc;
+    if ((rc = dev_alloc_n(ctx, &ctx->d_btiles, cap_vt))) return rc;
+    if ((rc = dev_alloc_n(ctx, &ctx->d_bdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
+  }
+  if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 4))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_plist, max_problems))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_params, max_problems))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->s, max_problems))) return rc;
@@ -376,7 +401,7 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   std::vector<ProbParams> zero(max_problems);
   std::memset(zero.data(), 0, sizeof(ProbParams) * max_problems);
   FSALE_HIP(hipMemcpy(ctx->d_params, zero.data(), sizeof(ProbParams) * max_problems, hipMemcpyHostToDevice));
-  const int counts[3] = {0, 0, 0};
+  const int counts[4] = {0, 0, 0, 0};
   FSALE_HIP(hipMemcpy(ctx->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice));
   ctx->probs.assign(max_problems, Problem{});
   if (solver == FSALE_SOLVER_MAIRE) return maire_alloc(ctx);
@@ -456,7 +481,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   // room for the caller's state arrays, parameters and scalars
   size_t bytes = 4 * ((size_t)nc * (t.vpc + 2 * t.fpc + m.nbk_deg + 2) + (size_t)nf * (2 + t.vpf) + P.nn) +
                  8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
-  if (maire) bytes += 4 * (size_t)nv * (1 + kMaxCornersPerVertex) + (size_t)nc * t.vpc + nv;
+  if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + (size_t)nc * t.vpc + nv;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
   auto rows = [&](int32_t *dst_base, int rows_n, size_t stride, int n, auto &&value) -> int {
     for (int k = 0; k < rows_n; ++k) {
@@ -546,6 +571,16 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
       if ((rc = st.put(m.cwtag + (size_t)k * m.cap_c + P.c0, cw, nc))) return rc;
     }
     if ((rc = st.put(m.vbnd + P.v0, vbnd, nv))) return rc;
+    // the boundary vertices, ascending, for the symmetry-solve launch
+    int nb = 0;
+    for (int v = 0; v < nv; ++v) nb += vbnd[v] ? 1 : 0;
+    int32_t *bvl = st.slot<int32_t>(nb > 0 ? nb : 1);
+    if (!bvl) return FSALE_ERR_ARG;
+    nb = 0;
+    for (int v = 0; v < nv; ++v)
+      if (vbnd[v]) bvl[nb++] = v + P.v0;
+    if ((rc = st.put(m.bv + P.v0, bvl, nb))) return rc;
+    ctx->probs[prob].nb = nb;
   }
   return FSALE_OK;
 }
@@ -942,6 +977,27 @@ const char *fsale_status_string(int status) {
 }
 
 const char *fsale_last_message(void) { return g_message.c_str(); }
+
+int fsale_qr_solve(int32_t n, const double *A, const double *b, double *x) {
+  if (!A || !b || !x || n < 1 || n > 4) return FSALE_ERR_ARG;
+  auto run = [&](auto tag) {
+    constexpr int NR = decltype(tag)::value;
+    double M[NR][NR], B[NR];
+    for (int r = 0; r < NR; ++r) {
+      B[r] = b[r];
+      for (int c = 0; c < NR; ++c) M[r][c] = A[r * NR + c];
+    }
+    qr_solve_fixed<NR>(M, B);
+    for (int r = 0; r < NR; ++r) x[r] = B[r];
+  };
+  switch (n) {
+    case 1: run(std::integral_constant<int, 1>()); break;
+    case 2: run(std::integral_constant<int, 2>()); break;
+    case 3: run(std::integral_constant<int, 3>()); break;
+    default: run(std::integral_constant<int, 4>()); break;
+  }
+  return FSALE_OK;
+}
 
 int fsale_batch_create(int solver, int dim, int verts_per_cell, int32_t max_problems, int64_t cell_capacity,
                        int64_t face_capacity, int64_t vertex_capacity, int64_t neighbour_capacity,

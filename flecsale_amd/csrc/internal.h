@@ -89,6 +89,8 @@ struct DevMesh {
                                 // id), ascending cell id, -1 padded
   uint8_t *cwtag = nullptr;     // [vpc][cap_c] Maire wedge boundary tags
   uint8_t *vbnd = nullptr;      // [cap_v] vertex on a boundary face
+  int32_t *bv = nullptr;        // [cap_v] a problem's boundary vertices (batch ids, ascending)
+                                // at positions [v0, v0 + nb) of its vertex range
   double *ccen[3] = {nullptr, nullptr, nullptr};
   double *cvol = nullptr, *cminlen = nullptr;
   double *fcen[3] = {nullptr, nullptr, nullptr};
@@ -161,6 +163,7 @@ struct MaireFields {
 struct Batch {
   TileList ct;                // cell tiles
   TileList vt;                // vertex tiles (Maire)
+  TileList bt;                // boundary-vertex tiles (Maire): entities index DevMesh::bv
   const ProbParams *pp = nullptr;
   DevScalars *s = nullptr;
   int cap_c_tiles = 0;        // partials stride
@@ -179,7 +182,7 @@ struct GraphCache {
 enum KernelId {
   K_EUL_DT = 0, K_EUL_GRAD, K_EUL_FLUX, K_EUL_UPDATE,
   K_MAIRE_NODAL, K_MAIRE_RESIDUAL, K_MAIRE_DT, K_MAIRE_MOVE, K_MAIRE_UPDATE, K_MAIRE_COPY,
-  K_EUL_FLUXUPD,
+  K_EUL_FLUXUPD, K_MAIRE_NODAL_BND,
   K_COUNT
 };
 
@@ -194,6 +197,7 @@ struct Problem {
   bool live = false;
   int c0 = 0, nc = 0, f0 = 0, nf = 0, v0 = 0, nv = 0, n0 = 0, nn = 0;   // ranges
   int ct0 = 0, nct = 0, vt0 = 0, nvt = 0;                               // tile ranges
+  int nb = 0;                                                           // boundary vertices (Maire)
   std::shared_ptr<const HostTopology> topo;
   ProbParams params{};
   bool have_state = false;
@@ -240,14 +244,16 @@ struct fsale_ctx {
   fsale::Tile *d_ctiles = nullptr, *d_vtiles = nullptr;
   fsale::Tile *d_cdirect = nullptr, *d_vdirect = nullptr;   // one-block-per-tile dispatch lists
   int n_cdirect = 0, n_vdirect = 0;
-  int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles, [2] problems
+  fsale::Tile *d_btiles = nullptr, *d_bdirect = nullptr;    // boundary-vertex tiles (Maire)
+  int n_bdirect = 0;
+  int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles, [2] problems, [3] boundary tiles
   int *d_plist = nullptr;              // problems with tiles in the lists
   int pgrid = 1;                       // blocks of the per-problem kernels
   hipStream_t upload = nullptr;        // admissions: uploads, geometry, initial state
   fsale::ProbParams *d_params = nullptr;
   fsale::DevScalars *s = nullptr;      // [max_problems] device
   fsale::DevScalars *h_s = nullptr;    // pinned host mirror
-  std::vector<fsale::Tile> h_ctiles, h_vtiles;
+  std::vector<fsale::Tile> h_ctiles, h_vtiles, h_btiles;
   bool tiles_dirty = true;
   void *stage = nullptr;               // pinned staging buffer for uploads
   size_t stage_bytes = 0;

@@ -358,6 +358,140 @@ __host__ __device__ inline void qr_solve(int rows, double *A, double *B) {
   }
 }
 
+// The same QR solve for a compile-time size, every loop unrolled so that the
+// matrices stay in registers (dynamically indexed arrays would live in
+// scratch or LDS).  The column pivot is applied by physically swapping
+// columns instead of indirecting through jpvt: physical column j holds the
+// reference's column jpvt[j], every operation reads the same values in the
+// same order, and jpvt only maps the solution back (B[jpvt[i]]).
+template <int NR>
+__host__ __device__ inline double perm_get(const double (&x)[NR], int idx) {
+  double r = x[0];
+#pragma unroll
+  for (int q = 1; q < NR; ++q) r = (idx == q) ? x[q] : r;
+  return r;
+}
+
+template <int NR>
+__host__ __device__ inline void perm_set(double (&x)[NR], int idx, double val) {
+#pragma unroll
+  for (int q = 0; q < NR; ++q) x[q] = (idx == q) ? val : x[q];
+}
+
+template <int NR>
+__host__ __device__ inline void qr_solve_fixed(double (&A)[NR][NR], double (&B)[NR]) {
+  int jpvt[NR];
+  double v[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    jpvt[q] = q;
+    v[q] = 0;
+  }
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    double cn[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      cn[j] = 0;
+#pragma unroll
+      for (int r = i; r < NR; ++r) cn[j] += A[r][j] * A[r][j];
+    }
+    int max_loc = 0;
+    double mx = 0;
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+      if (cn[j] > mx) {
+        mx = cn[j];
+        max_loc = j;
+      }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      if (j != i && j == max_loc) {
+        const int t = jpvt[i];
+        jpvt[i] = jpvt[j];
+        jpvt[j] = t;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const double a = A[r][i];
+          A[r][i] = A[r][j];
+          A[r][j] = a;
+        }
+      }
+    }
+    {
+      double norm = 0;
+#pragma unroll
+      for (int r = i; r < NR; ++r) norm += A[r][i] * A[r][i];
+      if (norm != 0) {
+        norm = sqrt(norm);
+        v[0] = A[i][i] - norm;
+#pragma unroll
+        for (int r = 1; r < NR - i; ++r) v[r] = A[r + i][i];
+        double n2 = 0;
+#pragma unroll
+        for (int r = 0; r < NR - i; ++r) n2 += v[r] * v[r];
+        if (n2 != 0) {
+          n2 = sqrt(n2);
+#pragma unroll
+          for (int r = 0; r < NR - i; ++r) v[r] *= (1.0 / n2);
+        }
+      }
+    }
+    double Ac[NR][NR], Bc[NR], hh[NR][NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      Bc[r] = B[r];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) Ac[r][q] = A[r][q];
+    }
+#pragma unroll
+    for (int j = 0; j < NR - i; ++j)
+#pragma unroll
+      for (int r = 0; r < NR - i; ++r) hh[j][r] = (r != j) ? -2 * v[j] * v[r] : 1 - 2 * v[j] * v[r];
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+#pragma unroll
+      for (int j = 0; j < NR - i; ++j) {
+        double temp = 0;
+#pragma unroll
+        for (int r = 0; r < NR - i; ++r) temp += hh[j][r] * Ac[r + i][k];
+        A[j + i][k] = temp;
+      }
+#pragma unroll
+    for (int j = 0; j < NR - i; ++j) {
+      double sum = 0;
+#pragma unroll
+      for (int r = 0; r < NR - i; ++r) sum += hh[j][r] * Bc[r + i];
+      B[j + i] = sum;
+    }
+  }
+  const double eps = DBL_EPSILON;
+  double Bc[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) Bc[q] = B[q];
+  int bottom = 0;
+#pragma unroll
+  for (int i = NR - 1; i >= 0; --i) {
+    if (fabs(A[i][NR - 1]) > eps) {
+      bottom = i;
+      break;
+    }
+  }
+#pragma unroll
+  for (int i = NR - 1; i >= 0; --i) {
+    if (i > bottom) continue;
+    double sum = 0;
+#pragma unroll
+    for (int j = NR - 1; j > i; --j) sum += perm_get<NR>(B, jpvt[j]) * A[i][j];
+    if (fabs(A[i][i]) > eps) {
+      const double temp = 1 / A[i][i];
+      perm_set<NR>(B, jpvt[i], (Bc[i] - sum) * temp);
+    } else {
+      perm_set<NR>(B, jpvt[i], 0.0);
+    }
+  }
+}
+
 // ------------------------------------------------------------ reductions
 // max of non-negative doubles through their IEEE bit patterns
 __device__ inline void atomic_max_nonneg(double *addr, double v) {

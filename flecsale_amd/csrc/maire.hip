@@ -68,10 +68,8 @@ __device__ inline void wedge_facets(const DevMesh &m, int c, int k, double n[2][
 
 // corner matrix M_pc = sum_w z_c l_w n_w n_w^T (ristra outer_product with
 // factor z_c l_w) and (l n)_pc = sum_w l_w n_w.
-__device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int c, int k, double M[2][2],
-                                     double np[2]) {
-  double n[2][2], l[2];
-  wedge_facets(m, c, k, n, l);
+__device__ inline void corner_matrix_from(const MaireFields &u, int c, const double n[2][2], const double l[2],
+                                          double M[2][2], double np[2]) {
   const double zc = u.d[c] * u.a[c];
   M[0][0] = M[0][1] = M[1][0] = M[1][1] = 0;
   np[0] = np[1] = 0;
@@ -84,25 +82,37 @@ __device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int
   }
 }
 
+__device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int c, int k, double M[2][2],
+                                     double np[2]) {
+  double n[2][2], l[2];
+  wedge_facets(m, c, k, n, l);
+  corner_matrix_from(u, c, n, l, M, np);
+}
+
 // Symmetry-constrained nodal velocity of boundary vertex vt
 // (apps/maire_hydro/tasks.h:317-420): symmetry normals accumulated per tag
 // over the vertex's boundary wedges in corner order, then the augmented system
 // [M_p N^T; N 0] solved by flecsale::linalg::qr (:383-418); Cramer's rule
 // when no wedge carries a symmetry tag.
+constexpr int kNodalKeep = 4;
+
+// keep: wedge normals, facet areas and tags of the first kNodalKeep corners,
+// already loaded by the caller
+struct KeptWedges {
+  double n[kNodalKeep][2][2], l[kNodalKeep][2];
+  int tags[kNodalKeep];
+};
+
 __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const double Mp[2][2], const double rhs[2],
-                                      double un[2]) {
+                                      const KeptWedges &kw, double un[2]) {
   int has[2] = {0, 0};
   double symn[2][2] = {{0, 0}, {0, 0}};
-  for (int q = 0; q < cnt; ++q) {
-    const int cn = m.vck[(size_t)q * m.cap_v + vt];
-    const int c = cn >> 2, k = cn & 3;
-    const int tags = m.cwtag[(size_t)k * m.cap_c + c];
-    if (!(tags & 0x44)) continue;
-    double n[2][2], l[2];
-    wedge_facets(m, c, k, n, l);
+  auto add_wedges = [&](int tags, const double n[2][2], const double l[2]) {
+#pragma unroll
     for (int w = 0; w < 2; ++w) {
       const int t = (tags >> (4 * w)) & 0xf;
       if (!(t & 4)) continue;
+#pragma unroll
       for (int tag = 0; tag < 2; ++tag) {
         if (!(t & (1 << tag))) continue;
         if (has[tag]) {
@@ -113,7 +123,21 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
         }
       }
     }
+  };
+#pragma unroll
+  for (int q = 0; q < kNodalKeep; ++q)
+    if (q < cnt && (kw.tags[q] & 0x44)) add_wedges(kw.tags[q], kw.n[q], kw.l[q]);
+  for (int q = kNodalKeep; q < cnt; ++q) {
+    const int cn = m.vck[(size_t)q * m.cap_v + vt];
+    const int c = cn >> 2, k = cn & 3;
+    const int tags = m.cwtag[(size_t)k * m.cap_c + c];
+    if (!(tags & 0x44)) continue;
+    double n[2][2], l[2];
+    wedge_facets(m, c, k, n, l);
+    add_wedges(tags, n, l);
   }
+  // augmented system [M_p N^T; N 0] solved by flecsale::linalg::qr
+  // (apps/maire_hydro/tasks.h:383-418), rows of the tags in order
   const int nsym = has[0] + has[1];
   if (nsym == 0) {
     const double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
@@ -121,21 +145,24 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
     un[1] = (Mp[0][0] * rhs[1] - rhs[0] * Mp[1][0]) / det;
     return;
   }
-  const int nr = 2 + nsym;
-  double A[16], bb[4] = {rhs[0], rhs[1], 0, 0};
-  for (int i = 0; i < 16; ++i) A[i] = 0;
-  for (int r = 0; r < 2; ++r)
-    for (int q = 0; q < 2; ++q) A[r * nr + q] = Mp[r][q];
-  int jj = 2;
-  for (int tag = 0; tag < 2; ++tag) {
-    if (!has[tag]) continue;
-    for (int r = 0; r < 2; ++r) A[r * nr + jj] = symn[tag][r];
-    for (int q = 0; q < 2; ++q) A[jj * nr + q] = symn[tag][q];
-    ++jj;
+  if (nsym == 1) {
+    const double sn0 = has[0] ? symn[0][0] : symn[1][0];
+    const double sn1 = has[0] ? symn[0][1] : symn[1][1];
+    double A[3][3] = {{Mp[0][0], Mp[0][1], sn0}, {Mp[1][0], Mp[1][1], sn1}, {sn0, sn1, 0}};
+    double bb[3] = {rhs[0], rhs[1], 0};
+    qr_solve_fixed<3>(A, bb);
+    un[0] = bb[0];
+    un[1] = bb[1];
+  } else {
+    double A[4][4] = {{Mp[0][0], Mp[0][1], symn[0][0], symn[1][0]},
+                      {Mp[1][0], Mp[1][1], symn[0][1], symn[1][1]},
+                      {symn[0][0], symn[0][1], 0, 0},
+                      {symn[1][0], symn[1][1], 0, 0}};
+    double bb[4] = {rhs[0], rhs[1], 0, 0};
+    qr_solve_fixed<4>(A, bb);
+    un[0] = bb[0];
+    un[1] = bb[1];
   }
-  qr_solve<4>(nr, A, bb);
-  un[0] = bb[0];
-  un[1] = bb[1];
 }
 
 // evaluate_nodal_state, one work-item per vertex.  The vertex's corners
@@ -143,7 +170,6 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
 // corner matrices stay in registers for the final F_pc -= M_pc u_p (a vertex
 // with more corners recomputes the rest), and the first four corners' loads
 // do not depend on each other.
-constexpr int kNodalKeep = 4;
 
 __device__ inline void corner_force(const MaireFields &u, int c, const double M[2][2], const double np[2],
                                     double F[2]) {
@@ -157,21 +183,36 @@ __device__ inline void corner_force(const MaireFields &u, int c, const double M[
   for (int d = 0; d < 2; ++d) F[d] += pc * np[d];
 }
 
+// BOUNDARY = false: the interior vertices (Cramer's rule); true: the vertices
+// on a boundary face, whose symmetry QR solve would otherwise hold the
+// registers (and the waves) of the whole interior launch.
+template <bool BOUNDARY>
 __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
-  for_each_tile(b.vt, [&](const Tile &tile) {
+  for_each_tile(BOUNDARY ? b.bt : b.vt, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
-    const int vt = tile.first + threadIdx.x;
+    int vt = tile.first + threadIdx.x;
+    if constexpr (BOUNDARY)
+      vt = m.bv[vt];
+    else if (m.vbnd[vt])
+      return;
     const int cnt = m.vc_cnt[vt];
     double M[kNodalKeep][2][2], np[kNodalKeep][2], F[kNodalKeep][2];
     int cc[kNodalKeep], kk[kNodalKeep];
+    KeptWedges kw;
 #pragma unroll
     for (int q = 0; q < kNodalKeep; ++q) {
       const int cn = q < cnt ? m.vck[(size_t)q * m.cap_v + vt] : -1;
       cc[q] = cn >> 2;
       kk[q] = cn & 3;
       if (cn >= 0) {
-        corner_matrix(m, u, cc[q], kk[q], M[q], np[q]);
+        if constexpr (BOUNDARY) {
+          wedge_facets(m, cc[q], kk[q], kw.n[q], kw.l[q]);
+          kw.tags[q] = m.cwtag[(size_t)kk[q] * m.cap_c + cc[q]];
+          corner_matrix_from(u, cc[q], kw.n[q], kw.l[q], M[q], np[q]);
+        } else {
+          corner_matrix(m, u, cc[q], kk[q], M[q], np[q]);
+        }
         corner_force(u, cc[q], M[q], np[q], F[q]);
       }
     }
@@ -198,8 +239,8 @@ __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u
       }
     }
     double un[2];
-    if (m.vbnd[vt]) {
-      boundary_solve(m, vt, cnt, Mp, rhs, un);
+    if constexpr (BOUNDARY) {
+      boundary_solve(m, vt, cnt, Mp, rhs, kw, un);
     } else {
       // ristra::math::solve (un-vendored): Cramer's rule
       const double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
@@ -433,7 +474,11 @@ bool check_ctx(fsale_ctx *ctx) {
 }
 
 int launch_nodal(fsale_ctx *ctx) {
-  return launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_NODAL));
+  int rc = launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal<false>, ctx->m, ctx->ma,
+                             batch_args(ctx, K_MAIRE_NODAL));
+  return rc ? rc
+            : launch_persistent(ctx, K_MAIRE_NODAL_BND, k_maire_nodal<true>, ctx->m, ctx->ma,
+                                batch_args(ctx, K_MAIRE_NODAL_BND));
 }
 int launch_residual(fsale_ctx *ctx) {
   return launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual, ctx->m, ctx->ma,

@@ -90,6 +90,12 @@ enum {
   FSALE_F_COUNT
 };
 
+/* flecsale::linalg::qr (flecsale/linalg/qr.h:44-94, detail/qr_impl.h:38-288):
+ * column-pivoted Householder solve of A x = b, n in 1..4, A row-major, on the
+ * host with the same arithmetic as the Lagrangian nodal kernel's symmetry
+ * solve (apps/maire_hydro/tasks.h:383-418). */
+int fsale_qr_solve(int32_t n, const double *A, const double *b, double *x);
+
 /* ------------------------------------------------------------ context life */
 /* Builds connectivity (faces, face->cell with cells[0] the lowest cell id and
  * the normal out of it, cell-vertex neighbours, vertex->corner) on the host,
@@ -125,7 +131,8 @@ int fsale_error_cell(fsale_ctx *ctx, int64_t *cell);
  * step loops run without graphs and each launch is bracketed by HIP events on
  * the context's stream.  Kernel ids: 0 dt, 1 gradients, 2 fluxes, 3 update
  * (Eulerian); 4 nodal, 5 residual, 6 dt, 7 move, 8 update, 9 save/restore
- * (Lagrangian); 10 fused fluxes + update (the Eulerian step loop's kernel).
+ * (Lagrangian); 10 fused fluxes + update (the Eulerian step loop's kernel);
+ * 11 nodal solve of the boundary vertices (Lagrangian symmetry QR).
  * total_ms / launches accumulate until fsale_reset_profile. */
 int fsale_set_profiling(fsale_ctx *ctx, int enable);
 int fsale_get_profile(fsale_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
@@ -263,7 +270,7 @@ int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_face
  * blocks_per_cu) blocks looping over the tiles.  Shapes change timing only,
  * never results.  Any context (a plain one is a batch of one). */
 int fsale_launch_shape(fsale_ctx *ctx, int32_t kernel, int32_t blocks_per_cu);
-/* blocks_per_cu[11] receives the current shape of every kernel id. */
+/* blocks_per_cu[12] receives the current shape of every kernel id. */
 int fsale_get_launch_shape(fsale_ctx *ctx, int32_t *blocks_per_cu);
 /* Autotuner: times nsteps eager steps of the resident problems under each
  * candidate shape (HIP events per launch) from one snapshot of the state,
