@@ -55,7 +55,8 @@ EXPORTS = [
     "fsale_maire_evaluate_time_step", "fsale_maire_apply_update", "fsale_maire_save", "fsale_maire_restore",
     "fsale_maire_advance",
     "fsale_batch_create", "fsale_batch_add_euler", "fsale_batch_add_maire", "fsale_batch_remove",
-    "fsale_batch_step", "fsale_batch_status", "fsale_batch_get_field", "fsale_batch_capacity",
+    "fsale_batch_step", "fsale_batch_status", "fsale_batch_get_field", "fsale_batch_get_fields",
+    "fsale_batch_capacity",
     "fsale_launch_shape", "fsale_get_launch_shape", "fsale_autotune",
     "fsale_server_create", "fsale_server_destroy", "fsale_server_clock", "fsale_server_submit",
     "fsale_server_poll", "fsale_server_times", "fsale_server_result", "fsale_server_stats",
@@ -161,6 +162,7 @@ def lib():
         L.fsale_batch_step.argtypes = [vp, ct.c_int, ct.c_int]
         L.fsale_batch_status.argtypes = [vp, i32, ct.POINTER(ProblemStatus), ct.POINTER(i32)]
         L.fsale_batch_get_field.argtypes = [vp, i32, ct.c_int, vp]
+        L.fsale_batch_get_fields.argtypes = [vp, i32, i32, vp, vp]
         L.fsale_batch_capacity.argtypes = [vp, ct.POINTER(i64), ct.POINTER(i64), ct.POINTER(i64),
                                            ct.POINTER(i64), ct.POINTER(i32)]
         L.fsale_launch_shape.argtypes = [vp, i32, i32]

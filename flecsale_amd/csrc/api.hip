@@ -7,6 +7,10 @@
 #include <cmath>
 #include <cstring>
 
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <type_traits>
 
 #include "internal.h"
@@ -275,9 +279,85 @@ namespace {
 // data visible to later work and releases the buffer.  A running round
 // cannot see the new problem: its tiles and its entry in the problem list are
 // uploaded by the next round's launch.
+// Host threads that fill the staging rows of an admission in parallel; the
+// calling thread works too.  Created on first use, kept for the process.
+class RowPool {
+ public:
+  static RowPool &get() {
+    static RowPool *pool = new RowPool();
+    return *pool;
+  }
+  void run(std::vector<std::function<void()>> &jobs) {
+    if (jobs.empty()) return;
+    if (workers_ == 0 || jobs.size() == 1) {
+      for (auto &j : jobs) j();
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      jobs_ = &jobs;
+      next_ = 0;
+      pending_ = jobs.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lock(mu_);
+    done_.wait(lock, [&] { return pending_ == 0; });
+    jobs_ = nullptr;
+  }
+
+ private:
+  RowPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    workers_ = (int)std::min<unsigned>(hw > 1 ? hw - 1 : 0, 7);
+    for (int i = 0; i < workers_; ++i) std::thread([this] { loop(); }).detach();
+  }
+  void work() {
+    for (;;) {
+      size_t i;
+      std::vector<std::function<void()>> *jobs;
+      {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (!jobs_ || next_ >= jobs_->size()) return;
+        i = next_++;
+        jobs = jobs_;
+      }
+      (*jobs)[i]();
+      std::lock_guard<std::mutex> lock(mu_);
+      if (--pending_ == 0) done_.notify_all();
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_.wait(lock, [&] { return gen_ != seen; });
+        seen = gen_;
+      }
+      work();
+    }
+  }
+  int workers_ = 0;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  std::vector<std::function<void()>> *jobs_ = nullptr;
+  size_t next_ = 0, pending_ = 0;
+  uint64_t gen_ = 0;
+};
+
 struct Stager {
   fsale_ctx *ctx;
   size_t used = 0;
+  // rows whose contents are computed on the row pool before their transfer
+  struct Deferred {
+    void *dst;
+    const void *src;
+    size_t bytes;
+  };
+  std::vector<Deferred> deferred;
+  std::vector<std::function<void()>> fills;
   explicit Stager(fsale_ctx *c) : ctx(c) {}
   int reserve(size_t bytes) {
     bytes += 64 * 64;
@@ -313,17 +393,44 @@ struct Stager {
     std::memcpy(r, src, n * sizeof(T));
     return put(dst, r, n);
   }
+  // a region of n elements filled later by fill(region, n), then sent to dst
+  template <class T, class F>
+  int defer(T *dst, size_t n, F fill) {
+    T *r = slot<T>(n);
+    if (!r) {
+      set_message("staging buffer overflow");
+      return FSALE_ERR_ARG;
+    }
+    deferred.push_back(Deferred{dst, r, n * sizeof(T)});
+    fills.emplace_back([r, n, fill]() { fill(r, n); });
+    return FSALE_OK;
+  }
+  // drops rows not yet filled (their fill functions may refer to locals of a
+  // caller that failed)
+  void discard() {
+    fills.clear();
+    deferred.clear();
+  }
+  int run_deferred() {
+    RowPool::get().run(fills);
+    fills.clear();
+    for (const Deferred &d : deferred)
+      if (d.bytes) FSALE_HIP(hipMemcpyAsync(d.dst, d.src, d.bytes, hipMemcpyHostToDevice, ctx->upload));
+    deferred.clear();
+    return FSALE_OK;
+  }
   int finish() {
+    int rc = run_deferred();
     FSALE_HIP(hipStreamSynchronize(ctx->upload));
     used = 0;
-    return FSALE_OK;
+    return rc;
   }
 };
 
 int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, long long cap_c, long long cap_f,
                 long long cap_v, long long cap_n) {
-  if (!((dim == 2 && vpc >= 3 && vpc <= 8) || (dim == 3 && vpc == 8))) {
-    set_message("unsupported cell type (quads/polygons in 2D, hexes in 3D)");
+  if (!((dim == 2 && vpc == 4) || (dim == 3 && vpc == 8))) {
+    set_message("unsupported cell type (quads in 2D, hexes in 3D)");
     return FSALE_ERR_MESH;
   }
   if (solver == FSALE_SOLVER_MAIRE && (dim != 2 || vpc != 4)) {
@@ -477,21 +584,28 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   *out_prob = prob;
   const int nc = t.nc, nf = t.nf, nv = t.nv;
   const bool maire = ctx->solver == FSALE_SOLVER_MAIRE;
+  // what the kernels of this context read: batched 2D Eulerian and Lagrangian
+  // steps rebuild face geometry from the vertices (the plain drop-in context
+  // keeps every face array for its task-level entry points and field reads);
+  // the CSR neighbour list only for cells past the k-major head
+  const bool faces = ctx->single || m.dim == 3;
+  const bool cell_faces = faces || !maire;
+  int max_deg = 0;
+  for (int c = 0; c < nc; ++c) max_deg = std::max(max_deg, t.nbr_off[c + 1] - t.nbr_off[c]);
+  const bool nbr_tail = !maire && (ctx->single || max_deg > m.nbk_deg);
+  ctx->probs[prob].faces_on_device = faces;
   // staging: connectivity rows, coordinates, Lagrangian lists and tags, and
   // room for the caller's state arrays, parameters and scalars
   size_t bytes = 4 * ((size_t)nc * (t.vpc + 2 * t.fpc + m.nbk_deg + 2) + (size_t)nf * (2 + t.vpf) + P.nn) +
                  8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
   if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + (size_t)nc * t.vpc + nv;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
-  auto rows = [&](int32_t *dst_base, int rows_n, size_t stride, int n, auto &&value) -> int {
+  // k-rows of connectivity, filled on the row pool
+  auto rows = [&](int32_t *dst_base, int rows_n, size_t stride, int n, auto value) -> int {
     for (int k = 0; k < rows_n; ++k) {
-      int32_t *r = st.slot<int32_t>(n);
-      if (!r) {
-        set_message("staging buffer overflow");
-        return FSALE_ERR_ARG;
-      }
-      for (int i = 0; i < n; ++i) r[i] = value(k, i);
-      int rc2 = st.put(dst_base + (size_t)k * stride + 0, r, n);
+      int rc2 = st.defer(dst_base + (size_t)k * stride, (size_t)n, [value, k](int32_t *r, size_t cnt) {
+        for (size_t i = 0; i < cnt; ++i) r[i] = value(k, (int)i);
+      });
       if (rc2) return rc2;
     }
     return FSALE_OK;
@@ -499,40 +613,49 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   // connectivity with batch ids, one k-row segment at a time
   if ((rc = rows(m.cv + P.c0, t.vpc, m.cap_c, nc, [&](int k, int c) { return t.cv[(size_t)c * t.vpc + k] + P.v0; })))
     return rc;
-  if ((rc = rows(m.cf + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
-         const int f = t.cf[(size_t)c * t.fpc + k];
-         return 2 * (f + P.f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
-       })))
+  if (faces && (rc = rows(m.cf + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
+                  const int f = t.cf[(size_t)c * t.fpc + k];
+                  return 2 * (f + P.f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
+                })))
     return rc;
-  if ((rc = rows(m.cfn + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
-         const int f = t.cf[(size_t)c * t.fpc + k];
-         const bool second = t.fc[2 * (size_t)f] != c;
-         const int other = second ? t.fc[2 * (size_t)f] : t.fc[2 * (size_t)f + 1];
-         return (other >= 0 ? other + P.c0 + 1 : 0) * 2 + (second ? 1 : 0);
-       })))
+  if (cell_faces && (rc = rows(m.cfn + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
+                       const int f = t.cf[(size_t)c * t.fpc + k];
+                       const bool second = t.fc[2 * (size_t)f] != c;
+                       const int other = second ? t.fc[2 * (size_t)f] : t.fc[2 * (size_t)f + 1];
+                       return (other >= 0 ? other + P.c0 + 1 : 0) * 2 + (second ? 1 : 0);
+                     })))
     return rc;
-  if ((rc = rows(m.fc + P.f0, 2, m.cap_f, nf, [&](int k, int f) {
-         const int c = t.fc[2 * (size_t)f + k];
-         return c >= 0 ? c + P.c0 : -1;
-       })))
+  if (ctx->single && (rc = rows(m.fc + P.f0, 2, m.cap_f, nf, [&](int k, int f) {
+                        const int c = t.fc[2 * (size_t)f + k];
+                        return c >= 0 ? c + P.c0 : -1;
+                      })))
     return rc;
-  if ((rc = rows(m.fv + P.f0, t.vpf, m.cap_f, nf, [&](int k, int f) { return t.fv[(size_t)f * t.vpf + k] + P.v0; })))
+  if (faces && (rc = rows(m.fv + P.f0, t.vpf, m.cap_f, nf,
+                          [&](int k, int f) { return t.fv[(size_t)f * t.vpf + k] + P.v0; })))
     return rc;
-  if ((rc = rows(m.nbr_start + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c] + P.n0; }))) return rc;
-  if ((rc = rows(m.nbr_cnt + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c + 1] - t.nbr_off[c]; }))) return rc;
-  if ((rc = rows(m.nbr + P.n0, 1, 0, P.nn, [&](int, int i) { return t.nbr[i] + P.c0; }))) return rc;
-  if ((rc = rows(m.nbk + P.c0, m.nbk_deg, m.cap_c, nc, [&](int k, int c) {
-         const int cnt = t.nbr_off[c + 1] - t.nbr_off[c];
-         return k < cnt ? t.nbr[t.nbr_off[c] + k] + P.c0 : -1;
-       })))
-    return rc;
-  for (int d = 0; d < m.dim; ++d) {
-    double *x = st.slot<double>(nv);
-    if (!x) return FSALE_ERR_ARG;
-    for (int v = 0; v < nv; ++v) x[v] = mesh.coords[(size_t)v * m.dim + d];
-    if ((rc = st.put(m.x[d] + P.v0, x, nv))) return rc;
+  if (!maire) {
+    if ((rc = rows(m.nbr_cnt + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c + 1] - t.nbr_off[c]; })))
+      return rc;
+    if ((rc = rows(m.nbk + P.c0, m.nbk_deg, m.cap_c, nc, [&](int k, int c) {
+           const int cnt = t.nbr_off[c + 1] - t.nbr_off[c];
+           return k < cnt ? t.nbr[t.nbr_off[c] + k] + P.c0 : -1;
+         })))
+      return rc;
   }
-  if ((rc = launch_geometry(ctx, prob, true, true, ctx->upload))) return rc;
+  if (nbr_tail) {
+    if ((rc = rows(m.nbr_start + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c] + P.n0; }))) return rc;
+    if ((rc = rows(m.nbr + P.n0, 1, 0, P.nn, [&](int, int i) { return t.nbr[i] + P.c0; }))) return rc;
+  }
+  for (int d = 0; d < m.dim; ++d) {
+    const double *xs = mesh.coords;
+    const int dim = m.dim;
+    if ((rc = st.defer(m.x[d] + P.v0, (size_t)nv, [xs, dim, d](double *x, size_t n) {
+           for (size_t v = 0; v < n; ++v) x[v] = xs[v * dim + d];
+         })))
+      return rc;
+  }
+  if ((rc = st.run_deferred())) return rc;
+  if ((rc = launch_geometry(ctx, prob, true, faces, ctx->upload))) return rc;
   if (maire) {
     // vertex -> corner lists and boundary tags from the initial face normals:
     // tag d when |n_d| is 1 within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162);
@@ -582,7 +705,9 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     if ((rc = st.put(m.bv + P.v0, bvl, nb))) return rc;
     ctx->probs[prob].nb = nb;
   }
-  return FSALE_OK;
+  // the deferred rows read this function's locals (P, the row lambdas): fill
+  // and send them before returning
+  return st.run_deferred();
 }
 
 // A problem's parameters and fresh scalars (time t, time step dt), staged.
@@ -693,9 +818,15 @@ int field_view(fsale_ctx *ctx, int prob, int field, FieldView &fv) {
     case FSALE_F_CELL_CENTROID: vec(m.ccen, N, P.nc, P.c0); fv.writable = false; return FSALE_OK;
     case FSALE_F_CELL_VOLUME: one(m.cvol, P.nc, P.c0); fv.writable = false; return FSALE_OK;
     case FSALE_F_CELL_MIN_LENGTH: one(m.cminlen, P.nc, P.c0); fv.writable = false; return FSALE_OK;
-    case FSALE_F_FACE_CENTROID: vec(m.fcen, N, P.nf, P.f0); fv.writable = false; return FSALE_OK;
-    case FSALE_F_FACE_NORMAL: vec(m.fnrm, N, P.nf, P.f0); fv.writable = false; return FSALE_OK;
-    case FSALE_F_FACE_AREA: one(m.farea, P.nf, P.f0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_FACE_CENTROID:
+    case FSALE_F_FACE_NORMAL:
+    case FSALE_F_FACE_AREA:
+      if (!P.faces_on_device) break;
+      if (field == FSALE_F_FACE_CENTROID) vec(m.fcen, N, P.nf, P.f0);
+      if (field == FSALE_F_FACE_NORMAL) vec(m.fnrm, N, P.nf, P.f0);
+      if (field == FSALE_F_FACE_AREA) one(m.farea, P.nf, P.f0);
+      fv.writable = false;
+      return FSALE_OK;
     case FSALE_F_COORDS: vec(m.x, N, P.nv, P.v0); return FSALE_OK;
     default: break;
   }
@@ -738,6 +869,56 @@ int get_problem_field(fsale_ctx *ctx, int prob, int field, double *host_out) {
   return check_device_error(ctx, prob);
 }
 
+// Several fields of one problem with one synchronisation: the arrays and the
+// problem's scalars go through one pinned buffer.
+int get_problem_fields(fsale_ctx *ctx, int prob, int n, const int32_t *fields, double *const *outs) {
+  if (!valid_prob(ctx, prob) || n <= 0 || !fields || !outs) return FSALE_ERR_ARG;
+  std::vector<FieldView> views(n);
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!outs[i]) return FSALE_ERR_ARG;
+    int rc = field_view(ctx, prob, fields[i], views[i]);
+    if (rc) return rc;
+    if (views[i].corner) {   // corner fields: one at a time
+      for (int j = 0; j < n; ++j)
+        if ((rc = get_problem_field(ctx, prob, fields[j], outs[j]))) return rc;
+      return FSALE_OK;
+    }
+    total += (size_t)views[i].ncomp * views[i].n;
+  }
+  const size_t bytes = total * sizeof(double) + sizeof(DevScalars) + 64;
+  if (bytes > ctx->dl_bytes) {
+    FSALE_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->dl) (void)hipHostFree(ctx->dl);
+    ctx->dl = nullptr;
+    ctx->dl_bytes = 0;
+    FSALE_HIP(hipHostMalloc(&ctx->dl, bytes, hipHostMallocDefault));
+    ctx->dl_bytes = bytes;
+  }
+  double *buf = static_cast<double *>(ctx->dl);
+  size_t off = 0;
+  for (const FieldView &fv : views)
+    for (int k = 0; k < fv.ncomp; ++k) {
+      FSALE_HIP(hipMemcpyAsync(buf + off, fv.ptrs[k] + fv.first, fv.n * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+      off += fv.n;
+    }
+  DevScalars *sc = reinterpret_cast<DevScalars *>(buf + off);
+  FSALE_HIP(hipMemcpyAsync(sc, ctx->s + prob, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+  FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->h_s[prob] = *sc;
+  off = 0;
+  for (int i = 0; i < n; ++i) {
+    const FieldView &fv = views[i];
+    for (int k = 0; k < fv.ncomp; ++k) {
+      for (int e = 0; e < fv.n; ++e) outs[i][(size_t)e * fv.ncomp + k] = buf[off + e];
+      off += fv.n;
+    }
+  }
+  if (sc->err_code) return check_device_error(ctx, prob);
+  return FSALE_OK;
+}
+
 int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in) {
   if (!valid_prob(ctx, prob) || !host_in) return FSALE_ERR_ARG;
   FieldView fv;
@@ -768,7 +949,7 @@ int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in
     }
   }
   ctx->probs[prob].partials_valid = false;
-  if (field == FSALE_F_COORDS) return launch_geometry(ctx, prob, true, true, ctx->stream);
+  if (field == FSALE_F_COORDS) return launch_geometry(ctx, prob, true, ctx->probs[prob].faces_on_device, ctx->stream);
   return FSALE_OK;
 }
 
@@ -1058,6 +1239,8 @@ int fsale_ctx_create(const fsale_mesh_desc *mesh, int solver, fsale_ctx **out) {
   Stager st(ctx);
   if ((rc = place_mesh(ctx, *mesh, &prob, st)) || (rc = stage_scalars(ctx, prob, st, 0.0, 0.0)) ||
       (rc = st.finish())) {
+    st.discard();
+    (void)hipStreamSynchronize(ctx->upload);
     fsale_ctx_destroy(ctx);
     return rc;
   }
@@ -1077,6 +1260,7 @@ int fsale_ctx_destroy(fsale_ctx *ctx) {
   for (void *p : ctx->allocations) (void)hipFree(p);
   if (ctx->h_s) (void)hipHostFree(ctx->h_s);
   if (ctx->stage) (void)hipHostFree(ctx->stage);
+  if (ctx->dl) (void)hipHostFree(ctx->dl);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->upload) (void)hipStreamDestroy(ctx->upload);
   delete ctx;
@@ -1226,6 +1410,7 @@ int fsale_batch_add_euler(fsale_ctx *ctx, const fsale_mesh_desc *mesh, fsale_eos
   int prob = -1, rc;
   Stager st(ctx);
   if ((rc = place_mesh(ctx, *mesh, &prob, st))) {
+    st.discard();
     (void)st.finish();
     if (prob >= 0 && prob < ctx->max_problems && ctx->probs[prob].live) fsale_batch_remove(ctx, prob);
     return rc;
@@ -1262,6 +1447,7 @@ int fsale_batch_add_maire(fsale_ctx *ctx, const fsale_mesh_desc *mesh, double cv
   int prob = -1, rc;
   Stager st(ctx);
   if ((rc = place_mesh(ctx, *mesh, &prob, st))) {
+    st.discard();
     (void)st.finish();
     if (prob >= 0 && prob < ctx->max_problems && ctx->probs[prob].live) fsale_batch_remove(ctx, prob);
     return rc;
@@ -1338,6 +1524,10 @@ int fsale_batch_status(fsale_ctx *ctx, int32_t max_out, fsale_problem_status *ou
 
 int fsale_batch_get_field(fsale_ctx *ctx, int32_t problem, int field, double *host_out) {
   return get_problem_field(ctx, problem, field, host_out);
+}
+
+int fsale_batch_get_fields(fsale_ctx *ctx, int32_t problem, int32_t n, const int32_t *fields, double *const *host_out) {
+  return get_problem_fields(ctx, problem, n, fields, host_out);
 }
 
 int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_faces, int64_t *free_vertices,

@@ -54,12 +54,30 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerField
       double vel[N];
       for (int d = 0; d < N; ++d) vel[d] = u.v[d][c];
       const double a = u.a[c], vol = m.cvol[c];
+      double px[FPC], py[FPC];
+      if constexpr (N == 2) {
+        for (int k = 0; k < FPC; ++k) {
+          const int v = m.cv[(size_t)k * m.cap_c + c];
+          px[k] = m.x[0][v];
+          py[k] = m.x[1][v];
+        }
+      }
 #pragma unroll
       for (int k = 0; k < FPC; ++k) {
-        const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
-        double n[N];
-        for (int d = 0; d < N; ++d) n[d] = m.fnrm[d][f];
-        const double delta_x = vol / m.farea[f];
+        double n[N], area;
+        if constexpr (N == 2) {
+          // the face's normal up to its sign (|v.n| does not see it) and its
+          // length, from the cell's own vertices, bit-identical to the stored
+          // face geometry
+          double fx, fy;
+          const int k1 = (k + 1 == FPC) ? 0 : k + 1;
+          face_geom_2d(px[k], py[k], px[k1], py[k1], fx, fy, n[0], n[1], area);
+        } else {
+          const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
+          for (int d = 0; d < N; ++d) n[d] = m.fnrm[d][f];
+          area = m.farea[f];
+        }
+        const double delta_x = vol / area;
         const double dti = (a + fabs(dotN<N>(vel, n))) / delta_x;
         dt_inv = smax(dti, dt_inv);
       }

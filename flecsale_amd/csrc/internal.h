@@ -200,6 +200,7 @@ struct Problem {
   int nb = 0;                                                           // boundary vertices (Maire)
   std::shared_ptr<const HostTopology> topo;
   ProbParams params{};
+  bool faces_on_device = true;   // face connectivity and geometry uploaded
   bool have_state = false;
   bool partials_valid = false;
 };
@@ -257,6 +258,8 @@ struct fsale_ctx {
   bool tiles_dirty = true;
   void *stage = nullptr;               // pinned staging buffer for uploads
   size_t stage_bytes = 0;
+  void *dl = nullptr;                  // pinned buffer for multi-field downloads
+  size_t dl_bytes = 0;
   fsale::GraphCache graph;
   fsale::LaunchShape shape;
   std::vector<void *> allocations;

@@ -85,15 +85,16 @@ int retire(fsale_server *S, int solver, int problem, const fsale_problem_status 
     R.rv.resize((size_t)R.nc * R.dim);
     R.re.resize(R.nc);
     R.rp.resize(R.nc);
-    if ((rc = fsale_batch_get_field(b, problem, FSALE_F_DENSITY, R.rd.data())) ||
-        (rc = fsale_batch_get_field(b, problem, FSALE_F_VELOCITY, R.rv.data())) ||
-        (rc = fsale_batch_get_field(b, problem, FSALE_F_INTERNAL_ENERGY, R.re.data())) ||
-        (rc = fsale_batch_get_field(b, problem, FSALE_F_PRESSURE, R.rp.data())))
-      return rc;
+    const int32_t ids[5] = {FSALE_F_DENSITY, FSALE_F_VELOCITY, FSALE_F_INTERNAL_ENERGY, FSALE_F_PRESSURE,
+                            FSALE_F_COORDS};
+    double *outs[5] = {R.rd.data(), R.rv.data(), R.re.data(), R.rp.data(), nullptr};
+    int n = 4;
     if (solver == FSALE_SOLVER_MAIRE) {
       R.rx.resize((size_t)R.nv * R.dim);
-      if ((rc = fsale_batch_get_field(b, problem, FSALE_F_COORDS, R.rx.data()))) return rc;
+      outs[4] = R.rx.data();
+      n = 5;
     }
+    if ((rc = fsale_batch_get_fields(b, problem, n, ids, outs))) return rc;
   }
   if ((rc = fsale_batch_remove(S->batch[solver], problem))) return rc;
   S->running[solver].erase(problem);

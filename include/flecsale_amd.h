@@ -261,6 +261,10 @@ int fsale_batch_step(fsale_ctx *ctx, int nsteps, int use_graph);
 /* Status of up to max_out resident problems in id order. */
 int fsale_batch_status(fsale_ctx *ctx, int32_t max_out, fsale_problem_status *out, int32_t *count);
 int fsale_batch_get_field(fsale_ctx *ctx, int32_t problem, int field, double *host_out);
+/* n fields of one problem (fields[i] into host_out[i]) with one device
+ * synchronisation. */
+int fsale_batch_get_fields(fsale_ctx *ctx, int32_t problem, int32_t n, const int32_t *fields,
+                           double *const *host_out);
 /* Free capacity (sums; a range still has to fit one contiguous gap). */
 int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_faces,
                          int64_t *free_vertices, int64_t *free_neighbours, int32_t *free_problems);
