@@ -166,6 +166,16 @@ def serve_stream(fa, inputs, n, steps=100, calib=24):
     t_gen = time.perf_counter()
     requests = serve_requests(inputs, n, steps)
     log(f"[serve] {n} requests built in {time.perf_counter() - t_gen:.1f}s")
+    # register every distinct mesh once (connectivity cache) before any timing,
+    # so both serving modes start from the same warm cache
+    t_reg = time.perf_counter()
+    seen = set()
+    for _, prob in requests:
+        key = id(prob["cells"])
+        if key not in seen:
+            seen.add(key)
+            fa.connectivity(prob["coords"], prob["cells"])
+    log(f"[serve] {len(seen)} distinct meshes registered in {time.perf_counter() - t_reg:.1f}s")
     cal = serve_trace(fa, requests[:calib], 1)
     mu = calib / cal["makespan_s"]
     rate = 2 * mu

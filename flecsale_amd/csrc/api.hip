@@ -34,6 +34,13 @@ int RangeAlloc::alloc(int n) {
   return -1;
 }
 
+bool RangeAlloc::fits(int n) const {
+  if (n <= 0) return true;
+  for (const auto &kv : free)
+    if (kv.second >= n) return true;
+  return false;
+}
+
 void RangeAlloc::release(int start, int n) {
   if (n <= 0) return;
   auto it = free.emplace(start, n).first;
@@ -535,6 +542,13 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     set_message("no free problem slot");
     return FSALE_ERR_OOM;
   }
+  // capacity known from the descriptor alone: fail before any connectivity work
+  if (!ctx->ra_c.fits(mesh.num_cells) || !ctx->ra_v.fits(mesh.num_vertices) ||
+      !ctx->ra_ct.fits((mesh.num_cells + kBlock - 1) / kBlock) ||
+      !ctx->ra_vt.fits((mesh.num_vertices + kBlock - 1) / kBlock)) {
+    set_message("batch capacity exhausted");
+    return FSALE_ERR_OOM;
+  }
   int rc;
   std::string err;
   std::shared_ptr<const HostTopology> topo = cached_topology(mesh, rc, err);
@@ -543,11 +557,10 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     return rc;
   }
   const HostTopology &t = *topo;
-  for (int v = 0; v < t.nv; ++v)
-    if (t.vc_off[v + 1] - t.vc_off[v] > kMaxCornersPerVertex) {
-      set_message("more than 8 cells around a vertex");
-      return FSALE_ERR_MESH;
-    }
+  if (t.max_corners > kMaxCornersPerVertex) {
+    set_message("more than 8 cells around a vertex");
+    return FSALE_ERR_MESH;
+  }
   Problem P;
   P.nc = t.nc;
   P.nf = t.nf;
@@ -590,9 +603,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   // the CSR neighbour list only for cells past the k-major head
   const bool faces = ctx->single || m.dim == 3;
   const bool cell_faces = faces || !maire;
-  int max_deg = 0;
-  for (int c = 0; c < nc; ++c) max_deg = std::max(max_deg, t.nbr_off[c + 1] - t.nbr_off[c]);
-  const bool nbr_tail = !maire && (ctx->single || max_deg > m.nbk_deg);
+  const bool nbr_tail = !maire && (ctx->single || t.max_neighbours > m.nbk_deg);
   ctx->probs[prob].faces_on_device = faces;
   // staging: connectivity rows, coordinates, Lagrangian lists and tags, and
   // room for the caller's state arrays, parameters and scalars
@@ -1301,13 +1312,14 @@ int fsale_get_topology(fsale_ctx *ctx, int32_t *face_cells, int32_t *cell_faces,
 int fsale_mesh_connectivity(const fsale_mesh_desc *mesh, int32_t *counts, int32_t *face_cells,
                             int32_t *cell_faces, int32_t *nbr_off, int32_t *nbr) {
   if (!mesh || !counts) return FSALE_ERR_ARG;
-  HostTopology t;
   std::string err;
-  int rc = build_topology(*mesh, t, err);
+  int rc;
+  std::shared_ptr<const HostTopology> tp = cached_topology(*mesh, rc, err);
   if (rc) {
     set_message(err);
     return rc;
   }
+  const HostTopology &t = *tp;
   counts[0] = t.nc;
   counts[1] = t.nv;
   counts[2] = t.nf;

@@ -57,6 +57,8 @@ struct HostTopology {
   std::vector<int32_t> nbr;
   std::vector<int32_t> vc_off;    // [nv+1]
   std::vector<int32_t> vc;        // corner ids c*vpc + k, ascending
+  int max_corners = 0;            // most corners around one vertex
+  int max_neighbours = 0;         // most cell-vertex neighbours of one cell
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
@@ -215,6 +217,7 @@ struct RangeAlloc {
     if (cap > 0) free[0] = cap;
   }
   int alloc(int n);          // -1 if no room
+  bool fits(int n) const;    // a free range of n exists
   void release(int start, int n);
   long long available() const;
 };

@@ -6,6 +6,7 @@
 // through the cells around its first vertex.
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -40,11 +41,23 @@ inline int vertex_faces(const HostTopology &t, int kk, int j) {
 }
 
 inline std::array<int32_t, 4> face_key(const HostTopology &t, int c, int lf) {
+  const int32_t *cv = &t.cv[(size_t)c * t.vpc];
+  if (t.dim == 2) {
+    const int32_t a = cv[lf], b = cv[lf + 1 == t.vpc ? 0 : lf + 1];
+    return {std::min(a, b), std::max(a, b), INT32_MAX, INT32_MAX};
+  }
   int lv[4];
   local_face(t.dim, t.vpc, lf, lv);
-  std::array<int32_t, 4> k{INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
-  for (int i = 0; i < t.vpf; ++i) k[i] = t.cv[(size_t)c * t.vpc + lv[i]];
-  std::sort(k.begin(), k.begin() + t.vpf);
+  std::array<int32_t, 4> k{cv[lv[0]], cv[lv[1]], cv[lv[2]], cv[lv[3]]};
+  // sorting network for 4 keys
+  auto cs = [&](int i, int j) {
+    if (k[j] < k[i]) std::swap(k[i], k[j]);
+  };
+  cs(0, 1);
+  cs(2, 3);
+  cs(0, 2);
+  cs(1, 3);
+  cs(1, 2);
   return k;
 }
 }  // namespace
@@ -77,11 +90,24 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
   // vertex -> corner (c*vpc+k), ascending cell id by construction
   t.vc_off.assign(t.nv + 1, 0);
   for (size_t i = 0; i < nslots; ++i) t.vc_off[t.cv[i] + 1]++;
-  for (int v = 0; v < t.nv; ++v) t.vc_off[v + 1] += t.vc_off[v];
+  for (int v = 0; v < t.nv; ++v) {
+    t.max_corners = std::max(t.max_corners, t.vc_off[v + 1]);
+    t.vc_off[v + 1] += t.vc_off[v];
+  }
   t.vc.resize(nslots);
   {
     std::vector<int32_t> fill(t.vc_off.begin(), t.vc_off.end() - 1);
     for (size_t i = 0; i < nslots; ++i) t.vc[fill[t.cv[i]]++] = (int32_t)i;
+  }
+  if (t.max_corners > kMaxCornersPerVertex) {
+    err = "more than 8 cells around a vertex";
+    return FSALE_ERR_MESH;
+  }
+  // cell and local vertex of each vertex-corner entry
+  std::vector<int32_t> vcell(nslots), vlocal(nslots);
+  for (size_t q = 0; q < nslots; ++q) {
+    vcell[q] = t.vc[q] / t.vpc;
+    vlocal[q] = t.vc[q] - vcell[q] * t.vpc;
   }
 
   // faces by first visit; each face is matched among the faces (with the
@@ -103,7 +129,7 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
       int other = -1, other_lf = -1, matches = 0;
       const int v0 = key[0];
       for (int q = t.vc_off[v0]; q < t.vc_off[v0 + 1]; ++q) {
-        const int c2 = t.vc[q] / t.vpc, kk = t.vc[q] % t.vpc;
+        const int c2 = vcell[q], kk = vlocal[q];
         if (c2 == c) continue;
         // only the faces of c2 that hold its local vertex kk can match
         const auto *k2 = &keys[(size_t)c2 * t.fpc];
@@ -154,21 +180,35 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
   t.nbr_off.assign(t.nc + 1, 0);
   t.nbr.clear();
   t.nbr.reserve((size_t)t.nc * (t.dim == 2 ? 8 : 26));
-  std::vector<int32_t> tmp;
-  tmp.reserve(64);
+  // gather the cells around the cell's vertices, then an insertion sort and
+  // duplicate removal (a few dozen entries)
+  int32_t tmp[8 * kMaxCornersPerVertex + 8];
   for (int c = 0; c < t.nc; ++c) {
-    tmp.clear();
+    int n = 0;
     for (int k = 0; k < t.vpc; ++k) {
       const int v = t.cv[(size_t)c * t.vpc + k];
       for (int q = t.vc_off[v]; q < t.vc_off[v + 1]; ++q) {
-        const int c2 = t.vc[q] / t.vpc;
-        if (c2 != c) tmp.push_back(c2);
+        const int c2 = vcell[q];
+        if (c2 != c && n < (int)(sizeof(tmp) / sizeof(tmp[0]))) tmp[n++] = c2;
       }
     }
-    std::sort(tmp.begin(), tmp.end());
-    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-    t.nbr.insert(t.nbr.end(), tmp.begin(), tmp.end());
+    for (int i = 1; i < n; ++i) {
+      const int32_t x = tmp[i];
+      int j = i - 1;
+      while (j >= 0 && tmp[j] > x) {
+        tmp[j + 1] = tmp[j];
+        --j;
+      }
+      tmp[j + 1] = x;
+    }
+    int count = 0;
+    for (int i = 0; i < n; ++i)
+      if (i == 0 || tmp[i] != tmp[i - 1]) {
+        t.nbr.push_back(tmp[i]);
+        ++count;
+      }
     t.nbr_off[c + 1] = (int32_t)t.nbr.size();
+    t.max_neighbours = std::max(t.max_neighbours, count);
   }
   return FSALE_OK;
 }
@@ -176,9 +216,17 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
 // Process-wide connectivity cache: meshes are identified by their cell table
 // (a hash to find candidates, the table itself to confirm), so every context
 // and batch that sees the same mesh again skips the build.  Oldest entries
-// are dropped past kTopologyCacheBytes.
+// are dropped past the budget.
 namespace {
-constexpr size_t kTopologyCacheBytes = size_t(4) << 30;
+// budget: FSALE_TOPOLOGY_CACHE_MB in the environment, default 8 GiB
+size_t topology_cache_budget() {
+  static const size_t b = [] {
+    const char *e = std::getenv("FSALE_TOPOLOGY_CACHE_MB");
+    const long long mb = e ? std::atoll(e) : 8192;
+    return (size_t)(mb > 0 ? mb : 0) << 20;
+  }();
+  return b;
+}
 
 struct TopologyCache {
   std::mutex mu;
@@ -236,7 +284,7 @@ std::shared_ptr<const HostTopology> cached_topology(const fsale_mesh_desc &mesh,
   C.map.emplace(h, t);
   C.order.emplace_back(h, t.get());
   C.bytes += b;
-  while (C.bytes > kTopologyCacheBytes && C.order.size() > 1) {
+  while (C.bytes > topology_cache_budget() && C.order.size() > 1) {
     const auto victim = C.order.front();
     C.order.pop_front();
     auto range = C.map.equal_range(victim.first);

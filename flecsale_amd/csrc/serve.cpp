@@ -40,6 +40,7 @@ struct Request {
   fsale_maire_params mp{};
   int nc = 0, nv = 0, dim = 0;
   int problem = -1;
+  int64_t blocked_epoch = -1;   // capacity epoch of the last failed admission
   fsale_problem_status status{};
   // results
   std::vector<double> rd, rv, re, rp, rx;
@@ -59,6 +60,7 @@ struct fsale_server {
   std::deque<int64_t> queue;                  // arrival order
   std::map<int, int64_t> running[2];          // batch problem -> request
   int64_t next_id = 0;
+  int64_t epoch = 0;            // retirements so far: capacity can only grow with them
   // host seconds spent per phase: admission, waiting for a round, result
   // download + retirement, launching rounds; rounds and admissions counted
   double t_admit = 0, t_wait = 0, t_retire = 0, t_launch = 0;
@@ -98,6 +100,7 @@ int retire(fsale_server *S, int solver, int problem, const fsale_problem_status 
   }
   if ((rc = fsale_batch_remove(S->batch[solver], problem))) return rc;
   S->running[solver].erase(problem);
+  ++S->epoch;
   R.state = Q_DONE;
   R.finished = now_s();
   return FSALE_OK;
@@ -333,6 +336,12 @@ int fsale_server_poll(fsale_server *S, int32_t *completed, int32_t *pending) {
     const double ta = now_s();
     int blocked = 0;
     for (auto it = S->queue.begin(); it != S->queue.end();) {
+      Request &Rq = S->reqs[*it];
+      if (Rq.blocked_epoch == S->epoch) {   // nothing was freed since it last failed
+        if (++blocked > S->cfg.lookahead) break;
+        ++it;
+        continue;
+      }
       rc = admit(S, *it);
       if (rc == FSALE_OK) {
         ++S->admissions;
@@ -352,6 +361,7 @@ int fsale_server_poll(fsale_server *S, int32_t *completed, int32_t *pending) {
         it = S->queue.erase(it);
         continue;
       }
+      Rq.blocked_epoch = S->epoch;
       if (++blocked > S->cfg.lookahead) break;
       ++it;
     }

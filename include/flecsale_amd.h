@@ -113,7 +113,10 @@ int fsale_get_topology(fsale_ctx *ctx, int32_t *face_cells, int32_t *cell_faces,
                        int32_t *nbr_off, int32_t *nbr);
 /* Host-only connectivity of a mesh, without a device (the reference's
  * connectivity dump, the print task of apps/hydro/tasks.h:741-766).  Call
- * with NULL arrays to get counts, then again with arrays of those sizes. */
+ * with NULL arrays to get counts, then again with arrays of those sizes.
+ * Connectivity is kept in a process-wide cache keyed by the cell table
+ * (budget FSALE_TOPOLOGY_CACHE_MB, default 8192), so this also registers a
+ * mesh ahead of the contexts and batches that will use it. */
 int fsale_mesh_connectivity(const fsale_mesh_desc *mesh, int32_t *counts, int32_t *face_cells,
                             int32_t *cell_faces, int32_t *nbr_off, int32_t *nbr);
 int fsale_get_field(fsale_ctx *ctx, int field, double *host_out);
