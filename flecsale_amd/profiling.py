@@ -73,8 +73,8 @@ def _short(name):
     return re.sub(r"\(.*", "", str(name).replace("fsale::(anonymous namespace)::", "").replace("void ", ""))
 
 
-def _per_kernel(path, counter=None):
-    """{kernel short name: (dispatches, mean counter value per dispatch)} from
+def _per_kernel(path, counter=None, reduce="mean"):
+    """{kernel short name: (dispatches, mean (or max) counter value per dispatch)} from
     rocprofv3 output: a counter_collection.csv (or a directory holding one),
     or a rocpd SQLite database."""
     acc = {}
@@ -107,18 +107,24 @@ def _per_kernel(path, counter=None):
             disp = r[idisp] if idisp is not None else None
             acc.setdefault(name, {}).setdefault(disp, 0.0)
             acc[name][disp] += float(r[iv])
+    if reduce == "max":
+        return {k: (len(v), max(v.values())) for k, v in acc.items()}
     return {k: (len(v), sum(v.values()) / len(v)) for k, v in acc.items()}
 
 
 def summarize(fetch_db, write_db, workload):
     fetch = _per_kernel(fetch_db)
     write = _per_kernel(write_db)
+    # calibration: the largest dispatch of the runtime's copy kernel is one of
+    # the 512 MiB device-to-device copies (smaller copy dispatches are uploads)
+    fmax = _per_kernel(fetch_db, reduce="max")
+    wmax = _per_kernel(write_db, reduce="max")
     copy = [k for k in fetch if "copy" in k.lower()]
     if not copy:
         raise SystemExit("no copy kernel found for calibration")
-    ck = max(copy, key=lambda k: fetch[k][1])
-    f_scale = COPY_BYTES / (fetch[ck][1] * 1024.0)
-    w_scale = COPY_BYTES / (write[ck][1] * 1024.0)
+    ck = max(copy, key=lambda k: fmax[k][1])
+    f_scale = COPY_BYTES / (fmax[ck][1] * 1024.0)
+    w_scale = COPY_BYTES / (wmax[ck][1] * 1024.0)
     table = {}
     for k in fetch:
         if k not in write or k in copy:
@@ -143,9 +149,34 @@ def summarize(fetch_db, write_db, workload):
         print(f"| {k} | {v['dispatches']} | {v['fetch_KiB']:.0f} | {v['write_KiB']:.0f} | {v['hbm_bytes']:.4e} |")
 
 
+def trace(trace_dir, out_path, command):
+    """Markdown summary of a rocprofv3 --kernel-trace --stats run (its
+    kernel_stats.csv): calls, average and total time per kernel."""
+    import csv
+    hits = [os.path.join(dp, f) for dp, _, fs in os.walk(trace_dir) for f in fs if f.endswith("kernel_stats.csv")]
+    if not hits:
+        raise SystemExit(f"no kernel_stats.csv under {trace_dir}")
+    rows = list(csv.DictReader(open(hits[0], newline="")))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    lines = [f"# rocprofv3 kernel trace: {os.path.basename(out_path).rsplit('.', 1)[0]}", "",
+             "Command (from the repository root on one MI355X):", "", "```", command, "```", "",
+             "| kernel | calls | avg us | min us | max us | total ms | % |", "|---|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.3f} | {100 * float(r['TotalDurationNs']) / total:.1f} |")
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("trace_dir")
+    t.add_argument("out")
+    t.add_argument("--command", default="")
     r = sub.add_parser("run")
     r.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d"])
     r.add_argument("--steps", type=int, default=3)
@@ -157,6 +188,8 @@ def main():
     a = ap.parse_args()
     if a.cmd == "run":
         run(a.workload, a.steps, a.size)
+    elif a.cmd == "trace":
+        trace(a.trace_dir, a.out, a.command)
     else:
         summarize(a.fetch_db, a.write_db, a.workload)
 
