@@ -421,8 +421,17 @@ struct Stager {
   int run_deferred() {
     RowPool::get().run(fills);
     fills.clear();
-    for (const Deferred &d : deferred)
-      if (d.bytes) FSALE_HIP(hipMemcpyAsync(d.dst, d.src, d.bytes, hipMemcpyHostToDevice, ctx->upload));
+    for (size_t i = 0; i < deferred.size(); ++i) {
+      const Deferred &d = deferred[i];
+      if (!d.bytes) continue;
+      const hipError_t e = hipMemcpyAsync(d.dst, d.src, d.bytes, hipMemcpyHostToDevice, ctx->upload);
+      if (e != hipSuccess) {
+        set_message(std::string("upload ") + std::to_string(i) + " of " + std::to_string(deferred.size()) + " (" +
+                    std::to_string(d.bytes) + " bytes): " + hipGetErrorString(e));
+        deferred.clear();
+        return e == hipErrorOutOfMemory ? FSALE_ERR_OOM : FSALE_ERR_HIP;
+      }
+    }
     deferred.clear();
     return FSALE_OK;
   }
@@ -484,6 +493,10 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   m.nbk_deg = dim == 2 ? kNbk2 : kNbk3;
   if ((rc = dev_alloc_n(ctx, &m.nbk, (size_t)cap_c * m.nbk_deg))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cfn, (size_t)cap_c * m.fpc))) return rc;
+  if ((rc = dev_alloc_n(ctx, &m.corig, cap_c))) return rc;
+  if (dim == 3 && solver == FSALE_SOLVER_EULER)
+    for (int i = 0; i < 7; ++i)
+      if ((rc = dev_alloc_n(ctx, &m.kf[i], (size_t)cap_c * m.fpc))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cvol, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cminlen, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.farea, cap_f))) return rc;
@@ -609,6 +622,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   // room for the caller's state arrays, parameters and scalars
   size_t bytes = 4 * ((size_t)nc * (t.vpc + 2 * t.fpc + m.nbk_deg + 2) + (size_t)nf * (2 + t.vpf) + P.nn) +
                  8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
+  bytes += 4 * (size_t)nc;   // original ids
   if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + (size_t)nc * t.vpc + nv;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
   // k-rows of connectivity, filled on the row pool
@@ -621,47 +635,60 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     }
     return FSALE_OK;
   };
-  // connectivity with batch ids, one k-row segment at a time
-  if ((rc = rows(m.cv + P.c0, t.vpc, m.cap_c, nc, [&](int k, int c) { return t.cv[(size_t)c * t.vpc + k] + P.v0; })))
+  // connectivity with batch ids in storage order (row position s holds the
+  // entity t.?idx[s]; every id it stores is a storage position), one k-row
+  // segment at a time
+  const int32_t *cidx = t.cidx.data(), *cpos = t.cpos.data(), *vidx = t.vidx.data(), *vpos = t.vpos.data(),
+                *fidx = t.fidx.data(), *fpos = t.fpos.data();
+  const int c0 = P.c0, v0 = P.v0, f0 = P.f0, n0 = P.n0;
+  if ((rc = rows(m.cv + c0, t.vpc, m.cap_c, nc,
+                 [&t, cidx, vpos, v0](int k, int s) { return vpos[t.cv[(size_t)cidx[s] * t.vpc + k]] + v0; })))
     return rc;
-  if (faces && (rc = rows(m.cf + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
+  if (faces && (rc = rows(m.cf + c0, t.fpc, m.cap_c, nc, [&t, cidx, fpos, f0](int k, int s) {
+                  const int c = cidx[s];
                   const int f = t.cf[(size_t)c * t.fpc + k];
-                  return 2 * (f + P.f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
+                  return 2 * (fpos[f] + f0) + (t.fc[2 * (size_t)f] == c ? 0 : 1);
                 })))
     return rc;
-  if (cell_faces && (rc = rows(m.cfn + P.c0, t.fpc, m.cap_c, nc, [&](int k, int c) {
+  if (cell_faces && (rc = rows(m.cfn + c0, t.fpc, m.cap_c, nc, [&t, cidx, cpos, c0](int k, int s) {
+                       const int c = cidx[s];
                        const int f = t.cf[(size_t)c * t.fpc + k];
                        const bool second = t.fc[2 * (size_t)f] != c;
                        const int other = second ? t.fc[2 * (size_t)f] : t.fc[2 * (size_t)f + 1];
-                       return (other >= 0 ? other + P.c0 + 1 : 0) * 2 + (second ? 1 : 0);
+                       return (other >= 0 ? cpos[other] + c0 + 1 : 0) * 2 + (second ? 1 : 0);
                      })))
     return rc;
-  if (ctx->single && (rc = rows(m.fc + P.f0, 2, m.cap_f, nf, [&](int k, int f) {
-                        const int c = t.fc[2 * (size_t)f + k];
-                        return c >= 0 ? c + P.c0 : -1;
+  if (ctx->single && (rc = rows(m.fc + f0, 2, m.cap_f, nf, [&t, fidx, cpos, c0](int k, int s) {
+                        const int c = t.fc[2 * (size_t)fidx[s] + k];
+                        return c >= 0 ? cpos[c] + c0 : -1;
                       })))
     return rc;
-  if (faces && (rc = rows(m.fv + P.f0, t.vpf, m.cap_f, nf,
-                          [&](int k, int f) { return t.fv[(size_t)f * t.vpf + k] + P.v0; })))
+  if (faces && (rc = rows(m.fv + f0, t.vpf, m.cap_f, nf, [&t, fidx, vpos, v0](int k, int s) {
+                  return vpos[t.fv[(size_t)fidx[s] * t.vpf + k]] + v0;
+                })))
     return rc;
+  if ((rc = rows(m.corig + c0, 1, 0, nc, [cidx](int, int s) { return cidx[s]; }))) return rc;
   if (!maire) {
-    if ((rc = rows(m.nbr_cnt + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c + 1] - t.nbr_off[c]; })))
+    if ((rc = rows(m.nbr_cnt + c0, 1, 0, nc,
+                   [&t, cidx](int, int s) { return t.nbr_off[cidx[s] + 1] - t.nbr_off[cidx[s]]; })))
       return rc;
-    if ((rc = rows(m.nbk + P.c0, m.nbk_deg, m.cap_c, nc, [&](int k, int c) {
+    if ((rc = rows(m.nbk + c0, m.nbk_deg, m.cap_c, nc, [&t, cidx, cpos, c0](int k, int s) {
+           const int c = cidx[s];
            const int cnt = t.nbr_off[c + 1] - t.nbr_off[c];
-           return k < cnt ? t.nbr[t.nbr_off[c] + k] + P.c0 : -1;
+           return k < cnt ? cpos[t.nbr[t.nbr_off[c] + k]] + c0 : -1;
          })))
       return rc;
   }
   if (nbr_tail) {
-    if ((rc = rows(m.nbr_start + P.c0, 1, 0, nc, [&](int, int c) { return t.nbr_off[c] + P.n0; }))) return rc;
-    if ((rc = rows(m.nbr + P.n0, 1, 0, P.nn, [&](int, int i) { return t.nbr[i] + P.c0; }))) return rc;
+    if ((rc = rows(m.nbr_start + c0, 1, 0, nc, [&t, cidx, n0](int, int s) { return t.nbr_off[cidx[s]] + n0; })))
+      return rc;
+    if ((rc = rows(m.nbr + n0, 1, 0, P.nn, [&t, cpos, c0](int, int i) { return cpos[t.nbr[i]] + c0; }))) return rc;
   }
   for (int d = 0; d < m.dim; ++d) {
     const double *xs = mesh.coords;
     const int dim = m.dim;
-    if ((rc = st.defer(m.x[d] + P.v0, (size_t)nv, [xs, dim, d](double *x, size_t n) {
-           for (size_t v = 0; v < n; ++v) x[v] = xs[v * dim + d];
+    if ((rc = st.defer(m.x[d] + v0, (size_t)nv, [xs, dim, d, vidx](double *x, size_t n) {
+           for (size_t s = 0; s < n; ++s) x[s] = xs[(size_t)vidx[s] * dim + d];
          })))
       return rc;
   }
@@ -671,15 +698,18 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     // vertex -> corner lists and boundary tags from the initial face normals:
     // tag d when |n_d| is 1 within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162);
     // the normals are evaluated here with the device's formula
-    if ((rc = rows(m.vc_cnt + P.v0, 1, 0, nv, [&](int, int v) { return t.vc_off[v + 1] - t.vc_off[v]; }))) return rc;
-    if ((rc = rows(m.vck + P.v0, kMaxCornersPerVertex, m.cap_v, nv, [&](int j, int v) {
-           return j < t.vc_off[v + 1] - t.vc_off[v] ? t.vc[t.vc_off[v] + j] + P.c0 * t.vpc : -1;
+    if ((rc = rows(m.vc_cnt + v0, 1, 0, nv,
+                   [&t, vidx](int, int s) { return t.vc_off[vidx[s] + 1] - t.vc_off[vidx[s]]; })))
+      return rc;
+    if ((rc = rows(m.vck + v0, kMaxCornersPerVertex, m.cap_v, nv, [&t, vidx, cpos, c0](int j, int s) {
+           const int v = vidx[s];
+           if (j >= t.vc_off[v + 1] - t.vc_off[v]) return -1;
+           const int corner = t.vc[t.vc_off[v] + j];
+           const int c = corner / t.vpc, k = corner - c * t.vpc;
+           return (cpos[c] + c0) * t.vpc + k;
          })))
       return rc;
-    std::vector<uint8_t> ftag(nf, 0);
-    uint8_t *vbnd = st.slot<uint8_t>(nv);
-    if (!vbnd) return FSALE_ERR_ARG;
-    std::memset(vbnd, 0, nv);
+    std::vector<uint8_t> ftag(nf, 0), vb(nv, 0);
     for (int f = 0; f < nf; ++f) {
       if (t.fc[2 * (size_t)f + 1] >= 0) continue;
       const int a = t.fv[2 * (size_t)f], b = t.fv[2 * (size_t)f + 1];
@@ -691,29 +721,36 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
       for (int d = 0; d < 2; ++d)
         if (std::fabs(std::fabs(n[d]) - 1.0) <= 1e-14) tag |= (uint8_t)(1u << d);
       ftag[f] = tag;
-      for (int i = 0; i < t.vpf; ++i) vbnd[t.fv[(size_t)f * t.vpf + i]] = 1;
+      for (int i = 0; i < t.vpf; ++i) vb[t.fv[(size_t)f * t.vpf + i]] = 1;
     }
     // corner (c,k): low nibble = face of edge (k-1 -> k), high nibble = edge (k -> k+1)
     for (int k = 0; k < t.vpc; ++k) {
       uint8_t *cw = st.slot<uint8_t>(nc);
       if (!cw) return FSALE_ERR_ARG;
       const int km = (k + t.vpc - 1) % t.vpc;
-      for (int c = 0; c < nc; ++c) {
+      for (int sp = 0; sp < nc; ++sp) {
+        const int c = cidx[sp];
         const int fm = t.cf[(size_t)c * t.fpc + km], fp = t.cf[(size_t)c * t.fpc + k];
-        cw[c] = (uint8_t)(ftag[fm] | (ftag[fp] << 4));
+        cw[sp] = (uint8_t)(ftag[fm] | (ftag[fp] << 4));
       }
-      if ((rc = st.put(m.cwtag + (size_t)k * m.cap_c + P.c0, cw, nc))) return rc;
+      if ((rc = st.put(m.cwtag + (size_t)k * m.cap_c + c0, cw, nc))) return rc;
     }
-    if ((rc = st.put(m.vbnd + P.v0, vbnd, nv))) return rc;
-    // the boundary vertices, ascending, for the symmetry-solve launch
+    uint8_t *vbnd = st.slot<uint8_t>(nv);
+    if (!vbnd) return FSALE_ERR_ARG;
     int nb = 0;
-    for (int v = 0; v < nv; ++v) nb += vbnd[v] ? 1 : 0;
+    for (int sp = 0; sp < nv; ++sp) {
+      vbnd[sp] = vb[vidx[sp]];
+      nb += vbnd[sp];
+    }
+    if ((rc = st.put(m.vbnd + v0, vbnd, nv))) return rc;
+    // the boundary vertices (storage positions, ascending) for the
+    // symmetry-solve launch
     int32_t *bvl = st.slot<int32_t>(nb > 0 ? nb : 1);
     if (!bvl) return FSALE_ERR_ARG;
     nb = 0;
-    for (int v = 0; v < nv; ++v)
-      if (vbnd[v]) bvl[nb++] = v + P.v0;
-    if ((rc = st.put(m.bv + P.v0, bvl, nb))) return rc;
+    for (int sp = 0; sp < nv; ++sp)
+      if (vbnd[sp]) bvl[nb++] = sp + v0;
+    if ((rc = st.put(m.bv + v0, bvl, nb))) return rc;
     ctx->probs[prob].nb = nb;
   }
   // the deferred rows read this function's locals (P, the row lambdas): fill
@@ -739,18 +776,22 @@ int stage_state(fsale_ctx *ctx, int prob, Stager &st, const double *d, const dou
                 const double *gamma) {
   const Problem &P = ctx->probs[prob];
   const int N = ctx->m.dim, nc = P.nc;
+  const int32_t *cidx = P.topo->cidx.data();
   const bool maire = ctx->solver == FSALE_SOLVER_MAIRE;
   double *dd = maire ? ctx->ma.d : ctx->eu.d;
   double *pp = maire ? ctx->ma.p : ctx->eu.p;
-  int rc;
-  if ((rc = st.copy(dd + P.c0, d, nc)) || (rc = st.copy(pp + P.c0, p, nc))) return rc;
-  for (int k = 0; k < N; ++k) {
+  // one row per component, in storage order
+  auto row = [&](double *dst, const double *src, int stride, int comp) -> int {
     double *r = st.slot<double>(nc);
     if (!r) return FSALE_ERR_ARG;
-    for (int c = 0; c < nc; ++c) r[c] = v[(size_t)c * N + k];
-    if ((rc = st.put((maire ? ctx->ma.v[k] : ctx->eu.v[k]) + P.c0, r, nc))) return rc;
-  }
-  if (maire && (rc = st.copy(ctx->ma.g + P.c0, gamma, nc))) return rc;
+    for (int sp = 0; sp < nc; ++sp) r[sp] = src[(size_t)cidx[sp] * stride + comp];
+    return st.put(dst + P.c0, r, nc);
+  };
+  int rc;
+  if ((rc = row(dd, d, 1, 0)) || (rc = row(pp, p, 1, 0))) return rc;
+  for (int k = 0; k < N; ++k)
+    if ((rc = row(maire ? ctx->ma.v[k] : ctx->eu.v[k], v, N, k))) return rc;
+  if (maire && (rc = row(ctx->ma.g, gamma, 1, 0))) return rc;
   return FSALE_OK;
 }
 
@@ -760,85 +801,93 @@ struct FieldView {
   int ncomp = 0;          // interleaved components on the host
   int n = 0;              // entities of the problem
   int first = 0;          // first batch id
-  bool corner = false;    // host corner id c*4+k, device [k][cap_c] + c0
+  const int32_t *pos = nullptr;   // storage position of host entity i (cell for corner fields)
+  bool corner = false;    // host corner id c*4+k, device [k][cap_c] + c0 + pos[c]
   bool writable = true;
 };
 
 int field_view(fsale_ctx *ctx, int prob, int field, FieldView &fv) {
   DevMesh &m = ctx->m;
   const Problem &P = ctx->probs[prob];
+  const HostTopology &t = *P.topo;
   const int N = m.dim;
-  auto one = [&](double *p, int n, int first) {
+  // kind: 'c' cells, 'v' vertices, 'f' faces (their storage positions)
+  auto order = [&](char kind) {
+    return kind == 'c' ? t.cpos.data() : kind == 'v' ? t.vpos.data() : t.fpos.data();
+  };
+  auto one = [&](double *p, int n, int first, char kind) {
     fv.ptrs[0] = p;
     fv.ncomp = 1;
     fv.n = n;
     fv.first = first;
+    fv.pos = order(kind);
   };
-  auto vec = [&](double *const *p, int ncomp, int n, int first) {
+  auto vec = [&](double *const *p, int ncomp, int n, int first, char kind) {
     for (int i = 0; i < ncomp; ++i) fv.ptrs[i] = p[i];
     fv.ncomp = ncomp;
     fv.n = n;
     fv.first = first;
+    fv.pos = order(kind);
   };
   if (ctx->solver == FSALE_SOLVER_EULER) {
     EulerFields &u = ctx->eu;
     switch (field) {
-      case FSALE_F_DENSITY: one(u.d, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_VELOCITY: vec(u.v, N, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_INTERNAL_ENERGY: one(u.e, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_PRESSURE: one(u.p, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_TEMPERATURE: one(u.T, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_SOUND_SPEED: one(u.a, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_GRAD_DENSITY: vec(u.gd, N, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_GRAD_VELOCITY: vec(u.gv, N * N, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_GRAD_ENERGY: vec(u.ge, N, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_GRAD_PRESSURE: vec(u.gp, N, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_GRAD_SOUND_SPEED: vec(u.ga, N, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_DENSITY: one(u.d, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_VELOCITY: vec(u.v, N, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_INTERNAL_ENERGY: one(u.e, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_PRESSURE: one(u.p, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_TEMPERATURE: one(u.T, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_SOUND_SPEED: one(u.a, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_GRAD_DENSITY: vec(u.gd, N, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_GRAD_VELOCITY: vec(u.gv, N * N, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_GRAD_ENERGY: vec(u.ge, N, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_GRAD_PRESSURE: vec(u.gp, N, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_GRAD_SOUND_SPEED: vec(u.ga, N, P.nc, P.c0, 'c'); return FSALE_OK;
       case FSALE_F_FLUX:
         if (!u.F[0]) break;
-        vec(u.F, N + 2, P.nf, P.f0);
+        vec(u.F, N + 2, P.nf, P.f0, 'f');
         return FSALE_OK;
       default: break;
     }
   } else {
     MaireFields &u = ctx->ma;
     switch (field) {
-      case FSALE_F_DENSITY: one(u.d, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_VELOCITY: vec(u.v, 2, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_INTERNAL_ENERGY: one(u.e, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_PRESSURE: one(u.p, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_TEMPERATURE: one(u.T, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_SOUND_SPEED: one(u.a, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_MASS: one(u.M, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_VOLUME: one(u.V, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_GAMMA: one(u.g, P.nc, P.c0); return FSALE_OK;
-      case FSALE_F_NODE_VELOCITY: vec(u.un, 2, P.nv, P.v0); return FSALE_OK;
+      case FSALE_F_DENSITY: one(u.d, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_VELOCITY: vec(u.v, 2, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_INTERNAL_ENERGY: one(u.e, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_PRESSURE: one(u.p, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_TEMPERATURE: one(u.T, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_SOUND_SPEED: one(u.a, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_MASS: one(u.M, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_VOLUME: one(u.V, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_GAMMA: one(u.g, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_NODE_VELOCITY: vec(u.un, 2, P.nv, P.v0, 'v'); return FSALE_OK;
       case FSALE_F_CORNER_NORMAL:
-        vec(u.npc, 2, P.nc * m.vpc, P.c0);
+        vec(u.npc, 2, P.nc * m.vpc, P.c0, 'c');
         fv.corner = true;
         return FSALE_OK;
       case FSALE_F_CORNER_FORCE:
-        vec(u.Fpc, 2, P.nc * m.vpc, P.c0);
+        vec(u.Fpc, 2, P.nc * m.vpc, P.c0, 'c');
         fv.corner = true;
         return FSALE_OK;
-      case FSALE_F_RESIDUAL: vec(u.dudt, 4, P.nc, P.c0); return FSALE_OK;
+      case FSALE_F_RESIDUAL: vec(u.dudt, 4, P.nc, P.c0, 'c'); return FSALE_OK;
       default: break;
     }
   }
   switch (field) {
-    case FSALE_F_CELL_CENTROID: vec(m.ccen, N, P.nc, P.c0); fv.writable = false; return FSALE_OK;
-    case FSALE_F_CELL_VOLUME: one(m.cvol, P.nc, P.c0); fv.writable = false; return FSALE_OK;
-    case FSALE_F_CELL_MIN_LENGTH: one(m.cminlen, P.nc, P.c0); fv.writable = false; return FSALE_OK;
+    case FSALE_F_CELL_CENTROID: vec(m.ccen, N, P.nc, P.c0, 'c'); fv.writable = false; return FSALE_OK;
+    case FSALE_F_CELL_VOLUME: one(m.cvol, P.nc, P.c0, 'c'); fv.writable = false; return FSALE_OK;
+    case FSALE_F_CELL_MIN_LENGTH: one(m.cminlen, P.nc, P.c0, 'c'); fv.writable = false; return FSALE_OK;
     case FSALE_F_FACE_CENTROID:
     case FSALE_F_FACE_NORMAL:
     case FSALE_F_FACE_AREA:
       if (!P.faces_on_device) break;
-      if (field == FSALE_F_FACE_CENTROID) vec(m.fcen, N, P.nf, P.f0);
-      if (field == FSALE_F_FACE_NORMAL) vec(m.fnrm, N, P.nf, P.f0);
-      if (field == FSALE_F_FACE_AREA) one(m.farea, P.nf, P.f0);
+      if (field == FSALE_F_FACE_CENTROID) vec(m.fcen, N, P.nf, P.f0, 'f');
+      if (field == FSALE_F_FACE_NORMAL) vec(m.fnrm, N, P.nf, P.f0, 'f');
+      if (field == FSALE_F_FACE_AREA) one(m.farea, P.nf, P.f0, 'f');
       fv.writable = false;
       return FSALE_OK;
-    case FSALE_F_COORDS: vec(m.x, N, P.nv, P.v0); return FSALE_OK;
+    case FSALE_F_COORDS: vec(m.x, N, P.nv, P.v0, 'v'); return FSALE_OK;
     default: break;
   }
   set_message("field not available for this solver");
@@ -867,14 +916,14 @@ int get_problem_field(fsale_ctx *ctx, int prob, int field, double *host_out) {
         FSALE_HIP(hipMemcpyAsync(buf.data(), fv.ptrs[k] + (size_t)kk * m.cap_c + fv.first, nc * sizeof(double),
                                  hipMemcpyDeviceToHost, ctx->stream));
         FSALE_HIP(hipStreamSynchronize(ctx->stream));
-        for (int c = 0; c < nc; ++c) host_out[((size_t)c * m.vpc + kk) * fv.ncomp + k] = buf[c];
+        for (int c = 0; c < nc; ++c) host_out[((size_t)c * m.vpc + kk) * fv.ncomp + k] = buf[fv.pos[c]];
       }
     } else {
       buf.resize(fv.n);
       FSALE_HIP(hipMemcpyAsync(buf.data(), fv.ptrs[k] + fv.first, fv.n * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream));
       FSALE_HIP(hipStreamSynchronize(ctx->stream));
-      for (int i = 0; i < fv.n; ++i) host_out[(size_t)i * fv.ncomp + k] = buf[i];
+      for (int i = 0; i < fv.n; ++i) host_out[(size_t)i * fv.ncomp + k] = buf[fv.pos[i]];
     }
   }
   return check_device_error(ctx, prob);
@@ -922,7 +971,7 @@ int get_problem_fields(fsale_ctx *ctx, int prob, int n, const int32_t *fields, d
   for (int i = 0; i < n; ++i) {
     const FieldView &fv = views[i];
     for (int k = 0; k < fv.ncomp; ++k) {
-      for (int e = 0; e < fv.n; ++e) outs[i][(size_t)e * fv.ncomp + k] = buf[off + e];
+      for (int e = 0; e < fv.n; ++e) outs[i][(size_t)e * fv.ncomp + k] = buf[off + fv.pos[e]];
       off += fv.n;
     }
   }
@@ -946,14 +995,14 @@ int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in
       const int nc = fv.n / m.vpc;
       buf.resize(nc);
       for (int kk = 0; kk < m.vpc; ++kk) {
-        for (int c = 0; c < nc; ++c) buf[c] = host_in[((size_t)c * m.vpc + kk) * fv.ncomp + k];
+        for (int c = 0; c < nc; ++c) buf[fv.pos[c]] = host_in[((size_t)c * m.vpc + kk) * fv.ncomp + k];
         FSALE_HIP(hipMemcpyAsync(fv.ptrs[k] + (size_t)kk * m.cap_c + fv.first, buf.data(), nc * sizeof(double),
                                  hipMemcpyHostToDevice, ctx->stream));
         FSALE_HIP(hipStreamSynchronize(ctx->stream));
       }
     } else {
       buf.resize(fv.n);
-      for (int i = 0; i < fv.n; ++i) buf[i] = host_in[(size_t)i * fv.ncomp + k];
+      for (int i = 0; i < fv.n; ++i) buf[fv.pos[i]] = host_in[(size_t)i * fv.ncomp + k];
       FSALE_HIP(hipMemcpyAsync(fv.ptrs[k] + fv.first, buf.data(), fv.n * sizeof(double), hipMemcpyHostToDevice,
                                ctx->stream));
       FSALE_HIP(hipStreamSynchronize(ctx->stream));

@@ -73,9 +73,9 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerField
           const int k1 = (k + 1 == FPC) ? 0 : k + 1;
           face_geom_2d(px[k], py[k], px[k1], py[k1], fx, fy, n[0], n[1], area);
         } else {
-          const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
-          for (int d = 0; d < N; ++d) n[d] = m.fnrm[d][f];
-          area = m.farea[f];
+          const size_t o = (size_t)k * m.cap_c + c;
+          for (int d = 0; d < N; ++d) n[d] = m.kf[3 + d][o];
+          area = m.kf[6][o];
         }
         const double delta_x = vol / area;
         const double dti = (a + fabs(dotN<N>(vel, n))) / delta_x;
@@ -261,7 +261,8 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
 }
 
 template <int N, int VPC>
-__global__ void __launch_bounds__(kBlock) k_eul_grad(DevMesh m, EulerFields u, Batch b) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_eul_grad(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x < tile.count) grad_cell<N, VPC>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
@@ -408,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
     }
     double vel[N], a;
     if (!update_cell<N>(u, u, c, du, delta_t / m.cvol[c], P.gamma, P.cv, vel, a))
-      flag_negative(b.s, tile.prob, c - P.cell_first, token);
+      flag_negative(b.s, tile.prob, m.corig[c], token);
   });
 }
 
@@ -453,12 +454,12 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
           else
             face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
         } else {
-          const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
+          const size_t o = (size_t)k * m.cap_c + c;
           for (int d = 0; d < N; ++d) {
-            fx[d] = m.fcen[d][f];
-            nk[k][d] = m.fnrm[d][f];
+            fx[d] = m.kf[d][o];
+            nk[k][d] = m.kf[3 + d][o];
           }
-          ak[k] = m.farea[f];
+          ak[k] = m.kf[6][o];
         }
         // the neighbour's loads do not wait on a branch (a boundary face
         // loads the cell's own data again and ignores it)
@@ -478,7 +479,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
       const double vol = m.cvol[c];
       double vel[N], a;
       if (!update_cell<N>(u, out, c, du, delta_t / vol, P.gamma, P.cv, vel, a))
-        flag_negative(b.s, tile.prob, c - P.cell_first, token);
+        flag_negative(b.s, tile.prob, m.corig[c], token);
       if (want_dt) {
 #pragma unroll
         for (int k = 0; k < FPC; ++k) {

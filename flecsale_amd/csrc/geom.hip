@@ -83,6 +83,22 @@ __global__ void __launch_bounds__(kBlock) k_face_geom_3d(DevMesh m, int f0, int 
   m.farea[f] = area;
 }
 
+// copies of the face geometry into each cell's local-face slots
+__global__ void __launch_bounds__(kBlock) k_cellface_geom_3d(DevMesh m, int c0, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = c0 + i;
+  for (int k = 0; k < m.fpc; ++k) {
+    const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
+    const size_t o = (size_t)k * m.cap_c + c;
+    for (int d = 0; d < 3; ++d) {
+      m.kf[d][o] = m.fcen[d][f];
+      m.kf[3 + d][o] = m.fnrm[d][f];
+    }
+    m.kf[6][o] = m.farea[f];
+  }
+}
+
 }  // namespace
 
 int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces, hipStream_t stream) {
@@ -103,6 +119,11 @@ int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces, hipStream_
     else
       hipLaunchKernelGGL(k_face_geom_3d, dim3(g), dim3(kBlock), 0, stream, m, P.f0, P.nf);
     FSALE_HIP(hipGetLastError());
+    if (m.kf[0] && P.nc > 0) {
+      hipLaunchKernelGGL(k_cellface_geom_3d, dim3((P.nc + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, m, P.c0,
+                         P.nc);
+      FSALE_HIP(hipGetLastError());
+    }
   }
   return FSALE_OK;
 }

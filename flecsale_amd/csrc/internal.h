@@ -59,6 +59,10 @@ struct HostTopology {
   std::vector<int32_t> vc;        // corner ids c*vpc + k, ascending
   int max_corners = 0;            // most corners around one vertex
   int max_neighbours = 0;         // most cell-vertex neighbours of one cell
+  // device storage order (a Z-order curve, for locality of the gathers):
+  // xidx[s] = the entity stored at position s, xpos its inverse.  No result
+  // depends on it: every ordered sum follows the original ids.
+  std::vector<int32_t> cidx, cpos, vidx, vpos, fidx, fpos;
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
@@ -86,6 +90,7 @@ struct DevMesh {
   // cell across local face k: (cell + 1) * 2 + (this cell is the face's
   // cells[1]); boundary faces hold 0
   int32_t *cfn = nullptr;       // [fpc][cap_c]
+  int32_t *corig = nullptr;     // [cap_c] the problem-local id of the cell stored at each position
   int32_t *vc_cnt = nullptr;    // [cap_v] corners of each vertex
   int32_t *vck = nullptr;       // [kMaxCornersPerVertex][cap_v] corner c*vpc + k (batch cell
                                 // id), ascending cell id, -1 padded
@@ -98,6 +103,10 @@ struct DevMesh {
   double *fcen[3] = {nullptr, nullptr, nullptr};
   double *fnrm[3] = {nullptr, nullptr, nullptr};
   double *farea = nullptr;
+  // 3D: the geometry of each cell's local faces again, k-major [fpc][cap_c]
+  // (centroid 0-2, normal 3-5 as stored for the face, area 6), so the fused
+  // step reads it coalesced instead of gathering 8 bytes per 128-byte line
+  double *kf[7] = {};
 };
 
 // A tile: `count` (<= kBlock) consecutive entities of problem `prob`

@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
     u.p[c] = eos_p_de(g, d, ie);
     u.T[c] = eos_T_de(P.cv, ie);
     u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
-    if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, c - P.cell_first, token);
+    if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, m.corig[c], token);
   });
 }
 

@@ -6,6 +6,7 @@
 // through the cells around its first vertex.
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -59,6 +60,97 @@ inline std::array<int32_t, 4> face_key(const HostTopology &t, int c, int lf) {
   cs(1, 3);
   cs(1, 2);
   return k;
+}
+}  // namespace
+
+namespace {
+// Spread the low 21 bits of x over every third bit (3D) or the low 31 bits
+// over every second bit (2D).
+inline uint64_t spread3(uint64_t x) {
+  x &= 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+inline uint64_t spread2(uint64_t x) {
+  x &= 0x7fffffff;
+  x = (x | x << 16) & 0x0000ffff0000ffffull;
+  x = (x | x << 8) & 0x00ff00ff00ff00ffull;
+  x = (x | x << 4) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | x << 2) & 0x3333333333333333ull;
+  x = (x | x << 1) & 0x5555555555555555ull;
+  return x;
+}
+
+// Storage order mode, FSALE_STORAGE_ORDER = identity (default) | morton |
+// blocks.  For meshes numbered with locality (structured, Exodus) the given
+// order coalesces the gathers best; the curves help meshes numbered without it.
+int storage_order_mode() {
+  static const int mode = [] {
+    const char *e = std::getenv("FSALE_STORAGE_ORDER");
+    if (!e) return 0;
+    if (!std::strcmp(e, "morton")) return 1;
+    if (!std::strcmp(e, "blocks")) return 2;
+    return 0;
+  }();
+  return mode;
+}
+
+// Order of n points: identity; Z-order (Morton) of the points; or Z-order of
+// spatial blocks of ~256 points with the original order inside a block.
+// idx[s] = the point at position s, pos its inverse; ties keep id order.
+void morton_order(int n, int dim, const std::vector<double> &pts, std::vector<int32_t> &idx,
+                  std::vector<int32_t> &pos) {
+  const int mode = storage_order_mode();
+  if (mode == 0) {
+    idx.resize(n);
+    pos.resize(n);
+    for (int i = 0; i < n; ++i) idx[i] = pos[i] = i;
+    return;
+  }
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d < dim; ++d) {
+      lo[d] = std::min(lo[d], pts[(size_t)i * dim + d]);
+      hi[d] = std::max(hi[d], pts[(size_t)i * dim + d]);
+    }
+  double levels[3];
+  for (int d = 0; d < dim; ++d) levels[d] = dim == 3 ? 2097151.0 : 2147483647.0;
+  if (mode == 2) {
+    // blocks of ~256 points: edge h with (prod extents / h^dim) = n / 256
+    double vol = 1;
+    int live = 0;
+    for (int d = 0; d < dim; ++d)
+      if (hi[d] > lo[d]) {
+        vol *= hi[d] - lo[d];
+        ++live;
+      }
+    const double nb = std::max(1.0, n / 256.0);
+    const double h = live ? std::pow(vol / nb, 1.0 / live) : 1.0;
+    for (int d = 0; d < dim; ++d) levels[d] = std::max(0.0, std::floor((hi[d] - lo[d]) / h));
+  }
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  for (int i = 0; i < n; ++i) {
+    uint64_t code = 0;
+    for (int d = 0; d < dim; ++d) {
+      const double span = hi[d] - lo[d];
+      const double u = span > 0 ? (pts[(size_t)i * dim + d] - lo[d]) / span : 0.0;
+      const double q0 = std::min(std::max(u, 0.0), 1.0) * levels[d];
+      const uint64_t q = (uint64_t)(mode == 2 ? std::floor(q0 - 1e-9 * levels[d]) : std::llround(q0));
+      code |= (dim == 3 ? spread3(q) : spread2(q)) << d;
+    }
+    key[i] = {code, i};
+  }
+  std::sort(key.begin(), key.end());
+  idx.resize(n);
+  pos.resize(n);
+  for (int s = 0; s < n; ++s) {
+    idx[s] = key[s].second;
+    pos[key[s].second] = s;
+  }
 }
 }  // namespace
 
@@ -210,6 +302,23 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
     t.nbr_off[c + 1] = (int32_t)t.nbr.size();
     t.max_neighbours = std::max(t.max_neighbours, count);
   }
+
+  // device storage order of cells, vertices and faces (vertex averages)
+  {
+    const int D = t.dim;
+    std::vector<double> pts((size_t)t.nc * D, 0.0);
+    for (int c = 0; c < t.nc; ++c)
+      for (int k = 0; k < t.vpc; ++k)
+        for (int d = 0; d < D; ++d) pts[(size_t)c * D + d] += mesh.coords[(size_t)t.cv[(size_t)c * t.vpc + k] * D + d];
+    morton_order(t.nc, D, pts, t.cidx, t.cpos);
+    pts.assign(mesh.coords, mesh.coords + (size_t)t.nv * D);
+    morton_order(t.nv, D, pts, t.vidx, t.vpos);
+    pts.assign((size_t)t.nf * D, 0.0);
+    for (int f = 0; f < t.nf; ++f)
+      for (int k = 0; k < t.vpf; ++k)
+        for (int d = 0; d < D; ++d) pts[(size_t)f * D + d] += mesh.coords[(size_t)t.fv[(size_t)f * t.vpf + k] * D + d];
+    morton_order(t.nf, D, pts, t.fidx, t.fpos);
+  }
   return FSALE_OK;
 }
 
@@ -242,7 +351,8 @@ TopologyCache &topology_cache() {
 
 size_t topology_bytes(const HostTopology &t) {
   return sizeof(int32_t) * (t.cv.size() + t.cf.size() + t.fc.size() + t.fv.size() + t.nbr_off.size() +
-                            t.nbr.size() + t.vc_off.size() + t.vc.size());
+                            t.nbr.size() + t.vc_off.size() + t.vc.size() + 2 * (t.cidx.size() + t.vidx.size() +
+                            t.fidx.size()));
 }
 
 uint64_t mesh_hash(const fsale_mesh_desc &m) {
