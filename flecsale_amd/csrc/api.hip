@@ -453,8 +453,10 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
     set_message("the Lagrangian solver supports 2D quad meshes");
     return FSALE_ERR_MESH;
   }
-  if (max_problems <= 0 || cap_c <= 0 || cap_f <= 0 || cap_v <= 0 || cap_n < 0 || cap_c > INT32_MAX / 8 ||
-      cap_f > INT32_MAX / 2 || cap_v > INT32_MAX / 2 || cap_n > INT32_MAX / 2) {
+  // entity capacities below 2^28: kernels address an 8-byte element of an
+  // entity array with a 32-bit byte offset
+  if (max_problems <= 0 || cap_c <= 0 || cap_f <= 0 || cap_v <= 0 || cap_n < 0 || cap_c > (1 << 28) ||
+      cap_f > (1 << 28) || cap_v > (1 << 28) || cap_n > INT32_MAX / 2) {
     set_message("bad batch capacity");
     return FSALE_ERR_ARG;
   }
@@ -672,10 +674,15 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     if ((rc = rows(m.nbr_cnt + c0, 1, 0, nc,
                    [&t, cidx](int, int s) { return t.nbr_off[cidx[s] + 1] - t.nbr_off[cidx[s]]; })))
       return rc;
-    if ((rc = rows(m.nbk + c0, m.nbk_deg, m.cap_c, nc, [&t, cidx, cpos, c0](int k, int s) {
+    // right-aligned: a cell with fewer neighbours than slots has its -1
+    // padding first (the gradient kernel reads the cell itself there, which
+    // leaves its min / max and sums exactly as they start)
+    const int deg = m.nbk_deg;
+    if ((rc = rows(m.nbk + c0, deg, m.cap_c, nc, [&t, cidx, cpos, c0, deg](int k, int s) {
            const int c = cidx[s];
            const int cnt = t.nbr_off[c + 1] - t.nbr_off[c];
-           return k < cnt ? cpos[t.nbr[t.nbr_off[c] + k]] + c0 : -1;
+           const int pad = cnt < deg ? deg - cnt : 0;
+           return k >= pad ? cpos[t.nbr[t.nbr_off[c] + k - pad]] + c0 : -1;
          })))
       return rc;
   }

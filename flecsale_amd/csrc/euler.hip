@@ -128,15 +128,16 @@ template <int N, int VPC>
 __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0, int first_order) {
   constexpr int NV = N + 4;
   const double tol = 1.e-9;
+  const uint32_t o0 = (uint32_t)c0 * 8u, i0 = (uint32_t)c0 * 4u;
   double u0[NV], umin[NV], umax[NV], dudx[NV][N], dxdx[N][N], x0[N];
-  for (int d = 0; d < N; ++d) x0[d] = m.ccen[d][c0];
+  for (int d = 0; d < N; ++d) x0[d] = ld(m.ccen[d], o0);
   {
     int i = 0;
-    u0[i++] = u.d[c0];
-    for (int d = 0; d < N; ++d) u0[i++] = u.v[d][c0];
-    u0[i++] = u.e[c0];
-    u0[i++] = u.p[c0];
-    u0[i++] = u.a[c0];
+    u0[i++] = ld(u.d, o0);
+    for (int d = 0; d < N; ++d) u0[i++] = ld(u.v[d], o0);
+    u0[i++] = ld(u.e, o0);
+    u0[i++] = ld(u.p, o0);
+    u0[i++] = ld(u.a, o0);
   }
   for (int j = 0; j < NV; ++j) {
     umin[j] = u0[j];
@@ -145,23 +146,49 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   }
   for (int r = 0; r < N; ++r)
     for (int q = 0; q < N; ++q) dxdx[r][q] = 0;
-  // cell_vertex_neighbors in ascending id order: the k-major head of the
-  // list (loads independent of each other), then the CSR tail if any
+  // Index loads first, all independent: the k-major neighbour head, the
+  // neighbour count and (2D) the cell's vertices, whose coordinates the
+  // limiter needs; the 3D kernel has no registers to spare for 24
+  // coordinates and loads them in the limiter.  In 2D a missing neighbour
+  // (a -1 slot; the rows are right-aligned, so these come before every real
+  // neighbour) reads the cell itself, so no load waits behind a branch, and
+  // it changes nothing: std::min(u0, u0) and std::max(u0, u0) are u0 bit for
+  // bit, and its products (du = dx = +0) add +0 to sums still at +0.
+  constexpr int NBK = (N == 2) ? kNbk2 : kNbk3;
+  const size_t row = (size_t)m.cap_c;
+  int nb[N == 2 ? NBK : 1];
+  if constexpr (N == 2) {
+#pragma unroll
+    for (int k = 0; k < NBK; ++k) nb[k] = ld(m.nbk + k * row, i0);
+  }
+  const int cnt = ld(m.nbr_cnt, i0);
+  double xv[VPC][N];
+  auto load_vertex = [&](int lv) {
+    const uint32_t ov = (uint32_t)ld(m.cv + lv * row, i0) * 8u;
+    for (int d = 0; d < N; ++d) xv[lv][d] = ld(m.x[d], ov);
+  };
+  if constexpr (N == 2) {
+#pragma unroll
+    for (int lv = 0; lv < VPC; ++lv) load_vertex(lv);
+  }
+  // cell_vertex_neighbors in ascending id order: the head, then the CSR tail
   auto neighbour = [&](int c1) {
+    const uint32_t o1 = (uint32_t)c1 * 8u;
     double u1[NV], dx[N];
     {
       int i = 0;
-      u1[i++] = u.d[c1];
-      for (int d = 0; d < N; ++d) u1[i++] = u.v[d][c1];
-      u1[i++] = u.e[c1];
-      u1[i++] = u.p[c1];
-      u1[i++] = u.a[c1];
+      u1[i++] = ld(u.d, o1);
+      for (int d = 0; d < N; ++d) u1[i++] = ld(u.v[d], o1);
+      u1[i++] = ld(u.e, o1);
+      u1[i++] = ld(u.p, o1);
+      u1[i++] = ld(u.a, o1);
     }
+    // umin = std::min(u1, umin), umax = std::max(u1, umax) (tasks.h:206-207)
     for (int j = 0; j < NV; ++j) {
-      umin[j] = fmin(u1[j], umin[j]);
-      umax[j] = fmax(u1[j], umax[j]);
+      umin[j] = smin(u1[j], umin[j]);
+      umax[j] = smax(u1[j], umax[j]);
     }
-    for (int d = 0; d < N; ++d) dx[d] = m.ccen[d][c1] - x0[d];
+    for (int d = 0; d < N; ++d) dx[d] = ld(m.ccen[d], o1) - x0[d];
     if constexpr (N == 2) {
       dxdx[0][0] += dx[0] * dx[0];
       dxdx[0][1] += dx[0] * dx[1];
@@ -179,15 +206,19 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
       for (int d = 0; d < N; ++d) dudx[j][d] += du * dx[d];
     }
   };
-  constexpr int NBK = (N == 2) ? kNbk2 : kNbk3;
+  if constexpr (N == 2) {
 #pragma unroll
-  for (int k = 0; k < NBK; ++k) {
-    const int c1 = m.nbk[(size_t)k * m.cap_c + c0];
-    if (c1 >= 0) neighbour(c1);
+    for (int k = 0; k < NBK; ++k) neighbour(nb[k] >= 0 ? nb[k] : c0);
+  } else {
+    // 26 neighbours hoisted at once would spill: one at a time behind its index
+#pragma unroll
+    for (int k = 0; k < NBK; ++k) {
+      const int c1 = ld(m.nbk + k * row, i0);
+      if (c1 >= 0) neighbour(c1);
+    }
   }
-  const int cnt = m.nbr_cnt[c0];
   if (cnt > NBK) {
-    const int q0 = m.nbr_start[c0];
+    const int q0 = ld(m.nbr_start, i0);
     for (int q = q0 + NBK; q < q0 + cnt; ++q) neighbour(m.nbr[q]);
   }
   double gu[NV][N];
@@ -227,13 +258,11 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   }
 #pragma unroll
   for (int lv = 0; lv < VPC; ++lv) {
-    const int vid = m.cv[(size_t)lv * m.cap_c + c0];
-    double xv[N];
-    for (int d = 0; d < N; ++d) xv[d] = m.x[d][vid];
+    if constexpr (N != 2) load_vertex(lv);
     double u1[NV];
     for (int j = 0; j < NV; ++j) u1[j] = u0[j];
     for (int d = 0; d < N; ++d) {
-      const double dx = xv[d] - x0[d];
+      const double dx = xv[lv][d] - x0[d];
       for (int j = 0; j < NV; ++j) u1[j] += dx * gu[j][d];
     }
     for (int j = 0; j < NV; ++j) {
@@ -252,11 +281,11 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
     for (int d = 0; d < N; ++d) gu[j][d] = first_order ? 0.0 : gu[j][d] * phi;
   }
   for (int d = 0; d < N; ++d) {
-    u.gd[d][c0] = gu[0][d];
-    for (int k = 0; k < N; ++k) u.gv[k * N + d][c0] = gu[1 + k][d];
-    u.ge[d][c0] = gu[1 + N][d];
-    u.gp[d][c0] = gu[2 + N][d];
-    u.ga[d][c0] = gu[3 + N][d];
+    st(u.gd[d], o0, gu[0][d]);
+    for (int k = 0; k < N; ++k) st(u.gv[k * N + d], o0, gu[1 + k][d]);
+    st(u.ge[d], o0, gu[1 + N][d]);
+    st(u.gp[d], o0, gu[2 + N][d]);
+    st(u.ga[d], o0, gu[3 + N][d]);
   }
 }
 
@@ -427,6 +456,11 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
     if (c == P.cell_first) b.s[tile.prob].t = b.s[tile.prob].t + delta_t;   // update_soln_time :554
     double dt_inv = 0;
     if ((int)threadIdx.x < tile.count) {
+      // the face-neighbour codes first: each face's neighbour loads then
+      // wait on one round trip, not two
+      int cnk[FPC];
+#pragma unroll
+      for (int k = 0; k < FPC; ++k) cnk[k] = m.cfn[(size_t)k * m.cap_c + c];
       CellData<N> own;
       load_cell<N>(m, u, c, own);
       double px[VPC], py[VPC];
@@ -442,7 +476,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
       double nk[FPC][N], ak[FPC];
 #pragma unroll
       for (int k = 0; k < FPC; ++k) {
-        const int cn = m.cfn[(size_t)k * m.cap_c + c];
+        const int cn = cnk[k];
         const bool second = cn & 1;     // this cell is the face's cells[1]
         const int other = (cn >> 1) - 1;
         double fx[N];

@@ -82,9 +82,10 @@ struct DevMesh {
   int32_t *nbr_start = nullptr; // [cap_c] cell-vertex neighbour list start
   int32_t *nbr_cnt = nullptr;   // [cap_c]
   int32_t *nbr = nullptr;       // [cap_n]
-  // the first nbk_deg neighbours of each cell again, k-major and -1 padded, so
-  // a gradient work-item issues all its neighbour loads at once (cells with
-  // more neighbours continue in the CSR list)
+  // the first nbk_deg neighbours of each cell again, k-major, so a gradient
+  // work-item issues all its neighbour loads at once (cells with more
+  // neighbours continue in the CSR list); a shorter list is right-aligned,
+  // -1 padding first
   int nbk_deg = 0;
   int32_t *nbk = nullptr;       // [nbk_deg][cap_c]
   // cell across local face k: (cell + 1) * 2 + (this cell is the face's

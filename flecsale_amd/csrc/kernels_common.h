@@ -16,6 +16,21 @@ namespace fsale {
 __host__ __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
 __host__ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 
+// Element access by byte offset for entity arrays (cells, faces, vertices):
+// batch capacities are held below 2^28 entities, so an offset of an 8-byte
+// element fits in 32 bits.  Formed that way, the load or store takes the
+// array's base from scalar registers plus one 32-bit offset per lane, instead
+// of a 64-bit address computed per lane per array.  For k-major tables pass
+// the row base (a scalar) and the lane's offset within the row.
+template <class T>
+__device__ __forceinline__ T ld(const T *a, uint32_t off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a) + off);
+}
+template <class T>
+__device__ __forceinline__ void st(T *a, uint32_t off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(a) + off) = v;
+}
+
 // ----------------------------------------------------------------- EOS
 // flecsale/eos/ideal_gas.h:178-226
 __host__ __device__ inline double eos_e_dp(double g, double d, double p) { return p / (d * (g - 1.0)); }
