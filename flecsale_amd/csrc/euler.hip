@@ -149,7 +149,7 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   // Index loads first, all independent: the k-major neighbour head, the
   // neighbour count and (2D) the cell's vertices, whose coordinates the
   // limiter needs; the 3D kernel has no registers to spare for 24
-  // coordinates and loads them in the limiter.  In 2D a missing neighbour
+  // coordinates and loads them in the limiter.  A missing neighbour
   // (a -1 slot; the rows are right-aligned, so these come before every real
   // neighbour) reads the cell itself, so no load waits behind a branch, and
   // it changes nothing: std::min(u0, u0) and std::max(u0, u0) are u0 bit for
@@ -210,11 +210,15 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
 #pragma unroll
     for (int k = 0; k < NBK; ++k) neighbour(nb[k] >= 0 ? nb[k] : c0);
   } else {
-    // 26 neighbours hoisted at once would spill: one at a time behind its index
+    // 28 neighbour slots hoisted at once would spill: groups of 4, each
+    // group's loads issued together
+#pragma unroll 1
+    for (int k0 = 0; k0 < NBK; k0 += 4) {
+      int g[4];
 #pragma unroll
-    for (int k = 0; k < NBK; ++k) {
-      const int c1 = ld(m.nbk + k * row, i0);
-      if (c1 >= 0) neighbour(c1);
+      for (int j = 0; j < 4; ++j) g[j] = ld(m.nbk + (k0 + j) * row, i0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) neighbour(g[j] >= 0 ? g[j] : c0);
     }
   }
   if (cnt > NBK) {

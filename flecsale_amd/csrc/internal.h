@@ -27,7 +27,7 @@ constexpr int kWaves = kBlock / 64;   // waves per block = dt partials per tile
 constexpr int kMaxCornersPerVertex = 8;
 // neighbours held k-major per cell: all cell-vertex neighbours of an interior
 // cell of a structured quad (8) or hex (26) mesh
-constexpr int kNbk2 = 8, kNbk3 = 26;
+constexpr int kNbk2 = 8, kNbk3 = 28;   // 3D: 26 hex neighbours, padded to groups of 4
 
 void set_message(const std::string &msg);
 

@@ -142,11 +142,25 @@ def summarize(fetch_db, write_db, workload):
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
-    print(f"copy kernel {ck}: fetch scale {f_scale:.4f}, write scale {w_scale:.4f}")
-    print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes (calibrated) |")
-    print("|---|---|---|---|---|")
+    w = workload
+    lines = [f"# rocprofv3 HBM traffic per launch: {w}", "", "Commands (repository root, one MI355X):", "", "```",
+             f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch_{w} -o run -- "
+             f"python3 -m flecsale_amd.profiling run --workload {w} --steps 3",
+             f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write_{w} -o run -- "
+             f"python3 -m flecsale_amd.profiling run --workload {w} --steps 3",
+             f"python3 -m flecsale_amd.profiling summarize gpurun_out/pmc_fetch_{w} gpurun_out/pmc_write_{w} "
+             f"--workload {w}", "```", "",
+             "FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what they read for the runtime's "
+             "512 MiB device-to-device copy (scale factors in the first line).", "",
+             f"copy kernel {ck}: fetch scale {f_scale:.4f}, write scale {w_scale:.4f}", "",
+             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes (calibrated) |",
+             "|---|---|---|---|---|"]
     for k, v in sorted(table.items(), key=lambda kv: -kv[1]["hbm_bytes"]):
-        print(f"| {k} | {v['dispatches']} | {v['fetch_KiB']:.0f} | {v['write_KiB']:.0f} | {v['hbm_bytes']:.4e} |")
+        lines.append(f"| {k} | {v['dispatches']} | {v['fetch_KiB']:.0f} | {v['write_KiB']:.0f} | "
+                     f"{v['hbm_bytes']:.4e} |")
+    with open(os.path.join(ROOT, "profiles", f"{workload}_traffic.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def trace(trace_dir, out_path, command):
