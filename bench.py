@@ -113,12 +113,12 @@ def serve_requests(inputs, n, steps):
             for i in range(n)]
 
 
-def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=256):
+def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=1024):
     """Serves `requests` (solver kind, problem) through fsale_server in `mode`
     (0 batched, 1 one plain context per request).  arrivals: seconds after
     the start (Poisson trace), or None to keep up to `window` requests queued
-    (saturated).  Requests are submitted when they arrive, while the previous
-    scheduling round runs."""
+    or running (saturated: more than both batches hold at once).  Requests are
+    submitted when they arrive, while the previous scheduling round runs."""
     n = len(requests)
     solver = {"euler": fa.SOLVER_EULER, "maire": fa.SOLVER_MAIRE}
     # resident capacity per batch: ~170 average requests (47K cells each)

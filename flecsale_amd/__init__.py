@@ -608,11 +608,12 @@ class Server:
         return c.value, p.value
 
     def stats(self):
-        """Host seconds per phase of the batched loop, rounds and admissions."""
-        out = np.zeros(6)
+        """Host seconds per phase of the batched loop, rounds, admissions and the
+        GPU seconds of the Eulerian and Lagrangian batches' rounds."""
+        out = np.zeros(8)
         _check(lib().fsale_server_stats(self.h, out.ctypes.data))
         return dict(admit_s=out[0], wait_s=out[1], retire_s=out[2], launch_s=out[3], rounds=int(out[4]),
-                    admissions=int(out[5]))
+                    admissions=int(out[5]), gpu_euler_s=out[6], gpu_maire_s=out[7])
 
     def times(self, rid):
         a, b, c = ct.c_double(), ct.c_double(), ct.c_double()

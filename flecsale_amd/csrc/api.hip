@@ -99,6 +99,8 @@ int read_scalars(fsale_ctx *ctx) {
   FSALE_HIP(hipMemcpyAsync(ctx->h_s, ctx->s, sizeof(DevScalars) * ctx->max_problems, hipMemcpyDeviceToHost,
                            ctx->stream));
   FSALE_HIP(hipStreamSynchronize(ctx->stream));
+  // no round is in flight: slots retired during one may be reused
+  for (auto &P : ctx->probs) P.quarantined = false;
   return FSALE_OK;
 }
 
@@ -549,7 +551,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   if (!mesh.coords || !mesh.cells) return FSALE_ERR_ARG;
   int prob = -1;
   for (int p = 0; p < ctx->max_problems; ++p)
-    if (!ctx->probs[p].live) {
+    if (!ctx->probs[p].live && !ctx->probs[p].quarantined) {
       prob = p;
       break;
     }
@@ -986,6 +988,104 @@ int get_problem_fields(fsale_ctx *ctx, int prob, int n, const int32_t *fields, d
   return FSALE_OK;
 }
 
+__global__ void k_zero_params(ProbParams *p) {
+  constexpr int words = (int)(sizeof(ProbParams) / sizeof(int));
+  static_assert(sizeof(ProbParams) % sizeof(int) == 0, "ProbParams is a whole number of words");
+  for (int i = threadIdx.x; i < words; i += blockDim.x) reinterpret_cast<int *>(p)[i] = 0;
+}
+
+int stage_field_downloads(fsale_ctx *ctx, const int *probs, int nprobs, int n, const int32_t *fields,
+                          double *const *const *outs, std::vector<PendingFields> &pend) {
+  if (n <= 0 || n > 8 || nprobs < 0) return FSALE_ERR_ARG;
+  pend.clear();
+  pend.resize(nprobs);
+  size_t total = 0;
+  for (int q = 0; q < nprobs; ++q) {
+    const int prob = probs[q];
+    if (!valid_prob(ctx, prob)) return FSALE_ERR_ARG;
+    PendingFields &pf = pend[q];
+    pf.topo = ctx->probs[prob].topo;
+    pf.n = n;
+    pf.off = total;
+    for (int i = 0; i < n; ++i) {
+      FieldView fv;
+      int rc = field_view(ctx, prob, fields[i], fv);
+      if (rc) return rc;
+      if (fv.corner || !outs[q][i]) return FSALE_ERR_ARG;
+      pf.ncomp[i] = fv.ncomp;
+      pf.count[i] = fv.n;
+      pf.pos[i] = fv.pos;
+      pf.outs[i] = outs[q][i];
+      total += (size_t)fv.ncomp * fv.n;
+    }
+  }
+  const size_t bytes = total * sizeof(double) + 64;
+  if (bytes > ctx->dl_bytes) {
+    FSALE_HIP(hipStreamSynchronize(ctx->upload));
+    if (ctx->dl) (void)hipHostFree(ctx->dl);
+    ctx->dl = nullptr;
+    ctx->dl_bytes = 0;
+    FSALE_HIP(hipHostMalloc(&ctx->dl, bytes + bytes / 2, hipHostMallocDefault));
+    ctx->dl_bytes = bytes + bytes / 2;
+  }
+  if (!ctx->ev_dl) FSALE_HIP(hipEventCreateWithFlags(&ctx->ev_dl, hipEventDisableTiming));
+  FSALE_HIP(hipEventRecord(ctx->ev_dl, ctx->stream));
+  FSALE_HIP(hipStreamWaitEvent(ctx->upload, ctx->ev_dl, 0));
+  double *buf = static_cast<double *>(ctx->dl);
+  for (int q = 0; q < nprobs; ++q) {
+    size_t off = pend[q].off;
+    for (int i = 0; i < n; ++i) {
+      FieldView fv;
+      (void)field_view(ctx, probs[q], fields[i], fv);
+      for (int k = 0; k < fv.ncomp; ++k) {
+        FSALE_HIP(hipMemcpyAsync(buf + off, fv.ptrs[k] + fv.first, fv.n * sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->upload));
+        off += fv.n;
+      }
+    }
+  }
+  return FSALE_OK;
+}
+
+int finish_field_downloads(fsale_ctx *ctx, std::vector<PendingFields> &pend) {
+  if (pend.empty()) return FSALE_OK;
+  FSALE_HIP(hipStreamSynchronize(ctx->upload));
+  const double *buf = static_cast<const double *>(ctx->dl);
+  for (PendingFields &pf : pend) {
+    size_t off = pf.off;
+    for (int i = 0; i < pf.n; ++i)
+      for (int k = 0; k < pf.ncomp[i]; ++k) {
+        double *out = pf.outs[i];
+        const int32_t *pos = pf.pos[i];
+        const int nc = pf.ncomp[i];
+        for (int e = 0; e < pf.count[i]; ++e) out[(size_t)e * nc + k] = buf[off + pos[e]];
+        off += pf.count[i];
+      }
+    pf.topo.reset();
+  }
+  pend.clear();
+  return FSALE_OK;
+}
+
+int retire_deferred(fsale_ctx *ctx, int prob) {
+  if (!ctx || ctx->single || !valid_prob(ctx, prob)) return FSALE_ERR_ARG;
+  Problem &P = ctx->probs[prob];
+  ctx->ra_c.release(P.c0, P.nc);
+  ctx->ra_f.release(P.f0, P.nf);
+  ctx->ra_v.release(P.v0, P.nv);
+  ctx->ra_n.release(P.n0, P.nn);
+  ctx->ra_ct.release(P.ct0, P.nct);
+  ctx->ra_vt.release(P.vt0, P.nvt);
+  P = Problem{};
+  P.quarantined = true;
+  ctx->tiles_dirty = true;
+  // after the round in flight (stream order, no host wait): the per-problem
+  // kernels skip a slot that owns no cells
+  hipLaunchKernelGGL(k_zero_params, dim3(1), dim3(64), 0, ctx->stream, ctx->d_params + prob);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
 int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in) {
   if (!valid_prob(ctx, prob) || !host_in) return FSALE_ERR_ARG;
   FieldView fv;
@@ -1171,11 +1271,18 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
     for (int n = 0; n < nsteps && !rc; ++n)
       rc = ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx);
     if (rc) break;
+    // the two nodal kernels are one launch at one block per tile: they are
+    // chosen together, by their summed time per step
+    double pair_ms = 0;
+    int64_t pair_nl = 0;
     for (int k = 0; k < K_COUNT; ++k) {
       double tot = 0;
       int64_t nl = 0;
       if ((rc = fsale_get_profile(ctx, k, &tot, &nl))) break;
-      if (nl > 0 && tot / nl < best[k]) {
+      if (k == K_MAIRE_NODAL || k == K_MAIRE_NODAL_BND) {
+        pair_ms += tot;
+        if (k == K_MAIRE_NODAL) pair_nl = nl;
+      } else if (nl > 0 && tot / nl < best[k]) {
         best[k] = tot / nl;
         per_step[k] = nl / nsteps;
         choice.blocks_per_cu[k] = c;
@@ -1184,6 +1291,12 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
       ctx->prof[k].launches = 0;
     }
     if (rc) break;
+    if (pair_nl > 0 && pair_ms / pair_nl < best[K_MAIRE_NODAL]) {
+      best[K_MAIRE_NODAL] = pair_ms / pair_nl;   // per nodal evaluation, both kinds
+      per_step[K_MAIRE_NODAL] = pair_nl / nsteps;
+      choice.blocks_per_cu[K_MAIRE_NODAL] = c;
+      choice.blocks_per_cu[K_MAIRE_NODAL_BND] = c;
+    }
   }
   ctx->profiling = was_profiling;
   for (int k = 0; k < K_COUNT; ++k) std::swap(saved[k], ctx->prof[k]);
@@ -1263,8 +1376,7 @@ int fsale_batch_create(int solver, int dim, int verts_per_cell, int32_t max_prob
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->upload, hipStreamNonBlocking) != hipSuccess) {
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    fsale_ctx_destroy(ctx);
     set_message("hipStreamCreate failed");
     return FSALE_ERR_HIP;
   }
@@ -1330,6 +1442,7 @@ int fsale_ctx_destroy(fsale_ctx *ctx) {
   if (ctx->dl) (void)hipHostFree(ctx->dl);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->upload) (void)hipStreamDestroy(ctx->upload);
+  if (ctx->ev_dl) (void)hipEventDestroy(ctx->ev_dl);
   delete ctx;
   return FSALE_OK;
 }
@@ -1608,7 +1721,7 @@ int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_face
   if (free_problems) {
     int n = 0;
     for (auto &P : ctx->probs)
-      if (!P.live) ++n;
+      if (!P.live && !P.quarantined) ++n;
     *free_problems = n;
   }
   return FSALE_OK;

@@ -207,6 +207,9 @@ struct ProfSlot {
 // Host record of one resident problem
 struct Problem {
   bool live = false;
+  // removed while a round that still lists it may be running (a deferred
+  // retirement): the slot is not reused until the stream has been waited for
+  bool quarantined = false;
   int c0 = 0, nc = 0, f0 = 0, nf = 0, v0 = 0, nv = 0, n0 = 0, nn = 0;   // ranges
   int ct0 = 0, nct = 0, vt0 = 0, nvt = 0;                               // tile ranges
   int nb = 0;                                                           // boundary vertices (Maire)
@@ -272,6 +275,7 @@ struct fsale_ctx {
   void *stage = nullptr;               // pinned staging buffer for uploads
   size_t stage_bytes = 0;
   void *dl = nullptr;                  // pinned buffer for multi-field downloads
+  hipEvent_t ev_dl = nullptr;          // deferred downloads: the compute stream's work so far
   size_t dl_bytes = 0;
   fsale::GraphCache graph;
   fsale::LaunchShape shape;
@@ -295,7 +299,30 @@ template <class T>
 int dev_alloc_n(fsale_ctx *ctx, T **p, size_t n) {
   return dev_alloc(ctx, reinterpret_cast<void **>(p), n * sizeof(T) + 16);
 }
-int read_scalars(fsale_ctx *ctx);      // sync + copy DevScalars to h_s
+int read_scalars(fsale_ctx *ctx);      // sync + copy DevScalars to h_s (and lift slot quarantines)
+
+// Retirement of finished problems while the next round runs (serving loop).
+// stage_field_downloads enqueues, on the upload stream and after all work
+// enqueued so far on the compute stream, the copies of `fields` of each
+// listed problem into the pinned download buffer; finish_field_downloads
+// waits for them and writes host_out (host order).  retire_deferred then
+// releases the problem's ranges without waiting for the compute stream: the
+// round in flight still lists the problem's tiles, but the problem is done,
+// so every kernel skips them; its slot stays unused until read_scalars has
+// waited for the stream.  Downloads precede, on the upload stream, every
+// later admission's upload into the released ranges.
+struct PendingFields {
+  std::shared_ptr<const HostTopology> topo;   // keeps the views' positions alive
+  int n = 0;
+  int ncomp[8] = {}, count[8] = {};
+  const int32_t *pos[8] = {};
+  double *outs[8] = {};
+  size_t off = 0;                              // doubles into the download buffer
+};
+int stage_field_downloads(fsale_ctx *ctx, const int *probs, int nprobs, int n, const int32_t *fields,
+                          double *const *const *outs, std::vector<PendingFields> &pend);
+int finish_field_downloads(fsale_ctx *ctx, std::vector<PendingFields> &pend);
+int retire_deferred(fsale_ctx *ctx, int prob);
 int check_device_error(fsale_ctx *ctx, int prob);
 int sync_tiles(fsale_ctx *ctx);        // upload the tile lists if they changed
 int upload_params(fsale_ctx *ctx, int prob);

@@ -8,6 +8,8 @@
 //   k_maire_update    apply_update (geometry + state) + update_state_from_energy
 //                     apps/maire_hydro/tasks.h:536-546, :110-133
 //   k_maire_copy      save_/restore_coordinates, save_/restore_solution :557-654
+//                     (task-level entry points; the driver step saves inside
+//                     the predictor's move and update and needs no restore)
 //
 // Loops over the batch's vertex tiles (nodal, move) and cell tiles (the
 // rest; batch_dev.h) with per-problem scalars.  Wedge facets (half edges: burton's
@@ -186,9 +188,10 @@ __device__ inline void corner_force(const MaireFields &u, int c, const double M[
 // BOUNDARY = false: the interior vertices (Cramer's rule); true: the vertices
 // on a boundary face, whose symmetry QR solve would otherwise hold the
 // registers (and the waves) of the whole interior launch.
+// one tile of interior (Cramer) or boundary (symmetry QR) vertices
 template <bool BOUNDARY>
-__global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
-  for_each_tile(BOUNDARY ? b.bt : b.vt, [&](const Tile &tile) {
+__device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
+  {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     int vt = tile.first + threadIdx.x;
@@ -270,7 +273,35 @@ __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u
       corner_force(u, cn >> 2, Mq, npq, Fq);
       finish(cn >> 2, cn & 3, Mq, npq, Fq);
     }
-  });
+  }
+}
+
+template <bool BOUNDARY>
+__global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
+  for_each_tile(BOUNDARY ? b.bt : b.vt, [&](const Tile &tile) { nodal_tile<BOUNDARY>(m, u, b, tile); });
+}
+
+// Both kinds in one launch over the one-block-per-tile lists: blocks
+// [0, nb) take the boundary tiles (nb a multiple of 8, so the interior list
+// keeps its XCD arrangement), the rest the interior tiles.  The boundary
+// vertices' solves are long serial chains in few waves; issued first, they
+// run while the interior waves fill the machine around them instead of
+// forming a launch of their own.  Held at 3 waves per SIMD for the interior
+// tiles (the boundary path then keeps part of its state in scratch).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
+  if ((int)blockIdx.x < nb) {
+    if ((int)blockIdx.x < b.bt.ndirect) {
+      const Tile t = b.bt.direct[blockIdx.x];
+      if (t.count > 0) nodal_tile<true>(m, u, b, t);
+    }
+  } else {
+    const int i = (int)blockIdx.x - nb;
+    if (i < b.vt.ndirect) {
+      const Tile t = b.vt.direct[i];
+      if (t.count > 0) nodal_tile<false>(m, u, b, t);
+    }
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, Batch b) {
@@ -340,17 +371,35 @@ __global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, cons
   }
 }
 
+// apply_update's mesh motion x = x0 + dt u (:517-534).  SAVE (the driver
+// step's predictor): save_coordinates first, x0 = x, and move from the value
+// just saved, so the step needs no separate save launch.
+template <bool SAVE>
 __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u, Batch b, double factor) {
   for_each_tile(b.vt, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const double delta_t = factor * b.s[tile.prob].dt;
-    if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
     const int vt = tile.first + threadIdx.x;
-    for (int d = 0; d < 2; ++d) m.x[d][vt] = u.x0[d][vt] + delta_t * u.un[d][vt];
+    double x0[2];
+    for (int d = 0; d < 2; ++d) {
+      if (SAVE) {
+        x0[d] = m.x[d][vt];
+        u.x0[d][vt] = x0[d];
+      } else {
+        x0[d] = u.x0[d][vt];
+      }
+    }
+    if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
+    for (int d = 0; d < 2; ++d) m.x[d][vt] = x0[d] + delta_t * u.un[d][vt];
   });
 }
 
+// MODE 0: the apply_update task (state from v, e); 1: the driver step's
+// predictor, which first saves v0 = v, e0 = e (save_solution); 2: the
+// corrector, which starts from v0, e0 (restore_solution) instead of copying
+// them back first.
+template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, Batch b, double factor,
                                                          int token) {
   for_each_tile(b.ct, [&](const Tile &tile) {
@@ -380,8 +429,22 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
     // update_state_from_flux lagrange_eqns.h:310-343
     const double M = u.M[c];
     const double inv_mass = 1 / M;
-    double vel[2] = {u.v[0][c], u.v[1][c]};
-    const double et = u.e[c] + dotN<2>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[3][c];
+    double vel[2], e0;
+    if (MODE == 2) {
+      vel[0] = u.v0[0][c];
+      vel[1] = u.v0[1][c];
+      e0 = u.e0[c];
+    } else {
+      vel[0] = u.v[0][c];
+      vel[1] = u.v[1][c];
+      e0 = u.e[c];
+    }
+    if (MODE == 1) {
+      u.v0[0][c] = vel[0];
+      u.v0[1][c] = vel[1];
+      u.e0[c] = e0;
+    }
+    const double et = e0 + dotN<2>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[3][c];
     for (int i = 0; i < 2; ++i) vel[i] += delta_t * inv_mass * u.dudt[1 + i][c];
     const double ie = et - dotN<2>(vel, vel) / 2;
     // update_volume :352-367
@@ -473,7 +536,21 @@ bool check_ctx(fsale_ctx *ctx) {
   return true;
 }
 
+// evaluate_nodal_state: one launch for both kinds of vertex when both use
+// one block per tile (timed as the nodal kernel), else interior vertices
+// (Cramer) and then the boundary vertices' symmetry solves
 int launch_nodal(fsale_ctx *ctx) {
+  Batch bi = batch_args(ctx, K_MAIRE_NODAL);
+  const Batch bb = batch_args(ctx, K_MAIRE_NODAL_BND);
+  if (bi.vt.direct && bb.bt.direct) {
+    bi.bt = bb.bt;
+    const int nb = (bb.bt.ndirect + 7) / 8 * 8;
+    ProfScope prof(ctx, K_MAIRE_NODAL);
+    hipLaunchKernelGGL(k_maire_nodal_both, dim3(nb + bi.vt.ndirect), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
+                       bi, nb);
+    FSALE_HIP(hipGetLastError());
+    return FSALE_OK;
+  }
   int rc = launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal<false>, ctx->m, ctx->ma,
                              batch_args(ctx, K_MAIRE_NODAL));
   return rc ? rc
@@ -495,11 +572,16 @@ int launch_dt(fsale_ctx *ctx) {
 }
 // token: the update launch's id for error attribution (predictor 1,
 // corrector 2, task-level apply_update 3)
-int launch_apply(fsale_ctx *ctx, double factor, int token) {
-  int rc = launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_MOVE), factor);
+// stage 0: the apply_update task; 1: the driver step's predictor (saves
+// coordinates and solution as it goes); 2: its corrector (from the saved ones)
+int launch_apply(fsale_ctx *ctx, double factor, int token, int stage) {
+  const Batch bm = batch_args(ctx, K_MAIRE_MOVE), bu = batch_args(ctx, K_MAIRE_UPDATE);
+  int rc = stage == 1 ? launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move<true>, ctx->m, ctx->ma, bm, factor)
+                      : launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move<false>, ctx->m, ctx->ma, bm, factor);
   if (rc) return rc;
-  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_UPDATE),
-                           factor, token);
+  if (stage == 1) return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<1>, ctx->m, ctx->ma, bu, factor, token);
+  if (stage == 2) return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<2>, ctx->m, ctx->ma, bu, factor, token);
+  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<0>, ctx->m, ctx->ma, bu, factor, token);
 }
 int launch_copy(fsale_ctx *ctx, int dir) {
   return launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_COPY), dir);
@@ -508,21 +590,25 @@ int launch_copy(fsale_ctx *ctx, int dir) {
 }  // namespace
 
 // One driver step (apps/maire_hydro/driver.h:330-538) of every live problem:
-// predictor half step, then the corrector from the saved state.
+// predictor half step, then the corrector from the saved state.  The
+// driver's save_coordinates / save_solution happen inside the predictor's
+// move and update (each saves what it reads before writing), and
+// restore_solution / restore_coordinates are not needed: the corrector's
+// update reads the saved v0, e0, and its move writes x = x0 + dt u (when it
+// does not move, dt <= 1e-14, the predictor did not move either and x is
+// still x0).
 int maire_step(fsale_ctx *ctx) {
   int rc;
   const int np = (int)ctx->probs.size();
   hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
   FSALE_HIP(hipGetLastError());
-  if ((rc = launch_copy(ctx, 0))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx))) return rc;
   if ((rc = launch_dt(ctx))) return rc;
-  if ((rc = launch_apply(ctx, 0.5, 1))) return rc;
+  if ((rc = launch_apply(ctx, 0.5, 1, 1))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx))) return rc;
-  if ((rc = launch_copy(ctx, 1))) return rc;
-  if ((rc = launch_apply(ctx, 1.0, 2))) return rc;
+  if ((rc = launch_apply(ctx, 1.0, 2, 2))) return rc;
   hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
@@ -621,7 +707,7 @@ int fsale_maire_apply_update(fsale_ctx *ctx, double factor) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
   int rc = sync_tiles(ctx);
-  return rc ? rc : launch_apply(ctx, factor, 3);
+  return rc ? rc : launch_apply(ctx, factor, 3, 0);
 }
 
 int fsale_maire_save(fsale_ctx *ctx) {

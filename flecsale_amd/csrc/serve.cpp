@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -16,11 +17,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/flecsale_amd.h"
+#include "internal.h"
 
-namespace fsale {
-void set_message(const std::string &msg);
-}
+using fsale::PendingFields;
+using fsale::finish_field_downloads;
+using fsale::retire_deferred;
+using fsale::stage_field_downloads;
 
 namespace {
 
@@ -61,6 +63,15 @@ struct fsale_server {
   std::map<int, int64_t> running[2];          // batch problem -> request
   int64_t next_id = 0;
   int64_t epoch = 0;            // retirements so far: capacity can only grow with them
+  // a batch retired problems in the last poll: their slots become reusable
+  // once the round that was running then has been waited for
+  bool retired_last[2] = {false, false};
+  // GPU duration of each batch's rounds (events around its launches): the
+  // last round's sets the next poll's admission budget
+  hipEvent_t ev_start[2] = {}, ev_end[2] = {};
+  bool timed[2] = {false, false};
+  double gpu_round_s = 0;       // last round (longest batch)
+  double gpu_s[2] = {0, 0};     // all rounds, per batch
   // host seconds spent per phase: admission, waiting for a round, result
   // download + retirement, launching rounds; rounds and admissions counted
   double t_admit = 0, t_wait = 0, t_retire = 0, t_launch = 0;
@@ -75,35 +86,37 @@ int fail(int rc, const std::string &msg) {
 }
 
 // copies the finished problem's fields out of its batch, then retires it
-int retire(fsale_server *S, int solver, int problem, const fsale_problem_status &st) {
+// Result arrays of a finished request and the batch fields they receive.
+int result_targets(Request &R, int solver, int32_t *ids, double **outs) {
+  R.rd.resize(R.nc);
+  R.rv.resize((size_t)R.nc * R.dim);
+  R.re.resize(R.nc);
+  R.rp.resize(R.nc);
+  ids[0] = FSALE_F_DENSITY;
+  ids[1] = FSALE_F_VELOCITY;
+  ids[2] = FSALE_F_INTERNAL_ENERGY;
+  ids[3] = FSALE_F_PRESSURE;
+  outs[0] = R.rd.data();
+  outs[1] = R.rv.data();
+  outs[2] = R.re.data();
+  outs[3] = R.rp.data();
+  if (solver != FSALE_SOLVER_MAIRE) return 4;
+  R.rx.resize((size_t)R.nv * R.dim);
+  ids[4] = FSALE_F_COORDS;
+  outs[4] = R.rx.data();
+  return 5;
+}
+
+// bookkeeping of a retired request (its results are on the host)
+void mark_retired(fsale_server *S, int solver, int problem, const fsale_problem_status &st) {
   const int64_t id = S->running[solver][problem];
   Request &R = S->reqs[id];
   R.status = st;
   R.status.problem = -1;
-  int rc = FSALE_OK;
-  if (st.error == 0) {
-    fsale_ctx *b = S->batch[solver];
-    R.rd.resize(R.nc);
-    R.rv.resize((size_t)R.nc * R.dim);
-    R.re.resize(R.nc);
-    R.rp.resize(R.nc);
-    const int32_t ids[5] = {FSALE_F_DENSITY, FSALE_F_VELOCITY, FSALE_F_INTERNAL_ENERGY, FSALE_F_PRESSURE,
-                            FSALE_F_COORDS};
-    double *outs[5] = {R.rd.data(), R.rv.data(), R.re.data(), R.rp.data(), nullptr};
-    int n = 4;
-    if (solver == FSALE_SOLVER_MAIRE) {
-      R.rx.resize((size_t)R.nv * R.dim);
-      outs[4] = R.rx.data();
-      n = 5;
-    }
-    if ((rc = fsale_batch_get_fields(b, problem, n, ids, outs))) return rc;
-  }
-  if ((rc = fsale_batch_remove(S->batch[solver], problem))) return rc;
   S->running[solver].erase(problem);
   ++S->epoch;
   R.state = Q_DONE;
   R.finished = now_s();
-  return FSALE_OK;
 }
 
 void release_inputs(Request &R) {
@@ -242,6 +255,9 @@ int fsale_server_create(const fsale_server_config *config, double frac_hbm, fsal
     if (!rc && c.dim == 2 && c.verts_per_cell == 4)
       rc = fsale_batch_create(FSALE_SOLVER_MAIRE, c.dim, c.verts_per_cell, c.max_problems, c.maire_cells,
                               c.maire_faces, c.maire_vertices, c.maire_neighbours, &S->batch[1]);
+    for (int s = 0; s < 2 && !rc; ++s)
+      if (hipEventCreate(&S->ev_start[s]) != hipSuccess || hipEventCreate(&S->ev_end[s]) != hipSuccess)
+        rc = fail(FSALE_ERR_HIP, "hipEventCreate failed");
     if (rc) {
       fsale_server_destroy(S);
       return rc;
@@ -256,6 +272,10 @@ int fsale_server_destroy(fsale_server *S) {
   if (!S) return FSALE_ERR_ARG;
   for (fsale_ctx *b : S->batch)
     if (b) fsale_ctx_destroy(b);
+  for (int s = 0; s < 2; ++s) {
+    if (S->ev_start[s]) (void)hipEventDestroy(S->ev_start[s]);
+    if (S->ev_end[s]) (void)hipEventDestroy(S->ev_end[s]);
+  }
   delete S;
   return FSALE_OK;
 }
@@ -305,37 +325,98 @@ int fsale_server_poll(fsale_server *S, int32_t *completed, int32_t *pending) {
       done = 1;
     }
   } else {
-    // the round launched by the previous poll has run meanwhile: retire what
-    // finished in it (this waits for it)
+    // the round launched by the previous poll has run meanwhile: wait for it
+    // and collect what finished in it
     std::vector<fsale_problem_status> st;
+    std::vector<fsale_problem_status> fin[2];
     for (int s = 0; s < 2; ++s) {
-      if (S->running[s].empty()) continue;
-      st.resize(S->running[s].size());
+      if (S->running[s].empty() && !S->retired_last[s]) continue;
+      st.resize(S->running[s].size() + 1);
       int32_t n = 0;
       double t0 = now_s();
       if ((rc = fsale_batch_status(S->batch[s], (int32_t)st.size(), st.data(), &n))) return rc;
-      double t1 = now_s();
-      S->t_wait += t1 - t0;
-      for (int i = 0; i < n; ++i) {
-        if (!st[i].done) continue;
-        if ((rc = retire(S, s, st[i].problem, st[i]))) return rc;
-        ++done;
-      }
-      S->t_retire += now_s() - t1;
+      S->t_wait += now_s() - t0;
+      if (S->retired_last[s]) ++S->epoch;   // their slots are free now
+      S->retired_last[s] = false;
+      for (int i = 0; i < n; ++i)
+        if (st[i].done) fin[s].push_back(st[i]);
     }
+    // their results are copied out on each batch's upload stream, behind the
+    // finished round and ahead of any admission into their ranges, while the
+    // next round already runs: finished problems are skipped by its kernels
+    const double tr = now_s();
+    std::vector<PendingFields> pend[2];
+    for (int s = 0; s < 2; ++s) {
+      std::vector<int> probs;
+      std::vector<std::array<double *, 5>> outs;
+      std::vector<double *const *> outp;
+      int32_t ids[5];
+      int nf = 0;
+      for (const auto &f : fin[s]) {
+        if (f.error != 0) continue;
+        Request &R = S->reqs[S->running[s][f.problem]];
+        std::array<double *, 5> o{};
+        nf = result_targets(R, s, ids, o.data());
+        probs.push_back(f.problem);
+        outs.push_back(o);
+      }
+      for (auto &o : outs) outp.push_back(o.data());
+      if (!probs.empty() &&
+          (rc = stage_field_downloads(S->batch[s], probs.data(), (int)probs.size(), nf, ids, outp.data(), pend[s])))
+        return rc;
+    }
+    S->t_retire += now_s() - tr;
+    // GPU time of the round just finished
+    double round_s = 0;
+    for (int s = 0; s < 2; ++s) {
+      if (!S->timed[s]) continue;
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, S->ev_start[s], S->ev_end[s]) == hipSuccess) {
+        S->gpu_s[s] += ms * 1e-3;
+        round_s = std::max(round_s, ms * 1e-3);
+      }
+      S->timed[s] = false;
+    }
+    if (round_s > 0) S->gpu_round_s = round_s;
     // launch the next round of the resident problems: both batches advance
     // concurrently, each on its own stream
     const double tl = now_s();
-    for (int s = 0; s < 2; ++s)
-      if (!S->running[s].empty() && (rc = fsale_batch_step(S->batch[s], S->cfg.chunk_steps, 1))) return rc;
+    bool launched = false;
+    for (int s = 0; s < 2; ++s) {
+      if (S->running[s].size() <= fin[s].size()) continue;
+      hipStream_t stream = S->batch[s]->stream;
+      FSALE_HIP(hipEventRecord(S->ev_start[s], stream));
+      if ((rc = fsale_batch_step(S->batch[s], S->cfg.chunk_steps, 1))) return rc;
+      FSALE_HIP(hipEventRecord(S->ev_end[s], stream));
+      S->timed[s] = true;
+      launched = true;
+    }
     S->t_launch += now_s() - tl;
-    if (!S->running[0].empty() || !S->running[1].empty()) ++S->rounds;
+    if (S->running[0].size() > fin[0].size() || S->running[1].size() > fin[1].size()) ++S->rounds;
+    // retire: results to the host, ranges released for the admissions below
+    const double tr2 = now_s();
+    for (int s = 0; s < 2; ++s) {
+      if ((rc = finish_field_downloads(S->batch[s], pend[s]))) return rc;
+      for (const auto &f : fin[s]) {
+        if ((rc = retire_deferred(S->batch[s], f.problem))) return rc;
+        mark_retired(S, s, f.problem, f);
+        S->retired_last[s] = true;
+        ++done;
+      }
+    }
+    S->t_retire += now_s() - tr2;
     // admission while the round runs: arrival order; a request that does not
     // fit blocks, but up to `lookahead` later requests may still be tried
-    // behind it.  Admitted requests join at the next round.
+    // behind it.  Admitted requests join at the next round.  The host time
+    // spent admitting is held to ~70% of the last round's GPU time (20 ms
+    // while nothing runs), so the next round is launched as soon as the GPU
+    // is done; problems admitted in one burst would also finish in one and
+    // leave the device idle while their successors are admitted.
     const double ta = now_s();
+    const double budget = (launched && S->gpu_round_s > 0) ? 0.7 * S->gpu_round_s - (ta - tl) : 0.02;
     int blocked = 0;
     for (auto it = S->queue.begin(); it != S->queue.end();) {
+      if (now_s() - ta > budget) break;
       Request &Rq = S->reqs[*it];
       if (Rq.blocked_epoch == S->epoch) {   // nothing was freed since it last failed
         if (++blocked > S->cfg.lookahead) break;
@@ -380,6 +461,8 @@ int fsale_server_stats(fsale_server *S, double *stats) {
   stats[3] = S->t_launch;
   stats[4] = (double)S->rounds;
   stats[5] = (double)S->admissions;
+  stats[6] = S->gpu_s[0];
+  stats[7] = S->gpu_s[1];
   return FSALE_OK;
 }
 

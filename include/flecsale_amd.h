@@ -336,9 +336,10 @@ int fsale_server_submit(fsale_server *server, const fsale_request *request, doub
  * requests retired, *pending the number queued or running afterwards.
  * Mode 1 serves the oldest queued request start to finish instead. */
 int fsale_server_poll(fsale_server *server, int32_t *completed, int32_t *pending);
-/* Host-side accounting of a batched server: stats[0..5] = seconds spent
+/* Accounting of a batched server: stats[0..7] = host seconds spent
  * admitting, waiting for rounds, retiring (result download + removal) and
- * launching rounds; rounds run; requests admitted. */
+ * launching rounds; rounds run; requests admitted; GPU seconds of the
+ * Eulerian and of the Lagrangian batch's rounds (HIP events). */
 int fsale_server_stats(fsale_server *server, double *stats);
 /* Timing of a finished request: arrival, admission and completion times. */
 int fsale_server_times(fsale_server *server, int64_t id, double *arrival, double *admitted,
