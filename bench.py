@@ -335,7 +335,7 @@ def main():
         traffic = load_traffic(args.workload)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1), "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": round(d["GBps"] / PEAK_HBM_GBS, 4),
-                    "traffic": (traffic or {}).get(dom)}
+                    "traffic": (traffic or {}).get(dom) or (traffic or {}).get(dom + "_both")}
         log("[bench] per-kernel: " + json.dumps({k: {kk: round(vv, 2) for kk, vv in v.items()}
                                                   for k, v in kernels.items()}))
 

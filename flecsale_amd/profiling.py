@@ -163,17 +163,49 @@ def summarize(fetch_db, write_db, workload):
     print("\n".join(lines))
 
 
-def trace(trace_dir, out_path, command):
-    """Markdown summary of a rocprofv3 --kernel-trace --stats run (its
-    kernel_stats.csv): calls, average and total time per kernel."""
+def _window_dispatches(trace_dir, anchor, last):
+    """Dispatches of a rocprofv3 kernel_trace.csv inside the time span of the
+    last `last` dispatches of the kernel whose name starts with `anchor`, as
+    rows shaped like kernel_stats.csv (one per kernel)."""
     import csv
-    hits = [os.path.join(dp, f) for dp, _, fs in os.walk(trace_dir) for f in fs if f.endswith("kernel_stats.csv")]
+    hits = [os.path.join(dp, f) for dp, _, fs in os.walk(trace_dir) for f in fs if f.endswith("kernel_trace.csv")]
     if not hits:
-        raise SystemExit(f"no kernel_stats.csv under {trace_dir}")
-    rows = list(csv.DictReader(open(hits[0], newline="")))
+        raise SystemExit(f"no kernel_trace.csv under {trace_dir}")
+    ev = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+          for r in csv.DictReader(open(hits[0], newline=""))]
+    ev.sort()
+    mine = [e for e in ev if _short(e[2]).startswith(anchor)]
+    if not mine:
+        raise SystemExit(f"no dispatch of {anchor}")
+    t0, t1 = mine[-last:][0][0], mine[-1][1]
+    per = {}
+    for b, e, name in ev:
+        if t0 <= b <= t1:
+            per.setdefault(name, []).append(e - b)
+    return [{"Name": n, "Calls": len(d), "AverageNs": sum(d) / len(d), "MinNs": min(d), "MaxNs": max(d),
+             "TotalDurationNs": sum(d)} for n, d in per.items()]
+
+
+def trace(trace_dir, out_path, command, last=None, anchor=None):
+    """Markdown summary of a rocprofv3 --kernel-trace --stats run: calls,
+    average and total time per kernel, over the whole run (kernel_stats.csv)
+    or (last, anchor) over the span of the anchor kernel's last `last`
+    dispatches (kernel_trace.csv): the benchmark's own timed replay, after its
+    launch-shape autotuning."""
+    import csv
+    if last:
+        rows = _window_dispatches(trace_dir, anchor, last)
+    else:
+        hits = [os.path.join(dp, f) for dp, _, fs in os.walk(trace_dir) for f in fs if f.endswith("kernel_stats.csv")]
+        if not hits:
+            raise SystemExit(f"no kernel_stats.csv under {trace_dir}")
+        rows = list(csv.DictReader(open(hits[0], newline="")))
     total = sum(float(r["TotalDurationNs"]) for r in rows)
+    scope = ([f"Dispatches within the span of the last {last} `{anchor}` dispatches: the benchmark's profiled "
+              "replay of its timed steps, at the launch shapes its autotuner chose (the earlier dispatches "
+              "include the autotuner's candidates).", ""] if last else [])
     lines = [f"# rocprofv3 kernel trace: {os.path.basename(out_path).rsplit('.', 1)[0]}", "",
-             "Command (from the repository root on one MI355X):", "", "```", command, "```", "",
+             "Command (from the repository root on one MI355X):", "", "```", command, "```", ""] + scope + [
              "| kernel | calls | avg us | min us | max us | total ms | % |", "|---|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
@@ -191,6 +223,9 @@ def main():
     t.add_argument("trace_dir")
     t.add_argument("out")
     t.add_argument("--command", default="")
+    t.add_argument("--last", type=int, default=None,
+                   help="summarise the span of the anchor kernel's last N dispatches")
+    t.add_argument("--anchor", default=None, help="kernel name prefix, e.g. k_eul_fluxupd")
     r = sub.add_parser("run")
     r.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d"])
     r.add_argument("--steps", type=int, default=3)
@@ -203,7 +238,7 @@ def main():
     if a.cmd == "run":
         run(a.workload, a.steps, a.size)
     elif a.cmd == "trace":
-        trace(a.trace_dir, a.out, a.command)
+        trace(a.trace_dir, a.out, a.command, a.last, a.anchor)
     else:
         summarize(a.fetch_db, a.write_db, a.workload)
 
