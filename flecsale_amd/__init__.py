@@ -39,8 +39,13 @@ KERNEL_NAMES = ["eul_dt", "eul_grad", "eul_flux", "eul_update", "maire_nodal", "
 K_EUL_FLUXUPD = 10
 K_MAIRE_NODAL_BND = 11
 
+ERR_ARG = 1
+ERR_MESH = 2
+ERR_HIP = 3
+ERR_OOM = 4
 ERR_NEG_STATE = 5
 ERR_DT = 6
+ERR_STATE = 7
 ERR_NO_DEVICE = 8
 
 # every symbol include/flecsale_amd.h declares
@@ -60,6 +65,7 @@ EXPORTS = [
     "fsale_launch_shape", "fsale_get_launch_shape", "fsale_autotune",
     "fsale_server_create", "fsale_server_destroy", "fsale_server_clock", "fsale_server_submit",
     "fsale_server_poll", "fsale_server_times", "fsale_server_result", "fsale_server_stats",
+    "fsale_exodus_info", "fsale_exodus_read", "fsale_dump",
 ]
 
 
@@ -124,6 +130,9 @@ def lib():
         L.fsale_status_string.argtypes = [ct.c_int]
         L.fsale_last_message.restype = ct.c_char_p
         L.fsale_qr_solve.argtypes = [i32, vp, vp, vp]
+        L.fsale_exodus_info.argtypes = [ct.c_char_p, vp, vp, vp, vp]
+        L.fsale_exodus_read.argtypes = [ct.c_char_p, vp, vp]
+        L.fsale_dump.argtypes = [vp, ct.c_char_p]
         L.fsale_ctx_create.argtypes = [ct.POINTER(MeshDesc), ct.c_int, ct.POINTER(vp)]
         L.fsale_ctx_destroy.argtypes = [vp]
         L.fsale_sync.argtypes = [vp]
@@ -227,6 +236,18 @@ def connectivity(coords, cells):
     return dict(face_cells=fc, cell_faces=cf, nbr_off=off, nbr=nbr[:nn])
 
 
+def read_exodus(path):
+    """Exodus-II mesh file (fsale_exodus_read): (coords [nv, dim] float64,
+    cells [nc, vpc] int32, 0-based, file order)."""
+    b = os.fsencode(path)
+    dim, nc, nv, vpc = ct.c_int32(), ct.c_int32(), ct.c_int32(), ct.c_int32()
+    _check(lib().fsale_exodus_info(b, ct.byref(dim), ct.byref(nc), ct.byref(nv), ct.byref(vpc)))
+    coords = np.zeros((nv.value, dim.value))
+    cells = np.zeros((nc.value, vpc.value), np.int32)
+    _check(lib().fsale_exodus_read(b, coords.ctypes.data, cells.ctypes.data))
+    return coords, cells
+
+
 class _Context:
     solver = None
 
@@ -257,6 +278,10 @@ class _Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def dump(self, path):
+        """The dump task's regression text file (fsale_dump)."""
+        _check(lib().fsale_dump(self.h, os.fsencode(path)))
 
     _shapes = None
 

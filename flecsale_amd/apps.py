@@ -1,0 +1,64 @@
+"""The reference's regression executables as drivers of this library: each
+reads an Exodus mesh (-m), sets the app's compiled-in problem at the cell
+centroids, runs the app's step loop on the GPU and writes the dump the
+reference's regression test compares with its .std golden.
+
+    python -m flecsale_amd.apps hydro_2d -m square_32x32.g        # shock_box_2d-solution_rank000000.txt
+    python -m flecsale_amd.apps hydro_3d -m sedov_20x20x20.g      # shock_box_3d-solution_rank000000.txt
+    python -m flecsale_amd.apps maire_hydro_2d -m sedov_32x32.g   # sedov_2d-solution_rank000000.txt
+    python -m flecsale_amd.numdiff -a 1e-14 <golden>.std shock_box_2d-solution_rank000000.txt
+
+hydro_2d / hydro_3d: apps/hydro/driver.h:237-450 with apps/hydro/{2d,3d}/
+inputs.cc; maire_hydro_2d: apps/maire_hydro/driver.h:330-566 with
+apps/maire_hydro/2d/inputs_regression.cc (the maire_hydro_2d_regression
+target).  The dump name is the app's prefix + "-solution" with the rank
+(apps/hydro/driver.h:441-443, apps/hydro/tasks.h:789-794; one rank here).
+"""
+import argparse
+import os
+import sys
+
+import flecsale_amd as fa
+from flecsale_amd import inputs
+
+APPS = {
+    "hydro_2d": ("shock_box_2d", fa.SOLVER_EULER, inputs.shock_box_2d),
+    "hydro_3d": ("shock_box_3d", fa.SOLVER_EULER, inputs.shock_box_3d),
+    "maire_hydro_2d": ("sedov_2d", fa.SOLVER_MAIRE, inputs.sedov_2d),
+}
+
+
+def run(app, mesh_path, out_dir="."):
+    """Runs `app` on the mesh file; returns the dump's path."""
+    prefix, solver, problem = APPS[app]
+    coords, cells = fa.read_exodus(mesh_path)
+    ctx_type = fa.EulerContext if solver == fa.SOLVER_EULER else fa.MaireContext
+    with ctx_type(coords, cells) as ctx:
+        # initial conditions at the library's cell centroids, as the reference
+        # evaluates them at burton's (apps/hydro/tasks.h:46-68)
+        prob = problem(coords, cells, ctx.get(fa.F_CELL_CENTROID))
+        if solver == fa.SOLVER_EULER:
+            ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+            ctx.advance(prob["cfl"], prob["final_time"], prob["max_steps"], flux_kind=prob["flux"])
+        else:
+            ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
+            ctx.set_time(0.0, prob["dt0"])
+            ctx.advance(prob["cfl"], prob["final_time"], prob["max_steps"])
+        path = os.path.join(out_dir, f"{prefix}-solution_rank{0:06d}.txt")
+        ctx.dump(path)
+    return path
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="run one of the reference's hydro apps on the GPU")
+    ap.add_argument("app", choices=sorted(APPS))
+    ap.add_argument("-m", "--mesh", required=True, help="Exodus-II mesh file")
+    ap.add_argument("-o", "--out-dir", default=".")
+    a = ap.parse_args(argv)
+    path = run(a.app, a.mesh, a.out_dir)
+    print(f"Dumping solution to: {path}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

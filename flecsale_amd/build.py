@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libflecsale_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FSALE_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "geom.hip", "euler.hip", "maire.hip", "mesh_host.cpp", "serve.cpp"]
+SOURCES = ["api.hip", "geom.hip", "euler.hip", "maire.hip", "mesh_host.cpp", "serve.cpp", "exodus.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -55,7 +55,7 @@ def build(verbose=False, jobs=None):
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-ldl"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

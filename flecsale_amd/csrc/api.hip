@@ -181,8 +181,11 @@ int sync_tiles(fsale_ctx *ctx) {
   for (int p = 0; p < (int)ctx->probs.size(); ++p) {
     const Problem &P = ctx->probs[p];
     if (!P.live) continue;
-    for (int i = 0; i < P.nct; ++i)
-      ctx->h_ctiles.push_back(Tile{p, P.c0 + i * kBlock, std::min(kBlock, P.nc - i * kBlock), P.ct0 + i});
+    const std::vector<int32_t> *ord = P.topo && !P.topo->tile_order.empty() ? &P.topo->tile_order : nullptr;
+    for (int i = 0; i < P.nct; ++i) {
+      const int j = ord ? (*ord)[i] : i;
+      ctx->h_ctiles.push_back(Tile{p, P.c0 + j * kBlock, std::min(kBlock, P.nc - j * kBlock), P.ct0 + j});
+    }
     for (int i = 0; i < P.nvt; ++i)
       ctx->h_vtiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nv - i * kBlock), P.vt0 + i});
     for (int i = 0; i * kBlock < P.nb; ++i)
@@ -1444,6 +1447,54 @@ int fsale_ctx_destroy(fsale_ctx *ctx) {
   if (ctx->upload) (void)hipStreamDestroy(ctx->upload);
   if (ctx->ev_dl) (void)hipEventDestroy(ctx->ev_dl);
   delete ctx;
+  return FSALE_OK;
+}
+
+int fsale_dump(fsale_ctx *ctx, const char *path) {
+  if (!ctx || !ctx->single || ctx->probs.empty() || !ctx->probs[0].live || !path) return FSALE_ERR_ARG;
+  const Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  const int N = ctx->m.dim, nc = P.nc, nv = P.nv;
+  std::vector<double> cen((size_t)nc * N), d(nc), e(nc), p(nc), v((size_t)nc * N), x((size_t)nv * N);
+  const int32_t ids[6] = {FSALE_F_CELL_CENTROID, FSALE_F_DENSITY, FSALE_F_INTERNAL_ENERGY, FSALE_F_PRESSURE,
+                          FSALE_F_VELOCITY, FSALE_F_COORDS};
+  double *outs[6] = {cen.data(), d.data(), e.data(), p.data(), v.data(), x.data()};
+  int rc = get_problem_fields(ctx, 0, 6, ids, outs);
+  if (rc) return rc;
+  if ((rc = read_scalars(ctx))) return rc;
+  FILE *f = std::fopen(path, "w");
+  if (!f) {
+    set_message(std::string("cannot write ") + path);
+    return FSALE_ERR_ARG;
+  }
+  std::fprintf(f, "# Solution time: %.14e\n# Number iterations: %lld\n", ctx->h_s[0].t,
+               (long long)ctx->h_s[0].steps);
+  std::fprintf(f, "# BEGIN CELLS\n# Total number: %d\n# local_id global_id ", nc);
+  for (int k = 0; k < N; ++k) std::fprintf(f, "centroid(%d) ", k);
+  std::fprintf(f, "density internal_energy pressure ");
+  for (int k = 0; k < N; ++k) std::fprintf(f, "velocity(%d) ", k);
+  std::fprintf(f, "\n");
+  for (int c = 0; c < nc; ++c) {
+    std::fprintf(f, "%d %d ", c, c);
+    for (int k = 0; k < N; ++k) std::fprintf(f, "%.14e ", cen[(size_t)c * N + k]);
+    std::fprintf(f, "%.14e %.14e %.14e ", d[c], e[c], p[c]);
+    for (int k = 0; k < N; ++k) std::fprintf(f, "%.14e ", v[(size_t)c * N + k]);
+    std::fprintf(f, "\n");
+  }
+  std::fprintf(f, "# END CELLS\n# BEGIN VERTICES\n# Total number: %d\n# local_id global_id ", nv);
+  for (int k = 0; k < N; ++k) std::fprintf(f, "coordinate(%d) ", k);
+  std::fprintf(f, "\n");
+  for (int i = 0; i < nv; ++i) {
+    std::fprintf(f, "%d %d ", i, i);
+    for (int k = 0; k < N; ++k) std::fprintf(f, "%.14e ", x[(size_t)i * N + k]);
+    std::fprintf(f, "\n");
+  }
+  std::fprintf(f, "# END VERTICES\n");
+  const bool ok = std::fclose(f) == 0;
+  if (!ok) {
+    set_message(std::string("cannot write ") + path);
+    return FSALE_ERR_ARG;
+  }
   return FSALE_OK;
 }
 

@@ -63,6 +63,13 @@ struct HostTopology {
   // xidx[s] = the entity stored at position s, xpos its inverse.  No result
   // depends on it: every ordered sum follows the original ids.
   std::vector<int32_t> cidx, cpos, vidx, vpos, fidx, fpos;
+  // 3D: the order in which the cell tiles (kBlock consecutive storage
+  // positions) are dealt to blocks.  The list is cut into 8 contiguous parts,
+  // one per XCD; each part is a slab along the second-longest axis, swept
+  // along the longest, so an XCD's blocks in flight cover a compact region
+  // whose neighbours stay in its L2 (a whole plane of a 128^3 mesh would
+  // not).  Empty: storage order.
+  std::vector<int32_t> tile_order;
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);

@@ -154,6 +154,44 @@ void morton_order(int n, int dim, const std::vector<double> &pts, std::vector<in
 }
 }  // namespace
 
+// Pencil order of a 3D mesh's cell tiles (HostTopology::tile_order) from the
+// tiles' centroids; cellsum holds each cell's vertex-coordinate sum.
+void tile_pencils(HostTopology &t, const std::vector<double> &cellsum) {
+  const int T = (t.nc + kBlock - 1) / kBlock;
+  t.tile_order.clear();
+  if (T < 64) return;
+  std::vector<double> tc((size_t)T * 3, 0.0);
+  for (int s = 0; s < t.nc; ++s) {
+    const int c = t.cidx.empty() ? s : t.cidx[s];
+    for (int d = 0; d < 3; ++d) tc[(size_t)(s / kBlock) * 3 + d] += cellsum[(size_t)c * 3 + d];
+  }
+  for (int j = 0; j < T; ++j) {
+    const double n = (double)std::min(kBlock, t.nc - j * kBlock);
+    for (int d = 0; d < 3; ++d) tc[(size_t)j * 3 + d] /= n;
+  }
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) lo[d] = hi[d] = tc[d];
+  for (int j = 1; j < T; ++j)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], tc[(size_t)j * 3 + d]);
+      hi[d] = std::max(hi[d], tc[(size_t)j * 3 + d]);
+    }
+  int ax[3] = {0, 1, 2};
+  std::sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+  const int sweep = ax[0], slab = ax[1];
+  std::vector<int32_t> &ord = t.tile_order;
+  ord.resize(T);
+  for (int j = 0; j < T; ++j) ord[j] = j;
+  auto key = [&](int j, int d) { return tc[(size_t)j * 3 + d]; };
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key(a, slab) < key(b, slab); });
+  for (int k = 0; k < 8; ++k) {
+    const auto b = ord.begin() + (size_t)T * k / 8, e = ord.begin() + (size_t)T * (k + 1) / 8;
+    std::stable_sort(b, e, [&](int x, int y) {
+      return key(x, sweep) < key(y, sweep) || (key(x, sweep) == key(y, sweep) && key(x, slab) < key(y, slab));
+    });
+  }
+}
+
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err) {
   if (!mesh.coords || !mesh.cells || mesh.num_cells <= 0 || mesh.num_vertices <= 0) {
     err = "empty mesh";
@@ -311,6 +349,7 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
       for (int k = 0; k < t.vpc; ++k)
         for (int d = 0; d < D; ++d) pts[(size_t)c * D + d] += mesh.coords[(size_t)t.cv[(size_t)c * t.vpc + k] * D + d];
     morton_order(t.nc, D, pts, t.cidx, t.cpos);
+    if (D == 3) tile_pencils(t, pts);
     pts.assign(mesh.coords, mesh.coords + (size_t)t.nv * D);
     morton_order(t.nv, D, pts, t.vidx, t.vpos);
     pts.assign((size_t)t.nf * D, 0.0);
@@ -351,7 +390,8 @@ TopologyCache &topology_cache() {
 
 size_t topology_bytes(const HostTopology &t) {
   return sizeof(int32_t) * (t.cv.size() + t.cf.size() + t.fc.size() + t.fv.size() + t.nbr_off.size() +
-                            t.nbr.size() + t.vc_off.size() + t.vc.size() + 2 * (t.cidx.size() + t.vidx.size() +
+                            t.nbr.size() + t.vc_off.size() + t.vc.size() + t.tile_order.size() +
+                            2 * (t.cidx.size() + t.vidx.size() +
                             t.fidx.size()));
 }
 

@@ -129,6 +129,27 @@ const char *fsale_status_string(int status);
 const char *fsale_last_message(void);
 /* first cell that failed (negative density/energy), or -1 */
 int fsale_error_cell(fsale_ctx *ctx, int64_t *cell);
+
+/* Exodus-II mesh files: what the apps' -m option reads (apps/hydro/2d/
+ * CMakeLists.txt:38-48 runs hydro_2d -m data/meshes/square_32x32.g; the
+ * reader itself is flecsi-sp's, not vendored).  Vertex coordinates and the
+ * element blocks' connectivity in file order (blocks concatenated, indices
+ * made 0-based); quad4 in 2D, hex8 in 3D.  netCDF classic encodings are read
+ * directly; netCDF-4 (HDF5) files through an HDF5 library loaded at run time
+ * (FSALE_HDF5_LIB or libhdf5.so).  fsale_exodus_info gives the sizes of the
+ * arrays fsale_exodus_read fills: coords[num_vertices*dim],
+ * cells[num_cells*verts_per_cell]. */
+int fsale_exodus_info(const char *path, int32_t *dim, int32_t *num_cells, int32_t *num_vertices,
+                      int32_t *verts_per_cell);
+int fsale_exodus_read(const char *path, double *coords, int32_t *cells);
+
+/* The dump task (apps/hydro/tasks.h:771-849; apps/maire_hydro/tasks.h:728):
+ * the regression text file of a plain context, one rank: solution time and
+ * iteration count, then per cell its local and global id, centroid,
+ * density, internal energy, pressure and velocity, then per vertex its ids
+ * and coordinates, 15 significant digits (scientific, precision 14), as the
+ * reference's .std goldens. */
+int fsale_dump(fsale_ctx *ctx, const char *path);
 /* Per-kernel timing (the reference's only timer is the wall-clock print
  * around the step loop, apps/hydro/driver.h:311,419).  While enabled, the
  * step loops run without graphs and each launch is bracketed by HIP events on

@@ -9,6 +9,10 @@ Outputs (all plain numpy .npz, loaded with allow_pickle=False):
   <case>.npz          the reference's regression golden (.std text dump) as
                       arrays: time, iterations, cell_centroid, density,
                       internal_energy, pressure, velocity, vertex_coords.
+  exodus/<file>       the Exodus-II mesh files themselves (data/meshes), for
+                      the library's own reader and the regression drivers.
+  std/<case>.std.gz   the one-rank .std goldens as text (gzip), for the
+                      numdiff comparison of the drivers' dumps.
 
 Mesh sources: data/meshes/sedov_32x32.g and sedov_20x20x20.g are
 netCDF-classic (64-bit offset) read with scipy.io.netcdf_file;
@@ -96,6 +100,20 @@ def main():
         g = read_golden(path, dim)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **g)
         print(name, g["time"], g["iterations"], g["density"].shape)
+
+
+    # the data files themselves: Exodus meshes and text goldens
+    import gzip
+    import shutil
+    os.makedirs(os.path.join(HERE, "exodus"), exist_ok=True)
+    os.makedirs(os.path.join(HERE, "std"), exist_ok=True)
+    for name in ["square_32x32.g", "sedov_32x32.g", "sedov_20x20x20.g", "cube_3k_tet.g", "uniform_32.exo"]:
+        shutil.copyfile(f"{REF}/data/meshes/{name}", os.path.join(HERE, "exodus", name))
+    for name in ["shock_box_2d_1procs", "shock_box_3d_1procs", "sedov_2d_1procs"]:
+        with open(goldens[name][0], "rb") as src, gzip.GzipFile(os.path.join(HERE, "std", name + ".std.gz"), "wb",
+                                                                mtime=0) as dst:
+            shutil.copyfileobj(src, dst)
+        print("copied", name)
 
 
 if __name__ == "__main__":

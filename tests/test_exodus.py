@@ -1,0 +1,86 @@
+"""Exodus-II reader (fsale_exodus_read) and the numdiff comparison, on the
+CPU.  The reader must reproduce, bit for bit, the coordinates and
+connectivity that independent readers give for the reference's meshes
+(tests/golden/meshes/*.npz: scipy's netCDF reader for the classic files,
+h5dump's raw export for the HDF5 one), and refuse the meshes the hydro paths
+cannot run (tetrahedra, 1-D n-sided elements)."""
+import gzip
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import flecsale_amd as fa
+from flecsale_amd import numdiff
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXO = os.path.join(HERE, "golden", "exodus")
+
+
+@pytest.mark.parametrize("name", ["sedov_32x32", "sedov_20x20x20", "square_32x32"])
+def test_reader_matches_independent_readers(name):
+    try:
+        coords, cells = fa.read_exodus(os.path.join(EXO, name + ".g"))
+    except fa.FsaleError as e:
+        if "HDF5 library" in str(e):
+            pytest.skip("no HDF5 library to read the netCDF-4 file with")
+        raise
+    ref = np.load(os.path.join(HERE, "golden", "meshes", name + ".npz"))
+    assert coords.dtype == np.float64 and cells.dtype == np.int32
+    assert np.array_equal(coords, ref["coords"])
+    assert np.array_equal(cells, ref["cells"])
+
+
+@pytest.mark.parametrize("name,why", [("cube_3k_tet.g", "4 nodes in 3D"), ("uniform_32.exo", "dimension 1"),
+                                      ("no_such_mesh.g", "cannot open")])
+def test_reader_refuses(name, why):
+    with pytest.raises(fa.FsaleError) as e:
+        fa.read_exodus(os.path.join(EXO, name))
+    assert e.value.status == fa.ERR_MESH
+    assert why in str(e.value)
+
+
+def _golden(tmp_path):
+    path = tmp_path / "shock_box_2d.std"
+    with gzip.open(os.path.join(HERE, "golden", "std", "shock_box_2d_1procs.std.gz"), "rb") as src, \
+            open(path, "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    return path
+
+
+def _edit(src, dst, line_no, col, new):
+    lines = open(src).read().splitlines(True)
+    parts = lines[line_no - 1].split(" ")
+    parts[col] = new
+    lines[line_no - 1] = " ".join(parts)
+    open(dst, "w").writelines(lines)
+
+
+def test_numdiff_rules(tmp_path):
+    g = _golden(tmp_path)
+    assert numdiff.compare(g, g) == []
+    # density of cell 0 (line 6, column 4) moved by 5e-14: fails at 1e-14
+    # absolute, passes at 1e-13; relative 1e-13 of a value near 1 passes
+    lines = open(g).read().splitlines()
+    d0 = float(lines[5].split()[4])
+    moved = tmp_path / "moved.txt"
+    _edit(g, moved, 6, 4, "%.14e" % (d0 + 5e-14))
+    assert numdiff.compare(g, moved, abs_tol=1e-14)
+    assert numdiff.compare(g, moved, abs_tol=1e-13) == []
+    assert numdiff.compare(g, moved, rel_tol=1e-13) == []
+    assert numdiff.compare(g, moved, rel_tol=1e-15)
+    # the default tolerance is absolute 1e-15
+    assert numdiff.compare(g, moved)
+    # text fields are ignored unless asked for
+    text = tmp_path / "text.txt"
+    _edit(g, text, 3, 1, "START")
+    assert numdiff.compare(g, text) == []
+    assert numdiff.compare(g, text, check_text=True)
+    # a missing column, and a file that ends early
+    short = tmp_path / "short.txt"
+    _edit(g, short, 6, 4, "")
+    assert any("columns" in f for f in numdiff.compare(g, short))
+    early = tmp_path / "early.txt"
+    open(early, "w").writelines(open(g).read().splitlines(True)[:100])
+    assert "tested file ended too early" in numdiff.compare(g, early)
