@@ -6,7 +6,8 @@ reference's regression test compares with its .std golden.
     python -m flecsale_amd.apps hydro_2d -m square_32x32.g        # shock_box_2d-solution_rank000000.txt
     python -m flecsale_amd.apps hydro_3d -m sedov_20x20x20.g      # shock_box_3d-solution_rank000000.txt
     python -m flecsale_amd.apps maire_hydro_2d -m sedov_32x32.g   # sedov_2d-solution_rank000000.txt
-    python -m flecsale_amd.numdiff -a 1e-14 <golden>.std shock_box_2d-solution_rank000000.txt
+    python -m flecsale_amd.apps hydro_2d -m square_32x32.g -f shock_box_2d.lua   # the problem from a Lua input
+    python -m flecsale_amd.numdiff -a 2e-14 <golden>.std shock_box_2d-solution_rank000000.txt
 
 hydro_2d / hydro_3d: apps/hydro/driver.h:237-450 with apps/hydro/{2d,3d}/
 inputs.cc; maire_hydro_2d: apps/maire_hydro/driver.h:330-566 with
@@ -18,8 +19,10 @@ import argparse
 import os
 import sys
 
+import numpy as np
+
 import flecsale_amd as fa
-from flecsale_amd import inputs
+from flecsale_amd import inputs, lua_input
 
 APPS = {
     "hydro_2d": ("shock_box_2d", fa.SOLVER_EULER, inputs.shock_box_2d),
@@ -28,15 +31,38 @@ APPS = {
 }
 
 
-def run(app, mesh_path, out_dir="."):
-    """Runs `app` on the mesh file; returns the dump's path."""
+def _lua_problem(path, coords, cells, centroids):
+    """The Eulerian problem of a Lua input file (inputs_t::load_lua and
+    get_initial_conditions, apps/hydro/inputs.h:103-186): the file's step
+    parameters and EOS, and its ics(x, t) at every cell centroid at t = 0."""
+    h = lua_input.hydro_inputs(path)
+    dim = centroids.shape[1]
+    d = np.empty(len(centroids))
+    p = np.empty(len(centroids))
+    v = np.zeros((len(centroids), dim))
+    for c, x in enumerate(centroids):
+        d[c], vel, p[c] = h["ics"](x, 0.0)
+        v[c, :len(vel)] = vel[:dim]
+    return h["prefix"], dict(coords=coords, cells=cells, eos=h["eos"], d=d, v=v, p=p, cfl=h["CFL"],
+                             final_time=h["final_time"], max_steps=h["max_steps"], flux=0)
+
+
+def run(app, mesh_path, out_dir=".", input_file=None):
+    """Runs `app` on the mesh file (with the problem of a Lua input file when
+    given, as the apps' -f option); returns the dump's path."""
     prefix, solver, problem = APPS[app]
+    if input_file and solver != fa.SOLVER_EULER:
+        raise ValueError("Lua input files configure the Eulerian apps (hydro_2d, hydro_3d)")
     coords, cells = fa.read_exodus(mesh_path)
     ctx_type = fa.EulerContext if solver == fa.SOLVER_EULER else fa.MaireContext
     with ctx_type(coords, cells) as ctx:
         # initial conditions at the library's cell centroids, as the reference
         # evaluates them at burton's (apps/hydro/tasks.h:46-68)
-        prob = problem(coords, cells, ctx.get(fa.F_CELL_CENTROID))
+        centroids = ctx.get(fa.F_CELL_CENTROID)
+        if input_file:
+            prefix, prob = _lua_problem(input_file, coords, cells, centroids)
+        else:
+            prob = problem(coords, cells, centroids)
         if solver == fa.SOLVER_EULER:
             ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
             ctx.advance(prob["cfl"], prob["final_time"], prob["max_steps"], flux_kind=prob["flux"])
@@ -53,9 +79,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="run one of the reference's hydro apps on the GPU")
     ap.add_argument("app", choices=sorted(APPS))
     ap.add_argument("-m", "--mesh", required=True, help="Exodus-II mesh file")
+    ap.add_argument("-f", "--input", default=None, help="Lua input file (hydro apps)")
     ap.add_argument("-o", "--out-dir", default=".")
     a = ap.parse_args(argv)
-    path = run(a.app, a.mesh, a.out_dir)
+    path = run(a.app, a.mesh, a.out_dir, a.input)
     print(f"Dumping solution to: {path}")
     return 0
 

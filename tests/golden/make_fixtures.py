@@ -13,6 +13,7 @@ Outputs (all plain numpy .npz, loaded with allow_pickle=False):
                       the library's own reader and the regression drivers.
   std/<case>.std.gz   the one-rank .std goldens as text (gzip), for the
                       numdiff comparison of the drivers' dumps.
+  lua/<file>.lua      the apps' Lua input files (apps/hydro/{2d,3d}).
 
 Mesh sources: data/meshes/sedov_32x32.g and sedov_20x20x20.g are
 netCDF-classic (64-bit offset) read with scipy.io.netcdf_file;
@@ -114,6 +115,9 @@ def main():
                                                                 mtime=0) as dst:
             shutil.copyfileobj(src, dst)
         print("copied", name)
+    os.makedirs(os.path.join(HERE, "lua"), exist_ok=True)
+    for d in ["2d", "3d"]:
+        shutil.copyfile(f"{REF}/apps/hydro/{d}/shock_box_{d}.lua", os.path.join(HERE, "lua", f"shock_box_{d}.lua"))
 
 
 if __name__ == "__main__":

@@ -24,14 +24,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 EXO = os.path.join(HERE, "golden", "exodus")
 
 
-@pytest.mark.parametrize("app,mesh,golden", [
-    ("hydro_2d", "square_32x32.g", "shock_box_2d_1procs"),
-    ("hydro_3d", "sedov_20x20x20.g", "shock_box_3d_1procs"),
-    ("maire_hydro_2d", "sedov_32x32.g", "sedov_2d_1procs"),
+@pytest.mark.parametrize("app,mesh,lua,golden", [
+    ("hydro_2d", "square_32x32.g", None, "shock_box_2d_1procs"),
+    ("hydro_2d", "square_32x32.g", "shock_box_2d.lua", "shock_box_2d_1procs"),
+    ("hydro_3d", "sedov_20x20x20.g", None, "shock_box_3d_1procs"),
+    ("hydro_3d", "sedov_20x20x20.g", "shock_box_3d.lua", "shock_box_3d_1procs"),
+    ("maire_hydro_2d", "sedov_32x32.g", None, "sedov_2d_1procs"),
 ])
-def test_regression_dump_matches_golden(app, mesh, golden, tmp_path):
+def test_regression_dump_matches_golden(app, mesh, lua, golden, tmp_path):
+    """flecsale_shock_box_{2d,3d}_1procs[_with_input] and the Maire sedov_2d
+    regression (the _with_input variants take the problem from the Lua file)."""
     try:
-        dump = apps.run(app, os.path.join(EXO, mesh), str(tmp_path))
+        dump = apps.run(app, os.path.join(EXO, mesh), str(tmp_path),
+                        os.path.join(HERE, "golden", "lua", lua) if lua else None)
     except fa.FsaleError as e:
         if "HDF5 library" in str(e):
             pytest.skip("no HDF5 library to read the netCDF-4 mesh with")
