@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -126,14 +127,15 @@ class NcClassic {
     uint64_t n = 1;
     std::vector<uint64_t> sh;
     for (uint64_t id : v->dims) {
-      if (dims_[id].second == 0) return false;   // record variable
-      n *= dims_[id].second;
-      sh.push_back(dims_[id].second);
+      const uint64_t len = dims_[id].second;
+      if (len == 0 || len > b_.size() || n > b_.size() / len) return false;   // record variable, or larger than the file
+      n *= len;
+      sh.push_back(len);
     }
     static const int size[] = {0, 1, 1, 2, 4, 4, 8, 1, 2, 4, 8, 8};
     if (v->type < 1 || v->type > 11) return false;
     const int w = size[v->type];
-    if (v->begin + n * w > b_.size()) return false;
+    if (v->begin > b_.size() || n > (b_.size() - v->begin) / w) return false;
     out.resize(n);
     const uint8_t *p = b_.data() + v->begin;
     for (uint64_t i = 0; i < n; ++i, p += w) {
@@ -220,23 +222,25 @@ int read_classic(std::vector<uint8_t> bytes, ExoMesh &m) {
   uint64_t dim = 0, nv = 0, nblk = 0;
   if (!nc.has_dim("num_dim", &dim) || !nc.has_dim("num_nodes", &nv) || !nc.has_dim("num_el_blk", &nblk))
     return fail_mesh("not an Exodus file (num_dim / num_nodes / num_el_blk missing)");
+  if (nv > (uint64_t)INT32_MAX || nblk > 1000000) return fail_mesh("Exodus mesh too large");
   m.dim = (int)dim;
   m.nv = (int)nv;
   if (m.dim != 2 && m.dim != 3)
     return fail_mesh("Exodus mesh of dimension " + std::to_string(m.dim) + ": the hydro paths are 2D and 3D");
-  m.coords.assign((size_t)m.nv * m.dim, 0.0);
+  // coordinates are read (and so checked against the file's size) before
+  // anything is sized from the header's node count
   static const char *names[3] = {"coordx", "coordy", "coordz"};
-  std::vector<double> c;
+  std::vector<double> comp[3], c;
   if (nc.var("coordx")) {
-    for (int d = 0; d < m.dim; ++d) {
-      if (!nc.read(names[d], c) || c.size() != nv) return fail_mesh(std::string("Exodus: bad ") + names[d]);
-      for (int i = 0; i < m.nv; ++i) m.coords[(size_t)i * m.dim + d] = c[i];
-    }
+    for (int d = 0; d < m.dim; ++d)
+      if (!nc.read(names[d], comp[d]) || comp[d].size() != nv) return fail_mesh(std::string("Exodus: bad ") + names[d]);
   } else {
     if (!nc.read("coord", c) || c.size() != nv * dim) return fail_mesh("Exodus: no coordinates");
-    for (int d = 0; d < m.dim; ++d)
-      for (int i = 0; i < m.nv; ++i) m.coords[(size_t)i * m.dim + d] = c[(size_t)d * m.nv + i];
+    for (int d = 0; d < m.dim; ++d) comp[d].assign(c.begin() + (size_t)d * m.nv, c.begin() + (size_t)(d + 1) * m.nv);
   }
+  m.coords.assign((size_t)m.nv * m.dim, 0.0);
+  for (int d = 0; d < m.dim; ++d)
+    for (int i = 0; i < m.nv; ++i) m.coords[(size_t)i * m.dim + d] = comp[d][i];
   for (uint64_t b = 1; b <= nblk; ++b) {
     std::vector<int32_t> conn;
     std::vector<uint64_t> shape;
@@ -245,6 +249,7 @@ int read_classic(std::vector<uint8_t> bytes, ExoMesh &m) {
     if (m.vpc && (int)shape[1] != m.vpc) return fail_mesh("Exodus: element blocks of different node counts");
     m.vpc = (int)shape[1];
     m.cells.insert(m.cells.end(), conn.begin(), conn.end());
+    if (m.cells.size() / m.vpc > (size_t)INT32_MAX) return fail_mesh("Exodus mesh too large");
   }
   m.nc = m.vpc ? (int)(m.cells.size() / m.vpc) : 0;
   return finish_mesh(m);
@@ -325,7 +330,10 @@ bool h5_read(Hdf5 &L, int64_t file, const char *name, int64_t type, std::vector<
   if (ok) ok = L.sdims(sp, dims, nullptr) >= 0;
   if (sp >= 0) L.sclose(sp);
   uint64_t n = 1;
-  for (int i = 0; ok && i < nd; ++i) n *= dims[i];
+  for (int i = 0; ok && i < nd; ++i) {
+    if (dims[i] != 0 && n > ((uint64_t)1 << 31) / dims[i]) ok = false;   // beyond what the library's arrays index
+    n *= dims[i];
+  }
   if (ok) {
     out.resize(n);
     ok = L.dread(ds, type, 0, 0, 0, out.data()) >= 0;   // H5S_ALL, H5S_ALL, H5P_DEFAULT
@@ -334,6 +342,8 @@ bool h5_read(Hdf5 &L, int64_t file, const char *name, int64_t type, std::vector<
   if (ok && shape) shape->assign(dims, dims + nd);
   return ok;
 }
+
+int read_exodus_file(const char *path, ExoMesh &m);
 
 int read_hdf5(const char *path, ExoMesh &m) {
   Hdf5 &L = hdf5();
@@ -379,6 +389,14 @@ int read_hdf5(const char *path, ExoMesh &m) {
 
 int read_exodus(const char *path, ExoMesh &m) {
   if (!path) return FSALE_ERR_ARG;
+  try {
+    return read_exodus_file(path, m);
+  } catch (const std::bad_alloc &) {
+    return fail_mesh(std::string("out of host memory reading ") + path);
+  }
+}
+
+int read_exodus_file(const char *path, ExoMesh &m) {
   FILE *fp = std::fopen(path, "rb");
   if (!fp) return fail_mesh(std::string("cannot open ") + path);
   std::vector<uint8_t> bytes;

@@ -84,3 +84,31 @@ def test_numdiff_rules(tmp_path):
     early = tmp_path / "early.txt"
     open(early, "w").writelines(open(g).read().splitlines(True)[:100])
     assert "tested file ended too early" in numdiff.compare(g, early)
+
+
+def test_reader_survives_damaged_files(tmp_path):
+    """Truncated files and a header claiming a huge dimension are refused
+    with a status, never read past the end of the file; a cut that spares
+    every variable the reader uses gives the whole mesh."""
+    src = open(os.path.join(EXO, "sedov_32x32.g"), "rb").read()
+    ref = np.load(os.path.join(HERE, "golden", "meshes", "sedov_32x32.npz"))
+    bad = tmp_path / "bad.g"
+    refused = 0
+    for cut in list(range(0, 1200, 37)) + list(range(1200, len(src), 997)):
+        bad.write_bytes(src[:cut])
+        try:
+            coords, cells = fa.read_exodus(str(bad))
+        except fa.FsaleError as e:
+            assert e.status == fa.ERR_MESH
+            refused += 1
+            continue
+        assert np.array_equal(coords, ref["coords"]) and np.array_equal(cells, ref["cells"])
+    assert refused > 40
+    # num_nodes (a 4-byte big-endian length after its name) set to 2^31 - 1
+    i = src.find(b"num_nodes")
+    assert i > 0
+    j = i + (len(b"num_nodes") + 3) // 4 * 4
+    big = src[:j] + (0x7FFFFFFF).to_bytes(4, "big") + src[j + 4:]
+    bad.write_bytes(big)
+    with pytest.raises(fa.FsaleError):
+        fa.read_exodus(str(bad))
