@@ -295,7 +295,7 @@ class _Context:
             F_CELL_VOLUME: (nc,), F_CELL_MIN_LENGTH: (nc,), F_FACE_CENTROID: (nf, N),
             F_FACE_NORMAL: (nf, N), F_FACE_AREA: (nf,), F_COORDS: (nv, N), F_MASS: (nc,),
             F_VOLUME: (nc,), F_NODE_VELOCITY: (nv, N), F_CORNER_NORMAL: (nc * self.vpc, N),
-            F_CORNER_FORCE: (nc * self.vpc, N), F_RESIDUAL: (nc, 4), F_GAMMA: (nc,),
+            F_CORNER_FORCE: (nc * self.vpc, N), F_RESIDUAL: (nc, N + 2), F_GAMMA: (nc,),
         }
         return table[field]
 
@@ -399,13 +399,13 @@ class EulerContext(_Context):
 
 
 class MaireContext(_Context):
-    """Device-resident apps/maire_hydro state over one 2D quad mesh."""
+    """Device-resident apps/maire_hydro state over one 2D quad or 3D hex mesh."""
     solver = SOLVER_MAIRE
 
     def initial_conditions(self, cv, gamma, d, v, p):
         g = np.ascontiguousarray(np.broadcast_to(gamma, (self.nc,)), dtype=np.float64)
         d = np.ascontiguousarray(np.broadcast_to(d, (self.nc,)), dtype=np.float64)
-        v = np.ascontiguousarray(np.broadcast_to(v, (self.nc, 2)), dtype=np.float64)
+        v = np.ascontiguousarray(np.broadcast_to(v, (self.nc, self.dim)), dtype=np.float64)
         p = np.ascontiguousarray(np.broadcast_to(p, (self.nc,)), dtype=np.float64)
         _check(lib().fsale_maire_initial_conditions(self.h, cv, g.ctypes.data, d.ctypes.data,
                                                     v.ctypes.data, p.ctypes.data))
@@ -495,7 +495,7 @@ class Batch:
         nc = desc.num_cells
         g = np.ascontiguousarray(np.broadcast_to(gamma, (nc,)), dtype=np.float64)
         d = np.ascontiguousarray(np.broadcast_to(d, (nc,)), dtype=np.float64)
-        v = np.ascontiguousarray(np.broadcast_to(v, (nc, 2)), dtype=np.float64)
+        v = np.ascontiguousarray(np.broadcast_to(v, (nc, self.dim)), dtype=np.float64)
         p = np.ascontiguousarray(np.broadcast_to(p, (nc,)), dtype=np.float64)
         prm = MaireParams(cfl[0], cfl[1], cfl[2], final_time, initial_time_step, int(max_steps))
         out = ct.c_int32()

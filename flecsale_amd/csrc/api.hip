@@ -454,10 +454,6 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
     set_message("unsupported cell type (quads in 2D, hexes in 3D)");
     return FSALE_ERR_MESH;
   }
-  if (solver == FSALE_SOLVER_MAIRE && (dim != 2 || vpc != 4)) {
-    set_message("the Lagrangian solver supports 2D quad meshes");
-    return FSALE_ERR_MESH;
-  }
   // entity capacities below 2^28: kernels address an 8-byte element of an
   // entity array with a 32-bit byte offset
   if (max_problems <= 0 || cap_c <= 0 || cap_f <= 0 || cap_v <= 0 || cap_n < 0 || cap_c > (1 << 28) ||
@@ -630,7 +626,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   size_t bytes = 4 * ((size_t)nc * (t.vpc + 2 * t.fpc + m.nbk_deg + 2) + (size_t)nf * (2 + t.vpf) + P.nn) +
                  8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
   bytes += 4 * (size_t)nc;   // original ids
-  if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + (size_t)nc * t.vpc + nv;
+  if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + 2 * (size_t)nc * t.vpc + nv + 64 * t.vpc;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
   // k-rows of connectivity, filled on the row pool
   auto rows = [&](int32_t *dst_base, int rows_n, size_t stride, int n, auto value) -> int {
@@ -721,29 +717,53 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
            return (cpos[c] + c0) * t.vpc + k;
          })))
       return rc;
+    // face tags: bit 0-2 = |n_x|, |n_y|, |n_z| is 1, bit 3 = boundary face
     std::vector<uint8_t> ftag(nf, 0), vb(nv, 0);
+    const double *xs = mesh.coords;
     for (int f = 0; f < nf; ++f) {
       if (t.fc[2 * (size_t)f + 1] >= 0) continue;
-      const int a = t.fv[2 * (size_t)f], b = t.fv[2 * (size_t)f + 1];
-      double cx, cy, nx, ny, area;
-      face_geom_2d(mesh.coords[2 * (size_t)a], mesh.coords[2 * (size_t)a + 1], mesh.coords[2 * (size_t)b],
-                   mesh.coords[2 * (size_t)b + 1], cx, cy, nx, ny, area);
-      const double n[2] = {nx, ny};
-      uint8_t tag = 0x4;   // bit 2: boundary face
-      for (int d = 0; d < 2; ++d)
+      double n[3] = {0, 0, 0};
+      if (m.dim == 2) {
+        const int a = t.fv[2 * (size_t)f], b = t.fv[2 * (size_t)f + 1];
+        double cx, cy, area;
+        face_geom_2d(xs[2 * (size_t)a], xs[2 * (size_t)a + 1], xs[2 * (size_t)b], xs[2 * (size_t)b + 1], cx, cy,
+                     n[0], n[1], area);
+      } else {
+        double q[4][3], cen[3], area;
+        for (int i = 0; i < 4; ++i)
+          for (int d = 0; d < 3; ++d) q[i][d] = xs[3 * (size_t)t.fv[4 * (size_t)f + i] + d];
+        quad3_geom(q, cen, n, area);
+      }
+      uint8_t tag = 0x8;
+      for (int d = 0; d < m.dim; ++d)
         if (std::fabs(std::fabs(n[d]) - 1.0) <= 1e-14) tag |= (uint8_t)(1u << d);
       ftag[f] = tag;
       for (int i = 0; i < t.vpf; ++i) vb[t.fv[(size_t)f * t.vpf + i]] = 1;
     }
-    // corner (c,k): low nibble = face of edge (k-1 -> k), high nibble = edge (k -> k+1)
+    // corner nibbles: 2D (face of edge k-1 -> k, face of edge k -> k+1) with
+    // the boundary bit at 2; 3D the three sides holding k, in side order
     for (int k = 0; k < t.vpc; ++k) {
-      uint8_t *cw = st.slot<uint8_t>(nc);
+      uint16_t *cw = st.slot<uint16_t>(nc);
       if (!cw) return FSALE_ERR_ARG;
-      const int km = (k + t.vpc - 1) % t.vpc;
+      int sides[3], ns = 0;
+      if (m.dim == 2) {
+        sides[0] = (k + t.vpc - 1) % t.vpc;
+        sides[1] = k;
+        ns = 2;
+      } else {
+        for (int lf = 0; lf < 6; ++lf)
+          for (int i = 0; i < 4; ++i)
+            if (hex_face_vertex(lf, i) == k) sides[ns++] = lf;
+      }
       for (int sp = 0; sp < nc; ++sp) {
         const int c = cidx[sp];
-        const int fm = t.cf[(size_t)c * t.fpc + km], fp = t.cf[(size_t)c * t.fpc + k];
-        cw[sp] = (uint8_t)(ftag[fm] | (ftag[fp] << 4));
+        uint16_t w = 0;
+        for (int j = 0; j < ns; ++j) {
+          uint8_t tg = ftag[t.cf[(size_t)c * t.fpc + sides[j]]];
+          if (m.dim == 2 && tg) tg = (uint8_t)((tg & 3) | 4);
+          w |= (uint16_t)(tg << (4 * j));
+        }
+        cw[sp] = w;
       }
       if ((rc = st.put(m.cwtag + (size_t)k * m.cap_c + c0, cw, nc))) return rc;
     }
@@ -865,7 +885,7 @@ int field_view(fsale_ctx *ctx, int prob, int field, FieldView &fv) {
     MaireFields &u = ctx->ma;
     switch (field) {
       case FSALE_F_DENSITY: one(u.d, P.nc, P.c0, 'c'); return FSALE_OK;
-      case FSALE_F_VELOCITY: vec(u.v, 2, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_VELOCITY: vec(u.v, N, P.nc, P.c0, 'c'); return FSALE_OK;
       case FSALE_F_INTERNAL_ENERGY: one(u.e, P.nc, P.c0, 'c'); return FSALE_OK;
       case FSALE_F_PRESSURE: one(u.p, P.nc, P.c0, 'c'); return FSALE_OK;
       case FSALE_F_TEMPERATURE: one(u.T, P.nc, P.c0, 'c'); return FSALE_OK;
@@ -873,16 +893,16 @@ int field_view(fsale_ctx *ctx, int prob, int field, FieldView &fv) {
       case FSALE_F_MASS: one(u.M, P.nc, P.c0, 'c'); return FSALE_OK;
       case FSALE_F_VOLUME: one(u.V, P.nc, P.c0, 'c'); return FSALE_OK;
       case FSALE_F_GAMMA: one(u.g, P.nc, P.c0, 'c'); return FSALE_OK;
-      case FSALE_F_NODE_VELOCITY: vec(u.un, 2, P.nv, P.v0, 'v'); return FSALE_OK;
+      case FSALE_F_NODE_VELOCITY: vec(u.un, N, P.nv, P.v0, 'v'); return FSALE_OK;
       case FSALE_F_CORNER_NORMAL:
-        vec(u.npc, 2, P.nc * m.vpc, P.c0, 'c');
+        vec(u.npc, N, P.nc * m.vpc, P.c0, 'c');
         fv.corner = true;
         return FSALE_OK;
       case FSALE_F_CORNER_FORCE:
-        vec(u.Fpc, 2, P.nc * m.vpc, P.c0, 'c');
+        vec(u.Fpc, N, P.nc * m.vpc, P.c0, 'c');
         fv.corner = true;
         return FSALE_OK;
-      case FSALE_F_RESIDUAL: vec(u.dudt, 4, P.nc, P.c0, 'c'); return FSALE_OK;
+      case FSALE_F_RESIDUAL: vec(u.dudt, N + 2, P.nc, P.c0, 'c'); return FSALE_OK;
       default: break;
     }
   }
@@ -1200,7 +1220,7 @@ void step_state(fsale_ctx *ctx, std::vector<std::pair<void *, size_t>> &out) {
   } else {
     MaireFields &u = ctx->ma;
     for (double *p : {u.V, u.p, u.d, u.e, u.T, u.a, m.cvol, m.cminlen}) out.push_back({p, C});
-    for (int d = 0; d < 2; ++d) {
+    for (int d = 0; d < m.dim; ++d) {
       out.push_back({u.v[d], C});
       out.push_back({m.ccen[d], C});
       out.push_back({m.x[d], V});

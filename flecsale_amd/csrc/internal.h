@@ -102,7 +102,11 @@ struct DevMesh {
   int32_t *vc_cnt = nullptr;    // [cap_v] corners of each vertex
   int32_t *vck = nullptr;       // [kMaxCornersPerVertex][cap_v] corner c*vpc + k (batch cell
                                 // id), ascending cell id, -1 padded
-  uint8_t *cwtag = nullptr;     // [vpc][cap_c] Maire wedge boundary tags
+  // Maire wedge boundary tags of each corner, one nibble per face holding
+  // the corner (2D: the faces of edges (k-1, k) and (k, k+1); 3D: the three
+  // hex sides holding k, in side order): bit 2 (2D) / bit 3 (3D) = boundary
+  // face, the low bits = its x / y / z symmetry tags
+  uint16_t *cwtag = nullptr;    // [vpc][cap_c]
   uint8_t *vbnd = nullptr;      // [cap_v] vertex on a boundary face
   int32_t *bv = nullptr;        // [cap_v] a problem's boundary vertices (batch ids, ascending)
                                 // at positions [v0, v0 + nb) of its vertex range
@@ -169,12 +173,12 @@ struct EulerFields {
 };
 
 struct MaireFields {
-  double *V = nullptr, *M = nullptr, *v[2] = {}, *p = nullptr, *d = nullptr, *e = nullptr,
+  double *V = nullptr, *M = nullptr, *v[3] = {}, *p = nullptr, *d = nullptr, *e = nullptr,
          *T = nullptr, *a = nullptr, *g = nullptr;
-  double *v0[2] = {}, *e0 = nullptr, *x0[2] = {};
-  double *un[2] = {};
-  double *npc[2] = {}, *Fpc[2] = {};   // corner arrays [vpc][cap_c]
-  double *dudt[4] = {};
+  double *v0[3] = {}, *e0 = nullptr, *x0[3] = {};
+  double *un[3] = {};
+  double *npc[3] = {}, *Fpc[3] = {};   // corner arrays [vpc][cap_c]
+  double *dudt[5] = {};                // volume, momentum[dim], energy
   double *partials = nullptr;          // per cell tile and wave: max a/min_length, max |dV/dt|/V
 };
 

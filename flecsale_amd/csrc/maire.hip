@@ -1,5 +1,5 @@
 // Cell-centred Lagrangian (Maire) step of apps/maire_hydro on the device, 2D
-// quads.
+// quads and 3D hexes.
 //
 //   k_maire_nodal     evaluate_nodal_state      apps/maire_hydro/tasks.h:221-453
 //   k_maire_residual  evaluate_residual         apps/maire_hydro/tasks.h:461-489
@@ -15,8 +15,9 @@
 // rest; batch_dev.h) with per-problem scalars.  Wedge facets (half edges: burton's
 // wedge facet_normal / facet_area) are recomputed from the vertex coordinates
 // where needed instead of being stored: the mesh moves every half step.
-// Corner arrays are k-major [4][cap_c] so the residual gather of a cell's
-// four corners is coalesced.
+// Corner arrays are k-major [vpc][cap_c] so the residual gather of a cell's
+// corners is coalesced.  The kernels are templated on the dimension N; a 3D
+// corner has six wedge facets, two on each hex side holding its vertex.
 #include "batch_dev.h"
 
 namespace fsale {
@@ -167,6 +168,218 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
   }
 }
 
+// ------------------------------------------------------------ 3D corners
+// For local vertex k of a hex, its three sides in side order, each packed as
+// side | next << 8 | opposite << 16 | previous << 24: the local vertices
+// after, opposite and before k in the side's outward list (Exodus hex8
+// sides {0,1,5,4} {1,2,6,5} {2,3,7,6} {0,4,7,3} {0,3,2,1} {4,5,6,7}).
+__constant__ uint32_t kHexRing[8][3] = {
+    {0x04050100, 0x03070403, 0x01020304}, {0x00040500, 0x05060201, 0x02030004},
+    {0x01050601, 0x06070302, 0x03000104}, {0x02060702, 0x07040003, 0x00010204},
+    {0x05010000, 0x00030703, 0x07060505}, {0x01000400, 0x06020101, 0x04070605},
+    {0x02010501, 0x07030202, 0x05040705}, {0x03020602, 0x04000303, 0x06050405}};
+
+// The six wedge facets of 3D corner (c, k), in order: for each side j
+// holding k, the triangles (x_k, m+, x_s) and (x_k, x_s, m-), m+ / m- the
+// midpoints of the side's edges after / before k and x_s the side's vertex
+// average summed as ((a0 + a2) + (a1 + a3)) (the same bits from both cells
+// of an interior side).  f(j, n, l) receives each facet's unit normal and
+// area (burton's wedge facet_normal / facet_area).
+template <class F>
+__device__ inline void for_each_facet3(const DevMesh &m, int c, int k, F &&f) {
+  const int ik = m.cv[(size_t)k * m.cap_c + c];
+  const double xk[3] = {m.x[0][ik], m.x[1][ik], m.x[2][ik]};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const uint32_t r = kHexRing[k][j];
+    const int in = m.cv[(size_t)((r >> 8) & 0xff) * m.cap_c + c];
+    const int io = m.cv[(size_t)((r >> 16) & 0xff) * m.cap_c + c];
+    const int ip = m.cv[(size_t)(r >> 24) * m.cap_c + c];
+    double xs[3], mp[3], mm[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double xn = m.x[d][in], xo = m.x[d][io], xp = m.x[d][ip];
+      xs[d] = ((xk[d] + xo) + (xn + xp)) * 0.25;
+      mp[d] = 0.5 * (xk[d] + xn);
+      mm[d] = 0.5 * (xp + xk[d]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double a[3], b[3], t[3], n[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a[d] = (h == 0 ? mp[d] : xs[d]) - xk[d];
+        b[d] = (h == 0 ? xs[d] : mm[d]) - xk[d];
+      }
+      cross3(a, b, t);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) t[d] = 0.5 * t[d];
+      const double len = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) n[d] = t[d] / len;
+      f(j, n, len);
+    }
+  }
+}
+
+// corner matrix M_pc = sum_w z_c l_w n_w n_w^T, (l n)_pc = sum_w l_w n_w and
+// the corner force before the nodal velocity, F_pc = M_pc u_c + p_c (l n)_pc
+// (apps/maire_hydro/tasks.h:263-303)
+__device__ inline void corner3(const DevMesh &m, const MaireFields &u, int c, int k, double M[3][3], double np[3],
+                               double F[3]) {
+  const double zc = u.d[c] * u.a[c];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    np[r] = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) M[r][q] = 0;
+  }
+  for_each_facet3(m, c, k, [&](int, const double n[3], double l) {
+    const double fact = zc * l;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) M[r][q] += fact * n[r] * n[q];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) np[d] += l * n[d];
+  });
+  const double uc[3] = {u.v[0][c], u.v[1][c], u.v[2][c]};
+  const double pc = u.p[c];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    F[r] = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) F[r] += M[r][q] * uc[q];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) F[d] += pc * np[d];
+}
+
+// ristra::math::solve (un-vendored) of the 3x3 nodal system: adjugate over
+// the determinant
+__device__ inline void solve3(const double M[3][3], const double r[3], double x[3]) {
+  const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+  const double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+  const double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  const double det = (M[0][0] * c00 + M[0][1] * c01) + M[0][2] * c02;
+  x[0] = ((c00 * r[0] + (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * r[1]) + (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * r[2]) /
+         det;
+  x[1] = ((c01 * r[0] + (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * r[1]) + (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * r[2]) /
+         det;
+  x[2] = ((c02 * r[0] + (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * r[1]) + (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * r[2]) /
+         det;
+}
+
+// the augmented system [M_p N^T; N 0] of NS symmetry rows (tags in order)
+template <int NS>
+__device__ inline void sym_solve3(const double Mp[3][3], const double rhs[3], const double (&sn)[3][3],
+                                  double un[3]) {
+  constexpr int NR = 3 + NS;
+  double A[NR][NR], b[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    b[r] = r < 3 ? rhs[r] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      A[r][q] = r < 3 ? (q < 3 ? Mp[r][q] : sn[q - 3][r]) : (q < 3 ? sn[r - 3][q] : 0.0);
+  }
+  qr_solve_fixed<NR>(A, b);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) un[d] = b[d];
+}
+
+// evaluate_nodal_state for one 3D vertex (apps/maire_hydro/tasks.h:249-448):
+// corner matrices summed in corner order, the 3x3 solve inside, the
+// symmetry-constrained QR solve on the boundary (rows of the x / y / z tags
+// present, normals summed over the vertex's boundary wedges in corner and
+// facet order, :338-418), then F_pc -= M_pc u_p.  The corner matrices are
+// recomputed for the scatter instead of being held (eight corners of nine
+// entries would not fit the registers).
+template <bool BOUNDARY>
+__device__ inline void nodal3_vertex(const DevMesh &m, const MaireFields &u, int vt) {
+  const int cnt = m.vc_cnt[vt];
+  double Mp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rhs[3] = {0, 0, 0};
+#pragma unroll 1
+  for (int q = 0; q < cnt; ++q) {
+    const int cn = m.vck[(size_t)q * m.cap_v + vt];
+    double M[3][3], np[3], F[3];
+    corner3(m, u, cn >> 3, cn & 7, M, np, F);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int c2 = 0; c2 < 3; ++c2) Mp[r][c2] += M[r][c2];
+      rhs[r] += F[r];
+    }
+  }
+  double un[3];
+  if constexpr (BOUNDARY) {
+    int has = 0;   // bit t: tag t seen
+    double sn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 1
+    for (int q = 0; q < cnt; ++q) {
+      const int cn = m.vck[(size_t)q * m.cap_v + vt];
+      const int c = cn >> 3, k = cn & 7;
+      const int tags = m.cwtag[(size_t)k * m.cap_c + c];
+      if (!(tags & 0x888)) continue;
+      for_each_facet3(m, c, k, [&](int j, const double n[3], double l) {
+        const int tg = (tags >> (4 * j)) & 0xf;
+        if (!(tg & 8)) return;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+          if (!(tg & (1 << t))) continue;
+          if (has & (1 << t)) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sn[t][d] += l * n[d];
+          } else {
+            has |= 1 << t;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sn[t][d] = l * n[d];
+          }
+        }
+      });
+    }
+    // the rows of the tags present, in tag order
+    const bool h0 = has & 1, h1 = has & 2;
+    const int ns = (has & 1) + ((has >> 1) & 1) + ((has >> 2) & 1);
+    double rows[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      rows[0][d] = h0 ? sn[0][d] : (h1 ? sn[1][d] : sn[2][d]);
+      rows[1][d] = (h0 && h1) ? sn[1][d] : sn[2][d];
+      rows[2][d] = sn[2][d];
+    }
+    if (ns == 0)
+      solve3(Mp, rhs, un);
+    else if (ns == 1)
+      sym_solve3<1>(Mp, rhs, rows, un);
+    else if (ns == 2)
+      sym_solve3<2>(Mp, rhs, rows, un);
+    else
+      sym_solve3<3>(Mp, rhs, rows, un);
+  } else {
+    solve3(Mp, rhs, un);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) u.un[d][vt] = un[d];
+  // F_pc -= M_pc u_p  (matrix_vector(-1, Mpc, un, 1, Fpc), tasks.h:435-448)
+#pragma unroll 1
+  for (int q = 0; q < cnt; ++q) {
+    const int cn = m.vck[(size_t)q * m.cap_v + vt];
+    const int c = cn >> 3, k = cn & 7;
+    double M[3][3], np[3], F[3];
+    corner3(m, u, c, k, M, np, F);
+    const size_t ci = (size_t)k * m.cap_c + c;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      u.npc[r][ci] = np[r];
+      double t = M[r][0] * un[0];
+      t += M[r][1] * un[1];
+      t += M[r][2] * un[2];
+      u.Fpc[r][ci] = -1.0 * t + 1.0 * F[r];
+    }
+  }
+}
+
 // evaluate_nodal_state, one work-item per vertex.  The vertex's corners
 // (ascending cell id, k-major table) are visited in order; the first four
 // corner matrices stay in registers for the final F_pc -= M_pc u_p (a vertex
@@ -189,7 +402,7 @@ __device__ inline void corner_force(const MaireFields &u, int c, const double M[
 // on a boundary face, whose symmetry QR solve would otherwise hold the
 // registers (and the waves) of the whole interior launch.
 // one tile of interior (Cramer) or boundary (symmetry QR) vertices
-template <bool BOUNDARY>
+template <bool BOUNDARY, int N>
 __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
   {
     if (!wave_live(b.s, tile.prob, 0)) return;
@@ -199,6 +412,10 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       vt = m.bv[vt];
     else if (m.vbnd[vt])
       return;
+    if constexpr (N == 3) {
+      nodal3_vertex<BOUNDARY>(m, u, vt);
+      return;
+    }
     const int cnt = m.vc_cnt[vt];
     double M[kNodalKeep][2][2], np[kNodalKeep][2], F[kNodalKeep][2];
     int cc[kNodalKeep], kk[kNodalKeep];
@@ -276,9 +493,9 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
   }
 }
 
-template <bool BOUNDARY>
+template <bool BOUNDARY, int N>
 __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
-  for_each_tile(BOUNDARY ? b.bt : b.vt, [&](const Tile &tile) { nodal_tile<BOUNDARY>(m, u, b, tile); });
+  for_each_tile(BOUNDARY ? b.bt : b.vt, [&](const Tile &tile) { nodal_tile<BOUNDARY, N>(m, u, b, tile); });
 }
 
 // Both kinds in one launch over the one-block-per-tile lists: blocks
@@ -288,43 +505,54 @@ __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u
 // run while the interior waves fill the machine around them instead of
 // forming a launch of their own.  Held at 3 waves per SIMD for the interior
 // tiles (the boundary path then keeps part of its state in scratch).
+template <int N>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
   if ((int)blockIdx.x < nb) {
     if ((int)blockIdx.x < b.bt.ndirect) {
       const Tile t = b.bt.direct[blockIdx.x];
-      if (t.count > 0) nodal_tile<true>(m, u, b, t);
+      if (t.count > 0) nodal_tile<true, N>(m, u, b, t);
     }
   } else {
     const int i = (int)blockIdx.x - nb;
     if (i < b.vt.ndirect) {
       const Tile t = b.vt.direct[i];
-      if (t.count > 0) nodal_tile<false>(m, u, b, t);
+      if (t.count > 0) nodal_tile<false, N>(m, u, b, t);
     }
   }
 }
 
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, Batch b) {
+  constexpr int V = N == 2 ? 4 : 8;
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const int c = tile.first + threadIdx.x;
-    double r[4] = {0, 0, 0, 0};
+    double r[N + 2];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int i = 0; i < N + 2; ++i) r[i] = 0;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
       const size_t ci = (size_t)k * m.cap_c + c;
       const int vt = m.cv[ci];
-      const double uv[2] = {u.un[0][vt], u.un[1][vt]};
-      const double F[2] = {u.Fpc[0][ci], u.Fpc[1][ci]};
-      const double n[2] = {u.npc[0][ci], u.npc[1][ci]};
+      double uv[N], F[N], n[N];
+#pragma unroll
+      for (int d = 0; d < N; ++d) {
+        uv[d] = u.un[d][vt];
+        F[d] = u.Fpc[d][ci];
+        n[d] = u.npc[d][ci];
+      }
       // compute_update lagrange_eqns.h:374-389
-      for (int d = 0; d < 2; ++d) {
+#pragma unroll
+      for (int d = 0; d < N; ++d) {
         r[0] += n[d] * uv[d];
-        r[3] -= F[d] * uv[d];
+        r[N + 1] -= F[d] * uv[d];
         r[1 + d] -= F[d];
       }
     }
-    for (int i = 0; i < 4; ++i) u.dudt[i][c] = r[i];
+#pragma unroll
+    for (int i = 0; i < N + 2; ++i) u.dudt[i][c] = r[i];
   });
 }
 
@@ -374,15 +602,16 @@ __global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, cons
 // apply_update's mesh motion x = x0 + dt u (:517-534).  SAVE (the driver
 // step's predictor): save_coordinates first, x0 = x, and move from the value
 // just saved, so the step needs no separate save launch.
-template <bool SAVE>
+template <bool SAVE, int N>
 __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u, Batch b, double factor) {
   for_each_tile(b.vt, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const double delta_t = factor * b.s[tile.prob].dt;
     const int vt = tile.first + threadIdx.x;
-    double x0[2];
-    for (int d = 0; d < 2; ++d) {
+    double x0[N];
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
       if (SAVE) {
         x0[d] = m.x[d][vt];
         u.x0[d][vt] = x0[d];
@@ -391,7 +620,8 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
       }
     }
     if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
-    for (int d = 0; d < 2; ++d) m.x[d][vt] = x0[d] + delta_t * u.un[d][vt];
+#pragma unroll
+    for (int d = 0; d < N; ++d) m.x[d][vt] = x0[d] + delta_t * u.un[d][vt];
   });
 }
 
@@ -399,7 +629,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
 // predictor, which first saves v0 = v, e0 = e (save_solution); 2: the
 // corrector, which starts from v0, e0 (restore_solution) instead of copying
 // them back first.
-template <int MODE>
+template <int MODE, int N>
 __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, Batch b, double factor,
                                                          int token) {
   for_each_tile(b.ct, [&](const Tile &tile) {
@@ -411,46 +641,56 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
     double vol;
     if (delta_t > 1e-14) {
       // burton update_geometry() for the moved cell
-      double px[4], py[4];
-      for (int i = 0; i < 4; ++i) {
-        const int v = m.cv[(size_t)i * m.cap_c + c];
-        px[i] = m.x[0][v];
-        py[i] = m.x[1][v];
+      if constexpr (N == 2) {
+        double px[4], py[4];
+        for (int i = 0; i < 4; ++i) {
+          const int v = m.cv[(size_t)i * m.cap_c + c];
+          px[i] = m.x[0][v];
+          py[i] = m.x[1][v];
+        }
+        double cx, cy, ml;
+        cell_geom_2d(4, px, py, cx, cy, vol, ml);
+        m.ccen[0][c] = cx;
+        m.ccen[1][c] = cy;
+        m.cvol[c] = vol;
+        m.cminlen[c] = ml;
+      } else {
+        double p[8][3], cen[3], ml;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int v = m.cv[(size_t)i * m.cap_c + c];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) p[i][d] = m.x[d][v];
+        }
+        hex_geom(p, cen, vol, ml);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) m.ccen[d][c] = cen[d];
+        m.cvol[c] = vol;
+        m.cminlen[c] = ml;
       }
-      double cx, cy, ml;
-      cell_geom_2d(4, px, py, cx, cy, vol, ml);
-      m.ccen[0][c] = cx;
-      m.ccen[1][c] = cy;
-      m.cvol[c] = vol;
-      m.cminlen[c] = ml;
     } else {
       vol = m.cvol[c];
     }
     // update_state_from_flux lagrange_eqns.h:310-343
     const double M = u.M[c];
     const double inv_mass = 1 / M;
-    double vel[2], e0;
-    if (MODE == 2) {
-      vel[0] = u.v0[0][c];
-      vel[1] = u.v0[1][c];
-      e0 = u.e0[c];
-    } else {
-      vel[0] = u.v[0][c];
-      vel[1] = u.v[1][c];
-      e0 = u.e[c];
-    }
+    double vel[N], e0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) vel[i] = MODE == 2 ? u.v0[i][c] : u.v[i][c];
+    e0 = MODE == 2 ? u.e0[c] : u.e[c];
     if (MODE == 1) {
-      u.v0[0][c] = vel[0];
-      u.v0[1][c] = vel[1];
+#pragma unroll
+      for (int i = 0; i < N; ++i) u.v0[i][c] = vel[i];
       u.e0[c] = e0;
     }
-    const double et = e0 + dotN<2>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[3][c];
-    for (int i = 0; i < 2; ++i) vel[i] += delta_t * inv_mass * u.dudt[1 + i][c];
-    const double ie = et - dotN<2>(vel, vel) / 2;
+    const double et = e0 + dotN<N>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[N + 1][c];
+#pragma unroll
+    for (int i = 0; i < N; ++i) vel[i] += delta_t * inv_mass * u.dudt[1 + i][c];
+    const double ie = et - dotN<N>(vel, vel) / 2;
     // update_volume :352-367
     const double d = M / vol;
-    u.v[0][c] = vel[0];
-    u.v[1][c] = vel[1];
+#pragma unroll
+    for (int i = 0; i < N; ++i) u.v[i][c] = vel[i];
     u.e[c] = ie;
     u.V[c] = vol;
     u.d[c] = d;
@@ -464,12 +704,13 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
 }
 
 // save (dir 0) / restore (dir 1) of coordinates, velocity and energy
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u, Batch b, int dir) {
   for_each_tile(b.vt, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const int i = tile.first + threadIdx.x;
-    for (int d = 0; d < 2; ++d) {
+    for (int d = 0; d < N; ++d) {
       if (dir == 0)
         u.x0[d][i] = m.x[d][i];
       else
@@ -481,12 +722,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
     if ((int)threadIdx.x >= tile.count) return;
     const int i = tile.first + threadIdx.x;
     if (dir == 0) {
-      u.v0[0][i] = u.v[0][i];
-      u.v0[1][i] = u.v[1][i];
+      for (int d = 0; d < N; ++d) u.v0[d][i] = u.v[d][i];
       u.e0[i] = u.e[i];
     } else {
-      u.v[0][i] = u.v0[0][i];
-      u.v[1][i] = u.v0[1][i];
+      for (int d = 0; d < N; ++d) u.v[d][i] = u.v0[d][i];
       u.e[i] = u.e0[i];
     }
   });
@@ -536,30 +775,35 @@ bool check_ctx(fsale_ctx *ctx) {
   return true;
 }
 
-// evaluate_nodal_state: one launch for both kinds of vertex when both use
-// one block per tile (timed as the nodal kernel), else interior vertices
-// (Cramer) and then the boundary vertices' symmetry solves
-int launch_nodal(fsale_ctx *ctx) {
+// evaluate_nodal_state: in 2D one launch for both kinds of vertex when both
+// use one block per tile (timed as the nodal kernel), else interior vertices
+// (Cramer) and then the boundary vertices' symmetry solves.  3D keeps two
+// launches: its boundary solve (QR up to 6x6) would cap the interior waves'
+// registers.
+template <int N>
+int launch_nodal_n(fsale_ctx *ctx) {
   Batch bi = batch_args(ctx, K_MAIRE_NODAL);
   const Batch bb = batch_args(ctx, K_MAIRE_NODAL_BND);
-  if (bi.vt.direct && bb.bt.direct) {
+  if (N == 2 && bi.vt.direct && bb.bt.direct) {
     bi.bt = bb.bt;
     const int nb = (bb.bt.ndirect + 7) / 8 * 8;
     ProfScope prof(ctx, K_MAIRE_NODAL);
-    hipLaunchKernelGGL(k_maire_nodal_both, dim3(nb + bi.vt.ndirect), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma,
-                       bi, nb);
+    hipLaunchKernelGGL(k_maire_nodal_both<2>, dim3(nb + bi.vt.ndirect), dim3(kBlock), 0, ctx->stream, ctx->m,
+                       ctx->ma, bi, nb);
     FSALE_HIP(hipGetLastError());
     return FSALE_OK;
   }
-  int rc = launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal<false>, ctx->m, ctx->ma,
+  int rc = launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal<false, N>, ctx->m, ctx->ma,
                              batch_args(ctx, K_MAIRE_NODAL));
   return rc ? rc
-            : launch_persistent(ctx, K_MAIRE_NODAL_BND, k_maire_nodal<true>, ctx->m, ctx->ma,
+            : launch_persistent(ctx, K_MAIRE_NODAL_BND, k_maire_nodal<true, N>, ctx->m, ctx->ma,
                                 batch_args(ctx, K_MAIRE_NODAL_BND));
 }
+int launch_nodal(fsale_ctx *ctx) { return ctx->m.dim == 3 ? launch_nodal_n<3>(ctx) : launch_nodal_n<2>(ctx); }
 int launch_residual(fsale_ctx *ctx) {
-  return launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual, ctx->m, ctx->ma,
-                           batch_args(ctx, K_MAIRE_RESIDUAL));
+  const Batch b = batch_args(ctx, K_MAIRE_RESIDUAL);
+  return ctx->m.dim == 3 ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<3>, ctx->m, ctx->ma, b)
+                         : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2>, ctx->m, ctx->ma, b);
 }
 int launch_dt(fsale_ctx *ctx) {
   int rc = launch_persistent(ctx, K_MAIRE_DT, k_maire_dt_partial, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_DT));
@@ -574,17 +818,25 @@ int launch_dt(fsale_ctx *ctx) {
 // corrector 2, task-level apply_update 3)
 // stage 0: the apply_update task; 1: the driver step's predictor (saves
 // coordinates and solution as it goes); 2: its corrector (from the saved ones)
-int launch_apply(fsale_ctx *ctx, double factor, int token, int stage) {
+template <int N>
+int launch_apply_n(fsale_ctx *ctx, double factor, int token, int stage) {
   const Batch bm = batch_args(ctx, K_MAIRE_MOVE), bu = batch_args(ctx, K_MAIRE_UPDATE);
-  int rc = stage == 1 ? launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move<true>, ctx->m, ctx->ma, bm, factor)
-                      : launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move<false>, ctx->m, ctx->ma, bm, factor);
+  int rc = stage == 1 ? launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move<true, N>, ctx->m, ctx->ma, bm, factor)
+                      : launch_persistent(ctx, K_MAIRE_MOVE, k_maire_move<false, N>, ctx->m, ctx->ma, bm, factor);
   if (rc) return rc;
-  if (stage == 1) return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<1>, ctx->m, ctx->ma, bu, factor, token);
-  if (stage == 2) return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<2>, ctx->m, ctx->ma, bu, factor, token);
-  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<0>, ctx->m, ctx->ma, bu, factor, token);
+  if (stage == 1)
+    return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<1, N>, ctx->m, ctx->ma, bu, factor, token);
+  if (stage == 2)
+    return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<2, N>, ctx->m, ctx->ma, bu, factor, token);
+  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_update<0, N>, ctx->m, ctx->ma, bu, factor, token);
+}
+int launch_apply(fsale_ctx *ctx, double factor, int token, int stage) {
+  return ctx->m.dim == 3 ? launch_apply_n<3>(ctx, factor, token, stage) : launch_apply_n<2>(ctx, factor, token, stage);
 }
 int launch_copy(fsale_ctx *ctx, int dir) {
-  return launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_COPY), dir);
+  const Batch b = batch_args(ctx, K_MAIRE_COPY);
+  return ctx->m.dim == 3 ? launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy<3>, ctx->m, ctx->ma, b, dir)
+                         : launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy<2>, ctx->m, ctx->ma, b, dir);
 }
 
 }  // namespace
@@ -621,7 +873,7 @@ int maire_alloc(fsale_ctx *ctx) {
   double **cell1[] = {&u.V, &u.M, &u.p, &u.d, &u.e, &u.T, &u.a, &u.g, &u.e0};
   for (double **p : cell1)
     if ((rc = dev_alloc_n(ctx, p, m.cap_c))) return rc;
-  for (int d = 0; d < 2; ++d) {
+  for (int d = 0; d < m.dim; ++d) {
     if ((rc = dev_alloc_n(ctx, &u.v[d], m.cap_c))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.v0[d], m.cap_c))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.x0[d], m.cap_v))) return rc;
@@ -629,7 +881,7 @@ int maire_alloc(fsale_ctx *ctx) {
     if ((rc = dev_alloc_n(ctx, &u.npc[d], (size_t)m.cap_c * m.vpc))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.Fpc[d], (size_t)m.cap_c * m.vpc))) return rc;
   }
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < m.dim + 2; ++i)
     if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &u.partials, 2 * (size_t)kWaves * (ctx->ra_ct.capacity + 1)))) return rc;
   return FSALE_OK;

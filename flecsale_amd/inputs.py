@@ -208,6 +208,52 @@ def maire_bubble_2d(n=512, steps=500, seed=MASTER_SEED):
                 final_time=1.0, max_steps=steps, dt0=1.e-5)
 
 
+def sedov_3d(coords, cells, centroids, volumes):
+    """apps/maire_hydro/3d/inputs.cc:30-101 (the reference's sedov_20x20x20 smoke
+    case): energy 0.106384 in the cells within sqrt(3) cbrt(V) / 2 (+ eps) of the
+    origin, CFL (0.25, 0.1, 1.01), dt0 1e-5, 10 steps; x / y / z symmetry walls."""
+    x = centroids
+    delta_r = np.finfo(np.float64).eps + math.sqrt(3) * np.cbrt(volumes) / 2
+    r = np.sqrt(x[:, 0] * x[:, 0] + x[:, 1] * x[:, 1] + x[:, 2] * x[:, 2])
+    p = np.where(r < delta_r, (1.4 - 1) * 1.0 * 0.106384 / volumes, 1.e-6)
+    nc = len(p)
+    return dict(coords=coords, cells=cells, cv=1.0, gamma=np.full(nc, 1.4), d=np.ones(nc),
+                v=np.zeros((nc, 3)), p=p, cfl=(0.25, 0.1, 1.01), final_time=1.0, max_steps=10, dt0=1.e-5)
+
+
+def maire_bubble_3d(n=64, steps=100, seed=MASTER_SEED):
+    """The two-gas shock-bubble problem on a jittered n^3 hex mesh of [0,1]^3: a
+    sphere r < 0.15 about the centre (gamma 1.67, rho 0.138), the shocked state
+    at x < 0.1 driven to +x, symmetry walls."""
+    coords, cells = box_hexes(n, n, n, jitter=0.15, seed=seed, stream=41)
+    x = coords[cells].mean(axis=1)
+    nc = len(cells)
+    g = np.full(nc, 1.4)
+    d = np.ones(nc)
+    v = np.zeros((nc, 3))
+    p = np.ones(nc)
+    r = np.sqrt(((x - 0.5) ** 2).sum(axis=1))
+    bub = r < 0.15
+    g[bub], d[bub] = 1.67, 0.138
+    shock = x[:, 0] < 0.1
+    d[shock], v[shock, 0], p[shock] = 1.376, 0.394, 1.5698
+    return dict(coords=coords, cells=cells, cv=1.0, gamma=g, d=d, v=v, p=p, cfl=(0.25, 0.1, 1.01),
+                final_time=1.0, max_steps=steps, dt0=1.e-5)
+
+
+def extrude_quads(coords, cells, nz, h, perm=(0, 1, 2)):
+    """The hex mesh of a 2D quad mesh extruded through nz layers of height h
+    (bottom quad, then the quad above it: Exodus hex8 order), its axes then
+    permuted by `perm` (a cyclic permutation keeps the hexes positive)."""
+    nv2 = len(coords)
+    xs = np.zeros((nv2 * (nz + 1), 3))
+    for j in range(nz + 1):
+        xs[j * nv2:(j + 1) * nv2, :2] = coords
+        xs[j * nv2:(j + 1) * nv2, 2] = j * h
+    layers = [np.concatenate([cells + j * nv2, cells + (j + 1) * nv2], axis=1) for j in range(nz)]
+    return np.ascontiguousarray(xs[:, list(perm)]), np.concatenate(layers).astype(np.int32)
+
+
 _SQUARES = {}
 
 

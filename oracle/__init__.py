@@ -55,6 +55,7 @@ def lib():
                              ct.c_int, ct.c_int, ct.c_int] + [_dp] * 6 + \
             [ct.POINTER(ct.c_double), ct.POINTER(ct.c_int)]
         L.om_qr_solve.argtypes = [ct.c_int, _dp, _dp]
+        L.om_solve3.argtypes = [_dp, _dp, _dp]
         L.om_set_variant.argtypes = [ct.c_int]
         L.om_eos_eval.argtypes = [ct.c_double, ct.c_double, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.om_maire_init.argtypes = [_vp, _dp, ct.c_double, _dp, _dp] + [_dp] * 6
@@ -228,6 +229,13 @@ def qr_solve(A, b):
     return b
 
 
+def solve3(M, r):
+    """The 3x3 nodal solve (adjugate over determinant)."""
+    u = np.zeros(3)
+    lib().om_solve3(_c(M).reshape(-1).copy(), _c(r).copy(), u)
+    return u
+
+
 # ---------------------------------------------------------------- Maire
 
 class MaireState:
@@ -267,15 +275,17 @@ def maire_run(mesh, s, cv, cfl, final_time, max_steps, dt0, soln_time=0.0):
 
 
 def maire_nodal(mesh, s):
-    nv, ncn = mesh.nv, mesh.nc * mesh.vpc
-    un = np.zeros((nv, 2)); npc = np.zeros((ncn, 2)); Fpc = np.zeros((ncn, 2))
+    """evaluate_nodal_state (apps/maire_hydro/tasks.h:221-453): nodal
+    velocities [nv, dim] and corner normals / forces [nc*vpc, dim]."""
+    nv, ncn, N = mesh.nv, mesh.nc * mesh.vpc, mesh.dim
+    un = np.zeros((nv, N)); npc = np.zeros((ncn, N)); Fpc = np.zeros((ncn, N))
     lib().om_maire_nodal(mesh.h, s.v.reshape(-1), s.p, s.d, s.a, un.reshape(-1), npc.reshape(-1),
                          Fpc.reshape(-1))
     return un, npc, Fpc
 
 
 def maire_residual(mesh, un, npc, Fpc):
-    dudt = np.zeros((mesh.nc, 4))
+    dudt = np.zeros((mesh.nc, mesh.dim + 2))
     lib().om_maire_residual(mesh.h, _c(un).reshape(-1), _c(npc).reshape(-1), _c(Fpc).reshape(-1),
                             dudt.reshape(-1))
     return dudt

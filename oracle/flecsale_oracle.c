@@ -29,6 +29,15 @@
  *     corner ordered (edge v_{k-1}v_k, edge v_k v_{k+1}); wedge facet = half
  *     edge: facet_area = |edge midpoint - vertex|, facet_normal = outward edge
  *     unit normal (pinned end-to-end by the sedov_2d golden);
+ *   - 3D Maire corner (c, k): the three hex sides holding k, in side order,
+ *     each with k = a_i of its outward list a_0..a_3 gives two wedge facets,
+ *     the triangles (x_k, m+, x_s) then (x_k, x_s, m-), m+ / m- the midpoints
+ *     of edges (a_i, a_{i+1}) / (a_{i-1}, a_i) and x_s = ((a_0 + a_2) +
+ *     (a_1 + a_3)) / 4 (the same bits from both cells of the side): facet area
+ *     vector = cross / 2, facet_area its length, facet_normal its direction;
+ *     ristra::math::solve (un-vendored) of the 3x3 nodal system is the
+ *     adjugate over the determinant (pinned end-to-end: a 3D run of the
+ *     sedov_2d problem extruded in z reproduces the sedov_2d golden);
  *   - min_length(c) = minimum distance between two vertices of c.
  */
 #include <math.h>
@@ -950,10 +959,157 @@ static void wedge_facets_2d(const om_mesh *m, int c, int k, double n[2][2], doub
   }
 }
 
+/* wedge facets of 3D corner (c, k): six triangles, two per hex side holding
+ * k (side order); area-weighted normal A = cross / 2 split into the unit
+ * normal n and the area l; lside = the local side of each facet. */
+static void wedge_facets_3d(const om_mesh *m, int c, int k, double n[6][3], double l[6], int lside[6]) {
+  const int *cv = m->cv + 8 * c;
+  const double *xk = m->x + 3 * cv[k];
+  int w = 0;
+  for (int s = 0; s < 6; ++s) {
+    int i = 0;
+    while (i < 4 && HEX_FACES[s][i] != k) ++i;
+    if (i == 4) continue;
+    const double *a[4];
+    for (int j = 0; j < 4; ++j) a[j] = m->x + 3 * cv[HEX_FACES[s][j]];
+    const double *xn = a[(i + 1) & 3], *xp = a[(i + 3) & 3];
+    double xs[3], mp[3], mm[3], u[3], v[3], t[3];
+    for (int d = 0; d < 3; ++d) {
+      xs[d] = ((a[0][d] + a[2][d]) + (a[1][d] + a[3][d])) * 0.25;
+      mp[d] = 0.5 * (xk[d] + xn[d]);
+      mm[d] = 0.5 * (xp[d] + xk[d]);
+    }
+    if (om_variant == 4) {   /* planted: the midpoints exchanged */
+      for (int d = 0; d < 3; ++d) { double t0 = mp[d]; mp[d] = mm[d]; mm[d] = t0; }
+    }
+    if (om_variant == 5) {   /* planted: the side's area centroid for x_s */
+      double nrm[3], area;
+      quad3_geom(a, xs, nrm, &area);
+    }
+    for (int h = 0; h < 2; ++h) {
+      const double *p = h == 0 ? mp : xs, *q = h == 0 ? xs : mm;
+      for (int d = 0; d < 3; ++d) { u[d] = p[d] - xk[d]; v[d] = q[d] - xk[d]; }
+      cross3(u, v, t);
+      for (int d = 0; d < 3; ++d) t[d] = 0.5 * t[d];
+      double len = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      for (int d = 0; d < 3; ++d) n[w][d] = t[d] / len;
+      l[w] = len;
+      lside[w] = s;
+      ++w;
+    }
+  }
+}
+
+/* ristra::math::solve (un-vendored) for 3x3: adjugate over determinant */
+void om_solve3(const double *M, const double *r, double *u) {
+  const double c00 = M[4] * M[8] - M[5] * M[7];
+  const double c01 = M[5] * M[6] - M[3] * M[8];
+  const double c02 = M[3] * M[7] - M[4] * M[6];
+  const double det = (M[0] * c00 + M[1] * c01) + M[2] * c02;
+  u[0] = ((c00 * r[0] + (M[2] * M[7] - M[1] * M[8]) * r[1]) + (M[1] * M[5] - M[2] * M[4]) * r[2]) / det;
+  u[1] = ((c01 * r[0] + (M[0] * M[8] - M[2] * M[6]) * r[1]) + (M[2] * M[3] - M[0] * M[5]) * r[2]) / det;
+  u[2] = ((c02 * r[0] + (M[1] * M[6] - M[0] * M[7]) * r[1]) + (M[0] * M[4] - M[1] * M[3]) * r[2]) / det;
+}
+
+/* evaluate_nodal_state tasks.h:221-453 for hexes: corner matrices from the
+ * six wedge facets of each corner, the 3x3 solve inside, symmetry rows of
+ * the x / y / z tags (apps/maire_hydro/3d/inputs.cc:116-176) on the boundary. */
+static void maire_nodal_3d(const om_mesh *m, const double *uc, const double *pc, const double *dc,
+                           const double *ac, double *un, double *npc, double *Fpc) {
+  #pragma omp parallel for schedule(static)
+  for (int vt = 0; vt < m->nv; ++vt) {
+    double Mp[3][3] = {{0}}, rhs[3] = {0, 0, 0};
+    int ncn = m->vc_off[vt + 1] - m->vc_off[vt];
+    double Mpc[16][3][3];
+    if (ncn > 16) ncn = 16;
+    for (int j = 0; j < ncn; ++j) {
+      int cn = m->vc[m->vc_off[vt] + j];
+      int cl = cn / 8, k = cn % 8;
+      double *F = Fpc + 3 * cn, *np = npc + 3 * cn;
+      for (int d = 0; d < 3; ++d) { F[d] = 0; np[d] = 0; }
+      double zc = dc[cl] * ac[cl];
+      double n[6][3], l[6];
+      int ls[6];
+      wedge_facets_3d(m, cl, k, n, l, ls);
+      memset(Mpc[j], 0, sizeof(Mpc[j]));
+      for (int w = 0; w < 6; ++w) {
+        double fact = zc * l[w];
+        for (int r = 0; r < 3; ++r)
+          for (int s = 0; s < 3; ++s) Mpc[j][r][s] += fact * n[w][r] * n[w][s];
+        for (int dd = 0; dd < 3; ++dd) np[dd] += l[w] * n[w][dd];
+      }
+      for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) Mp[r][s] += Mpc[j][r][s];
+      const double *u = uc + 3 * cl;
+      for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) F[r] += Mpc[j][r][s] * u[s];
+      for (int dd = 0; dd < 3; ++dd) {
+        F[dd] += pc[cl] * np[dd];
+        rhs[dd] += F[dd];
+      }
+    }
+    double *u = un + 3 * vt;
+    int has[3] = {0, 0, 0};
+    double symn[3][3];
+    if (m->vbnd[vt]) {
+      for (int j = 0; j < ncn; ++j) {
+        int cn = m->vc[m->vc_off[vt] + j];
+        int cl = cn / 8, k = cn % 8;
+        double n[6][3], l[6];
+        int ls[6];
+        wedge_facets_3d(m, cl, k, n, l, ls);
+        for (int w = 0; w < 6; ++w) {
+          int f = m->cf[cl * 6 + ls[w]];
+          if (m->fc[2 * f + 1] >= 0) continue;
+          for (int tag = 0; tag < 3; ++tag) {
+            if (!(m->ftag[f] & (1u << tag))) continue;
+            if (om_variant == 6 && tag == 2) continue;   /* planted: z walls not symmetric */
+            if (has[tag]) { for (int dd = 0; dd < 3; ++dd) symn[tag][dd] += l[w] * n[w][dd]; }
+            else { has[tag] = 1; for (int dd = 0; dd < 3; ++dd) symn[tag][dd] = l[w] * n[w][dd]; }
+          }
+        }
+      }
+    }
+    const int nsym = has[0] + has[1] + has[2];
+    if (nsym == 0) {
+      om_solve3(&Mp[0][0], rhs, u);
+    } else {
+      int nr = 3 + nsym;
+      double A[36] = {0}, b[6] = {0};
+      for (int r = 0; r < 3; ++r) b[r] = rhs[r];
+      for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) A[r * nr + s] = Mp[r][s];
+      int jj = 3;
+      for (int tag = 0; tag < 3; ++tag) {
+        if (!has[tag]) continue;
+        for (int r = 0; r < 3; ++r) A[r * nr + jj] = symn[tag][r];
+        for (int s = 0; s < 3; ++s) A[jj * nr + s] = symn[tag][s];
+        ++jj;
+      }
+      om_qr_solve(nr, A, b);
+      for (int r = 0; r < 3; ++r) u[r] = b[r];
+    }
+    for (int j = 0; j < ncn; ++j) {
+      int cn = m->vc[m->vc_off[vt] + j];
+      double *F = Fpc + 3 * cn;
+      for (int r = 0; r < 3; ++r) {
+        double s = Mpc[j][r][0] * u[0];
+        s += Mpc[j][r][1] * u[1];
+        s += Mpc[j][r][2] * u[2];
+        F[r] = -1.0 * s + 1.0 * F[r];
+      }
+    }
+  }
+}
+
 /* evaluate_nodal_state tasks.h:221-453 (2D, symmetry boundary conditions of
- * inputs_regression.cc:116-162: both tags symmetric).  un[nv*2]. */
+ * inputs_regression.cc:116-162: both tags symmetric).  un[nv*dim]. */
 void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const double *dc,
                     const double *ac, double *un, double *npc, double *Fpc) {
+  if (m->dim == 3) {
+    maire_nodal_3d(m, uc, pc, dc, ac, un, npc, Fpc);
+    return;
+  }
   const int nvpc = m->vpc;
   #pragma omp parallel for schedule(static)
   for (int vt = 0; vt < m->nv; ++vt) {
@@ -1047,24 +1203,24 @@ void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const 
 }
 
 /* evaluate_residual tasks.h:461-489 with compute_update lagrange_eqns.h:374-389.
- * dudt[nc*4] = (mass, mom x, mom y, energy). */
+ * dudt[nc*(dim+2)] = (volume, momentum[dim], energy). */
 void om_maire_residual(const om_mesh *m, const double *un, const double *npc, const double *Fpc,
                        double *dudt) {
-  const int nvpc = m->vpc;
+  const int nvpc = m->vpc, N = m->dim;
   #pragma omp parallel for schedule(static)
   for (int c = 0; c < m->nc; ++c) {
-    double r[4] = {0, 0, 0, 0};
+    double r[5] = {0, 0, 0, 0, 0};
     for (int k = 0; k < nvpc; ++k) {
       int cn = c * nvpc + k;
-      const double *u = un + 2 * m->cv[cn];
-      const double *F = Fpc + 2 * cn, *n = npc + 2 * cn;
-      for (int dd = 0; dd < 2; ++dd) {
+      const double *u = un + N * m->cv[cn];
+      const double *F = Fpc + N * cn, *n = npc + N * cn;
+      for (int dd = 0; dd < N; ++dd) {
         r[0] += n[dd] * u[dd];
-        r[3] -= F[dd] * u[dd];
+        r[N + 1] -= F[dd] * u[dd];
         r[1 + dd] -= F[dd];
       }
     }
-    memcpy(dudt + 4 * c, r, sizeof(r));
+    memcpy(dudt + (N + 2) * c, r, sizeof(double) * (N + 2));
   }
 }
 
@@ -1075,7 +1231,7 @@ double om_maire_time_step(const om_mesh *m, double cfl_acc, double cfl_vol, doub
   for (int c = 0; c < m->nc; ++c) {
     double dti = a[c] / m->cminlen[c];
     acc_inv = smax(dti, acc_inv);
-    double dVdt = dudt[4 * c];
+    double dVdt = dudt[(m->dim + 2) * c];
     dti = fabs(dVdt) / m->cvol[c];
     vol_inv = smax(dti, vol_inv);
   }
@@ -1091,19 +1247,20 @@ double om_maire_time_step(const om_mesh *m, double cfl_acc, double cfl_vol, doub
 int om_maire_apply_update(om_mesh *m, double delta_t, const double *x0, const double *un,
                           const double *dudt, double *V, const double *M, double *uc, double *d,
                           double *e, int *bad_cell) {
+  const int N = m->dim;
   if (delta_t > 1e-14) {   /* flecsale::config::test_tolerance */
     for (int vt = 0; vt < m->nv; ++vt)
-      for (int dd = 0; dd < 2; ++dd) m->x[2 * vt + dd] = x0[2 * vt + dd] + delta_t * un[2 * vt + dd];
+      for (int dd = 0; dd < N; ++dd) m->x[N * vt + dd] = x0[N * vt + dd] + delta_t * un[N * vt + dd];
     om_mesh_update_geometry(m);
   }
   int bad = -1;
   for (int c = 0; c < m->nc; ++c) {
     double inv_mass = 1 / M[c];
-    double *vel = uc + 2 * c;
-    const double *du = dudt + 4 * c;
-    double et = e[c] + dot(vel, vel, 2) / 2 + delta_t * inv_mass * du[3];
-    for (int i = 0; i < 2; ++i) vel[i] += delta_t * inv_mass * du[1 + i];
-    e[c] = et - dot(vel, vel, 2) / 2;
+    double *vel = uc + N * c;
+    const double *du = dudt + (N + 2) * c;
+    double et = e[c] + dot(vel, vel, N) / 2 + delta_t * inv_mass * du[N + 1];
+    for (int i = 0; i < N; ++i) vel[i] += delta_t * inv_mass * du[1 + i];
+    e[c] = et - dot(vel, vel, N) / 2;
     if (e[c] < 0 && bad < 0) bad = c;
     V[c] = m->cvol[c];
     d[c] = M[c] / m->cvol[c];
@@ -1129,19 +1286,19 @@ int om_maire_run(om_mesh *m, const double *g, double cv, double cfl_acc, double 
                  double cfl_growth, double final_time, int max_steps, double *dt_io, double *V,
                  const double *M, double *uc, double *p, double *d, double *e, double *T,
                  double *a, double *soln_time, int *steps_done) {
-  const int nc = m->nc, nv = m->nv, ncn = nc * m->vpc;
-  double *x0 = (double *)malloc(sizeof(double) * nv * 2);
-  double *uc0 = (double *)malloc(sizeof(double) * nc * 2);
+  const int nc = m->nc, nv = m->nv, ncn = nc * m->vpc, N = m->dim;
+  double *x0 = (double *)malloc(sizeof(double) * nv * N);
+  double *uc0 = (double *)malloc(sizeof(double) * nc * N);
   double *ec0 = (double *)malloc(sizeof(double) * nc);
-  double *un = (double *)malloc(sizeof(double) * nv * 2);
-  double *npc = (double *)malloc(sizeof(double) * ncn * 2);
-  double *Fpc = (double *)malloc(sizeof(double) * ncn * 2);
-  double *dudt = (double *)malloc(sizeof(double) * nc * 4);
+  double *un = (double *)malloc(sizeof(double) * nv * N);
+  double *npc = (double *)malloc(sizeof(double) * ncn * N);
+  double *Fpc = (double *)malloc(sizeof(double) * ncn * N);
+  double *dudt = (double *)malloc(sizeof(double) * nc * (N + 2));
   double t = *soln_time, ts = *dt_io;
   int n, rc = OM_OK, bad;
   for (n = 0; n < max_steps && t < final_time; ++n) {
-    memcpy(x0, m->x, sizeof(double) * nv * 2);
-    memcpy(uc0, uc, sizeof(double) * nc * 2);
+    memcpy(x0, m->x, sizeof(double) * nv * N);
+    memcpy(uc0, uc, sizeof(double) * nc * N);
     memcpy(ec0, e, sizeof(double) * nc);
     om_maire_nodal(m, uc, p, d, a, un, npc, Fpc);
     om_maire_residual(m, un, npc, Fpc, dudt);
@@ -1152,8 +1309,8 @@ int om_maire_run(om_mesh *m, const double *g, double cv, double cfl_acc, double 
     om_maire_from_energy(nc, g, cv, d, e, p, T, a);
     om_maire_nodal(m, uc, p, d, a, un, npc, Fpc);
     om_maire_residual(m, un, npc, Fpc, dudt);
-    memcpy(m->x, x0, sizeof(double) * nv * 2);
-    memcpy(uc, uc0, sizeof(double) * nc * 2);
+    memcpy(m->x, x0, sizeof(double) * nv * N);
+    memcpy(uc, uc0, sizeof(double) * nc * N);
     memcpy(e, ec0, sizeof(double) * nc);
     rc = om_maire_apply_update(m, ts, x0, un, dudt, V, M, uc, d, e, &bad);
     if (rc) break;
