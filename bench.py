@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: zone-updates/s of the Eulerian MUSCL step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload muscl2d|muscl3d|maire2d]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload muscl2d|muscl3d|maire2d|maire3d]
 
 Default workload (the configuration the metric is quoted on): second-order
 MUSCL (LSQ gradients + vertex limiter, HLLC flux) four-quadrant Riemann
@@ -61,6 +61,16 @@ def kernel_bytes_maire(nc, nv, ncn):
     + state (M, v, e, dU, gamma in; v, e, V, d, p, T, a out); save/restore: x, v, e copies."""
     return {4: 56 * nc + 36 * nv + 36 * ncn, 5: 48 * nc + 16 * nv + 32 * ncn, 6: 32 * nc,
             7: 48 * nv, 8: 184 * nc + 16 * nv, 9: 48 * nc + 32 * nv}
+
+
+def kernel_bytes_maire3(nc, nv, ncn):
+    """The same model for hexes (3 velocity components, 8 corners of 12 doubles'
+    worth of corner output per cell): nodal 80 B/cell (d, a, p, v, cell->vertex),
+    52 B/vertex (coords, u_p out, corner count), 52 B/corner (vertex->corner
+    entry, npc and Fpc out); residual 72 B/cell, 24 B/vertex, 48 B/corner; move
+    72 B/vertex; update 232 B/cell + 24 B/vertex."""
+    return {4: 80 * nc + 52 * nv + 52 * ncn, 5: 72 * nc + 24 * nv + 48 * ncn, 6: 32 * nc,
+            7: 72 * nv, 8: 232 * nc + 24 * nv, 9: 64 * nc + 48 * nv}
 
 
 def load_traffic(workload):
@@ -202,7 +212,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d", "serve"])
+    ap.add_argument("--workload", default="muscl2d",
+                    choices=["muscl2d", "muscl3d", "maire2d", "maire3d", "serve"])
     ap.add_argument("--requests", type=int, default=None,
                     help="requests of the served stream (default 256 beside a step benchmark, 4096 for serve)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
@@ -240,6 +251,14 @@ def main():
         prob = inputs.kelvin_helmholtz_3d(n, steps=K)
         kind, dim = "euler", 3
         config = {"workload": f"MUSCL3D Kelvin-Helmholtz, {n}^3 hexes jittered +-10%, HLLE, CFL 1/3",
+                  "cells": n ** 3, "steps_per_run": K}
+    elif args.workload == "maire3d":
+        n = args.size or 128
+        K = args.steps or 100
+        prob = inputs.maire_bubble_3d(n, steps=K)
+        kind, dim = "maire", 3
+        config = {"workload": f"Maire cell-centred Lagrangian, {n}^3 hexes jittered +-15%, two ideal gases "
+                              f"(bubble gamma 1.67), left-driven shock, symmetry walls on six sides",
                   "cells": n ** 3, "steps_per_run": K}
     else:
         n = args.size or 512
@@ -307,7 +326,7 @@ def main():
         nparts = 8 * ((nc + 256 * 8 - 1) // (256 * 8))
         kb = {0: 8 * nparts, 1: tasks[1], 10: tasks[10]}
     else:
-        kb = kernel_bytes_maire(nc, ctx.nv, nc * 4)
+        kb = (kernel_bytes_maire if dim == 2 else kernel_bytes_maire3)(nc, ctx.nv, nc * ctx.vpc)
         launches_per_step = {4: 2, 5: 2, 6: 1, 7: 2, 8: 2, 9: 2}
         bytes_per_step = sum(kb[k] * launches_per_step[k] for k in kb)
 

@@ -6,13 +6,16 @@ reference's regression test compares with its .std golden.
     python -m flecsale_amd.apps hydro_2d -m square_32x32.g        # shock_box_2d-solution_rank000000.txt
     python -m flecsale_amd.apps hydro_3d -m sedov_20x20x20.g      # shock_box_3d-solution_rank000000.txt
     python -m flecsale_amd.apps maire_hydro_2d -m sedov_32x32.g   # sedov_2d-solution_rank000000.txt
+    python -m flecsale_amd.apps maire_hydro_3d -m sedov_20x20x20.g   # sedov_3d-solution_rank000000.txt
     python -m flecsale_amd.apps hydro_2d -m square_32x32.g -f shock_box_2d.lua   # the problem from a Lua input
     python -m flecsale_amd.numdiff -a 2e-14 <golden>.std shock_box_2d-solution_rank000000.txt
 
 hydro_2d / hydro_3d: apps/hydro/driver.h:237-450 with apps/hydro/{2d,3d}/
 inputs.cc; maire_hydro_2d: apps/maire_hydro/driver.h:330-566 with
 apps/maire_hydro/2d/inputs_regression.cc (the maire_hydro_2d_regression
-target).  The dump name is the app's prefix + "-solution" with the rank
+target); maire_hydro_3d: the same driver with apps/maire_hydro/3d/inputs.cc
+(whose test, apps/maire_hydro/3d/CMakeLists.txt:24-32, runs to completion
+without a golden).  The dump name is the app's prefix + "-solution" with the rank
 (apps/hydro/driver.h:441-443, apps/hydro/tasks.h:789-794; one rank here).
 """
 import argparse
@@ -28,6 +31,7 @@ APPS = {
     "hydro_2d": ("shock_box_2d", fa.SOLVER_EULER, inputs.shock_box_2d),
     "hydro_3d": ("shock_box_3d", fa.SOLVER_EULER, inputs.shock_box_3d),
     "maire_hydro_2d": ("sedov_2d", fa.SOLVER_MAIRE, inputs.sedov_2d),
+    "maire_hydro_3d": ("sedov_3d", fa.SOLVER_MAIRE, inputs.sedov_3d),
 }
 
 
@@ -61,6 +65,9 @@ def run(app, mesh_path, out_dir=".", input_file=None):
         centroids = ctx.get(fa.F_CELL_CENTROID)
         if input_file:
             prefix, prob = _lua_problem(input_file, coords, cells, centroids)
+        elif app == "maire_hydro_3d":
+            # sedov_3d sizes its energy deposit by each cell's volume
+            prob = problem(coords, cells, centroids, ctx.get(fa.F_CELL_VOLUME))
         else:
             prob = problem(coords, cells, centroids)
         if solver == fa.SOLVER_EULER:

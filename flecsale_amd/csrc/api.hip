@@ -448,6 +448,8 @@ struct Stager {
   }
 };
 
+inline int vt_popcount(unsigned x) { return (int)(x & 1) + (int)((x >> 1) & 1) + (int)((x >> 2) & 1); }
+
 int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, long long cap_c, long long cap_f,
                 long long cap_v, long long cap_n) {
   if (!((dim == 2 && vpc == 4) || (dim == 3 && vpc == 8))) {
@@ -627,6 +629,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
                  8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
   bytes += 4 * (size_t)nc;   // original ids
   if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + 2 * (size_t)nc * t.vpc + nv + 64 * t.vpc;
+  if (maire && m.dim == 3) bytes += 32 * (size_t)nc + 64;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
   // k-rows of connectivity, filled on the row pool
   auto rows = [&](int32_t *dst_base, int rows_n, size_t stride, int n, auto value) -> int {
@@ -709,6 +712,13 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     if ((rc = rows(m.vc_cnt + v0, 1, 0, nv,
                    [&t, vidx](int, int s) { return t.vc_off[vidx[s] + 1] - t.vc_off[vidx[s]]; })))
       return rc;
+    if (m.dim == 3) {
+      // the 3D nodal solve's packed cell -> vertex rows, [cell][8]
+      if ((rc = st.defer(ctx->ma.cv8 + 8 * (size_t)c0, 8 * (size_t)nc, [&t, cidx, vpos, v0](int32_t *r, size_t n) {
+             for (size_t i = 0; i < n; ++i) r[i] = vpos[t.cv[(size_t)cidx[i / 8] * 8 + i % 8]] + v0;
+           })))
+        return rc;
+    }
     if ((rc = rows(m.vck + v0, kMaxCornersPerVertex, m.cap_v, nv, [&t, vidx, cpos, c0](int j, int s) {
            const int v = vidx[s];
            if (j >= t.vc_off[v + 1] - t.vc_off[v]) return -1;
@@ -718,7 +728,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
          })))
       return rc;
     // face tags: bit 0-2 = |n_x|, |n_y|, |n_z| is 1, bit 3 = boundary face
-    std::vector<uint8_t> ftag(nf, 0), vb(nv, 0);
+    std::vector<uint8_t> ftag(nf, 0), vb(nv, 0), vtags(nv, 0);
     const double *xs = mesh.coords;
     for (int f = 0; f < nf; ++f) {
       if (t.fc[2 * (size_t)f + 1] >= 0) continue;
@@ -738,7 +748,10 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
       for (int d = 0; d < m.dim; ++d)
         if (std::fabs(std::fabs(n[d]) - 1.0) <= 1e-14) tag |= (uint8_t)(1u << d);
       ftag[f] = tag;
-      for (int i = 0; i < t.vpf; ++i) vb[t.fv[(size_t)f * t.vpf + i]] = 1;
+      for (int i = 0; i < t.vpf; ++i) {
+        vb[t.fv[(size_t)f * t.vpf + i]] = 1;
+        vtags[t.fv[(size_t)f * t.vpf + i]] |= (uint8_t)(tag & 7);
+      }
     }
     // corner nibbles: 2D (face of edge k-1 -> k, face of edge k -> k+1) with
     // the boundary bit at 2; 3D the three sides holding k, in side order
@@ -775,13 +788,17 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
       nb += vbnd[sp];
     }
     if ((rc = st.put(m.vbnd + v0, vbnd, nv))) return rc;
-    // the boundary vertices (storage positions, ascending) for the
-    // symmetry-solve launch
+    // the boundary vertices (storage positions) for the symmetry-solve
+    // launch, grouped by their number of symmetry tags (the size of their QR
+    // system), ascending within a group: a wave then takes one solve path
     int32_t *bvl = st.slot<int32_t>(nb > 0 ? nb : 1);
     if (!bvl) return FSALE_ERR_ARG;
     nb = 0;
-    for (int sp = 0; sp < nv; ++sp)
-      if (vbnd[sp]) bvl[nb++] = sp + v0;
+    for (int ns = 0; ns <= 3; ++ns)
+      for (int sp = 0; sp < nv; ++sp) {
+        const int vt = vt_popcount(vtags[vidx[sp]]);
+        if (vbnd[sp] && vt == ns) bvl[nb++] = sp + v0;
+      }
     if ((rc = st.put(m.bv + v0, bvl, nb))) return rc;
     ctx->probs[prob].nb = nb;
   }

@@ -180,6 +180,12 @@ struct MaireFields {
   double *npc[3] = {}, *Fpc[3] = {};   // corner arrays [vpc][cap_c]
   double *dudt[5] = {};                // volume, momentum[dim], energy
   double *partials = nullptr;          // per cell tile and wave: max a/min_length, max |dV/dt|/V
+  // 3D: what the nodal solve gathers, packed per entity so that a corner's
+  // gathers touch one cache line per cell or vertex instead of one per array
+  // (xa and cs refreshed from the arrays above before every nodal launch)
+  int32_t *cv8 = nullptr;              // [cap_c][8] cell vertices (batch ids), written with the mesh
+  double *xa = nullptr;                // [cap_v][4] coordinates (x, y, z, 0)
+  double *cs = nullptr;                // [cap_c][6] (rho a, p, v, 0)
 };
 
 // Kernel arguments common to all step kernels

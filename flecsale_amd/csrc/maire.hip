@@ -169,39 +169,45 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
 }
 
 // ------------------------------------------------------------ 3D corners
-// For local vertex k of a hex, its three sides in side order, each packed as
-// side | next << 8 | opposite << 16 | previous << 24: the local vertices
-// after, opposite and before k in the side's outward list (Exodus hex8
-// sides {0,1,5,4} {1,2,6,5} {2,3,7,6} {0,4,7,3} {0,3,2,1} {4,5,6,7}).
-__constant__ uint32_t kHexRing[8][3] = {
-    {0x04050100, 0x03070403, 0x01020304}, {0x00040500, 0x05060201, 0x02030004},
-    {0x01050601, 0x06070302, 0x03000104}, {0x02060702, 0x07040003, 0x00010204},
-    {0x05010000, 0x00030703, 0x07060505}, {0x01000400, 0x06020101, 0x04070605},
-    {0x02010501, 0x07030202, 0x05040705}, {0x03020602, 0x04000303, 0x06050405}};
+// For local vertex k of a hex, its three sides j in side order (Exodus hex8
+// sides {0,1,5,4} {1,2,6,5} {2,3,7,6} {0,4,7,3} {0,3,2,1} {4,5,6,7}): the
+// local vertices after, opposite and before k in side j's outward list,
+// nibble k of kRingNext[j], kRingOpp[j], kRingPrev[j].
+constexpr uint32_t kRingNext[3] = {0x65407651u, 0x32170324u, 0x47652103u};
+constexpr uint32_t kRingOpp[3] = {0x21016545u, 0x03234767u, 0x54761032u};
+constexpr uint32_t kRingPrev[3] = {0x32152104u, 0x47607653u, 0x65470321u};
 
-// The six wedge facets of 3D corner (c, k), in order: for each side j
-// holding k, the triangles (x_k, m+, x_s) and (x_k, x_s, m-), m+ / m- the
-// midpoints of the side's edges after / before k and x_s the side's vertex
-// average summed as ((a0 + a2) + (a1 + a3)) (the same bits from both cells
-// of an interior side).  f(j, n, l) receives each facet's unit normal and
-// area (burton's wedge facet_normal / facet_area).
+// The six wedge facets of 3D corner (c, k) at vertex ik, in order: for each
+// side j holding k, the triangles (x_k, m+, x_s) and (x_k, x_s, m-), m+ / m-
+// the midpoints of the side's edges after / before k and x_s the side's
+// vertex average summed as ((a0 + a2) + (a1 + a3)) (the same bits from both
+// cells of an interior side).  f(j, n, l) receives each facet's unit normal
+// and area (burton's wedge facet_normal / facet_area; the normal is the area
+// vector times the reciprocal of its length).
 template <class F>
-__device__ inline void for_each_facet3(const DevMesh &m, int c, int k, F &&f) {
-  const int ik = m.cv[(size_t)k * m.cap_c + c];
-  const double xk[3] = {m.x[0][ik], m.x[1][ik], m.x[2][ik]};
+__device__ inline void for_each_facet3(const MaireFields &u, int c, int k, int ik, F &&f) {
+  const int32_t *row = u.cv8 + 8 * (size_t)c;
+  auto coords = [&](int v, double x[3]) {
+    const double4 t = reinterpret_cast<const double4 *>(u.xa)[v];
+    x[0] = t.x;
+    x[1] = t.y;
+    x[2] = t.z;
+  };
+  double xk[3];
+  coords(ik, xk);
+  const int sh = 4 * k;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const uint32_t r = kHexRing[k][j];
-    const int in = m.cv[(size_t)((r >> 8) & 0xff) * m.cap_c + c];
-    const int io = m.cv[(size_t)((r >> 16) & 0xff) * m.cap_c + c];
-    const int ip = m.cv[(size_t)(r >> 24) * m.cap_c + c];
+    double xn[3], xo[3], xp[3];
+    coords(row[(kRingNext[j] >> sh) & 0xf], xn);
+    coords(row[(kRingOpp[j] >> sh) & 0xf], xo);
+    coords(row[(kRingPrev[j] >> sh) & 0xf], xp);
     double xs[3], mp[3], mm[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double xn = m.x[d][in], xo = m.x[d][io], xp = m.x[d][ip];
-      xs[d] = ((xk[d] + xo) + (xn + xp)) * 0.25;
-      mp[d] = 0.5 * (xk[d] + xn);
-      mm[d] = 0.5 * (xp + xk[d]);
+      xs[d] = ((xk[d] + xo[d]) + (xn[d] + xp[d])) * 0.25;
+      mp[d] = 0.5 * (xk[d] + xn[d]);
+      mm[d] = 0.5 * (xp[d] + xk[d]);
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -215,8 +221,9 @@ __device__ inline void for_each_facet3(const DevMesh &m, int c, int k, F &&f) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) t[d] = 0.5 * t[d];
       const double len = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      const double inv = 1.0 / len;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) n[d] = t[d] / len;
+      for (int d = 0; d < 3; ++d) n[d] = t[d] * inv;
       f(j, n, len);
     }
   }
@@ -225,26 +232,38 @@ __device__ inline void for_each_facet3(const DevMesh &m, int c, int k, F &&f) {
 // corner matrix M_pc = sum_w z_c l_w n_w n_w^T, (l n)_pc = sum_w l_w n_w and
 // the corner force before the nodal velocity, F_pc = M_pc u_c + p_c (l n)_pc
 // (apps/maire_hydro/tasks.h:263-303)
-__device__ inline void corner3(const DevMesh &m, const MaireFields &u, int c, int k, double M[3][3], double np[3],
-                               double F[3]) {
-  const double zc = u.d[c] * u.a[c];
+// KEEP: also the facets' l n products in facet order (the boundary solve's
+// symmetry normals are sums of them)
+template <bool KEEP = false>
+__device__ inline void corner3(const MaireFields &u, int c, int k, int vt, double M[3][3], double np[3],
+                               double F[3], double (*ln)[3] = nullptr) {
+  const double2 s0 = reinterpret_cast<const double2 *>(u.cs)[3 * (size_t)c];
+  const double2 s1 = reinterpret_cast<const double2 *>(u.cs)[3 * (size_t)c + 1];
+  const double2 s2 = reinterpret_cast<const double2 *>(u.cs)[3 * (size_t)c + 2];
+  const double zc = s0.x;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     np[r] = 0;
 #pragma unroll
     for (int q = 0; q < 3; ++q) M[r][q] = 0;
   }
-  for_each_facet3(m, c, k, [&](int, const double n[3], double l) {
+  int w = 0;
+  for_each_facet3(u, c, k, vt, [&](int, const double n[3], double l) {
     const double fact = zc * l;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int q = 0; q < 3; ++q) M[r][q] += fact * n[r] * n[q];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) np[d] += l * n[d];
+    for (int d = 0; d < 3; ++d) {
+      const double t = l * n[d];
+      np[d] += t;
+      if constexpr (KEEP) ln[w][d] = t;
+    }
+    ++w;
   });
-  const double uc[3] = {u.v[0][c], u.v[1][c], u.v[2][c]};
-  const double pc = u.p[c];
+  const double uc[3] = {s1.x, s1.y, s2.x};
+  const double pc = s0.y;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     F[r] = 0;
@@ -253,6 +272,28 @@ __device__ inline void corner3(const DevMesh &m, const MaireFields &u, int c, in
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) F[d] += pc * np[d];
+}
+
+// refreshes the packed gathers of the 3D nodal solve from the cell and
+// vertex arrays: z_c = rho_c a_c (the product the corner matrix takes), p_c,
+// v_c; the coordinates (the packed cell -> vertex rows are written with the
+// mesh)
+__global__ void __launch_bounds__(kBlock) k_maire3_pack(DevMesh m, MaireFields u, Batch b) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!wave_live(b.s, tile.prob, 0)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const int c = tile.first + threadIdx.x;
+    double2 *cs = reinterpret_cast<double2 *>(u.cs) + 3 * (size_t)c;
+    cs[0] = make_double2(u.d[c] * u.a[c], u.p[c]);
+    cs[1] = make_double2(u.v[0][c], u.v[1][c]);
+    cs[2] = make_double2(u.v[2][c], 0.0);
+  });
+  for_each_tile(b.vt, [&](const Tile &tile) {
+    if (!wave_live(b.s, tile.prob, 0)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const int i = tile.first + threadIdx.x;
+    reinterpret_cast<double4 *>(u.xa)[i] = make_double4(m.x[0][i], m.x[1][i], m.x[2][i], 0.0);
+  });
 }
 
 // ristra::math::solve (un-vendored) of the 3x3 nodal system: adjugate over
@@ -288,59 +329,135 @@ __device__ inline void sym_solve3(const double Mp[3][3], const double rhs[3], co
   for (int d = 0; d < 3; ++d) un[d] = b[d];
 }
 
-// evaluate_nodal_state for one 3D vertex (apps/maire_hydro/tasks.h:249-448):
-// corner matrices summed in corner order, the 3x3 solve inside, the
-// symmetry-constrained QR solve on the boundary (rows of the x / y / z tags
-// present, normals summed over the vertex's boundary wedges in corner and
-// facet order, :338-418), then F_pc -= M_pc u_p.  The corner matrices are
-// recomputed for the scatter instead of being held (eight corners of nine
-// entries would not fit the registers).
+// evaluate_nodal_state for 3D vertices (apps/maire_hydro/tasks.h:249-448),
+// eight lanes per vertex: lane q of a group builds corner q's matrix, normal
+// and force once and posts the matrix and force in LDS; lane q then sums
+// entries q and q + 8 of the twelve over the vertex's corners in corner
+// order (the sequential sums, bit for bit), every lane of the group reads
+// the twelve sums, solves (the 3x3 system inside; on the boundary the QR of
+// [M_p N^T; N 0] with the rows of the x / y / z tags present, normals summed
+// over the vertex's boundary wedges in corner and facet order, :338-418),
+// and finishes its own corner, F_pc -= M_pc u_p.  cnt = 0: no vertex; such
+// lanes still take part in the exchanges (every lane of the block calls this
+// together: it holds block barriers).
+constexpr int kGroups3 = kBlock / kMaxCornersPerVertex;   // vertices per pass of a block
+
+struct Nodal3Lds {
+  double corner[kGroups3][kMaxCornersPerVertex][12];   // M (row-major), F
+  double sum[kGroups3][12];
+};
+struct Nodal3BndLds {
+  double ln[kGroups3][kMaxCornersPerVertex][18];       // l n of the six facets
+  int tags[kGroups3][kMaxCornersPerVertex];
+  double sn[kGroups3][9];                              // symmetry normals [tag][d]
+  int has[kGroups3][3];
+};
+
 template <bool BOUNDARY>
-__device__ inline void nodal3_vertex(const DevMesh &m, const MaireFields &u, int vt) {
-  const int cnt = m.vc_cnt[vt];
-  double Mp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rhs[3] = {0, 0, 0};
-#pragma unroll 1
-  for (int q = 0; q < cnt; ++q) {
+__device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int vt, int cnt, int g, int q,
+                                    Nodal3Lds &L, Nodal3BndLds *B) {
+  const bool mine = q < cnt;
+  double M[3][3], np[3], F[3], ln[6][3];
+  int c = 0, k = 0;
+  if (mine) {
     const int cn = m.vck[(size_t)q * m.cap_v + vt];
-    double M[3][3], np[3], F[3];
-    corner3(m, u, cn >> 3, cn & 7, M, np, F);
+    c = cn >> 3;
+    k = cn & 7;
+    corner3<BOUNDARY>(u, c, k, vt, M, np, F, ln);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      np[r] = F[r] = 0;
+#pragma unroll
+      for (int s2 = 0; s2 < 3; ++s2) M[r][s2] = 0;
+    }
+#pragma unroll
+    for (int w = 0; w < 6; ++w)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) ln[w][d] = 0;
+  }
+  {
+    double *row = L.corner[g][q];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
-      for (int c2 = 0; c2 < 3; ++c2) Mp[r][c2] += M[r][c2];
-      rhs[r] += F[r];
+      for (int s2 = 0; s2 < 3; ++s2) row[3 * r + s2] = M[r][s2];
+      row[9 + r] = F[r];
     }
   }
-  double un[3];
   if constexpr (BOUNDARY) {
-    int has = 0;   // bit t: tag t seen
-    double sn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double *row = B->ln[g][q];
+#pragma unroll
+    for (int w = 0; w < 6; ++w)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) row[3 * w + d] = ln[w][d];
+    B->tags[g][q] = mine ? (int)m.cwtag[(size_t)k * m.cap_c + c] : 0;
+  }
+  __syncthreads();
+  // entries q and q + 8 summed over the corners in order; the sums start at
+  // +0 and never become -0, so adding +0 for a missing corner is exact
+  {
+    double s0 = 0, s1 = 0;
+#pragma unroll
+    for (int j = 0; j < kMaxCornersPerVertex; ++j) {
+      const double a0 = L.corner[g][j][q];
+      const double a1 = L.corner[g][j][q < 4 ? q + 8 : q];   // lanes 4-7: unused
+      s0 += j < cnt ? a0 : 0.0;
+      s1 += j < cnt ? a1 : 0.0;
+    }
+    L.sum[g][q] = s0;
+    if (q < 4) L.sum[g][q + 8] = s1;
+  }
+  if constexpr (BOUNDARY) {
+    // symmetry normal entry (t, d) = P / 3, P % 3 for P = q and, on lane 0,
+    // P = 8: first boundary wedge of tag t assigns, later ones add
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      const int P = pass == 0 ? q : 8;
+      if (pass == 1 && q != 0) break;
+      const int t = P / 3, d = P - 3 * t;
+      double sum = 0;
+      int has = 0;
 #pragma unroll 1
-    for (int q = 0; q < cnt; ++q) {
-      const int cn = m.vck[(size_t)q * m.cap_v + vt];
-      const int c = cn >> 3, k = cn & 7;
-      const int tags = m.cwtag[(size_t)k * m.cap_c + c];
-      if (!(tags & 0x888)) continue;
-      for_each_facet3(m, c, k, [&](int j, const double n[3], double l) {
-        const int tg = (tags >> (4 * j)) & 0xf;
-        if (!(tg & 8)) return;
+      for (int j = 0; j < kMaxCornersPerVertex; ++j) {
+        if (j >= cnt) break;
+        const int tj = B->tags[g][j];
+        if (!(tj & 0x888)) continue;
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
-          if (!(tg & (1 << t))) continue;
-          if (has & (1 << t)) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) sn[t][d] += l * n[d];
+        for (int w = 0; w < 6; ++w) {
+          const int tg = (tj >> (4 * (w >> 1))) & 0xf;
+          if (!(tg & 8) || !(tg & (1 << t))) continue;
+          const double v = B->ln[g][j][3 * w + d];
+          if (has) {
+            sum += v;
           } else {
-            has |= 1 << t;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) sn[t][d] = l * n[d];
+            sum = v;
+            has = 1;
           }
         }
-      });
+      }
+      B->sn[g][P] = sum;
+      if (d == 0) B->has[g][t] = has;
     }
+  }
+  __syncthreads();
+  double Mp[3][3], rhs[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int s2 = 0; s2 < 3; ++s2) Mp[r][s2] = L.sum[g][3 * r + s2];
+    rhs[r] = L.sum[g][9 + r];
+  }
+  double un[3] = {0, 0, 0};
+  if constexpr (BOUNDARY) {
+    double sn[3][3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sn[t][d] = B->sn[g][3 * t + d];
+    const bool h0 = B->has[g][0], h1 = B->has[g][1], h2 = B->has[g][2];
+    const int ns = (int)h0 + (int)h1 + (int)h2;
     // the rows of the tags present, in tag order
-    const bool h0 = has & 1, h1 = has & 2;
-    const int ns = (has & 1) + ((has >> 1) & 1) + ((has >> 2) & 1);
     double rows[3][3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -348,26 +465,25 @@ __device__ inline void nodal3_vertex(const DevMesh &m, const MaireFields &u, int
       rows[1][d] = (h0 && h1) ? sn[1][d] : sn[2][d];
       rows[2][d] = sn[2][d];
     }
-    if (ns == 0)
-      solve3(Mp, rhs, un);
-    else if (ns == 1)
-      sym_solve3<1>(Mp, rhs, rows, un);
-    else if (ns == 2)
-      sym_solve3<2>(Mp, rhs, rows, un);
-    else
-      sym_solve3<3>(Mp, rhs, rows, un);
+    if (cnt > 0) {
+      if (ns == 0)
+        solve3(Mp, rhs, un);
+      else if (ns == 1)
+        sym_solve3<1>(Mp, rhs, rows, un);
+      else if (ns == 2)
+        sym_solve3<2>(Mp, rhs, rows, un);
+      else
+        sym_solve3<3>(Mp, rhs, rows, un);
+    }
   } else {
-    solve3(Mp, rhs, un);
+    if (cnt > 0) solve3(Mp, rhs, un);
   }
+  if (cnt > 0 && q == 0) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) u.un[d][vt] = un[d];
+    for (int d = 0; d < 3; ++d) u.un[d][vt] = un[d];
+  }
   // F_pc -= M_pc u_p  (matrix_vector(-1, Mpc, un, 1, Fpc), tasks.h:435-448)
-#pragma unroll 1
-  for (int q = 0; q < cnt; ++q) {
-    const int cn = m.vck[(size_t)q * m.cap_v + vt];
-    const int c = cn >> 3, k = cn & 7;
-    double M[3][3], np[3], F[3];
-    corner3(m, u, c, k, M, np, F);
+  if (mine) {
     const size_t ci = (size_t)k * m.cap_c + c;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -377,6 +493,33 @@ __device__ inline void nodal3_vertex(const DevMesh &m, const MaireFields &u, int
       t += M[r][2] * un[2];
       u.Fpc[r][ci] = -1.0 * t + 1.0 * F[r];
     }
+  }
+  // the next pass of the block overwrites the exchange arrays
+  __syncthreads();
+}
+
+// one tile of 3D vertices, 32 per pass of the block's 256 lanes
+template <bool BOUNDARY>
+__device__ inline void nodal3_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
+  if (!wave_live(b.s, tile.prob, 0)) return;
+  __shared__ Nodal3Lds lds;
+  __shared__ typename std::conditional<BOUNDARY, Nodal3BndLds, char>::type blds;
+  constexpr int G = kMaxCornersPerVertex;
+  const int q = threadIdx.x & (G - 1), g = threadIdx.x / G;
+  for (int base = 0; base < tile.count; base += kGroups3) {
+    const int i = base + g;
+    int vt = tile.first + i, cnt = 0;
+    if (i < tile.count) {
+      if constexpr (BOUNDARY) {
+        vt = m.bv[vt];
+        cnt = m.vc_cnt[vt];
+      } else {
+        cnt = m.vbnd[vt] ? 0 : m.vc_cnt[vt];
+      }
+    }
+    Nodal3BndLds *bp = nullptr;
+    if constexpr (BOUNDARY) bp = &blds;
+    nodal3_group<BOUNDARY>(m, u, vt, cnt, g, q, lds, bp);
   }
 }
 
@@ -404,7 +547,9 @@ __device__ inline void corner_force(const MaireFields &u, int c, const double M[
 // one tile of interior (Cramer) or boundary (symmetry QR) vertices
 template <bool BOUNDARY, int N>
 __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
-  {
+  if constexpr (N == 3) {
+    nodal3_tile<BOUNDARY>(m, u, b, tile);
+  } else {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
     int vt = tile.first + threadIdx.x;
@@ -412,10 +557,6 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       vt = m.bv[vt];
     else if (m.vbnd[vt])
       return;
-    if constexpr (N == 3) {
-      nodal3_vertex<BOUNDARY>(m, u, vt);
-      return;
-    }
     const int cnt = m.vc_cnt[vt];
     double M[kNodalKeep][2][2], np[kNodalKeep][2], F[kNodalKeep][2];
     int cc[kNodalKeep], kk[kNodalKeep];
@@ -784,7 +925,20 @@ template <int N>
 int launch_nodal_n(fsale_ctx *ctx) {
   Batch bi = batch_args(ctx, K_MAIRE_NODAL);
   const Batch bb = batch_args(ctx, K_MAIRE_NODAL_BND);
-  if (N == 2 && bi.vt.direct && bb.bt.direct) {
+  if constexpr (N == 3) {
+    // the packing is timed with the interior launch as the nodal kernel
+    {
+      ProfScope prof(ctx, K_MAIRE_NODAL);
+      hipLaunchKernelGGL(k_maire3_pack, dim3(persistent_grid(ctx, K_MAIRE_COPY)), dim3(kBlock), 0, ctx->stream,
+                         ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_COPY));
+      FSALE_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_maire_nodal<false, 3>), dim3(persistent_grid(ctx, K_MAIRE_NODAL)), dim3(kBlock), 0,
+                         ctx->stream, ctx->m, ctx->ma, bi);
+      FSALE_HIP(hipGetLastError());
+    }
+    return launch_persistent(ctx, K_MAIRE_NODAL_BND, k_maire_nodal<true, 3>, ctx->m, ctx->ma, bb);
+  }
+  if (bi.vt.direct && bb.bt.direct) {
     bi.bt = bb.bt;
     const int nb = (bb.bt.ndirect + 7) / 8 * 8;
     ProfScope prof(ctx, K_MAIRE_NODAL);
@@ -884,6 +1038,11 @@ int maire_alloc(fsale_ctx *ctx) {
   for (int i = 0; i < m.dim + 2; ++i)
     if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &u.partials, 2 * (size_t)kWaves * (ctx->ra_ct.capacity + 1)))) return rc;
+  if (m.dim == 3) {
+    if ((rc = dev_alloc_n(ctx, &u.cv8, (size_t)m.cap_c * 8 + 8))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.xa, (size_t)m.cap_v * 4 + 8))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.cs, (size_t)m.cap_c * 6 + 8))) return rc;
+  }
   return FSALE_OK;
 }
 

@@ -34,7 +34,8 @@
  *     the triangles (x_k, m+, x_s) then (x_k, x_s, m-), m+ / m- the midpoints
  *     of edges (a_i, a_{i+1}) / (a_{i-1}, a_i) and x_s = ((a_0 + a_2) +
  *     (a_1 + a_3)) / 4 (the same bits from both cells of the side): facet area
- *     vector = cross / 2, facet_area its length, facet_normal its direction;
+ *     vector = cross / 2, facet_area its length l, facet_normal the vector
+ *     times the reciprocal 1 / l;
  *     ristra::math::solve (un-vendored) of the 3x3 nodal system is the
  *     adjugate over the determinant (pinned end-to-end: a 3D run of the
  *     sedov_2d problem extruded in z reproduces the sedov_2d golden);
@@ -992,7 +993,8 @@ static void wedge_facets_3d(const om_mesh *m, int c, int k, double n[6][3], doub
       cross3(u, v, t);
       for (int d = 0; d < 3; ++d) t[d] = 0.5 * t[d];
       double len = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-      for (int d = 0; d < 3; ++d) n[w][d] = t[d] / len;
+      double inv = 1.0 / len;
+      for (int d = 0; d < 3; ++d) n[w][d] = t[d] * inv;
       l[w] = len;
       lside[w] = s;
       ++w;

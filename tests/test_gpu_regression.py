@@ -47,3 +47,32 @@ def test_regression_dump_matches_golden(app, mesh, lua, golden, tmp_path):
     assert os.path.basename(dump) == golden.rsplit("_", 1)[0] + "-solution_rank000000.txt"
     failures = numdiff.compare(std, dump, abs_tol=2e-14)
     assert failures == [], failures[:5]
+
+
+def test_maire_hydro_3d_smoke_case(oracle_lib, tmp_path):
+    """flecsale_sedov_3d_1procs (apps/maire_hydro/3d/CMakeLists.txt:24-32) runs
+    sedov_3d on sedov_20x20x20.g to completion with no golden; here the dump
+    also equals the oracle's state printed the same way."""
+    import numpy as np
+    from conftest import load_mesh
+    from flecsale_amd import inputs
+    dump = apps.run("maire_hydro_3d", os.path.join(EXO, "sedov_20x20x20.g"), str(tmp_path))
+    assert os.path.basename(dump) == "sedov_3d-solution_rank000000.txt"
+    O = oracle_lib
+    coords, cells = load_mesh("sedov_20x20x20")
+    mesh = O.Mesh(coords, cells)
+    geo = mesh.geometry()
+    prob = inputs.sedov_3d(coords, cells, geo["cell_centroid"], geo["cell_volume"])
+    s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
+    rc, t, n, dt = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], prob["max_steps"],
+                               prob["dt0"])
+    assert rc == 0 and n == 10
+    with open(dump) as f:
+        lines = f.read().splitlines()
+    assert lines[0] == f"# Solution time: {t:.14e}" and lines[1] == "# Number iterations: 10"
+    start = lines.index("# BEGIN CELLS") + 3
+    rows = np.array([[float(x) for x in ln.split()] for ln in lines[start:start + mesh.nc]])
+    assert rows.shape == (8000, 11)
+    # columns: ids, centroid(3), density, internal energy, pressure, velocity(3)
+    for col, want in [(5, s.d), (6, s.e), (7, s.p), (8, s.v[:, 0]), (9, s.v[:, 1]), (10, s.v[:, 2])]:
+        assert np.allclose(rows[:, col], want, rtol=1e-13, atol=0), col
