@@ -25,6 +25,8 @@ LIB_PATH = os.path.join(HERE, "libflecsale_amd.so")
 FLUX_HLLE, FLUX_RUSANOV, FLUX_HLLC = 0, 1, 2
 BOUNDARY_WALL, BOUNDARY_TRANSMISSIVE = 0x00, 0x10   # OR-ed into flux_kind
 SOLVER_EULER, SOLVER_MAIRE = 0, 1
+# Maire boundary condition kinds (fsale_maire_set_boundary)
+BC_SYMMETRY, BC_PRESSURE, BC_VELOCITY, BC_NONE = 0, 1, 2, 3
 
 (F_DENSITY, F_VELOCITY, F_INTERNAL_ENERGY, F_PRESSURE, F_TEMPERATURE, F_SOUND_SPEED,
  F_GRAD_DENSITY, F_GRAD_VELOCITY, F_GRAD_ENERGY, F_GRAD_PRESSURE, F_GRAD_SOUND_SPEED,
@@ -57,6 +59,7 @@ EXPORTS = [
     "fsale_eul_initial_conditions", "fsale_eul_evaluate_time_step", "fsale_eul_evaluate_gradients",
     "fsale_eul_evaluate_fluxes", "fsale_eul_apply_update", "fsale_eul_advance",
     "fsale_maire_initial_conditions", "fsale_maire_evaluate_nodal_state", "fsale_maire_evaluate_residual",
+    "fsale_maire_set_boundary", "fsale_batch_set_maire_boundary",
     "fsale_maire_evaluate_time_step", "fsale_maire_apply_update", "fsale_maire_save", "fsale_maire_restore",
     "fsale_maire_advance",
     "fsale_batch_create", "fsale_batch_add_euler", "fsale_batch_add_maire", "fsale_batch_remove",
@@ -155,6 +158,8 @@ def lib():
         L.fsale_eul_advance.argtypes = [vp, Eos, dp, dp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]
         L.fsale_maire_initial_conditions.argtypes = [vp, dp, vp, vp, vp, vp]
         L.fsale_maire_evaluate_nodal_state.argtypes = [vp]
+        L.fsale_maire_set_boundary.argtypes = [vp, ct.c_int, ct.c_int, vp]
+        L.fsale_batch_set_maire_boundary.argtypes = [vp, i32, ct.c_int, ct.c_int, vp]
         L.fsale_maire_evaluate_residual.argtypes = [vp]
         L.fsale_maire_evaluate_time_step.argtypes = [vp, dp, dp, dp, dp, ct.POINTER(dp)]
         L.fsale_maire_apply_update.argtypes = [vp, dp]
@@ -410,6 +415,15 @@ class MaireContext(_Context):
         _check(lib().fsale_maire_initial_conditions(self.h, cv, g.ctypes.data, d.ctypes.data,
                                                     v.ctypes.data, p.ctypes.data))
 
+    def set_boundary(self, tag, kind, value=None):
+        """Boundary condition of the faces of symmetry tag `tag` (0 x, 1 y, 2 z):
+        BC_SYMMETRY, BC_PRESSURE (value = p), BC_VELOCITY (value = u), BC_NONE."""
+        v = np.zeros(3)
+        if value is not None:
+            a = np.atleast_1d(np.asarray(value, dtype=np.float64))
+            v[:len(a)] = a
+        _check(lib().fsale_maire_set_boundary(self.h, int(tag), int(kind), v.ctypes.data))
+
     def evaluate_nodal_state(self):
         _check(lib().fsale_maire_evaluate_nodal_state(self.h))
 
@@ -507,6 +521,14 @@ class Batch:
     def remove(self, problem):
         _check(lib().fsale_batch_remove(self.h, int(problem)))
         self.meshes.pop(problem, None)
+
+    def set_maire_boundary(self, problem, tag, kind, value=None):
+        """MaireContext.set_boundary for one resident problem."""
+        v = np.zeros(3)
+        if value is not None:
+            a = np.atleast_1d(np.asarray(value, dtype=np.float64))
+            v[:len(a)] = a
+        _check(lib().fsale_batch_set_maire_boundary(self.h, int(problem), int(tag), int(kind), v.ctypes.data))
 
     def step(self, nsteps, use_graph=True):
         _check(lib().fsale_batch_step(self.h, int(nsteps), int(use_graph)))

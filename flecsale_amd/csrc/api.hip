@@ -705,6 +705,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   }
   if ((rc = st.run_deferred())) return rc;
   if ((rc = launch_geometry(ctx, prob, true, faces, ctx->upload))) return rc;
+  if ((rc = maire3_pack_problem(ctx, prob, ctx->upload))) return rc;
   if (maire) {
     // vertex -> corner lists and boundary tags from the initial face normals:
     // tag d when |n_d| is 1 within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162);
@@ -1156,8 +1157,10 @@ int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in
     }
   }
   ctx->probs[prob].partials_valid = false;
-  if (field == FSALE_F_COORDS) return launch_geometry(ctx, prob, true, ctx->probs[prob].faces_on_device, ctx->stream);
-  return FSALE_OK;
+  if (field == FSALE_F_COORDS &&
+      (rc = launch_geometry(ctx, prob, true, ctx->probs[prob].faces_on_device, ctx->stream)))
+    return rc;
+  return maire3_pack_problem(ctx, prob, ctx->stream);
 }
 
 // d, v, p of one problem from host arrays (v interleaved)
@@ -1738,6 +1741,12 @@ int fsale_batch_add_maire(fsale_ctx *ctx, const fsale_mesh_desc *mesh, double cv
   P.have_state = true;
   *problem = prob;
   return FSALE_OK;
+}
+
+int fsale_batch_set_maire_boundary(fsale_ctx *ctx, int32_t problem, int tag, int kind, const double *value) {
+  if (!ctx || ctx->single || ctx->solver != FSALE_SOLVER_MAIRE || !valid_prob(ctx, problem)) return FSALE_ERR_ARG;
+  int rc = set_maire_boundary(ctx, problem, tag, kind, value);
+  return rc ? rc : upload_params(ctx, problem);
 }
 
 int fsale_batch_remove(fsale_ctx *ctx, int32_t problem) {

@@ -145,6 +145,10 @@ struct ProbParams {
   int kind, first_order, stop_at_final, pad;
   int tile_first, tile_count;              // the problem's cell tiles (dt partials)
   int cell_first, cell_count;
+  // Maire boundary conditions of the faces of each symmetry tag (x, y, z):
+  // FSALE_BC_* kind, pressure in bc_val[t][0] or velocity in bc_val[t][:dim]
+  int bc_kind[3], bc_pad;
+  double bc_val[3][3];
 };
 
 // Per-problem scalars that stay on the device between tasks (the dt future,
@@ -182,7 +186,8 @@ struct MaireFields {
   double *partials = nullptr;          // per cell tile and wave: max a/min_length, max |dV/dt|/V
   // 3D: what the nodal solve gathers, packed per entity so that a corner's
   // gathers touch one cache line per cell or vertex instead of one per array
-  // (xa and cs refreshed from the arrays above before every nodal launch)
+  // (xa and cs written by every kernel that writes their sources, and by
+  // maire3_pack_problem after host writes)
   int32_t *cv8 = nullptr;              // [cap_c][8] cell vertices (batch ids), written with the mesh
   double *xa = nullptr;                // [cap_v][4] coordinates (x, y, z, 0)
   double *cs = nullptr;                // [cap_c][6] (rho a, p, v, 0)
@@ -317,6 +322,10 @@ int dev_alloc_n(fsale_ctx *ctx, T **p, size_t n) {
   return dev_alloc(ctx, reinterpret_cast<void **>(p), n * sizeof(T) + 16);
 }
 int read_scalars(fsale_ctx *ctx);      // sync + copy DevScalars to h_s (and lift slot quarantines)
+int set_maire_boundary(fsale_ctx *ctx, int prob, int tag, int kind, const double *value);
+// 3D Lagrangian contexts: refreshes the nodal solve's packed copies of one
+// problem's cell state and coordinates (no-op otherwise)
+int maire3_pack_problem(fsale_ctx *ctx, int prob, hipStream_t stream);
 
 // Retirement of finished problems while the next round runs (serving loop).
 // stage_field_downloads enqueues, on the upload stream and after all work

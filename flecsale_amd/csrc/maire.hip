@@ -25,6 +25,19 @@ namespace {
 
 constexpr double kMinSoundSpeed = 1.e-6;   // flecsale/eqns/lagrange_eqns.h:54
 
+// 3D: the packed copy of what the nodal solve gathers per cell, (rho a, p,
+// v), written by every kernel that writes one of them
+__device__ inline void put_cs(const MaireFields &u, int c, double d, double a, double p, double v0, double v1,
+                              double v2) {
+  double2 *cs = reinterpret_cast<double2 *>(u.cs) + 3 * (size_t)c;
+  cs[0] = make_double2(d * a, p);
+  cs[1] = make_double2(v0, v1);
+  cs[2] = make_double2(v2, 0.0);
+}
+__device__ inline void put_xa(const MaireFields &u, int v, double x, double y, double z) {
+  reinterpret_cast<double4 *>(u.xa)[v] = make_double4(x, y, z, 0.0);
+}
+
 // ------------------------------------------------------------ state init
 __global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u, const ProbParams *pp, int prob) {
   const ProbParams &P = pp[prob];
@@ -43,7 +56,9 @@ __global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u, c
   u.d[c] = d;
   u.e[c] = ie;
   u.T[c] = eos_T_de(P.cv, ie);
-  u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
+  const double a = smax(eos_a_de(g, ie), kMinSoundSpeed);
+  u.a[c] = a;
+  if (m.dim == 3) put_cs(u, c, d, a, u.p[c], u.v[0][c], u.v[1][c], u.v[2][c]);
 }
 
 // wedge facets of corner (c, k): wedge 0 on edge v_{k-1} -> v_k, wedge 1 on
@@ -92,12 +107,23 @@ __device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int
   corner_matrix_from(u, c, n, l, M, np);
 }
 
-// Symmetry-constrained nodal velocity of boundary vertex vt
-// (apps/maire_hydro/tasks.h:317-420): symmetry normals accumulated per tag
-// over the vertex's boundary wedges in corner order, then the augmented system
-// [M_p N^T; N 0] solved by flecsale::linalg::qr (:383-418); Cramer's rule
-// when no wedge carries a symmetry tag.
+// Nodal velocity of boundary vertex vt (apps/maire_hydro/tasks.h:317-420)
+// under the problem's boundary conditions: the first of the vertex's tags
+// (ascending) whose condition prescribes a velocity sets u_p, with no solve
+// and no scatter (the task's `continue`); otherwise each boundary wedge, in
+// corner order, subtracts l p n from the right-hand side for a prescribed
+// pressure, or adds its l n to its tag's symmetry normal, and the augmented
+// system [M_p N^T; N 0] is solved by flecsale::linalg::qr (:383-418) --
+// Cramer's rule when no wedge carries a symmetry tag.
 constexpr int kNodalKeep = 4;
+
+// the first tag (ascending) of tag mask `vtags` whose condition prescribes
+// the velocity, or -1
+__device__ inline int velocity_tag(const ProbParams &P, int vtags, int dim) {
+  for (int t = 0; t < dim; ++t)
+    if ((vtags & (1 << t)) && P.bc_kind[t] == FSALE_BC_VELOCITY) return t;
+  return -1;
+}
 
 // keep: wedge normals, facet areas and tags of the first kNodalKeep corners,
 // already loaded by the caller
@@ -106,8 +132,27 @@ struct KeptWedges {
   int tags[kNodalKeep];
 };
 
-__device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const double Mp[2][2], const double rhs[2],
-                                      const KeptWedges &kw, double un[2]) {
+// returns false for a prescribed velocity (the vertex's corners keep
+// F_pc = M_pc u_c + p_c (l n)_pc)
+__device__ inline bool boundary_solve(const DevMesh &m, const ProbParams &P, int vt, int cnt, const double Mp[2][2],
+                                      double rhs[2], const KeptWedges &kw, double un[2]) {
+  {
+    int vtags = 0;
+#pragma unroll
+    for (int q = 0; q < kNodalKeep; ++q)
+      if (q < cnt) vtags |= kw.tags[q];
+    for (int q = kNodalKeep; q < cnt; ++q) {
+      const int cn = m.vck[(size_t)q * m.cap_v + vt];
+      vtags |= m.cwtag[(size_t)(cn & 3) * m.cap_c + (cn >> 2)];
+    }
+    // nibble w: bit 2 boundary, bits 0-1 its tags
+    const int vel = velocity_tag(P, (vtags | (vtags >> 4)) & 3, 2);
+    if (vel >= 0) {
+      un[0] = P.bc_val[vel][0];
+      un[1] = P.bc_val[vel][1];
+      return false;
+    }
+  }
   int has[2] = {0, 0};
   double symn[2][2] = {{0, 0}, {0, 0}};
   auto add_wedges = [&](int tags, const double n[2][2], const double l[2]) {
@@ -118,11 +163,17 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
 #pragma unroll
       for (int tag = 0; tag < 2; ++tag) {
         if (!(t & (1 << tag))) continue;
-        if (has[tag]) {
-          for (int d = 0; d < 2; ++d) symn[tag][d] += l[w] * n[w][d];
-        } else {
-          has[tag] = 1;
-          for (int d = 0; d < 2; ++d) symn[tag][d] = l[w] * n[w][d];
+        const int kind = P.bc_kind[tag];
+        if (kind == FSALE_BC_PRESSURE) {
+          const double fact = l[w] * P.bc_val[tag][0];
+          for (int d = 0; d < 2; ++d) rhs[d] -= fact * n[w][d];
+        } else if (kind == FSALE_BC_SYMMETRY) {
+          if (has[tag]) {
+            for (int d = 0; d < 2; ++d) symn[tag][d] += l[w] * n[w][d];
+          } else {
+            has[tag] = 1;
+            for (int d = 0; d < 2; ++d) symn[tag][d] = l[w] * n[w][d];
+          }
         }
       }
     }
@@ -146,7 +197,7 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
     const double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
     un[0] = (rhs[0] * Mp[1][1] - Mp[0][1] * rhs[1]) / det;
     un[1] = (Mp[0][0] * rhs[1] - rhs[0] * Mp[1][0]) / det;
-    return;
+    return true;
   }
   if (nsym == 1) {
     const double sn0 = has[0] ? symn[0][0] : symn[1][0];
@@ -166,6 +217,7 @@ __device__ inline void boundary_solve(const DevMesh &m, int vt, int cnt, const d
     un[0] = bb[0];
     un[1] = bb[1];
   }
+  return true;
 }
 
 // ------------------------------------------------------------ 3D corners
@@ -232,11 +284,11 @@ __device__ inline void for_each_facet3(const MaireFields &u, int c, int k, int i
 // corner matrix M_pc = sum_w z_c l_w n_w n_w^T, (l n)_pc = sum_w l_w n_w and
 // the corner force before the nodal velocity, F_pc = M_pc u_c + p_c (l n)_pc
 // (apps/maire_hydro/tasks.h:263-303)
-// KEEP: also the facets' l n products in facet order (the boundary solve's
-// symmetry normals are sums of them)
+// KEEP: also each facet's unit normal and area, nl[w] = (n, l), in facet
+// order (the boundary conditions' pressure and symmetry terms)
 template <bool KEEP = false>
 __device__ inline void corner3(const MaireFields &u, int c, int k, int vt, double M[3][3], double np[3],
-                               double F[3], double (*ln)[3] = nullptr) {
+                               double F[3], double (*nl)[4] = nullptr) {
   const double2 s0 = reinterpret_cast<const double2 *>(u.cs)[3 * (size_t)c];
   const double2 s1 = reinterpret_cast<const double2 *>(u.cs)[3 * (size_t)c + 1];
   const double2 s2 = reinterpret_cast<const double2 *>(u.cs)[3 * (size_t)c + 2];
@@ -255,10 +307,11 @@ __device__ inline void corner3(const MaireFields &u, int c, int k, int vt, doubl
 #pragma unroll
       for (int q = 0; q < 3; ++q) M[r][q] += fact * n[r] * n[q];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double t = l * n[d];
-      np[d] += t;
-      if constexpr (KEEP) ln[w][d] = t;
+    for (int d = 0; d < 3; ++d) np[d] += l * n[d];
+    if constexpr (KEEP) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) nl[w][d] = n[d];
+      nl[w][3] = l;
     }
     ++w;
   });
@@ -274,26 +327,19 @@ __device__ inline void corner3(const MaireFields &u, int c, int k, int vt, doubl
   for (int d = 0; d < 3; ++d) F[d] += pc * np[d];
 }
 
-// refreshes the packed gathers of the 3D nodal solve from the cell and
-// vertex arrays: z_c = rho_c a_c (the product the corner matrix takes), p_c,
-// v_c; the coordinates (the packed cell -> vertex rows are written with the
-// mesh)
-__global__ void __launch_bounds__(kBlock) k_maire3_pack(DevMesh m, MaireFields u, Batch b) {
-  for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!wave_live(b.s, tile.prob, 0)) return;
-    if ((int)threadIdx.x >= tile.count) return;
-    const int c = tile.first + threadIdx.x;
-    double2 *cs = reinterpret_cast<double2 *>(u.cs) + 3 * (size_t)c;
-    cs[0] = make_double2(u.d[c] * u.a[c], u.p[c]);
-    cs[1] = make_double2(u.v[0][c], u.v[1][c]);
-    cs[2] = make_double2(u.v[2][c], 0.0);
-  });
-  for_each_tile(b.vt, [&](const Tile &tile) {
-    if (!wave_live(b.s, tile.prob, 0)) return;
-    if ((int)threadIdx.x >= tile.count) return;
-    const int i = tile.first + threadIdx.x;
-    reinterpret_cast<double4 *>(u.xa)[i] = make_double4(m.x[0][i], m.x[1][i], m.x[2][i], 0.0);
-  });
+// the packed gathers of one problem's cells [c0, c0 + nc) and vertices
+// [v0, v0 + nv) from the cell and vertex arrays (after host writes; the
+// step's kernels keep them current themselves)
+__global__ void __launch_bounds__(kBlock) k_maire3_pack(DevMesh m, MaireFields u, int c0, int nc, int v0, int nv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) {
+    const int c = c0 + i;
+    put_cs(u, c, u.d[c], u.a[c], u.p[c], u.v[0][c], u.v[1][c], u.v[2][c]);
+  }
+  if (i < nv) {
+    const int v = v0 + i;
+    put_xa(u, v, m.x[0][v], m.x[1][v], m.x[2][v]);
+  }
 }
 
 // ristra::math::solve (un-vendored) of the 3x3 nodal system: adjugate over
@@ -347,17 +393,22 @@ struct Nodal3Lds {
   double sum[kGroups3][12];
 };
 struct Nodal3BndLds {
-  double ln[kGroups3][kMaxCornersPerVertex][18];       // l n of the six facets
+  double facet[kGroups3][kMaxCornersPerVertex][6][4];  // n, l of the six facets
   int tags[kGroups3][kMaxCornersPerVertex];
   double sn[kGroups3][9];                              // symmetry normals [tag][d]
   int has[kGroups3][3];
 };
 
-template <bool BOUNDARY>
-__device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int vt, int cnt, int g, int q,
-                                    Nodal3Lds &L, Nodal3BndLds *B) {
+// KIND 0: interior vertices; 1: boundary vertices with a prescribed velocity
+// or at most one symmetry row; 2: the other boundary vertices (two or three
+// symmetry rows: the 5x5 and 6x6 solves, whose registers would otherwise
+// hold down the first boundary launch's occupancy)
+template <int KIND>
+__device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, const ProbParams &P, int vt, int cnt,
+                                    int g, int q, Nodal3Lds &L, Nodal3BndLds *B) {
+  constexpr bool BOUNDARY = KIND != 0;
   const bool mine = q < cnt;
-  double M[3][3], np[3], F[3], ln[6][3];
+  double M[3][3], np[3], F[3], ln[6][4];
   int c = 0, k = 0;
   if (mine) {
     const int cn = m.vck[(size_t)q * m.cap_v + vt];
@@ -374,7 +425,7 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
 #pragma unroll
     for (int w = 0; w < 6; ++w)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) ln[w][d] = 0;
+      for (int d = 0; d < 4; ++d) ln[w][d] = 0;
   }
   {
     double *row = L.corner[g][q];
@@ -386,11 +437,10 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
     }
   }
   if constexpr (BOUNDARY) {
-    double *row = B->ln[g][q];
 #pragma unroll
     for (int w = 0; w < 6; ++w)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) row[3 * w + d] = ln[w][d];
+      for (int d = 0; d < 4; ++d) B->facet[g][q][w][d] = ln[w][d];
     B->tags[g][q] = mine ? (int)m.cwtag[(size_t)k * m.cap_c + c] : 0;
   }
   __syncthreads();
@@ -413,21 +463,21 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
     // P = 8: first boundary wedge of tag t assigns, later ones add
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
-      const int P = pass == 0 ? q : 8;
+      const int Pe = pass == 0 ? q : 8;
       if (pass == 1 && q != 0) break;
-      const int t = P / 3, d = P - 3 * t;
+      const int t = Pe / 3, d = Pe - 3 * t;
       double sum = 0;
       int has = 0;
 #pragma unroll 1
       for (int j = 0; j < kMaxCornersPerVertex; ++j) {
-        if (j >= cnt) break;
+        if (j >= cnt || P.bc_kind[t] != FSALE_BC_SYMMETRY) break;
         const int tj = B->tags[g][j];
         if (!(tj & 0x888)) continue;
 #pragma unroll
         for (int w = 0; w < 6; ++w) {
           const int tg = (tj >> (4 * (w >> 1))) & 0xf;
           if (!(tg & 8) || !(tg & (1 << t))) continue;
-          const double v = B->ln[g][j][3 * w + d];
+          const double v = B->facet[g][j][w][3] * B->facet[g][j][w][d];
           if (has) {
             sum += v;
           } else {
@@ -436,7 +486,7 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
           }
         }
       }
-      B->sn[g][P] = sum;
+      B->sn[g][Pe] = sum;
       if (d == 0) B->has[g][t] = has;
     }
   }
@@ -449,7 +499,35 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
     rhs[r] = L.sum[g][9 + r];
   }
   double un[3] = {0, 0, 0};
+  bool scatter = true;
   if constexpr (BOUNDARY) {
+    // the vertex's tags; a prescribed velocity ends the solve here
+    int vtags = 0;
+    for (int j = 0; j < cnt; ++j) {
+      const int tj = B->tags[g][j];
+#pragma unroll
+      for (int s2 = 0; s2 < 3; ++s2)
+        if ((tj >> (4 * s2)) & 8) vtags |= (tj >> (4 * s2)) & 7;
+    }
+    const int vel = velocity_tag(P, vtags, 3);
+    // prescribed pressures: rhs -= (l p) n over the boundary wedges in order
+    if (vel < 0 && (P.bc_kind[0] == FSALE_BC_PRESSURE || P.bc_kind[1] == FSALE_BC_PRESSURE ||
+                    P.bc_kind[2] == FSALE_BC_PRESSURE)) {
+#pragma unroll 1
+      for (int j = 0; j < cnt; ++j) {
+        const int tj = B->tags[g][j];
+        for (int w = 0; w < 6; ++w) {
+          const int tg = (tj >> (4 * (w >> 1))) & 0xf;
+          if (!(tg & 8)) continue;
+          for (int t = 0; t < 3; ++t) {
+            if (!(tg & (1 << t)) || P.bc_kind[t] != FSALE_BC_PRESSURE) continue;
+            const double fact = B->facet[g][j][w][3] * P.bc_val[t][0];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) rhs[d] -= fact * B->facet[g][j][w][d];
+          }
+        }
+      }
+    }
     double sn[3][3];
 #pragma unroll
     for (int t = 0; t < 3; ++t)
@@ -465,15 +543,22 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
       rows[1][d] = (h0 && h1) ? sn[1][d] : sn[2][d];
       rows[2][d] = sn[2][d];
     }
-    if (cnt > 0) {
-      if (ns == 0)
-        solve3(Mp, rhs, un);
-      else if (ns == 1)
-        sym_solve3<1>(Mp, rhs, rows, un);
-      else if (ns == 2)
-        sym_solve3<2>(Mp, rhs, rows, un);
-      else
-        sym_solve3<3>(Mp, rhs, rows, un);
+    if (vel >= 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) un[d] = P.bc_val[vel][d];
+      scatter = false;
+    } else if (cnt > 0) {
+      if constexpr (KIND == 1) {
+        if (ns == 0)
+          solve3(Mp, rhs, un);
+        else
+          sym_solve3<1>(Mp, rhs, rows, un);
+      } else {
+        if (ns == 2)
+          sym_solve3<2>(Mp, rhs, rows, un);
+        else
+          sym_solve3<3>(Mp, rhs, rows, un);
+      }
     }
   } else {
     if (cnt > 0) solve3(Mp, rhs, un);
@@ -491,7 +576,7 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
       double t = M[r][0] * un[0];
       t += M[r][1] * un[1];
       t += M[r][2] * un[2];
-      u.Fpc[r][ci] = -1.0 * t + 1.0 * F[r];
+      u.Fpc[r][ci] = scatter ? -1.0 * t + 1.0 * F[r] : F[r];
     }
   }
   // the next pass of the block overwrites the exchange arrays
@@ -499,13 +584,15 @@ __device__ inline void nodal3_group(const DevMesh &m, const MaireFields &u, int 
 }
 
 // one tile of 3D vertices, 32 per pass of the block's 256 lanes
-template <bool BOUNDARY>
+template <int KIND>
 __device__ inline void nodal3_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
   if (!wave_live(b.s, tile.prob, 0)) return;
+  constexpr bool BOUNDARY = KIND != 0;
   __shared__ Nodal3Lds lds;
   __shared__ typename std::conditional<BOUNDARY, Nodal3BndLds, char>::type blds;
   constexpr int G = kMaxCornersPerVertex;
   const int q = threadIdx.x & (G - 1), g = threadIdx.x / G;
+  const ProbParams &P = b.pp[tile.prob];
   for (int base = 0; base < tile.count; base += kGroups3) {
     const int i = base + g;
     int vt = tile.first + i, cnt = 0;
@@ -518,8 +605,32 @@ __device__ inline void nodal3_tile(const DevMesh &m, const MaireFields &u, const
       }
     }
     Nodal3BndLds *bp = nullptr;
-    if constexpr (BOUNDARY) bp = &blds;
-    nodal3_group<BOUNDARY>(m, u, vt, cnt, g, q, lds, bp);
+    if constexpr (BOUNDARY) {
+      bp = &blds;
+      // the vertex's class from its corners' tags: the symmetry rows its
+      // solve will have (tags whose condition is symmetry), or a velocity
+      int tq = 0;
+      if (q < cnt) {
+        const int cn = m.vck[(size_t)q * m.cap_v + vt];
+        tq = m.cwtag[(size_t)(cn & 7) * m.cap_c + (cn >> 3)];
+      }
+      blds.tags[g][q] = tq;
+      __syncthreads();
+      int vtags = 0;
+      for (int j = 0; j < cnt; ++j) {
+        const int tj = blds.tags[g][j];
+#pragma unroll
+        for (int s2 = 0; s2 < 3; ++s2)
+          if ((tj >> (4 * s2)) & 8) vtags |= (tj >> (4 * s2)) & 7;
+      }
+      int ns = 0;
+#pragma unroll
+      for (int t = 0; t < 3; ++t) ns += ((vtags >> t) & 1) && P.bc_kind[t] == FSALE_BC_SYMMETRY;
+      const bool small = velocity_tag(P, vtags, 3) >= 0 || ns <= 1;
+      if (small != (KIND == 1)) cnt = 0;
+      __syncthreads();
+    }
+    nodal3_group<KIND>(m, u, P, vt, cnt, g, q, lds, bp);
   }
 }
 
@@ -548,7 +659,7 @@ __device__ inline void corner_force(const MaireFields &u, int c, const double M[
 template <bool BOUNDARY, int N>
 __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
   if constexpr (N == 3) {
-    nodal3_tile<BOUNDARY>(m, u, b, tile);
+    nodal3_tile<BOUNDARY ? 1 : 0>(m, u, b, tile);
   } else {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x >= tile.count) return;
@@ -600,8 +711,9 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       }
     }
     double un[2];
+    bool scatter = true;
     if constexpr (BOUNDARY) {
-      boundary_solve(m, vt, cnt, Mp, rhs, kw, un);
+      scatter = boundary_solve(m, b.pp[tile.prob], vt, cnt, Mp, rhs, kw, un);
     } else {
       // ristra::math::solve (un-vendored): Cramer's rule
       const double det = Mp[0][0] * Mp[1][1] - Mp[0][1] * Mp[1][0];
@@ -618,7 +730,7 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       for (int r = 0; r < 2; ++r) {
         double t = Mq[r][0] * un[0];
         t += Mq[r][1] * un[1];
-        u.Fpc[r][ci] = -1.0 * t + 1.0 * Fq[r];
+        u.Fpc[r][ci] = scatter ? -1.0 * t + 1.0 * Fq[r] : Fq[r];
       }
     };
 #pragma unroll
@@ -637,6 +749,11 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
 template <bool BOUNDARY, int N>
 __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
   for_each_tile(BOUNDARY ? b.bt : b.vt, [&](const Tile &tile) { nodal_tile<BOUNDARY, N>(m, u, b, tile); });
+}
+
+// 3D boundary vertices with two or three symmetry rows
+__global__ void __launch_bounds__(kBlock) k_maire3_nodal_bnd_big(DevMesh m, MaireFields u, Batch b) {
+  for_each_tile(b.bt, [&](const Tile &tile) { nodal3_tile<2>(m, u, b, tile); });
 }
 
 // Both kinds in one launch over the one-block-per-tile lists: blocks
@@ -761,8 +878,13 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
       }
     }
     if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
+    double x[N];
 #pragma unroll
-    for (int d = 0; d < N; ++d) m.x[d][vt] = x0[d] + delta_t * u.un[d][vt];
+    for (int d = 0; d < N; ++d) {
+      x[d] = x0[d] + delta_t * u.un[d][vt];
+      m.x[d][vt] = x[d];
+    }
+    if constexpr (N == 3) put_xa(u, vt, x[0], x[1], x[2]);
   });
 }
 
@@ -837,9 +959,12 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
     u.d[c] = d;
     // update_state_from_energy lagrange_eqns.h:280-302
     const double g = u.g[c];
-    u.p[c] = eos_p_de(g, d, ie);
+    const double p = eos_p_de(g, d, ie);
+    const double a = smax(eos_a_de(g, ie), kMinSoundSpeed);
+    u.p[c] = p;
     u.T[c] = eos_T_de(P.cv, ie);
-    u.a[c] = smax(eos_a_de(g, ie), kMinSoundSpeed);
+    u.a[c] = a;
+    if constexpr (N == 3) put_cs(u, c, d, a, p, vel[0], vel[1], vel[2]);
     if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, m.corig[c], token);
   });
 }
@@ -857,6 +982,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
       else
         m.x[d][i] = u.x0[d][i];
     }
+    if (N == 3 && dir == 1) put_xa(u, i, m.x[0][i], m.x[1][i], m.x[2][i]);
   });
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
@@ -868,6 +994,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
     } else {
       for (int d = 0; d < N; ++d) u.v[d][i] = u.v0[d][i];
       u.e[i] = u.e0[i];
+      if constexpr (N == 3) put_cs(u, i, u.d[i], u.a[i], u.p[i], u.v[0][i], u.v[1][i], u.v[2][i]);
     }
   });
 }
@@ -926,17 +1053,15 @@ int launch_nodal_n(fsale_ctx *ctx) {
   Batch bi = batch_args(ctx, K_MAIRE_NODAL);
   const Batch bb = batch_args(ctx, K_MAIRE_NODAL_BND);
   if constexpr (N == 3) {
-    // the packing is timed with the interior launch as the nodal kernel
-    {
-      ProfScope prof(ctx, K_MAIRE_NODAL);
-      hipLaunchKernelGGL(k_maire3_pack, dim3(persistent_grid(ctx, K_MAIRE_COPY)), dim3(kBlock), 0, ctx->stream,
-                         ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_COPY));
-      FSALE_HIP(hipGetLastError());
-      hipLaunchKernelGGL((k_maire_nodal<false, 3>), dim3(persistent_grid(ctx, K_MAIRE_NODAL)), dim3(kBlock), 0,
-                         ctx->stream, ctx->m, ctx->ma, bi);
-      FSALE_HIP(hipGetLastError());
-    }
-    return launch_persistent(ctx, K_MAIRE_NODAL_BND, k_maire_nodal<true, 3>, ctx->m, ctx->ma, bb);
+    int rc = launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal<false, 3>, ctx->m, ctx->ma, bi);
+    if (rc) return rc;
+    ProfScope prof(ctx, K_MAIRE_NODAL_BND);
+    const int gb = persistent_grid(ctx, K_MAIRE_NODAL_BND);
+    hipLaunchKernelGGL((k_maire_nodal<true, 3>), dim3(gb), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, bb);
+    FSALE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_maire3_nodal_bnd_big, dim3(gb), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, bb);
+    FSALE_HIP(hipGetLastError());
+    return FSALE_OK;
   }
   if (bi.vt.direct && bb.bt.direct) {
     bi.bt = bb.bt;
@@ -1046,6 +1171,16 @@ int maire_alloc(fsale_ctx *ctx) {
   return FSALE_OK;
 }
 
+int maire3_pack_problem(fsale_ctx *ctx, int prob, hipStream_t stream) {
+  const Problem &P = ctx->probs[prob];
+  const int n = P.nc > P.nv ? P.nc : P.nv;
+  if (ctx->solver != FSALE_SOLVER_MAIRE || ctx->m.dim != 3 || n == 0) return FSALE_OK;
+  hipLaunchKernelGGL(k_maire3_pack, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ctx->m, ctx->ma, P.c0,
+                     P.nc, P.v0, P.nv);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
 int maire_initial_state(fsale_ctx *ctx, int prob, hipStream_t stream) {
   const Problem &P = ctx->probs[prob];
   const int g = (P.nc + kBlock - 1) / kBlock;
@@ -1053,6 +1188,27 @@ int maire_initial_state(fsale_ctx *ctx, int prob, hipStream_t stream) {
   hipLaunchKernelGGL(k_maire_ic, dim3(g), dim3(kBlock), 0, stream, ctx->m, ctx->ma, (const ProbParams *)ctx->d_params,
                      prob);
   FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
+// records a boundary condition in the problem's host parameters (the
+// caller uploads them)
+int set_maire_boundary(fsale_ctx *ctx, int prob, int tag, int kind, const double *value) {
+  const int dim = ctx->m.dim;
+  if (tag < 0 || tag >= dim || kind < FSALE_BC_SYMMETRY || kind > FSALE_BC_NONE) {
+    set_message("boundary condition: tag 0..dim-1, kind FSALE_BC_*");
+    return FSALE_ERR_ARG;
+  }
+  const int nval = kind == FSALE_BC_PRESSURE ? 1 : kind == FSALE_BC_VELOCITY ? dim : 0;
+  if (nval && !value) return FSALE_ERR_ARG;
+  for (int i = 0; i < nval; ++i)
+    if (!std::isfinite(value[i])) {
+      set_message("boundary condition value is not finite");
+      return FSALE_ERR_ARG;
+    }
+  ProbParams &P = ctx->probs[prob].params;
+  P.bc_kind[tag] = kind;
+  for (int d = 0; d < 3; ++d) P.bc_val[tag][d] = d < nval ? value[d] : 0.0;
   return FSALE_OK;
 }
 
@@ -1081,6 +1237,12 @@ int fsale_maire_initial_conditions(fsale_ctx *ctx, double cv, const double *gamm
   if ((rc = maire_initial_state(ctx, 0, ctx->stream))) return rc;
   P.have_state = true;
   return reset_scalars(ctx, 0, 0.0, 0.0);
+}
+
+int fsale_maire_set_boundary(fsale_ctx *ctx, int tag, int kind, const double *value) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  int rc = set_maire_boundary(ctx, 0, tag, kind, value);
+  return rc ? rc : upload_params(ctx, 0);
 }
 
 int fsale_maire_evaluate_nodal_state(fsale_ctx *ctx) {

@@ -59,7 +59,10 @@ def run(workload, steps, size):
         ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
         ctx.advance(prob["cfl"], prob["final_time"], steps, flux_kind=prob["flux"], use_graph=False)
     else:
-        prob = inputs.maire_bubble_2d(size or 512, steps=steps)
+        if workload == "maire3d":
+            prob = inputs.maire_bubble_3d(size or 128, steps=steps)
+        else:
+            prob = inputs.maire_bubble_2d(size or 512, steps=steps)
         ctx = fa.MaireContext(prob["coords"], prob["cells"])
         ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
         ctx.set_time(0.0, prob["dt0"])
@@ -227,7 +230,7 @@ def main():
                    help="summarise the span of the anchor kernel's last N dispatches")
     t.add_argument("--anchor", default=None, help="kernel name prefix, e.g. k_eul_fluxupd")
     r = sub.add_parser("run")
-    r.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d"])
+    r.add_argument("--workload", default="muscl2d", choices=["muscl2d", "muscl3d", "maire2d", "maire3d"])
     r.add_argument("--steps", type=int, default=3)
     r.add_argument("--size", type=int, default=None)
     s = sub.add_parser("summarize")

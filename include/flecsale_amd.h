@@ -197,10 +197,24 @@ int fsale_eul_advance(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_ti
  * has one eos_t; a uniform array reproduces it). */
 int fsale_maire_initial_conditions(fsale_ctx *ctx, double cv, const double *gamma,
                                    const double *d, const double *v, const double *p);
-/* evaluate_nodal_state (apps/maire_hydro/tasks.h:221-453) with symmetry
- * conditions on every boundary face whose normal is an axis (tags of
- * apps/maire_hydro/2d/inputs_regression.cc:134-162). */
+/* evaluate_nodal_state (apps/maire_hydro/tasks.h:221-453) with the boundary
+ * conditions of fsale_maire_set_boundary (symmetry by default) on the
+ * boundary faces whose normal is an axis (tags of
+ * apps/maire_hydro/2d/inputs_regression.cc:134-162,
+ * apps/maire_hydro/3d/inputs.cc:116-176). */
 int fsale_maire_evaluate_nodal_state(fsale_ctx *ctx);
+/* Boundary condition classes (apps/maire_hydro/types.h:88-126,
+ * boundary_condition_t: has_prescribed_velocity / has_prescribed_pressure /
+ * has_symmetry with velocity(x, t) / pressure(x, t)) with constant values,
+ * per tag: the boundary faces whose unit normal has |n_tag| = 1 within 1e-14
+ * (tag 0 x, 1 y, 2 z).  FSALE_BC_SYMMETRY (symmetry_boundary_condition_t,
+ * the default), FSALE_BC_PRESSURE (value[0] = p: each boundary wedge of the
+ * face subtracts l p n from its vertex's right-hand side, tasks.h:349-356),
+ * FSALE_BC_VELOCITY (value[0..dim-1] = u: a vertex of the face takes u as its
+ * velocity, no solve and, as the task's `continue` does, no corner-force
+ * scatter, tasks.h:322-336), FSALE_BC_NONE (the face adds no condition). */
+enum { FSALE_BC_SYMMETRY = 0, FSALE_BC_PRESSURE = 1, FSALE_BC_VELOCITY = 2, FSALE_BC_NONE = 3 };
+int fsale_maire_set_boundary(fsale_ctx *ctx, int tag, int kind, const double *value);
 /* evaluate_residual (apps/maire_hydro/tasks.h:461-489). */
 int fsale_maire_evaluate_residual(fsale_ctx *ctx);
 /* evaluate_time_step (apps/maire_hydro/tasks.h:142-190) followed by the
@@ -277,6 +291,8 @@ int fsale_batch_add_euler(fsale_ctx *ctx, const fsale_mesh_desc *mesh, fsale_eos
 int fsale_batch_add_maire(fsale_ctx *ctx, const fsale_mesh_desc *mesh, double cv, const double *gamma,
                           const fsale_maire_params *params, const double *d, const double *v,
                           const double *p, int32_t *problem);
+/* fsale_maire_set_boundary for one resident Lagrangian problem of a batch. */
+int fsale_batch_set_maire_boundary(fsale_ctx *ctx, int32_t problem, int tag, int kind, const double *value);
 /* Retire a problem and release its ranges (synchronises the stream). */
 int fsale_batch_remove(fsale_ctx *ctx, int32_t problem);
 /* nsteps driver steps of every resident problem that is not done; captured

@@ -56,6 +56,7 @@ def lib():
             [ct.POINTER(ct.c_double), ct.POINTER(ct.c_int)]
         L.om_qr_solve.argtypes = [ct.c_int, _dp, _dp]
         L.om_solve3.argtypes = [_dp, _dp, _dp]
+        L.om_maire_set_bc.argtypes = [ct.POINTER(ct.c_int), _dp]
         L.om_set_variant.argtypes = [ct.c_int]
         L.om_eos_eval.argtypes = [ct.c_double, ct.c_double, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.om_maire_init.argtypes = [_vp, _dp, ct.c_double, _dp, _dp] + [_dp] * 6
@@ -237,6 +238,16 @@ def solve3(M, r):
 
 
 # ---------------------------------------------------------------- Maire
+BC_SYMMETRY, BC_PRESSURE, BC_VELOCITY, BC_NONE = 0, 1, 2, 3
+
+
+def maire_set_boundary(kinds=(0, 0, 0), values=None):
+    """Boundary condition per symmetry tag (x, y, z faces): kinds BC_*, values
+    [3][3] (pressure in [t][0], velocity in [t][:dim]); global to the oracle."""
+    k = (ct.c_int * 3)(*[int(x) for x in kinds])
+    v = np.zeros((3, 3)) if values is None else np.ascontiguousarray(values, dtype=np.float64).reshape(3, 3)
+    lib().om_maire_set_bc(k, v.reshape(-1).copy())
+
 
 class MaireState:
     """Cell state of apps/maire_hydro (V, M, v, p, d, e, T, a) + per-cell gamma."""

@@ -916,6 +916,31 @@ void om_qr_solve(int rows, double *A, double *B) {
 
 #define MIN_SOUND_SPEED 1.e-6   /* lagrange_eqns.h:54 */
 
+/* Boundary conditions of the faces of each symmetry tag (0 x, 1 y, 2 z):
+ * boundary_condition_t of apps/maire_hydro/types.h:88-126 with constant
+ * values.  0 symmetry (symmetry_boundary_condition_t, the apps' choice and
+ * the default), 1 prescribed pressure val[t][0], 2 prescribed velocity
+ * val[t][0..dim-1], 3 none. */
+#define BC_SYMMETRY 0
+#define BC_PRESSURE 1
+#define BC_VELOCITY 2
+static int om_bc_kind[3] = {BC_SYMMETRY, BC_SYMMETRY, BC_SYMMETRY};
+static double om_bc_val[3][3];
+void om_maire_set_bc(const int *kind, const double *val) {
+  for (int t = 0; t < 3; ++t) {
+    om_bc_kind[t] = kind[t];
+    for (int d = 0; d < 3; ++d) om_bc_val[t][d] = val[3 * t + d];
+  }
+}
+
+/* the first tag (ascending) of a vertex whose condition prescribes its
+ * velocity, or -1 (tasks.h:322-336); vtags = the tags of its boundary faces */
+static int velocity_tag(unsigned vtags, int dim) {
+  for (int t = 0; t < dim; ++t)
+    if ((vtags & (1u << t)) && om_bc_kind[t] == BC_VELOCITY) return t;
+  return -1;
+}
+
 /* update_state_from_pressure lagrange_eqns.h:245-271 */
 static void ml_from_pressure(double g, double cv, double M, double V, double p, double *d,
                              double *e, double *T, double *a) {
@@ -1054,6 +1079,19 @@ static void maire_nodal_3d(const om_mesh *m, const double *uc, const double *pc,
     int has[3] = {0, 0, 0};
     double symn[3][3];
     if (m->vbnd[vt]) {
+      unsigned vtags = 0;
+      for (int j = 0; j < ncn; ++j) {
+        int cn = m->vc[m->vc_off[vt] + j];
+        int cl = cn / 8, k = cn % 8;
+        for (int s = 0; s < 6; ++s)
+          for (int i = 0; i < 4; ++i)
+            if (HEX_FACES[s][i] == k) vtags |= m->ftag[m->cf[cl * 6 + s]];
+      }
+      const int vel = velocity_tag(vtags, 3);
+      if (vel >= 0) {   /* as in 2D: u_p set, no solve, no scatter */
+        for (int dd = 0; dd < 3; ++dd) u[dd] = om_bc_val[vel][dd];
+        continue;
+      }
       for (int j = 0; j < ncn; ++j) {
         int cn = m->vc[m->vc_off[vt] + j];
         int cl = cn / 8, k = cn % 8;
@@ -1066,8 +1104,13 @@ static void maire_nodal_3d(const om_mesh *m, const double *uc, const double *pc,
           for (int tag = 0; tag < 3; ++tag) {
             if (!(m->ftag[f] & (1u << tag))) continue;
             if (om_variant == 6 && tag == 2) continue;   /* planted: z walls not symmetric */
-            if (has[tag]) { for (int dd = 0; dd < 3; ++dd) symn[tag][dd] += l[w] * n[w][dd]; }
-            else { has[tag] = 1; for (int dd = 0; dd < 3; ++dd) symn[tag][dd] = l[w] * n[w][dd]; }
+            if (om_bc_kind[tag] == BC_PRESSURE) {
+              double fact = l[w] * om_bc_val[tag][0];
+              for (int dd = 0; dd < 3; ++dd) rhs[dd] -= fact * n[w][dd];
+            } else if (om_bc_kind[tag] == BC_SYMMETRY) {
+              if (has[tag]) { for (int dd = 0; dd < 3; ++dd) symn[tag][dd] += l[w] * n[w][dd]; }
+              else { has[tag] = 1; for (int dd = 0; dd < 3; ++dd) symn[tag][dd] = l[w] * n[w][dd]; }
+            }
           }
         }
       }
@@ -1151,6 +1194,21 @@ void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const 
     double symn[2][2];
     int has[2] = {0, 0};
     if (m->vbnd[vt]) {
+      unsigned vtags = 0;
+      for (int j = 0; j < ncn; ++j) {
+        int cn = m->vc[m->vc_off[vt] + j];
+        int cl = cn / nvpc, k = cn % nvpc;
+        int km = (k + nvpc - 1) % nvpc;
+        vtags |= m->ftag[m->cf[cl * m->fpc + km]] | m->ftag[m->cf[cl * m->fpc + k]];
+      }
+      const int vel = velocity_tag(vtags, 2);
+      if (vel >= 0) {
+        /* prescribed velocity: u_p set, no solve, and the task's `continue`
+         * skips this vertex's scatter (F_pc keeps M_pc u_c + p_c n_pc) */
+        u[0] = om_bc_val[vel][0];
+        u[1] = om_bc_val[vel][1];
+        continue;
+      }
       /* wedges of the vertex: corners in order, wedges (minus, plus) */
       for (int j = 0; j < ncn; ++j) {
         int cn = m->vc[m->vc_off[vt] + j];
@@ -1163,8 +1221,13 @@ void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const 
           if (m->fc[2 * f + 1] >= 0) continue;
           for (int tag = 0; tag < 2; ++tag) {
             if (!(m->ftag[f] & (1u << tag))) continue;
-            if (has[tag]) { for (int dd = 0; dd < 2; ++dd) symn[tag][dd] += l[w] * n[w][dd]; }
-            else { has[tag] = 1; for (int dd = 0; dd < 2; ++dd) symn[tag][dd] = l[w] * n[w][dd]; }
+            if (om_bc_kind[tag] == BC_PRESSURE) {
+              double fact = l[w] * om_bc_val[tag][0];
+              for (int dd = 0; dd < 2; ++dd) rhs[dd] -= fact * n[w][dd];
+            } else if (om_bc_kind[tag] == BC_SYMMETRY) {
+              if (has[tag]) { for (int dd = 0; dd < 2; ++dd) symn[tag][dd] += l[w] * n[w][dd]; }
+              else { has[tag] = 1; for (int dd = 0; dd < 2; ++dd) symn[tag][dd] = l[w] * n[w][dd]; }
+            }
           }
         }
       }

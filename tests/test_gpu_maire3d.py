@@ -156,3 +156,35 @@ def test_maire_3d_batch_alone_vs_mix(oracle_lib):
         got = b.state(ids[1])
         for k, want in [("d", s.d), ("v", s.v), ("e", s.e), ("p", s.p)]:
             assert np.array_equal(got[k], want), k
+
+
+def test_task_sequence_3d_matches_oracle(oracle_lib):
+    """The driver step as separate tasks (save, nodal, residual, dt, half
+    update, nodal, residual, restore, full update), then a host write of the
+    velocity before the next nodal solve: each equals the oracle."""
+    O = oracle_lib
+    prob = inputs.maire_bubble_3d(7, steps=2)
+    print("task sequence, bubble 7^3", flush=True)
+    mesh, s, ctx = _setup(O, prob)
+    with ctx:
+        rc, t_o, n_o, dt_o = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], 1, prob["dt0"])
+        assert rc == 0
+        ctx.save()
+        ctx.evaluate_nodal_state()
+        ctx.evaluate_residual()
+        dt = ctx.evaluate_time_step(prob["cfl"], prob["final_time"])
+        assert dt == dt_o
+        ctx.apply_update(0.5)
+        ctx.evaluate_nodal_state()
+        ctx.evaluate_residual()
+        ctx.restore()
+        ctx.apply_update(1.0)
+        _bitwise(ctx, s, mesh, "tasks")
+        # a host write of the velocity reaches the next nodal solve
+        v = s.v * 0.5 + 0.01
+        ctx.set(fa.F_VELOCITY, v)
+        s.v[:] = v
+        ctx.evaluate_nodal_state()
+        un, npc, Fpc = O.maire_nodal(mesh, s)
+        assert np.array_equal(ctx.get(fa.F_NODE_VELOCITY), un)
+        assert np.array_equal(ctx.get(fa.F_CORNER_FORCE), Fpc)

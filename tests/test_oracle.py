@@ -319,3 +319,90 @@ def test_sedov_3d_octant_symmetry(oracle_lib):
         # vertices' QR solves carry ~5e-12 of round-off
         assert rel_linf(s.p[other], s.p[mine]) <= 2e-11
         assert rel_linf(s.v[other][:, list(perm)], s.v[mine]) <= 2e-11
+
+
+# ------------------------------------------------------------ Maire boundary conditions
+# The reference declares the classes (apps/maire_hydro/types.h:88-126) and the
+# task applies them (apps/maire_hydro/tasks.h:317-370), but no app or test
+# instantiates a pressure or velocity condition: what pins them here is what
+# the mathematics fixes.
+
+def _with_bc(O, kinds, values, fn):
+    O.maire_set_boundary(kinds, values)
+    try:
+        return fn()
+    finally:
+        O.maire_set_boundary()
+
+
+def _uniform(O, dim, p0=2.0):
+    from flecsale_amd import inputs
+    if dim == 2:
+        coords, cells = inputs.box_quads(12, 10, jitter=0.2, seed=5)
+    else:
+        coords, cells = inputs.box_hexes(6, 5, 4, jitter=0.2, seed=5)
+    mesh = O.Mesh(coords, cells)
+    nc = len(cells)
+    s = O.maire_initial_state(mesh, np.full(nc, 1.4), 1.0, np.ones(nc), np.zeros((nc, dim)), np.full(nc, p0))
+    return coords, mesh, s
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pressure_condition_holds_a_gas_at_rest(oracle_lib, dim):
+    """Every wall at the gas's own pressure: the wall terms l p n cancel the
+    corners' p (l n) and nothing moves (to round-off)."""
+    O = oracle_lib
+    coords, mesh, s = _uniform(O, dim)
+    vals = np.zeros((3, 3))
+    vals[:, 0] = 2.0
+
+    def run():
+        un, npc, Fpc = O.maire_nodal(mesh, s)
+        rc, t, n, dt = O.maire_run(mesh, s, 1.0, (0.25, 0.1, 1.01), 1.0, 5, 1e-3)
+        return un, rc
+
+    un, rc = _with_bc(O, (1, 1, 1), vals, run)
+    assert rc == 0
+    assert np.max(np.abs(un)) <= 1e-14
+    assert np.max(np.abs(s.v)) <= 1e-13 and np.max(np.abs(s.p - 2.0)) <= 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lower_outside_pressure_pushes_walls_out(oracle_lib, dim):
+    """Outside pressure 0.5 < 2: every wall vertex moves outward along its
+    wall normal (the sign of rhs -= l p n), interior vertices stay at rest."""
+    O = oracle_lib
+    coords, mesh, s = _uniform(O, dim)
+    vals = np.zeros((3, 3))
+    vals[:, 0] = 0.5
+    un, npc, Fpc = _with_bc(O, (1, 1, 1), vals, lambda: O.maire_nodal(mesh, s))
+    for d in range(dim):
+        lo = np.abs(coords[:, d]) < 1e-12
+        hi = np.abs(coords[:, d] - 1) < 1e-12
+        assert np.all(un[lo, d] < -1e-3) and np.all(un[hi, d] > 1e-3)
+    inner = np.all((coords > 1e-12) & (coords < 1 - 1e-12), axis=1)
+    assert np.max(np.abs(un[inner])) <= 1e-14
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_velocity_condition_moves_the_wall_exactly(oracle_lib, dim):
+    """x walls driven at u = 0.3 e_x (y / z walls symmetric): their vertices
+    move by u t exactly, the gas between them is set moving to +x."""
+    O = oracle_lib
+    coords, mesh, s = _uniform(O, dim)
+    vals = np.zeros((3, 3))
+    vals[0, 0] = 0.3
+    x0 = mesh.geometry()["coords"].copy()
+
+    def run():
+        return O.maire_run(mesh, s, 1.0, (0.25, 0.1, 1.01), 1.0, 30, 1e-3)
+
+    rc, t, n, dt = _with_bc(O, (2, 0, 0), vals, run)
+    assert rc == 0 and n == 30
+    x = mesh.geometry()["coords"]
+    wall = (np.abs(coords[:, 0]) < 1e-12) | (np.abs(coords[:, 0] - 1) < 1e-12)
+    assert np.max(np.abs(x[wall, 0] - (x0[wall, 0] + 0.3 * t))) <= 1e-14
+    assert np.max(np.abs(x[wall, 1:] - x0[wall, 1:])) == 0.0
+    # the left wall pushes the gas at rest to +x, the receding right wall
+    # lets it follow: every cell's x velocity lies between 0 and the walls'
+    assert np.all(s.v[:, 0] > 0) and np.all(s.v[:, 0] < 0.3)
