@@ -1172,9 +1172,15 @@ int set_problem_state(fsale_ctx *ctx, int prob, const double *d, const double *v
 }
 
 
+namespace {
+int run_steps(fsale_ctx *ctx, int nsteps, int use_graph);
+}
+
 // Driver steps of every live problem; chunks of up to 10 steps are replayed
 // from a captured graph (the graph reads tiles, parameters and scalars from
-// device memory, so it stays valid while problems come and go).
+// device memory, so it stays valid while problems come and go).  Eulerian
+// steps work on packed copies of the state (EulerFields::gs), made before
+// the steps and unpacked after them.
 int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   int rc;
   if ((rc = sync_tiles(ctx))) return rc;
@@ -1189,6 +1195,17 @@ int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
         if (P.live) P.partials_valid = true;
     }
   }
+  const bool euler = ctx->solver == FSALE_SOLVER_EULER;
+  if (euler && (rc = euler_pack(ctx))) return rc;
+  if ((rc = run_steps(ctx, nsteps, use_graph))) return rc;
+  return euler ? euler_unpack(ctx) : FSALE_OK;
+}
+
+namespace {
+
+// the steps of batch_steps, eager or from the captured graph
+int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
+  int rc;
   auto step = [&]() { return ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx); };
   if (!use_graph || nsteps < 2 || ctx->profiling) {
     for (int n = 0; n < nsteps; ++n)
@@ -1224,8 +1241,6 @@ int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
     if ((rc = step())) return rc;
   return FSALE_OK;
 }
-
-namespace {
 
 // Device arrays that carry a batch from one step to the next (everything
 // else a step writes is rewritten before it is read): (pointer, bytes).
@@ -1311,6 +1326,7 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
   for (int c : cands) {
     for (int k = 0; k < K_COUNT; ++k) ctx->shape.blocks_per_cu[k] = c;
     if ((rc = snap.restore(ctx))) break;
+    if (ctx->solver == FSALE_SOLVER_EULER && (rc = euler_pack(ctx))) break;
     for (int n = 0; n < nsteps && !rc; ++n)
       rc = ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx);
     if (rc) break;

@@ -124,14 +124,59 @@ __global__ void __launch_bounds__(kFinalBlock) k_eul_dt_finalize(Batch b, const 
 }
 
 // ----------------------------------------------------------- gradients
-template <int N, int VPC>
+// Waves per SIMD the 2D gradient kernel is register-limited to.
+#ifndef FSALE_GRAD_WAVES_2D
+#define FSALE_GRAD_WAVES_2D 3
+#endif
+
+// Packed per-cell records of the driver loop (EulerFields::gs, gg): state
+// {d, v[N], e, p, a} then the centroid at C; gradients {d, v, e, p, a} x N.
+// A record is stored as pairs of doubles, one plane (array) per pair, so a
+// lane loads 16 bytes per instruction and a wave's load covers 1 KiB of
+// consecutive memory: half the instructions of one 8-byte load per field,
+// the same cache lines.  (A whole record per lane, 64 bytes apart, touches
+// four times the cache lines per instruction and measured slower.)
+template <int N>
+struct Rec {
+  static constexpr int S = 2 * N + 4;            // doubles per state record
+  static constexpr int C = N + 4;                // centroid offset in it
+  static constexpr int G = (N == 2) ? 12 : 22;   // doubles per gradient record (3D: 21 + 1 pad)
+};
+
+template <int K, int P>
+__device__ __forceinline__ void ld_rec(double2 *const (&pl)[P], int c, double (&r)[K]) {
+  static_assert(K / 2 <= P, "record larger than its planes");
+  const uint32_t off = (uint32_t)c * 16u;
+#pragma unroll
+  for (int i = 0; i < K / 2; ++i) {
+    const double2 w = ld(pl[i], off);
+    r[2 * i] = w.x;
+    r[2 * i + 1] = w.y;
+  }
+}
+template <int K, int P>
+__device__ __forceinline__ void st_rec(double2 *const (&pl)[P], int c, const double (&r)[K]) {
+  static_assert(K / 2 <= P, "record larger than its planes");
+  const uint32_t off = (uint32_t)c * 16u;
+#pragma unroll
+  for (int i = 0; i < K / 2; ++i) st(pl[i], off, make_double2(r[2 * i], r[2 * i + 1]));
+}
+
+// PK: read the packed state records (gs) and write packed gradients (gg),
+// as the driver loop does; otherwise the per-field arrays (task level).
+template <int N, int VPC, bool PK>
 __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0, int first_order) {
   constexpr int NV = N + 4;
   const double tol = 1.e-9;
   const uint32_t o0 = (uint32_t)c0 * 8u, i0 = (uint32_t)c0 * 4u;
   double u0[NV], umin[NV], umax[NV], dudx[NV][N], dxdx[N][N], x0[N];
-  for (int d = 0; d < N; ++d) x0[d] = ld(m.ccen[d], o0);
-  {
+  if constexpr (PK) {
+    double r[Rec<N>::S];
+    ld_rec(u.gs, c0, r);
+    for (int j = 0; j < NV; ++j) u0[j] = r[j];
+    for (int d = 0; d < N; ++d) x0[d] = r[Rec<N>::C + d];
+  } else {
+    for (int d = 0; d < N; ++d) x0[d] = ld(m.ccen[d], o0);
     int i = 0;
     u0[i++] = ld(u.d, o0);
     for (int d = 0; d < N; ++d) u0[i++] = ld(u.v[d], o0);
@@ -167,28 +212,29 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
     const uint32_t ov = (uint32_t)ld(m.cv + lv * row, i0) * 8u;
     for (int d = 0; d < N; ++d) xv[lv][d] = ld(m.x[d], ov);
   };
-  if constexpr (N == 2) {
-#pragma unroll
-    for (int lv = 0; lv < VPC; ++lv) load_vertex(lv);
-  }
   // cell_vertex_neighbors in ascending id order: the head, then the CSR tail
   auto neighbour = [&](int c1) {
     const uint32_t o1 = (uint32_t)c1 * 8u;
     double u1[NV], dx[N];
-    {
+    if constexpr (PK) {
+      double r[Rec<N>::S];
+      ld_rec(u.gs, c1, r);
+      for (int j = 0; j < NV; ++j) u1[j] = r[j];
+      for (int d = 0; d < N; ++d) dx[d] = r[Rec<N>::C + d] - x0[d];
+    } else {
       int i = 0;
       u1[i++] = ld(u.d, o1);
       for (int d = 0; d < N; ++d) u1[i++] = ld(u.v[d], o1);
       u1[i++] = ld(u.e, o1);
       u1[i++] = ld(u.p, o1);
       u1[i++] = ld(u.a, o1);
+      for (int d = 0; d < N; ++d) dx[d] = ld(m.ccen[d], o1) - x0[d];
     }
     // umin = std::min(u1, umin), umax = std::max(u1, umax) (tasks.h:206-207)
     for (int j = 0; j < NV; ++j) {
-      umin[j] = smin(u1[j], umin[j]);
-      umax[j] = smax(u1[j], umax[j]);
+      umin[j] = hmin(u1[j], umin[j]);
+      umax[j] = hmax(u1[j], umax[j]);
     }
-    for (int d = 0; d < N; ++d) dx[d] = ld(m.ccen[d], o1) - x0[d];
     if constexpr (N == 2) {
       dxdx[0][0] += dx[0] * dx[0];
       dxdx[0][1] += dx[0] * dx[1];
@@ -210,20 +256,25 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
 #pragma unroll
     for (int k = 0; k < NBK; ++k) neighbour(nb[k] >= 0 ? nb[k] : c0);
   } else {
-    // 28 neighbour slots hoisted at once would spill: groups of 4, each
-    // group's loads issued together
+    // 28 neighbour slots hoisted at once would spill: groups of four (two
+    // for the 12-double packed records), each group's loads issued together
+    constexpr int GRP = PK ? 2 : 4;
 #pragma unroll 1
-    for (int k0 = 0; k0 < NBK; k0 += 4) {
-      int g[4];
+    for (int k0 = 0; k0 < NBK; k0 += GRP) {
+      int g[GRP];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) g[j] = ld(m.nbk + (k0 + j) * row, i0);
+      for (int j = 0; j < GRP; ++j) g[j] = ld(m.nbk + (k0 + j) * row, i0);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) neighbour(g[j] >= 0 ? g[j] : c0);
+      for (int j = 0; j < GRP; ++j) neighbour(g[j] >= 0 ? g[j] : c0);
     }
   }
   if (cnt > NBK) {
     const int q0 = ld(m.nbr_start, i0);
     for (int q = q0 + NBK; q < q0 + cnt; ++q) neighbour(m.nbr[q]);
+  }
+  if constexpr (N == 2) {
+#pragma unroll
+    for (int lv = 0; lv < VPC; ++lv) load_vertex(lv);
   }
   double gu[NV][N];
   if constexpr (N == 2) {
@@ -254,12 +305,12 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   // the minimum over the first kind is (umax-u0)/max(du) and over the second
   // (umin-u0)/min(du): two divisions per variable instead of one per vertex,
   // with the same phi (up to the sign of a zero phi, which only flips the
-  // sign of a zero gradient and never changes a reconstructed value).
-  double dpos[NV], dneg[NV];
-  for (int j = 0; j < NV; ++j) {
-    dpos[j] = 0;
-    dneg[j] = 0;
-  }
+  // sign of a zero gradient and never changes a reconstructed value).  The
+  // maximum over every vertex's du is the maximum over those above tol
+  // whenever one is, so the extrema need no branch.  A quotient whose
+  // numerator is at least its divisor in magnitude rounds to >= 1 and leaves
+  // phi alone, so only the limiting cases divide.
+  double dmax[NV], dmin[NV];
 #pragma unroll
   for (int lv = 0; lv < VPC; ++lv) {
     if constexpr (N != 2) load_vertex(lv);
@@ -271,34 +322,80 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
     }
     for (int j = 0; j < NV; ++j) {
       const double du = u1[j] - u0[j];
-      if (du > tol)
-        dpos[j] = fmax(dpos[j], du);
-      else if (du < -tol)
-        dneg[j] = fmin(dneg[j], du);
+      dmax[j] = lv ? hmax(dmax[j], du) : du;
+      dmin[j] = lv ? hmin(dmin[j], du) : du;
     }
   }
   for (int j = 0; j < NV; ++j) {
     double phi = 1;
-    if (dpos[j] > tol) phi = smin(phi, (umax[j] - u0[j]) / dpos[j]);
-    if (dneg[j] < -tol) phi = smin(phi, (umin[j] - u0[j]) / dneg[j]);
+    const double hi = umax[j] - u0[j], lo = umin[j] - u0[j];
+    if (dmax[j] > tol && hi < dmax[j]) phi = hi / dmax[j];
+    if (dmin[j] < -tol && lo > dmin[j]) phi = smin(phi, lo / dmin[j]);
     phi = smax(phi, 0);
     for (int d = 0; d < N; ++d) gu[j][d] = first_order ? 0.0 : gu[j][d] * phi;
   }
-  for (int d = 0; d < N; ++d) {
-    st(u.gd[d], o0, gu[0][d]);
-    for (int k = 0; k < N; ++k) st(u.gv[k * N + d], o0, gu[1 + k][d]);
-    st(u.ge[d], o0, gu[1 + N][d]);
-    st(u.gp[d], o0, gu[2 + N][d]);
-    st(u.ga[d], o0, gu[3 + N][d]);
+  if constexpr (PK) {
+    // the record is gu row by row: grad v[k][d] (d/dx_d of v_k) at N + k*N + d
+    double r[Rec<N>::G];
+    for (int j = 0; j < NV; ++j)
+      for (int d = 0; d < N; ++d) r[j * N + d] = gu[j][d];
+    for (int i = NV * N; i < Rec<N>::G; ++i) r[i] = 0;
+    st_rec(u.gg, c0, r);
+  } else {
+    for (int d = 0; d < N; ++d) {
+      st(u.gd[d], o0, gu[0][d]);
+      for (int k = 0; k < N; ++k) st(u.gv[k * N + d], o0, gu[1 + k][d]);
+      st(u.ge[d], o0, gu[1 + N][d]);
+      st(u.gp[d], o0, gu[2 + N][d]);
+      st(u.ga[d], o0, gu[3 + N][d]);
+    }
   }
 }
 
-template <int N, int VPC>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+template <int N, int VPC, bool PK>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(N == 2 ? FSALE_GRAD_WAVES_2D : 3)))
 k_eul_grad(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
-    if ((int)threadIdx.x < tile.count) grad_cell<N, VPC>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
+    if ((int)threadIdx.x < tile.count)
+      grad_cell<N, VPC, PK>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
+  });
+}
+
+// ------------------------------------------------- packed driver records
+// The per-field state into the state records (gs), every cell of the tile
+// lists whatever its problem's flags, so a batch of steps starts from them.
+template <int N>
+__global__ void __launch_bounds__(kBlock) k_eul_pack(DevMesh m, EulerFields u, Batch b) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if ((int)threadIdx.x >= tile.count) return;
+    const int c = tile.first + threadIdx.x;
+    double r[Rec<N>::S];
+    r[0] = u.d[c];
+    for (int k = 0; k < N; ++k) r[1 + k] = u.v[k][c];
+    r[N + 1] = u.e[c];
+    r[N + 2] = u.p[c];
+    r[N + 3] = u.a[c];
+    for (int d = 0; d < N; ++d) r[Rec<N>::C + d] = m.ccen[d][c];
+    st_rec(u.gs, c, r);
+  });
+}
+
+// ... and back when the batch of steps ends; T = e / cv as the update
+// computes it (update_state_from_energy, flecsale/eqns/euler_eqns.h:359-375).
+template <int N>
+__global__ void __launch_bounds__(kBlock) k_eul_unpack(EulerFields u, Batch b) {
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if ((int)threadIdx.x >= tile.count) return;
+    const int c = tile.first + threadIdx.x;
+    double r[Rec<N>::S];
+    ld_rec(u.gs, c, r);
+    u.d[c] = r[0];
+    for (int k = 0; k < N; ++k) u.v[k][c] = r[1 + k];
+    u.e[c] = r[N + 1];
+    u.p[c] = r[N + 2];
+    u.a[c] = r[N + 3];
+    u.T[c] = eos_T_de(b.pp[tile.prob].cv, r[N + 1]);
   });
 }
 
@@ -324,6 +421,27 @@ __device__ inline void load_cell(const DevMesh &m, const EulerFields &u, int c, 
     q.cx[i] = m.ccen[i][c];
   }
   for (int i = 0; i < N * N; ++i) q.gv[i] = u.gv[i][c];
+}
+
+// the same from the driver loop's packed records
+template <int N>
+__device__ inline void load_cell_pk(const EulerFields &u, int c, CellData<N> &q) {
+  double r[Rec<N>::S], g[Rec<N>::G];
+  ld_rec(u.gs, c, r);
+  ld_rec(u.gg, c, g);
+  q.d = r[0];
+  for (int k = 0; k < N; ++k) q.v[k] = r[1 + k];
+  q.e = r[N + 1];
+  q.p = r[N + 2];
+  q.a = r[N + 3];
+  for (int i = 0; i < N; ++i) {
+    q.cx[i] = r[Rec<N>::C + i];
+    q.gd[i] = g[i];
+    q.ge[i] = g[(N + 1) * N + i];
+    q.gp[i] = g[(N + 2) * N + i];
+    q.ga[i] = g[(N + 3) * N + i];
+  }
+  for (int i = 0; i < N * N; ++i) q.gv[i] = g[N + i];
 }
 
 // face state of apps/hydro/tasks.h:418-432: slot 2 (pressure) gets dx.grad_e
@@ -387,22 +505,28 @@ __global__ void __launch_bounds__(kBlock) k_eul_flux(DevMesh m, EulerFields u, B
 // -------------------------------------------------------------- update
 // update_state_from_flux euler_eqns.h:383-410 + update_state_from_energy
 // :359-375 for one cell; returns false on negative density / energy.
+// (den0, vel, ie0) -> (mass, vel, ie)
 template <int N>
-__device__ inline bool update_cell(const EulerFields &in, const EulerFields &out, int c, double *du, double sc,
-                                   double g, double cv, double *vel_out, double &a_out) {
+__device__ inline void update_values(double den0, double *vel, double ie0, double *du, double sc, double &mass,
+                                     double &ie) {
   for (int i = 0; i < N + 2; ++i) du[i] *= sc;
-  double vel[N];
-  for (int d = 0; d < N; ++d) vel[d] = in.v[d][c];
-  const double den0 = in.d[c];
-  const double mass = den0 + du[0];
-  const double et0 = in.e[c] + 0.5 * dotN<N>(vel, vel);
+  mass = den0 + du[0];
+  const double et0 = ie0 + 0.5 * dotN<N>(vel, vel);
   const double ener = den0 * et0 + du[N + 1];
   const double inv_mass = 1 / mass;
   for (int d = 0; d < N; ++d) {
     const double mom = den0 * vel[d] + du[1 + d];
     vel[d] = mom * inv_mass;
   }
-  const double ie = ener * inv_mass - 0.5 * dotN<N>(vel, vel);
+  ie = ener * inv_mass - 0.5 * dotN<N>(vel, vel);
+}
+
+template <int N>
+__device__ inline bool update_cell(const EulerFields &in, const EulerFields &out, int c, double *du, double sc,
+                                   double g, double cv, double *vel_out, double &a_out) {
+  double vel[N], mass, ie;
+  for (int d = 0; d < N; ++d) vel[d] = in.v[d][c];
+  update_values<N>(in.d[c], vel, in.e[c], du, sc, mass, ie);
   out.d[c] = mass;
   for (int d = 0; d < N; ++d) {
     out.v[d][c] = vel[d];
@@ -447,6 +571,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
 }
 
 // fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
+// over the packed records: reads u.gs and u.gg, writes out.gs
 template <int N, int FPC, int VPC>
 __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, Batch b,
                                                         double factor, int want_dt, int token) {
@@ -466,7 +591,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
 #pragma unroll
       for (int k = 0; k < FPC; ++k) cnk[k] = m.cfn[(size_t)k * m.cap_c + c];
       CellData<N> own;
-      load_cell<N>(m, u, c, own);
+      load_cell_pk<N>(u, c, own);
       double px[VPC], py[VPC];
       if constexpr (N == 2) {
         for (int k = 0; k < VPC; ++k) {
@@ -502,7 +627,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
         // the neighbour's loads do not wait on a branch (a boundary face
         // loads the cell's own data again and ignores it)
         CellData<N> nb;
-        load_cell<N>(m, u, other >= 0 ? other : c, nb);
+        load_cell_pk<N>(u, other >= 0 ? other : c, nb);
         double fl[N + 2];
         if (!second)
           face_flux_area<N>(kind, own, nb, other >= 0, fx, nk[k], ak[k], fl);
@@ -515,8 +640,21 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
         }
       }
       const double vol = m.cvol[c];
-      double vel[N], a;
-      if (!update_cell<N>(u, out, c, du, delta_t / vol, P.gamma, P.cv, vel, a))
+      double vel[N], mass, ie;
+      for (int d = 0; d < N; ++d) vel[d] = own.v[d];
+      update_values<N>(own.d, vel, own.e, du, delta_t / vol, mass, ie);
+      const double a = eos_a_de(P.gamma, ie);
+      {
+        double r[Rec<N>::S];
+        r[0] = mass;
+        for (int d = 0; d < N; ++d) r[1 + d] = vel[d];
+        r[N + 1] = ie;
+        r[N + 2] = eos_p_de(P.gamma, mass, ie);
+        r[N + 3] = a;
+        for (int d = 0; d < N; ++d) r[Rec<N>::C + d] = own.cx[d];
+        st_rec(out.gs, c, r);
+      }
+      if (ie < 0 || mass < 0)   // apps/hydro/tasks.h:654-655
         flag_negative(b.s, tile.prob, m.corig[c], token);
       if (want_dt) {
 #pragma unroll
@@ -546,19 +684,9 @@ int launch_persistent(fsale_ctx *ctx, int kid, K kernel, A... args) {
 // the same fields with the two state buffers exchanged
 EulerFields swapped(const EulerFields &a) {
   EulerFields b = a;
-  b.d = a.alt_d;
-  b.e = a.alt_e;
-  b.p = a.alt_p;
-  b.T = a.alt_T;
-  b.a = a.alt_a;
-  b.alt_d = a.d;
-  b.alt_e = a.e;
-  b.alt_p = a.p;
-  b.alt_T = a.T;
-  b.alt_a = a.a;
-  for (int d = 0; d < 3; ++d) {
-    b.v[d] = a.alt_v[d];
-    b.alt_v[d] = a.v[d];
+  for (int i = 0; i < EulerFields::kPlanesS; ++i) {
+    b.gs[i] = a.alt_gs[i];
+    b.alt_gs[i] = a.gs[i];
   }
   return b;
 }
@@ -579,8 +707,14 @@ int launch_finalize(fsale_ctx *ctx, int driver) {
 
 int launch_grad(fsale_ctx *ctx, const EulerFields &u) {
   const Batch b = batch_args(ctx, K_EUL_GRAD);
-  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4>, ctx->m, u, b);
-  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8>, ctx->m, u, b);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, false>, ctx->m, u, b);
+  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, false>, ctx->m, u, b);
+}
+
+int launch_grad_pk(fsale_ctx *ctx, const EulerFields &u) {
+  const Batch b = batch_args(ctx, K_EUL_GRAD);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, true>, ctx->m, u, b);
+  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, true>, ctx->m, u, b);
 }
 
 // token: the launch's id for error attribution (first half step 1, second 2)
@@ -615,26 +749,57 @@ int euler_step(fsale_ctx *ctx) {
   int rc;
   const EulerFields a = ctx->eu, b = swapped(ctx->eu);
   if ((rc = launch_finalize(ctx, 1))) return rc;
-  if ((rc = launch_grad(ctx, a))) return rc;
+  if ((rc = launch_grad_pk(ctx, a))) return rc;
   if ((rc = launch_fluxupd(ctx, 0.5, false, 1, a, b))) return rc;
-  if ((rc = launch_grad(ctx, b))) return rc;
+  if ((rc = launch_grad_pk(ctx, b))) return rc;
   if ((rc = launch_fluxupd(ctx, 0.5, true, 2, b, a))) return rc;
   return FSALE_OK;
 }
 
 int euler_dt_partials(fsale_ctx *ctx) { return launch_dt_partial(ctx); }
 
+// Around a batch of driver steps: the per-field state into the packed
+// records, and back.  Every cell of the tile lists is copied, also of
+// problems that do not step, so both directions leave those unchanged.
+int euler_pack(fsale_ctx *ctx) {
+  const Batch b = batch_args(ctx, -1);
+  const dim3 grid(persistent_grid(ctx, -1)), block(kBlock);
+  if (ctx->m.dim == 2)
+    hipLaunchKernelGGL(k_eul_pack<2>, grid, block, 0, ctx->stream, ctx->m, ctx->eu, b);
+  else
+    hipLaunchKernelGGL(k_eul_pack<3>, grid, block, 0, ctx->stream, ctx->m, ctx->eu, b);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
+int euler_unpack(fsale_ctx *ctx) {
+  const Batch b = batch_args(ctx, -1);
+  const dim3 grid(persistent_grid(ctx, -1)), block(kBlock);
+  if (ctx->m.dim == 2)
+    hipLaunchKernelGGL(k_eul_unpack<2>, grid, block, 0, ctx->stream, ctx->eu, b);
+  else
+    hipLaunchKernelGGL(k_eul_unpack<3>, grid, block, 0, ctx->stream, ctx->eu, b);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
 int euler_alloc(fsale_ctx *ctx) {
   const DevMesh &m = ctx->m;
   EulerFields &u = ctx->eu;
   const int N = m.dim;
   int rc;
-  double **cell1[] = {&u.d, &u.e, &u.p, &u.T, &u.a, &u.alt_d, &u.alt_e, &u.alt_p, &u.alt_T, &u.alt_a};
+  double **cell1[] = {&u.d, &u.e, &u.p, &u.T, &u.a};
   for (double **p : cell1)
     if ((rc = dev_alloc_n(ctx, p, m.cap_c))) return rc;
+  const int ps = (N == 2 ? Rec<2>::S : Rec<3>::S) / 2, pg = (N == 2 ? Rec<2>::G : Rec<3>::G) / 2;
+  for (int i = 0; i < ps; ++i) {
+    if ((rc = dev_alloc_n(ctx, &u.gs[i], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.alt_gs[i], m.cap_c))) return rc;
+  }
+  for (int i = 0; i < pg; ++i)
+    if ((rc = dev_alloc_n(ctx, &u.gg[i], m.cap_c))) return rc;
   for (int d = 0; d < N; ++d) {
     if ((rc = dev_alloc_n(ctx, &u.v[d], m.cap_c))) return rc;
-    if ((rc = dev_alloc_n(ctx, &u.alt_v[d], m.cap_c))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.gd[d], m.cap_c))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.ge[d], m.cap_c))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.gp[d], m.cap_c))) return rc;

@@ -167,11 +167,20 @@ struct DevScalars {
 struct EulerFields {
   double *d = nullptr, *v[3] = {nullptr, nullptr, nullptr}, *e = nullptr, *p = nullptr,
          *T = nullptr, *a = nullptr;
-  // second state buffer: the fused flux+update kernel reads one and writes
-  // the other (its neighbours must see the old state); a step flips twice
-  double *alt_d = nullptr, *alt_v[3] = {nullptr, nullptr, nullptr}, *alt_e = nullptr, *alt_p = nullptr,
-         *alt_T = nullptr, *alt_a = nullptr;
   double *gd[3] = {}, *gv[9] = {}, *ge[3] = {}, *gp[3] = {}, *ga[3] = {};
+  // The driver loop's working copies, one record per cell holding what a
+  // neighbour gathers, stored as pairs of doubles in planes of capacity
+  // pairs, so a gather takes one 16-byte load per pair instead of one 8-byte
+  // load per field:
+  //   gs: {d, v[N], e, p, a, centroid[N]} (2D: 8 doubles; 3D: 10),
+  //       twice: the fused flux+update kernel reads one and writes the other
+  //       (its neighbours must see the old state), so a step flips twice;
+  //   gg: gradients {d[N], v[N][N], e[N], p[N], a[N]} (2D: 12; 3D: 21 + 1 pad).
+  // Packed from the fields above when a batch of steps starts, unpacked when
+  // it ends.
+  // Each plane's base is its own pointer (a scalar base for every load).
+  static constexpr int kPlanesS = 5, kPlanesG = 11;   // the 3D counts; 2D uses 4 and 6
+  double2 *gs[kPlanesS] = {}, *alt_gs[kPlanesS] = {}, *gg[kPlanesG] = {};
   double *F[5] = {};
   double *partials = nullptr;    // per cell tile and wave: max of dt^-1
 };
@@ -363,6 +372,8 @@ int maire_initial_state(fsale_ctx *ctx, int prob, hipStream_t stream);
 int euler_alloc(fsale_ctx *ctx);
 int euler_dt_partials(fsale_ctx *ctx);
 int euler_step(fsale_ctx *ctx);       // one driver step over every live problem
+int euler_pack(fsale_ctx *ctx);       // per-field state -> the driver loop's packed records
+int euler_unpack(fsale_ctx *ctx);     // and back
 // Maire (maire.hip)
 int maire_alloc(fsale_ctx *ctx);
 int maire_step(fsale_ctx *ctx);

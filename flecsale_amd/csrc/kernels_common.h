@@ -16,6 +16,22 @@ namespace fsale {
 __host__ __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
 __host__ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 
+// One v_min_f64 / v_max_f64 each.  On ordered operands they return the value
+// smin / smax return; only the sign of a zero result can differ (+0 vs -0),
+// and that never changes a sum, a product's magnitude or a comparison.  A
+// compare and two selects per smin, or fmin's operand canonicalisation,
+// cost three instructions where these cost one.
+__device__ __forceinline__ double hmin(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double hmax(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // Element access by byte offset for entity arrays (cells, faces, vertices):
 // batch capacities are held below 2^28 entities, so an offset of an 8-byte
 // element fits in 32 bits.  Formed that way, the load or store takes the

@@ -186,7 +186,9 @@ int fsale_eul_apply_update(fsale_ctx *ctx, fsale_eos eos, double factor);
  * 2 x [gradients, fluxes, apply_update(0.5)] }, all on the device, captured
  * once as a hipGraph when use_graph != 0. stop_at_final != 0 turns steps after
  * t >= final_time into no-ops on the device (the reference loop keeps
- * stepping with dt = 0). */
+ * stepping with dt = 0).  The loop keeps its gradients and face fluxes
+ * internal: the FSALE_F_GRAD_* and FSALE_F_FLUX fields hold what the last
+ * task-level evaluate_gradients / evaluate_fluxes call wrote. */
 int fsale_eul_advance(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_time, int nsteps,
                       int flux_kind, int first_order, int stop_at_final, int use_graph);
 
