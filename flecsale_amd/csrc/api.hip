@@ -502,6 +502,8 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if (dim == 3 && solver == FSALE_SOLVER_EULER)
     for (int i = 0; i < 7; ++i)
       if ((rc = dev_alloc_n(ctx, &m.kf[i], (size_t)cap_c * m.fpc))) return rc;
+  if (dim == 2 && solver == FSALE_SOLVER_EULER)
+    if ((rc = dev_alloc_n(ctx, &m.xy, cap_v))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cvol, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cminlen, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.farea, cap_f))) return rc;

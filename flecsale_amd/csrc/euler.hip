@@ -57,9 +57,9 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerField
       double px[FPC], py[FPC];
       if constexpr (N == 2) {
         for (int k = 0; k < FPC; ++k) {
-          const int v = m.cv[(size_t)k * m.cap_c + c];
-          px[k] = m.x[0][v];
-          py[k] = m.x[1][v];
+          const double2 w = m.xy[m.cv[(size_t)k * m.cap_c + c]];
+          px[k] = w.x;
+          py[k] = w.y;
         }
       }
 #pragma unroll
@@ -209,8 +209,14 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   const int cnt = ld(m.nbr_cnt, i0);
   double xv[VPC][N];
   auto load_vertex = [&](int lv) {
-    const uint32_t ov = (uint32_t)ld(m.cv + lv * row, i0) * 8u;
-    for (int d = 0; d < N; ++d) xv[lv][d] = ld(m.x[d], ov);
+    const uint32_t iv = (uint32_t)ld(m.cv + lv * row, i0);
+    if constexpr (N == 2) {
+      const double2 w = ld(m.xy, iv * 16u);
+      xv[lv][0] = w.x;
+      xv[lv][1] = w.y;
+    } else {
+      for (int d = 0; d < N; ++d) xv[lv][d] = ld(m.x[d], iv * 8u);
+    }
   };
   // cell_vertex_neighbors in ascending id order: the head, then the CSR tail
   auto neighbour = [&](int c1) {
@@ -595,9 +601,9 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
       double px[VPC], py[VPC];
       if constexpr (N == 2) {
         for (int k = 0; k < VPC; ++k) {
-          const int v = m.cv[(size_t)k * m.cap_c + c];
-          px[k] = m.x[0][v];
-          py[k] = m.x[1][v];
+          const double2 w = m.xy[m.cv[(size_t)k * m.cap_c + c]];
+          px[k] = w.x;
+          py[k] = w.y;
         }
       }
       double du[N + 2];

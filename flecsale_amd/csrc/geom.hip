@@ -99,6 +99,13 @@ __global__ void __launch_bounds__(kBlock) k_cellface_geom_3d(DevMesh m, int c0, 
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_vertex_pairs_2d(DevMesh m, int v0, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = v0 + i;
+  m.xy[v] = make_double2(m.x[0][v], m.x[1][v]);
+}
+
 }  // namespace
 
 int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces, hipStream_t stream) {
@@ -111,6 +118,11 @@ int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces, hipStream_
     else
       hipLaunchKernelGGL(k_cell_geom_3d, dim3(g), dim3(kBlock), 0, stream, m, P.c0, P.nc);
     FSALE_HIP(hipGetLastError());
+    if (m.xy && P.nv > 0) {
+      hipLaunchKernelGGL(k_vertex_pairs_2d, dim3((P.nv + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, m, P.v0,
+                         P.nv);
+      FSALE_HIP(hipGetLastError());
+    }
   }
   if (faces && P.nf > 0) {
     const int g = (P.nf + kBlock - 1) / kBlock;

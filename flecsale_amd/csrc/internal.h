@@ -119,6 +119,9 @@ struct DevMesh {
   // (centroid 0-2, normal 3-5 as stored for the face, area 6), so the fused
   // step reads it coalesced instead of gathering 8 bytes per 128-byte line
   double *kf[7] = {};
+  // 2D Eulerian: vertex coordinates again as (x, y) pairs [cap_v], one
+  // 16-byte gather per vertex
+  double2 *xy = nullptr;
 };
 
 // A tile: `count` (<= kBlock) consecutive entities of problem `prob`
