@@ -502,7 +502,7 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if (dim == 3 && solver == FSALE_SOLVER_EULER)
     for (int i = 0; i < 7; ++i)
       if ((rc = dev_alloc_n(ctx, &m.kf[i], (size_t)cap_c * m.fpc))) return rc;
-  if (dim == 2 && solver == FSALE_SOLVER_EULER)
+  if (dim == 2)
     if ((rc = dev_alloc_n(ctx, &m.xy, cap_v))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cvol, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cminlen, cap_c))) return rc;
@@ -707,7 +707,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   }
   if ((rc = st.run_deferred())) return rc;
   if ((rc = launch_geometry(ctx, prob, true, faces, ctx->upload))) return rc;
-  if ((rc = maire3_pack_problem(ctx, prob, ctx->upload))) return rc;
+  if ((rc = maire_pack_problem(ctx, prob, ctx->upload))) return rc;
   if (maire) {
     // vertex -> corner lists and boundary tags from the initial face normals:
     // tag d when |n_d| is 1 within 1e-14 (apps/maire_hydro/2d/inputs_regression.cc:134-162);
@@ -1162,7 +1162,7 @@ int set_problem_field(fsale_ctx *ctx, int prob, int field, const double *host_in
   if (field == FSALE_F_COORDS &&
       (rc = launch_geometry(ctx, prob, true, ctx->probs[prob].faces_on_device, ctx->stream)))
     return rc;
-  return maire3_pack_problem(ctx, prob, ctx->stream);
+  return maire_pack_problem(ctx, prob, ctx->stream);
 }
 
 // d, v, p of one problem from host arrays (v interleaved)
@@ -1208,10 +1208,12 @@ namespace {
 // the steps of batch_steps, eager or from the captured graph
 int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   int rc;
-  auto step = [&]() { return ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx); };
+  auto step = [&](bool first) {
+    return ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx, first);
+  };
   if (!use_graph || nsteps < 2 || ctx->profiling) {
     for (int n = 0; n < nsteps; ++n)
-      if ((rc = step())) return rc;
+      if ((rc = step(n == 0))) return rc;
     return FSALE_OK;
   }
   const int chunk = nsteps < 10 ? nsteps : 10;
@@ -1226,7 +1228,7 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
     hipGraph_t graph;
     FSALE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     for (int n = 0; n < chunk; ++n) {
-      rc = step();
+      rc = step(n == 0);
       if (rc) {
         (void)hipStreamEndCapture(ctx->stream, &graph);
         return rc;
@@ -1240,7 +1242,7 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   int n = 0;
   for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(ctx->graph.exec, ctx->stream));
   for (; n < nsteps; ++n)
-    if ((rc = step())) return rc;
+    if ((rc = step(false))) return rc;
   return FSALE_OK;
 }
 
@@ -1264,6 +1266,17 @@ void step_state(fsale_ctx *ctx, std::vector<std::pair<void *, size_t>> &out) {
     }
   }
   out.push_back({ctx->s, sizeof(DevScalars) * ctx->max_problems});
+}
+
+// The packed copies the step kernels read, rebuilt from the arrays a
+// snapshot restores: the Eulerian state records, the Lagrangian nodal
+// solve's cell state and coordinates.
+int refresh_packed(fsale_ctx *ctx) {
+  if (ctx->solver == FSALE_SOLVER_EULER) return euler_pack(ctx);
+  int rc;
+  for (int p = 0; p < ctx->max_problems; ++p)
+    if (ctx->probs[p].live && (rc = maire_pack_problem(ctx, p, ctx->stream))) return rc;
+  return FSALE_OK;
 }
 
 struct Snapshot {
@@ -1328,9 +1341,9 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
   for (int c : cands) {
     for (int k = 0; k < K_COUNT; ++k) ctx->shape.blocks_per_cu[k] = c;
     if ((rc = snap.restore(ctx))) break;
-    if (ctx->solver == FSALE_SOLVER_EULER && (rc = euler_pack(ctx))) break;
+    if ((rc = refresh_packed(ctx))) break;
     for (int n = 0; n < nsteps && !rc; ++n)
-      rc = ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx);
+      rc = ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx, n == 0);
     if (rc) break;
     // the two nodal kernels are one launch at one block per tile: they are
     // chosen together, by their summed time per step
@@ -1362,6 +1375,7 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
   ctx->profiling = was_profiling;
   for (int k = 0; k < K_COUNT; ++k) std::swap(saved[k], ctx->prof[k]);
   int rc2 = snap.restore(ctx);
+  if (rc2 == FSALE_OK) rc2 = refresh_packed(ctx);
   for (int p = 0; p < ctx->max_problems && !rc2; ++p)
     if (ctx->probs[p].live) rc2 = upload_params(ctx, p);
   if (rc2 == FSALE_OK) rc2 = read_scalars(ctx);

@@ -119,8 +119,8 @@ struct DevMesh {
   // (centroid 0-2, normal 3-5 as stored for the face, area 6), so the fused
   // step reads it coalesced instead of gathering 8 bytes per 128-byte line
   double *kf[7] = {};
-  // 2D Eulerian: vertex coordinates again as (x, y) pairs [cap_v], one
-  // 16-byte gather per vertex
+  // 2D: vertex coordinates again as (x, y) pairs [cap_v], one 16-byte
+  // gather per vertex (the Lagrangian kernels that move vertices write both)
   double2 *xy = nullptr;
 };
 
@@ -199,10 +199,13 @@ struct MaireFields {
   // 3D: what the nodal solve gathers, packed per entity so that a corner's
   // gathers touch one cache line per cell or vertex instead of one per array
   // (xa and cs written by every kernel that writes their sources, and by
-  // maire3_pack_problem after host writes)
+  // maire_pack_problem after host writes)
   int32_t *cv8 = nullptr;              // [cap_c][8] cell vertices (batch ids), written with the mesh
   double *xa = nullptr;                // [cap_v][4] coordinates (x, y, z, 0)
   double *cs = nullptr;                // [cap_c][6] (rho a, p, v, 0)
+  // 2D: the same as two planes of pairs, (rho a, p) and (v_x, v_y) [cap_c]
+  // (the coordinates are DevMesh::xy)
+  double2 *cs2[2] = {};
 };
 
 // Kernel arguments common to all step kernels
@@ -335,9 +338,9 @@ int dev_alloc_n(fsale_ctx *ctx, T **p, size_t n) {
 }
 int read_scalars(fsale_ctx *ctx);      // sync + copy DevScalars to h_s (and lift slot quarantines)
 int set_maire_boundary(fsale_ctx *ctx, int prob, int tag, int kind, const double *value);
-// 3D Lagrangian contexts: refreshes the nodal solve's packed copies of one
+// Lagrangian contexts: refreshes the nodal solve's packed copies of one
 // problem's cell state and coordinates (no-op otherwise)
-int maire3_pack_problem(fsale_ctx *ctx, int prob, hipStream_t stream);
+int maire_pack_problem(fsale_ctx *ctx, int prob, hipStream_t stream);
 
 // Retirement of finished problems while the next round runs (serving loop).
 // stage_field_downloads enqueues, on the upload stream and after all work
@@ -379,5 +382,5 @@ int euler_pack(fsale_ctx *ctx);       // per-field state -> the driver loop's pa
 int euler_unpack(fsale_ctx *ctx);     // and back
 // Maire (maire.hip)
 int maire_alloc(fsale_ctx *ctx);
-int maire_step(fsale_ctx *ctx);
+int maire_step(fsale_ctx *ctx, bool first);
 }  // namespace fsale

@@ -37,6 +37,12 @@ __device__ inline void put_cs(const MaireFields &u, int c, double d, double a, d
 __device__ inline void put_xa(const MaireFields &u, int v, double x, double y, double z) {
   reinterpret_cast<double4 *>(u.xa)[v] = make_double4(x, y, z, 0.0);
 }
+// 2D: (rho a, p) and (v_x, v_y) planes; coordinate pairs
+__device__ inline void put_cs2(const MaireFields &u, int c, double d, double a, double p, double v0, double v1) {
+  u.cs2[0][c] = make_double2(d * a, p);
+  u.cs2[1][c] = make_double2(v0, v1);
+}
+__device__ inline void put_xy(const DevMesh &m, int v, double x, double y) { m.xy[v] = make_double2(x, y); }
 
 // ------------------------------------------------------------ state init
 __global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u, const ProbParams *pp, int prob) {
@@ -58,7 +64,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_ic(DevMesh m, MaireFields u, c
   u.T[c] = eos_T_de(P.cv, ie);
   const double a = smax(eos_a_de(g, ie), kMinSoundSpeed);
   u.a[c] = a;
-  if (m.dim == 3) put_cs(u, c, d, a, u.p[c], u.v[0][c], u.v[1][c], u.v[2][c]);
+  if (m.dim == 3)
+    put_cs(u, c, d, a, u.p[c], u.v[0][c], u.v[1][c], u.v[2][c]);
+  else
+    put_cs2(u, c, d, a, u.p[c], u.v[0][c], u.v[1][c]);
 }
 
 // wedge facets of corner (c, k): wedge 0 on edge v_{k-1} -> v_k, wedge 1 on
@@ -68,9 +77,10 @@ __device__ inline void wedge_facets(const DevMesh &m, int c, int k, double n[2][
   const int km = (k + 3) & 3, kp = (k + 1) & 3;
   const int im = m.cv[(size_t)km * m.cap_c + c], ik = m.cv[(size_t)k * m.cap_c + c],
             ip = m.cv[(size_t)kp * m.cap_c + c];
-  const double xm = m.x[0][im], ym = m.x[1][im];
-  const double xk = m.x[0][ik], yk = m.x[1][ik];
-  const double xp = m.x[0][ip], yp = m.x[1][ip];
+  const double2 wm = m.xy[im], wk = m.xy[ik], wp = m.xy[ip];
+  const double xm = wm.x, ym = wm.y;
+  const double xk = wk.x, yk = wk.y;
+  const double xp = wp.x, yp = wp.y;
   const double ax[2] = {xm, xk}, ay[2] = {ym, yk}, bx[2] = {xk, xp}, by[2] = {yk, yp};
 #pragma unroll
   for (int w = 0; w < 2; ++w) {
@@ -88,7 +98,7 @@ __device__ inline void wedge_facets(const DevMesh &m, int c, int k, double n[2][
 // factor z_c l_w) and (l n)_pc = sum_w l_w n_w.
 __device__ inline void corner_matrix_from(const MaireFields &u, int c, const double n[2][2], const double l[2],
                                           double M[2][2], double np[2]) {
-  const double zc = u.d[c] * u.a[c];
+  const double zc = u.cs2[0][c].x;   // rho a, as the producers of rho and a multiplied it
   M[0][0] = M[0][1] = M[1][0] = M[1][1] = 0;
   np[0] = np[1] = 0;
 #pragma unroll
@@ -339,6 +349,18 @@ __global__ void __launch_bounds__(kBlock) k_maire3_pack(DevMesh m, MaireFields u
   if (i < nv) {
     const int v = v0 + i;
     put_xa(u, v, m.x[0][v], m.x[1][v], m.x[2][v]);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_maire2_pack(DevMesh m, MaireFields u, int c0, int nc, int v0, int nv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) {
+    const int c = c0 + i;
+    put_cs2(u, c, u.d[c], u.a[c], u.p[c], u.v[0][c], u.v[1][c]);
+  }
+  if (i < nv) {
+    const int v = v0 + i;
+    put_xy(m, v, m.x[0][v], m.x[1][v]);
   }
 }
 
@@ -642,8 +664,9 @@ __device__ inline void nodal3_tile(const DevMesh &m, const MaireFields &u, const
 
 __device__ inline void corner_force(const MaireFields &u, int c, const double M[2][2], const double np[2],
                                     double F[2]) {
-  const double uc[2] = {u.v[0][c], u.v[1][c]};
-  const double pc = u.p[c];
+  const double2 zp = u.cs2[0][c], vc = u.cs2[1][c];
+  const double uc[2] = {vc.x, vc.y};
+  const double pc = zp.y;
   for (int r = 0; r < 2; ++r) {
     F[r] = 0;
     F[r] += M[r][0] * uc[0];
@@ -780,12 +803,16 @@ k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
   }
 }
 
-template <int N>
+// DT: also the time step's per-wave partials of the new residual (the
+// driver step's predictor, whose time step follows), as k_maire_dt_partial
+// computes them.
+template <int N, bool DT>
 __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, Batch b) {
   constexpr int V = N == 2 ? 4 : 8;
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
-    if ((int)threadIdx.x >= tile.count) return;
+    double acc = 0, vol = 0;
+    if ((int)threadIdx.x < tile.count) {
     const int c = tile.first + threadIdx.x;
     double r[N + 2];
 #pragma unroll
@@ -811,6 +838,20 @@ __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireField
     }
 #pragma unroll
     for (int i = 0; i < N + 2; ++i) u.dudt[i][c] = r[i];
+    if (DT) {
+      acc = u.a[c] / m.cminlen[c];
+      vol = fabs(r[0]) / m.cvol[c];
+    }
+    }
+    if (DT) {
+      acc = wave_max(acc);
+      vol = wave_max(vol);
+      if ((threadIdx.x & 63) == 0) {
+        const size_t i = 2 * ((size_t)tile.slot * kWaves + (threadIdx.x >> 6));
+        u.partials[i] = acc;
+        u.partials[i + 1] = vol;
+      }
+    }
   });
 }
 
@@ -884,7 +925,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
       x[d] = x0[d] + delta_t * u.un[d][vt];
       m.x[d][vt] = x[d];
     }
-    if constexpr (N == 3) put_xa(u, vt, x[0], x[1], x[2]);
+    if constexpr (N == 3)
+      put_xa(u, vt, x[0], x[1], x[2]);
+    else
+      put_xy(m, vt, x[0], x[1]);
   });
 }
 
@@ -907,9 +951,9 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
       if constexpr (N == 2) {
         double px[4], py[4];
         for (int i = 0; i < 4; ++i) {
-          const int v = m.cv[(size_t)i * m.cap_c + c];
-          px[i] = m.x[0][v];
-          py[i] = m.x[1][v];
+          const double2 w = m.xy[m.cv[(size_t)i * m.cap_c + c]];
+          px[i] = w.x;
+          py[i] = w.y;
         }
         double cx, cy, ml;
         cell_geom_2d(4, px, py, cx, cy, vol, ml);
@@ -964,7 +1008,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
     u.p[c] = p;
     u.T[c] = eos_T_de(P.cv, ie);
     u.a[c] = a;
-    if constexpr (N == 3) put_cs(u, c, d, a, p, vel[0], vel[1], vel[2]);
+    if constexpr (N == 3)
+      put_cs(u, c, d, a, p, vel[0], vel[1], vel[2]);
+    else
+      put_cs2(u, c, d, a, p, vel[0], vel[1]);
     if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, m.corig[c], token);
   });
 }
@@ -982,7 +1029,12 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
       else
         m.x[d][i] = u.x0[d][i];
     }
-    if (N == 3 && dir == 1) put_xa(u, i, m.x[0][i], m.x[1][i], m.x[2][i]);
+    if (dir == 1) {
+      if constexpr (N == 3)
+        put_xa(u, i, m.x[0][i], m.x[1][i], m.x[2][i]);
+      else
+        put_xy(m, i, m.x[0][i], m.x[1][i]);
+    }
   });
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
@@ -994,7 +1046,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_copy(DevMesh m, MaireFields u,
     } else {
       for (int d = 0; d < N; ++d) u.v[d][i] = u.v0[d][i];
       u.e[i] = u.e0[i];
-      if constexpr (N == 3) put_cs(u, i, u.d[i], u.a[i], u.p[i], u.v[0][i], u.v[1][i], u.v[2][i]);
+      if constexpr (N == 3)
+        put_cs(u, i, u.d[i], u.a[i], u.p[i], u.v[0][i], u.v[1][i], u.v[2][i]);
+      else
+        put_cs2(u, i, u.d[i], u.a[i], u.p[i], u.v[0][i], u.v[1][i]);
     }
   });
 }
@@ -1010,15 +1065,20 @@ __global__ void k_maire_begin(Batch b) {
     S.active = (S.err_code == 0) && (S.steps < P.max_steps) && (S.t < P.final_time);
   }
 }
+// ... which also opens the next step as k_maire_begin would, so a step that
+// follows another in one batch of steps needs no begin launch
 __global__ void k_maire_end(Batch b) {
   const int nprob = *b.np;
   for (int ip = blockIdx.x * blockDim.x + threadIdx.x; ip < nprob; ip += gridDim.x * blockDim.x) {
     const int p = b.plist[ip];
-    if (b.pp[p].cell_count == 0) continue;
+    const ProbParams &P = b.pp[p];
+    if (P.cell_count == 0) continue;
     DevScalars &S = b.s[p];
-    if (!S.active || S.err_code) continue;
-    S.t += S.dt;
-    S.steps = S.steps + 1;
+    if (S.active && !S.err_code) {
+      S.t += S.dt;
+      S.steps = S.steps + 1;
+    }
+    S.active = (S.err_code == 0) && (S.steps < P.max_steps) && (S.t < P.final_time);
   }
 }
 
@@ -1079,19 +1139,25 @@ int launch_nodal_n(fsale_ctx *ctx) {
                                 batch_args(ctx, K_MAIRE_NODAL_BND));
 }
 int launch_nodal(fsale_ctx *ctx) { return ctx->m.dim == 3 ? launch_nodal_n<3>(ctx) : launch_nodal_n<2>(ctx); }
-int launch_residual(fsale_ctx *ctx) {
+int launch_residual(fsale_ctx *ctx, bool with_dt = false) {
   const Batch b = batch_args(ctx, K_MAIRE_RESIDUAL);
-  return ctx->m.dim == 3 ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<3>, ctx->m, ctx->ma, b)
-                         : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2>, ctx->m, ctx->ma, b);
+  if (with_dt)
+    return ctx->m.dim == 3
+               ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<3, true>, ctx->m, ctx->ma, b)
+               : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, true>, ctx->m, ctx->ma, b);
+  return ctx->m.dim == 3 ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<3, false>, ctx->m, ctx->ma, b)
+                         : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, false>, ctx->m, ctx->ma, b);
 }
-int launch_dt(fsale_ctx *ctx) {
-  int rc = launch_persistent(ctx, K_MAIRE_DT, k_maire_dt_partial, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_DT));
-  if (rc) return rc;
+int launch_dt_finalize(fsale_ctx *ctx) {
   ProfScope prof(ctx, K_MAIRE_DT);
   hipLaunchKernelGGL(k_maire_dt_finalize, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream, batch_args(ctx, -1),
                      (const double *)ctx->ma.partials);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
+}
+int launch_dt(fsale_ctx *ctx) {
+  int rc = launch_persistent(ctx, K_MAIRE_DT, k_maire_dt_partial, ctx->m, ctx->ma, batch_args(ctx, K_MAIRE_DT));
+  return rc ? rc : launch_dt_finalize(ctx);
 }
 // token: the update launch's id for error attribution (predictor 1,
 // corrector 2, task-level apply_update 3)
@@ -1127,15 +1193,18 @@ int launch_copy(fsale_ctx *ctx, int dir) {
 // restore_solution / restore_coordinates are not needed: the corrector's
 // update reads the saved v0, e0, and its move writes x = x0 + dt u (when it
 // does not move, dt <= 1e-14, the predictor did not move either and x is
-// still x0).
-int maire_step(fsale_ctx *ctx) {
+// still x0).  `first`: the first step of a batch of steps, which opens its
+// step with k_maire_begin (later ones are opened by the previous k_maire_end).
+int maire_step(fsale_ctx *ctx, bool first) {
   int rc;
   const int np = (int)ctx->probs.size();
-  hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
-  FSALE_HIP(hipGetLastError());
+  if (first) {
+    hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
+    FSALE_HIP(hipGetLastError());
+  }
   if ((rc = launch_nodal(ctx))) return rc;
-  if ((rc = launch_residual(ctx))) return rc;
-  if ((rc = launch_dt(ctx))) return rc;
+  if ((rc = launch_residual(ctx, true))) return rc;
+  if ((rc = launch_dt_finalize(ctx))) return rc;
   if ((rc = launch_apply(ctx, 0.5, 1, 1))) return rc;
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx))) return rc;
@@ -1163,6 +1232,9 @@ int maire_alloc(fsale_ctx *ctx) {
   for (int i = 0; i < m.dim + 2; ++i)
     if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &u.partials, 2 * (size_t)kWaves * (ctx->ra_ct.capacity + 1)))) return rc;
+  if (m.dim == 2)
+    for (int i = 0; i < 2; ++i)
+      if ((rc = dev_alloc_n(ctx, &u.cs2[i], m.cap_c))) return rc;
   if (m.dim == 3) {
     if ((rc = dev_alloc_n(ctx, &u.cv8, (size_t)m.cap_c * 8 + 8))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.xa, (size_t)m.cap_v * 4 + 8))) return rc;
@@ -1171,12 +1243,16 @@ int maire_alloc(fsale_ctx *ctx) {
   return FSALE_OK;
 }
 
-int maire3_pack_problem(fsale_ctx *ctx, int prob, hipStream_t stream) {
+int maire_pack_problem(fsale_ctx *ctx, int prob, hipStream_t stream) {
   const Problem &P = ctx->probs[prob];
+  if (ctx->solver != FSALE_SOLVER_MAIRE) return FSALE_OK;
   const int n = P.nc > P.nv ? P.nc : P.nv;
-  if (ctx->solver != FSALE_SOLVER_MAIRE || ctx->m.dim != 3 || n == 0) return FSALE_OK;
-  hipLaunchKernelGGL(k_maire3_pack, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ctx->m, ctx->ma, P.c0,
-                     P.nc, P.v0, P.nv);
+  if (n == 0) return FSALE_OK;
+  const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+  if (ctx->m.dim == 2)
+    hipLaunchKernelGGL(k_maire2_pack, grid, block, 0, stream, ctx->m, ctx->ma, P.c0, P.nc, P.v0, P.nv);
+  else
+    hipLaunchKernelGGL(k_maire3_pack, grid, block, 0, stream, ctx->m, ctx->ma, P.c0, P.nc, P.v0, P.nv);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
