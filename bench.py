@@ -46,9 +46,18 @@ def kernel_bytes_euler(nc, nf, nv, dim):
         b = {0: 52 * nc + 24 * nf, 1: 216 * nc + 16 * nv, 2: 160 * nc + 80 * nf, 3: 116 * nc + 40 * nf}
     else:
         b = {0: 68 * nc + 32 * nf, 1: 392 * nc + 24 * nv, 2: 248 * nc + 104 * nf, 3: 140 * nc + 48 * nf}
-    # the step loop's fused kernel does the flux and update tasks, and every
-    # second launch also the dt task of the next step
-    b[10] = b[2] + b[3] + b[0] // 2
+    # The step loop's fused kernel does the flux and update tasks (and every
+    # second launch the next step's dt) without the face flux array, the
+    # face->cell map and the update's second read of the state between them:
+    # its own compulsory bytes are, per cell, the state (N+4 values), the
+    # gradients ((N+4)*N), centroid, volume, cell->face codes (int32) and the
+    # new state written (d, v, e, p, T, a); per face, its centroid, normal and
+    # area.  The sum of the two tasks' bytes is kept as "task_bytes".
+    if dim == 2:
+        b[10] = 240 * nc + 40 * nf
+    else:
+        b[10] = 344 * nc + 56 * nf
+    b["task_10"] = b[2] + b[3] + b[0] // 2
     return b
 
 
@@ -325,10 +334,12 @@ def main():
         # gradients, fused fluxes + update (+ next dt)
         nparts = 8 * ((nc + 256 * 8 - 1) // (256 * 8))
         kb = {0: 8 * nparts, 1: tasks[1], 10: tasks[10]}
+        task_bytes = {10: tasks["task_10"]}
     else:
         kb = (kernel_bytes_maire if dim == 2 else kernel_bytes_maire3)(nc, ctx.nv, nc * ctx.vpc)
         launches_per_step = {4: 2, 5: 2, 6: 1, 7: 2, 8: 2, 9: 2}
         bytes_per_step = sum(kb[k] * launches_per_step[k] for k in kb)
+        task_bytes = {}
 
     roofline = None
     kernels = {}
@@ -349,6 +360,8 @@ def main():
                 avg_s = tot / nl * 1e-3
                 kernels[fa.KERNEL_NAMES[k]] = {"avg_us": avg_s * 1e6, "launches": nl,
                                                "alg_bytes": kb[k], "GBps": kb[k] / avg_s / 1e9}
+                if k in task_bytes:
+                    kernels[fa.KERNEL_NAMES[k]]["task_bytes"] = task_bytes[k]
         dom = max(kernels, key=lambda name: kernels[name]["avg_us"] * kernels[name]["launches"])
         d = kernels[dom]
         traffic = load_traffic(args.workload)
@@ -387,6 +400,7 @@ def main():
         "config": config,
         "alg_bytes_per_step": bytes_per_step,
         "alg_GBps": bytes_per_step * K / el / 1e9,
+        "alg_frac_of_peak": round(bytes_per_step * K / el / 1e9 / PEAK_HBM_GBS, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

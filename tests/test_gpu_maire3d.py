@@ -188,3 +188,20 @@ def test_task_sequence_3d_matches_oracle(oracle_lib):
         un, npc, Fpc = O.maire_nodal(mesh, s)
         assert np.array_equal(ctx.get(fa.F_NODE_VELOCITY), un)
         assert np.array_equal(ctx.get(fa.F_CORNER_FORCE), Fpc)
+
+
+def test_autotune_keeps_3d_results(oracle_lib):
+    """The autotuner times candidate shapes from a snapshot of a moved mesh
+    and restores it, packed copies of state and coordinates included: steps
+    after it equal the oracle's continuation bit for bit."""
+    O = oracle_lib
+    prob = inputs.maire_bubble_3d(6, steps=8, seed=5)
+    mesh, s, ctx = _setup(O, prob)
+    print("3D autotune regression 6^3", flush=True)
+    with ctx:
+        ctx.advance(prob["cfl"], prob["final_time"], 3)
+        assert ctx.autotune(2) > 0
+        ctx.advance(prob["cfl"], prob["final_time"], 5)
+        rc, t_o, n_o, dt_o = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], 8, prob["dt0"])
+        assert rc == 0 and n_o == 8
+        _bitwise(ctx, s, mesh, "after autotune")
