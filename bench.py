@@ -378,6 +378,7 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']:.4e} zone-updates/s ({cpu['sample']})")
 
     ctx_shape = ctx.get_launch_shape()
+    lds = ctx.lds_staging() if kind == "euler" else None
     ctx.close()
     serving = None
     if not args.no_serve:
@@ -405,6 +406,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "launch_shape": ctx_shape,
+        "lds_staging": lds,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -65,7 +65,7 @@ EXPORTS = [
     "fsale_batch_create", "fsale_batch_add_euler", "fsale_batch_add_maire", "fsale_batch_remove",
     "fsale_batch_step", "fsale_batch_status", "fsale_batch_get_field", "fsale_batch_get_fields",
     "fsale_batch_capacity",
-    "fsale_launch_shape", "fsale_get_launch_shape", "fsale_autotune",
+    "fsale_launch_shape", "fsale_get_launch_shape", "fsale_lds_staging", "fsale_autotune",
     "fsale_server_create", "fsale_server_destroy", "fsale_server_clock", "fsale_server_submit",
     "fsale_server_poll", "fsale_server_times", "fsale_server_result", "fsale_server_stats",
     "fsale_exodus_info", "fsale_exodus_read", "fsale_dump",
@@ -181,6 +181,7 @@ def lib():
                                            ct.POINTER(i64), ct.POINTER(i32)]
         L.fsale_launch_shape.argtypes = [vp, i32, i32]
         L.fsale_get_launch_shape.argtypes = [vp, vp]
+        L.fsale_lds_staging.argtypes = [vp, ct.POINTER(i32)]
         L.fsale_autotune.argtypes = [vp, i32, ct.POINTER(dp)]
         L.fsale_server_create.argtypes = [ct.POINTER(ServerConfig), dp, ct.POINTER(vp)]
         L.fsale_server_destroy.argtypes = [vp]
@@ -349,6 +350,12 @@ class _Context:
         out = np.zeros(len(KERNEL_NAMES), np.int32)
         _check(lib().fsale_get_launch_shape(self.h, out.ctypes.data))
         return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
+
+    def lds_staging(self):
+        """Whether the Eulerian driver kernels stage cell tiles in LDS."""
+        on = ct.c_int32()
+        _check(lib().fsale_lds_staging(self.h, ct.byref(on)))
+        return bool(on.value)
 
     def autotune(self, nsteps=3):
         """Pick each kernel's launch shape by timing real steps; returns the tuned step's kernel ms."""
@@ -576,6 +583,12 @@ class Batch:
         out = np.zeros(len(KERNEL_NAMES), np.int32)
         _check(lib().fsale_get_launch_shape(self.h, out.ctypes.data))
         return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
+
+    def lds_staging(self):
+        """Whether the Eulerian driver kernels stage cell tiles in LDS."""
+        on = ct.c_int32()
+        _check(lib().fsale_lds_staging(self.h, ct.byref(on)))
+        return bool(on.value)
 
     def autotune(self, nsteps=3):
         """Pick each kernel's launch shape by timing real steps; returns the tuned step's kernel ms."""

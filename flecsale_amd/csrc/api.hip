@@ -192,8 +192,14 @@ int sync_tiles(fsale_ctx *ctx) {
       ctx->h_btiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nb - i * kBlock), P.vt0 + i});
   }
   std::vector<int> plist;
+  ctx->lds = ctx->lds_all = ctx->m.lnb != nullptr;
   for (int p = 0; p < (int)ctx->probs.size(); ++p)
-    if (ctx->probs[p].live) plist.push_back(p);
+    if (ctx->probs[p].live) {
+      plist.push_back(p);
+      if (!ctx->probs[p].lds) ctx->lds = false;
+      if (!ctx->probs[p].lds_all) ctx->lds_all = false;
+    }
+  if (!ctx->lds) ctx->lds_all = false;
   const int counts[4] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size(),
                          (int)ctx->h_btiles.size()};
   if (!plist.empty())
@@ -504,6 +510,12 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
       if ((rc = dev_alloc_n(ctx, &m.kf[i], (size_t)cap_c * m.fpc))) return rc;
   if (dim == 2)
     if ((rc = dev_alloc_n(ctx, &m.xy, cap_v))) return rc;
+  if (dim == 2 && vpc == 4 && solver == FSALE_SOLVER_EULER) {
+    if ((rc = dev_alloc_n(ctx, &m.halo_cnt, (size_t)cap_ct))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.halo, (size_t)cap_ct * kHalo2))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.lnb, (size_t)cap_c * (kNbk2 / 2)))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.lfn, (size_t)cap_c * 2))) return rc;
+  }
   if ((rc = dev_alloc_n(ctx, &m.cvol, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cminlen, cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.farea, cap_f))) return rc;
@@ -630,6 +642,8 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   size_t bytes = 4 * ((size_t)nc * (t.vpc + 2 * t.fpc + m.nbk_deg + 2) + (size_t)nf * (2 + t.vpf) + P.nn) +
                  8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
   bytes += 4 * (size_t)nc;   // original ids
+  const bool lds = m.lnb && !t.halo_cnt.empty();
+  if (lds) bytes += 4 * ((size_t)P.nct * (1 + kHalo2) + (size_t)nc * (kNbk2 / 2 + 2));
   if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + 2 * (size_t)nc * t.vpc + nv + 64 * t.vpc;
   if (maire && m.dim == 3) bytes += 32 * (size_t)nc + 64;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
@@ -689,6 +703,27 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
            const int cnt = t.nbr_off[c + 1] - t.nbr_off[c];
            const int pad = cnt < deg ? deg - cnt : 0;
            return k >= pad ? cpos[t.nbr[t.nbr_off[c] + k - pad]] + c0 : -1;
+         })))
+      return rc;
+  }
+  ctx->probs[prob].lds = lds;
+  ctx->probs[prob].lds_all =
+      lds && std::all_of(t.halo_cnt.begin(), t.halo_cnt.end(), [](int32_t h) { return h >= 0; });
+  if (lds) {
+    // the tiles' LDS staging plan (tile slots from P.ct0), halo cells as batch ids
+    const int ct0 = P.ct0;
+    if ((rc = rows(m.halo_cnt + ct0, 1, 0, P.nct, [&t](int, int j) { return t.halo_cnt[j]; }))) return rc;
+    if ((rc = rows(m.halo + (size_t)ct0 * kHalo2, 1, 0, P.nct * kHalo2,
+                   [&t, c0](int, int i) { return t.halo[i] + c0; })))
+      return rc;
+    if ((rc = rows(m.lnb + c0, kNbk2 / 2, m.cap_c, nc, [&t](int k, int s) {
+           const uint16_t *q = t.nb_slot.data() + (size_t)s * kNbk2 + 2 * k;
+           return (int32_t)((uint32_t)q[0] | ((uint32_t)q[1] << 16));
+         })))
+      return rc;
+    if ((rc = rows(m.lfn + c0, 2, m.cap_c, nc, [&t](int k, int s) {
+           const uint16_t *q = t.face_slot.data() + (size_t)s * 4 + 2 * k;
+           return (int32_t)((uint32_t)q[0] | ((uint32_t)q[1] << 16));
          })))
       return rc;
   }
@@ -1220,6 +1255,7 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   std::vector<double> key = {(double)ctx->solver, (double)chunk};
   for (int k = 0; k < K_COUNT; ++k) key.push_back((double)persistent_grid(ctx, k));
   key.push_back((double)ctx->pgrid);
+  key.push_back((double)ctx->lds + 2.0 * ctx->lds_all);
   if (!ctx->graph.exec || ctx->graph.key != key) {
     if (ctx->graph.exec) {
       (void)hipGraphExecDestroy(ctx->graph.exec);
@@ -1866,6 +1902,14 @@ int fsale_launch_shape(fsale_ctx *ctx, int32_t kernel, int32_t blocks_per_cu) {
 int fsale_get_launch_shape(fsale_ctx *ctx, int32_t *blocks_per_cu) {
   if (!ctx || !blocks_per_cu) return FSALE_ERR_ARG;
   for (int k = 0; k < K_COUNT; ++k) blocks_per_cu[k] = ctx->shape.blocks_per_cu[k];
+  return FSALE_OK;
+}
+
+int fsale_lds_staging(fsale_ctx *ctx, int32_t *on) {
+  if (!ctx || !on) return FSALE_ERR_ARG;
+  int rc;
+  if ((rc = sync_tiles(ctx))) return rc;
+  *on = ctx->lds ? 1 : 0;
   return FSALE_OK;
 }
 

@@ -162,17 +162,62 @@ __device__ __forceinline__ void st_rec(double2 *const (&pl)[P], int c, const dou
   for (int i = 0; i < K / 2; ++i) st(pl[i], off, make_double2(r[2 * i], r[2 * i + 1]));
 }
 
-// PK: read the packed state records (gs) and write packed gradients (gg),
-// as the driver loop does; otherwise the per-field arrays (task level).
-template <int N, int VPC, bool PK>
-__device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0, int first_order) {
+// LDS staging of a 2D tile: planes of pairs, the tile's cells at slots
+// [0, kBlock), its halo from slot kBlock (plane i at sh + i * kLdsW)
+constexpr int kLdsW = kBlock + kHalo2;
+
+// One record from the staged planes.
+template <int K>
+__device__ __forceinline__ void lds_rec(const double2 *sh, int slot, double (&r)[K]) {
+#pragma unroll
+  for (int i = 0; i < K / 2; ++i) {
+    const double2 w = sh[i * kLdsW + slot];
+    r[2 * i] = w.x;
+    r[2 * i + 1] = w.y;
+  }
+}
+
+// Stages one tile with h halo cells: the records of its cells and of its
+// halo, planes `plane(i)` for i < NP, into sh (all threads of the block;
+// barriers before, so the previous tile's reads are done, and after).  A
+// tile of a problem that does not step is staged too (its waves then
+// return): reads only, of cells and halo lists in bounds whoever owns them.
+template <int NP, class P>
+__device__ __forceinline__ void stage_tile(double2 *sh, const DevMesh &m, const Tile &tile, int h, P &&plane) {
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < tile.count) {
+    const uint32_t off = (uint32_t)(tile.first + t) * 16u;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) sh[i * kLdsW + t] = ld(plane(i), off);
+  }
+  for (int j = t; j < h; j += kBlock) {
+    const uint32_t off = (uint32_t)m.halo[(size_t)tile.slot * kHalo2 + j] * 16u;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) sh[i * kLdsW + kBlock + j] = ld(plane(i), off);
+  }
+  __syncthreads();
+}
+
+// MODE 0: the per-field arrays (task level); 1: the packed state records
+// (gs), as the driver loop does; 2 (2D): the records staged in LDS (sh),
+// neighbours by their LDS slots (DevMesh::lnb).  Modes 1 and 2 write the
+// packed gradients (gg).
+template <int N, int VPC, int MODE>
+__device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u, int c0, int first_order,
+                                 const double2 *sh = nullptr) {
+  static_assert(MODE != 2 || N == 2, "LDS staging is 2D");
+  constexpr bool PK = MODE != 0;
   constexpr int NV = N + 4;
   const double tol = 1.e-9;
   const uint32_t o0 = (uint32_t)c0 * 8u, i0 = (uint32_t)c0 * 4u;
   double u0[NV], umin[NV], umax[NV], dudx[NV][N], dxdx[N][N], x0[N];
   if constexpr (PK) {
     double r[Rec<N>::S];
-    ld_rec(u.gs, c0, r);
+    if constexpr (MODE == 2)
+      lds_rec(sh, (int)threadIdx.x, r);
+    else
+      ld_rec(u.gs, c0, r);
     for (int j = 0; j < NV; ++j) u0[j] = r[j];
     for (int d = 0; d < N; ++d) x0[d] = r[Rec<N>::C + d];
   } else {
@@ -202,11 +247,19 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   constexpr int NBK = (N == 2) ? kNbk2 : kNbk3;
   const size_t row = (size_t)m.cap_c;
   int nb[N == 2 ? NBK : 1];
-  if constexpr (N == 2) {
+  if constexpr (MODE == 2) {
+    // LDS slots; the tile's plan holds every neighbour in the k-major head
+#pragma unroll
+    for (int k = 0; k < NBK / 2; ++k) {
+      const uint32_t w = (uint32_t)ld(m.lnb + k * row, i0);
+      nb[2 * k] = (int)(w & 0xffffu);
+      nb[2 * k + 1] = (int)(w >> 16);
+    }
+  } else if constexpr (N == 2) {
 #pragma unroll
     for (int k = 0; k < NBK; ++k) nb[k] = ld(m.nbk + k * row, i0);
   }
-  const int cnt = ld(m.nbr_cnt, i0);
+  const int cnt = MODE == 2 ? 0 : ld(m.nbr_cnt, i0);
   double xv[VPC][N];
   auto load_vertex = [&](int lv) {
     const uint32_t iv = (uint32_t)ld(m.cv + lv * row, i0);
@@ -224,7 +277,10 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
     double u1[NV], dx[N];
     if constexpr (PK) {
       double r[Rec<N>::S];
-      ld_rec(u.gs, c1, r);
+      if constexpr (MODE == 2)
+        lds_rec(sh, c1, r);
+      else
+        ld_rec(u.gs, c1, r);
       for (int j = 0; j < NV; ++j) u1[j] = r[j];
       for (int d = 0; d < N; ++d) dx[d] = r[Rec<N>::C + d] - x0[d];
     } else {
@@ -258,7 +314,10 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
       for (int d = 0; d < N; ++d) dudx[j][d] += du * dx[d];
     }
   };
-  if constexpr (N == 2) {
+  if constexpr (MODE == 2) {
+#pragma unroll
+    for (int k = 0; k < NBK; ++k) neighbour(nb[k]);
+  } else if constexpr (N == 2) {
 #pragma unroll
     for (int k = 0; k < NBK; ++k) neighbour(nb[k] >= 0 ? nb[k] : c0);
   } else {
@@ -358,23 +417,57 @@ __device__ inline void grad_cell(const DevMesh &m, const EulerFields &u, int c0,
   }
 }
 
-template <int N, int VPC, bool PK>
+template <int N, int VPC, int MODE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(N == 2 ? FSALE_GRAD_WAVES_2D : 3)))
 k_eul_grad(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x < tile.count)
-      grad_cell<N, VPC, PK>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
+      grad_cell<N, VPC, MODE>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
+  });
+}
+
+// 2D quads with LDS staging plans: a tile with a plan has its state records
+// and its halo's staged in LDS and takes the gradients from there; a tile
+// without one (halo_cnt -1: a halo over kHalo2 cells or a cell past the
+// k-major head) gathers from memory.  The choice is per tile, so uniform
+// across the block and its barriers.
+// ALL: every tile of the launch has a plan (the memory path is not
+// compiled in, which leaves the staged path more registers).
+template <int VPC, bool ALL>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_eul_grad_lds(DevMesh m, EulerFields u, Batch b) {
+  constexpr int NP = Rec<2>::S / 2;
+  __shared__ double2 sh[NP * kLdsW];
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    const int first_order = b.pp[tile.prob].first_order;
+    const int h = m.halo_cnt[tile.slot];
+    if (!ALL && h < 0) {
+      if (!wave_live(b.s, tile.prob, 0)) return;
+      if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 1>(m, u, tile.first + threadIdx.x, first_order);
+      return;
+    }
+    stage_tile<NP>(sh, m, tile, h, [&](int i) { return u.gs[i]; });
+    if (!wave_live(b.s, tile.prob, 0)) return;
+    if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 2>(m, u, tile.first + threadIdx.x, first_order, sh);
   });
 }
 
 // ------------------------------------------------- packed driver records
-// The per-field state into the state records (gs), every cell of the tile
-// lists whatever its problem's flags, so a batch of steps starts from them.
+// The per-field state into the state records (gs) of the problems that step
+// in this batch of steps: no error latched, step budget left, and (when the
+// loop stops at the final time) time left, the step flag the first step's
+// time-step kernel will compute.  The decision is kept in the problem's
+// scalars for the unpack.  Problems that do not step are not touched: a
+// finished problem's ranges may already hold a new admission's state.
 template <int N>
 __global__ void __launch_bounds__(kBlock) k_eul_pack(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if ((int)threadIdx.x >= tile.count) return;
+    const ProbParams &P = b.pp[tile.prob];
+    DevScalars &S = b.s[tile.prob];
+    const bool steps = S.err_code == 0 && S.steps < P.max_steps && (!P.stop_at_final || S.t < P.final_time);
+    if (tile.first == P.cell_first && threadIdx.x == 0) S.packed = steps;
+    if (!steps || (int)threadIdx.x >= tile.count) return;
     const int c = tile.first + threadIdx.x;
     double r[Rec<N>::S];
     r[0] = u.d[c];
@@ -387,12 +480,13 @@ __global__ void __launch_bounds__(kBlock) k_eul_pack(DevMesh m, EulerFields u, B
   });
 }
 
-// ... and back when the batch of steps ends; T = e / cv as the update
-// computes it (update_state_from_energy, flecsale/eqns/euler_eqns.h:359-375).
+// ... and back when the batch of steps ends, for the problems packed; T = e
+// / cv as the update computes it (update_state_from_energy,
+// flecsale/eqns/euler_eqns.h:359-375).
 template <int N>
 __global__ void __launch_bounds__(kBlock) k_eul_unpack(EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if ((int)threadIdx.x >= tile.count) return;
+    if (!b.s[tile.prob].packed || (int)threadIdx.x >= tile.count) return;
     const int c = tile.first + threadIdx.x;
     double r[Rec<N>::S];
     ld_rec(u.gs, c, r);
@@ -429,12 +523,18 @@ __device__ inline void load_cell(const DevMesh &m, const EulerFields &u, int c, 
   for (int i = 0; i < N * N; ++i) q.gv[i] = u.gv[i][c];
 }
 
-// the same from the driver loop's packed records
-template <int N>
-__device__ inline void load_cell_pk(const EulerFields &u, int c, CellData<N> &q) {
+// the same from the driver loop's packed records: from memory, or (2D) from
+// a tile staged in LDS, state planes first, then the gradient planes
+template <int N, bool LDS = false>
+__device__ inline void load_cell_pk(const EulerFields &u, int c, CellData<N> &q, const double2 *sh = nullptr) {
   double r[Rec<N>::S], g[Rec<N>::G];
-  ld_rec(u.gs, c, r);
-  ld_rec(u.gg, c, g);
+  if constexpr (LDS) {
+    lds_rec(sh, c, r);
+    lds_rec(sh + (Rec<N>::S / 2) * kLdsW, c, g);
+  } else {
+    ld_rec(u.gs, c, r);
+    ld_rec(u.gg, c, g);
+  }
   q.d = r[0];
   for (int k = 0; k < N; ++k) q.v[k] = r[1 + k];
   q.e = r[N + 1];
@@ -577,103 +677,137 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
 }
 
 // fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
-// over the packed records: reads u.gs and u.gg, writes out.gs
-template <int N, int FPC, int VPC>
-__global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, Batch b,
-                                                        double factor, int want_dt, int token) {
-  for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!wave_live(b.s, tile.prob, token)) return;
-    const ProbParams &P = b.pp[tile.prob];
-    const int kind = P.kind;
-    double delta_t = b.s[tile.prob].dt;
-    delta_t *= factor;
-    const int c = tile.first + threadIdx.x;
-    if (c == P.cell_first) b.s[tile.prob].t = b.s[tile.prob].t + delta_t;   // update_soln_time :554
-    double dt_inv = 0;
-    if ((int)threadIdx.x < tile.count) {
-      // the face-neighbour codes first: each face's neighbour loads then
-      // wait on one round trip, not two
-      int cnk[FPC];
+// of one tile over the packed records: reads u.gs and u.gg, writes out.gs;
+// LDS: from the tile's records staged in sh
+template <int N, int FPC, int VPC, bool LDS>
+__device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields &u, const EulerFields &out, const Batch &b,
+                                    double factor, int want_dt, int token, const Tile &tile, const double2 *sh) {
+  if (!wave_live(b.s, tile.prob, token)) return;
+  const ProbParams &P = b.pp[tile.prob];
+  const int kind = P.kind;
+  double delta_t = b.s[tile.prob].dt;
+  delta_t *= factor;
+  const int c = tile.first + threadIdx.x;
+  if (c == P.cell_first) b.s[tile.prob].t = b.s[tile.prob].t + delta_t;   // update_soln_time :554
+  double dt_inv = 0;
+  if ((int)threadIdx.x < tile.count) {
+    // the face-neighbour codes first: each face's neighbour loads then
+    // wait on one round trip, not two
+    int cnk[FPC], slot[FPC];
 #pragma unroll
-      for (int k = 0; k < FPC; ++k) cnk[k] = m.cfn[(size_t)k * m.cap_c + c];
-      CellData<N> own;
-      load_cell_pk<N>(u, c, own);
-      double px[VPC], py[VPC];
-      if constexpr (N == 2) {
-        for (int k = 0; k < VPC; ++k) {
-          const double2 w = m.xy[m.cv[(size_t)k * m.cap_c + c]];
-          px[k] = w.x;
-          py[k] = w.y;
-        }
-      }
-      double du[N + 2];
-      for (int i = 0; i < N + 2; ++i) du[i] = 0;
-      double nk[FPC][N], ak[FPC];
+    for (int k = 0; k < FPC; ++k) cnk[k] = m.cfn[(size_t)k * m.cap_c + c];
+    if constexpr (LDS) {
 #pragma unroll
-      for (int k = 0; k < FPC; ++k) {
-        const int cn = cnk[k];
-        const bool second = cn & 1;     // this cell is the face's cells[1]
-        const int other = (cn >> 1) - 1;
-        double fx[N];
-        if constexpr (N == 2) {
-          const int k1 = (k + 1 == VPC) ? 0 : k + 1;
-          // the face runs v_k -> v_k1 out of cells[0], reversed for cells[1]
-          if (!second)
-            face_geom_2d(px[k], py[k], px[k1], py[k1], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
-          else
-            face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
-        } else {
-          const size_t o = (size_t)k * m.cap_c + c;
-          for (int d = 0; d < N; ++d) {
-            fx[d] = m.kf[d][o];
-            nk[k][d] = m.kf[3 + d][o];
-          }
-          ak[k] = m.kf[6][o];
-        }
-        // the neighbour's loads do not wait on a branch (a boundary face
-        // loads the cell's own data again and ignores it)
-        CellData<N> nb;
-        load_cell_pk<N>(u, other >= 0 ? other : c, nb);
-        double fl[N + 2];
-        if (!second)
-          face_flux_area<N>(kind, own, nb, other >= 0, fx, nk[k], ak[k], fl);
-        else
-          face_flux_area<N>(kind, nb, own, true, fx, nk[k], ak[k], fl);
-        if (second) {
-          for (int i = 0; i < N + 2; ++i) du[i] += fl[i];
-        } else {
-          for (int i = 0; i < N + 2; ++i) du[i] -= fl[i];
-        }
-      }
-      const double vol = m.cvol[c];
-      double vel[N], mass, ie;
-      for (int d = 0; d < N; ++d) vel[d] = own.v[d];
-      update_values<N>(own.d, vel, own.e, du, delta_t / vol, mass, ie);
-      const double a = eos_a_de(P.gamma, ie);
-      {
-        double r[Rec<N>::S];
-        r[0] = mass;
-        for (int d = 0; d < N; ++d) r[1 + d] = vel[d];
-        r[N + 1] = ie;
-        r[N + 2] = eos_p_de(P.gamma, mass, ie);
-        r[N + 3] = a;
-        for (int d = 0; d < N; ++d) r[Rec<N>::C + d] = own.cx[d];
-        st_rec(out.gs, c, r);
-      }
-      if (ie < 0 || mass < 0)   // apps/hydro/tasks.h:654-655
-        flag_negative(b.s, tile.prob, m.corig[c], token);
-      if (want_dt) {
-#pragma unroll
-        for (int k = 0; k < FPC; ++k) {
-          const double delta_x = vol / ak[k];
-          const double dti = (a + fabs(dotN<N>(vel, nk[k]))) / delta_x;
-          dt_inv = smax(dti, dt_inv);
-        }
+      for (int k = 0; k < FPC / 2; ++k) {
+        const uint32_t w = (uint32_t)m.lfn[(size_t)k * m.cap_c + c];
+        slot[2 * k] = (int)(w & 0xffffu);
+        slot[2 * k + 1] = (int)(w >> 16);
       }
     }
+    CellData<N> own;
+    if constexpr (LDS)
+      load_cell_pk<N, true>(u, (int)threadIdx.x, own, sh);
+    else
+      load_cell_pk<N>(u, c, own);
+    double px[VPC], py[VPC];
+    if constexpr (N == 2) {
+      for (int k = 0; k < VPC; ++k) {
+        const double2 w = m.xy[m.cv[(size_t)k * m.cap_c + c]];
+        px[k] = w.x;
+        py[k] = w.y;
+      }
+    }
+    double du[N + 2];
+    for (int i = 0; i < N + 2; ++i) du[i] = 0;
+    double nk[FPC][N], ak[FPC];
+#pragma unroll
+    for (int k = 0; k < FPC; ++k) {
+      const int cn = cnk[k];
+      const bool second = cn & 1;     // this cell is the face's cells[1]
+      const int other = (cn >> 1) - 1;
+      double fx[N];
+      if constexpr (N == 2) {
+        const int k1 = (k + 1 == VPC) ? 0 : k + 1;
+        // the face runs v_k -> v_k1 out of cells[0], reversed for cells[1]
+        if (!second)
+          face_geom_2d(px[k], py[k], px[k1], py[k1], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
+        else
+          face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
+      } else {
+        const size_t o = (size_t)k * m.cap_c + c;
+        for (int d = 0; d < N; ++d) {
+          fx[d] = m.kf[d][o];
+          nk[k][d] = m.kf[3 + d][o];
+        }
+        ak[k] = m.kf[6][o];
+      }
+      // the neighbour's loads do not wait on a branch (a boundary face
+      // loads the cell's own data again and ignores it)
+      CellData<N> nb;
+      if constexpr (LDS)
+        load_cell_pk<N, true>(u, slot[k], nb, sh);
+      else
+        load_cell_pk<N>(u, other >= 0 ? other : c, nb);
+      double fl[N + 2];
+      if (!second)
+        face_flux_area<N>(kind, own, nb, other >= 0, fx, nk[k], ak[k], fl);
+      else
+        face_flux_area<N>(kind, nb, own, true, fx, nk[k], ak[k], fl);
+      if (second) {
+        for (int i = 0; i < N + 2; ++i) du[i] += fl[i];
+      } else {
+        for (int i = 0; i < N + 2; ++i) du[i] -= fl[i];
+      }
+    }
+    const double vol = m.cvol[c];
+    double vel[N], mass, ie;
+    for (int d = 0; d < N; ++d) vel[d] = own.v[d];
+    update_values<N>(own.d, vel, own.e, du, delta_t / vol, mass, ie);
+    const double a = eos_a_de(P.gamma, ie);
+    {
+      double r[Rec<N>::S];
+      r[0] = mass;
+      for (int d = 0; d < N; ++d) r[1 + d] = vel[d];
+      r[N + 1] = ie;
+      r[N + 2] = eos_p_de(P.gamma, mass, ie);
+      r[N + 3] = a;
+      for (int d = 0; d < N; ++d) r[Rec<N>::C + d] = own.cx[d];
+      st_rec(out.gs, c, r);
+    }
+    if (ie < 0 || mass < 0)   // apps/hydro/tasks.h:654-655
+      flag_negative(b.s, tile.prob, m.corig[c], token);
     if (want_dt) {
-      dt_inv = wave_max(dt_inv);
-      if ((threadIdx.x & 63) == 0) u.partials[tile.slot * kWaves + (threadIdx.x >> 6)] = dt_inv;
+#pragma unroll
+      for (int k = 0; k < FPC; ++k) {
+        const double delta_x = vol / ak[k];
+        const double dti = (a + fabs(dotN<N>(vel, nk[k]))) / delta_x;
+        dt_inv = smax(dti, dt_inv);
+      }
+    }
+  }
+  if (want_dt) {
+    dt_inv = wave_max(dt_inv);
+    if ((threadIdx.x & 63) == 0) u.partials[tile.slot * kWaves + (threadIdx.x >> 6)] = dt_inv;
+  }
+}
+
+// LDS (2D quads with staging plans): a tile with a plan is staged in LDS
+// and its face neighbours read there (DevMesh::lfn); a tile without one
+// gathers from memory.  The choice is per tile, uniform across the block.
+// ALL: every tile of the launch has a plan (the memory path not compiled in).
+template <int N, int FPC, int VPC, bool LDS, bool ALL = false>
+__global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, Batch b,
+                                                        double factor, int want_dt, int token) {
+  static_assert(!LDS || N == 2, "LDS staging is 2D");
+  constexpr int NP = (Rec<N>::S + Rec<N>::G) / 2;
+  __shared__ double2 sh[LDS ? NP * kLdsW : 1];
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    const int h = LDS ? m.halo_cnt[tile.slot] : -1;
+    if (ALL || h >= 0) {
+      stage_tile<NP>(sh, m, tile, h, [&](int i) { return i < Rec<N>::S / 2 ? u.gs[i] : u.gg[i - Rec<N>::S / 2]; });
+      fluxupd_tile<N, FPC, VPC, LDS>(m, u, out, b, factor, want_dt, token, tile, sh);
+    } else {
+      fluxupd_tile<N, FPC, VPC, false>(m, u, out, b, factor, want_dt, token, tile, sh);
     }
   });
 }
@@ -713,25 +847,34 @@ int launch_finalize(fsale_ctx *ctx, int driver) {
 
 int launch_grad(fsale_ctx *ctx, const EulerFields &u) {
   const Batch b = batch_args(ctx, K_EUL_GRAD);
-  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, false>, ctx->m, u, b);
-  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, false>, ctx->m, u, b);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, 0>, ctx->m, u, b);
+  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 0>, ctx->m, u, b);
 }
 
 int launch_grad_pk(fsale_ctx *ctx, const EulerFields &u) {
   const Batch b = batch_args(ctx, K_EUL_GRAD);
-  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, true>, ctx->m, u, b);
-  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, true>, ctx->m, u, b);
+  if (ctx->m.dim == 2 && ctx->lds_all)
+    return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, true>, ctx->m, u, b);
+  if (ctx->m.dim == 2 && ctx->lds) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, false>, ctx->m, u, b);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, 1>, ctx->m, u, b);
+  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 1>, ctx->m, u, b);
 }
 
 // token: the launch's id for error attribution (first half step 1, second 2)
 int launch_fluxupd(fsale_ctx *ctx, double factor, bool want_dt, int token, const EulerFields &in,
                    const EulerFields &out) {
   const Batch b = batch_args(ctx, K_EUL_FLUXUPD);
+  if (ctx->m.dim == 2 && ctx->lds_all)
+    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4, true, true>, ctx->m, in, out, b, factor,
+                             (int)want_dt, token);
+  if (ctx->m.dim == 2 && ctx->lds)
+    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4, true>, ctx->m, in, out, b, factor,
+                             (int)want_dt, token);
   if (ctx->m.dim == 2)
-    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4>, ctx->m, in, out, b, factor, (int)want_dt,
-                             token);
-  return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8>, ctx->m, in, out, b, factor, (int)want_dt,
-                           token);
+    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4, false>, ctx->m, in, out, b, factor,
+                             (int)want_dt, token);
+  return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8, false>, ctx->m, in, out, b, factor,
+                           (int)want_dt, token);
 }
 
 bool check_ctx(fsale_ctx *ctx) {
@@ -764,9 +907,8 @@ int euler_step(fsale_ctx *ctx) {
 
 int euler_dt_partials(fsale_ctx *ctx) { return launch_dt_partial(ctx); }
 
-// Around a batch of driver steps: the per-field state into the packed
-// records, and back.  Every cell of the tile lists is copied, also of
-// problems that do not step, so both directions leave those unchanged.
+// Around a batch of driver steps: the per-field state of the problems that
+// step into the packed records, and back.
 int euler_pack(fsale_ctx *ctx) {
   const Batch b = batch_args(ctx, -1);
   const dim3 grid(persistent_grid(ctx, -1)), block(kBlock);

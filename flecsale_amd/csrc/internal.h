@@ -28,6 +28,10 @@ constexpr int kMaxCornersPerVertex = 8;
 // neighbours held k-major per cell: all cell-vertex neighbours of an interior
 // cell of a structured quad (8) or hex (26) mesh
 constexpr int kNbk2 = 8, kNbk3 = 28;   // 3D: 26 hex neighbours, padded to groups of 4
+// 2D quad tiles staged in LDS by the Eulerian driver kernels: at most this
+// many cells outside a tile (a 16 x 16 block of a structured mesh has 68),
+// sized so that three blocks of the fused kernel's staging fit one CU's LDS
+constexpr int kHalo2 = 80;
 
 void set_message(const std::string &msg);
 
@@ -70,6 +74,17 @@ struct HostTopology {
   // whose neighbours stay in its L2 (a whole plane of a 128^3 mesh would
   // not).  Empty: storage order.
   std::vector<int32_t> tile_order;
+  // 2D quads: LDS staging plan of the cell tiles (kBlock consecutive storage
+  // positions each).  halo_cnt[tile]: the number of cells outside the tile
+  // that its cells' neighbour lists name (-1: more than kHalo2, or a cell
+  // with more neighbours than the k-major head holds); halo[tile*kHalo2 + i]:
+  // their storage positions, ascending.  nb_slot[s*kNbk2 + k]: the LDS slot
+  // (position - tile start, or kBlock + halo index) of neighbour slot k of
+  // the cell at storage position s, right-aligned like the k-major head
+  // (padding slots name the cell itself); face_slot[s*4 + k]: that of the
+  // cell across local face k (the cell itself on the boundary).
+  std::vector<int32_t> halo_cnt, halo;
+  std::vector<uint16_t> nb_slot, face_slot;
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
@@ -122,6 +137,14 @@ struct DevMesh {
   // 2D: vertex coordinates again as (x, y) pairs [cap_v], one 16-byte
   // gather per vertex (the Lagrangian kernels that move vertices write both)
   double2 *xy = nullptr;
+  // 2D quads, Eulerian: the cell tiles' LDS staging plan (HostTopology::
+  // halo_cnt, halo) by tile slot, in batch ids, and per cell the LDS slots of
+  // its neighbour slots (lnb) and face neighbours (lfn) as 16-bit pairs,
+  // k-major: row k holds slots 2k (low half) and 2k + 1
+  int32_t *halo_cnt = nullptr;   // [tile capacity]
+  int32_t *halo = nullptr;       // [tile capacity * kHalo2]
+  int32_t *lnb = nullptr;        // [kNbk2 / 2][cap_c]
+  int32_t *lfn = nullptr;        // [2][cap_c]
 };
 
 // A tile: `count` (<= kBlock) consecutive entities of problem `prob`
@@ -164,7 +187,7 @@ struct DevScalars {
   long long err_cell;   // lowest failing cell (local id of the problem)
   long long steps;      // completed steps
   int err_token;        // launch that raised err_code (0: none)
-  int pad;
+  int packed;           // Eulerian: steps in the current batch of steps (its state was packed)
 };
 
 struct EulerFields {
@@ -253,6 +276,8 @@ struct Problem {
   std::shared_ptr<const HostTopology> topo;
   ProbParams params{};
   bool faces_on_device = true;   // face connectivity and geometry uploaded
+  bool lds = false;              // the cell tiles' LDS staging plan is uploaded
+  bool lds_all = false;          // ... and every tile has one (no tile gathers from memory)
   bool have_state = false;
   bool partials_valid = false;
 };
@@ -309,6 +334,8 @@ struct fsale_ctx {
   fsale::DevScalars *h_s = nullptr;    // pinned host mirror
   std::vector<fsale::Tile> h_ctiles, h_vtiles, h_btiles;
   bool tiles_dirty = true;
+  bool lds = false;                    // every live problem has LDS staging plans (sync_tiles)
+  bool lds_all = false;                // ... for every one of its tiles
   void *stage = nullptr;               // pinned staging buffer for uploads
   size_t stage_bytes = 0;
   void *dl = nullptr;                  // pinned buffer for multi-field downloads

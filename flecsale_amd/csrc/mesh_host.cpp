@@ -85,26 +85,33 @@ inline uint64_t spread2(uint64_t x) {
   return x;
 }
 
-// Storage order mode, FSALE_STORAGE_ORDER = identity (default) | morton |
-// blocks.  For meshes numbered with locality (structured, Exodus) the given
-// order coalesces the gathers best; the curves help meshes numbered without it.
-int storage_order_mode() {
+// Storage order mode, FSALE_STORAGE_ORDER = identity | morton | blocks, by
+// default blocks in 2D and identity in 3D.  2D: blocks of ~256 cells make
+// the Eulerian cell tiles compact, so they are staged in LDS with small
+// halos.  3D: the given order coalesces the gathers best on meshes numbered
+// with locality (structured, Exodus).
+int storage_order_mode(int dim) {
   static const int mode = [] {
     const char *e = std::getenv("FSALE_STORAGE_ORDER");
-    if (!e) return 0;
+    if (!e) return -1;
+    if (!std::strcmp(e, "identity")) return 0;
     if (!std::strcmp(e, "morton")) return 1;
     if (!std::strcmp(e, "blocks")) return 2;
-    return 0;
+    return -1;
   }();
-  return mode;
+  return mode >= 0 ? mode : (dim == 2 ? 2 : 0);
 }
 
 // Order of n points: identity; Z-order (Morton) of the points; or Z-order of
 // spatial blocks of ~256 points with the original order inside a block.
 // idx[s] = the point at position s, pos its inverse; ties keep id order.
+// box (optional, 2*dim values lo then hi): the region the blocks divide,
+// e.g. the mesh's extent for cell centroids, so that a structured mesh's
+// blocks hold whole 16 x 16 (x 16 ...) cell groups instead of being cut at
+// the centroids' own, half a cell smaller, extent.
 void morton_order(int n, int dim, const std::vector<double> &pts, std::vector<int32_t> &idx,
-                  std::vector<int32_t> &pos) {
-  const int mode = storage_order_mode();
+                  std::vector<int32_t> &pos, const double *box = nullptr) {
+  const int mode = storage_order_mode(dim);
   if (mode == 0) {
     idx.resize(n);
     pos.resize(n);
@@ -116,6 +123,11 @@ void morton_order(int n, int dim, const std::vector<double> &pts, std::vector<in
     for (int d = 0; d < dim; ++d) {
       lo[d] = std::min(lo[d], pts[(size_t)i * dim + d]);
       hi[d] = std::max(hi[d], pts[(size_t)i * dim + d]);
+    }
+  if (box && mode == 2)
+    for (int d = 0; d < dim; ++d) {
+      lo[d] = std::min(lo[d], box[d]);
+      hi[d] = std::max(hi[d], box[dim + d]);
     }
   double levels[3];
   for (int d = 0; d < dim; ++d) levels[d] = dim == 3 ? 2097151.0 : 2147483647.0;
@@ -153,6 +165,50 @@ void morton_order(int n, int dim, const std::vector<double> &pts, std::vector<in
   }
 }
 }  // namespace
+
+// LDS staging plan of a 2D quad mesh's cell tiles (HostTopology::halo_cnt,
+// halo, nb_slot, face_slot), in storage positions.
+void build_halo_plan(HostTopology &t) {
+  const int T = (t.nc + kBlock - 1) / kBlock;
+  t.halo_cnt.assign(T, -1);
+  t.halo.assign((size_t)T * kHalo2, 0);
+  t.nb_slot.assign((size_t)t.nc * kNbk2, 0);
+  t.face_slot.assign((size_t)t.nc * 4, 0);
+  std::vector<int32_t> h;
+  for (int j = 0; j < T; ++j) {
+    const int s0 = j * kBlock, s1 = std::min(t.nc, s0 + kBlock);
+    h.clear();
+    bool ok = true;
+    for (int s = s0; s < s1 && ok; ++s) {
+      const int c = t.cidx[s];
+      if (t.nbr_off[c + 1] - t.nbr_off[c] > kNbk2) ok = false;
+      for (int q = t.nbr_off[c]; q < t.nbr_off[c + 1]; ++q) {
+        const int p = t.cpos[t.nbr[q]];
+        if (p < s0 || p >= s1) h.push_back(p);
+      }
+    }
+    std::sort(h.begin(), h.end());
+    h.erase(std::unique(h.begin(), h.end()), h.end());
+    if (!ok || (int)h.size() > kHalo2) continue;
+    t.halo_cnt[j] = (int32_t)h.size();
+    std::copy(h.begin(), h.end(), t.halo.begin() + (size_t)j * kHalo2);
+    auto slot = [&](int p) {
+      if (p >= s0 && p < s1) return (uint16_t)(p - s0);
+      return (uint16_t)(kBlock + (std::lower_bound(h.begin(), h.end(), p) - h.begin()));
+    };
+    for (int s = s0; s < s1; ++s) {
+      const int c = t.cidx[s];
+      const int off = t.nbr_off[c], cnt = t.nbr_off[c + 1] - off, pad = kNbk2 - cnt;
+      for (int k = 0; k < kNbk2; ++k)
+        t.nb_slot[(size_t)s * kNbk2 + k] = k >= pad ? slot(t.cpos[t.nbr[off + k - pad]]) : (uint16_t)(s - s0);
+      for (int k = 0; k < 4; ++k) {
+        const int f = t.cf[(size_t)c * 4 + k];
+        const int other = t.fc[2 * (size_t)f] == c ? t.fc[2 * (size_t)f + 1] : t.fc[2 * (size_t)f];
+        t.face_slot[(size_t)s * 4 + k] = other >= 0 ? slot(t.cpos[other]) : (uint16_t)(s - s0);
+      }
+    }
+  }
+}
 
 // Pencil order of a 3D mesh's cell tiles (HostTopology::tile_order) from the
 // tiles' centroids; cellsum holds each cell's vertex-coordinate sum.
@@ -344,11 +400,26 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
   // device storage order of cells, vertices and faces (vertex averages)
   {
     const int D = t.dim;
+    // the mesh's extent, scaled like the vertex sums the cells and faces are
+    // placed by
+    double ext[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int v = 0; v < t.nv; ++v)
+      for (int d = 0; d < D; ++d) {
+        ext[d] = std::min(ext[d], mesh.coords[(size_t)v * D + d]);
+        ext[3 + d] = std::max(ext[3 + d], mesh.coords[(size_t)v * D + d]);
+      }
+    double cbox[6], fbox[6];
+    for (int d = 0; d < D; ++d) {
+      cbox[d] = t.vpc * ext[d];
+      cbox[D + d] = t.vpc * ext[3 + d];
+      fbox[d] = t.vpf * ext[d];
+      fbox[D + d] = t.vpf * ext[3 + d];
+    }
     std::vector<double> pts((size_t)t.nc * D, 0.0);
     for (int c = 0; c < t.nc; ++c)
       for (int k = 0; k < t.vpc; ++k)
         for (int d = 0; d < D; ++d) pts[(size_t)c * D + d] += mesh.coords[(size_t)t.cv[(size_t)c * t.vpc + k] * D + d];
-    morton_order(t.nc, D, pts, t.cidx, t.cpos);
+    morton_order(t.nc, D, pts, t.cidx, t.cpos, cbox);
     if (D == 3) tile_pencils(t, pts);
     pts.assign(mesh.coords, mesh.coords + (size_t)t.nv * D);
     morton_order(t.nv, D, pts, t.vidx, t.vpos);
@@ -356,8 +427,9 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
     for (int f = 0; f < t.nf; ++f)
       for (int k = 0; k < t.vpf; ++k)
         for (int d = 0; d < D; ++d) pts[(size_t)f * D + d] += mesh.coords[(size_t)t.fv[(size_t)f * t.vpf + k] * D + d];
-    morton_order(t.nf, D, pts, t.fidx, t.fpos);
+    morton_order(t.nf, D, pts, t.fidx, t.fpos, fbox);
   }
+  if (t.dim == 2 && t.vpc == 4) build_halo_plan(t);
   return FSALE_OK;
 }
 
@@ -392,7 +464,8 @@ size_t topology_bytes(const HostTopology &t) {
   return sizeof(int32_t) * (t.cv.size() + t.cf.size() + t.fc.size() + t.fv.size() + t.nbr_off.size() +
                             t.nbr.size() + t.vc_off.size() + t.vc.size() + t.tile_order.size() +
                             2 * (t.cidx.size() + t.vidx.size() +
-                            t.fidx.size()));
+                            t.fidx.size()) + t.halo_cnt.size() + t.halo.size()) +
+         sizeof(uint16_t) * (t.nb_slot.size() + t.face_slot.size());
 }
 
 uint64_t mesh_hash(const fsale_mesh_desc &m) {
