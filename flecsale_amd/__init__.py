@@ -352,10 +352,10 @@ class _Context:
         return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
 
     def lds_staging(self):
-        """Whether the Eulerian driver kernels stage cell tiles in LDS."""
+        """Eulerian driver kernels' LDS staging: 2 every tile, 1 some tiles, 0 none."""
         on = ct.c_int32()
         _check(lib().fsale_lds_staging(self.h, ct.byref(on)))
-        return bool(on.value)
+        return int(on.value)
 
     def autotune(self, nsteps=3):
         """Pick each kernel's launch shape by timing real steps; returns the tuned step's kernel ms."""
@@ -585,10 +585,10 @@ class Batch:
         return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
 
     def lds_staging(self):
-        """Whether the Eulerian driver kernels stage cell tiles in LDS."""
+        """Eulerian driver kernels' LDS staging: 2 every tile, 1 some tiles, 0 none."""
         on = ct.c_int32()
         _check(lib().fsale_lds_staging(self.h, ct.byref(on)))
-        return bool(on.value)
+        return int(on.value)
 
     def autotune(self, nsteps=3):
         """Pick each kernel's launch shape by timing real steps; returns the tuned step's kernel ms."""

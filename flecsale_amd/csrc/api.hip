@@ -643,7 +643,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
                  8 * (size_t)nv * m.dim + 8 * (size_t)nc * (m.dim + 3) + sizeof(ProbParams) + sizeof(DevScalars);
   bytes += 4 * (size_t)nc;   // original ids
   const bool lds = m.lnb && !t.halo_cnt.empty();
-  if (lds) bytes += 4 * ((size_t)P.nct * (1 + kHalo2) + (size_t)nc * (kNbk2 / 2 + 2));
+  if (lds) bytes += 4 * (size_t)P.nct * (1 + kHalo2);
   if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + 2 * (size_t)nc * t.vpc + nv + 64 * t.vpc;
   if (maire && m.dim == 3) bytes += 32 * (size_t)nc + 64;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
@@ -716,16 +716,6 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     if ((rc = rows(m.halo + (size_t)ct0 * kHalo2, 1, 0, P.nct * kHalo2,
                    [&t, c0](int, int i) { return t.halo[i] + c0; })))
       return rc;
-    if ((rc = rows(m.lnb + c0, kNbk2 / 2, m.cap_c, nc, [&t](int k, int s) {
-           const uint16_t *q = t.nb_slot.data() + (size_t)s * kNbk2 + 2 * k;
-           return (int32_t)((uint32_t)q[0] | ((uint32_t)q[1] << 16));
-         })))
-      return rc;
-    if ((rc = rows(m.lfn + c0, 2, m.cap_c, nc, [&t](int k, int s) {
-           const uint16_t *q = t.face_slot.data() + (size_t)s * 4 + 2 * k;
-           return (int32_t)((uint32_t)q[0] | ((uint32_t)q[1] << 16));
-         })))
-      return rc;
   }
   if (nbr_tail) {
     if ((rc = rows(m.nbr_start + c0, 1, 0, nc, [&t, cidx, n0](int, int s) { return t.nbr_off[cidx[s]] + n0; })))
@@ -741,6 +731,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
       return rc;
   }
   if ((rc = st.run_deferred())) return rc;
+  if (lds && (rc = euler_lds_slots(ctx, prob, ctx->upload))) return rc;
   if ((rc = launch_geometry(ctx, prob, true, faces, ctx->upload))) return rc;
   if ((rc = maire_pack_problem(ctx, prob, ctx->upload))) return rc;
   if (maire) {
@@ -1909,7 +1900,7 @@ int fsale_lds_staging(fsale_ctx *ctx, int32_t *on) {
   if (!ctx || !on) return FSALE_ERR_ARG;
   int rc;
   if ((rc = sync_tiles(ctx))) return rc;
-  *on = ctx->lds ? 1 : 0;
+  *on = ctx->lds_all ? 2 : ctx->lds ? 1 : 0;
   return FSALE_OK;
 }
 

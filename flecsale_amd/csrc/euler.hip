@@ -453,6 +453,41 @@ k_eul_grad_lds(DevMesh m, EulerFields u, Batch b) {
   });
 }
 
+// LDS slots of one problem's cells (DevMesh::lnb, lfn): a neighbour in the
+// cell's own tile is its position in the tile, any other its place in the
+// tile's sorted halo list (which holds every one for a tile with a plan)
+// after the kBlock own slots; padding slots and boundary faces name the
+// cell itself.
+__global__ void __launch_bounds__(kBlock) k_lds_slots(DevMesh m, int c0, int nc, int ct0) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  const int j = i / kBlock, slot = ct0 + j;
+  const int first = c0 + j * kBlock, count = min(kBlock, nc - j * kBlock);
+  const int h = m.halo_cnt[slot];
+  if (h < 0) return;
+  const int32_t *hl = m.halo + (size_t)slot * kHalo2;
+  const int c = c0 + i;
+  const size_t row = (size_t)m.cap_c;
+  auto local = [&](int n) -> uint32_t {
+    if (n < 0) return (uint32_t)(c - first);
+    if (n >= first && n < first + count) return (uint32_t)(n - first);
+    int lo = 0, hi = h;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (hl[mid] < n)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return (uint32_t)(kBlock + lo);
+  };
+  for (int k = 0; k < kNbk2 / 2; ++k)
+    m.lnb[k * row + c] = (int32_t)(local(m.nbk[2 * k * row + c]) | (local(m.nbk[(2 * k + 1) * row + c]) << 16));
+  for (int k = 0; k < 2; ++k)
+    m.lfn[k * row + c] = (int32_t)(local((m.cfn[2 * k * row + c] >> 1) - 1) |
+                                   (local((m.cfn[(2 * k + 1) * row + c] >> 1) - 1) << 16));
+}
+
 // ------------------------------------------------- packed driver records
 // The per-field state into the state records (gs) of the problems that step
 // in this batch of steps: no error latched, step budget left, and (when the
@@ -909,6 +944,15 @@ int euler_dt_partials(fsale_ctx *ctx) { return launch_dt_partial(ctx); }
 
 // Around a batch of driver steps: the per-field state of the problems that
 // step into the packed records, and back.
+int euler_lds_slots(fsale_ctx *ctx, int prob, hipStream_t stream) {
+  const Problem &P = ctx->probs[prob];
+  if (!ctx->m.lnb || P.nc == 0) return FSALE_OK;
+  hipLaunchKernelGGL(k_lds_slots, dim3((P.nc + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ctx->m, P.c0, P.nc,
+                     P.ct0);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
 int euler_pack(fsale_ctx *ctx) {
   const Batch b = batch_args(ctx, -1);
   const dim3 grid(persistent_grid(ctx, -1)), block(kBlock);

@@ -78,13 +78,9 @@ struct HostTopology {
   // positions each).  halo_cnt[tile]: the number of cells outside the tile
   // that its cells' neighbour lists name (-1: more than kHalo2, or a cell
   // with more neighbours than the k-major head holds); halo[tile*kHalo2 + i]:
-  // their storage positions, ascending.  nb_slot[s*kNbk2 + k]: the LDS slot
-  // (position - tile start, or kBlock + halo index) of neighbour slot k of
-  // the cell at storage position s, right-aligned like the k-major head
-  // (padding slots name the cell itself); face_slot[s*4 + k]: that of the
-  // cell across local face k (the cell itself on the boundary).
+  // their storage positions, ascending.  The cells' LDS slots are derived
+  // from these on the device (k_lds_slots).
   std::vector<int32_t> halo_cnt, halo;
-  std::vector<uint16_t> nb_slot, face_slot;
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
@@ -368,6 +364,9 @@ int set_maire_boundary(fsale_ctx *ctx, int prob, int tag, int kind, const double
 // Lagrangian contexts: refreshes the nodal solve's packed copies of one
 // problem's cell state and coordinates (no-op otherwise)
 int maire_pack_problem(fsale_ctx *ctx, int prob, hipStream_t stream);
+// 2D Eulerian: the LDS slots of each cell's neighbours and face neighbours
+// (DevMesh::lnb, lfn) from its k-major rows and its tile's halo list
+int euler_lds_slots(fsale_ctx *ctx, int prob, hipStream_t stream);
 
 // Retirement of finished problems while the next round runs (serving loop).
 // stage_field_downloads enqueues, on the upload stream and after all work

@@ -85,12 +85,16 @@ inline uint64_t spread2(uint64_t x) {
   return x;
 }
 
-// Storage order mode, FSALE_STORAGE_ORDER = identity | morton | blocks, by
-// default blocks in 2D and identity in 3D.  2D: blocks of ~256 cells make
-// the Eulerian cell tiles compact, so they are staged in LDS with small
-// halos.  3D: the given order coalesces the gathers best on meshes numbered
-// with locality (structured, Exodus).
-int storage_order_mode(int dim) {
+// Storage order mode, FSALE_STORAGE_ORDER = identity | morton | blocks.  By
+// default: blocks for 2D meshes of more than 512^2 cells, else identity.
+// Blocks of ~256 cells make a large 2D mesh's Eulerian cell tiles compact,
+// so they are staged in LDS with small halos (1024^2 four-quadrant MUSCL:
+// 4.49e9 vs 3.75e9 zone-updates/s).  Smaller meshes keep their own order:
+// their blocks rarely align with the 256-cell tiles, and the served stream
+// of 32^2..512^2 meshes ran at 2.48e9 zone-updates/s in identity order and
+// 2.07e9 in blocks.  3D: the given order coalesces the gathers best on
+// meshes numbered with locality (structured, Exodus).
+int storage_order_mode(int dim, int num_cells) {
   static const int mode = [] {
     const char *e = std::getenv("FSALE_STORAGE_ORDER");
     if (!e) return -1;
@@ -99,7 +103,7 @@ int storage_order_mode(int dim) {
     if (!std::strcmp(e, "blocks")) return 2;
     return -1;
   }();
-  return mode >= 0 ? mode : (dim == 2 ? 2 : 0);
+  return mode >= 0 ? mode : (dim == 2 && num_cells > 512 * 512 ? 2 : 0);
 }
 
 // Order of n points: identity; Z-order (Morton) of the points; or Z-order of
@@ -110,8 +114,7 @@ int storage_order_mode(int dim) {
 // blocks hold whole 16 x 16 (x 16 ...) cell groups instead of being cut at
 // the centroids' own, half a cell smaller, extent.
 void morton_order(int n, int dim, const std::vector<double> &pts, std::vector<int32_t> &idx,
-                  std::vector<int32_t> &pos, const double *box = nullptr) {
-  const int mode = storage_order_mode(dim);
+                  std::vector<int32_t> &pos, int mode, const double *box = nullptr) {
   if (mode == 0) {
     idx.resize(n);
     pos.resize(n);
@@ -167,13 +170,11 @@ void morton_order(int n, int dim, const std::vector<double> &pts, std::vector<in
 }  // namespace
 
 // LDS staging plan of a 2D quad mesh's cell tiles (HostTopology::halo_cnt,
-// halo, nb_slot, face_slot), in storage positions.
+// halo), in storage positions.
 void build_halo_plan(HostTopology &t) {
   const int T = (t.nc + kBlock - 1) / kBlock;
   t.halo_cnt.assign(T, -1);
   t.halo.assign((size_t)T * kHalo2, 0);
-  t.nb_slot.assign((size_t)t.nc * kNbk2, 0);
-  t.face_slot.assign((size_t)t.nc * 4, 0);
   std::vector<int32_t> h;
   for (int j = 0; j < T; ++j) {
     const int s0 = j * kBlock, s1 = std::min(t.nc, s0 + kBlock);
@@ -192,21 +193,6 @@ void build_halo_plan(HostTopology &t) {
     if (!ok || (int)h.size() > kHalo2) continue;
     t.halo_cnt[j] = (int32_t)h.size();
     std::copy(h.begin(), h.end(), t.halo.begin() + (size_t)j * kHalo2);
-    auto slot = [&](int p) {
-      if (p >= s0 && p < s1) return (uint16_t)(p - s0);
-      return (uint16_t)(kBlock + (std::lower_bound(h.begin(), h.end(), p) - h.begin()));
-    };
-    for (int s = s0; s < s1; ++s) {
-      const int c = t.cidx[s];
-      const int off = t.nbr_off[c], cnt = t.nbr_off[c + 1] - off, pad = kNbk2 - cnt;
-      for (int k = 0; k < kNbk2; ++k)
-        t.nb_slot[(size_t)s * kNbk2 + k] = k >= pad ? slot(t.cpos[t.nbr[off + k - pad]]) : (uint16_t)(s - s0);
-      for (int k = 0; k < 4; ++k) {
-        const int f = t.cf[(size_t)c * 4 + k];
-        const int other = t.fc[2 * (size_t)f] == c ? t.fc[2 * (size_t)f + 1] : t.fc[2 * (size_t)f];
-        t.face_slot[(size_t)s * 4 + k] = other >= 0 ? slot(t.cpos[other]) : (uint16_t)(s - s0);
-      }
-    }
   }
 }
 
@@ -419,15 +405,16 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
     for (int c = 0; c < t.nc; ++c)
       for (int k = 0; k < t.vpc; ++k)
         for (int d = 0; d < D; ++d) pts[(size_t)c * D + d] += mesh.coords[(size_t)t.cv[(size_t)c * t.vpc + k] * D + d];
-    morton_order(t.nc, D, pts, t.cidx, t.cpos, cbox);
+    const int mode = storage_order_mode(D, t.nc);
+    morton_order(t.nc, D, pts, t.cidx, t.cpos, mode, cbox);
     if (D == 3) tile_pencils(t, pts);
     pts.assign(mesh.coords, mesh.coords + (size_t)t.nv * D);
-    morton_order(t.nv, D, pts, t.vidx, t.vpos);
+    morton_order(t.nv, D, pts, t.vidx, t.vpos, mode);
     pts.assign((size_t)t.nf * D, 0.0);
     for (int f = 0; f < t.nf; ++f)
       for (int k = 0; k < t.vpf; ++k)
         for (int d = 0; d < D; ++d) pts[(size_t)f * D + d] += mesh.coords[(size_t)t.fv[(size_t)f * t.vpf + k] * D + d];
-    morton_order(t.nf, D, pts, t.fidx, t.fpos, fbox);
+    morton_order(t.nf, D, pts, t.fidx, t.fpos, mode, fbox);
   }
   if (t.dim == 2 && t.vpc == 4) build_halo_plan(t);
   return FSALE_OK;
@@ -464,8 +451,7 @@ size_t topology_bytes(const HostTopology &t) {
   return sizeof(int32_t) * (t.cv.size() + t.cf.size() + t.fc.size() + t.fv.size() + t.nbr_off.size() +
                             t.nbr.size() + t.vc_off.size() + t.vc.size() + t.tile_order.size() +
                             2 * (t.cidx.size() + t.vidx.size() +
-                            t.fidx.size()) + t.halo_cnt.size() + t.halo.size()) +
-         sizeof(uint16_t) * (t.nb_slot.size() + t.face_slot.size());
+                            t.fidx.size()) + t.halo_cnt.size() + t.halo.size());
 }
 
 uint64_t mesh_hash(const fsale_mesh_desc &m) {

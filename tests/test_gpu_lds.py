@@ -20,25 +20,40 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_default_2d_order_stages_every_tile_and_matches_oracle(oracle_lib):
-    """The default 2D storage order cuts a structured 64 x 64 mesh into 16 x 16
-    blocks (68 cells around each): every tile is staged."""
+STAGED = r"""
+import json, sys
+sys.path.insert(0, sys.argv[1])
+import flecsale_amd as fa
+from flecsale_amd import inputs
+prob = inputs.quadrants_2d(64, steps=6)
+with fa.EulerContext(prob["coords"], prob["cells"]) as ctx:
+    lds = ctx.lds_staging()
+    ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+    ctx.advance(prob["cfl"], prob["final_time"], 6, prob["flux"])
+    print(json.dumps({"lds": lds, "time": ctx.time()[0], "state": {k: v.tolist() for k, v in ctx.state().items()}}))
+"""
+
+
+def test_blocked_order_stages_every_tile_and_matches_oracle(oracle_lib):
+    """Spatial blocks of ~256 cells (the default order above 512^2 cells, set
+    here for a 64 x 64 mesh) cut a structured mesh into 16 x 16 tiles with 68
+    cells around each: every tile is staged, and six MUSCL steps equal the
+    oracle bit for bit."""
     O = oracle_lib
     prob = inputs.quadrants_2d(64, steps=6)
     mesh = O.Mesh(prob["coords"], prob["cells"])
     s = O.euler_initial_state(mesh, *prob["eos"], prob["d"], prob["v"], prob["p"])
     rc, t_o, n_o = O.euler_run(mesh, s, *prob["eos"], prob["cfl"], prob["final_time"], 6, flux_kind=prob["flux"])
     assert rc == 0 and n_o == 6
-    print("lds 64^2, 6 steps", flush=True)
-    with fa.EulerContext(prob["coords"], prob["cells"]) as ctx:
-        assert ctx.lds_staging()
-        ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
-        ctx.advance(prob["cfl"], prob["final_time"], 6, prob["flux"])
-        t, dt, n = ctx.time()
-        assert n == 6 and t == t_o
-        st = ctx.state()
+    print("lds 64^2, 6 steps, blocked order", flush=True)
+    env = dict(os.environ, FSALE_STORAGE_ORDER="blocks")
+    r = subprocess.run([sys.executable, "-c", STAGED, ROOT], env=env, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["lds"] == 2 and out["time"] == t_o
     for k, want in [("d", s.d), ("v", s.v), ("e", s.e), ("p", s.p), ("T", s.T), ("a", s.a)]:
-        assert np.array_equal(st[k], want), f"{k}: {rel_linf(st[k], want):.3e}"
+        got = np.asarray(out["state"][k])
+        assert np.array_equal(got, want), f"{k}: {rel_linf(got, want):.3e}"
 
 
 SCRIPT = r"""
@@ -81,17 +96,17 @@ def test_staged_and_memory_tiles_mix():
     """In the mesh's own order a 100 x 10 tube's tiles (2.6 rows of 100) have
     ~200 cells around them and gather from memory, while a 20 x 20 mesh's
     tiles are staged; in one batch both kinds of tile run in the same
-    launches.  Each problem equals itself alone, and the identity runs equal
-    the default (blocked, all staged) order's."""
+    launches.  Each problem equals itself alone, and the identity-order runs
+    equal the blocked order's (every tile staged)."""
     print("lds mix, identity order", flush=True)
     ident = _run("identity")
-    print("lds mix, default order", flush=True)
-    dflt = _run("blocks")
-    assert ident["sod_lds"] and ident["small_lds"] and ident["mix_lds"]
-    for out in (ident, dflt):
+    print("lds mix, blocked order", flush=True)
+    blk = _run("blocks")
+    assert ident["sod_lds"] == 1 and ident["small_lds"] == 2 and ident["mix_lds"] == 1
+    for out in (ident, blk):
         for got, want in zip(out["mix"], (out["sod"], out["small"])):
             for k in want:
                 assert np.array_equal(np.asarray(got[k]), np.asarray(want[k])), k
     for key in ("sod", "small"):
         for k in ident[key]:
-            assert np.array_equal(np.asarray(ident[key][k]), np.asarray(dflt[key][k])), (key, k)
+            assert np.array_equal(np.asarray(ident[key][k]), np.asarray(blk[key][k])), (key, k)
