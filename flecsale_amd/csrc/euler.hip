@@ -6,12 +6,18 @@
 //   k_eul_update                           apply_update        apps/hydro/tasks.h:589-659
 //   k_eul_fluxupd   evaluate_fluxes + apply_update (+ the next step's dt^-1
 //                   partials), one work-item per cell: the driver loop's path
+//   k_eul_grad_lds  the driver loop's gradients in 2D, tiles staged in LDS
 //
 // Every kernel but the task-level face kernel walks the batch's tile list
 // (batch_dev.h: one block per tile, or a persistent grid), so one launch
 // advances every resident problem; problem scalars (EOS, CFL, dt, time,
-// flags) come from per-problem records.  SoA FP64 state; k-major
-// fixed-degree connectivity so neighbouring lanes read neighbouring words.
+// flags) come from per-problem records.  SoA FP64 state for the task-level
+// kernels; the driver loop works on packed per-cell records (pairs of
+// doubles in planes, EulerFields::gs / gg), packed before a batch of steps
+// and unpacked after it.  k-major fixed-degree connectivity so neighbouring
+// lanes read neighbouring words.  In 2D a tile whose neighbours outside it
+// are few (its halo, DevMesh::halo) is staged in LDS with them, and its
+// cells read their neighbours there.
 //
 // k_eul_fluxupd recomputes, for each cell, the numerical flux of each of its
 // faces exactly as k_eul_flux does (same left/right states, same operation
@@ -20,7 +26,8 @@
 // In 2D the face centroid / normal / length come from the cell's own
 // vertices (bit-identical to the stored face geometry: reversing an edge
 // negates it exactly).  The flux array never goes through HBM.  Neighbours
-// must read the old state, so the state ping-pongs between two buffers.
+// must read the old state, so the state records ping-pong between two
+// buffers.
 //
 // The time step: each wave writes the max of dt^-1 over its cells to a
 // partials slot (no atomics, no block barrier); one block per problem folds
