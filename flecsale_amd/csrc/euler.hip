@@ -490,9 +490,15 @@ __global__ void __launch_bounds__(kBlock) k_lds_slots(DevMesh m, int c0, int nc,
   };
   for (int k = 0; k < kNbk2 / 2; ++k)
     m.lnb[k * row + c] = (int32_t)(local(m.nbk[2 * k * row + c]) | (local(m.nbk[(2 * k + 1) * row + c]) << 16));
+  // faces: the slot, bit 14 set when this cell is the face's cells[1], bit
+  // 15 on the boundary (what the face-neighbour code holds, so the staged
+  // kernel does not read that code)
+  auto face = [&](int code) -> uint32_t {
+    const int other = (code >> 1) - 1;
+    return local(other) | ((uint32_t)(code & 1) << 14) | ((uint32_t)(other < 0) << 15);
+  };
   for (int k = 0; k < 2; ++k)
-    m.lfn[k * row + c] = (int32_t)(local((m.cfn[2 * k * row + c] >> 1) - 1) |
-                                   (local((m.cfn[(2 * k + 1) * row + c] >> 1) - 1) << 16));
+    m.lfn[k * row + c] = (int32_t)(face(m.cfn[2 * k * row + c]) | (face(m.cfn[(2 * k + 1) * row + c]) << 16));
 }
 
 // ------------------------------------------------- packed driver records
@@ -735,16 +741,21 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
   if ((int)threadIdx.x < tile.count) {
     // the face-neighbour codes first: each face's neighbour loads then
     // wait on one round trip, not two
+    // LDS: the face-neighbour codes' orientation and boundary bits ride in
+    // the LDS slot words (k_lds_slots)
     int cnk[FPC], slot[FPC];
-#pragma unroll
-    for (int k = 0; k < FPC; ++k) cnk[k] = m.cfn[(size_t)k * m.cap_c + c];
     if constexpr (LDS) {
 #pragma unroll
       for (int k = 0; k < FPC / 2; ++k) {
         const uint32_t w = (uint32_t)m.lfn[(size_t)k * m.cap_c + c];
-        slot[2 * k] = (int)(w & 0xffffu);
-        slot[2 * k + 1] = (int)(w >> 16);
+        slot[2 * k] = (int)(w & 0x3fffu);
+        slot[2 * k + 1] = (int)((w >> 16) & 0x3fffu);
+        cnk[2 * k] = (int)((w >> 14) & 1u) | ((w & 0x8000u) ? 0 : 2);
+        cnk[2 * k + 1] = (int)((w >> 30) & 1u) | ((w & 0x80000000u) ? 0 : 2);
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < FPC; ++k) cnk[k] = m.cfn[(size_t)k * m.cap_c + c];
     }
     CellData<N> own;
     if constexpr (LDS)
