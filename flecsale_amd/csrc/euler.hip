@@ -148,6 +148,9 @@ struct Rec {
   static constexpr int S = 2 * N + 4;            // doubles per state record
   static constexpr int C = N + 4;                // centroid offset in it
   static constexpr int G = (N == 2) ? 12 : 22;   // doubles per gradient record (3D: 21 + 1 pad)
+  // state planes from here on hold centroid values only: static, written to
+  // both state buffers when the records are packed and never by a step
+  static constexpr int P_STATIC = (C + 1) / 2;
 };
 
 template <int K, int P>
@@ -161,12 +164,13 @@ __device__ __forceinline__ void ld_rec(double2 *const (&pl)[P], int c, double (&
     r[2 * i + 1] = w.y;
   }
 }
-template <int K, int P>
+// planes [I0, I1) of a record
+template <int K, int P, int I0 = 0, int I1 = K / 2>
 __device__ __forceinline__ void st_rec(double2 *const (&pl)[P], int c, const double (&r)[K]) {
-  static_assert(K / 2 <= P, "record larger than its planes");
+  static_assert(K / 2 <= P && I1 <= K / 2, "record larger than its planes");
   const uint32_t off = (uint32_t)c * 16u;
 #pragma unroll
-  for (int i = 0; i < K / 2; ++i) st(pl[i], off, make_double2(r[2 * i], r[2 * i + 1]));
+  for (int i = I0; i < I1; ++i) st(pl[i], off, make_double2(r[2 * i], r[2 * i + 1]));
 }
 
 // LDS staging of a 2D tile: planes of pairs, the tile's cells at slots
@@ -525,6 +529,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_pack(DevMesh m, EulerFields u, B
     r[N + 3] = u.a[c];
     for (int d = 0; d < N; ++d) r[Rec<N>::C + d] = m.ccen[d][c];
     st_rec(u.gs, c, r);
+    st_rec<Rec<N>::S, EulerFields::kPlanesS, Rec<N>::P_STATIC>(u.alt_gs, c, r);
   });
 }
 
@@ -825,7 +830,7 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
       r[N + 2] = eos_p_de(P.gamma, mass, ie);
       r[N + 3] = a;
       for (int d = 0; d < N; ++d) r[Rec<N>::C + d] = own.cx[d];
-      st_rec(out.gs, c, r);
+      st_rec<Rec<N>::S, EulerFields::kPlanesS, 0, Rec<N>::P_STATIC>(out.gs, c, r);   // the centroid planes stay
     }
     if (ie < 0 || mass < 0)   // apps/hydro/tasks.h:654-655
       flag_negative(b.s, tile.prob, m.corig[c], token);
