@@ -330,10 +330,10 @@ def main():
         tasks = kernel_bytes_euler(nc, ctx.nf, ctx.nv, dim)
         # task model: one dt, two (gradients, fluxes, update) per step
         bytes_per_step = tasks[0] + 2 * (tasks[1] + tasks[2] + tasks[3])
-        # kernels the step loop launches: dt finalize (reads the block partials),
-        # gradients, fused fluxes + update (+ next dt)
-        nparts = 8 * ((nc + 256 * 8 - 1) // (256 * 8))
-        kb = {0: 8 * nparts, 1: tasks[1], 10: tasks[10]}
+        # kernels the step loop launches: gradients (the step's first launch
+        # also folds the time-step partials into dt), fused fluxes + update
+        # (+ the next step's dt partials)
+        kb = {1: tasks[1], 10: tasks[10]}
         task_bytes = {10: tasks["task_10"]}
     else:
         kb = (kernel_bytes_maire if dim == 2 else kernel_bytes_maire3)(nc, ctx.nv, nc * ctx.vpc)

@@ -61,17 +61,17 @@ __device__ inline bool block_active(const DevScalars *s, int prob) {
 // not matter.  Result (per component) valid in every thread.
 constexpr int kFinalBlock = 1024;
 
-template <int S>
+template <int S, int B = kFinalBlock>
 __device__ inline void final_max(const double *a, int n, double (&r)[S]) {
-  __shared__ double red[S][kFinalBlock / 64];
+  __shared__ double red[S][B / 64];
   double acc[4][S];
   for (int k = 0; k < 4; ++k)
     for (int j = 0; j < S; ++j) acc[k][j] = 0;
   int i = threadIdx.x;
-  for (; i + 3 * kFinalBlock < n; i += 4 * kFinalBlock)
+  for (; i + 3 * B < n; i += 4 * B)
     for (int k = 0; k < 4; ++k)
-      for (int j = 0; j < S; ++j) acc[k][j] = smax(a[(size_t)(i + k * kFinalBlock) * S + j], acc[k][j]);
-  for (; i < n; i += kFinalBlock)
+      for (int j = 0; j < S; ++j) acc[k][j] = smax(a[(size_t)(i + k * B) * S + j], acc[k][j]);
+  for (; i < n; i += B)
     for (int j = 0; j < S; ++j) acc[0][j] = smax(a[(size_t)i * S + j], acc[0][j]);
   __syncthreads();
   for (int j = 0; j < S; ++j) {
@@ -82,7 +82,7 @@ __device__ inline void final_max(const double *a, int n, double (&r)[S]) {
   __syncthreads();
   for (int j = 0; j < S; ++j) {
     double b = red[j][0];
-    for (int w = 1; w < kFinalBlock / 64; ++w) b = smax(red[j][w], b);
+    for (int w = 1; w < B / 64; ++w) b = smax(red[j][w], b);
     r[j] = b;
   }
 }

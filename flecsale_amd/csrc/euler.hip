@@ -97,15 +97,15 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerField
 // Per problem: dt = min(CFL / max dt^-1, t_final - t) and the step flag.
 // driver != 0: the step loop (step budget, optional stop at t_final);
 // driver == 0: the task-level evaluate_time_step.
-__global__ void __launch_bounds__(kFinalBlock) k_eul_dt_finalize(Batch b, const double *partials, int driver) {
-  const int nprob = *b.np;
-  for (int ip = blockIdx.x; ip < nprob; ip += gridDim.x) {
-    const int p = b.plist[ip];
-    const ProbParams &P = b.pp[p];
-    if (P.cell_count == 0) continue;
-    double mx[1];
-    final_max<1>(partials + (size_t)P.tile_first * kWaves, P.tile_count * kWaves, mx);
-    if (threadIdx.x != 0) continue;
+// One problem, by a whole block of B threads.
+template <int B>
+__device__ inline void finalize_problem(const Batch &b, const double *partials, int driver, int p) {
+  const ProbParams &P = b.pp[p];
+  if (P.cell_count == 0) return;
+  double mx[1];
+  final_max<1, B>(partials + (size_t)P.tile_first * kWaves, P.tile_count * kWaves, mx);
+  if (threadIdx.x != 0) return;
+  {
     DevScalars &S = b.s[p];
     const double t = S.t;
     int active = (S.err_code == 0);
@@ -128,6 +128,11 @@ __global__ void __launch_bounds__(kFinalBlock) k_eul_dt_finalize(Batch b, const 
     }
     S.active = active;
   }
+}
+
+__global__ void __launch_bounds__(kFinalBlock) k_eul_dt_finalize(Batch b, const double *partials, int driver) {
+  const int nprob = *b.np;
+  for (int ip = blockIdx.x; ip < nprob; ip += gridDim.x) finalize_problem<kFinalBlock>(b, partials, driver, b.plist[ip]);
 }
 
 // ----------------------------------------------------------- gradients
@@ -428,11 +433,20 @@ __device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u
   }
 }
 
-template <int N, int VPC, int MODE>
+// FIN (the driver step's first gradient launch): the block of each
+// problem's first tile also finalizes the problem's time step (the work of
+// k_eul_dt_finalize(driver), one launch less per step).  The step flag is
+// then being written during the launch, so no wave waits on it: the
+// gradients of a problem that does not step are computed and never read.
+template <int N, int VPC, int MODE, bool FIN = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(N == 2 ? FSALE_GRAD_WAVES_2D : 3)))
 k_eul_grad(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!wave_live(b.s, tile.prob, 0)) return;
+    if constexpr (FIN) {
+      if (tile.first == b.pp[tile.prob].cell_first) finalize_problem<kBlock>(b, u.partials, 1, tile.prob);
+    } else if (!wave_live(b.s, tile.prob, 0)) {
+      return;
+    }
     if ((int)threadIdx.x < tile.count)
       grad_cell<N, VPC, MODE>(m, u, tile.first + threadIdx.x, b.pp[tile.prob].first_order);
   });
@@ -445,21 +459,23 @@ k_eul_grad(DevMesh m, EulerFields u, Batch b) {
 // across the block and its barriers.
 // ALL: every tile of the launch has a plan (the memory path is not
 // compiled in, which leaves the staged path more registers).
-template <int VPC, bool ALL>
+template <int VPC, bool ALL, bool FIN = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_eul_grad_lds(DevMesh m, EulerFields u, Batch b) {
   constexpr int NP = Rec<2>::S / 2;
   __shared__ double2 sh[NP * kLdsW];
   for_each_tile(b.ct, [&](const Tile &tile) {
+    // FIN: as k_eul_grad
+    if (FIN && tile.first == b.pp[tile.prob].cell_first) finalize_problem<kBlock>(b, u.partials, 1, tile.prob);
     const int first_order = b.pp[tile.prob].first_order;
     const int h = m.halo_cnt[tile.slot];
     if (!ALL && h < 0) {
-      if (!wave_live(b.s, tile.prob, 0)) return;
+      if (!FIN && !wave_live(b.s, tile.prob, 0)) return;
       if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 1>(m, u, tile.first + threadIdx.x, first_order);
       return;
     }
     stage_tile<NP>(sh, m, tile, h, [&](int i) { return u.gs[i]; });
-    if (!wave_live(b.s, tile.prob, 0)) return;
+    if (!FIN && !wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 2>(m, u, tile.first + threadIdx.x, first_order, sh);
   });
 }
@@ -909,13 +925,16 @@ int launch_grad(fsale_ctx *ctx, const EulerFields &u) {
   return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 0>, ctx->m, u, b);
 }
 
+// fin: the step's first gradient launch, which also finalizes its time step
+template <bool FIN>
 int launch_grad_pk(fsale_ctx *ctx, const EulerFields &u) {
   const Batch b = batch_args(ctx, K_EUL_GRAD);
   if (ctx->m.dim == 2 && ctx->lds_all)
-    return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, true>, ctx->m, u, b);
-  if (ctx->m.dim == 2 && ctx->lds) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, false>, ctx->m, u, b);
-  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, 1>, ctx->m, u, b);
-  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 1>, ctx->m, u, b);
+    return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, true, FIN>, ctx->m, u, b);
+  if (ctx->m.dim == 2 && ctx->lds)
+    return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, false, FIN>, ctx->m, u, b);
+  if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, 1, FIN>, ctx->m, u, b);
+  return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 1, FIN>, ctx->m, u, b);
 }
 
 // token: the launch's id for error attribution (first half step 1, second 2)
@@ -955,10 +974,9 @@ bool valid_kind(int k) { return k >= 0 && (k & 0xf) <= 2 && (k >> 4) <= 1; }
 int euler_step(fsale_ctx *ctx) {
   int rc;
   const EulerFields a = ctx->eu, b = swapped(ctx->eu);
-  if ((rc = launch_finalize(ctx, 1))) return rc;
-  if ((rc = launch_grad_pk(ctx, a))) return rc;
+  if ((rc = launch_grad_pk<true>(ctx, a))) return rc;   // + the step's time step
   if ((rc = launch_fluxupd(ctx, 0.5, false, 1, a, b))) return rc;
-  if ((rc = launch_grad_pk(ctx, b))) return rc;
+  if ((rc = launch_grad_pk<false>(ctx, b))) return rc;
   if ((rc = launch_fluxupd(ctx, 0.5, true, 2, b, a))) return rc;
   return FSALE_OK;
 }
