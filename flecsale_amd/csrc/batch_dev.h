@@ -63,19 +63,23 @@ constexpr int kFinalBlock = 1024;
 
 template <int S, int B = kFinalBlock>
 __device__ inline void final_max(const double *a, int n, double (&r)[S]) {
+  // U independent loads in flight per thread: a smaller block keeps more
+  constexpr int U = B >= kFinalBlock ? 4 : 16;
   __shared__ double red[S][B / 64];
-  double acc[4][S];
-  for (int k = 0; k < 4; ++k)
+  double acc[U][S];
+  for (int k = 0; k < U; ++k)
     for (int j = 0; j < S; ++j) acc[k][j] = 0;
   int i = threadIdx.x;
-  for (; i + 3 * B < n; i += 4 * B)
-    for (int k = 0; k < 4; ++k)
+  for (; i + (U - 1) * B < n; i += U * B)
+#pragma unroll
+    for (int k = 0; k < U; ++k)
       for (int j = 0; j < S; ++j) acc[k][j] = smax(a[(size_t)(i + k * B) * S + j], acc[k][j]);
   for (; i < n; i += B)
     for (int j = 0; j < S; ++j) acc[0][j] = smax(a[(size_t)i * S + j], acc[0][j]);
   __syncthreads();
   for (int j = 0; j < S; ++j) {
-    double v = smax(smax(acc[0][j], acc[1][j]), smax(acc[2][j], acc[3][j]));
+    double v = acc[0][j];
+    for (int k = 1; k < U; ++k) v = smax(v, acc[k][j]);
     v = wave_max(v);
     if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = v;
   }
