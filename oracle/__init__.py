@@ -58,6 +58,8 @@ def lib():
         L.om_solve3.argtypes = [_dp, _dp, _dp]
         L.om_maire_set_bc.argtypes = [ct.POINTER(ct.c_int), _dp]
         L.om_set_variant.argtypes = [ct.c_int]
+        L.oe_face_flux.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]
+        L.om_mesh_set_orders.argtypes = [_vp, _vp, _vp]
         L.om_eos_eval.argtypes = [ct.c_double, ct.c_double, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.om_maire_init.argtypes = [_vp, _dp, ct.c_double, _dp, _dp] + [_dp] * 6
         L.om_maire_nodal.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -207,6 +209,17 @@ def euler_run(mesh, s, gamma, cv, cfl, final_time, max_steps, flux_kind=0, first
     return rc, t.value, n.value
 
 
+def face_flux(kind, wl, wr, n):
+    """The oracle's flux functions on face states: wl, wr [nf, N+4] as
+    (d, v[N], p, e, a); n [nf, N] unit normals; kind 0 HLLE, 1 Rusanov,
+    2 HLLC, 3 physical flux of wl, 4 wall flux of wl.  Returns [nf, N+2]."""
+    wl = _c(np.atleast_2d(wl)); wr = _c(np.atleast_2d(wr)); n = _c(np.atleast_2d(n))
+    nf, N = n.shape
+    F = np.zeros((nf, N + 2))
+    lib().oe_face_flux(int(kind), N, nf, wl.reshape(-1), wr.reshape(-1), n.reshape(-1), F.reshape(-1))
+    return F
+
+
 def set_variant(v):
     """Plant a known mistake in the oracle (tests only); 0 restores the reference arithmetic."""
     lib().om_set_variant(int(v))
@@ -293,6 +306,12 @@ def maire_nodal(mesh, s):
     lib().om_maire_nodal(mesh.h, s.v.reshape(-1), s.p, s.d, s.a, un.reshape(-1), npc.reshape(-1),
                          Fpc.reshape(-1))
     return un, npc, Fpc
+
+
+def maire_time_step(mesh, cfl, prev_dt, a, dudt):
+    """evaluate_time_step (apps/maire_hydro/tasks.h:142-190): the minimum of
+    the acoustic, volume and growth limits; cfl = (accoustic, volume, growth)."""
+    return lib().om_maire_time_step(mesh.h, cfl[0], cfl[1], cfl[2], prev_dt, _c(a), _c(dudt).reshape(-1))
 
 
 def maire_residual(mesh, un, npc, Fpc):

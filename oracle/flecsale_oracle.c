@@ -57,7 +57,16 @@
  * convention (0 = the reference's arithmetic):
  *   1 = reconstruct pressure with grad_p and energy with grad_e (no slot swap)
  *   2 = wedge facet_area = 0.5 * edge length
- *   3 = 2D cell centroid = vertex average */
+ *   3 = 2D cell centroid = vertex average
+ *   4 = 3D wedge midpoints exchanged, 5 = 3D side centre = the side's area
+ *       centroid, 6 = 3D z walls without symmetry rows
+ *   7 = 2D cell centroid = each moment divided by 6 * area (not multiplied
+ *       by the reciprocal of 3 * twice the area)
+ *   8 = HLLC star energy without the pressure term, 9 = HLLC star momentum
+ *       shifted by (S* - u) without the normal, 10 = Rusanov dissipation
+ *       from the slower wave, 11 = Maire acoustic time step from the cell's
+ *       mean edge length instead of min_length, 12 = Maire time step without
+ *       the volume limit */
 static int om_variant = 0;
 void om_set_variant(int v) { om_variant = v; }
 
@@ -137,8 +146,14 @@ static void cell_geom_2d(const om_mesh *m, int c, double *cen, double *vol, doub
     sy += (p[1] + q[1]) * cr;
   }
   double area = 0.5 * a2;
-  cen[0] = sx / (6.0 * area);
-  cen[1] = sy / (6.0 * area);
+  /* centroid = moments times the reciprocal of 3 * (twice the area): with
+   * this form every printed value of the shock_box_2d golden is reproduced
+   * digit for digit; dividing each moment by 6 * area instead flips the last
+   * printed digit of 2 centroids (and, through them, ~2000 state values). */
+  double r = 1 / (3.0 * a2);
+  cen[0] = sx * r;
+  cen[1] = sy * r;
+  if (om_variant == 7) { cen[0] = sx / (6.0 * area); cen[1] = sy / (6.0 * area); }
   if (om_variant == 3) {
     double ax = 0, ay = 0;
     for (int i = 0; i < n; ++i) { ax += m->x[2 * m->cv[c * n + i]]; ay += m->x[2 * m->cv[c * n + i] + 1]; }
@@ -154,6 +169,15 @@ static void cell_geom_2d(const om_mesh *m, int c, double *cen, double *vol, doub
       double l = sqrt(dx * dx + dy * dy);
       ml = smin(ml, l);
     }
+  if (om_variant == 11) {   /* planted: mean edge length */
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double *p = m->x + 2 * m->cv[c * n + i];
+      const double *q = m->x + 2 * m->cv[c * n + (i + 1) % n];
+      s += sqrt((q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]));
+    }
+    ml = s / n;
+  }
   *minlen = ml;
 }
 
@@ -214,6 +238,8 @@ static void cell_geom_3d(const om_mesh *m, int c, double *cen, double *vol, doub
     for (int i = 0; i < 4; ++i) q[i] = m->x + 3 * cv[HEX_FACES[lf][i]];
     double fm[3];
     for (int d = 0; d < 3; ++d) fm[d] = (((q[0][d] + q[1][d]) + q[2][d]) + q[3][d]) * 0.25;
+    if (om_variant == 14)   /* planted: each side split about its first vertex */
+      for (int d = 0; d < 3; ++d) fm[d] = q[0][d];
     for (int i = 0; i < 4; ++i) {
       const double *p = q[i], *r = q[(i + 1) % 4];
       double u[3], v[3], w[3], h[3];
@@ -225,6 +251,8 @@ static void cell_geom_3d(const om_mesh *m, int c, double *cen, double *vol, doub
     }
   }
   for (int d = 0; d < 3; ++d) cen[d] = C[d] / (4.0 * V);
+  if (om_variant == 13)   /* planted: vertex average */
+    for (int d = 0; d < 3; ++d) cen[d] = cm[d];
   *vol = V;
   double ml = DBL_MAX;
   for (int i = 0; i < 8; ++i)
@@ -499,6 +527,7 @@ static void rusanov_flux(const ew *wl, const ew *wr, const double *n, int N, dou
   double sl = wl->a + fabs(dot(wl->v, n, N));
   double sr = wr->a + fabs(dot(wr->v, n, N));
   double s = smax(sl, sr);
+  if (om_variant == 10) s = smin(sl, sr);   /* planted: the slower wave */
   e_delta(wl, wr, N, du);
   double hs = s / 2;
   for (int i = 0; i < N + 2; ++i) f[i] = (fl[i] + fr[i]) / 2 - du[i] * hs;
@@ -526,6 +555,9 @@ static void hllc_flux(const ew *wl, const ew *wr, const double *n, int N, double
   us[0] = coef;
   for (int i = 0; i < N; ++i) us[1 + i] = coef * (w->v[i] + (sM - un) * n[i]);
   us[N + 1] = coef * (Ek / w->d + (sM - un) * (sM + w->p / (w->d * (s - un))));
+  if (om_variant == 8) us[N + 1] = coef * (Ek / w->d + (sM - un) * sM);
+  if (om_variant == 9)
+    for (int i = 0; i < N; ++i) us[1 + i] = coef * (w->v[i] + (sM - un));
   double uk[5];
   uk[0] = w->d;
   for (int i = 0; i < N; ++i) uk[1 + i] = w->d * w->v[i];
@@ -538,6 +570,32 @@ static void wall_flux(const ew *u, const double *n, int N, double *f) {
   f[0] = 0;
   for (int i = 0; i < N; ++i) f[1 + i] = u->p * n[i];
   f[N + 1] = 0;
+}
+
+/* Test hook: the face flux functions on given states.  wl, wr: [nf][N+4] as
+ * (d, v[N], p, e, a); n: [nf][N] unit normals; F: [nf][N+2].  kind 0 HLLE,
+ * 1 Rusanov, 2 HLLC, 3 physical flux of wl, 4 wall flux of wl. */
+void oe_face_flux(int kind, int N, int nf, const double *wl, const double *wr, const double *n,
+                  double *F) {
+  for (int f = 0; f < nf; ++f) {
+    ew s[2];
+    const double *src[2] = {wl + (N + 4) * f, wr + (N + 4) * f};
+    for (int k = 0; k < 2; ++k) {
+      s[k].d = src[k][0];
+      for (int i = 0; i < 3; ++i) s[k].v[i] = i < N ? src[k][1 + i] : 0.0;
+      s[k].p = src[k][N + 1];
+      s[k].e = src[k][N + 2];
+      s[k].a = src[k][N + 3];
+      s[k].T = 0.0;
+    }
+    const double *nn = n + N * f;
+    double *out = F + (N + 2) * f;
+    if (kind == 0) hlle_flux(&s[0], &s[1], nn, N, out);
+    else if (kind == 1) rusanov_flux(&s[0], &s[1], nn, N, out);
+    else if (kind == 2) hllc_flux(&s[0], &s[1], nn, N, out);
+    else if (kind == 3) e_flux(&s[0], nn, N, out);
+    else wall_flux(&s[0], nn, N, out);
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1301,6 +1359,7 @@ double om_maire_time_step(const om_mesh *m, double cfl_acc, double cfl_vol, doub
     vol_inv = smax(dti, vol_inv);
   }
   double dts[3] = {cfl_acc / acc_inv, cfl_vol / vol_inv, cfl_growth * prev_dt};
+  if (om_variant == 12) dts[1] = DBL_MAX;   /* planted: no volume limit */
   int it = 0;
   for (int i = 1; i < 3; ++i)
     if (dts[i] < dts[it]) it = i;
@@ -1387,4 +1446,12 @@ int om_maire_run(om_mesh *m, const double *g, double cv, double cfl_acc, double 
   *dt_io = ts;
   free(x0); free(uc0); free(ec0); free(un); free(npc); free(Fpc); free(dudt);
   return rc;
+}
+
+/* Test hook: replace the cell-vertex neighbour lists (same CSR offsets) and
+ * the per-cell face order, to measure what a golden says about orders the
+ * reference leaves to its (un-vendored) mesh library.  NULL keeps a table. */
+void om_mesh_set_orders(om_mesh *m, const int *nbr, const int *cell_faces) {
+  if (nbr) memcpy(m->nbr, nbr, sizeof(int) * m->nbr_off[m->nc]);
+  if (cell_faces) memcpy(m->cf, cell_faces, sizeof(int) * m->nc * m->fpc);
 }
