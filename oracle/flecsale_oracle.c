@@ -1135,7 +1135,7 @@ static void maire_nodal_3d(const om_mesh *m, const double *uc, const double *pc,
     }
     double *u = un + 3 * vt;
     int has[3] = {0, 0, 0};
-    double symn[3][3];
+    double symn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     if (m->vbnd[vt]) {
       unsigned vtags = 0;
       for (int j = 0; j < ncn; ++j) {
@@ -1249,7 +1249,7 @@ void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const 
     }
     double *u = un + 2 * vt;
     int nsym = 0;
-    double symn[2][2];
+    double symn[2][2] = {{0, 0}, {0, 0}};
     int has[2] = {0, 0};
     if (m->vbnd[vt]) {
       unsigned vtags = 0;

@@ -82,6 +82,88 @@ def test_maire_512_matches_oracle(oracle_lib):
         assert np.array_equal(x, want), "maire 512^2 coordinates not bitwise"
 
 
+# ------------------------------------------------------------ full step counts
+# Each configuration to its stated step count through the device-resident
+# driver loop (fsale_*_advance: graph-replayed chunks of steps, time-step
+# partials and their finalize carried from step to step on the device),
+# against the oracle run for the same steps.  Bound: relative L-inf per field
+# <= 1e-11 at every chunk end and <= 1e-9 at the final step; the kernels keep
+# the reference's operation order, so every chunk end is also bit-identical.
+
+FINAL_TOL = 1e-9
+
+
+def _euler_full(O, prob, steps, chunk, what):
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    s = _euler_state(O, mesh, prob)
+    flux = prob["flux"]
+    t_o = 0.0
+    with fa.EulerContext(prob["coords"], prob["cells"]) as ctx:
+        ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+        done = 0
+        while done < steps:
+            n = min(chunk, steps - done)
+            rc, t_o, k = O.euler_run(mesh, s, *prob["eos"], prob["cfl"], prob["final_time"], n,
+                                     flux_kind=flux, soln_time=t_o)
+            assert rc == 0 and k == n
+            ctx.advance(prob["cfl"], prob["final_time"], n, flux_kind=flux)
+            done += n
+            t, _, k = ctx.time()
+            assert k == done and t == t_o, (what, done)
+            _cmp_euler(ctx, s, f"{what} step {done}", exact=True)
+        got = ctx.state()
+        for key, want in [("d", s.d), ("v", s.v), ("e", s.e), ("p", s.p)]:
+            assert rel_linf(got[key], want) <= FINAL_TOL
+
+
+@pytest.mark.timeout(600)
+def test_muscl2d_1024_200_steps_match_oracle(oracle_lib):
+    """The four-quadrant problem to its 200 steps (chunks of 20)."""
+    prob = inputs.quadrants_2d(1024, steps=200, flux=fa.FLUX_HLLC | fa.BOUNDARY_TRANSMISSIVE)
+    _euler_full(oracle_lib, prob, 200, 20, "muscl2d 1024^2")
+
+
+@pytest.mark.timeout(900)
+def test_muscl3d_128_100_steps_match_oracle(oracle_lib):
+    """The jittered-hex Kelvin-Helmholtz layer to its 100 steps (chunks of 10)."""
+    prob = inputs.kelvin_helmholtz_3d(128, steps=100)
+    _euler_full(oracle_lib, prob, 100, 10, "muscl3d 128^3")
+
+
+@pytest.mark.timeout(600)
+def test_maire_512_500_steps_match_oracle(oracle_lib):
+    """The two-gas bubble to its 500 steps (chunks of 50).  The time step grows
+    by 1.01 a step from 1e-5 until the acoustic limit C_acc min_length / a
+    binds (past step ~250 here): the last steps' dt is below the growth
+    limit's, so that branch is exercised and compared too."""
+    O = oracle_lib
+    steps, chunk = 500, 50
+    prob = inputs.maire_bubble_2d(512, steps=steps)
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
+    t_o, dt_o = 0.0, prob["dt0"]
+    with fa.MaireContext(prob["coords"], prob["cells"]) as ctx:
+        ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
+        ctx.set_time(0.0, prob["dt0"])
+        done = 0
+        while done < steps:
+            rc, t_o, k, dt_o = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], chunk, dt_o,
+                                           soln_time=t_o)
+            assert rc == 0 and k == chunk
+            ctx.advance(prob["cfl"], prob["final_time"], chunk)
+            done += chunk
+            t, dt, k = ctx.time()
+            assert k == done and t == t_o and dt == dt_o, done
+            got = ctx.state()
+            for key in ("d", "v", "e", "p", "a", "V"):
+                want = np.asarray(getattr(s, key)).reshape(got[key].shape)
+                assert rel_linf(got[key], want) <= (FINAL_TOL if done == steps else STEP_TOL), (key, done)
+                assert np.array_equal(got[key], want), f"maire 512^2 {key} not bitwise at step {done}"
+        x = ctx.get(fa.F_COORDS)
+        assert np.array_equal(x, np.asarray(mesh.geometry()["coords"]).reshape(x.shape))
+    assert dt_o < 0.5 * prob["dt0"] * 1.01 ** steps
+
+
 def test_muscl2d_1024_walled_box_conserves():
     """The 1024^2 mesh with reflecting walls and the quadrant densities and
     pressures at rest (no wall separation): total mass and energy after 10
