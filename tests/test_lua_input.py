@@ -1,8 +1,9 @@
-"""Lua input files without Lua (flecsale_amd.lua_input): the reference's
-apps/hydro/{2d,3d} input files give the same problem as the apps' compiled-in
-inputs (apps/hydro/{2d,3d}/inputs.cc), and the interpreter follows Lua's
-semantics on the constructs such files use (values checked against what the
-Lua 5.3 reference manual fixes)."""
+"""Lua input files without Lua (flecsale_amd.lua_input): input decks written
+for these tests (tests/golden/lua/), in the table layout the apps' Lua reader
+expects (apps/hydro/inputs.h:99-186), give the same problems as the apps'
+compiled-in inputs (apps/hydro/{2d,3d}/inputs.cc); and the interpreter
+follows Lua's semantics on the constructs such files use (values checked
+against what the Lua 5.3 reference manual fixes)."""
 import os
 
 import numpy as np
