@@ -1247,27 +1247,55 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   for (int k = 0; k < K_COUNT; ++k) key.push_back((double)persistent_grid(ctx, k));
   key.push_back((double)ctx->pgrid);
   key.push_back((double)ctx->lds + 2.0 * ctx->lds_all);
-  if (!ctx->graph.exec || ctx->graph.key != key) {
-    if (ctx->graph.exec) {
-      (void)hipGraphExecDestroy(ctx->graph.exec);
-      ctx->graph.exec = nullptr;
+  GraphCache &gc = ctx->graph;
+  GraphEntry *g = nullptr;
+  for (auto &e : gc.entries)
+    if (e.key == key) g = &e;
+  if (!g) {
+    if (gc.entries.size() >= GraphCache::kMaxEntries) {
+      // evict the least recently used, once its last launch has completed
+      auto lru = std::min_element(gc.entries.begin(), gc.entries.end(),
+                                  [](const GraphEntry &a, const GraphEntry &b) { return a.used < b.used; });
+      FSALE_HIP(hipEventSynchronize(lru->done));
+      (void)hipGraphExecDestroy(lru->exec);
+      (void)hipEventDestroy(lru->done);
+      gc.entries.erase(lru);
     }
+    GraphEntry e;
+    e.key = key;
+    FSALE_HIP(hipEventCreateWithFlags(&e.done, hipEventDisableTiming));
     hipGraph_t graph;
-    FSALE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (err != hipSuccess) {
+      (void)hipEventDestroy(e.done);
+      set_message(std::string("hipStreamBeginCapture: ") + hipGetErrorString(err));
+      return FSALE_ERR_HIP;
+    }
     for (int n = 0; n < chunk; ++n) {
       rc = step(n == 0);
       if (rc) {
         (void)hipStreamEndCapture(ctx->stream, &graph);
+        (void)hipEventDestroy(e.done);
         return rc;
       }
     }
-    FSALE_HIP(hipStreamEndCapture(ctx->stream, &graph));
-    FSALE_HIP(hipGraphInstantiate(&ctx->graph.exec, graph, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(graph);
-    ctx->graph.key = key;
+    err = hipStreamEndCapture(ctx->stream, &graph);
+    if (err == hipSuccess) {
+      err = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+    }
+    if (err != hipSuccess) {
+      (void)hipEventDestroy(e.done);
+      set_message(std::string("graph capture: ") + hipGetErrorString(err));
+      return FSALE_ERR_HIP;
+    }
+    gc.entries.push_back(e);
+    g = &gc.entries.back();
   }
+  g->used = ++gc.clock;
   int n = 0;
-  for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(ctx->graph.exec, ctx->stream));
+  for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(g->exec, ctx->stream));
+  FSALE_HIP(hipEventRecord(g->done, ctx->stream));
   for (; n < nsteps; ++n)
     if ((rc = step(false))) return rc;
   return FSALE_OK;
@@ -1537,7 +1565,10 @@ int fsale_ctx_destroy(fsale_ctx *ctx) {
     for (hipEvent_t e : p.start) (void)hipEventDestroy(e);
     for (hipEvent_t e : p.stop) (void)hipEventDestroy(e);
   }
-  if (ctx->graph.exec) (void)hipGraphExecDestroy(ctx->graph.exec);
+  for (auto &g : ctx->graph.entries) {   // the stream is idle: no launch in flight
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipEventDestroy(g.done);
+  }
   for (void *p : ctx->allocations) (void)hipFree(p);
   if (ctx->h_s) (void)hipHostFree(ctx->h_s);
   if (ctx->stage) (void)hipHostFree(ctx->stage);

@@ -239,10 +239,24 @@ struct Batch {
   const int *np = nullptr;    // their count
 };
 
-struct GraphCache {
+// Instantiated step graphs of a context, keyed by what capture bakes in
+// (chunk length, launch grids, staging mode).  An executable graph is
+// destroyed only after an event recorded behind its last launch has
+// completed: a graph launch is asynchronous, and a key change between two
+// calls (serving rounds change the persistent grid with the live-problem
+// count) used to destroy an exec whose previous launch could still be
+// running.  A few are kept, so alternating keys do not recapture.
+struct GraphEntry {
   hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;   // recorded on the context's stream after each launch
   std::vector<double> key;
-  int steps = 0;
+  uint64_t used = 0;
+};
+
+struct GraphCache {
+  static constexpr size_t kMaxEntries = 6;
+  std::vector<GraphEntry> entries;
+  uint64_t clock = 0;
 };
 
 // Per-kernel timing with HIP events on the context's stream (profiling mode

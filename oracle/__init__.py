@@ -28,12 +28,27 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
 
 
+def load(path):
+    """Use the restatement built at `path` (e.g. liboracle_native.so, built for
+    the host it runs on) instead of liboracle.so; before any other call."""
+    global LIB
+    if _lib is not None and path != LIB:
+        raise RuntimeError("the oracle library is already loaded from " + LIB)
+    LIB = path
+
+
+def set_threads(n):
+    """OpenMP threads of the restatement's parallel loops."""
+    lib().om_set_threads(int(n))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
         L = ct.CDLL(LIB)
+        L.om_set_threads.argtypes = [ct.c_int]
         L.om_mesh_create.restype = _vp
         L.om_mesh_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _ip]
         L.om_mesh_destroy.argtypes = [_vp]

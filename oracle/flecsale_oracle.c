@@ -70,6 +70,11 @@
 static int om_variant = 0;
 void om_set_variant(int v) { om_variant = v; }
 
+/* OpenMP threads of the parallel loops (the timed CPU baseline runs on all
+ * host cores, then on one). */
+void omp_set_num_threads(int);
+void om_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+
 /* std::min / std::max semantics (first argument wins ties / NaN order). */
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static inline double smax(double a, double b) { return (a < b) ? b : a; }

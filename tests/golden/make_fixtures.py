@@ -11,9 +11,12 @@ Outputs (all plain numpy .npz, loaded with allow_pickle=False):
                       internal_energy, pressure, velocity, vertex_coords.
   exodus/<file>       the Exodus-II mesh files themselves (data/meshes), for
                       the library's own reader and the regression drivers.
-  std/<case>.std.gz   the one-rank .std goldens as text (gzip), for the
-                      numdiff comparison of the drivers' dumps.
-  lua/<file>.lua      the apps' Lua input files (apps/hydro/{2d,3d}).
+  std/<case>.std.gz   the .std goldens as text (gzip), for the numdiff
+                      comparison of the drivers' dumps (one rank, and each
+                      rank of the two-rank runs).
+
+The Lua input decks under lua/ are written for this repository, not taken
+from the reference.
 
 Mesh sources: data/meshes/sedov_32x32.g and sedov_20x20x20.g are
 netCDF-classic (64-bit offset) read with scipy.io.netcdf_file;
@@ -95,6 +98,8 @@ def main():
         "shock_box_2d_2procs_rank0": (f"{REF}/apps/hydro/2d/shock_box_2d-solution_rank000000.txt.2procs.std", 2),
         "shock_box_2d_2procs_rank1": (f"{REF}/apps/hydro/2d/shock_box_2d-solution_rank000001.txt.2procs.std", 2),
         "shock_box_3d_1procs": (f"{REF}/apps/hydro/3d/shock_box_3d-solution_rank000000.txt.1procs.std", 3),
+        "shock_box_3d_2procs_rank0": (f"{REF}/apps/hydro/3d/shock_box_3d-solution_rank000000.txt.2procs.std", 3),
+        "shock_box_3d_2procs_rank1": (f"{REF}/apps/hydro/3d/shock_box_3d-solution_rank000001.txt.2procs.std", 3),
         "sedov_2d_1procs": (f"{REF}/apps/maire_hydro/2d/sedov_2d-solution_rank000000.txt.1procs.std", 2),
     }
     for name, (path, dim) in goldens.items():
@@ -110,14 +115,12 @@ def main():
     os.makedirs(os.path.join(HERE, "std"), exist_ok=True)
     for name in ["square_32x32.g", "sedov_32x32.g", "sedov_20x20x20.g", "cube_3k_tet.g", "uniform_32.exo"]:
         shutil.copyfile(f"{REF}/data/meshes/{name}", os.path.join(HERE, "exodus", name))
-    for name in ["shock_box_2d_1procs", "shock_box_3d_1procs", "sedov_2d_1procs"]:
+    for name in ["shock_box_2d_1procs", "shock_box_3d_1procs", "sedov_2d_1procs", "shock_box_2d_2procs_rank0",
+                 "shock_box_2d_2procs_rank1", "shock_box_3d_2procs_rank0", "shock_box_3d_2procs_rank1"]:
         with open(goldens[name][0], "rb") as src, gzip.GzipFile(os.path.join(HERE, "std", name + ".std.gz"), "wb",
                                                                 mtime=0) as dst:
             shutil.copyfileobj(src, dst)
         print("copied", name)
-    os.makedirs(os.path.join(HERE, "lua"), exist_ok=True)
-    for d in ["2d", "3d"]:
-        shutil.copyfile(f"{REF}/apps/hydro/{d}/shock_box_{d}.lua", os.path.join(HERE, "lua", f"shock_box_{d}.lua"))
 
 
 if __name__ == "__main__":
