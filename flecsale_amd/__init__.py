@@ -57,7 +57,7 @@ EXPORTS = [
     "fsale_status_string", "fsale_last_message", "fsale_qr_solve", "fsale_error_cell", "fsale_set_profiling",
     "fsale_get_profile", "fsale_reset_profile",
     "fsale_eul_initial_conditions", "fsale_eul_evaluate_time_step", "fsale_eul_evaluate_gradients",
-    "fsale_eul_evaluate_fluxes", "fsale_eul_apply_update", "fsale_eul_advance",
+    "fsale_eul_evaluate_fluxes", "fsale_eul_apply_update", "fsale_eul_advance", "fsale_eul_time_step_noalloc",
     "fsale_maire_initial_conditions", "fsale_maire_evaluate_nodal_state", "fsale_maire_evaluate_residual",
     "fsale_maire_set_boundary", "fsale_batch_set_maire_boundary",
     "fsale_maire_evaluate_time_step", "fsale_maire_apply_update", "fsale_maire_save", "fsale_maire_restore",
@@ -156,6 +156,7 @@ def lib():
         L.fsale_eul_evaluate_fluxes.argtypes = [vp, ct.c_int]
         L.fsale_eul_apply_update.argtypes = [vp, Eos, dp]
         L.fsale_eul_advance.argtypes = [vp, Eos, dp, dp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]
+        L.fsale_eul_time_step_noalloc.argtypes = [i32, i32, i32] + [vp] * 6 + [dp, dp, dp, vp, vp, vp]
         L.fsale_maire_initial_conditions.argtypes = [vp, dp, vp, vp, vp, vp]
         L.fsale_maire_evaluate_nodal_state.argtypes = [vp]
         L.fsale_maire_set_boundary.argtypes = [vp, ct.c_int, ct.c_int, vp]

@@ -173,6 +173,23 @@ int fsale_eul_initial_conditions(fsale_ctx *ctx, fsale_eos eos, const double *d,
  * dt = min(CFL / max_{c,f} (a_c + |v_c.n_f|) A_f / V_c, t_final - t).  The
  * result stays on the device for apply_update; dt_out may be NULL (no sync). */
 int fsale_eul_evaluate_time_step(fsale_ctx *ctx, double cfl, double final_time, double *dt_out);
+/* No counterpart in the reference.  The same time step for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): every argument but the scalars is
+ * a device pointer the caller owns -- cell_faces [num_cells*faces_per_cell]
+ * int32 (mesh.faces(c) order), face_normal [num_faces*dim], face_area,
+ * cell_volume, velocity [num_cells*dim], sound_speed -- and the result goes
+ * to *dt_out with FSALE_OK or FSALE_ERR_DT (no finite time step) in
+ * *status_out, both on the device, asynchronously on `stream` (a
+ * hipStream_t; NULL = the null stream).  The partial maxima live in a fixed
+ * device array of the library (2 MiB per GPU, kept until the process ends):
+ * at most 2^24 cells, larger sizes return FSALE_ERR_ARG before any launch.
+ * Calls that overlap on one GPU share that array and race: serialise them. */
+int fsale_eul_time_step_noalloc(int32_t dim, int32_t num_cells, int32_t faces_per_cell,
+                                const int32_t *cell_faces, const double *face_normal, const double *face_area,
+                                const double *cell_volume, const double *velocity, const double *sound_speed,
+                                double cfl, double final_time, double soln_time, double *dt_out,
+                                int32_t *status_out, void *stream);
 /* evaluate_gradients (apps/hydro/tasks.h:145-321): LSQ gradients with the
  * vertex limiter.  first_order != 0 zeroes them (build-side switch). */
 int fsale_eul_evaluate_gradients(fsale_ctx *ctx, int first_order);
