@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(HERE, "libflecsale_amd.so")
 FLUX_HLLE, FLUX_RUSANOV, FLUX_HLLC = 0, 1, 2
 BOUNDARY_WALL, BOUNDARY_TRANSMISSIVE = 0x00, 0x10   # OR-ed into flux_kind
 SOLVER_EULER, SOLVER_MAIRE = 0, 1
+HALO_STATE, HALO_GRADIENTS = 0, 1   # fsale_halo_exchange field sets
 # Maire boundary condition kinds (fsale_maire_set_boundary)
 BC_SYMMETRY, BC_PRESSURE, BC_VELOCITY, BC_NONE = 0, 1, 2, 3
 
@@ -58,6 +59,7 @@ EXPORTS = [
     "fsale_get_profile", "fsale_reset_profile",
     "fsale_eul_initial_conditions", "fsale_eul_evaluate_time_step", "fsale_eul_evaluate_gradients",
     "fsale_eul_evaluate_fluxes", "fsale_eul_apply_update", "fsale_eul_advance", "fsale_eul_time_step_noalloc",
+    "fsale_eul_set_owned_cells", "fsale_halo_create", "fsale_halo_exchange", "fsale_halo_destroy",
     "fsale_maire_initial_conditions", "fsale_maire_evaluate_nodal_state", "fsale_maire_evaluate_residual",
     "fsale_maire_set_boundary", "fsale_batch_set_maire_boundary",
     "fsale_maire_evaluate_time_step", "fsale_maire_apply_update", "fsale_maire_save", "fsale_maire_restore",
@@ -109,7 +111,8 @@ class ServerConfig(ct.Structure):
                 ("euler_neighbours", ct.c_int64),
                 ("maire_cells", ct.c_int64), ("maire_faces", ct.c_int64), ("maire_vertices", ct.c_int64),
                 ("maire_neighbours", ct.c_int64),
-                ("chunk_steps", ct.c_int32), ("lookahead", ct.c_int32)]
+                ("chunk_steps", ct.c_int32), ("lookahead", ct.c_int32), ("eager", ct.c_int32),
+                ("reserved", ct.c_int32)]
 
 
 class Request(ct.Structure):
@@ -156,6 +159,10 @@ def lib():
         L.fsale_eul_evaluate_fluxes.argtypes = [vp, ct.c_int]
         L.fsale_eul_apply_update.argtypes = [vp, Eos, dp]
         L.fsale_eul_advance.argtypes = [vp, Eos, dp, dp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]
+        L.fsale_eul_set_owned_cells.argtypes = [vp, i32]
+        L.fsale_halo_create.argtypes = [vp, vp, i32, vp, vp, ct.POINTER(vp)]
+        L.fsale_halo_exchange.argtypes = [vp, ct.c_int]
+        L.fsale_halo_destroy.argtypes = [vp]
         L.fsale_eul_time_step_noalloc.argtypes = [i32, i32, i32] + [vp] * 6 + [dp, dp, dp, vp, vp, vp]
         L.fsale_maire_initial_conditions.argtypes = [vp, dp, vp, vp, vp, vp]
         L.fsale_maire_evaluate_nodal_state.argtypes = [vp]
@@ -604,9 +611,9 @@ class Server:
     at a time in plain contexts (mode 1)."""
 
     def __init__(self, mode=0, dim=2, verts_per_cell=4, max_problems=1024, euler=(0, 0, 0, 0), maire=(0, 0, 0, 0),
-                 chunk_steps=10, lookahead=64, frac_hbm=0.5):
+                 chunk_steps=10, lookahead=64, frac_hbm=0.5, graphs=True):
         cfg = ServerConfig(int(mode), int(dim), int(verts_per_cell), int(max_problems), *[int(x) for x in euler],
-                           *[int(x) for x in maire], int(chunk_steps), int(lookahead))
+                           *[int(x) for x in maire], int(chunk_steps), int(lookahead), int(not graphs), 0)
         h = ct.c_void_p()
         _check(lib().fsale_server_create(ct.byref(cfg), float(frac_hbm), ct.byref(h)))
         self.h = h

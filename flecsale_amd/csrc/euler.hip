@@ -56,7 +56,11 @@ template <int N, int FPC>
 __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
     double dt_inv = 0;
-    if ((int)threadIdx.x < tile.count) {
+    const int owned = b.pp[tile.prob].owned;
+    // mesh.cells(flecsi::owned) (apps/hydro/tasks.h:350): a ghost cell of a
+    // partitioned context contributes nothing
+    if ((int)threadIdx.x < tile.count &&
+        (owned == 0 || m.corig[tile.first + threadIdx.x] < owned)) {
       const int c = tile.first + threadIdx.x;
       double vel[N];
       for (int d = 0; d < N; ++d) vel[d] = u.v[d][c];
@@ -727,6 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
     const int c = tile.first + threadIdx.x;
     if (c == P.cell_first) b.s[tile.prob].t = b.s[tile.prob].t + delta_t;   // update_soln_time :554
     if ((int)threadIdx.x >= tile.count) return;
+    if (P.owned != 0 && m.corig[c] >= P.owned) return;   // mesh.cells(flecsi::owned) :614
     double du[N + 2];
     for (int i = 0; i < N + 2; ++i) du[i] = 0;
 #pragma unroll
@@ -1103,6 +1108,18 @@ int fsale_eul_evaluate_time_step(fsale_ctx *ctx, double cfl, double final_time, 
   return FSALE_OK;
 }
 
+int fsale_eul_set_owned_cells(fsale_ctx *ctx, int32_t num_owned) {
+  if (!check_ctx(ctx)) return FSALE_ERR_ARG;
+  Problem &P = ctx->probs[0];
+  if (num_owned <= 0 || num_owned > P.nc) {
+    set_message("the owned cells are the first 1..num_cells cells");
+    return FSALE_ERR_ARG;
+  }
+  P.params.owned = num_owned == P.nc ? 0 : num_owned;
+  P.partials_valid = false;
+  return upload_params(ctx, 0);
+}
+
 int fsale_eul_evaluate_gradients(fsale_ctx *ctx, int first_order) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   Problem &P = ctx->probs[0];
@@ -1154,6 +1171,10 @@ int fsale_eul_advance(fsale_ctx *ctx, fsale_eos eos, double cfl, double final_ti
   if (!check_ctx(ctx) || nsteps < 0 || !valid_kind(flux_kind)) return FSALE_ERR_ARG;
   Problem &P = ctx->probs[0];
   if (!P.have_state) return FSALE_ERR_STATE;
+  if (P.params.owned != 0) {
+    set_message("a context with ghost cells steps through the task-level entry points and halo exchanges");
+    return FSALE_ERR_STATE;
+  }
   if (nsteps == 0) return FSALE_OK;
   int rc;
   if ((rc = read_scalars(ctx))) return rc;

@@ -164,7 +164,9 @@ struct ProbParams {
   double cfl, final_time;                  // Eulerian CFL / final time
   double cfl_acc, cfl_vol, cfl_growth;     // Maire time_constants_t
   long long max_steps;                     // step budget: active while steps < max_steps
-  int kind, first_order, stop_at_final, pad;
+  int kind, first_order, stop_at_final;
+  int owned;                               // Eulerian: cells with original id >= owned are ghosts
+                                           // of another rank (0: every cell is owned)
   int tile_first, tile_count;              // the problem's cell tiles (dt partials)
   int cell_first, cell_count;
   // Maire boundary conditions of the faces of each symmetry tag (x, y, z):

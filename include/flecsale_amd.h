@@ -196,6 +196,28 @@ int fsale_eul_evaluate_gradients(fsale_ctx *ctx, int first_order);
 /* evaluate_fluxes (apps/hydro/tasks.h:385-536) with flux_function =
  * flux_kind and boundary_flux = wall_flux (apps/hydro/types.h:132-136). */
 int fsale_eul_evaluate_fluxes(fsale_ctx *ctx, int flux_kind);
+/* ------------------------------------------------------------ ranks
+ * A partitioned mesh (apps/hydro run on several ranks): each rank's cells --
+ * its owned cells first, then its ghosts, one layer of cells sharing a vertex
+ * with an owned cell -- are a context of their own.  The owned/ghost split
+ * replaces FleCSI's index spaces (mesh.cells(flecsi::owned) in
+ * evaluate_time_step and apply_update, apps/hydro/tasks.h:350,614): with
+ * num_owned < num_cells the context's time step and update skip its ghost
+ * cells (their state arrives from the owner), and the whole-driver
+ * fsale_eul_advance is refused (FSALE_ERR_STATE). */
+int fsale_eul_set_owned_cells(fsale_ctx *ctx, int32_t num_owned);
+/* Ghost refresh between two ranks' contexts on one device, FleCSI's ghost
+ * copy of a dense field (apps/hydro/driver.h:329-375): cell dst_cells[i] of
+ * `dst` receives the values of cell src_cells[i] of `src` (original cell ids
+ * of each context).  An exchange is asynchronous, ordered after the work
+ * queued on src's stream and before src's later work. */
+typedef struct fsale_halo fsale_halo;
+enum { FSALE_HALO_STATE = 0,        /* d, v, e, p, T, a (after apply_update)          */
+       FSALE_HALO_GRADIENTS = 1 };  /* the five gradients (after evaluate_gradients) */
+int fsale_halo_create(fsale_ctx *dst, fsale_ctx *src, int32_t n, const int32_t *dst_cells,
+                      const int32_t *src_cells, fsale_halo **out);
+int fsale_halo_exchange(fsale_halo *halo, int what);
+int fsale_halo_destroy(fsale_halo *halo);
 /* apply_update (apps/hydro/tasks.h:589-659): dt * factor from the device dt,
  * advances the solution time like update_soln_time (:545-576). */
 int fsale_eul_apply_update(fsale_ctx *ctx, fsale_eos eos, double factor);
@@ -374,6 +396,8 @@ typedef struct {
   int64_t maire_cells, maire_faces, maire_vertices, maire_neighbours;
   int32_t chunk_steps;       /* steps per scheduling round                               */
   int32_t lookahead;         /* queued requests that may pass a blocked one              */
+  int32_t eager;             /* 1: launch each step's kernels instead of replaying graphs */
+  int32_t reserved;          /* 0                                                        */
 } fsale_server_config;
 
 typedef struct {
