@@ -164,7 +164,7 @@ def serve_requests(inputs, n, steps):
             for i in range(n)]
 
 
-def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=1024):
+def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=1024, graphs=True):
     """Serves `requests` (solver kind, problem) through fsale_server in `mode`
     (0 batched, 1 one plain context per request).  arrivals: seconds after
     the start (Poisson trace), or None to keep up to `window` requests queued
@@ -175,7 +175,7 @@ def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=1024):
     # resident capacity per batch: ~170 average requests (47K cells each)
     cap = capacity or (8 << 20, 17 << 20, 9 << 20, 66 << 20)
     with fa.Server(mode=mode, max_problems=1024, euler=cap, maire=cap, chunk_steps=10, lookahead=64,
-                   frac_hbm=0.5) as srv:
+                   frac_hbm=0.5, graphs=graphs) as srv:
         t0 = srv.clock()
         ids, cells = [], []
         i = done = pending = 0
@@ -208,7 +208,7 @@ def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=1024):
             "mean_s": float(lat.mean()), "errors": errors, "host": host}
 
 
-def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False):
+def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs=True):
     """The mixed-size request stream: one-at-a-time service rate mu from the
     first `calib` requests; a Poisson trace at 2 mu served both ways (latency
     under the same saturated trace); and both ways with the queue kept full
@@ -227,7 +227,7 @@ def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False):
             seen.add(key)
             fa.connectivity(prob["coords"], prob["cells"])
     log(f"[serve] {len(seen)} distinct meshes registered in {time.perf_counter() - t_reg:.1f}s")
-    cal = serve_trace(fa, requests[:calib], 1)
+    cal = serve_trace(fa, requests[:calib], 1, graphs=graphs)
     mu = calib / cal["makespan_s"]
     rate = 2 * mu
     arrivals = stream_arrivals(n, rate)
@@ -236,7 +236,7 @@ def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False):
            "stream": "two-state Riemann problems, side round(exp(U[ln 32, ln 512])), alternating MUSCL "
                      "(HLLE, walls) / Maire (symmetry walls), gamma 1.4 or 1.67"}
     for name, mode in (("batched", 0), ("one_at_a_time", 1)):
-        r = serve_trace(fa, requests, mode, arrivals=arrivals)
+        r = serve_trace(fa, requests, mode, arrivals=arrivals, graphs=graphs)
         log(f"[serve] {name} trace: {r['zu_per_s']:.3e} ZU/s, p50 {r['p50_s']*1e3:.1f} ms, "
             f"p99 {r['p99_s']*1e3:.1f} ms, makespan {r['makespan_s']:.2f} s")
         out[name + "_trace"] = r
@@ -244,7 +244,7 @@ def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False):
             # requests arrive at twice its service rate: from the first
             # arrival on it is never idle, so the trace is its saturated run
             continue
-        r = serve_trace(fa, requests, mode)
+        r = serve_trace(fa, requests, mode, graphs=graphs)
         log(f"[serve] {name} saturated: {r['zu_per_s']:.3e} ZU/s, makespan {r['makespan_s']:.2f} s"
             + (f", host {json.dumps({k: round(v, 3) for k, v in r['host'].items()})}" if r["host"] else ""))
         out[name + "_saturated"] = r
@@ -267,6 +267,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--size", type=int, default=None, help="override mesh size (cells per side)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="served stream: launch each step's kernels instead of replaying captured graphs")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--shape", type=int, default=None,
                     help="blocks per CU for every kernel (0 = one block per tile); default: autotune")
@@ -432,7 +434,7 @@ def main():
     if not args.no_serve:
         n_req = args.requests or 4096
         log(f"[bench] served stream sample: {n_req} requests")
-        serving = serve_stream(fa, inputs, n_req)
+        serving = serve_stream(fa, inputs, n_req, graphs=not args.eager)
     out = {
         "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
         "value": value,
@@ -488,7 +490,7 @@ def stream_cpu_baseline(inputs, sides, steps, budget_s=12.0):
 def main_serve(args, fa, inputs):
     n = args.requests or 4096
     steps = args.steps or 100
-    out = serve_stream(fa, inputs, n, steps, one_saturated=True)
+    out = serve_stream(fa, inputs, n, steps, one_saturated=True, graphs=not args.eager)
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU restatement on the first requests of the stream")

@@ -164,13 +164,13 @@ int serve_alone(fsale_server *S, int64_t id) {
     rc = fsale_eul_initial_conditions(ctx, R.eos, R.d.data(), R.v.data(), R.p.data());
     if (!rc)
       rc = fsale_eul_advance(ctx, R.eos, R.ep.cfl, R.ep.final_time, (int)R.ep.max_steps, R.ep.flux_kind,
-                             R.ep.first_order, R.ep.stop_at_final, 1);
+                             R.ep.first_order, R.ep.stop_at_final, S->cfg.eager ? 0 : 1);
   } else {
     rc = fsale_maire_initial_conditions(ctx, R.cv, R.gamma.data(), R.d.data(), R.v.data(), R.p.data());
     if (!rc) rc = fsale_set_time(ctx, 0.0, R.mp.initial_time_step);
     if (!rc)
       rc = fsale_maire_advance(ctx, R.mp.cfl_accoustic, R.mp.cfl_volume, R.mp.cfl_growth, R.mp.final_time,
-                               (int)R.mp.max_steps, 1);
+                               (int)R.mp.max_steps, S->cfg.eager ? 0 : 1);
   }
   int err = rc ? rc : fsale_sync(ctx);
   if (err == FSALE_ERR_NEG_STATE || err == FSALE_ERR_DT) {
@@ -386,7 +386,7 @@ int fsale_server_poll(fsale_server *S, int32_t *completed, int32_t *pending) {
       if (S->running[s].size() <= fin[s].size()) continue;
       hipStream_t stream = S->batch[s]->stream;
       FSALE_HIP(hipEventRecord(S->ev_start[s], stream));
-      if ((rc = fsale_batch_step(S->batch[s], S->cfg.chunk_steps, 1))) return rc;
+      if ((rc = fsale_batch_step(S->batch[s], S->cfg.chunk_steps, S->cfg.eager ? 0 : 1))) return rc;
       FSALE_HIP(hipEventRecord(S->ev_end[s], stream));
       S->timed[s] = true;
       launched = true;
