@@ -208,7 +208,7 @@ def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=1024, g
             "mean_s": float(lat.mean()), "errors": errors, "host": host}
 
 
-def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs=True):
+def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs=True, modes=("batched", "one_at_a_time")):
     """The mixed-size request stream: one-at-a-time service rate mu from the
     first `calib` requests; a Poisson trace at 2 mu served both ways (latency
     under the same saturated trace); and both ways with the queue kept full
@@ -236,6 +236,8 @@ def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs
            "stream": "two-state Riemann problems, side round(exp(U[ln 32, ln 512])), alternating MUSCL "
                      "(HLLE, walls) / Maire (symmetry walls), gamma 1.4 or 1.67"}
     for name, mode in (("batched", 0), ("one_at_a_time", 1)):
+        if name not in modes:
+            continue
         r = serve_trace(fa, requests, mode, arrivals=arrivals, graphs=graphs)
         log(f"[serve] {name} trace: {r['zu_per_s']:.3e} ZU/s, p50 {r['p50_s']*1e3:.1f} ms, "
             f"p99 {r['p99_s']*1e3:.1f} ms, makespan {r['makespan_s']:.2f} s")
@@ -248,6 +250,8 @@ def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs
         log(f"[serve] {name} saturated: {r['zu_per_s']:.3e} ZU/s, makespan {r['makespan_s']:.2f} s"
             + (f", host {json.dumps({k: round(v, 3) for k, v in r['host'].items()})}" if r["host"] else ""))
         out[name + "_saturated"] = r
+    if "one_at_a_time" not in modes:
+        return out
     one = out.get("one_at_a_time_saturated", out["one_at_a_time_trace"])
     out["throughput_ratio"] = out["batched_saturated"]["zu_per_s"] / one["zu_per_s"]
     out["throughput_ratio_basis"] = ("batched saturated / one at a time saturated" if one_saturated else
@@ -267,6 +271,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--size", type=int, default=None, help="override mesh size (cells per side)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="served stream: skip the one-at-a-time comparison (kernel traces of the batched path)")
     ap.add_argument("--eager", action="store_true",
                     help="served stream: launch each step's kernels instead of replaying captured graphs")
     ap.add_argument("--no-profile", action="store_true")
@@ -490,7 +496,8 @@ def stream_cpu_baseline(inputs, sides, steps, budget_s=12.0):
 def main_serve(args, fa, inputs):
     n = args.requests or 4096
     steps = args.steps or 100
-    out = serve_stream(fa, inputs, n, steps, one_saturated=True, graphs=not args.eager)
+    out = serve_stream(fa, inputs, n, steps, one_saturated=True, graphs=not args.eager,
+                       modes=("batched",) if args.batched_only else ("batched", "one_at_a_time"))
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU restatement on the first requests of the stream")
