@@ -440,14 +440,19 @@ __device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u
 // FIN (the driver step's first gradient launch): the block of each
 // problem's first tile also finalizes the problem's time step (the work of
 // k_eul_dt_finalize(driver), one launch less per step).  The step flag is
-// then being written during the launch, so no wave waits on it: the
-// gradients of a problem that does not step are computed and never read.
+// then being written during the launch, so no wave waits on it; the tiles of
+// a problem that was not packed for this batch of steps (S.packed, set by
+// k_eul_pack before the batch and not written during it) are skipped, as
+// every other kernel skips them: a finished problem's ranges may already
+// hold a new admission.  The gradients of a packed problem that stops
+// within the batch are computed over its own ranges and never read.
 template <int N, int VPC, int MODE, bool FIN = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(N == 2 ? FSALE_GRAD_WAVES_2D : 3)))
 k_eul_grad(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
     if constexpr (FIN) {
       if (tile.first == b.pp[tile.prob].cell_first) finalize_problem<kBlock>(b, u.partials, 1, tile.prob);
+      if (!b.s[tile.prob].packed) return;
     } else if (!wave_live(b.s, tile.prob, 0)) {
       return;
     }
@@ -471,6 +476,7 @@ k_eul_grad_lds(DevMesh m, EulerFields u, Batch b) {
   for_each_tile(b.ct, [&](const Tile &tile) {
     // FIN: as k_eul_grad
     if (FIN && tile.first == b.pp[tile.prob].cell_first) finalize_problem<kBlock>(b, u.partials, 1, tile.prob);
+    if (FIN && !b.s[tile.prob].packed) return;   // uniform across the block
     const int first_order = b.pp[tile.prob].first_order;
     const int h = m.halo_cnt[tile.slot];
     if (!ALL && h < 0) {

@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+OM_ERR_CORNERS = 5   # flecsale_oracle.c: a vertex with more than 16 corners
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -77,6 +78,7 @@ def lib():
         L.om_mesh_set_orders.argtypes = [_vp, _vp, _vp]
         L.om_eos_eval.argtypes = [ct.c_double, ct.c_double, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.om_maire_init.argtypes = [_vp, _dp, ct.c_double, _dp, _dp] + [_dp] * 6
+        L.om_maire_nodal.restype = ct.c_int
         L.om_maire_nodal.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.om_maire_residual.argtypes = [_vp, _dp, _dp, _dp, _dp]
         L.om_maire_time_step.restype = ct.c_double
@@ -310,6 +312,8 @@ def maire_run(mesh, s, cv, cfl, final_time, max_steps, dt0, soln_time=0.0):
     n = ct.c_int(0)
     rc = lib().om_maire_run(mesh.h, s.g, cv, cfl[0], cfl[1], cfl[2], final_time, max_steps, ct.byref(dt),
                             s.V, s.M, s.v.reshape(-1), s.p, s.d, s.e, s.T, s.a, ct.byref(t), ct.byref(n))
+    if rc == OM_ERR_CORNERS:
+        raise ValueError("oracle maire_run: a vertex has more than 16 corners")
     return rc, t.value, n.value, dt.value
 
 
@@ -318,8 +322,10 @@ def maire_nodal(mesh, s):
     velocities [nv, dim] and corner normals / forces [nc*vpc, dim]."""
     nv, ncn, N = mesh.nv, mesh.nc * mesh.vpc, mesh.dim
     un = np.zeros((nv, N)); npc = np.zeros((ncn, N)); Fpc = np.zeros((ncn, N))
-    lib().om_maire_nodal(mesh.h, s.v.reshape(-1), s.p, s.d, s.a, un.reshape(-1), npc.reshape(-1),
-                         Fpc.reshape(-1))
+    rc = lib().om_maire_nodal(mesh.h, s.v.reshape(-1), s.p, s.d, s.a, un.reshape(-1), npc.reshape(-1),
+                              Fpc.reshape(-1))
+    if rc == OM_ERR_CORNERS:
+        raise ValueError("oracle maire_nodal: a vertex has more than 16 corners")
     return un, npc, Fpc
 
 

@@ -52,6 +52,8 @@
 #define OM_ERR_NEG_STATE 2      /* negative density or internal energy */
 #define OM_ERR_DT 3             /* non-positive inverse time step */
 #define OM_ERR_ALLOC 4
+#define OM_ERR_CORNERS 5         /* a vertex with more corners than the nodal solve holds */
+#define OM_MAX_CORNERS 16
 
 /* Planted-mistake switch for the tests that show the goldens discriminate a
  * convention (0 = the reference's arithmetic):
@@ -1110,8 +1112,7 @@ static void maire_nodal_3d(const om_mesh *m, const double *uc, const double *pc,
   for (int vt = 0; vt < m->nv; ++vt) {
     double Mp[3][3] = {{0}}, rhs[3] = {0, 0, 0};
     int ncn = m->vc_off[vt + 1] - m->vc_off[vt];
-    double Mpc[16][3][3];
-    if (ncn > 16) ncn = 16;
+    double Mpc[OM_MAX_CORNERS][3][3];   /* ncn <= OM_MAX_CORNERS, checked by om_maire_nodal */
     for (int j = 0; j < ncn; ++j) {
       int cn = m->vc[m->vc_off[vt] + j];
       int cl = cn / 8, k = cn % 8;
@@ -1212,19 +1213,23 @@ static void maire_nodal_3d(const om_mesh *m, const double *uc, const double *pc,
 
 /* evaluate_nodal_state tasks.h:221-453 (2D, symmetry boundary conditions of
  * inputs_regression.cc:116-162: both tags symmetric).  un[nv*dim]. */
-void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const double *dc,
-                    const double *ac, double *un, double *npc, double *Fpc) {
+/* Returns OM_ERR_CORNERS, computing nothing, when a vertex has more than
+ * OM_MAX_CORNERS corners (the per-vertex corner matrices are held in a fixed
+ * array). */
+int om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const double *dc,
+                   const double *ac, double *un, double *npc, double *Fpc) {
+  for (int vt = 0; vt < m->nv; ++vt)
+    if (m->vc_off[vt + 1] - m->vc_off[vt] > OM_MAX_CORNERS) return OM_ERR_CORNERS;
   if (m->dim == 3) {
     maire_nodal_3d(m, uc, pc, dc, ac, un, npc, Fpc);
-    return;
+    return OM_OK;
   }
   const int nvpc = m->vpc;
   #pragma omp parallel for schedule(static)
   for (int vt = 0; vt < m->nv; ++vt) {
     double Mp[2][2] = {{0, 0}, {0, 0}}, rhs[2] = {0, 0};
     int ncn = m->vc_off[vt + 1] - m->vc_off[vt];
-    double Mpc[16][2][2];
-    if (ncn > 16) ncn = 16;
+    double Mpc[OM_MAX_CORNERS][2][2];   /* ncn <= OM_MAX_CORNERS, checked above */
     for (int j = 0; j < ncn; ++j) {
       int cn = m->vc[m->vc_off[vt] + j];
       int cl = cn / nvpc, k = cn % nvpc;
@@ -1328,6 +1333,7 @@ void om_maire_nodal(const om_mesh *m, const double *uc, const double *pc, const 
       }
     }
   }
+  return OM_OK;
 }
 
 /* evaluate_residual tasks.h:461-489 with compute_update lagrange_eqns.h:374-389.
@@ -1429,14 +1435,14 @@ int om_maire_run(om_mesh *m, const double *g, double cv, double cfl_acc, double 
     memcpy(x0, m->x, sizeof(double) * nv * N);
     memcpy(uc0, uc, sizeof(double) * nc * N);
     memcpy(ec0, e, sizeof(double) * nc);
-    om_maire_nodal(m, uc, p, d, a, un, npc, Fpc);
+    if ((rc = om_maire_nodal(m, uc, p, d, a, un, npc, Fpc))) break;
     om_maire_residual(m, un, npc, Fpc, dudt);
     ts = om_maire_time_step(m, cfl_acc, cfl_vol, cfl_growth, ts, a, dudt);
     ts = smin(ts, final_time - t);
     rc = om_maire_apply_update(m, 0.5 * ts, x0, un, dudt, V, M, uc, d, e, &bad);
     if (rc) break;
     om_maire_from_energy(nc, g, cv, d, e, p, T, a);
-    om_maire_nodal(m, uc, p, d, a, un, npc, Fpc);
+    if ((rc = om_maire_nodal(m, uc, p, d, a, un, npc, Fpc))) break;
     om_maire_residual(m, un, npc, Fpc, dudt);
     memcpy(m->x, x0, sizeof(double) * nv * N);
     memcpy(uc, uc0, sizeof(double) * nc * N);

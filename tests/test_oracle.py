@@ -406,3 +406,33 @@ def test_velocity_condition_moves_the_wall_exactly(oracle_lib, dim):
     # the left wall pushes the gas at rest to +x, the receding right wall
     # lets it follow: every cell's x velocity lies between 0 and the walls'
     assert np.all(s.v[:, 0] > 0) and np.all(s.v[:, 0] < 0.3)
+
+
+def _quad_fan(n):
+    """n quads (centre, p_i, q_i, p_i+1), counter-clockwise, around the
+    vertex at the origin: that vertex has n corners."""
+    th = 2 * np.pi * np.arange(n) / n
+    tm = th + np.pi / n
+    p = np.stack([np.cos(th), np.sin(th)], 1)
+    q = 2 * np.stack([np.cos(tm), np.sin(tm)], 1)
+    coords = np.vstack([[0.0, 0.0], p, q])
+    cells = [[0, 1 + i, 1 + n + i, 1 + (i + 1) % n] for i in range(n)]
+    return coords, np.array(cells, np.int32)
+
+
+@pytest.mark.parametrize("n", [16, 17])
+def test_maire_nodal_refuses_more_corners_than_it_holds(oracle_lib, n):
+    """The oracle's nodal solve holds at most 16 corners per vertex: a vertex
+    with more makes it fail loudly instead of dropping corners."""
+    O = oracle_lib
+    coords, cells = _quad_fan(n)
+    mesh = O.Mesh(coords, cells)
+    s = O.maire_initial_state(mesh, np.full(n, 1.4), 1.0, np.ones(n), np.zeros((n, 2)), np.ones(n))
+    if n <= 16:
+        un, npc, Fpc = O.maire_nodal(mesh, s)
+        assert np.max(np.abs(un[0])) <= 1e-14   # the centre of a uniform gas at rest
+    else:
+        with pytest.raises(ValueError, match="more than 16 corners"):
+            O.maire_nodal(mesh, s)
+        with pytest.raises(ValueError, match="more than 16 corners"):
+            O.maire_run(mesh, s, 1.0, (0.25, 0.1, 1.01), 1.0, 2, 1e-3)
