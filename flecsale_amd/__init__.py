@@ -70,7 +70,7 @@ EXPORTS = [
     "fsale_launch_shape", "fsale_get_launch_shape", "fsale_lds_staging", "fsale_autotune",
     "fsale_server_create", "fsale_server_destroy", "fsale_server_clock", "fsale_server_submit",
     "fsale_server_poll", "fsale_server_times", "fsale_server_result", "fsale_server_stats",
-    "fsale_exodus_info", "fsale_exodus_read", "fsale_dump",
+    "fsale_exodus_info", "fsale_exodus_read", "fsale_dump", "fsale_output", "fsale_exodus_write",
 ]
 
 
@@ -139,6 +139,9 @@ def lib():
         L.fsale_exodus_info.argtypes = [ct.c_char_p, vp, vp, vp, vp]
         L.fsale_exodus_read.argtypes = [ct.c_char_p, vp, vp]
         L.fsale_dump.argtypes = [vp, ct.c_char_p]
+        L.fsale_output.argtypes = [vp, ct.c_char_p, ct.c_char_p, ct.c_int64, ct.c_int32]
+        L.fsale_exodus_write.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int32, ct.c_int32, vp, ct.c_int32,
+                                         ct.c_int32, vp, ct.c_double, ct.c_int32, vp, vp]
         L.fsale_ctx_create.argtypes = [ct.POINTER(MeshDesc), ct.c_int, ct.POINTER(vp)]
         L.fsale_ctx_destroy.argtypes = [vp]
         L.fsale_sync.argtypes = [vp]
@@ -250,6 +253,22 @@ def connectivity(coords, cells):
     return dict(face_cells=fc, cell_faces=cf, nbr_off=off, nbr=nbr[:nn])
 
 
+def write_exodus(path, coords, cells, time=0.0, variables=None, title=""):
+    """One time step of an Exodus-II file (fsale_exodus_write): the mesh
+    (coords [nv, dim], cells [nc, vpc] 0-based) and the cell variables of
+    `variables` ({name: values [nc]}, in order)."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    items = list((variables or {}).items())
+    vals = [np.ascontiguousarray(np.broadcast_to(v, (len(cells),)), dtype=np.float64) for _, v in items]
+    names = (ct.c_char_p * max(1, len(items)))(*[k.encode() for k, _ in items])
+    ptrs = (ct.c_void_p * max(1, len(items)))(*[v.ctypes.data for v in vals])
+    _check(lib().fsale_exodus_write(os.fsencode(path), title.encode(), coords.shape[1], len(coords),
+                                    coords.ctypes.data, len(cells), cells.shape[1], cells.ctypes.data,
+                                    float(time), len(items), ct.cast(names, ct.c_void_p),
+                                    ct.cast(ptrs, ct.c_void_p)))
+
+
 def read_exodus(path):
     """Exodus-II mesh file (fsale_exodus_read): (coords [nv, dim] float64,
     cells [nc, vpc] int32, 0-based, file order)."""
@@ -296,6 +315,12 @@ class _Context:
     def dump(self, path):
         """The dump task's regression text file (fsale_dump)."""
         _check(lib().fsale_dump(self.h, os.fsencode(path)))
+
+    def output(self, prefix, iteration, postfix="exo", rank=0):
+        """The output task's Exodus-II file (fsale_output); returns its path."""
+        _check(lib().fsale_output(self.h, os.fsencode(prefix), os.fsencode(postfix), iteration, rank))
+        sep = "." if self.solver == SOLVER_EULER else "_"
+        return f"{prefix}_rank{rank:06d}{sep}{iteration:06d}.{postfix}"
 
     _shapes = None
 

@@ -34,6 +34,52 @@ APPS = {
     "maire_hydro_3d": ("sedov_3d", fa.SOLVER_MAIRE, inputs.sedov_3d),
 }
 
+# inputs_t::output_freq of the compiled-in problems (apps/hydro/2d/inputs.cc:31,
+# apps/hydro/3d/inputs.cc:42, apps/maire_hydro/2d/inputs_regression.cc:69,
+# apps/maire_hydro/3d/inputs.cc:52)
+OUTPUT_FREQ = {"hydro_2d": 1000000, "hydro_3d": 1000000, "maire_hydro_2d": 20, "maire_hydro_3d": 10}
+
+EPSILON = np.finfo(np.float64).eps
+
+
+def _advance(ctx, solver, prob, nsteps):
+    if solver == fa.SOLVER_EULER:
+        ctx.advance(prob["cfl"], prob["final_time"], nsteps, flux_kind=prob["flux"])
+    else:
+        ctx.advance(prob["cfl"], prob["final_time"], nsteps)
+
+
+def _run_with_output(ctx, solver, prob, freq, prefix):
+    """The drivers' step loop with the output task's schedule; returns the
+    Exodus files written.  Both write the initial state (iteration 0), then
+    after every step whose count is a multiple of output_freq or that is the
+    last of max_steps (apps/hydro/driver.h:390-405,
+    apps/maire_hydro/driver.h:517-534); hydro once more after the loop when
+    the last count was not a multiple (driver.h:424-437), Maire also after the
+    step that reaches the final time (driver.h:521), where its loop stops."""
+    max_steps = prob["max_steps"]
+    if freq <= 0:
+        _advance(ctx, solver, prob, max_steps)
+        return []
+    files = [ctx.output(prefix, 0)]
+    done = 0
+    while done < max_steps:
+        nxt = min((done // freq + 1) * freq, max_steps)
+        _advance(ctx, solver, prob, nxt - done)
+        t, _, n = ctx.time()
+        if n < nxt:   # Maire: the loop stopped at the final time
+            if abs(t - prob["final_time"]) < EPSILON:
+                files.append(ctx.output(prefix, n))
+            done = n
+            break
+        done = n
+        files.append(ctx.output(prefix, done))
+    # hydro's post-loop output: the file of the last in-loop output again
+    # (the loop always ends at max_steps, which has written it)
+    if solver == fa.SOLVER_EULER and done % freq != 0:
+        ctx.output(prefix, done)
+    return files
+
 
 def _lua_problem(path, coords, cells, centroids):
     """The Eulerian problem of a Lua input file (inputs_t::load_lua and
@@ -48,13 +94,16 @@ def _lua_problem(path, coords, cells, centroids):
         d[c], vel, p[c] = h["ics"](x, 0.0)
         v[c, :len(vel)] = vel[:dim]
     return h["prefix"], dict(coords=coords, cells=cells, eos=h["eos"], d=d, v=v, p=p, cfl=h["CFL"],
-                             final_time=h["final_time"], max_steps=h["max_steps"], flux=0)
+                             final_time=h["final_time"], max_steps=h["max_steps"], flux=0), h["output_freq"]
 
 
-def run(app, mesh_path, out_dir=".", input_file=None):
+def run(app, mesh_path, out_dir=".", input_file=None, output=False):
     """Runs `app` on the mesh file (with the problem of a Lua input file when
-    given, as the apps' -f option); returns the dump's path."""
+    given, as the apps' -f option); returns the dump's path, and with
+    output=True also the Exodus files of the output task (the reference
+    writes them whenever output_freq > 0)."""
     prefix, solver, problem = APPS[app]
+    freq = OUTPUT_FREQ[app]
     if input_file and solver != fa.SOLVER_EULER:
         raise ValueError("Lua input files configure the Eulerian apps (hydro_2d, hydro_3d)")
     coords, cells = fa.read_exodus(mesh_path)
@@ -64,7 +113,7 @@ def run(app, mesh_path, out_dir=".", input_file=None):
         # evaluates them at burton's (apps/hydro/tasks.h:46-68)
         centroids = ctx.get(fa.F_CELL_CENTROID)
         if input_file:
-            prefix, prob = _lua_problem(input_file, coords, cells, centroids)
+            prefix, prob, freq = _lua_problem(input_file, coords, cells, centroids)
         elif app == "maire_hydro_3d":
             # sedov_3d sizes its energy deposit by each cell's volume
             prob = problem(coords, cells, centroids, ctx.get(fa.F_CELL_VOLUME))
@@ -72,14 +121,13 @@ def run(app, mesh_path, out_dir=".", input_file=None):
             prob = problem(coords, cells, centroids)
         if solver == fa.SOLVER_EULER:
             ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
-            ctx.advance(prob["cfl"], prob["final_time"], prob["max_steps"], flux_kind=prob["flux"])
         else:
             ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
             ctx.set_time(0.0, prob["dt0"])
-            ctx.advance(prob["cfl"], prob["final_time"], prob["max_steps"])
+        files = _run_with_output(ctx, solver, prob, freq if output else 0, os.path.join(out_dir, prefix))
         path = os.path.join(out_dir, f"{prefix}-solution_rank{0:06d}.txt")
         ctx.dump(path)
-    return path
+    return (path, files) if output else path
 
 
 def main(argv=None):
@@ -88,8 +136,11 @@ def main(argv=None):
     ap.add_argument("-m", "--mesh", required=True, help="Exodus-II mesh file")
     ap.add_argument("-f", "--input", default=None, help="Lua input file (hydro apps)")
     ap.add_argument("-o", "--out-dir", default=".")
+    ap.add_argument("--no-output", action="store_true", help="skip the output task's Exodus files")
     a = ap.parse_args(argv)
-    path = run(a.app, a.mesh, a.out_dir, a.input)
+    path, files = run(a.app, a.mesh, a.out_dir, a.input, output=not a.no_output)
+    for f in files:
+        print(f"Output written to: {f}")
     print(f"Dumping solution to: {path}")
     return 0
 

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libflecsale_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FSALE_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "geom.hip", "euler.hip", "maire.hip", "noalloc.hip", "halo.hip", "mesh_host.cpp", "serve.cpp", "exodus.cpp"]
+SOURCES = ["api.hip", "geom.hip", "euler.hip", "maire.hip", "noalloc.hip", "halo.hip", "mesh_host.cpp", "serve.cpp", "exodus.cpp", "exodus_write.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -1628,6 +1628,31 @@ int fsale_dump(fsale_ctx *ctx, const char *path) {
   return FSALE_OK;
 }
 
+int fsale_output(fsale_ctx *ctx, const char *prefix, const char *postfix, int64_t iteration, int32_t rank) {
+  if (!ctx || !ctx->single || ctx->probs.empty() || !ctx->probs[0].live || !prefix || !postfix || iteration < 0 ||
+      rank < 0)
+    return FSALE_ERR_ARG;
+  const Problem &P = ctx->probs[0];
+  if (!P.have_state) return FSALE_ERR_STATE;
+  const int N = ctx->m.dim, nc = P.nc, nv = P.nv;
+  const bool euler = ctx->solver == FSALE_SOLVER_EULER;
+  std::vector<double> x((size_t)nv * N), d(nc), p(nc);
+  const int32_t ids[3] = {FSALE_F_COORDS, FSALE_F_DENSITY, FSALE_F_PRESSURE};
+  double *outs[3] = {x.data(), d.data(), p.data()};
+  int rc = get_problem_fields(ctx, 0, euler ? 3 : 2, ids, outs);
+  if (rc) return rc;
+  if ((rc = read_scalars(ctx))) return rc;
+  // apps/hydro/tasks.h:723-725: prefix_rankRRRRRR.IIIIII.postfix;
+  // apps/maire_hydro/tasks.h:681-683: prefix_rankRRRRRR_IIIIII.postfix
+  char tail[64];
+  std::snprintf(tail, sizeof(tail), "_rank%06d%c%06lld.", (int)rank, euler ? '.' : '_', (long long)iteration);
+  const std::string path = std::string(prefix) + tail + postfix;
+  const char *names[2] = {"density", "pressure"};
+  const double *vals[2] = {d.data(), p.data()};
+  return exodus_write(path.c_str(), prefix, N, nv, x.data(), nc, ctx->m.vpc, P.topo->cv.data(), ctx->h_s[0].t,
+                      euler ? 2 : 1, names, vals);
+}
+
 int fsale_sync(fsale_ctx *ctx) {
   if (!ctx) return FSALE_ERR_ARG;
   int rc = read_scalars(ctx);

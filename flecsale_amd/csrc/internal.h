@@ -84,6 +84,10 @@ struct HostTopology {
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
+// Exodus-II output of one time step (exodus_write.cpp); cells 0-based [nc][vpc]
+int exodus_write(const char *path, const char *title, int dim, int nv, const double *coords, int nc, int vpc,
+                 const int32_t *cells, double time, int nvars, const char *const *names,
+                 const double *const *values);
 // connectivity of a mesh from the process-wide cache (built on a miss)
 std::shared_ptr<const HostTopology> cached_topology(const fsale_mesh_desc &mesh, int &rc, std::string &err);
 

@@ -150,6 +150,25 @@ int fsale_exodus_read(const char *path, double *coords, int32_t *cells);
  * and coordinates, 15 significant digits (scientific, precision 14), as the
  * reference's .std goldens. */
 int fsale_dump(fsale_ctx *ctx, const char *path);
+/* The output task (apps/hydro/tasks.h:703-734; apps/maire_hydro/tasks.h:
+ * 661-692) of a plain context: an Exodus-II file of the mesh as it is now
+ * (moved, for a Lagrangian context), its solution time and the task's cell
+ * variables, Eulerian "density" and "pressure", Lagrangian "density".  The
+ * file is named as the task names it, Eulerian
+ * <prefix>_rank<rank:06>.<iteration:06>.<postfix>, Lagrangian
+ * <prefix>_rank<rank:06>_<iteration:06>.<postfix> (the drivers pass postfix
+ * "exo", apps/hydro/driver.h:285); prefix may hold a directory.  netCDF
+ * classic with 64-bit offsets, read back by fsale_exodus_read. */
+int fsale_output(fsale_ctx *ctx, const char *prefix, const char *postfix, int64_t iteration, int32_t rank);
+/* The writer under fsale_output, for host arrays: one element block of
+ * quad4 (dim 2, verts_per_cell 4) or hex8 (dim 3, 8) cells, 0-based
+ * vertex ids [num_cells * verts_per_cell], coords [num_vertices * dim], one
+ * time step holding num_vars cell variables values[i][num_cells] named
+ * names[i] (at most 32 characters).  No counterpart in the reference beyond
+ * the output task's call into flecsi-sp's io_exodus::write (not vendored). */
+int fsale_exodus_write(const char *path, const char *title, int32_t dim, int32_t num_vertices,
+                       const double *coords, int32_t num_cells, int32_t verts_per_cell, const int32_t *cells,
+                       double time, int32_t num_vars, const char *const *names, const double *const *values);
 /* Per-kernel timing (the reference's only timer is the wall-clock print
  * around the step loop, apps/hydro/driver.h:311,419).  While enabled, the
  * step loops run without graphs and each launch is bracketed by HIP events on

@@ -112,3 +112,57 @@ def test_reader_survives_damaged_files(tmp_path):
     bad.write_bytes(big)
     with pytest.raises(fa.FsaleError):
         fa.read_exodus(str(bad))
+
+
+# ------------------------------------------------------------ Exodus output
+
+def _netcdf(path):
+    import scipy.io
+    return scipy.io.netcdf_file(path, "r", mmap=False)
+
+
+@pytest.mark.parametrize("name", ["sedov_32x32", "sedov_20x20x20"])
+def test_writer_round_trip(name, tmp_path):
+    """fsale_exodus_write (the output task's file): the reference's mesh and
+    two cell variables written out read back bit for bit through this
+    library's reader and through scipy's independent netCDF reader, with the
+    Exodus-II names the reference's own mesh files use."""
+    ref = np.load(os.path.join(HERE, "golden", "meshes", name + ".npz"))
+    coords, cells = ref["coords"], ref["cells"]
+    rng = np.random.default_rng(3)
+    d, p = rng.random(len(cells)), rng.random(len(cells)) * 1e-300
+    out = str(tmp_path / "out.exo")
+    fa.write_exodus(out, coords, cells, 0.1489, {"density": d, "pressure": p}, title=name)
+    c2, e2 = fa.read_exodus(out)
+    assert np.array_equal(c2, coords) and np.array_equal(e2, cells)
+    f = _netcdf(out)
+    orig = _netcdf(os.path.join(EXO, name + ".g"))
+    dim = coords.shape[1]
+    assert f.version_byte == 2
+    for k in ("num_dim", "num_nodes", "num_elem", "num_el_blk", "num_el_in_blk1", "num_nod_per_el1"):
+        assert f.dimensions[k] == orig.dimensions[k], k
+    for k in ["coordx", "coordy", "coordz"][:dim] + ["connect1", "eb_status", "eb_prop1"]:
+        assert np.array_equal(f.variables[k][:], orig.variables[k][:]), k
+    assert f.variables["connect1"].elem_type == (b"QUAD4" if dim == 2 else b"HEX8")
+    assert f.variables["eb_prop1"].name == b"ID"
+    assert f.floating_point_word_size == 8 and f.title == name.encode()
+    assert list(f.variables["time_whole"][:]) == [0.1489]
+    names = [b"".join(r).decode() for r in f.variables["name_elem_var"][:]]
+    assert names == ["density", "pressure"]
+    assert np.array_equal(f.variables["vals_elem_var1eb1"][0], d)
+    assert np.array_equal(f.variables["vals_elem_var2eb1"][0], p)
+
+
+def test_writer_refuses(tmp_path):
+    coords = np.array([[0, 0], [1, 0], [1, 1], [0, 1.0]])
+    out = str(tmp_path / "x.exo")
+    with pytest.raises(fa.FsaleError):
+        fa.write_exodus(out, coords, np.array([[0, 1, 2, 4]]))            # vertex out of range
+    with pytest.raises(fa.FsaleError):
+        fa.write_exodus(out, coords, np.array([[0, 1, 2]]))               # triangles
+    with pytest.raises(fa.FsaleError):
+        fa.write_exodus(out, coords, np.array([[0, 1, 2, 3]]), 0.0, {"x" * 33: [1.0]})
+    with pytest.raises(fa.FsaleError):
+        fa.write_exodus(str(tmp_path / "no_dir" / "x.exo"), coords, np.array([[0, 1, 2, 3]]))
+    fa.write_exodus(out, coords, np.array([[0, 1, 2, 3]]))                 # no variables
+    assert "num_elem_var" not in _netcdf(out).dimensions

@@ -13,6 +13,7 @@ import gzip
 import os
 import shutil
 
+import numpy as np
 import pytest
 
 import flecsale_amd as fa
@@ -76,3 +77,58 @@ def test_maire_hydro_3d_smoke_case(oracle_lib, tmp_path):
     # columns: ids, centroid(3), density, internal energy, pressure, velocity(3)
     for col, want in [(5, s.d), (6, s.e), (7, s.p), (8, s.v[:, 0]), (9, s.v[:, 1]), (10, s.v[:, 2])]:
         assert np.allclose(rows[:, col], want, rtol=1e-13, atol=0), col
+
+
+def _dump_columns(path, dim):
+    """Per-cell text columns of a dump (density, pressure) and the vertex
+    coordinate columns, as printed."""
+    cells, verts, sec = [], [], None
+    for line in open(path):
+        if line.startswith("# BEGIN"):
+            sec = cells if "CELLS" in line else verts
+        elif line.startswith("# END"):
+            sec = None
+        elif not line.startswith("#") and sec is not None:
+            sec.append(line.split())
+    return ([r[2 + dim] for r in cells], [r[4 + dim] for r in cells], [r[2:2 + dim] for r in verts])
+
+
+@pytest.mark.parametrize("app,mesh,names", [
+    ("hydro_2d", "square_32x32.g", ["shock_box_2d_rank000000.000000.exo", "shock_box_2d_rank000000.000020.exo"]),
+    ("maire_hydro_2d", "sedov_32x32.g", ["sedov_2d_rank000000_000000.exo", "sedov_2d_rank000000_000020.exo"]),
+])
+def test_output_task_files(app, mesh, names, tmp_path):
+    """The output task (apps/hydro/tasks.h:703-734, apps/maire_hydro/
+    tasks.h:661-692) on the regression problems: the files the drivers'
+    schedule writes (the initial state, then step 20: output_freq 1e6 and 20
+    with max_steps 20), named as the tasks name them; the last one holds the
+    dumped state and the (moved) mesh, printed digit for digit as the dump
+    prints them, at the dump's time."""
+    import scipy.io
+    try:
+        dump, files = apps.run(app, os.path.join(EXO, mesh), str(tmp_path), output=True)
+    except fa.FsaleError as e:
+        if "HDF5 library" in str(e):
+            pytest.skip("no HDF5 library to read the netCDF-4 mesh with")
+        raise
+    assert [os.path.basename(f) for f in files] == names
+    first = scipy.io.netcdf_file(files[0], "r", mmap=False)
+    last = scipy.io.netcdf_file(files[-1], "r", mmap=False)
+    euler = app.startswith("hydro")
+    var_names = [b"".join(r).decode() for r in last.variables["name_elem_var"][:]]
+    assert var_names == (["density", "pressure"] if euler else ["density"])
+    assert first.variables["time_whole"][0] == 0.0
+    d0 = first.variables["vals_elem_var1eb1"][0]
+    assert set(np.unique(d0)) <= ({0.125, 1.0} if euler else {1.0})
+    dim = 2
+    d_txt, p_txt, x_txt = _dump_columns(dump, dim)
+    with open(dump) as f:
+        t_txt = f.readline().split()[-1]
+    assert "%.14e" % last.variables["time_whole"][0] == t_txt
+    assert ["%.14e" % q for q in last.variables["vals_elem_var1eb1"][0]] == d_txt
+    if euler:
+        assert ["%.14e" % q for q in last.variables["vals_elem_var2eb1"][0]] == p_txt
+    x = np.stack([last.variables["coordx"][:], last.variables["coordy"][:]], 1)
+    assert [["%.14e" % q for q in r] for r in x] == x_txt
+    coords, cells = fa.read_exodus(files[-1])
+    assert np.array_equal(cells, fa.read_exodus(os.path.join(EXO, mesh))[1])

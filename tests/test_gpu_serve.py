@@ -97,3 +97,37 @@ def test_request_that_never_fits_waits_and_others_pass():
             assert st.done == 1 and st.steps == 6
         with pytest.raises(fa.FsaleError):
             srv.result(ib)               # 1600 cells never fit 1200: still queued
+
+
+def test_served_stream_requests_match_oracle(oracle_lib):
+    """Requests of the 4096-request served stream itself (sides 34..434, 100
+    steps each, both solvers): served together with continuous batching, each
+    result agrees with the oracle's run of the same request within the final-
+    time bound (1e-9 relative L-infinity per field), at the same time and
+    step count; a request the oracle finds failing fails the same way."""
+    from conftest import rel_linf
+    O = oracle_lib
+    idx = list(range(16)) + [29, 32]
+    sides = inputs.stream_sides(4096)
+    stream = [("euler" if i % 2 == 0 else "maire",
+               inputs.riemann_2d(int(sides[i]), "euler" if i % 2 == 0 else "maire", index=i, steps=100))
+              for i in idx]
+    assert max(int(sides[i]) for i in idx) >= 400
+    served, _, _ = _serve(stream, 0, chunk_steps=10)
+    for i, ((kind, prob), (st, r)) in enumerate(zip(stream, served)):
+        mesh = O.Mesh(prob["coords"], prob["cells"])
+        if kind == "euler":
+            s = O.euler_initial_state(mesh, prob["eos"][0], prob["eos"][1], prob["d"], prob["v"], prob["p"])
+            rc, t_o, n_o = O.euler_run(mesh, s, prob["eos"][0], prob["eos"][1], prob["cfl"], prob["final_time"],
+                                       100, prob["flux"])
+        else:
+            s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
+            rc, t_o, n_o, _ = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], 100, prob["dt0"])
+        if rc:
+            assert st.error != 0, (idx[i], rc)
+            continue
+        assert st.error == 0 and st.steps == n_o == 100 and st.time == t_o, idx[i]
+        for k, want in [("d", s.d), ("v", s.v), ("e", s.e), ("p", s.p)]:
+            assert rel_linf(r[k], want) <= 1e-9, (idx[i], k)
+        if kind == "maire":
+            assert rel_linf(r["x"], mesh.geometry()["coords"]) <= 1e-9, idx[i]
