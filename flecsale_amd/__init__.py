@@ -20,7 +20,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libflecsale_amd.so")
+# FSALE_LIB: another build of the library to load (e.g. one compiled with other options)
+LIB_PATH = os.environ.get("FSALE_LIB") or os.path.join(HERE, "libflecsale_amd.so")
 
 FLUX_HLLE, FLUX_RUSANOV, FLUX_HLLC = 0, 1, 2
 BOUNDARY_WALL, BOUNDARY_TRANSMISSIVE = 0x00, 0x10   # OR-ed into flux_kind

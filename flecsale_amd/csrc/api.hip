@@ -193,13 +193,20 @@ int sync_tiles(fsale_ctx *ctx) {
   }
   std::vector<int> plist;
   ctx->lds = ctx->lds_all = ctx->m.lnb != nullptr;
+  ctx->s3 = false;
+  ctx->s3_all = ctx->solver == FSALE_SOLVER_EULER && ctx->m.dim == 3;
   for (int p = 0; p < (int)ctx->probs.size(); ++p)
     if (ctx->probs[p].live) {
       plist.push_back(p);
       if (!ctx->probs[p].lds) ctx->lds = false;
       if (!ctx->probs[p].lds_all) ctx->lds_all = false;
+      if (ctx->probs[p].params.s_nx)
+        ctx->s3 = true;
+      else
+        ctx->s3_all = false;
     }
   if (!ctx->lds) ctx->lds_all = false;
+  if (!ctx->s3) ctx->s3_all = false;
   const int counts[4] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size(),
                          (int)ctx->h_btiles.size()};
   if (!plist.empty())
@@ -624,6 +631,10 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   P.params.cell_count = P.nc;
   P.params.tile_first = P.ct0;
   P.params.tile_count = P.nct;
+  if (ctx->solver == FSALE_SOLVER_EULER) {
+    P.params.s_nx = topo->s_nx;
+    P.params.s_np = topo->s_np;
+  }
   ctx->probs[prob] = P;
   ctx->tiles_dirty = true;
   *out_prob = prob;
@@ -1246,7 +1257,7 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   std::vector<double> key = {(double)ctx->solver, (double)chunk};
   for (int k = 0; k < K_COUNT; ++k) key.push_back((double)persistent_grid(ctx, k));
   key.push_back((double)ctx->pgrid);
-  key.push_back((double)ctx->lds + 2.0 * ctx->lds_all);
+  key.push_back((double)ctx->lds + 2.0 * ctx->lds_all + 4.0 * ctx->s3 + 8.0 * ctx->s3_all);
   GraphCache &gc = ctx->graph;
   GraphEntry *g = nullptr;
   for (auto &e : gc.entries)
@@ -1363,6 +1374,11 @@ struct Snapshot {
 // steps of the resident batch from the same snapshot, each launch timed
 // with HIP events; each kernel keeps its fastest shape.  The batch's state,
 // step budgets and profiling counters are left as they were.
+// A candidate shape replaces the best so far only when it is faster by this
+// fraction: per-launch timings of a few steps vary by a few percent, and one
+// block per tile (the first candidate) keeps each XCD's tiles contiguous.
+constexpr double kTuneMargin = 0.02;
+
 int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
   int rc;
   if ((rc = sync_tiles(ctx))) return rc;
@@ -1411,7 +1427,7 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
       if (k == K_MAIRE_NODAL || k == K_MAIRE_NODAL_BND) {
         pair_ms += tot;
         if (k == K_MAIRE_NODAL) pair_nl = nl;
-      } else if (nl > 0 && tot / nl < best[k]) {
+      } else if (nl > 0 && tot / nl < best[k] * (c ? 1.0 - kTuneMargin : 1.0)) {
         best[k] = tot / nl;
         per_step[k] = nl / nsteps;
         choice.blocks_per_cu[k] = c;
@@ -1420,7 +1436,7 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
       ctx->prof[k].launches = 0;
     }
     if (rc) break;
-    if (pair_nl > 0 && pair_ms / pair_nl < best[K_MAIRE_NODAL]) {
+    if (pair_nl > 0 && pair_ms / pair_nl < best[K_MAIRE_NODAL] * (c ? 1.0 - kTuneMargin : 1.0)) {
       best[K_MAIRE_NODAL] = pair_ms / pair_nl;   // per nodal evaluation, both kinds
       per_step[K_MAIRE_NODAL] = pair_nl / nsteps;
       choice.blocks_per_cu[K_MAIRE_NODAL] = c;
@@ -1956,7 +1972,10 @@ int fsale_lds_staging(fsale_ctx *ctx, int32_t *on) {
   if (!ctx || !on) return FSALE_ERR_ARG;
   int rc;
   if ((rc = sync_tiles(ctx))) return rc;
-  *on = ctx->lds_all ? 2 : ctx->lds ? 1 : 0;
+  if (ctx->m.dim == 3)
+    *on = ctx->s3_all ? 2 : ctx->s3 ? 1 : 0;
+  else
+    *on = ctx->lds_all ? 2 : ctx->lds ? 1 : 0;
   return FSALE_OK;
 }
 

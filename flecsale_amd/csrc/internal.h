@@ -32,6 +32,11 @@ constexpr int kNbk2 = 8, kNbk3 = 28;   // 3D: 26 hex neighbours, padded to group
 // many cells outside a tile (a 16 x 16 block of a structured mesh has 68),
 // sized so that three blocks of the fused kernel's staging fit one CU's LDS
 constexpr int kHalo2 = 80;
+// 3D hex tiles of a structured mesh (cells numbered x fastest, then y, then
+// z) staged in LDS by the Eulerian gradient kernel one neighbour plane at a
+// time: a tile's neighbours in the plane dz away lie in kBlock + 2 nx + 2
+// consecutive cells, so meshes up to nx = 191 cells per row are staged
+constexpr int kLds3 = 640;
 
 void set_message(const std::string &msg);
 
@@ -81,6 +86,10 @@ struct HostTopology {
   // their storage positions, ascending.  The cells' LDS slots are derived
   // from these on the device (k_lds_slots).
   std::vector<int32_t> halo_cnt, halo;
+  // 3D: the mesh is a structured nx x ny x nz hex grid numbered x fastest
+  // (every cell's cell-vertex neighbours are the grid's, in that numbering)
+  // and is stored in that order: s_nx = nx and s_np = nx * ny, else 0
+  int s_nx = 0, s_np = 0;
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
@@ -173,6 +182,7 @@ struct ProbParams {
                                            // of another rank (0: every cell is owned)
   int tile_first, tile_count;              // the problem's cell tiles (dt partials)
   int cell_first, cell_count;
+  int s_nx, s_np;                          // 3D structured grid strides (HostTopology), 0 if none
   // Maire boundary conditions of the faces of each symmetry tag (x, y, z):
   // FSALE_BC_* kind, pressure in bc_val[t][0] or velocity in bc_val[t][:dim]
   int bc_kind[3], bc_pad;
@@ -352,6 +362,7 @@ struct fsale_ctx {
   bool tiles_dirty = true;
   bool lds = false;                    // every live problem has LDS staging plans (sync_tiles)
   bool lds_all = false;                // ... for every one of its tiles
+  bool s3 = false, s3_all = false;     // some / every live problem is a structured 3D grid (s_nx)
   void *stage = nullptr;               // pinned staging buffer for uploads
   size_t stage_bytes = 0;
   void *dl = nullptr;                  // pinned buffer for multi-field downloads

@@ -46,6 +46,21 @@ template <class T>
 __device__ __forceinline__ void st(T *a, uint32_t off, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(a) + off) = v;
 }
+// the same with the non-temporal hint: streamed data that no other lane or
+// launch reads again soon, kept from displacing what is reused in the caches
+template <class T>
+__device__ __forceinline__ T ld_nt(const T *a, uint32_t off) {
+  return __builtin_nontemporal_load(reinterpret_cast<const T *>(reinterpret_cast<const char *>(a) + off));
+}
+template <class T>
+__device__ __forceinline__ void st_nt(T *a, uint32_t off, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(a) + off));
+}
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(double2 *a, uint32_t off, double2 v) {
+  dvec2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<dvec2 *>(reinterpret_cast<char *>(a) + off));
+}
 
 // ----------------------------------------------------------------- EOS
 // flecsale/eos/ideal_gas.h:178-226

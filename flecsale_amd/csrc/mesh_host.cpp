@@ -196,6 +196,44 @@ void build_halo_plan(HostTopology &t) {
   }
 }
 
+// A 3D hex mesh whose cells form a structured nx x ny x nz grid numbered x
+// fastest (HostTopology::s_nx, s_np): cell 0's neighbours name nx and nx*ny,
+// and every cell's ascending neighbour list must be the grid's (dz, dy, dx)
+// lexicographic list.  Off with FSALE_STRUCTURED=0, or when a tile's plane
+// range would not fit the staging (nx > (kLds3 - kBlock - 2) / 2).
+void detect_structured(HostTopology &t) {
+  t.s_nx = t.s_np = 0;
+  static const bool on = [] {
+    const char *e = std::getenv("FSALE_STRUCTURED");
+    return !(e && !std::strcmp(e, "0"));
+  }();
+  if (!on || t.dim != 3 || t.nc < 8) return;
+  for (int s = 0; s < t.nc; ++s)
+    if (t.cidx[s] != s) return;   // stored in another order
+  if (t.nbr_off[1] - t.nbr_off[0] != 7) return;
+  const int *n0 = t.nbr.data() + t.nbr_off[0];
+  const int nx = n0[1], np = n0[3];
+  if (nx < 2 || np < 2 * nx || np % nx || t.nc % np || t.nc / np < 2) return;
+  if (2 * nx + 2 + kBlock > kLds3) return;
+  const int ny = np / nx, nz = t.nc / np;
+  int want[26];
+  for (int c = 0; c < t.nc; ++c) {
+    const int i = c % nx, j = (c / nx) % ny, k = c / np;
+    int n = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if ((dz | dy | dx) == 0) continue;
+          if (i + dx < 0 || i + dx >= nx || j + dy < 0 || j + dy >= ny || k + dz < 0 || k + dz >= nz) continue;
+          want[n++] = c + dz * np + dy * nx + dx;
+        }
+    if (t.nbr_off[c + 1] - t.nbr_off[c] != n) return;
+    if (!std::equal(want, want + n, t.nbr.data() + t.nbr_off[c])) return;
+  }
+  t.s_nx = nx;
+  t.s_np = np;
+}
+
 // Pencil order of a 3D mesh's cell tiles (HostTopology::tile_order) from the
 // tiles' centroids; cellsum holds each cell's vertex-coordinate sum.
 void tile_pencils(HostTopology &t, const std::vector<double> &cellsum) {
@@ -417,6 +455,7 @@ int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &er
     morton_order(t.nf, D, pts, t.fidx, t.fpos, mode, fbox);
   }
   if (t.dim == 2 && t.vpc == 4) build_halo_plan(t);
+  detect_structured(t);
   return FSALE_OK;
 }
 

@@ -376,14 +376,16 @@ int fsale_batch_capacity(fsale_ctx *ctx, int64_t *free_cells, int64_t *free_face
 int fsale_launch_shape(fsale_ctx *ctx, int32_t kernel, int32_t blocks_per_cu);
 /* blocks_per_cu[12] receives the current shape of every kernel id. */
 int fsale_get_launch_shape(fsale_ctx *ctx, int32_t *blocks_per_cu);
-/* LDS staging of the Eulerian driver kernels (2D quads): each tile of 256
+/* LDS staging of the Eulerian driver kernels.  2D quads: each tile of 256
  * cells with at most 80 cells around it is staged in LDS with them, the
- * others gather from memory.  *on = 2: every resident tile is staged; 1:
- * some are; 0: none (3D, or not an Eulerian context).  The default storage
- * order of 2D meshes above 512^2 cells, spatial blocks of ~256 cells, makes
- * a structured mesh's tiles 16 x 16 blocks with 68 cells around each.  No
- * counterpart in the reference (a property of this implementation's
- * kernels); results are the same either way. */
+ * others gather from memory; the default storage order of 2D meshes above
+ * 512^2 cells, spatial blocks of ~256 cells, makes a structured mesh's
+ * tiles 16 x 16 blocks with 68 cells around each.  3D hexes: the gradient
+ * kernel stages the tiles of a structured grid numbered x fastest (rows of
+ * at most 191 cells) one neighbour plane at a time.  *on = 2: every
+ * resident problem's tiles are staged; 1: some are; 0: none (or not an
+ * Eulerian context).  No counterpart in the reference (a property of this
+ * implementation's kernels); results are the same either way. */
 int fsale_lds_staging(fsale_ctx *ctx, int32_t *on);
 /* Autotuner: times nsteps eager steps of the resident problems under each
  * candidate shape (HIP events per launch) from one snapshot of the state,

@@ -110,3 +110,83 @@ def test_staged_and_memory_tiles_mix():
     for key in ("sod", "small"):
         for k in ident[key]:
             assert np.array_equal(np.asarray(ident[key][k]), np.asarray(blk[key][k])), (key, k)
+
+
+# ------------------------------------------------------------ 3D plane staging
+
+def _kh_box(nx, ny, nz, steps=5, perm=None):
+    """A Kelvin-Helmholtz-like shear state on a jittered nx x ny x nz hex box
+    (cells numbered x fastest, or renumbered by perm)."""
+    coords, cells = inputs.box_hexes(nx, ny, nz, jitter=0.1, seed=3)
+    if perm is not None:
+        cells = cells[perm]
+    x = coords[cells].mean(axis=1)
+    nc = len(cells)
+    rng = np.random.default_rng(nx * 10000 + ny * 100 + nz)
+    inner = np.abs(x[:, 1] - 0.5) < 0.25
+    v = np.zeros((nc, 3))
+    v[:, 0] = np.where(inner, 0.5, -0.5)
+    v[:, 1] = 0.05 * rng.uniform(-1, 1, nc)
+    v[:, 2] = 0.05 * rng.uniform(-1, 1, nc)
+    return dict(coords=coords, cells=cells, eos=(1.4, 1.0), d=np.where(inner, 2.0, 1.0) + 0.1 * rng.random(nc),
+                v=v, p=2.5 + 0.1 * rng.random(nc), cfl=1.0 / 3.0, final_time=1.0, max_steps=steps, flux=0)
+
+
+def _oracle_run(O, prob):
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    s = O.euler_initial_state(mesh, *prob["eos"], prob["d"], prob["v"], prob["p"])
+    rc, t, n = O.euler_run(mesh, s, *prob["eos"], prob["cfl"], prob["final_time"], prob["max_steps"], prob["flux"])
+    assert rc == 0 and n == prob["max_steps"]
+    return s, t
+
+
+@pytest.mark.parametrize("shape,staged", [
+    ((13, 11, 7), 2),     # rows of 13: tiles straddle rows and planes
+    ((9, 8, 7), 2),
+    ((2, 2, 2), 2),       # the smallest grid: one tile, every plane clipped
+    ((64, 4, 3), 2),
+    ((191, 2, 2), 2),     # the widest row the staging holds
+    ((192, 2, 2), 0),     # one cell wider: the neighbour planes gather from memory
+])
+def test_structured_3d_grids_match_oracle(oracle_lib, shape, staged):
+    """The gradient kernel stages the neighbour planes of a structured grid's
+    tiles in LDS (fsale_lds_staging 2) when its rows hold at most 191 cells;
+    staged or not, a run equals the oracle's bit for bit."""
+    print("3D plane staging", shape, flush=True)
+    prob = _kh_box(*shape)
+    s, t = _oracle_run(oracle_lib, prob)
+    with fa.EulerContext(prob["coords"], prob["cells"]) as ctx:
+        assert ctx.lds_staging() == staged
+        ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+        ctx.advance(prob["cfl"], prob["final_time"], prob["max_steps"], flux_kind=prob["flux"])
+        got = ctx.state()
+        assert ctx.time()[0] == t
+    for k in ("d", "v", "e", "p", "T", "a"):
+        assert np.array_equal(got[k], np.asarray(getattr(s, k)).reshape(got[k].shape)), k
+
+
+def test_structured_and_renumbered_3d_grids_mix(oracle_lib):
+    """A grid numbered x fastest is staged; the same grid with its cells
+    renumbered is not (its neighbour lists are not the grid's in that
+    numbering).  In one batch both run in the same launches (the staged
+    kernel with its memory path), each equal to the oracle bit for bit."""
+    print("3D plane staging: mixed batch", flush=True)
+    grid = _kh_box(11, 9, 8, steps=4)
+    perm = np.random.default_rng(5).permutation(11 * 9 * 8)
+    shuffled = _kh_box(11, 9, 8, steps=4, perm=perm)
+    want = [_oracle_run(oracle_lib, p) for p in (grid, shuffled)]
+    with fa.EulerContext(shuffled["coords"], shuffled["cells"]) as ctx:
+        assert ctx.lds_staging() == 0
+    nc = 2 * len(grid["cells"])
+    nv = 2 * len(grid["coords"])
+    with fa.Batch(fa.SOLVER_EULER, 3, 8, 4, nc + 512, 4 * nc + 512, nv + 512, 27 * nc + 512) as b:
+        ids = [b.add_euler(p["coords"], p["cells"], p["eos"], p["d"], p["v"], p["p"], p["cfl"],
+                           p["final_time"], p["max_steps"], flux_kind=p["flux"]) for p in (grid, shuffled)]
+        assert b.lds_staging() == 1
+        b.step(4)
+        st = b.status()
+        for pid, (s, t) in zip(ids, want):
+            assert st[pid].steps == 4 and st[pid].time == t
+            got = b.state(pid)
+            for k in ("d", "v", "e", "p"):
+                assert np.array_equal(got[k], np.asarray(getattr(s, k)).reshape(got[k].shape)), (pid, k)
