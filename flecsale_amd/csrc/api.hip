@@ -276,7 +276,9 @@ Batch batch_args(const fsale_ctx *ctx, int kid) {
 // grid of CUs x n blocks, never more blocks than the capacity has tiles.
 int persistent_grid(const fsale_ctx *ctx, int kid) {
   const bool vertex_list = kid == K_MAIRE_NODAL || kid == K_MAIRE_MOVE || kid == K_MAIRE_NODAL_BND;
-  const bool both = kid == K_MAIRE_COPY;
+  // the Lagrangian update also moves the vertices (a 2D driver step's
+  // k_maire_move_update walks both lists)
+  const bool both = kid == K_MAIRE_COPY || (kid == K_MAIRE_UPDATE && ctx->solver == FSALE_SOLVER_MAIRE);
   const int bpc = (kid >= 0 && kid < K_COUNT) ? ctx->shape.blocks_per_cu[kid] : 0;
   if (bpc <= 0) {
     const int g = kid == K_MAIRE_NODAL_BND ? ctx->n_bdirect
@@ -512,9 +514,11 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if ((rc = dev_alloc_n(ctx, &m.nbk, (size_t)cap_c * m.nbk_deg))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.cfn, (size_t)cap_c * m.fpc))) return rc;
   if ((rc = dev_alloc_n(ctx, &m.corig, cap_c))) return rc;
-  if (dim == 3 && solver == FSALE_SOLVER_EULER)
-    for (int i = 0; i < 7; ++i)
-      if ((rc = dev_alloc_n(ctx, &m.kf[i], (size_t)cap_c * m.fpc))) return rc;
+  if (dim == 3 && solver == FSALE_SOLVER_EULER) {
+    for (int i = 0; i < 3; ++i)
+      if ((rc = dev_alloc_n(ctx, &m.kfp[i], (size_t)cap_c * m.fpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.kfa, (size_t)cap_c * m.fpc))) return rc;
+  }
   if (dim == 2)
     if ((rc = dev_alloc_n(ctx, &m.xy, cap_v))) return rc;
   if (dim == 2 && vpc == 4 && solver == FSALE_SOLVER_EULER) {
@@ -1245,12 +1249,14 @@ namespace {
 // the steps of batch_steps, eager or from the captured graph
 int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   int rc;
-  auto step = [&](bool first) {
-    return ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx, first);
+  // first / last: the step opens / closes a run of consecutive steps (the
+  // Lagrangian step defers its close to the next step of the run)
+  auto step = [&](bool first, bool last) {
+    return ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx, first, last);
   };
   if (!use_graph || nsteps < 2 || ctx->profiling) {
     for (int n = 0; n < nsteps; ++n)
-      if ((rc = step(n == 0))) return rc;
+      if ((rc = step(n == 0, n == nsteps - 1))) return rc;
     return FSALE_OK;
   }
   const int chunk = nsteps < 10 ? nsteps : 10;
@@ -1283,7 +1289,7 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
       return FSALE_ERR_HIP;
     }
     for (int n = 0; n < chunk; ++n) {
-      rc = step(n == 0);
+      rc = step(n == 0, n == chunk - 1);
       if (rc) {
         (void)hipStreamEndCapture(ctx->stream, &graph);
         (void)hipEventDestroy(e.done);
@@ -1307,8 +1313,8 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   int n = 0;
   for (; n + chunk <= nsteps; n += chunk) FSALE_HIP(hipGraphLaunch(g->exec, ctx->stream));
   FSALE_HIP(hipEventRecord(g->done, ctx->stream));
-  for (; n < nsteps; ++n)
-    if ((rc = step(false))) return rc;
+  for (const int n0 = n; n < nsteps; ++n)
+    if ((rc = step(n == n0, n == nsteps - 1))) return rc;
   return FSALE_OK;
 }
 
@@ -1414,7 +1420,7 @@ int autotune(fsale_ctx *ctx, int nsteps, double *best_ms) {
     if ((rc = snap.restore(ctx))) break;
     if ((rc = refresh_packed(ctx))) break;
     for (int n = 0; n < nsteps && !rc; ++n)
-      rc = ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx, n == 0);
+      rc = ctx->solver == FSALE_SOLVER_EULER ? euler_step(ctx) : maire_step(ctx, n == 0, n == nsteps - 1);
     if (rc) break;
     // the two nodal kernels are one launch at one block per tile: they are
     // chosen together, by their summed time per step

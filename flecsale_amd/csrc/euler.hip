@@ -85,8 +85,11 @@ __global__ void __launch_bounds__(kBlock) k_eul_dt_partial(DevMesh m, EulerField
           face_geom_2d(px[k], py[k], px[k1], py[k1], fx, fy, n[0], n[1], area);
         } else {
           const size_t o = (size_t)k * m.cap_c + c;
-          for (int d = 0; d < N; ++d) n[d] = m.kf[3 + d][o];
-          area = m.kf[6][o];
+          const double2 w1 = m.kfp[1][o], w2 = m.kfp[2][o];
+          n[0] = w1.y;
+          n[1] = w2.x;
+          n[2] = w2.y;
+          area = m.kfa[o];
         }
         const double delta_x = vol / area;
         const double dti = (a + fabs(dotN<N>(vel, n))) / delta_x;
@@ -900,22 +903,22 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
         else
           face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
       } else {
-#if FSALE_NT3
         const size_t row = (size_t)k * m.cap_c;
         const uint32_t o = (uint32_t)c * 8u;
-        for (int d = 0; d < N; ++d) {
-          fx[d] = ld_nt(m.kf[d] + row, o);
-          nk[k][d] = ld_nt(m.kf[3 + d] + row, o);
-        }
-        ak[k] = ld_nt(m.kf[6] + row, o);
+#if FSALE_NT3
+        const double2 w0 = ld_nt(m.kfp[0] + row, 2 * o), w1 = ld_nt(m.kfp[1] + row, 2 * o),
+                      w2 = ld_nt(m.kfp[2] + row, 2 * o);
+        ak[k] = ld_nt(m.kfa + row, o);
 #else
-        const size_t o = (size_t)k * m.cap_c + c;
-        for (int d = 0; d < N; ++d) {
-          fx[d] = m.kf[d][o];
-          nk[k][d] = m.kf[3 + d][o];
-        }
-        ak[k] = m.kf[6][o];
+        const double2 w0 = ld(m.kfp[0] + row, 2 * o), w1 = ld(m.kfp[1] + row, 2 * o), w2 = ld(m.kfp[2] + row, 2 * o);
+        ak[k] = ld(m.kfa + row, o);
 #endif
+        fx[0] = w0.x;
+        fx[1] = w0.y;
+        fx[2] = w1.x;
+        nk[k][0] = w1.y;
+        nk[k][1] = w2.x;
+        nk[k][2] = w2.y;
       }
       // the neighbour's loads do not wait on a branch (a boundary face
       // loads the cell's own data again and ignores it)

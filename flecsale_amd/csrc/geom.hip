@@ -91,11 +91,10 @@ __global__ void __launch_bounds__(kBlock) k_cellface_geom_3d(DevMesh m, int c0, 
   for (int k = 0; k < m.fpc; ++k) {
     const int f = m.cf[(size_t)k * m.cap_c + c] >> 1;
     const size_t o = (size_t)k * m.cap_c + c;
-    for (int d = 0; d < 3; ++d) {
-      m.kf[d][o] = m.fcen[d][f];
-      m.kf[3 + d][o] = m.fnrm[d][f];
-    }
-    m.kf[6][o] = m.farea[f];
+    m.kfp[0][o] = make_double2(m.fcen[0][f], m.fcen[1][f]);
+    m.kfp[1][o] = make_double2(m.fcen[2][f], m.fnrm[0][f]);
+    m.kfp[2][o] = make_double2(m.fnrm[1][f], m.fnrm[2][f]);
+    m.kfa[o] = m.farea[f];
   }
 }
 
@@ -131,7 +130,7 @@ int launch_geometry(fsale_ctx *ctx, int prob, bool cells, bool faces, hipStream_
     else
       hipLaunchKernelGGL(k_face_geom_3d, dim3(g), dim3(kBlock), 0, stream, m, P.f0, P.nf);
     FSALE_HIP(hipGetLastError());
-    if (m.kf[0] && P.nc > 0) {
+    if (m.kfa && P.nc > 0) {
       hipLaunchKernelGGL(k_cellface_geom_3d, dim3((P.nc + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, m, P.c0,
                          P.nc);
       FSALE_HIP(hipGetLastError());

@@ -139,10 +139,12 @@ struct DevMesh {
   double *fcen[3] = {nullptr, nullptr, nullptr};
   double *fnrm[3] = {nullptr, nullptr, nullptr};
   double *farea = nullptr;
-  // 3D: the geometry of each cell's local faces again, k-major [fpc][cap_c]
-  // (centroid 0-2, normal 3-5 as stored for the face, area 6), so the fused
-  // step reads it coalesced instead of gathering 8 bytes per 128-byte line
-  double *kf[7] = {};
+  // 3D: the geometry of each cell's local faces again, k-major [fpc][cap_c],
+  // so the fused step reads it coalesced instead of gathering 8 bytes per
+  // 128-byte line: pairs (centroid x, y), (centroid z, normal x), (normal y,
+  // z) as stored for the face, and the area; four loads per face
+  double2 *kfp[3] = {};
+  double *kfa = nullptr;
   // 2D: vertex coordinates again as (x, y) pairs [cap_v], one 16-byte
   // gather per vertex (the Lagrangian kernels that move vertices write both)
   double2 *xy = nullptr;
@@ -241,6 +243,11 @@ struct MaireFields {
   // 2D: the same as two planes of pairs, (rho a, p) and (v_x, v_y) [cap_c]
   // (the coordinates are DevMesh::xy)
   double2 *cs2[2] = {};
+  // 2D driver steps: the coordinates at the start of the step (x0 again, as
+  // pairs), saved by the step's first nodal solve; the predictor and
+  // corrector updates move the vertices from these and rebuild each cell's
+  // geometry from the same values
+  double2 *x0p = nullptr;
 };
 
 // Kernel arguments common to all step kernels
@@ -439,5 +446,5 @@ int euler_pack(fsale_ctx *ctx);       // per-field state -> the driver loop's pa
 int euler_unpack(fsale_ctx *ctx);     // and back
 // Maire (maire.hip)
 int maire_alloc(fsale_ctx *ctx);
-int maire_step(fsale_ctx *ctx, bool first);
+int maire_step(fsale_ctx *ctx, bool first, bool last);
 }  // namespace fsale

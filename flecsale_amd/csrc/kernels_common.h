@@ -57,6 +57,10 @@ __device__ __forceinline__ void st_nt(T *a, uint32_t off, T v) {
   __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(a) + off));
 }
 typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_nt(const double2 *a, uint32_t off) {
+  const dvec2 w = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(reinterpret_cast<const char *>(a) + off));
+  return make_double2(w.x, w.y);
+}
 __device__ __forceinline__ void st_nt(double2 *a, uint32_t off, double2 v) {
   dvec2 w = {v.x, v.y};
   __builtin_nontemporal_store(w, reinterpret_cast<dvec2 *>(reinterpret_cast<char *>(a) + off));

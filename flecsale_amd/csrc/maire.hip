@@ -679,18 +679,43 @@ __device__ inline void corner_force(const MaireFields &u, int c, const double M[
 // on a boundary face, whose symmetry QR solve would otherwise hold the
 // registers (and the waves) of the whole interior launch.
 // one tile of interior (Cramer) or boundary (symmetry QR) vertices
-template <bool BOUNDARY, int N>
+// Whether a problem's waves work in a 2D driver step's first nodal solve and
+// residual.  PEND: the previous step's close (k_maire_end's update of t,
+// steps and the step flag) has not run yet -- it runs in k_maire_close_dt
+// right after these two launches -- so the step flag it would set is
+// computed here, from the same values, which nothing writes meanwhile.
+template <bool PEND>
+__device__ inline bool step_live(const Batch &b, int prob) {
+  if (!wave_live(b.s, prob, 0)) return false;
+  if constexpr (PEND) {
+    const DevScalars &S = b.s[prob];
+    const ProbParams &P = b.pp[prob];
+    return S.err_code == 0 && S.steps + 1 < P.max_steps && S.t + S.dt < P.final_time;
+  }
+  return true;
+}
+
+// SAVE (2D, a driver step's first nodal solve): each vertex also saves its
+// coordinates as the step's start (x0, x0p; save_coordinates), which no
+// kernel of this launch reads.
+template <bool BOUNDARY, int N, bool SAVE = false, bool PEND = false>
 __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
   if constexpr (N == 3) {
     nodal3_tile<BOUNDARY ? 1 : 0>(m, u, b, tile);
   } else {
-    if (!wave_live(b.s, tile.prob, 0)) return;
+    if (!step_live<PEND>(b, tile.prob)) return;
     if ((int)threadIdx.x >= tile.count) return;
     int vt = tile.first + threadIdx.x;
     if constexpr (BOUNDARY)
       vt = m.bv[vt];
     else if (m.vbnd[vt])
       return;
+    if constexpr (SAVE) {
+      const double2 w = m.xy[vt];
+      u.x0p[vt] = w;
+      u.x0[0][vt] = w.x;
+      u.x0[1][vt] = w.y;
+    }
     const int cnt = m.vc_cnt[vt];
     double M[kNodalKeep][2][2], np[kNodalKeep][2], F[kNodalKeep][2];
     int cc[kNodalKeep], kk[kNodalKeep];
@@ -769,9 +794,10 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
   }
 }
 
-template <bool BOUNDARY, int N>
+template <bool BOUNDARY, int N, bool SAVE = false, bool PEND = false>
 __global__ void __launch_bounds__(kBlock) k_maire_nodal(DevMesh m, MaireFields u, Batch b) {
-  for_each_tile(BOUNDARY ? b.bt : b.vt, [&](const Tile &tile) { nodal_tile<BOUNDARY, N>(m, u, b, tile); });
+  for_each_tile(BOUNDARY ? b.bt : b.vt,
+                [&](const Tile &tile) { nodal_tile<BOUNDARY, N, SAVE, PEND>(m, u, b, tile); });
 }
 
 // 3D boundary vertices with two or three symmetry rows
@@ -786,19 +812,19 @@ __global__ void __launch_bounds__(kBlock) k_maire3_nodal_bnd_big(DevMesh m, Mair
 // run while the interior waves fill the machine around them instead of
 // forming a launch of their own.  Held at 3 waves per SIMD for the interior
 // tiles (the boundary path then keeps part of its state in scratch).
-template <int N>
+template <int N, bool SAVE = false, bool PEND = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
   if ((int)blockIdx.x < nb) {
     if ((int)blockIdx.x < b.bt.ndirect) {
       const Tile t = b.bt.direct[blockIdx.x];
-      if (t.count > 0) nodal_tile<true, N>(m, u, b, t);
+      if (t.count > 0) nodal_tile<true, N, SAVE, PEND>(m, u, b, t);
     }
   } else {
     const int i = (int)blockIdx.x - nb;
     if (i < b.vt.ndirect) {
       const Tile t = b.vt.direct[i];
-      if (t.count > 0) nodal_tile<false, N>(m, u, b, t);
+      if (t.count > 0) nodal_tile<false, N, SAVE, PEND>(m, u, b, t);
     }
   }
 }
@@ -806,11 +832,11 @@ k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
 // DT: also the time step's per-wave partials of the new residual (the
 // driver step's predictor, whose time step follows), as k_maire_dt_partial
 // computes them.
-template <int N, bool DT>
+template <int N, bool DT, bool PEND = false>
 __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, Batch b) {
   constexpr int V = N == 2 ? 4 : 8;
   for_each_tile(b.ct, [&](const Tile &tile) {
-    if (!wave_live(b.s, tile.prob, 0)) return;
+    if (!step_live<PEND>(b, tile.prob)) return;
     double acc = 0, vol = 0;
     if ((int)threadIdx.x < tile.count) {
     const int c = tile.first + threadIdx.x;
@@ -875,14 +901,28 @@ __global__ void __launch_bounds__(kBlock) k_maire_dt_partial(DevMesh m, MaireFie
   });
 }
 
+// the driver step's close (apps/maire_hydro/driver.h:514-515): t += dt,
+// one step more, and the next step's flag (k_maire_begin's)
+__device__ inline void close_step(DevScalars &S, const ProbParams &P) {
+  if (S.active && !S.err_code) {
+    S.t += S.dt;
+    S.steps = S.steps + 1;
+  }
+  S.active = (S.err_code == 0) && (S.steps < P.max_steps) && (S.t < P.final_time);
+}
+
 // evaluate_time_step :179-188 (first minimum of the three limits) and the
-// driver's cap min(dt, t_final - t) (driver.h:391), one block per problem
-__global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, const double *partials) {
+// driver's cap min(dt, t_final - t) (driver.h:391), one block per problem.
+// CLOSE: first the previous step's close_step (a 2D driver step defers it
+// to here, after its first nodal solve and residual).
+template <bool CLOSE>
+__device__ inline void dt_finalize(const Batch &b, const double *partials) {
   const int nprob = *b.np;
   for (int ip = blockIdx.x; ip < nprob; ip += gridDim.x) {
     const int p = b.plist[ip];
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
+    if (CLOSE && threadIdx.x == 0) close_step(b.s[p], P);
     if (!block_active(b.s, p)) continue;
     double mx[2];
     final_max<2>(partials + 2 * (size_t)P.tile_first * kWaves, P.tile_count * kWaves, mx);
@@ -896,6 +936,12 @@ __global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, cons
       if (dts[i] < dts[it]) it = i;
     S.dt = smin(dts[it], P.final_time - S.t);
   }
+}
+__global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, const double *partials) {
+  dt_finalize<false>(b, partials);
+}
+__global__ void __launch_bounds__(kFinalBlock) k_maire_close_dt(Batch b, const double *partials) {
+  dt_finalize<true>(b, partials);
 }
 
 // apply_update's mesh motion x = x0 + dt u (:517-534).  SAVE (the driver
@@ -936,10 +982,14 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
 // predictor, which first saves v0 = v, e0 = e (save_solution); 2: the
 // corrector, which starts from v0, e0 (restore_solution) instead of copying
 // them back first.
-template <int MODE, int N>
-__global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, Batch b, double factor,
-                                                         int token) {
-  for_each_tile(b.ct, [&](const Tile &tile) {
+// MOVED (2D driver steps): the cell's vertices are moved here from the
+// step's start, x0 + dt u as k_maire_move computes them, instead of read
+// back after a move launch (k_maire_move_update moves them in the same
+// launch).
+template <int MODE, int N, bool MOVED = false>
+__device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields &u, const Batch &b, double factor,
+                                            int token, const Tile &tile) {
+  {
     if (!wave_live(b.s, tile.prob, token)) return;
     if ((int)threadIdx.x >= tile.count) return;
     const ProbParams &P = b.pp[tile.prob];
@@ -951,9 +1001,16 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
       if constexpr (N == 2) {
         double px[4], py[4];
         for (int i = 0; i < 4; ++i) {
-          const double2 w = m.xy[m.cv[(size_t)i * m.cap_c + c]];
-          px[i] = w.x;
-          py[i] = w.y;
+          const int vt = m.cv[(size_t)i * m.cap_c + c];
+          if constexpr (MOVED) {
+            const double2 w = u.x0p[vt];
+            px[i] = w.x + delta_t * u.un[0][vt];
+            py[i] = w.y + delta_t * u.un[1][vt];
+          } else {
+            const double2 w = m.xy[vt];
+            px[i] = w.x;
+            py[i] = w.y;
+          }
         }
         double cx, cy, ml;
         cell_geom_2d(4, px, py, cx, cy, vol, ml);
@@ -1013,7 +1070,38 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
     else
       put_cs2(u, c, d, a, p, vel[0], vel[1]);
     if (ie < 0 || d < 0) flag_negative(b.s, tile.prob, m.corig[c], token);
+  }
+}
+
+template <int MODE, int N>
+__global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields u, Batch b, double factor,
+                                                         int token) {
+  for_each_tile(b.ct, [&](const Tile &tile) { update_tile<MODE, N>(m, u, b, factor, token, tile); });
+}
+
+// A 2D driver step's apply_update in one launch: the mesh move x = x0 + dt u
+// (:517-534) from the step's start coordinates saved by its first nodal
+// solve (x0p), and the cells' update (update_tile MOVED), which takes its
+// vertices' moved positions from the same values, so no block waits on
+// another.  MODE 1: the predictor, 2: the corrector.  The vertex tiles use
+// the launch's token, as the update does: an error this launch raises does
+// not stop its moves (a separate move launch would have completed first).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFields u, Batch b, double factor,
+                                                              int token) {
+  for_each_tile(b.vt, [&](const Tile &tile) {
+    if (!wave_live(b.s, tile.prob, token)) return;
+    if ((int)threadIdx.x >= tile.count) return;
+    const double delta_t = factor * b.s[tile.prob].dt;
+    if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
+    const int vt = tile.first + threadIdx.x;
+    const double2 w = u.x0p[vt];
+    const double x = w.x + delta_t * u.un[0][vt], y = w.y + delta_t * u.un[1][vt];
+    m.x[0][vt] = x;
+    m.x[1][vt] = y;
+    put_xy(m, vt, x, y);
   });
+  for_each_tile(b.ct, [&](const Tile &tile) { update_tile<MODE, 2, true>(m, u, b, factor, token, tile); });
 }
 
 // save (dir 0) / restore (dir 1) of coordinates, velocity and energy
@@ -1073,12 +1161,7 @@ __global__ void k_maire_end(Batch b) {
     const int p = b.plist[ip];
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
-    DevScalars &S = b.s[p];
-    if (S.active && !S.err_code) {
-      S.t += S.dt;
-      S.steps = S.steps + 1;
-    }
-    S.active = (S.err_code == 0) && (S.steps < P.max_steps) && (S.t < P.final_time);
+    close_step(b.s[p], P);
   }
 }
 
@@ -1108,7 +1191,7 @@ bool check_ctx(fsale_ctx *ctx) {
 // (Cramer) and then the boundary vertices' symmetry solves.  3D keeps two
 // launches: its boundary solve (QR up to 6x6) would cap the interior waves'
 // registers.
-template <int N>
+template <int N, bool SAVE = false, bool PEND = false>
 int launch_nodal_n(fsale_ctx *ctx) {
   Batch bi = batch_args(ctx, K_MAIRE_NODAL);
   const Batch bb = batch_args(ctx, K_MAIRE_NODAL_BND);
@@ -1127,15 +1210,15 @@ int launch_nodal_n(fsale_ctx *ctx) {
     bi.bt = bb.bt;
     const int nb = (bb.bt.ndirect + 7) / 8 * 8;
     ProfScope prof(ctx, K_MAIRE_NODAL);
-    hipLaunchKernelGGL(k_maire_nodal_both<2>, dim3(nb + bi.vt.ndirect), dim3(kBlock), 0, ctx->stream, ctx->m,
-                       ctx->ma, bi, nb);
+    hipLaunchKernelGGL((k_maire_nodal_both<2, SAVE, PEND>), dim3(nb + bi.vt.ndirect), dim3(kBlock), 0, ctx->stream,
+                       ctx->m, ctx->ma, bi, nb);
     FSALE_HIP(hipGetLastError());
     return FSALE_OK;
   }
-  int rc = launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal<false, N>, ctx->m, ctx->ma,
+  int rc = launch_persistent(ctx, K_MAIRE_NODAL, k_maire_nodal<false, N, SAVE, PEND>, ctx->m, ctx->ma,
                              batch_args(ctx, K_MAIRE_NODAL));
   return rc ? rc
-            : launch_persistent(ctx, K_MAIRE_NODAL_BND, k_maire_nodal<true, N>, ctx->m, ctx->ma,
+            : launch_persistent(ctx, K_MAIRE_NODAL_BND, k_maire_nodal<true, N, SAVE, PEND>, ctx->m, ctx->ma,
                                 batch_args(ctx, K_MAIRE_NODAL_BND));
 }
 int launch_nodal(fsale_ctx *ctx) { return ctx->m.dim == 3 ? launch_nodal_n<3>(ctx) : launch_nodal_n<2>(ctx); }
@@ -1148,10 +1231,14 @@ int launch_residual(fsale_ctx *ctx, bool with_dt = false) {
   return ctx->m.dim == 3 ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<3, false>, ctx->m, ctx->ma, b)
                          : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, false>, ctx->m, ctx->ma, b);
 }
-int launch_dt_finalize(fsale_ctx *ctx) {
+int launch_dt_finalize(fsale_ctx *ctx, bool close = false) {
   ProfScope prof(ctx, K_MAIRE_DT);
-  hipLaunchKernelGGL(k_maire_dt_finalize, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream, batch_args(ctx, -1),
-                     (const double *)ctx->ma.partials);
+  if (close)
+    hipLaunchKernelGGL(k_maire_close_dt, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream, batch_args(ctx, -1),
+                       (const double *)ctx->ma.partials);
+  else
+    hipLaunchKernelGGL(k_maire_dt_finalize, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream,
+                       batch_args(ctx, -1), (const double *)ctx->ma.partials);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -1178,6 +1265,13 @@ int launch_apply_n(fsale_ctx *ctx, double factor, int token, int stage) {
 int launch_apply(fsale_ctx *ctx, double factor, int token, int stage) {
   return ctx->m.dim == 3 ? launch_apply_n<3>(ctx, factor, token, stage) : launch_apply_n<2>(ctx, factor, token, stage);
 }
+// a 2D driver step's move + update (stage 1 predictor, 2 corrector)
+int launch_move_update(fsale_ctx *ctx, double factor, int token, int stage) {
+  const Batch b = batch_args(ctx, K_MAIRE_UPDATE);
+  if (stage == 1)
+    return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_move_update<1>, ctx->m, ctx->ma, b, factor, token);
+  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_move_update<2>, ctx->m, ctx->ma, b, factor, token);
+}
 int launch_copy(fsale_ctx *ctx, int dir) {
   const Batch b = batch_args(ctx, K_MAIRE_COPY);
   return ctx->m.dim == 3 ? launch_persistent(ctx, K_MAIRE_COPY, k_maire_copy<3>, ctx->m, ctx->ma, b, dir)
@@ -1186,7 +1280,8 @@ int launch_copy(fsale_ctx *ctx, int dir) {
 
 }  // namespace
 
-// One driver step (apps/maire_hydro/driver.h:330-538) of every live problem:
+// One driver step (apps/maire_hydro/driver.h:330-538) of every live problem
+// (`first` / `last`: the first / last step of a batch of steps):
 // predictor half step, then the corrector from the saved state.  The
 // driver's save_coordinates / save_solution happen inside the predictor's
 // move and update (each saves what it reads before writing), and
@@ -1195,12 +1290,35 @@ int launch_copy(fsale_ctx *ctx, int dir) {
 // does not move, dt <= 1e-14, the predictor did not move either and x is
 // still x0).  `first`: the first step of a batch of steps, which opens its
 // step with k_maire_begin (later ones are opened by the previous k_maire_end).
-int maire_step(fsale_ctx *ctx, bool first) {
+int maire_step(fsale_ctx *ctx, bool first, bool last) {
   int rc;
   const int np = (int)ctx->probs.size();
   if (first) {
     hipLaunchKernelGGL(k_maire_begin, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
     FSALE_HIP(hipGetLastError());
+  }
+  if (ctx->m.dim == 2) {
+    // the first nodal solve saves the step's start coordinates, the moves
+    // ride in the updates, and the close of a step runs with the next
+    // step's time step (k_maire_close_dt), or after the last step of the
+    // batch of steps (k_maire_end): six launches per step
+    if ((rc = first ? launch_nodal_n<2, true, false>(ctx) : launch_nodal_n<2, true, true>(ctx))) return rc;
+    {
+      const Batch b = batch_args(ctx, K_MAIRE_RESIDUAL);
+      rc = first ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, true, false>, ctx->m, ctx->ma, b)
+                 : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, true, true>, ctx->m, ctx->ma, b);
+      if (rc) return rc;
+    }
+    if ((rc = launch_dt_finalize(ctx, !first))) return rc;
+    if ((rc = launch_move_update(ctx, 0.5, 1, 1))) return rc;
+    if ((rc = launch_nodal(ctx))) return rc;
+    if ((rc = launch_residual(ctx))) return rc;
+    if ((rc = launch_move_update(ctx, 1.0, 2, 2))) return rc;
+    if (last) {
+      hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
+      FSALE_HIP(hipGetLastError());
+    }
+    return FSALE_OK;
   }
   if ((rc = launch_nodal(ctx))) return rc;
   if ((rc = launch_residual(ctx, true))) return rc;
@@ -1232,9 +1350,11 @@ int maire_alloc(fsale_ctx *ctx) {
   for (int i = 0; i < m.dim + 2; ++i)
     if ((rc = dev_alloc_n(ctx, &u.dudt[i], m.cap_c))) return rc;
   if ((rc = dev_alloc_n(ctx, &u.partials, 2 * (size_t)kWaves * (ctx->ra_ct.capacity + 1)))) return rc;
-  if (m.dim == 2)
+  if (m.dim == 2) {
     for (int i = 0; i < 2; ++i)
       if ((rc = dev_alloc_n(ctx, &u.cs2[i], m.cap_c))) return rc;
+    if ((rc = dev_alloc_n(ctx, &u.x0p, m.cap_v))) return rc;
+  }
   if (m.dim == 3) {
     if ((rc = dev_alloc_n(ctx, &u.cv8, (size_t)m.cap_c * 8 + 8))) return rc;
     if ((rc = dev_alloc_n(ctx, &u.xa, (size_t)m.cap_v * 4 + 8))) return rc;

@@ -258,7 +258,13 @@ void tile_pencils(HostTopology &t, const std::vector<double> &cellsum) {
     }
   int ax[3] = {0, 1, 2};
   std::sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+#ifdef FSALE_TORD_EXP
+  if (std::getenv("FSALE_NO_PENCIL")) return;
+  const bool swp = std::getenv("FSALE_PENCIL_SWAP") != nullptr;
+  const int sweep = swp ? ax[1] : ax[0], slab = swp ? ax[0] : ax[1];
+#else
   const int sweep = ax[0], slab = ax[1];
+#endif
   std::vector<int32_t> &ord = t.tile_order;
   ord.resize(T);
   for (int j = 0; j < T; ++j) ord[j] = j;
