@@ -17,7 +17,10 @@ every array it consumes once and writes each output once) over its average
 launch time, measured with HIP events on the context's stream in a profiled
 replay of the same K steps.  `cpu_baseline` times the CPU restatement
 (oracle/, OpenMP over the host cores) on a bounded sample of the same
-workload.
+workload.  With the default workload the line also carries
+`other_configs`: the MUSCL3D 128^3 (100 steps) and Maire 512^2 (500 steps)
+configurations measured the same way in the same run, and `serving`, the
+4096-request served stream, batched against one context per request.
 """
 import argparse
 import json
@@ -259,43 +262,14 @@ def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="muscl2d",
-                    choices=["muscl2d", "muscl3d", "maire2d", "maire3d", "serve"])
-    ap.add_argument("--requests", type=int, default=None,
-                    help="requests of the served stream (default 4096)")
-    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
-    ap.add_argument("--size", type=int, default=None, help="override mesh size (cells per side)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batched-only", action="store_true",
-                    help="served stream: skip the one-at-a-time comparison (kernel traces of the batched path)")
-    ap.add_argument("--eager", action="store_true",
-                    help="served stream: launch each step's kernels instead of replaying captured graphs")
-    ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--shape", type=int, default=None,
-                    help="blocks per CU for every kernel (0 = one block per tile); default: autotune")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                   "(domain decomposition + halo exchange), not a measured path",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    import flecsale_amd as fa
-    from flecsale_amd import inputs
-
-    if args.workload == "serve":
-        main_serve(args, fa, inputs)
-        return
-
+def measure(args, fa, inputs, workload, steps=None, size=None, with_cpu=True):
+    """One workload on one context: warm-up, the timed K steps (graph replay),
+    a profiled replay of the same steps for the per-kernel times, the CPU
+    restatement on a bounded sample.  Returns the bench line's fields."""
     t_build = time.perf_counter()
-    if args.workload == "muscl2d":
-        n = args.size or 1024
-        K = args.steps or 200
+    if workload == "muscl2d":
+        n = size or 1024
+        K = steps or 200
         prob = inputs.quadrants_2d(n, steps=K, flux=fa.FLUX_HLLC | fa.BOUNDARY_TRANSMISSIVE)
         kind, dim = "euler", 2
         config = {"workload": f"MUSCL2D four-quadrant Riemann problem, {n}x{n} quads of [0,1]^2, "
@@ -303,25 +277,25 @@ def main():
                               f"transmissive boundaries (with reflecting walls the quadrant problem "
                               f"reaches a negative state at step 6)",
                   "cells": n * n, "steps_per_run": K}
-    elif args.workload == "muscl3d":
-        n = args.size or 128
-        K = args.steps or 100
+    elif workload == "muscl3d":
+        n = size or 128
+        K = steps or 100
         prob = inputs.kelvin_helmholtz_3d(n, steps=K)
         kind, dim = "euler", 3
         config = {"workload": f"MUSCL3D Kelvin-Helmholtz, {n}^3 hexes jittered +-10%, HLLE, CFL 1/3, "
                               f"reflecting walls",
                   "cells": n ** 3, "steps_per_run": K}
-    elif args.workload == "maire3d":
-        n = args.size or 128
-        K = args.steps or 100
+    elif workload == "maire3d":
+        n = size or 128
+        K = steps or 100
         prob = inputs.maire_bubble_3d(n, steps=K)
         kind, dim = "maire", 3
         config = {"workload": f"Maire cell-centred Lagrangian, {n}^3 hexes jittered +-15%, two ideal gases "
                               f"(bubble gamma 1.67), left-driven shock, symmetry walls on six sides",
                   "cells": n ** 3, "steps_per_run": K}
     else:
-        n = args.size or 512
-        K = args.steps or 500
+        n = size or 512
+        K = steps or 500
         prob = inputs.maire_bubble_2d(n, steps=K)
         kind, dim = "maire", 2
         config = {"workload": f"Maire cell-centred Lagrangian, {n}x{n} quads jittered +-15%, two ideal "
@@ -420,7 +394,7 @@ def main():
                     kernels[fa.KERNEL_NAMES[k]]["task_bytes"] = task_bytes[k]
         dom = max(kernels, key=lambda name: kernels[name]["avg_us"] * kernels[name]["launches"])
         d = kernels[dom]
-        traffic = load_traffic(args.workload)
+        traffic = load_traffic(workload)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1), "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": round(d["GBps"] / PEAK_HBM_GBS, 4),
                     "traffic": (traffic or {}).get(dom) or (traffic or {}).get(dom + "_both")}
@@ -428,7 +402,7 @@ def main():
                                                   for k, v in kernels.items()}))
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if with_cpu and not args.no_cpu_baseline:
         log("[bench] timing the CPU restatement on a bounded sample")
         cpu = cpu_baseline(prob, kind)
         log(f"[bench] cpu baseline {cpu['value']:.4e} zone-updates/s ({cpu['sample']})")
@@ -436,6 +410,62 @@ def main():
     ctx_shape = ctx.get_launch_shape()
     lds = ctx.lds_staging() if kind == "euler" else None
     ctx.close()
+    return {
+        "value": value, "steps": K, "warmup": W, "ms_per_step": el * 1e3 / K, "config": config,
+        "alg_bytes_per_step": bytes_per_step, "alg_GBps": bytes_per_step * K / el / 1e9,
+        "alg_frac_of_peak": round(bytes_per_step * K / el / 1e9 / PEAK_HBM_GBS, 4),
+        "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels, "launch_shape": ctx_shape,
+        "lds_staging": lds,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="muscl2d",
+                    choices=["muscl2d", "muscl3d", "maire2d", "maire3d", "serve"])
+    ap.add_argument("--requests", type=int, default=None,
+                    help="requests of the served stream (default 4096)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--size", type=int, default=None, help="override mesh size (cells per side)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="served stream: skip the one-at-a-time comparison (kernel traces of the batched path)")
+    ap.add_argument("--eager", action="store_true",
+                    help="served stream: launch each step's kernels instead of replaying captured graphs")
+    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="default workload: skip the MUSCL3D 128^3 and Maire 512^2 lines measured beside it")
+    ap.add_argument("--shape", type=int, default=None,
+                    help="blocks per CU for every kernel (0 = one block per tile); default: autotune")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                                   "(domain decomposition + halo exchange), not a measured path",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import flecsale_amd as fa
+    from flecsale_amd import inputs
+
+    if args.workload == "serve":
+        main_serve(args, fa, inputs)
+        return
+
+    r = measure(args, fa, inputs, args.workload, args.steps, args.size)
+    # the other single-GPU configurations, at their own step counts, measured
+    # in the same run (not the metric's workload)
+    others = None
+    if args.workload == "muscl2d" and args.size is None and not args.no_other_configs:
+        others = {}
+        for w in ("muscl3d", "maire2d"):
+            log(f"[bench] other configuration: {w}")
+            o = measure(args, fa, inputs, w)
+            others[w] = {k: o[k] for k in ("value", "steps", "warmup", "ms_per_step", "config",
+                                           "alg_bytes_per_step", "alg_frac_of_peak", "roofline",
+                                           "cpu_baseline", "kernels", "lds_staging")}
     serving = None
     if not args.no_serve:
         n_req = args.requests or 4096
@@ -443,26 +473,27 @@ def main():
         serving = serve_stream(fa, inputs, n_req, graphs=not args.eager)
     out = {
         "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
-        "value": value,
+        "value": r["value"],
         "unit": "zone-updates/s",
         "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": el * 1e3 / K,
+        "steps": r["steps"],
+        "warmup": r["warmup"],
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (deterministic generated mesh and initial data)",
-        "config": config,
-        "alg_bytes_per_step": bytes_per_step,
-        "alg_GBps": bytes_per_step * K / el / 1e9,
-        "alg_frac_of_peak": round(bytes_per_step * K / el / 1e9 / PEAK_HBM_GBS, 4),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": kernels,
-        "launch_shape": ctx_shape,
-        "lds_staging": lds,
+        "config": r["config"],
+        "alg_bytes_per_step": r["alg_bytes_per_step"],
+        "alg_GBps": r["alg_GBps"],
+        "alg_frac_of_peak": r["alg_frac_of_peak"],
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu_baseline"],
+        "kernels": r["kernels"],
+        "launch_shape": r["launch_shape"],
+        "lds_staging": r["lds_staging"],
+        "other_configs": others,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
