@@ -213,19 +213,30 @@ __device__ __forceinline__ void lds_rec(const double2 *sh, int slot, double (&r)
 // barriers before, so the previous tile's reads are done, and after).  A
 // tile of a problem that does not step is staged too (its waves then
 // return): reads only, of cells and halo lists in bounds whoever owns them.
-template <int NP, class P>
-__device__ __forceinline__ void stage_tile(double2 *sh, const DevMesh &m, const Tile &tile, int h, P &&plane) {
+// The thread's halo entry (thread j < h stages halo cell j: kHalo2 <=
+// kBlock).  The fused kernel loads it ahead of the staging, with its other
+// reads, so that the halo's record loads do not wait on it there (the
+// gradient kernel, measured slower that way, loads it in the staging).
+static_assert(kHalo2 <= kBlock, "one halo cell per thread");
+__device__ __forceinline__ int halo_entry(const DevMesh &m, const Tile &tile, int h) {
+  const int j = threadIdx.x;
+  return j < h ? m.halo[(size_t)tile.slot * kHalo2 + j] : 0;
+}
+template <int NP, bool PRE, class P>
+__device__ __forceinline__ void stage_tile(double2 *sh, const DevMesh &m, const Tile &tile, int h, int hcell,
+                                           P &&plane) {
   __syncthreads();
   const int t = threadIdx.x;
+  if constexpr (!PRE) hcell = halo_entry(m, tile, h);
   if (t < tile.count) {
     const uint32_t off = (uint32_t)(tile.first + t) * 16u;
 #pragma unroll
     for (int i = 0; i < NP; ++i) sh[i * kLdsW + t] = ld(plane(i), off);
   }
-  for (int j = t; j < h; j += kBlock) {
-    const uint32_t off = (uint32_t)m.halo[(size_t)tile.slot * kHalo2 + j] * 16u;
+  if (t < h) {
+    const uint32_t off = (uint32_t)hcell * 16u;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) sh[i * kLdsW + kBlock + j] = ld(plane(i), off);
+    for (int i = 0; i < NP; ++i) sh[i * kLdsW + kBlock + t] = ld(plane(i), off);
   }
   __syncthreads();
 }
@@ -540,7 +551,7 @@ k_eul_grad_lds(DevMesh m, EulerFields u, Batch b) {
       if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 1>(m, u, tile.first + threadIdx.x, first_order);
       return;
     }
-    stage_tile<NP>(sh, m, tile, h, [&](int i) { return u.gs[i]; });
+    stage_tile<NP, false>(sh, m, tile, h, 0, [&](int i) { return u.gs[i]; });
     if (!FIN && !wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 2>(m, u, tile.first + threadIdx.x, first_order, sh);
   });
@@ -840,12 +851,35 @@ __global__ void __launch_bounds__(kBlock) k_eul_update(DevMesh m, EulerFields u,
   });
 }
 
+// What a 2D staged tile's cell reads from memory besides its tile's records:
+// its LDS slot words, its vertices' coordinates and its volume, loaded before
+// the tile is staged so that their round trips overlap the staging's.
+struct Pre2 {
+  uint32_t w[2];
+  double px[4], py[4], vol;
+};
+__device__ __forceinline__ void preload_2d(const DevMesh &m, const Tile &tile, Pre2 &q) {
+  const int c = tile.first + ((int)threadIdx.x < tile.count ? (int)threadIdx.x : 0);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) q.w[k] = (uint32_t)m.lfn[(size_t)k * m.cap_c + c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2 w = m.xy[m.cv[(size_t)k * m.cap_c + c]];
+    q.px[k] = w.x;
+    q.py[k] = w.y;
+  }
+  q.vol = m.cvol[c];
+}
+
 // fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
 // of one tile over the packed records: reads u.gs and u.gg, writes out.gs;
-// LDS: from the tile's records staged in sh
+// LDS: from the tile's records staged in sh, with the cell's other reads
+// preloaded in *pre (2D quads)
 template <int N, int FPC, int VPC, bool LDS>
 __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields &u, const EulerFields &out, const Batch &b,
-                                    double factor, int want_dt, int token, const Tile &tile, const double2 *sh) {
+                                    double factor, int want_dt, int token, const Tile &tile, const double2 *sh,
+                                    const Pre2 *pre = nullptr) {
+  static_assert(!LDS || (N == 2 && FPC == 4 && VPC == 4), "staged tiles are 2D quads");
   if (!wave_live(b.s, tile.prob, token)) return;
   const ProbParams &P = b.pp[tile.prob];
   const int kind = P.kind;
@@ -863,7 +897,7 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
     if constexpr (LDS) {
 #pragma unroll
       for (int k = 0; k < FPC / 2; ++k) {
-        const uint32_t w = (uint32_t)m.lfn[(size_t)k * m.cap_c + c];
+        const uint32_t w = pre->w[k];
         slot[2 * k] = (int)(w & 0x3fffu);
         slot[2 * k + 1] = (int)((w >> 16) & 0x3fffu);
         cnk[2 * k] = (int)((w >> 14) & 1u) | ((w & 0x8000u) ? 0 : 2);
@@ -879,7 +913,12 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
     else
       load_cell_pk<N>(u, c, own);
     double px[VPC], py[VPC];
-    if constexpr (N == 2) {
+    if constexpr (LDS) {
+      for (int k = 0; k < VPC; ++k) {
+        px[k] = pre->px[k];
+        py[k] = pre->py[k];
+      }
+    } else if constexpr (N == 2) {
       for (int k = 0; k < VPC; ++k) {
         const double2 w = m.xy[m.cv[(size_t)k * m.cap_c + c]];
         px[k] = w.x;
@@ -938,7 +977,11 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
         for (int i = 0; i < N + 2; ++i) du[i] -= fl[i];
       }
     }
-    const double vol = m.cvol[c];
+    double vol;
+    if constexpr (LDS)
+      vol = pre->vol;
+    else
+      vol = m.cvol[c];
     double vel[N], mass, ie;
     for (int d = 0; d < N; ++d) vel[d] = own.v[d];
     update_values<N>(own.d, vel, own.e, du, delta_t / vol, mass, ie);
@@ -990,8 +1033,14 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
   for_each_tile(b.ct, [&](const Tile &tile) {
     const int h = LDS ? m.halo_cnt[tile.slot] : -1;
     if (ALL || h >= 0) {
-      stage_tile<NP>(sh, m, tile, h, [&](int i) { return i < Rec<N>::S / 2 ? u.gs[i] : u.gg[i - Rec<N>::S / 2]; });
-      fluxupd_tile<N, FPC, VPC, LDS>(m, u, out, b, factor, want_dt, token, tile, sh);
+      if constexpr (LDS) {
+        const int hcell = halo_entry(m, tile, h);
+        Pre2 pre;
+        preload_2d(m, tile, pre);
+        stage_tile<NP, true>(sh, m, tile, h, hcell,
+                       [&](int i) { return i < Rec<N>::S / 2 ? u.gs[i] : u.gg[i - Rec<N>::S / 2]; });
+        fluxupd_tile<N, FPC, VPC, true>(m, u, out, b, factor, want_dt, token, tile, sh, &pre);
+      }
     } else {
       fluxupd_tile<N, FPC, VPC, false>(m, u, out, b, factor, want_dt, token, tile, sh);
     }
