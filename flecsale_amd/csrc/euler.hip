@@ -222,10 +222,12 @@ __device__ __forceinline__ int halo_entry(const DevMesh &m, const Tile &tile, in
   const int j = threadIdx.x;
   return j < h ? m.halo[(size_t)tile.slot * kHalo2 + j] : 0;
 }
+// `again`: the block may have staged another tile before (a persistent
+// grid), whose reads must be done first.
 template <int NP, bool PRE, class P>
 __device__ __forceinline__ void stage_tile(double2 *sh, const DevMesh &m, const Tile &tile, int h, int hcell,
-                                           P &&plane) {
-  __syncthreads();
+                                           P &&plane, bool again = true) {
+  if (again) __syncthreads();
   const int t = threadIdx.x;
   if constexpr (!PRE) hcell = halo_entry(m, tile, h);
   if (t < tile.count) {
@@ -551,7 +553,7 @@ k_eul_grad_lds(DevMesh m, EulerFields u, Batch b) {
       if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 1>(m, u, tile.first + threadIdx.x, first_order);
       return;
     }
-    stage_tile<NP, false>(sh, m, tile, h, 0, [&](int i) { return u.gs[i]; });
+    stage_tile<NP, false>(sh, m, tile, h, 0, [&](int i) { return u.gs[i]; }, b.ct.direct == nullptr);
     if (!FIN && !wave_live(b.s, tile.prob, 0)) return;
     if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 2>(m, u, tile.first + threadIdx.x, first_order, sh);
   });
@@ -1038,7 +1040,8 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
         Pre2 pre;
         preload_2d(m, tile, pre);
         stage_tile<NP, true>(sh, m, tile, h, hcell,
-                       [&](int i) { return i < Rec<N>::S / 2 ? u.gs[i] : u.gg[i - Rec<N>::S / 2]; });
+                             [&](int i) { return i < Rec<N>::S / 2 ? u.gs[i] : u.gg[i - Rec<N>::S / 2]; },
+                             b.ct.direct == nullptr);
         fluxupd_tile<N, FPC, VPC, true>(m, u, out, b, factor, want_dt, token, tile, sh, &pre);
       }
     } else {
