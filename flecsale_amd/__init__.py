@@ -386,7 +386,8 @@ class _Context:
         return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
 
     def lds_staging(self):
-        """Eulerian driver kernels' LDS staging: 2 every tile, 1 some tiles, 0 none."""
+        """Eulerian driver kernels' LDS staging: 2 every tile, 1 some tiles, 0 none
+        (3D: plus 4 when the fused step runs over bricks)."""
         on = ct.c_int32()
         _check(lib().fsale_lds_staging(self.h, ct.byref(on)))
         return int(on.value)
@@ -619,7 +620,8 @@ class Batch:
         return {KERNEL_NAMES[k]: int(out[k]) for k in range(len(KERNEL_NAMES))}
 
     def lds_staging(self):
-        """Eulerian driver kernels' LDS staging: 2 every tile, 1 some tiles, 0 none."""
+        """Eulerian driver kernels' LDS staging: 2 every tile, 1 some tiles, 0 none
+        (3D: plus 4 when the fused step runs over bricks)."""
         on = ct.c_int32()
         _check(lib().fsale_lds_staging(self.h, ct.byref(on)))
         return int(on.value)

@@ -143,6 +143,16 @@ int reset_scalars(fsale_ctx *ctx, int prob, double t, double dt) {
 
 namespace {
 
+// The 3D fused step over bricks of structured grids: on unless
+// FSALE_BRICKS=0 (the gathering kernel then runs, with the same results).
+bool brick_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("FSALE_BRICKS");
+    return !(e && !std::strcmp(e, "0"));
+  }();
+  return on;
+}
+
 // Grid of a one-block-per-tile launch over `live` tiles: a power of two (so a
 // captured graph stays valid while the batch changes within a factor of
 // two), at least 8, at most the tile capacity rounded up to 8.
@@ -195,11 +205,15 @@ int sync_tiles(fsale_ctx *ctx) {
   ctx->lds = ctx->lds_all = ctx->m.lnb != nullptr;
   ctx->s3 = false;
   ctx->s3_all = ctx->solver == FSALE_SOLVER_EULER && ctx->m.dim == 3;
+  ctx->fp3 = false;
+  ctx->brick_all = ctx->solver == FSALE_SOLVER_EULER && ctx->m.dim == 3 && ctx->m.brick && brick_enabled();
   for (int p = 0; p < (int)ctx->probs.size(); ++p)
     if (ctx->probs[p].live) {
       plist.push_back(p);
+      if (!ctx->probs[p].brick) ctx->brick_all = false;
       if (!ctx->probs[p].lds) ctx->lds = false;
       if (!ctx->probs[p].lds_all) ctx->lds_all = false;
+      if (ctx->probs[p].params.s_fp) ctx->fp3 = true;
       if (ctx->probs[p].params.s_nx)
         ctx->s3 = true;
       else
@@ -207,6 +221,7 @@ int sync_tiles(fsale_ctx *ctx) {
     }
   if (!ctx->lds) ctx->lds_all = false;
   if (!ctx->s3) ctx->s3_all = false;
+  if (plist.empty()) ctx->brick_all = false;
   const int counts[4] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size(),
                          (int)ctx->h_btiles.size()};
   if (!plist.empty())
@@ -518,6 +533,7 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
     for (int i = 0; i < 3; ++i)
       if ((rc = dev_alloc_n(ctx, &m.kfp[i], (size_t)cap_c * m.fpc))) return rc;
     if ((rc = dev_alloc_n(ctx, &m.kfa, (size_t)cap_c * m.fpc))) return rc;
+    if ((rc = dev_alloc_n(ctx, &m.brick, (size_t)cap_ct))) return rc;
   }
   if (dim == 2)
     if ((rc = dev_alloc_n(ctx, &m.xy, cap_v))) return rc;
@@ -638,6 +654,7 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   if (ctx->solver == FSALE_SOLVER_EULER) {
     P.params.s_nx = topo->s_nx;
     P.params.s_np = topo->s_np;
+    P.params.s_fp = topo->s_fp;
   }
   ctx->probs[prob] = P;
   ctx->tiles_dirty = true;
@@ -659,6 +676,8 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
   bytes += 4 * (size_t)nc;   // original ids
   const bool lds = m.lnb && !t.halo_cnt.empty();
   if (lds) bytes += 4 * (size_t)P.nct * (1 + kHalo2);
+  const bool brick = m.brick && !t.brick.empty() && (int)t.brick.size() == P.nct;
+  if (brick) bytes += 4 * (size_t)P.nct;
   if (maire) bytes += 4 * (size_t)nv * (2 + kMaxCornersPerVertex) + 2 * (size_t)nc * t.vpc + nv + 64 * t.vpc;
   if (maire && m.dim == 3) bytes += 32 * (size_t)nc + 64;
   if ((rc = st.reserve(bytes + 64 * 64))) return rc;
@@ -732,6 +751,9 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
                    [&t, c0](int, int i) { return t.halo[i] + c0; })))
       return rc;
   }
+  ctx->probs[prob].brick = brick;
+  if (brick && (rc = rows(m.brick + P.ct0, 1, 0, P.nct, [&t, c0](int, int j) { return t.brick[j] + c0; })))
+    return rc;
   if (nbr_tail) {
     if ((rc = rows(m.nbr_start + c0, 1, 0, nc, [&t, cidx, n0](int, int s) { return t.nbr_off[cidx[s]] + n0; })))
       return rc;
@@ -1263,7 +1285,8 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   std::vector<double> key = {(double)ctx->solver, (double)chunk};
   for (int k = 0; k < K_COUNT; ++k) key.push_back((double)persistent_grid(ctx, k));
   key.push_back((double)ctx->pgrid);
-  key.push_back((double)ctx->lds + 2.0 * ctx->lds_all + 4.0 * ctx->s3 + 8.0 * ctx->s3_all);
+  key.push_back((double)ctx->lds + 2.0 * ctx->lds_all + 4.0 * ctx->s3 + 8.0 * ctx->s3_all + 16.0 * ctx->fp3 +
+                32.0 * ctx->brick_all);
   GraphCache &gc = ctx->graph;
   GraphEntry *g = nullptr;
   for (auto &e : gc.entries)
@@ -1979,7 +2002,7 @@ int fsale_lds_staging(fsale_ctx *ctx, int32_t *on) {
   int rc;
   if ((rc = sync_tiles(ctx))) return rc;
   if (ctx->m.dim == 3)
-    *on = ctx->s3_all ? 2 : ctx->s3 ? 1 : 0;
+    *on = (ctx->s3_all ? 2 : ctx->s3 ? 1 : 0) + (ctx->brick_all ? 4 : 0);
   else
     *on = ctx->lds_all ? 2 : ctx->lds ? 1 : 0;
   return FSALE_OK;

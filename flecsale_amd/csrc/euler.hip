@@ -746,21 +746,28 @@ __device__ inline void reconstruct(const CellData<N> &q, const double *fx, EStat
   }
 }
 
+// numerical flux of one face times its area from the two face states (wr
+// unused on a boundary face)
+template <int N>
+__device__ __forceinline__ void face_flux_w(int kind, const EState<N> &wl, const EState<N> &wr, bool interior,
+                                            const double *n, double area, double *out) {
+  if (interior)
+    face_flux<N>(kind & 0xf, wl, wr, n, out);
+  else if ((kind >> 4) == 1)
+    e_flux<N>(wl, n, out);   // transmissive boundary
+  else
+    wall_flux<N>(wl, n, out);
+  for (int i = 0; i < N + 2; ++i) out[i] *= area;
+}
+
 // numerical flux of one face times its area (flux(f) of evaluate_fluxes)
 template <int N>
 __device__ inline void face_flux_area(int kind, const CellData<N> &q0, const CellData<N> &q1, bool interior,
                                       const double *fx, const double *n, double area, double *out) {
   EState<N> wl, wr;
   reconstruct<N>(q0, fx, wl);
-  if (interior) {
-    reconstruct<N>(q1, fx, wr);
-    face_flux<N>(kind & 0xf, wl, wr, n, out);
-  } else if ((kind >> 4) == 1) {
-    e_flux<N>(wl, n, out);   // transmissive boundary
-  } else {
-    wall_flux<N>(wl, n, out);
-  }
-  for (int i = 0; i < N + 2; ++i) out[i] *= area;
+  if (interior) reconstruct<N>(q1, fx, wr);
+  face_flux_w<N>(kind, wl, wr, interior, n, area, out);
 }
 
 // task-level evaluate_fluxes over the faces of one problem
@@ -876,8 +883,12 @@ __device__ __forceinline__ void preload_2d(const DevMesh &m, const Tile &tile, P
 // fused evaluate_fluxes + apply_update (+ dt^-1 partials of the new state)
 // of one tile over the packed records: reads u.gs and u.gg, writes out.gs;
 // LDS: from the tile's records staged in sh, with the cell's other reads
-// preloaded in *pre (2D quads)
-template <int N, int FPC, int VPC, bool LDS>
+// preloaded in *pre (2D quads).  FP (3D): a problem whose hex sides pair up
+// (ProbParams::s_fp) reads the geometry of a cell's -x, -y and -z sides from
+// the +x, +y and +z slots of the neighbours across them (the same faces):
+// each face's geometry is then read from memory once, by its lower cell, and
+// again from cache by the upper one.
+template <int N, int FPC, int VPC, bool LDS, bool FP = false>
 __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields &u, const EulerFields &out, const Batch &b,
                                     double factor, int want_dt, int token, const Tile &tile, const double2 *sh,
                                     const Pre2 *pre = nullptr) {
@@ -944,16 +955,32 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
         else
           face_geom_2d(px[k1], py[k1], px[k], py[k], fx[0], fx[1], nk[k][0], nk[k][1], ak[k]);
       } else {
-        const size_t row = (size_t)k * m.cap_c;
-        const uint32_t o = (uint32_t)c * 8u;
+        double2 w0, w1, w2;
+        if constexpr (FP) {
+          // kHexFaces sides: -x 3 / +x 1, -y 0 / +y 2, -z 4 / +z 5
+          const int up = k == 3 ? 1 : k == 0 ? 2 : k == 4 ? 5 : -1;
+          const bool pair = up >= 0 && P.s_fp && other >= 0;
+          const size_t row = (size_t)(pair ? up : k) * m.cap_c;
+          const uint32_t o = (uint32_t)(pair ? other : c) * 8u;
+          w0 = ld(m.kfp[0] + row, 2 * o);
+          w1 = ld(m.kfp[1] + row, 2 * o);
+          w2 = ld(m.kfp[2] + row, 2 * o);
+          ak[k] = ld(m.kfa + row, o);
+        } else {
+          const size_t row = (size_t)k * m.cap_c;
+          const uint32_t o = (uint32_t)c * 8u;
 #if FSALE_NT3
-        const double2 w0 = ld_nt(m.kfp[0] + row, 2 * o), w1 = ld_nt(m.kfp[1] + row, 2 * o),
-                      w2 = ld_nt(m.kfp[2] + row, 2 * o);
-        ak[k] = ld_nt(m.kfa + row, o);
+          w0 = ld_nt(m.kfp[0] + row, 2 * o);
+          w1 = ld_nt(m.kfp[1] + row, 2 * o);
+          w2 = ld_nt(m.kfp[2] + row, 2 * o);
+          ak[k] = ld_nt(m.kfa + row, o);
 #else
-        const double2 w0 = ld(m.kfp[0] + row, 2 * o), w1 = ld(m.kfp[1] + row, 2 * o), w2 = ld(m.kfp[2] + row, 2 * o);
-        ak[k] = ld(m.kfa + row, o);
+          w0 = ld(m.kfp[0] + row, 2 * o);
+          w1 = ld(m.kfp[1] + row, 2 * o);
+          w2 = ld(m.kfp[2] + row, 2 * o);
+          ak[k] = ld(m.kfa + row, o);
 #endif
+        }
         fx[0] = w0.x;
         fx[1] = w0.y;
         fx[2] = w1.x;
@@ -1026,7 +1053,7 @@ __device__ __forceinline__ void fluxupd_tile(const DevMesh &m, const EulerFields
 // and its face neighbours read there (DevMesh::lfn); a tile without one
 // gathers from memory.  The choice is per tile, uniform across the block.
 // ALL: every tile of the launch has a plan (the memory path not compiled in).
-template <int N, int FPC, int VPC, bool LDS, bool ALL = false>
+template <int N, int FPC, int VPC, bool LDS, bool ALL = false, bool FP = false>
 __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u, EulerFields out, Batch b,
                                                         double factor, int want_dt, int token) {
   static_assert(!LDS || N == 2, "LDS staging is 2D");
@@ -1045,8 +1072,256 @@ __global__ void __launch_bounds__(kBlock) k_eul_fluxupd(DevMesh m, EulerFields u
         fluxupd_tile<N, FPC, VPC, true>(m, u, out, b, factor, want_dt, token, tile, sh, &pre);
       }
     } else {
-      fluxupd_tile<N, FPC, VPC, false>(m, u, out, b, factor, want_dt, token, tile, sh);
+      fluxupd_tile<N, FPC, VPC, false, FP>(m, u, out, b, factor, want_dt, token, tile, sh);
     }
+  });
+}
+
+// ------------------------------------------- 3D structured grids in bricks
+// A problem on a structured hex grid whose sides pair up (ProbParams::s_fp)
+// with nx, ny multiples of 8 and nz of 4 runs the fused step over 8 x 8 x 4
+// bricks of cells, one per tile slot (DevMesh::brick: the brick's first
+// cell).  Each cell reconstructs its own state at its six sides from its own
+// record; the bricks' neighbours across the brick's sides (256 cells, one per
+// thread) reconstruct theirs at the side they share with the brick.  Those
+// face states meet in LDS, one direction at a time: a cell gathers one
+// record (its own) and one halo record instead of its six neighbours'.  Each
+// face state and each flux is formed by the same operations, on the same
+// values, as the gathering kernel forms it, and the fluxes are summed in the
+// cell's side order, so the results are the same bits.
+constexpr int kBrX = kBrickX, kBrY = kBrickY, kBrZ = kBrickZ;
+static_assert(kBrX == 8 && kBrY == 8 && kBrZ == 4, "the thread -> cell and halo maps assume 8 x 8 x 4");
+constexpr int kBrW = kBlock;   // LDS plane width: slots of the brick's cells, or of its 256 halo cells
+
+// a face state in LDS: 4 planes of pairs (the 8th double unused)
+__device__ __forceinline__ void put_face_state(double2 *sh, int slot, const EState<3> &w) {
+  sh[slot] = make_double2(w.d, w.v[0]);
+  sh[kBrW + slot] = make_double2(w.v[1], w.v[2]);
+  sh[2 * kBrW + slot] = make_double2(w.p, w.e);
+  sh[3 * kBrW + slot].x = w.a;
+}
+__device__ __forceinline__ void get_face_state(const double2 *sh, int slot, EState<3> &w) {
+  const double2 a = sh[slot], b = sh[kBrW + slot], c = sh[2 * kBrW + slot];
+  w.d = a.x;
+  w.v[0] = a.y;
+  w.v[1] = b.x;
+  w.v[2] = b.y;
+  w.p = c.x;
+  w.e = c.y;
+  w.a = sh[3 * kBrW + slot].x;
+}
+
+// Geometry of side k of cell c (the neighbour across it: other, or -1): a
+// -x, -y or -z side with a neighbour reads the neighbour's +x, +y, +z slot,
+// the same face (kHexFaces sides: -x 3 / +x 1, -y 0 / +y 2, -z 4 / +z 5).
+__device__ __forceinline__ void side_geom(const DevMesh &m, int k, int c, int other, double *fx, double *n,
+                                          double &area) {
+  const int up = k == 3 ? 1 : k == 0 ? 2 : k == 4 ? 5 : -1;
+  const bool pair = up >= 0 && other >= 0;
+  const size_t row = (size_t)(pair ? up : k) * m.cap_c;
+  const uint32_t o = (uint32_t)(pair ? other : c) * 8u;
+  const double2 w0 = ld(m.kfp[0] + row, 2 * o), w1 = ld(m.kfp[1] + row, 2 * o), w2 = ld(m.kfp[2] + row, 2 * o);
+  area = ld(m.kfa + row, o);
+  fx[0] = w0.x;
+  fx[1] = w0.y;
+  fx[2] = w1.x;
+  n[0] = w1.y;
+  n[1] = w2.x;
+  n[2] = w2.y;
+}
+
+// One brick.  LDS: hal (the halo cells' face states, slot = thread), slo /
+// shi (the brick's cells' states at their low / high side of the current
+// direction).  All threads of the block take part in every barrier; a brick
+// of a problem that does not step returns as a whole (the step flag and the
+// error token are the same for every wave of the launch).
+__device__ __forceinline__ void fluxupd_brick(const DevMesh &m, const EulerFields &u, const EulerFields &out,
+                                              const Batch &b, double factor, int want_dt, int token,
+                                              const Tile &tile, double2 *hal, double2 *slo, double2 *shi) {
+  constexpr int N = 3;
+  if (!wave_live(b.s, tile.prob, token)) return;
+  const ProbParams &P = b.pp[tile.prob];
+  const int kind = P.kind;
+  const int nx = P.s_nx, np = P.s_np;
+  const int base = m.brick[tile.slot];
+  // the brick's place in the grid
+  const int lb = base - P.cell_first;
+  const int bz = lb / np, by = (lb - bz * np) / nx, bx = lb - bz * np - by * nx;
+  const int ny = np / nx, nz = P.cell_count / np;
+  const int t = threadIdx.x;
+  const int ti = t & 7, tj = (t >> 3) & 7, tk = t >> 6;
+  const int c = base + ti + tj * nx + tk * np;
+  double delta_t = b.s[tile.prob].dt;
+  delta_t *= factor;
+  if (c == P.cell_first) b.s[tile.prob].t = b.s[tile.prob].t + delta_t;   // update_soln_time :554
+
+  // the halo: thread t's cell outside the brick and the side it shares with
+  // it (t < 64: below in z, < 128: above, < 160: -x, < 192: +x, < 224: -y,
+  // else +y), its face state written to hal[t]
+  {
+    int hc, gk, gc;   // halo cell, the shared face's slot: side gk of cell gc
+    bool in;
+    if (t < 128) {
+      const int i = t & 7, j = (t >> 3) & 7;
+      if (t < 64) {
+        hc = base + i + j * nx - np;
+        in = bz > 0;
+        gk = 5;
+        gc = hc;
+      } else {
+        hc = base + i + j * nx + kBrZ * np;
+        in = bz + kBrZ < nz;
+        gk = 5;
+        gc = hc - np;
+      }
+    } else if (t < 192) {
+      const int j = t & 7, k = (t >> 3) & 3;
+      if (t < 160) {
+        hc = base - 1 + j * nx + k * np;
+        in = bx > 0;
+        gk = 1;
+        gc = hc;
+      } else {
+        hc = base + kBrX + j * nx + k * np;
+        in = bx + kBrX < nx;
+        gk = 1;
+        gc = hc - 1;
+      }
+    } else {
+      const int i = t & 7, k = (t >> 3) & 3;
+      if (t < 224) {
+        hc = base + i - nx + k * np;
+        in = by > 0;
+        gk = 2;
+        gc = hc;
+      } else {
+        hc = base + i + kBrY * nx + k * np;
+        in = by + kBrY < ny;
+        gk = 2;
+        gc = hc - nx;
+      }
+    }
+    if (in) {
+      CellData<N> q;
+      load_cell_pk<N>(u, hc, q);
+      const size_t row = (size_t)gk * m.cap_c;
+      const uint32_t o = (uint32_t)gc * 16u;
+      const double2 w0 = ld(m.kfp[0] + row, o), w1 = ld(m.kfp[1] + row, o);
+      const double fx[3] = {w0.x, w0.y, w1.x};
+      EState<N> w;
+      reconstruct<N>(q, fx, w);
+      put_face_state(hal, t, w);
+    }
+  }
+
+  int cn[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) cn[k] = m.cfn[(size_t)k * m.cap_c + c];
+  CellData<N> own;
+  load_cell_pk<N>(u, c, own);
+  const double vol = m.cvol[c];
+  double du[N + 2];
+  for (int i = 0; i < N + 2; ++i) du[i] = 0;
+  double nk[6][N], ak[6], f2[N + 2];
+
+  // one direction: sides lo and hi of each cell, neighbours in the brick at
+  // thread t -/+ st while the cell's place along it (pos) is inside, else in
+  // the halo at slots hlo / hhi.  The fluxes of the two sides, in fl / fh.
+  auto pass = [&](int klo, int khi, int pos, int last, int st, int hlo, int hhi, double *fl, double *fh) {
+    double fxl[N], fxh[N];
+    const int ol = (cn[klo] >> 1) - 1, oh = (cn[khi] >> 1) - 1;
+    side_geom(m, klo, c, ol, fxl, nk[klo], ak[klo]);
+    side_geom(m, khi, c, oh, fxh, nk[khi], ak[khi]);
+    EState<N> wl, wh;
+    reconstruct<N>(own, fxl, wl);
+    reconstruct<N>(own, fxh, wh);
+    __syncthreads();   // the previous direction's reads are done
+    put_face_state(slo, t, wl);
+    put_face_state(shi, t, wh);
+    __syncthreads();
+    {
+      EState<N> wn;
+      get_face_state(pos > 0 ? shi : hal, pos > 0 ? t - st : hlo, wn);
+      if (cn[klo] & 1)
+        face_flux_w<N>(kind, wn, wl, true, nk[klo], ak[klo], fl);
+      else
+        face_flux_w<N>(kind, wl, wn, ol >= 0, nk[klo], ak[klo], fl);
+    }
+    {
+      EState<N> wn;
+      get_face_state(pos < last ? slo : hal, pos < last ? t + st : hhi, wn);
+      if (cn[khi] & 1)
+        face_flux_w<N>(kind, wn, wh, true, nk[khi], ak[khi], fh);
+      else
+        face_flux_w<N>(kind, wh, wn, oh >= 0, nk[khi], ak[khi], fh);
+    }
+  };
+  auto acc = [&](int k, const double *fl) {
+    if (cn[k] & 1) {
+      for (int i = 0; i < N + 2; ++i) du[i] += fl[i];
+    } else {
+      for (int i = 0; i < N + 2; ++i) du[i] -= fl[i];
+    }
+  };
+  {
+    // y: sides 0 (-y) and 2 (+y); side 0 is the first of the sum
+    double f0[N + 2];
+    pass(0, 2, tj, kBrY - 1, kBrX, 192 + ti + 8 * tk, 224 + ti + 8 * tk, f0, f2);
+    acc(0, f0);
+  }
+  {
+    // x: sides 3 (-x) and 1 (+x)
+    double f3[N + 2], f1[N + 2];
+    pass(3, 1, ti, kBrX - 1, 1, 128 + tj + 8 * tk, 160 + tj + 8 * tk, f3, f1);
+    acc(1, f1);
+    acc(2, f2);
+    acc(3, f3);
+  }
+  {
+    // z: sides 4 (-z) and 5 (+z)
+    double f4[N + 2], f5[N + 2];
+    pass(4, 5, tk, kBrZ - 1, kBrX * kBrY, ti + 8 * tj, 64 + ti + 8 * tj, f4, f5);
+    acc(4, f4);
+    acc(5, f5);
+  }
+
+  double vel[N], mass, ie;
+  for (int d = 0; d < N; ++d) vel[d] = own.v[d];
+  update_values<N>(own.d, vel, own.e, du, delta_t / vol, mass, ie);
+  const double a = eos_a_de(P.gamma, ie);
+  {
+    double r[Rec<N>::S];
+    r[0] = mass;
+    for (int d = 0; d < N; ++d) r[1 + d] = vel[d];
+    r[N + 1] = ie;
+    r[N + 2] = eos_p_de(P.gamma, mass, ie);
+    r[N + 3] = a;
+    for (int d = 0; d < N; ++d) r[Rec<N>::C + d] = own.cx[d];
+    const uint32_t off = (uint32_t)c * 16u;
+#pragma unroll
+    for (int i = 0; i < Rec<N>::P_STATIC; ++i) st_nt(out.gs[i], off, make_double2(r[2 * i], r[2 * i + 1]));
+  }
+  if (ie < 0 || mass < 0)   // apps/hydro/tasks.h:654-655
+    flag_negative(b.s, tile.prob, m.corig[c], token);
+  if (want_dt) {
+    double dt_inv = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double delta_x = vol / ak[k];
+      const double dti = (a + fabs(dotN<N>(vel, nk[k]))) / delta_x;
+      dt_inv = smax(dti, dt_inv);
+    }
+    dt_inv = wave_max(dt_inv);
+    if ((t & 63) == 0) u.partials[tile.slot * kWaves + (t >> 6)] = dt_inv;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+k_eul_fluxupd_brick(DevMesh m, EulerFields u, EulerFields out, Batch b, double factor, int want_dt, int token) {
+  __shared__ double2 sh[3 * 4 * kBrW];
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    __syncthreads();   // a persistent block's previous brick is done with the LDS
+    fluxupd_brick(m, u, out, b, factor, want_dt, token, tile, sh, sh + 4 * kBrW, sh + 8 * kBrW);
   });
 }
 
@@ -1116,6 +1391,12 @@ int launch_fluxupd(fsale_ctx *ctx, double factor, bool want_dt, int token, const
   if (ctx->m.dim == 2)
     return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4, false>, ctx->m, in, out, b, factor,
                              (int)want_dt, token);
+  if (ctx->brick_all)
+    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd_brick, ctx->m, in, out, b, factor, (int)want_dt,
+                             token);
+  if (ctx->fp3)
+    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8, false, false, true>, ctx->m, in, out, b,
+                             factor, (int)want_dt, token);
   return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8, false>, ctx->m, in, out, b, factor,
                            (int)want_dt, token);
 }

@@ -37,6 +37,9 @@ constexpr int kHalo2 = 80;
 // time: a tile's neighbours in the plane dz away lie in kBlock + 2 nx + 2
 // consecutive cells, so meshes up to nx = 191 cells per row are staged
 constexpr int kLds3 = 640;
+// bricks of cells of the 3D fused step on structured grids (one cell per thread)
+constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
+static_assert(kBrickX * kBrickY * kBrickZ == kBlock, "one brick cell per thread");
 
 void set_message(const std::string &msg);
 
@@ -90,6 +93,18 @@ struct HostTopology {
   // (every cell's cell-vertex neighbours are the grid's, in that numbering)
   // and is stored in that order: s_nx = nx and s_np = nx * ny, else 0
   int s_nx = 0, s_np = 0;
+  // 3D structured grid whose hex sides pair up: for every cell, its -x, -y
+  // and -z sides (local faces 3, 0, 4) are, where a neighbour lies across
+  // them, that neighbour's +x, +y and +z sides (local faces 1, 2, 5)
+  int s_fp = 0;
+  // With s_fp and nx, ny multiples of 8 and nz of 4: the fused step covers
+  // the grid in 8 x 8 x 4 bricks of cells, one per cell tile; brick[j] is
+  // the first cell of the brick of tile j.  Tiles are dispatched in
+  // tile_order; the bricks are dealt to them so that the k-th tile
+  // dispatched gets the k-th brick of a sweep in which each eighth of the
+  // dispatch list (one XCD's) covers a slab of whole brick layers along z,
+  // swept along y in rows of bricks along x and z.  Empty: no bricks.
+  std::vector<int32_t> brick;
 };
 
 int build_topology(const fsale_mesh_desc &mesh, HostTopology &t, std::string &err);
@@ -156,6 +171,9 @@ struct DevMesh {
   int32_t *halo = nullptr;       // [tile capacity * kHalo2]
   int32_t *lnb = nullptr;        // [kNbk2 / 2][cap_c]
   int32_t *lfn = nullptr;        // [2][cap_c]
+  // 3D Eulerian: the first cell of the 8 x 8 x 4 brick of cells that the
+  // fused step covers for each cell tile slot (HostTopology::brick)
+  int32_t *brick = nullptr;      // [tile capacity]
 };
 
 // A tile: `count` (<= kBlock) consecutive entities of problem `prob`
@@ -187,7 +205,8 @@ struct ProbParams {
   int s_nx, s_np;                          // 3D structured grid strides (HostTopology), 0 if none
   // Maire boundary conditions of the faces of each symmetry tag (x, y, z):
   // FSALE_BC_* kind, pressure in bc_val[t][0] or velocity in bc_val[t][:dim]
-  int bc_kind[3], bc_pad;
+  int bc_kind[3];
+  int s_fp;                                // HostTopology::s_fp: sides pair up across the grid
   double bc_val[3][3];
 };
 
@@ -311,6 +330,7 @@ struct Problem {
   bool faces_on_device = true;   // face connectivity and geometry uploaded
   bool lds = false;              // the cell tiles' LDS staging plan is uploaded
   bool lds_all = false;          // ... and every tile has one (no tile gathers from memory)
+  bool brick = false;            // the fused step runs over bricks (HostTopology::brick uploaded)
   bool have_state = false;
   bool partials_valid = false;
 };
@@ -370,6 +390,8 @@ struct fsale_ctx {
   bool lds = false;                    // every live problem has LDS staging plans (sync_tiles)
   bool lds_all = false;                // ... for every one of its tiles
   bool s3 = false, s3_all = false;     // some / every live problem is a structured 3D grid (s_nx)
+  bool fp3 = false;                    // some live problem's hex sides pair up (s_fp)
+  bool brick_all = false;              // every live problem's fused step runs over bricks
   void *stage = nullptr;               // pinned staging buffer for uploads
   size_t stage_bytes = 0;
   void *dl = nullptr;                  // pinned buffer for multi-field downloads

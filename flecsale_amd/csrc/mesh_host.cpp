@@ -196,13 +196,40 @@ void build_halo_plan(HostTopology &t) {
   }
 }
 
+// The bricks of a structured grid (HostTopology::brick), dealt to the tiles
+// in dispatch order (tile_order, or storage order).
+void brick_plan(HostTopology &t, int nx, int ny, int nz) {
+  t.brick.clear();
+  if (nx % kBrickX || ny % kBrickY || nz % kBrickZ) return;
+  const int BX = nx / kBrickX, BY = ny / kBrickY, BZ = nz / kBrickZ;
+  const int T = BX * BY * BZ;
+  if (T * kBlock != t.nc) return;
+  // sweep: XCD part (a slab of brick layers along z), then y, then z, then x
+  std::vector<int64_t> key(T);
+  std::vector<int32_t> ord(T);
+  for (int q = 0; q < T; ++q) {
+    const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
+    const int part = (int)((int64_t)bz * 8 / BZ);
+    key[q] = (((int64_t)part * BY + by) * BZ + bz) * BX + bx;
+    ord[q] = q;
+  }
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+  t.brick.assign(T, 0);
+  for (int i = 0; i < T; ++i) {
+    const int j = t.tile_order.empty() ? i : t.tile_order[i];
+    const int q = ord[i];
+    const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
+    t.brick[j] = bx * kBrickX + by * kBrickY * nx + bz * kBrickZ * nx * ny;
+  }
+}
+
 // A 3D hex mesh whose cells form a structured nx x ny x nz grid numbered x
 // fastest (HostTopology::s_nx, s_np): cell 0's neighbours name nx and nx*ny,
 // and every cell's ascending neighbour list must be the grid's (dz, dy, dx)
 // lexicographic list.  Off with FSALE_STRUCTURED=0, or when a tile's plane
 // range would not fit the staging (nx > (kLds3 - kBlock - 2) / 2).
 void detect_structured(HostTopology &t) {
-  t.s_nx = t.s_np = 0;
+  t.s_nx = t.s_np = t.s_fp = 0;
   static const bool on = [] {
     const char *e = std::getenv("FSALE_STRUCTURED");
     return !(e && !std::strcmp(e, "0"));
@@ -232,6 +259,20 @@ void detect_structured(HostTopology &t) {
   }
   t.s_nx = nx;
   t.s_np = np;
+  // the -x, -y, -z sides of a cell against the +x, +y, +z sides of the
+  // neighbour across them (local faces of kHexFaces: 3/1, 0/2, 4/5)
+  static constexpr int kLow[3] = {3, 0, 4}, kHigh[3] = {1, 2, 5};
+  const int stride[3] = {1, nx, np};
+  for (int c = 0; c < t.nc; ++c) {
+    const int at[3] = {c % nx, (c / nx) % ny, c / np};
+    for (int a = 0; a < 3; ++a) {
+      if (at[a] == 0) continue;
+      const int n = c - stride[a];
+      if (t.cf[(size_t)c * 6 + kLow[a]] != t.cf[(size_t)n * 6 + kHigh[a]]) return;
+    }
+  }
+  t.s_fp = 1;
+  brick_plan(t, nx, ny, nz);
 }
 
 // Pencil order of a 3D mesh's cell tiles (HostTopology::tile_order) from the

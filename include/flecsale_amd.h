@@ -384,7 +384,10 @@ int fsale_get_launch_shape(fsale_ctx *ctx, int32_t *blocks_per_cu);
  * kernel stages the tiles of a structured grid numbered x fastest (rows of
  * at most 191 cells) one neighbour plane at a time.  *on = 2: every
  * resident problem's tiles are staged; 1: some are; 0: none (or not an
- * Eulerian context).  No counterpart in the reference (a property of this
+ * Eulerian context).  3D, plus 4: the fused flux + update kernel runs over
+ * 8 x 8 x 4 bricks of cells whose neighbours' face states meet in LDS (every
+ * resident problem a structured grid with nx, ny multiples of 8 and nz of
+ * 4).  No counterpart in the reference (a property of this
  * implementation's kernels); results are the same either way. */
 int fsale_lds_staging(fsale_ctx *ctx, int32_t *on);
 /* Autotuner: times nsteps eager steps of the resident problems under each

@@ -190,3 +190,55 @@ def test_structured_and_renumbered_3d_grids_mix(oracle_lib):
             got = b.state(pid)
             for k in ("d", "v", "e", "p"):
                 assert np.array_equal(got[k], np.asarray(getattr(s, k)).reshape(got[k].shape)), (pid, k)
+
+
+# ------------------------------------------------ 3D fused step over bricks
+
+@pytest.mark.parametrize("shape,flux", [
+    ((8, 8, 4), 0),          # one brick: every side of it on the boundary (HLLE, walls)
+    ((16, 8, 8), 2),         # 2 x 1 x 2 bricks (HLLC)
+    ((24, 16, 12), 1),       # 3 x 2 x 3 bricks, a brick with neighbours on all six sides (Rusanov)
+    ((16, 16, 8), 0x12),     # HLLC with transmissive boundaries
+])
+def test_brick_grids_match_oracle(oracle_lib, shape, flux):
+    """Structured grids with nx, ny multiples of 8 and nz of 4 run the fused
+    flux + update kernel over 8 x 8 x 4 bricks (fsale_lds_staging 2 + 4):
+    each cell's face states meet its neighbours' in LDS, and the run equals
+    the oracle's bit for bit."""
+    print("3D bricks", shape, hex(flux), flush=True)
+    prob = _kh_box(*shape, steps=5)
+    prob["flux"] = flux
+    s, t = _oracle_run(oracle_lib, prob)
+    with fa.EulerContext(prob["coords"], prob["cells"]) as ctx:
+        assert ctx.lds_staging() == 6
+        ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+        ctx.advance(prob["cfl"], prob["final_time"], prob["max_steps"], flux_kind=prob["flux"])
+        got = ctx.state()
+        assert ctx.time()[0] == t
+    for k in ("d", "v", "e", "p", "T", "a"):
+        assert np.array_equal(got[k], np.asarray(getattr(s, k)).reshape(got[k].shape)), k
+
+
+def test_brick_and_gathering_grids_mix(oracle_lib):
+    """A batch holding a brick-shaped grid and one that is not runs the
+    gathering fused kernel for both (fsale_lds_staging 2, no bricks); alone,
+    the brick-shaped grid runs over bricks.  Every run equals the oracle."""
+    print("3D bricks: mixed batch", flush=True)
+    probs = [_kh_box(16, 8, 8, steps=4), _kh_box(13, 11, 7, steps=4)]
+    want = [_oracle_run(oracle_lib, p) for p in probs]
+    nc = sum(len(p["cells"]) for p in probs)
+    nv = sum(len(p["coords"]) for p in probs)
+    for members in ([0], [0, 1]):
+        with fa.Batch(fa.SOLVER_EULER, 3, 8, 4, nc + 512, 4 * nc + 512, nv + 512, 27 * nc + 512) as b:
+            ids = [b.add_euler(probs[i]["coords"], probs[i]["cells"], probs[i]["eos"], probs[i]["d"],
+                               probs[i]["v"], probs[i]["p"], probs[i]["cfl"], probs[i]["final_time"],
+                               probs[i]["max_steps"], flux_kind=probs[i]["flux"]) for i in members]
+            assert b.lds_staging() == (6 if len(members) == 1 else 2)
+            b.step(4)
+            st = b.status()
+            for pid, i in zip(ids, members):
+                s, t = want[i]
+                assert st[pid].steps == 4 and st[pid].time == t
+                got = b.state(pid)
+                for k in ("d", "v", "e", "p"):
+                    assert np.array_equal(got[k], np.asarray(getattr(s, k)).reshape(got[k].shape)), (i, k)
