@@ -102,8 +102,8 @@ struct HostTopology {
   // the first cell of the brick of tile j.  Tiles are dispatched in
   // tile_order; the bricks are dealt to them so that the k-th tile
   // dispatched gets the k-th brick of a sweep in which each eighth of the
-  // dispatch list (one XCD's) covers a slab of whole brick layers along z,
-  // swept along y in rows of bricks along x and z.  Empty: no bricks.
+  // dispatch list (one XCD's) covers a slab of whole brick rows along y,
+  // swept along z in layers of brick rows along x.  Empty: no bricks.
   std::vector<int32_t> brick;
 };
 

@@ -204,13 +204,13 @@ void brick_plan(HostTopology &t, int nx, int ny, int nz) {
   const int BX = nx / kBrickX, BY = ny / kBrickY, BZ = nz / kBrickZ;
   const int T = BX * BY * BZ;
   if (T * kBlock != t.nc) return;
-  // sweep: XCD part (a slab of brick layers along z), then y, then z, then x
+  // sweep: XCD part (a slab of brick rows along y), then z, then y, then x
   std::vector<int64_t> key(T);
   std::vector<int32_t> ord(T);
   for (int q = 0; q < T; ++q) {
     const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
-    const int part = (int)((int64_t)bz * 8 / BZ);
-    key[q] = (((int64_t)part * BY + by) * BZ + bz) * BX + bx;
+    const int part = (int)((int64_t)by * 8 / BY);
+    key[q] = (((int64_t)part * BZ + bz) * BY + by) * BX + bx;
     ord[q] = q;
   }
   std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
