@@ -249,18 +249,30 @@ struct S3Tile {
   int first, count, cf, nc, nx, np;
 };
 
+// The staging of an 8 x 8 x 4 brick of a structured grid (ProbParams::
+// s_brick) for the gradients: the brick and the ring of cells around it,
+// 10 x 10 x 6 slots (x fastest; slots off the grid are not written).  For
+// one cell: its slot, its place in the grid and the grid's extent, and
+// whether the brick lies inside the grid with its whole ring.
+constexpr int kB4 = (kBrickX + 2) * (kBrickY + 2) * (kBrickZ + 2);
+struct B4Cell {
+  int slot, ix, jy, kz, nx, ny, nz, interior;
+};
+
 // MODE 0: the per-field arrays (task level); 1: the packed state records
 // (gs), as the driver loop does; 2 (2D): the records staged in LDS (sh),
 // neighbours by their LDS slots (DevMesh::lnb); 3 (3D structured grids):
 // the records of each neighbour plane staged in LDS in turn (sh, a
 // contiguous range of cells per plane; every thread of the block takes part
-// in the staging, `valid` lanes compute and store).  Modes 1-3 write the
-// packed gradients (gg).
+// in the staging, `valid` lanes compute and store); 4 (3D grids in bricks):
+// the brick and its ring staged in LDS (sh, kB4 slots per plane, b4).
+// Modes 1-4 write the packed gradients (gg).
 template <int N, int VPC, int MODE>
 __device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u, int c0, int first_order,
-                                 double2 *sh = nullptr, const S3Tile &s3t = S3Tile{}, bool valid = true) {
+                                 double2 *sh = nullptr, const S3Tile &s3t = S3Tile{}, bool valid = true,
+                                 const B4Cell &b4 = B4Cell{}) {
   static_assert(MODE != 2 || N == 2, "LDS staging is 2D");
-  static_assert(MODE != 3 || N == 3, "plane staging is 3D");
+  static_assert((MODE != 3 && MODE != 4) || N == 3, "plane and brick staging are 3D");
   constexpr bool PK = MODE != 0;
   constexpr int NV = N + 4;
   const double tol = 1.e-9;
@@ -270,6 +282,8 @@ __device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u
     double r[Rec<N>::S];
     if constexpr (MODE == 2)
       lds_rec(sh, (int)threadIdx.x, r);
+    else if constexpr (MODE == 4)
+      lds_rec<Rec<N>::S, kB4>(sh, b4.slot, r);
     else
       ld_rec(u.gs, c0, r);
     for (int j = 0; j < NV; ++j) u0[j] = r[j];
@@ -313,7 +327,7 @@ __device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u
 #pragma unroll
     for (int k = 0; k < NBK; ++k) nb[k] = ld(m.nbk + k * row, i0);
   }
-  const int cnt = (MODE == 2 || MODE == 3) ? 0 : ld(m.nbr_cnt, i0);
+  const int cnt = (MODE >= 2) ? 0 : ld(m.nbr_cnt, i0);
   double xv[VPC][N];
   auto load_vertex = [&](int lv) {
     const uint32_t iv = (uint32_t)ld(m.cv + lv * row, i0);
@@ -335,6 +349,8 @@ __device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u
         lds_rec(sh, c1, r);
       else if constexpr (MODE == 3)
         lds_rec<Rec<N>::S, kLds3>(sh, c1, r);
+      else if constexpr (MODE == 4)
+        lds_rec<Rec<N>::S, kB4>(sh, c1, r);
       else
         ld_rec(u.gs, c1, r);
       for (int j = 0; j < NV; ++j) u1[j] = r[j];
@@ -398,6 +414,36 @@ __device__ __forceinline__ void grad_cell(const DevMesh &m, const EulerFields &u
           for (int dx = -1; dx <= 1; ++dx) {
             if ((dz == 0 && dy == 0 && dx == 0) || ix + dx < 0 || ix + dx >= s3t.nx) continue;
             neighbour(c0 + dz * s3t.np + dy * s3t.nx + dx - lo);
+          }
+        }
+      }
+    }
+  } else if constexpr (MODE == 4) {
+    // the grid's neighbours in ascending id order, (dz, dy, dx)
+    // lexicographic, at slot offsets dz (10 x 10) + dy 10 + dx; a brick with
+    // its whole ring inside the grid needs no test (and its loop no branch),
+    // otherwise a neighbour off the grid is skipped, as the neighbour list
+    // leaves it out
+    constexpr int SX = 1, SY = kBrickX + 2, SZ = (kBrickX + 2) * (kBrickY + 2);
+    if (b4.interior) {
+#pragma unroll
+      for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx)
+            if (dz != 0 || dy != 0 || dx != 0) neighbour(b4.slot + dz * SZ + dy * SY + dx * SX);
+    } else {
+#pragma unroll
+      for (int dz = -1; dz <= 1; ++dz) {
+        if (b4.kz + dz < 0 || b4.kz + dz >= b4.nz) continue;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+          if (b4.jy + dy < 0 || b4.jy + dy >= b4.ny) continue;
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx) {
+            if ((dz == 0 && dy == 0 && dx == 0) || b4.ix + dx < 0 || b4.ix + dx >= b4.nx) continue;
+            neighbour(b4.slot + dz * SZ + dy * SY + dx * SX);
           }
         }
       }
@@ -584,6 +630,43 @@ k_eul_grad_s3(DevMesh m, EulerFields u, Batch b) {
     const bool valid = (int)threadIdx.x < tile.count;
     const S3Tile s3t{tile.first, tile.count, P.cell_first, P.cell_count, P.s_nx, P.s_np};
     grad_cell<3, 8, 3>(m, u, valid ? tile.first + (int)threadIdx.x : tile.first, first_order, sh, s3t, valid);
+  });
+}
+
+// 3D grids in bricks (every live problem with ProbParams::s_brick; tile
+// slot -> brick as the fused step: DevMesh::brick): the block stages its
+// brick and the ring around it once and takes every neighbour from LDS.
+// FIN and the liveness rules as k_eul_grad_s3.
+template <bool FIN = false>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_eul_grad_brick(DevMesh m, EulerFields u, Batch b) {
+  constexpr int NP = Rec<3>::S / 2;
+  constexpr int RX = kBrickX + 2, RY = kBrickY + 2;
+  __shared__ double2 sh[NP * kB4];
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    const ProbParams &P = b.pp[tile.prob];
+    if (FIN && tile.first == P.cell_first) finalize_problem<kBlock>(b, u.partials, 1, tile.prob);
+    if (FIN && !b.s[tile.prob].packed) return;
+    if (!FIN && !wave_live(b.s, tile.prob, 0)) return;
+    const int nx = P.s_nx, np = P.s_np, ny = np / nx, nz = P.cell_count / np;
+    const int base = m.brick[tile.slot];
+    const int lb = base - P.cell_first;
+    const int z0 = lb / np, y0 = (lb - z0 * np) / nx, x0 = lb - z0 * np - y0 * nx;
+    const bool interior = x0 > 0 && x0 + kBrickX < nx && y0 > 0 && y0 + kBrickY < ny && z0 > 0 && z0 + kBrickZ < nz;
+    __syncthreads();   // a persistent block's previous brick is done with the LDS
+    for (int q = threadIdx.x; q < kB4; q += kBlock) {
+      const int qi = q % RX, qj = (q / RX) % RY, qk = q / (RX * RY);
+      const int gx = x0 + qi - 1, gy = y0 + qj - 1, gz = z0 + qk - 1;
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+        const uint32_t off = (uint32_t)(base + (qi - 1) + (qj - 1) * nx + (qk - 1) * np) * 16u;
+#pragma unroll
+        for (int pl = 0; pl < NP; ++pl) sh[pl * kB4 + q] = ld(u.gs[pl], off);
+      }
+    }
+    __syncthreads();
+    const int t = threadIdx.x, ti = t % kBrickX, tj = (t / kBrickX) % kBrickY, tk = t / (kBrickX * kBrickY);
+    const B4Cell b4{((tk + 1) * RY + tj + 1) * RX + ti + 1, x0 + ti, y0 + tj, z0 + tk, nx, ny, nz, interior};
+    grad_cell<3, 8, 4>(m, u, base + ti + tj * nx + tk * np, P.first_order, sh, S3Tile{}, true, b4);
   });
 }
 
@@ -1373,6 +1456,7 @@ int launch_grad_pk(fsale_ctx *ctx, const EulerFields &u) {
   if (ctx->m.dim == 2 && ctx->lds)
     return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, false, FIN>, ctx->m, u, b);
   if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, 1, FIN>, ctx->m, u, b);
+  if (ctx->brick_all) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_brick<FIN>, ctx->m, u, b);
   if (ctx->s3_all) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_s3<true, FIN>, ctx->m, u, b);
   if (ctx->s3) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_s3<false, FIN>, ctx->m, u, b);
   return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 1, FIN>, ctx->m, u, b);
