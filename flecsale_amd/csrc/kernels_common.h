@@ -419,10 +419,19 @@ __host__ __device__ inline void qr_solve(int rows, double *A, double *B) {
 // same order, and jpvt only maps the solution back (B[jpvt[i]]).
 template <int NR>
 __host__ __device__ inline double perm_get(const double (&x)[NR], int idx) {
-  double r = x[0];
+  // x[idx] as the OR of each element's bits under an all-ones or all-zeros
+  // mask: the select chain this replaces was turned back into an indexed
+  // load from a stack copy of x (scratch memory, one round trip per read)
+  unsigned long long r = 0;
 #pragma unroll
-  for (int q = 1; q < NR; ++q) r = (idx == q) ? x[q] : r;
-  return r;
+  for (int q = 0; q < NR; ++q) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &x[q], sizeof b);
+    r |= b & (0ull - (unsigned long long)(idx == q));
+  }
+  double d;
+  __builtin_memcpy(&d, &r, sizeof d);
+  return d;
 }
 
 template <int NR>

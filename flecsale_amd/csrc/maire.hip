@@ -126,6 +126,9 @@ __device__ inline void corner_matrix(const DevMesh &m, const MaireFields &u, int
 // system [M_p N^T; N 0] is solved by flecsale::linalg::qr (:383-418) --
 // Cramer's rule when no wedge carries a symmetry tag.
 constexpr int kNodalKeep = 4;
+// boundary vertices of quad meshes have one or two corners: the boundary
+// path keeps two (its registers then fit beside the interior path's)
+constexpr int kNodalKeepBnd = 2;
 
 // the first tag (ascending) of tag mask `vtags` whose condition prescribes
 // the velocity, or -1
@@ -137,21 +140,23 @@ __device__ inline int velocity_tag(const ProbParams &P, int vtags, int dim) {
 
 // keep: wedge normals, facet areas and tags of the first kNodalKeep corners,
 // already loaded by the caller
+template <int K>
 struct KeptWedges {
-  double n[kNodalKeep][2][2], l[kNodalKeep][2];
-  int tags[kNodalKeep];
+  double n[K][2][2], l[K][2];
+  int tags[K];
 };
 
 // returns false for a prescribed velocity (the vertex's corners keep
 // F_pc = M_pc u_c + p_c (l n)_pc)
+template <int K>
 __device__ inline bool boundary_solve(const DevMesh &m, const ProbParams &P, int vt, int cnt, const double Mp[2][2],
-                                      double rhs[2], const KeptWedges &kw, double un[2]) {
+                                      double rhs[2], const KeptWedges<K> &kw, double un[2]) {
   {
     int vtags = 0;
 #pragma unroll
-    for (int q = 0; q < kNodalKeep; ++q)
+    for (int q = 0; q < K; ++q)
       if (q < cnt) vtags |= kw.tags[q];
-    for (int q = kNodalKeep; q < cnt; ++q) {
+    for (int q = K; q < cnt; ++q) {
       const int cn = m.vck[(size_t)q * m.cap_v + vt];
       vtags |= m.cwtag[(size_t)(cn & 3) * m.cap_c + (cn >> 2)];
     }
@@ -189,9 +194,9 @@ __device__ inline bool boundary_solve(const DevMesh &m, const ProbParams &P, int
     }
   };
 #pragma unroll
-  for (int q = 0; q < kNodalKeep; ++q)
+  for (int q = 0; q < K; ++q)
     if (q < cnt && (kw.tags[q] & 0x44)) add_wedges(kw.tags[q], kw.n[q], kw.l[q]);
-  for (int q = kNodalKeep; q < cnt; ++q) {
+  for (int q = K; q < cnt; ++q) {
     const int cn = m.vck[(size_t)q * m.cap_v + vt];
     const int c = cn >> 2, k = cn & 3;
     const int tags = m.cwtag[(size_t)k * m.cap_c + c];
@@ -706,40 +711,71 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
     if (!step_live<PEND>(b, tile.prob)) return;
     if ((int)threadIdx.x >= tile.count) return;
     int vt = tile.first + threadIdx.x;
-    if constexpr (BOUNDARY)
+    // interior: a boundary vertex (the boundary path's) computes like the
+    // others with a corner count of 0, so that nothing it stores and no
+    // gather waits on the boundary flag (a branch on it would)
+    int cn0[BOUNDARY ? 1 : kNodalKeep];
+    bool own = true;
+    if constexpr (BOUNDARY) {
       vt = m.bv[vt];
-    else if (m.vbnd[vt])
-      return;
-    if constexpr (SAVE) {
+    } else {
+#pragma unroll
+      for (int q = 0; q < kNodalKeep; ++q) cn0[q] = m.vck[(size_t)q * m.cap_v + vt];
+      own = !m.vbnd[vt];
+    }
+    if (SAVE && own) {
       const double2 w = m.xy[vt];
       u.x0p[vt] = w;
       u.x0[0][vt] = w.x;
       u.x0[1][vt] = w.y;
     }
-    const int cnt = m.vc_cnt[vt];
-    double M[kNodalKeep][2][2], np[kNodalKeep][2], F[kNodalKeep][2];
-    int cc[kNodalKeep], kk[kNodalKeep];
-    KeptWedges kw;
+    const int cnt = m.vc_cnt[vt] & -(int)own;
+    constexpr int KEEP = BOUNDARY ? kNodalKeepBnd : kNodalKeep;
+    double M[KEEP][2][2], np[KEEP][2], F[KEEP][2];
+    int cc[KEEP], kk[KEEP];
+    KeptWedges<KEEP> kw;
+    if constexpr (BOUNDARY) {
 #pragma unroll
-    for (int q = 0; q < kNodalKeep; ++q) {
-      const int cn = q < cnt ? m.vck[(size_t)q * m.cap_v + vt] : -1;
-      cc[q] = cn >> 2;
-      kk[q] = cn & 3;
-      if (cn >= 0) {
-        if constexpr (BOUNDARY) {
+      for (int q = 0; q < KEEP; ++q) {
+        const int cn = q < cnt ? m.vck[(size_t)q * m.cap_v + vt] : -1;
+        cc[q] = cn >> 2;
+        kk[q] = cn & 3;
+        if (cn >= 0) {
           wedge_facets(m, cc[q], kk[q], kw.n[q], kw.l[q]);
           kw.tags[q] = m.cwtag[(size_t)kk[q] * m.cap_c + cc[q]];
           corner_matrix_from(u, cc[q], kw.n[q], kw.l[q], M[q], np[q]);
-        } else {
-          corner_matrix(m, u, cc[q], kk[q], M[q], np[q]);
+          corner_force(u, cc[q], M[q], np[q], F[q]);
+          // n_pc is final here: stored now, its registers free for the solve
+          const size_t ci = (size_t)kk[q] * m.cap_c + cc[q];
+          u.npc[0][ci] = np[q][0];
+          u.npc[1][ci] = np[q][1];
         }
+      }
+    } else {
+      // every kept corner's gathers without a branch, so that the four
+      // corners' chains (corner -> cell vertices -> coordinates) overlap: a
+      // missing corner (-1 past the vertex's count) computes the first
+      // corner again and is not used
+#pragma unroll
+      for (int q = 0; q < KEEP; ++q) {
+        const int cn = cn0[q] >= 0 ? cn0[q] : cn0[0];
+        cc[q] = cn >> 2;
+        kk[q] = cn & 3;
+        corner_matrix(m, u, cc[q], kk[q], M[q], np[q]);
         corner_force(u, cc[q], M[q], np[q], F[q]);
       }
+#pragma unroll
+      for (int q = 0; q < KEEP; ++q)
+        if (q < cnt) {
+          const size_t ci = (size_t)kk[q] * m.cap_c + cc[q];
+          u.npc[0][ci] = np[q][0];
+          u.npc[1][ci] = np[q][1];
+        }
     }
     // M_p = sum_c M_pc and rhs = sum_c F_pc in corner order
     double Mp[2][2] = {{0, 0}, {0, 0}}, rhs[2] = {0, 0};
 #pragma unroll
-    for (int q = 0; q < kNodalKeep; ++q) {
+    for (int q = 0; q < KEEP; ++q) {
       if (q >= cnt) break;
       for (int r = 0; r < 2; ++r) {
         Mp[r][0] += M[q][r][0];
@@ -747,7 +783,7 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
         rhs[r] += F[q][r];
       }
     }
-    for (int q = kNodalKeep; q < cnt; ++q) {
+    for (int q = KEEP; q < cnt; ++q) {
       const int cn = m.vck[(size_t)q * m.cap_v + vt];
       double Mq[2][2], npq[2], Fq[2];
       corner_matrix(m, u, cn >> 2, cn & 3, Mq, npq);
@@ -768,13 +804,13 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       un[0] = (rhs[0] * Mp[1][1] - Mp[0][1] * rhs[1]) / det;
       un[1] = (Mp[0][0] * rhs[1] - rhs[0] * Mp[1][0]) / det;
     }
-    u.un[0][vt] = un[0];
-    u.un[1][vt] = un[1];
+    if (own) {
+      u.un[0][vt] = un[0];
+      u.un[1][vt] = un[1];
+    }
     // F_pc -= M_pc u_p  (matrix_vector(-1, Mpc, un, 1, Fpc), tasks.h:435-448)
-    auto finish = [&](int c, int k, const double Mq[2][2], const double npq[2], const double Fq[2]) {
+    auto finish = [&](int c, int k, const double Mq[2][2], const double Fq[2]) {
       const size_t ci = (size_t)k * m.cap_c + c;
-      u.npc[0][ci] = npq[0];
-      u.npc[1][ci] = npq[1];
       for (int r = 0; r < 2; ++r) {
         double t = Mq[r][0] * un[0];
         t += Mq[r][1] * un[1];
@@ -782,14 +818,17 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       }
     };
 #pragma unroll
-    for (int q = 0; q < kNodalKeep; ++q)
-      if (q < cnt) finish(cc[q], kk[q], M[q], np[q], F[q]);
-    for (int q = kNodalKeep; q < cnt; ++q) {
+    for (int q = 0; q < KEEP; ++q)
+      if (q < cnt) finish(cc[q], kk[q], M[q], F[q]);
+    for (int q = KEEP; q < cnt; ++q) {
       const int cn = m.vck[(size_t)q * m.cap_v + vt];
       double Mq[2][2], npq[2], Fq[2];
       corner_matrix(m, u, cn >> 2, cn & 3, Mq, npq);
       corner_force(u, cn >> 2, Mq, npq, Fq);
-      finish(cn >> 2, cn & 3, Mq, npq, Fq);
+      const size_t ci = (size_t)(cn & 3) * m.cap_c + (cn >> 2);
+      u.npc[0][ci] = npq[0];
+      u.npc[1][ci] = npq[1];
+      finish(cn >> 2, cn & 3, Mq, Fq);
     }
   }
 }
@@ -812,8 +851,11 @@ __global__ void __launch_bounds__(kBlock) k_maire3_nodal_bnd_big(DevMesh m, Mair
 // run while the interior waves fill the machine around them instead of
 // forming a launch of their own.  Held at 3 waves per SIMD for the interior
 // tiles (the boundary path then keeps part of its state in scratch).
+#ifndef FSALE_NODAL_WAVES
+#define FSALE_NODAL_WAVES 3
+#endif
 template <int N, bool SAVE = false, bool PEND = false>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FSALE_NODAL_WAVES)))
 k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
   if ((int)blockIdx.x < nb) {
     if ((int)blockIdx.x < b.bt.ndirect) {
