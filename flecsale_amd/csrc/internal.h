@@ -221,6 +221,14 @@ struct DevScalars {
   long long steps;      // completed steps
   int err_token;        // launch that raised err_code (0: none)
   int packed;           // Eulerian: steps in the current batch of steps (its state was packed)
+  // Lagrangian 2D driver step: the step as its predictor update sees it
+  // (written by the predictor's residual launch: the previous step closed,
+  // and the dt it closed with), so that the update launch can finalize the
+  // time step and close the previous step itself without reading what it
+  // writes (k_maire_move_update FOLD)
+  double snap_t, snap_dt;
+  long long snap_steps;
+  int snap_active, snap_pad;
 };
 
 struct EulerFields {

@@ -873,11 +873,33 @@ k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
 
 // DT: also the time step's per-wave partials of the new residual (the
 // driver step's predictor, whose time step follows), as k_maire_dt_partial
-// computes them.
-template <int N, bool DT, bool PEND = false>
+// computes them.  TILE (2D driver step, with DT): one partial pair per cell
+// tile, at 2 slot (a block-wide max), and the block of each problem's first
+// tile records the step as its predictor update will see it (DevScalars::
+// snap_*: with PEND the previous step closed as close_step closes it).
+template <int N, bool DT, bool PEND = false, bool TILE = false>
 __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireFields u, Batch b) {
   constexpr int V = N == 2 ? 4 : 8;
+  static_assert(!TILE || DT, "tile partials are time-step partials");
   for_each_tile(b.ct, [&](const Tile &tile) {
+    if (TILE && threadIdx.x == 0 && tile.first == b.pp[tile.prob].cell_first) {
+      DevScalars &S = b.s[tile.prob];
+      const ProbParams &P = b.pp[tile.prob];
+      double t = S.t;
+      long long steps = S.steps;
+      int active = S.active;
+      if (PEND) {
+        if (S.active && !S.err_code) {
+          t += S.dt;
+          steps = steps + 1;
+        }
+        active = (S.err_code == 0) && (steps < P.max_steps) && (t < P.final_time);
+      }
+      S.snap_t = t;
+      S.snap_dt = S.dt;
+      S.snap_steps = steps;
+      S.snap_active = active;
+    }
     if (!step_live<PEND>(b, tile.prob)) return;
     double acc = 0, vol = 0;
     if ((int)threadIdx.x < tile.count) {
@@ -914,7 +936,23 @@ __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireField
     if (DT) {
       acc = wave_max(acc);
       vol = wave_max(vol);
-      if ((threadIdx.x & 63) == 0) {
+      if (TILE) {
+        __shared__ double red[2][kWaves];
+        if ((threadIdx.x & 63) == 0) {
+          red[0][threadIdx.x >> 6] = acc;
+          red[1][threadIdx.x >> 6] = vol;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          for (int w = 1; w < kWaves; ++w) {
+            acc = smax(red[0][w], acc);
+            vol = smax(red[1][w], vol);
+          }
+          u.partials[2 * (size_t)tile.slot] = acc;
+          u.partials[2 * (size_t)tile.slot + 1] = vol;
+        }
+        __syncthreads();   // red is free again (a persistent block's next tile)
+      } else if ((threadIdx.x & 63) == 0) {
         const size_t i = 2 * ((size_t)tile.slot * kWaves + (threadIdx.x >> 6));
         u.partials[i] = acc;
         u.partials[i + 1] = vol;
@@ -1028,15 +1066,26 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
 // step's start, x0 + dt u as k_maire_move computes them, instead of read
 // back after a move launch (k_maire_move_update moves them in the same
 // launch).
-template <int MODE, int N, bool MOVED = false>
+// FS (a 2D driver step's predictor, FoldedStep): the step's flag and time
+// step as the launch computed them, instead of the scalars it is writing.
+struct FoldedStep {
+  int active;
+  double dt;
+};
+template <int MODE, int N, bool MOVED = false, bool FS = false>
 __device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields &u, const Batch &b, double factor,
-                                            int token, const Tile &tile) {
+                                            int token, const Tile &tile, const FoldedStep &fs = FoldedStep{}) {
   {
-    if (!wave_live(b.s, tile.prob, token)) return;
+    if constexpr (FS) {
+      const int et = b.s[tile.prob].err_token;
+      if (!fs.active || !(et == 0 || et == token)) return;
+    } else {
+      if (!wave_live(b.s, tile.prob, token)) return;
+    }
     if ((int)threadIdx.x >= tile.count) return;
     const ProbParams &P = b.pp[tile.prob];
     const int c = tile.first + threadIdx.x;
-    const double delta_t = factor * b.s[tile.prob].dt;
+    const double delta_t = factor * (FS ? fs.dt : b.s[tile.prob].dt);
     double vol;
     if (delta_t > 1e-14) {
       // burton update_geometry() for the moved cell
@@ -1128,13 +1177,82 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
 // another.  MODE 1: the predictor, 2: the corrector.  The vertex tiles use
 // the launch's token, as the update does: an error this launch raises does
 // not stop its moves (a separate move launch would have completed first).
-template <int MODE>
+// FOLD (the predictor): the launch also does what k_maire_close_dt /
+// k_maire_dt_finalize would do before it.  Every block finalizes the time
+// step of its tiles' problems itself from the per-tile partials and the
+// step recorded by the residual launch (DevScalars::snap_*, k_maire_residual
+// TILE), which no block of this launch writes; the block of each problem's
+// first cell tile writes the closed step and the new dt back to the
+// problem's scalars, which no block of this launch reads.
+__device__ inline FoldedStep folded_step(const Batch &b, const double *tp, int prob) {
+  const DevScalars &S = b.s[prob];
+  const ProbParams &P = b.pp[prob];
+  FoldedStep fs{S.snap_active, 0.0};
+  // max over the problem's tile partials (block-wide; max is exact, so the
+  // order does not matter), four loads in flight per thread
+  __shared__ double red[2][kWaves];
+  const double2 *pp = reinterpret_cast<const double2 *>(tp) + P.tile_first;
+  double2 a4[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  int i = threadIdx.x;
+  for (; i + 3 * kBlock < P.tile_count; i += 4 * kBlock) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double2 w = pp[i + k * kBlock];
+      a4[k].x = smax(w.x, a4[k].x);
+      a4[k].y = smax(w.y, a4[k].y);
+    }
+  }
+  for (; i < P.tile_count; i += kBlock) {
+    const double2 w = pp[i];
+    a4[0].x = smax(w.x, a4[0].x);
+    a4[0].y = smax(w.y, a4[0].y);
+  }
+  double mx[2] = {smax(smax(a4[0].x, a4[1].x), smax(a4[2].x, a4[3].x)),
+                  smax(smax(a4[0].y, a4[1].y), smax(a4[2].y, a4[3].y))};
+  __syncthreads();   // red is free (the block's previous call)
+  for (int j = 0; j < 2; ++j) {
+    mx[j] = wave_max(mx[j]);
+    if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = mx[j];
+  }
+  __syncthreads();
+  for (int j = 0; j < 2; ++j) {
+    mx[j] = red[j][0];
+    for (int w = 1; w < kWaves; ++w) mx[j] = smax(red[j][w], mx[j]);
+  }
+  // evaluate_time_step :179-188 and the driver's cap (driver.h:391), as dt_finalize
+  const double dts[3] = {P.cfl_acc / mx[0], P.cfl_vol / mx[1], P.cfl_growth * S.snap_dt};
+  int it = 0;
+  for (int i = 1; i < 3; ++i)
+    if (dts[i] < dts[it]) it = i;
+  fs.dt = smin(dts[it], P.final_time - S.snap_t);
+  return fs;
+}
+
+template <int MODE, bool FOLD = false>
 __global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFields u, Batch b, double factor,
                                                               int token) {
+  // FOLD: the block's last problem and its folded step (a block's vertex
+  // and cell tiles are mostly of one problem: one reduction for both)
+  int fprob = -1;
+  FoldedStep fcache{0, 0.0};
+  auto folded = [&](int prob) {
+    if (prob != fprob) {
+      fcache = folded_step(b, u.partials, prob);
+      fprob = prob;
+    }
+    return fcache;
+  };
   for_each_tile(b.vt, [&](const Tile &tile) {
-    if (!wave_live(b.s, tile.prob, token)) return;
+    FoldedStep fs{0, 0.0};
+    if constexpr (FOLD) {
+      fs = folded(tile.prob);
+      const int et = b.s[tile.prob].err_token;
+      if (!fs.active || !(et == 0 || et == token)) return;
+    } else {
+      if (!wave_live(b.s, tile.prob, token)) return;
+    }
     if ((int)threadIdx.x >= tile.count) return;
-    const double delta_t = factor * b.s[tile.prob].dt;
+    const double delta_t = factor * (FOLD ? fs.dt : b.s[tile.prob].dt);
     if (!(delta_t > 1e-14)) return;   // do_step, flecsale::config::test_tolerance
     const int vt = tile.first + threadIdx.x;
     const double2 w = u.x0p[vt];
@@ -1143,7 +1261,21 @@ __global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFi
     m.x[1][vt] = y;
     put_xy(m, vt, x, y);
   });
-  for_each_tile(b.ct, [&](const Tile &tile) { update_tile<MODE, 2, true>(m, u, b, factor, token, tile); });
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if constexpr (FOLD) {
+      const FoldedStep fs = folded(tile.prob);
+      if (threadIdx.x == 0 && tile.first == b.pp[tile.prob].cell_first) {
+        DevScalars &S = b.s[tile.prob];
+        S.t = S.snap_t;
+        S.steps = S.snap_steps;
+        S.active = S.snap_active;
+        if (fs.active) S.dt = fs.dt;
+      }
+      update_tile<MODE, 2, true, true>(m, u, b, factor, token, tile, fs);
+    } else {
+      update_tile<MODE, 2, true>(m, u, b, factor, token, tile);
+    }
+  });
 }
 
 // save (dir 0) / restore (dir 1) of coordinates, velocity and energy
@@ -1347,12 +1479,19 @@ int maire_step(fsale_ctx *ctx, bool first, bool last) {
     if ((rc = first ? launch_nodal_n<2, true, false>(ctx) : launch_nodal_n<2, true, true>(ctx))) return rc;
     {
       const Batch b = batch_args(ctx, K_MAIRE_RESIDUAL);
-      rc = first ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, true, false>, ctx->m, ctx->ma, b)
-                 : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, true, true>, ctx->m, ctx->ma, b);
+      rc = first ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, true, false, true>, ctx->m, ctx->ma,
+                                     b)
+                 : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, true, true, true>, ctx->m, ctx->ma,
+                                     b);
       if (rc) return rc;
     }
-    if ((rc = launch_dt_finalize(ctx, !first))) return rc;
-    if ((rc = launch_move_update(ctx, 0.5, 1, 1))) return rc;
+    {
+      // the time step's finalize (and the previous step's close) ride in
+      // the predictor's update
+      const Batch b = batch_args(ctx, K_MAIRE_UPDATE);
+      if ((rc = launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_move_update<1, true>, ctx->m, ctx->ma, b, 0.5, 1)))
+        return rc;
+    }
     if ((rc = launch_nodal(ctx))) return rc;
     if ((rc = launch_residual(ctx))) return rc;
     if ((rc = launch_move_update(ctx, 1.0, 2, 2))) return rc;
