@@ -397,7 +397,8 @@ def measure(args, fa, inputs, workload, steps=None, size=None, with_cpu=True):
         traffic = load_traffic(workload)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1), "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": round(d["GBps"] / PEAK_HBM_GBS, 4),
-                    "traffic": (traffic or {}).get(dom) or (traffic or {}).get(dom + "_both")}
+                    "traffic": next((traffic[k] for k in (dom, dom + "_both", dom + "_brick")
+                                     if traffic and traffic.get(k)), None)}
         log("[bench] per-kernel: " + json.dumps({k: {kk: round(vv, 2) for kk, vv in v.items()}
                                                   for k, v in kernels.items()}))
 
