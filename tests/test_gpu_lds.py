@@ -242,3 +242,27 @@ def test_brick_and_gathering_grids_mix(oracle_lib):
                 got = b.state(pid)
                 for k in ("d", "v", "e", "p"):
                     assert np.array_equal(got[k], np.asarray(getattr(s, k)).reshape(got[k].shape)), (i, k)
+
+
+def test_two_brick_grids_in_one_batch_graph_and_eager(oracle_lib):
+    """Two brick-shaped grids of different extents share the brick launches
+    (each brick finds its problem's strides); a graph-replayed run and an
+    eager one both equal the oracle bit for bit."""
+    print("3D bricks: two grids, graph and eager", flush=True)
+    probs = [_kh_box(16, 8, 8, steps=6), _kh_box(8, 16, 12, steps=6)]
+    probs[1]["flux"] = 2
+    want = [_oracle_run(oracle_lib, p) for p in probs]
+    nc = sum(len(p["cells"]) for p in probs)
+    nv = sum(len(p["coords"]) for p in probs)
+    for graph in (True, False):
+        with fa.Batch(fa.SOLVER_EULER, 3, 8, 4, nc + 512, 4 * nc + 512, nv + 512, 27 * nc + 512) as b:
+            ids = [b.add_euler(p["coords"], p["cells"], p["eos"], p["d"], p["v"], p["p"], p["cfl"],
+                               p["final_time"], p["max_steps"], flux_kind=p["flux"]) for p in probs]
+            assert b.lds_staging() == 6
+            b.step(6, use_graph=graph)
+            st = b.status()
+            for pid, (s, t) in zip(ids, want):
+                assert st[pid].steps == 6 and st[pid].time == t, (graph, pid)
+                got = b.state(pid)
+                for k in ("d", "v", "e", "p"):
+                    assert np.array_equal(got[k], np.asarray(getattr(s, k)).reshape(got[k].shape)), (graph, pid, k)
