@@ -211,11 +211,13 @@ def serve_trace(fa, requests, mode, arrivals=None, capacity=None, window=1024, g
             "mean_s": float(lat.mean()), "errors": errors, "host": host}
 
 
-def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs=True, modes=("batched", "one_at_a_time")):
+def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs=True, modes=("batched", "one_at_a_time"),
+                 cpu=False):
     """The mixed-size request stream: one-at-a-time service rate mu from the
     first `calib` requests; a Poisson trace at 2 mu served both ways (latency
     under the same saturated trace); and both ways with the queue kept full
-    (throughput capacity)."""
+    (throughput capacity).  cpu: also the same trace replayed on the host
+    cores (cpu_replay)."""
     from flecsale_amd.inputs import stream_arrivals
     t_gen = time.perf_counter()
     requests = serve_requests(inputs, n, steps)
@@ -253,6 +255,13 @@ def serve_stream(fa, inputs, n, steps=100, calib=24, one_saturated=False, graphs
         log(f"[serve] {name} saturated: {r['zu_per_s']:.3e} ZU/s, makespan {r['makespan_s']:.2f} s"
             + (f", host {json.dumps({k: round(v, 3) for k, v in r['host'].items()})}" if r["host"] else ""))
         out[name + "_saturated"] = r
+    if cpu:
+        log("[serve] timing the CPU restatement on the first requests of the stream")
+        base = stream_cpu_baseline(inputs, inputs.stream_sides(n), steps)
+        out["cpu_baseline"] = {k: v for k, v in base.items() if k != "per_kind"}
+        out["cpu_replay"] = cpu_replay(requests, arrivals, steps, base)
+        log(f"[serve] cpu replay: {out['cpu_replay']['zu_per_s']:.3e} ZU/s, "
+            f"p50 {out['cpu_replay']['p50_s']:.1f} s, p99 {out['cpu_replay']['p99_s']:.1f} s")
     if "one_at_a_time" not in modes:
         return out
     one = out.get("one_at_a_time_saturated", out["one_at_a_time_trace"])
@@ -471,7 +480,7 @@ def main():
     if not args.no_serve:
         n_req = args.requests or 4096
         log(f"[bench] served stream sample: {n_req} requests")
-        serving = serve_stream(fa, inputs, n_req, graphs=not args.eager)
+        serving = serve_stream(fa, inputs, n_req, graphs=not args.eager, cpu=not args.no_cpu_baseline)
     out = {
         "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
         "value": r["value"],
@@ -501,13 +510,16 @@ def main():
 
 
 def stream_cpu_baseline(inputs, sides, steps, budget_s=12.0):
-    """CPU restatement on the first requests of the stream, one after another."""
+    """CPU restatement on the first requests of the stream, one after another
+    (zone-updates per second overall and per solver kind)."""
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     zu = k = 0
+    per_kind = {kind: [0, 0.0] for kind in SERVE_KINDS}   # zone-updates, seconds
     while k < 2 or (time.perf_counter() - t0 < budget_s and k < len(sides)):
         kind = SERVE_KINDS[k % 2]
+        t_req = time.perf_counter()
         prob = inputs.riemann_2d(int(sides[k]), kind, index=k, steps=steps)
         mesh = O.Mesh(prob["coords"], prob["cells"])
         if kind == "euler":
@@ -518,23 +530,46 @@ def stream_cpu_baseline(inputs, sides, steps, budget_s=12.0):
             s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
             rc, t, n, dt = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], steps, prob["dt0"])
         zu += len(prob["cells"]) * n
+        per_kind[kind][0] += len(prob["cells"]) * n
+        per_kind[kind][1] += time.perf_counter() - t_req
         k += 1
     el = time.perf_counter() - t0
     return dict(value=zu / el, unit="zone-updates/s", cores=threads, kind="port",
                 sample=f"first {k} requests of the stream served one after another, {el:.1f} s, "
-                       f"OMP_NUM_THREADS={threads}")
+                       f"OMP_NUM_THREADS={threads}",
+                per_kind={kind: v[0] / v[1] for kind, v in per_kind.items() if v[1] > 0})
+
+
+def cpu_replay(requests, arrivals, steps, base):
+    """The seeded Poisson trace replayed on the host cores: each request
+    served in arrival order, one after another, for its zone-updates at the
+    CPU rate measured for its solver kind on the stream's first requests
+    (stream_cpu_baseline).  The service times are that model's, not timed
+    one by one: the whole stream would keep the host busy for about 20
+    minutes."""
+    rates = base["per_kind"]
+    t_free = 0.0
+    lat, zu = [], 0
+    for (kind, prob), a in zip(requests, arrivals):
+        z = len(prob["cells"]) * steps
+        start = max(a, t_free)
+        t_free = start + z / rates[kind]
+        lat.append(t_free - a)
+        zu += z
+    span = float(t_free - min(arrivals))
+    return {"requests": len(requests), "makespan_s": span, "zone_updates": int(zu), "zu_per_s": zu / span,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(np.mean(lat)), "cores": base["cores"],
+            "rates": {k: round(v, 1) for k, v in rates.items()}, "basis": base["sample"]}
 
 
 def main_serve(args, fa, inputs):
     n = args.requests or 4096
     steps = args.steps or 100
     out = serve_stream(fa, inputs, n, steps, one_saturated=True, graphs=not args.eager,
-                       modes=("batched",) if args.batched_only else ("batched", "one_at_a_time"))
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("[bench] timing the CPU restatement on the first requests of the stream")
-        cpu = stream_cpu_baseline(inputs, inputs.stream_sides(n), steps)
-        log(f"[bench] cpu baseline {cpu['value']:.4e} zone-updates/s ({cpu['sample']})")
+                       modes=("batched",) if args.batched_only else ("batched", "one_at_a_time"),
+                       cpu=not args.no_cpu_baseline)
+    cpu = out.pop("cpu_baseline", None)
     b = out["batched_saturated"]
     line = {
         "metric": "zone-updates/s on 1 MI355X; batched p50/p99 latency vs one-at-a-time GPU",
