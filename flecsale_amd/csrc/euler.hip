@@ -1447,6 +1447,20 @@ int launch_grad(fsale_ctx *ctx, const EulerFields &u) {
   return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 0>, ctx->m, u, b);
 }
 
+// One block per cell tile whatever the kernel's launch shape: the brick
+// kernels measured slower on every persistent grid (gradients 338-383 us
+// against 259, fused step 380-456 against 319 at 128^3), and the tuner's
+// few timed steps have picked one now and then.
+template <class K, class... A>
+int launch_direct(fsale_ctx *ctx, int kid, K kernel, const DevMesh &m, const EulerFields &u, Batch b, A... args) {
+  ProfScope prof(ctx, kid);
+  b.ct.direct = ctx->d_cdirect;
+  b.ct.ndirect = ctx->n_cdirect;
+  hipLaunchKernelGGL(kernel, dim3(std::max(8, ctx->n_cdirect)), dim3(kBlock), 0, ctx->stream, m, u, b, args...);
+  FSALE_HIP(hipGetLastError());
+  return FSALE_OK;
+}
+
 // fin: the step's first gradient launch, which also finalizes its time step
 template <bool FIN>
 int launch_grad_pk(fsale_ctx *ctx, const EulerFields &u) {
@@ -1456,7 +1470,7 @@ int launch_grad_pk(fsale_ctx *ctx, const EulerFields &u) {
   if (ctx->m.dim == 2 && ctx->lds)
     return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_lds<4, false, FIN>, ctx->m, u, b);
   if (ctx->m.dim == 2) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<2, 4, 1, FIN>, ctx->m, u, b);
-  if (ctx->brick_all) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_brick<FIN>, ctx->m, u, b);
+  if (ctx->brick_all) return launch_direct(ctx, K_EUL_GRAD, k_eul_grad_brick<FIN>, ctx->m, u, b);
   if (ctx->s3_all) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_s3<true, FIN>, ctx->m, u, b);
   if (ctx->s3) return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad_s3<false, FIN>, ctx->m, u, b);
   return launch_persistent(ctx, K_EUL_GRAD, k_eul_grad<3, 8, 1, FIN>, ctx->m, u, b);
@@ -1475,9 +1489,17 @@ int launch_fluxupd(fsale_ctx *ctx, double factor, bool want_dt, int token, const
   if (ctx->m.dim == 2)
     return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<2, 4, 4, false>, ctx->m, in, out, b, factor,
                              (int)want_dt, token);
-  if (ctx->brick_all)
-    return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd_brick, ctx->m, in, out, b, factor, (int)want_dt,
-                             token);
+  if (ctx->brick_all) {
+    // one block per brick, as launch_direct
+    ProfScope prof(ctx, K_EUL_FLUXUPD);
+    Batch bd = b;
+    bd.ct.direct = ctx->d_cdirect;
+    bd.ct.ndirect = ctx->n_cdirect;
+    hipLaunchKernelGGL(k_eul_fluxupd_brick, dim3(std::max(8, ctx->n_cdirect)), dim3(kBlock), 0, ctx->stream, ctx->m,
+                       in, out, bd, factor, (int)want_dt, token);
+    FSALE_HIP(hipGetLastError());
+    return FSALE_OK;
+  }
   if (ctx->fp3)
     return launch_persistent(ctx, K_EUL_FLUXUPD, k_eul_fluxupd<3, 6, 8, false, false, true>, ctx->m, in, out, b,
                              factor, (int)want_dt, token);
