@@ -686,9 +686,10 @@ __device__ inline void corner_force(const MaireFields &u, int c, const double M[
 // one tile of interior (Cramer) or boundary (symmetry QR) vertices
 // Whether a problem's waves work in a 2D driver step's first nodal solve and
 // residual.  PEND: the previous step's close (k_maire_end's update of t,
-// steps and the step flag) has not run yet -- it runs in k_maire_close_dt
-// right after these two launches -- so the step flag it would set is
-// computed here, from the same values, which nothing writes meanwhile.
+// steps and the step flag) has not run yet -- the predictor's update
+// (k_maire_move_update FOLD) writes it right after these two launches -- so
+// the step flag it would set is computed here, from the same values, which
+// nothing writes meanwhile.
 template <bool PEND>
 __device__ inline bool step_live(const Batch &b, int prob) {
   if (!wave_live(b.s, prob, 0)) return false;
@@ -993,16 +994,12 @@ __device__ inline void close_step(DevScalars &S, const ProbParams &P) {
 
 // evaluate_time_step :179-188 (first minimum of the three limits) and the
 // driver's cap min(dt, t_final - t) (driver.h:391), one block per problem.
-// CLOSE: first the previous step's close_step (a 2D driver step defers it
-// to here, after its first nodal solve and residual).
-template <bool CLOSE>
 __device__ inline void dt_finalize(const Batch &b, const double *partials) {
   const int nprob = *b.np;
   for (int ip = blockIdx.x; ip < nprob; ip += gridDim.x) {
     const int p = b.plist[ip];
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
-    if (CLOSE && threadIdx.x == 0) close_step(b.s[p], P);
     if (!block_active(b.s, p)) continue;
     double mx[2];
     final_max<2>(partials + 2 * (size_t)P.tile_first * kWaves, P.tile_count * kWaves, mx);
@@ -1018,10 +1015,7 @@ __device__ inline void dt_finalize(const Batch &b, const double *partials) {
   }
 }
 __global__ void __launch_bounds__(kFinalBlock) k_maire_dt_finalize(Batch b, const double *partials) {
-  dt_finalize<false>(b, partials);
-}
-__global__ void __launch_bounds__(kFinalBlock) k_maire_close_dt(Batch b, const double *partials) {
-  dt_finalize<true>(b, partials);
+  dt_finalize(b, partials);
 }
 
 // apply_update's mesh motion x = x0 + dt u (:517-534).  SAVE (the driver
@@ -1177,8 +1171,9 @@ __global__ void __launch_bounds__(kBlock) k_maire_update(DevMesh m, MaireFields 
 // another.  MODE 1: the predictor, 2: the corrector.  The vertex tiles use
 // the launch's token, as the update does: an error this launch raises does
 // not stop its moves (a separate move launch would have completed first).
-// FOLD (the predictor): the launch also does what k_maire_close_dt /
-// k_maire_dt_finalize would do before it.  Every block finalizes the time
+// FOLD (the predictor): the launch also closes the previous step
+// (close_step) and finalizes the time step (dt_finalize), which would
+// otherwise be a launch of their own before it.  Every block finalizes the time
 // step of its tiles' problems itself from the per-tile partials and the
 // step recorded by the residual launch (DevScalars::snap_*, k_maire_residual
 // TILE), which no block of this launch writes; the block of each problem's
@@ -1405,14 +1400,10 @@ int launch_residual(fsale_ctx *ctx, bool with_dt = false) {
   return ctx->m.dim == 3 ? launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<3, false>, ctx->m, ctx->ma, b)
                          : launch_persistent(ctx, K_MAIRE_RESIDUAL, k_maire_residual<2, false>, ctx->m, ctx->ma, b);
 }
-int launch_dt_finalize(fsale_ctx *ctx, bool close = false) {
+int launch_dt_finalize(fsale_ctx *ctx) {
   ProfScope prof(ctx, K_MAIRE_DT);
-  if (close)
-    hipLaunchKernelGGL(k_maire_close_dt, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream, batch_args(ctx, -1),
-                       (const double *)ctx->ma.partials);
-  else
-    hipLaunchKernelGGL(k_maire_dt_finalize, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream,
-                       batch_args(ctx, -1), (const double *)ctx->ma.partials);
+  hipLaunchKernelGGL(k_maire_dt_finalize, dim3(ctx->pgrid), dim3(kFinalBlock), 0, ctx->stream, batch_args(ctx, -1),
+                     (const double *)ctx->ma.partials);
   FSALE_HIP(hipGetLastError());
   return FSALE_OK;
 }
@@ -1474,8 +1465,8 @@ int maire_step(fsale_ctx *ctx, bool first, bool last) {
   if (ctx->m.dim == 2) {
     // the first nodal solve saves the step's start coordinates, the moves
     // ride in the updates, and the close of a step runs with the next
-    // step's time step (k_maire_close_dt), or after the last step of the
-    // batch of steps (k_maire_end): six launches per step
+    // step's time step in its predictor's update, or after the last step of
+    // the batch of steps (k_maire_end): five launches per step
     if ((rc = first ? launch_nodal_n<2, true, false>(ctx) : launch_nodal_n<2, true, true>(ctx))) return rc;
     {
       const Batch b = batch_args(ctx, K_MAIRE_RESIDUAL);
