@@ -225,7 +225,8 @@ def main():
     t = sub.add_parser("trace")
     t.add_argument("trace_dir")
     t.add_argument("out")
-    t.add_argument("--command", default="")
+    t.add_argument("--command", required=True,
+                   help="the full rocprofv3 command that made the trace, written into the summary")
     t.add_argument("--last", type=int, default=None,
                    help="summarise the span of the anchor kernel's last N dispatches")
     t.add_argument("--anchor", default=None, help="kernel name prefix, e.g. k_eul_fluxupd")
