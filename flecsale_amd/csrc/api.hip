@@ -1246,9 +1246,22 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph);
 // device memory, so it stays valid while problems come and go).  Eulerian
 // steps work on packed copies of the state (EulerFields::gs), made before
 // the steps and unpacked after them.
+__global__ void k_task_open(DevScalars *s) {
+  if (threadIdx.x == 0 && s->err_token == 0) s->active = 1;
+}
+
+int open_tasks(fsale_ctx *ctx) {
+  if (ctx->tasks_open) return FSALE_OK;
+  hipLaunchKernelGGL(k_task_open, dim3(1), dim3(64), 0, ctx->stream, ctx->s);
+  FSALE_HIP(hipGetLastError());
+  ctx->tasks_open = true;
+  return FSALE_OK;
+}
+
 int batch_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   int rc;
   if ((rc = sync_tiles(ctx))) return rc;
+  ctx->tasks_open = false;
   if (ctx->h_ctiles.empty() || nsteps <= 0) return FSALE_OK;
   if (ctx->solver == FSALE_SOLVER_EULER) {
     bool need = false;

@@ -1646,7 +1646,7 @@ int fsale_eul_evaluate_time_step(fsale_ctx *ctx, double cfl, double final_time, 
   P.params.final_time = final_time;
   int rc;
   if ((rc = upload_params(ctx, 0))) return rc;
-  if ((rc = sync_tiles(ctx))) return rc;
+  if ((rc = sync_tiles(ctx)) || (rc = open_tasks(ctx))) return rc;
   if ((rc = launch_dt_partial(ctx))) return rc;
   if ((rc = launch_finalize(ctx, 0))) return rc;
   P.partials_valid = true;
@@ -1675,7 +1675,7 @@ int fsale_eul_evaluate_gradients(fsale_ctx *ctx, int first_order) {
   int rc;
   P.params.first_order = first_order != 0;
   if ((rc = upload_params(ctx, 0))) return rc;
-  if ((rc = sync_tiles(ctx))) return rc;
+  if ((rc = sync_tiles(ctx)) || (rc = open_tasks(ctx))) return rc;
   return launch_grad(ctx, ctx->eu);
 }
 
@@ -1685,7 +1685,7 @@ int fsale_eul_evaluate_fluxes(fsale_ctx *ctx, int flux_kind) {
   if (!P.have_state) return FSALE_ERR_STATE;
   P.params.kind = flux_kind;
   int rc;
-  if ((rc = upload_params(ctx, 0))) return rc;
+  if ((rc = upload_params(ctx, 0)) || (rc = open_tasks(ctx))) return rc;
   ProfScope prof(ctx, K_EUL_FLUX);
   const int g = (P.nf + kBlock - 1) / kBlock;
   if (ctx->m.dim == 2)
@@ -1707,7 +1707,7 @@ int fsale_eul_apply_update(fsale_ctx *ctx, fsale_eos eos, double factor) {
   P.partials_valid = false;
   int rc;
   if ((rc = upload_params(ctx, 0))) return rc;
-  if ((rc = sync_tiles(ctx))) return rc;
+  if ((rc = sync_tiles(ctx)) || (rc = open_tasks(ctx))) return rc;
   const Batch b = batch_args(ctx, K_EUL_UPDATE);
   if (ctx->m.dim == 2)
     return launch_persistent(ctx, K_EUL_UPDATE, k_eul_update<2, 4>, ctx->m, ctx->eu, b, factor, 3);

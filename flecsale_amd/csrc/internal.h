@@ -400,6 +400,7 @@ struct fsale_ctx {
   bool s3 = false, s3_all = false;     // some / every live problem is a structured 3D grid (s_nx)
   bool fp3 = false;                    // some live problem's hex sides pair up (s_fp)
   bool brick_all = false;              // every live problem's fused step runs over bricks
+  bool tasks_open = true;              // no driver run since the step flag was last set for task calls
   void *stage = nullptr;               // pinned staging buffer for uploads
   size_t stage_bytes = 0;
   void *dl = nullptr;                  // pinned buffer for multi-field downloads
@@ -428,6 +429,12 @@ int dev_alloc_n(fsale_ctx *ctx, T **p, size_t n) {
   return dev_alloc(ctx, reinterpret_cast<void **>(p), n * sizeof(T) + 16);
 }
 int read_scalars(fsale_ctx *ctx);      // sync + copy DevScalars to h_s (and lift slot quarantines)
+// Task-level entry points of a single-mesh context: a driver run leaves the
+// problem's step flag (DevScalars::active) cleared once its step budget or
+// final time is reached, and every step kernel skips a problem whose flag is
+// clear; the first task call after a run sets it again (an error the run
+// latched still stops the kernels, as it stops the run).
+int open_tasks(fsale_ctx *ctx);
 int set_maire_boundary(fsale_ctx *ctx, int prob, int tag, int kind, const double *value);
 // Lagrangian contexts: refreshes the nodal solve's packed copies of one
 // problem's cell state and coordinates (no-op otherwise)

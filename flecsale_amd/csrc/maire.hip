@@ -1617,6 +1617,7 @@ int fsale_maire_evaluate_nodal_state(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
   int rc = sync_tiles(ctx);
+  if (!rc) rc = open_tasks(ctx);
   return rc ? rc : launch_nodal(ctx);
 }
 
@@ -1624,6 +1625,7 @@ int fsale_maire_evaluate_residual(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
   int rc = sync_tiles(ctx);
+  if (!rc) rc = open_tasks(ctx);
   return rc ? rc : launch_residual(ctx);
 }
 
@@ -1637,7 +1639,8 @@ int fsale_maire_evaluate_time_step(fsale_ctx *ctx, double cfl_accoustic, double 
   P.params.cfl_growth = cfl_growth;
   P.params.final_time = final_time;
   int rc;
-  if ((rc = upload_params(ctx, 0)) || (rc = sync_tiles(ctx)) || (rc = launch_dt(ctx))) return rc;
+  if ((rc = upload_params(ctx, 0)) || (rc = sync_tiles(ctx)) || (rc = open_tasks(ctx)) || (rc = launch_dt(ctx)))
+    return rc;
   if (!dt_out) return FSALE_OK;
   if ((rc = check_device_error(ctx, 0))) return rc;
   *dt_out = ctx->h_s[0].dt;
@@ -1648,18 +1651,21 @@ int fsale_maire_apply_update(fsale_ctx *ctx, double factor) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   if (!ctx->probs[0].have_state) return FSALE_ERR_STATE;
   int rc = sync_tiles(ctx);
+  if (!rc) rc = open_tasks(ctx);
   return rc ? rc : launch_apply(ctx, factor, 3, 0);
 }
 
 int fsale_maire_save(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   int rc = sync_tiles(ctx);
+  if (!rc) rc = open_tasks(ctx);
   return rc ? rc : launch_copy(ctx, 0);
 }
 
 int fsale_maire_restore(fsale_ctx *ctx) {
   if (!check_ctx(ctx)) return FSALE_ERR_ARG;
   int rc = sync_tiles(ctx);
+  if (!rc) rc = open_tasks(ctx);
   return rc ? rc : launch_copy(ctx, 1);
 }
 

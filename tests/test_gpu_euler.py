@@ -189,6 +189,32 @@ def test_sod_first_order_to_final_time(oracle_lib):
         assert d.min() >= 0.125 - 1e-12 and d.max() <= 1.0 + 1e-12
 
 
+def test_task_calls_after_a_run(oracle_lib):
+    """A run that used up its step budget leaves the problem's step flag
+    cleared; the task-level entry points called after it still act on the
+    run's final state (time step, then one explicit half step by tasks)."""
+    O = oracle_lib
+    prob = inputs.quadrants_2d(48, steps=6)
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    s = _oracle_state(O, mesh, prob["eos"], prob["d"], prob["v"], prob["p"])
+    rc, t_o, n_o = O.euler_run(mesh, s, *prob["eos"], prob["cfl"], prob["final_time"], 6, flux_kind=prob["flux"])
+    assert rc == 0 and n_o == 6
+    with fa.EulerContext(prob["coords"], prob["cells"]) as ctx:
+        ctx.initial_conditions(prob["eos"], prob["d"], prob["v"], prob["p"])
+        ctx.advance(prob["cfl"], prob["final_time"], 6, flux_kind=prob["flux"])
+        _cmp_state(ctx, s, 0.0, "run")
+        dt = ctx.evaluate_time_step(prob["cfl"], prob["final_time"])
+        assert dt == O.euler_time_step(mesh, s, prob["cfl"], prob["final_time"], t_o)
+        ctx.evaluate_gradients()
+        ctx.evaluate_fluxes(prob["flux"])
+        ctx.apply_update(0.5)
+        g = O.euler_gradients(mesh, s)
+        F = O.euler_fluxes(mesh, s, g, prob["flux"])
+        rc, _ = O.euler_apply_update(mesh, s, *prob["eos"], F, dt, 0.5)
+        assert rc == 0
+        _cmp_state(ctx, s, 0.0, "tasks after the run")
+
+
 def test_graph_and_eager_agree_bitwise():
     prob = inputs.quadrants_2d(64, steps=23)
     out = []

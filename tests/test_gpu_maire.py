@@ -95,6 +95,35 @@ def test_bubble_run_matches_oracle(oracle_lib, n, steps):
         _cmp(ctx, s, mesh, FINAL_TOL, "bubble")
 
 
+def test_task_calls_after_a_run(oracle_lib):
+    """A run that used up its step budget leaves the problem's step flag
+    cleared; the task-level entry points called after it still act on the
+    run's final state (the nodal solve, the residual and the time step)."""
+    O = oracle_lib
+    prob = inputs.maire_bubble_2d(16, steps=15)
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
+    rc, t_o, n_o, dt_o = O.maire_run(mesh, s, prob["cv"], prob["cfl"], prob["final_time"], 15, prob["dt0"])
+    assert rc == 0 and n_o == 15
+    un_o, npc_o, Fpc_o = O.maire_nodal(mesh, s)
+    dudt_o = O.maire_residual(mesh, un_o, npc_o, Fpc_o)
+    with fa.MaireContext(prob["coords"], prob["cells"]) as ctx:
+        ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
+        ctx.set_time(0.0, prob["dt0"])
+        ctx.advance(prob["cfl"], prob["final_time"], 15)
+        st = ctx.state()
+        for k, want in [("d", s.d), ("v", s.v), ("e", s.e), ("p", s.p)]:
+            assert np.array_equal(st[k], want), k
+        ctx.evaluate_nodal_state()
+        assert np.array_equal(ctx.get(fa.F_NODE_VELOCITY), un_o)
+        assert np.array_equal(ctx.get(fa.F_CORNER_NORMAL), npc_o)
+        assert np.array_equal(ctx.get(fa.F_CORNER_FORCE), Fpc_o)
+        ctx.evaluate_residual()
+        assert np.array_equal(ctx.get(fa.F_RESIDUAL), dudt_o)
+        dt = ctx.evaluate_time_step(prob["cfl"], prob["final_time"])
+        assert dt == min(O.maire_time_step(mesh, prob["cfl"], dt_o, s.a, dudt_o), prob["final_time"] - t_o)
+
+
 def test_mass_and_total_energy_conserved():
     """Compatible discretisation: mass exactly, total energy to round-off (walls do no work)."""
     prob = inputs.maire_bubble_2d(32, steps=30)
