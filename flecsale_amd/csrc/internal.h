@@ -228,7 +228,10 @@ struct DevScalars {
   // writes (k_maire_move_update FOLD)
   double snap_t, snap_dt;
   long long snap_steps;
-  int snap_active, snap_pad;
+  int snap_active;
+  // Lagrangian 2D driver steps ran since the last k_maire_settle: the fields
+  // no kernel of a step reads are not current (k_maire_settle writes them)
+  int lazy;
 };
 
 struct EulerFields {

@@ -1062,6 +1062,10 @@ __global__ void __launch_bounds__(kBlock) k_maire_move(DevMesh m, MaireFields u,
 // launch).
 // FS (a 2D driver step's predictor, FoldedStep): the step's flag and time
 // step as the launch computed them, instead of the scalars it is writing.
+// MOVED also leaves unwritten what no kernel of a driver step reads: the
+// centroid, volume copy (V), density, pressure, temperature and velocity
+// (the velocity is read from its packed copy); k_maire_settle writes them
+// at the end of a batch of steps, from the same inputs.
 struct FoldedStep {
   int active;
   double dt;
@@ -1099,8 +1103,10 @@ __device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields 
         }
         double cx, cy, ml;
         cell_geom_2d(4, px, py, cx, cy, vol, ml);
-        m.ccen[0][c] = cx;
-        m.ccen[1][c] = cy;
+        if (!MOVED) {
+          m.ccen[0][c] = cx;
+          m.ccen[1][c] = cy;
+        }
         m.cvol[c] = vol;
         m.cminlen[c] = ml;
       } else {
@@ -1124,8 +1130,14 @@ __device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields 
     const double M = u.M[c];
     const double inv_mass = 1 / M;
     double vel[N], e0;
+    if constexpr (MOVED && MODE == 1) {
+      const double2 w = u.cs2[1][c];
+      vel[0] = w.x;
+      vel[1] = w.y;
+    } else {
 #pragma unroll
-    for (int i = 0; i < N; ++i) vel[i] = MODE == 2 ? u.v0[i][c] : u.v[i][c];
+      for (int i = 0; i < N; ++i) vel[i] = MODE == 2 ? u.v0[i][c] : u.v[i][c];
+    }
     e0 = MODE == 2 ? u.e0[c] : u.e[c];
     if (MODE == 1) {
 #pragma unroll
@@ -1138,17 +1150,21 @@ __device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields 
     const double ie = et - dotN<N>(vel, vel) / 2;
     // update_volume :352-367
     const double d = M / vol;
+    if (!MOVED) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) u.v[i][c] = vel[i];
+      for (int i = 0; i < N; ++i) u.v[i][c] = vel[i];
+      u.V[c] = vol;
+      u.d[c] = d;
+    }
     u.e[c] = ie;
-    u.V[c] = vol;
-    u.d[c] = d;
     // update_state_from_energy lagrange_eqns.h:280-302
     const double g = u.g[c];
     const double p = eos_p_de(g, d, ie);
     const double a = smax(eos_a_de(g, ie), kMinSoundSpeed);
-    u.p[c] = p;
-    u.T[c] = eos_T_de(P.cv, ie);
+    if (!MOVED) {
+      u.p[c] = p;
+      u.T[c] = eos_T_de(P.cv, ie);
+    }
     u.a[c] = a;
     if constexpr (N == 3)
       put_cs(u, c, d, a, p, vel[0], vel[1], vel[2]);
@@ -1252,9 +1268,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFi
     const int vt = tile.first + threadIdx.x;
     const double2 w = u.x0p[vt];
     const double x = w.x + delta_t * u.un[0][vt], y = w.y + delta_t * u.un[1][vt];
-    m.x[0][vt] = x;
-    m.x[1][vt] = y;
-    put_xy(m, vt, x, y);
+    put_xy(m, vt, x, y);   // m.x follows in k_maire_settle
   });
   for_each_tile(b.ct, [&](const Tile &tile) {
     if constexpr (FOLD) {
@@ -1264,12 +1278,56 @@ __global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFi
         S.t = S.snap_t;
         S.steps = S.snap_steps;
         S.active = S.snap_active;
-        if (fs.active) S.dt = fs.dt;
+        if (fs.active) {
+          S.dt = fs.dt;
+          S.lazy = 1;
+        }
       }
       update_tile<MODE, 2, true, true>(m, u, b, factor, token, tile, fs);
     } else {
       update_tile<MODE, 2, true>(m, u, b, factor, token, tile);
     }
+  });
+}
+
+// The end of a batch of 2D driver steps: for every problem whose steps ran
+// (DevScalars::lazy), what the steps' updates left unwritten (update_tile
+// MOVED), as they would have written it: the coordinates from their pairs,
+// the centroids by cell_geom_2d from the coordinates, volume, density,
+// pressure and temperature from the cell volume, mass and internal energy,
+// and the velocity from its packed copy.  The same functions of the same
+// values: the same bits.
+__global__ void __launch_bounds__(kBlock) k_maire_settle(DevMesh m, MaireFields u, Batch b) {
+  for_each_tile(b.vt, [&](const Tile &tile) {
+    if (!b.s[tile.prob].lazy || (int)threadIdx.x >= tile.count) return;
+    const int v = tile.first + threadIdx.x;
+    const double2 w = m.xy[v];
+    m.x[0][v] = w.x;
+    m.x[1][v] = w.y;
+  });
+  for_each_tile(b.ct, [&](const Tile &tile) {
+    if (!b.s[tile.prob].lazy || (int)threadIdx.x >= tile.count) return;
+    const ProbParams &P = b.pp[tile.prob];
+    const int c = tile.first + threadIdx.x;
+    double px[4], py[4];
+    for (int i = 0; i < 4; ++i) {
+      const double2 w = m.xy[m.cv[(size_t)i * m.cap_c + c]];
+      px[i] = w.x;
+      py[i] = w.y;
+    }
+    double cx, cy, vol, ml;
+    cell_geom_2d(4, px, py, cx, cy, vol, ml);
+    m.ccen[0][c] = cx;
+    m.ccen[1][c] = cy;
+    const double V = m.cvol[c], ie = u.e[c];
+    const double d = u.M[c] / V;
+    u.V[c] = V;
+    u.d[c] = d;
+    u.p[c] = eos_p_de(u.g[c], d, ie);
+    u.T[c] = eos_T_de(P.cv, ie);
+    const double2 vc = u.cs2[1][c];
+    u.v[0][c] = vc.x;
+    u.v[1][c] = vc.y;
   });
 }
 
@@ -1331,6 +1389,7 @@ __global__ void k_maire_end(Batch b) {
     const ProbParams &P = b.pp[p];
     if (P.cell_count == 0) continue;
     close_step(b.s[p], P);
+    b.s[p].lazy = 0;
   }
 }
 
@@ -1487,6 +1546,9 @@ int maire_step(fsale_ctx *ctx, bool first, bool last) {
     if ((rc = launch_residual(ctx))) return rc;
     if ((rc = launch_move_update(ctx, 1.0, 2, 2))) return rc;
     if (last) {
+      if ((rc = launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_settle, ctx->m, ctx->ma,
+                                  batch_args(ctx, K_MAIRE_UPDATE))))
+        return rc;
       hipLaunchKernelGGL(k_maire_end, dim3(small_grid(np)), dim3(kBlock), 0, ctx->stream, batch_args(ctx, -1));
       FSALE_HIP(hipGetLastError());
     }
