@@ -872,6 +872,34 @@ k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
   }
 }
 
+// evaluate_residual of cell c (compute_update lagrange_eqns.h:374-389): its
+// corners' normals and forces against its vertices' velocities, in local
+// vertex order; vt / uv return the vertex ids and velocities it read.
+template <int N, int V>
+__device__ __forceinline__ void cell_residual(const DevMesh &m, const MaireFields &u, int c, double r[N + 2],
+                                              int vt[V], double uv[V][N]) {
+#pragma unroll
+  for (int i = 0; i < N + 2; ++i) r[i] = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const size_t ci = (size_t)k * m.cap_c + c;
+    vt[k] = m.cv[ci];
+    double F[N], n[N];
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+      uv[k][d] = u.un[d][vt[k]];
+      F[d] = u.Fpc[d][ci];
+      n[d] = u.npc[d][ci];
+    }
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+      r[0] += n[d] * uv[k][d];
+      r[N + 1] -= F[d] * uv[k][d];
+      r[1 + d] -= F[d];
+    }
+  }
+}
+
 // DT: also the time step's per-wave partials of the new residual (the
 // driver step's predictor, whose time step follows), as k_maire_dt_partial
 // computes them.  TILE (2D driver step, with DT): one partial pair per cell
@@ -905,28 +933,9 @@ __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireField
     double acc = 0, vol = 0;
     if ((int)threadIdx.x < tile.count) {
     const int c = tile.first + threadIdx.x;
-    double r[N + 2];
-#pragma unroll
-    for (int i = 0; i < N + 2; ++i) r[i] = 0;
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-      const size_t ci = (size_t)k * m.cap_c + c;
-      const int vt = m.cv[ci];
-      double uv[N], F[N], n[N];
-#pragma unroll
-      for (int d = 0; d < N; ++d) {
-        uv[d] = u.un[d][vt];
-        F[d] = u.Fpc[d][ci];
-        n[d] = u.npc[d][ci];
-      }
-      // compute_update lagrange_eqns.h:374-389
-#pragma unroll
-      for (int d = 0; d < N; ++d) {
-        r[0] += n[d] * uv[d];
-        r[N + 1] -= F[d] * uv[d];
-        r[1 + d] -= F[d];
-      }
-    }
+    double r[N + 2], uv[V][N];
+    int vt[V];
+    cell_residual<N, V>(m, u, c, r, vt, uv);
 #pragma unroll
     for (int i = 0; i < N + 2; ++i) u.dudt[i][c] = r[i];
     if (DT) {
@@ -1070,9 +1079,15 @@ struct FoldedStep {
   int active;
   double dt;
 };
-template <int MODE, int N, bool MOVED = false, bool FS = false>
+// RES (a 2D driver step's corrector, whose time step is known): the cell's
+// residual computed here (cell_residual, as k_maire_residual computes and
+// stores it) from this half step's nodal solve, instead of in a launch of
+// its own.  The gate is the update's: before this launch only an earlier
+// launch can have latched an error, which stops both.
+template <int MODE, int N, bool MOVED = false, bool FS = false, bool RES = false>
 __device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields &u, const Batch &b, double factor,
                                             int token, const Tile &tile, const FoldedStep &fs = FoldedStep{}) {
+  static_assert(!RES || (N == 2 && MOVED), "the residual rides in 2D driver updates");
   {
     if constexpr (FS) {
       const int et = b.s[tile.prob].err_token;
@@ -1084,14 +1099,25 @@ __device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields 
     const ProbParams &P = b.pp[tile.prob];
     const int c = tile.first + threadIdx.x;
     const double delta_t = factor * (FS ? fs.dt : b.s[tile.prob].dt);
+    double rr[N + 2], ruv[4][2];
+    int rvt[4];
+    if constexpr (RES) {
+      cell_residual<2, 4>(m, u, c, rr, rvt, ruv);
+#pragma unroll
+      for (int i = 0; i < N + 2; ++i) u.dudt[i][c] = rr[i];
+    }
     double vol;
     if (delta_t > 1e-14) {
       // burton update_geometry() for the moved cell
       if constexpr (N == 2) {
         double px[4], py[4];
         for (int i = 0; i < 4; ++i) {
-          const int vt = m.cv[(size_t)i * m.cap_c + c];
-          if constexpr (MOVED) {
+          const int vt = RES ? rvt[i] : m.cv[(size_t)i * m.cap_c + c];
+          if constexpr (RES) {
+            const double2 w = u.x0p[vt];
+            px[i] = w.x + delta_t * ruv[i][0];
+            py[i] = w.y + delta_t * ruv[i][1];
+          } else if constexpr (MOVED) {
             const double2 w = u.x0p[vt];
             px[i] = w.x + delta_t * u.un[0][vt];
             py[i] = w.y + delta_t * u.un[1][vt];
@@ -1144,9 +1170,12 @@ __device__ __forceinline__ void update_tile(const DevMesh &m, const MaireFields 
       for (int i = 0; i < N; ++i) u.v0[i][c] = vel[i];
       u.e0[c] = e0;
     }
-    const double et = e0 + dotN<N>(vel, vel) / 2 + delta_t * inv_mass * u.dudt[N + 1][c];
+    double rate[N + 1];   // energy, then momentum
 #pragma unroll
-    for (int i = 0; i < N; ++i) vel[i] += delta_t * inv_mass * u.dudt[1 + i][c];
+    for (int i = 0; i <= N; ++i) rate[i] = RES ? rr[i == 0 ? N + 1 : i] : u.dudt[i == 0 ? N + 1 : i][c];
+    const double et = e0 + dotN<N>(vel, vel) / 2 + delta_t * inv_mass * rate[0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) vel[i] += delta_t * inv_mass * rate[1 + i];
     const double ie = et - dotN<N>(vel, vel) / 2;
     // update_volume :352-367
     const double d = M / vol;
@@ -1239,7 +1268,7 @@ __device__ inline FoldedStep folded_step(const Batch &b, const double *tp, int p
   return fs;
 }
 
-template <int MODE, bool FOLD = false>
+template <int MODE, bool FOLD = false, bool RES = false>
 __global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFields u, Batch b, double factor,
                                                               int token) {
   // FOLD: the block's last problem and its folded step (a block's vertex
@@ -1283,9 +1312,9 @@ __global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFi
           S.lazy = 1;
         }
       }
-      update_tile<MODE, 2, true, true>(m, u, b, factor, token, tile, fs);
+      update_tile<MODE, 2, true, true, RES>(m, u, b, factor, token, tile, fs);
     } else {
-      update_tile<MODE, 2, true>(m, u, b, factor, token, tile);
+      update_tile<MODE, 2, true, false, RES>(m, u, b, factor, token, tile);
     }
   });
 }
@@ -1489,12 +1518,10 @@ int launch_apply_n(fsale_ctx *ctx, double factor, int token, int stage) {
 int launch_apply(fsale_ctx *ctx, double factor, int token, int stage) {
   return ctx->m.dim == 3 ? launch_apply_n<3>(ctx, factor, token, stage) : launch_apply_n<2>(ctx, factor, token, stage);
 }
-// a 2D driver step's move + update (stage 1 predictor, 2 corrector)
-int launch_move_update(fsale_ctx *ctx, double factor, int token, int stage) {
-  const Batch b = batch_args(ctx, K_MAIRE_UPDATE);
-  if (stage == 1)
-    return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_move_update<1>, ctx->m, ctx->ma, b, factor, token);
-  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_move_update<2>, ctx->m, ctx->ma, b, factor, token);
+// a 2D driver step's corrector: residual, move and update (token 2)
+int launch_corrector_update(fsale_ctx *ctx) {
+  return launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_move_update<2, false, true>, ctx->m, ctx->ma,
+                           batch_args(ctx, K_MAIRE_UPDATE), 1.0, 2);
 }
 int launch_copy(fsale_ctx *ctx, int dir) {
   const Batch b = batch_args(ctx, K_MAIRE_COPY);
@@ -1523,9 +1550,10 @@ int maire_step(fsale_ctx *ctx, bool first, bool last) {
   }
   if (ctx->m.dim == 2) {
     // the first nodal solve saves the step's start coordinates, the moves
-    // ride in the updates, and the close of a step runs with the next
-    // step's time step in its predictor's update, or after the last step of
-    // the batch of steps (k_maire_end): five launches per step
+    // ride in the updates, the corrector's residual in its update, and the
+    // close of a step runs with the next step's time step in its
+    // predictor's update, or after the last step of the batch of steps
+    // (k_maire_settle, k_maire_end): five launches per step
     if ((rc = first ? launch_nodal_n<2, true, false>(ctx) : launch_nodal_n<2, true, true>(ctx))) return rc;
     {
       const Batch b = batch_args(ctx, K_MAIRE_RESIDUAL);
@@ -1542,9 +1570,9 @@ int maire_step(fsale_ctx *ctx, bool first, bool last) {
       if ((rc = launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_move_update<1, true>, ctx->m, ctx->ma, b, 0.5, 1)))
         return rc;
     }
+    // the corrector's residual rides in its update: its time step is known
     if ((rc = launch_nodal(ctx))) return rc;
-    if ((rc = launch_residual(ctx))) return rc;
-    if ((rc = launch_move_update(ctx, 1.0, 2, 2))) return rc;
+    if ((rc = launch_corrector_update(ctx))) return rc;
     if (last) {
       if ((rc = launch_persistent(ctx, K_MAIRE_UPDATE, k_maire_settle, ctx->m, ctx->ma,
                                   batch_args(ctx, K_MAIRE_UPDATE))))
