@@ -702,8 +702,8 @@ __device__ inline bool step_live(const Batch &b, int prob) {
 }
 
 // SAVE (2D, a driver step's first nodal solve): each vertex also saves its
-// coordinates as the step's start (x0, x0p; save_coordinates), which no
-// kernel of this launch reads.
+// coordinates as the step's start (x0p; save_coordinates), which no kernel
+// of this launch reads.
 template <bool BOUNDARY, int N, bool SAVE = false, bool PEND = false>
 __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const Batch &b, const Tile &tile) {
   if constexpr (N == 3) {
@@ -724,12 +724,7 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       for (int q = 0; q < kNodalKeep; ++q) cn0[q] = m.vck[(size_t)q * m.cap_v + vt];
       own = !m.vbnd[vt];
     }
-    if (SAVE && own) {
-      const double2 w = m.xy[vt];
-      u.x0p[vt] = w;
-      u.x0[0][vt] = w.x;
-      u.x0[1][vt] = w.y;
-    }
+    if (SAVE && own) u.x0p[vt] = m.xy[vt];   // u.x0 follows in k_maire_settle
     const int cnt = m.vc_cnt[vt] & -(int)own;
     constexpr int KEEP = BOUNDARY ? kNodalKeepBnd : kNodalKeep;
     double M[KEEP][2][2], np[KEEP][2], F[KEEP][2];
@@ -1321,7 +1316,8 @@ __global__ void __launch_bounds__(kBlock) k_maire_move_update(DevMesh m, MaireFi
 
 // The end of a batch of 2D driver steps: for every problem whose steps ran
 // (DevScalars::lazy), what the steps' updates left unwritten (update_tile
-// MOVED), as they would have written it: the coordinates from their pairs,
+// MOVED), as they would have written it: the coordinates and the last
+// step's saved start coordinates (x0, as save_coordinates) from their pairs,
 // the centroids by cell_geom_2d from the coordinates, volume, density,
 // pressure and temperature from the cell volume, mass and internal energy,
 // and the velocity from its packed copy.  The same functions of the same
@@ -1330,9 +1326,11 @@ __global__ void __launch_bounds__(kBlock) k_maire_settle(DevMesh m, MaireFields 
   for_each_tile(b.vt, [&](const Tile &tile) {
     if (!b.s[tile.prob].lazy || (int)threadIdx.x >= tile.count) return;
     const int v = tile.first + threadIdx.x;
-    const double2 w = m.xy[v];
+    const double2 w = m.xy[v], w0 = u.x0p[v];
     m.x[0][v] = w.x;
     m.x[1][v] = w.y;
+    u.x0[0][v] = w0.x;
+    u.x0[1][v] = w0.y;
   });
   for_each_tile(b.ct, [&](const Tile &tile) {
     if (!b.s[tile.prob].lazy || (int)threadIdx.x >= tile.count) return;
