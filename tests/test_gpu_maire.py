@@ -124,6 +124,32 @@ def test_task_calls_after_a_run(oracle_lib):
         assert dt == min(O.maire_time_step(mesh, prob["cfl"], dt_o, s.a, dudt_o), prob["final_time"] - t_o)
 
 
+def test_run_to_final_time_inside_a_graph_chunk(oracle_lib):
+    """The final time reached a few steps into a replayed chunk of steps:
+    every field the run leaves (state, temperature, sound speed, volume,
+    coordinates, centroids) equals the oracle's run to the same time."""
+    O = oracle_lib
+    prob = inputs.maire_bubble_2d(24, steps=200)
+    mesh = O.Mesh(prob["coords"], prob["cells"])
+    s = O.maire_initial_state(mesh, prob["gamma"], prob["cv"], prob["d"], prob["v"], prob["p"])
+    final_time = 2.5e-4
+    rc, t_o, n_o, _ = O.maire_run(mesh, s, prob["cv"], prob["cfl"], final_time, 200, prob["dt0"])
+    assert rc == 0 and 10 < n_o < 200 and n_o % 10 != 0
+    geo = mesh.geometry()
+    with fa.MaireContext(prob["coords"], prob["cells"]) as ctx:
+        ctx.initial_conditions(prob["cv"], prob["gamma"], prob["d"], prob["v"], prob["p"])
+        ctx.set_time(0.0, prob["dt0"])
+        ctx.advance(prob["cfl"], final_time, 200)
+        t, _, n = ctx.time()
+        assert n == n_o and t == t_o
+        st = ctx.state()
+        for k, want in [("d", s.d), ("v", s.v), ("e", s.e), ("p", s.p), ("T", s.T), ("a", s.a), ("V", s.V)]:
+            assert np.array_equal(st[k], np.asarray(want).reshape(st[k].shape)), k
+        assert np.array_equal(ctx.get(fa.F_COORDS), geo["coords"])
+        assert np.array_equal(ctx.get(fa.F_CELL_CENTROID), geo["cell_centroid"])
+        assert np.array_equal(ctx.get(fa.F_CELL_VOLUME), geo["cell_volume"])
+
+
 def test_mass_and_total_energy_conserved():
     """Compatible discretisation: mass exactly, total energy to round-off (walls do no work)."""
     prob = inputs.maire_bubble_2d(32, steps=30)
