@@ -452,7 +452,7 @@ def main():
                     help="blocks per CU for every kernel (0 = one block per tile); default: autotune")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+        print(json.dumps({"error": "flecsale_amd runs on one GPU; multi-GPU sharding is a design "
                                    "(domain decomposition + halo exchange), not a measured path",
                           "n_gpus": args.gpus}))
         sys.exit(2)

@@ -2,11 +2,11 @@
 
 Two passes, each under its own rocprofv3 run (the counters do not fit one pass):
 
-    rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- \\
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d prof/pmc_fetch -o run -- \\
         python3 -m flecsale_amd.profiling run
-    rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- \\
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d prof/pmc_write -o run -- \\
         python3 -m flecsale_amd.profiling run
-    python3 -m flecsale_amd.profiling summarize gpurun_out/pmc_fetch gpurun_out/pmc_write --workload muscl2d
+    python3 -m flecsale_amd.profiling summarize prof/pmc_fetch prof/pmc_write --workload muscl2d
 
 `run` first launches a plain device-to-device copy of a known size (512 MiB,
 larger than the 256 MiB Infinity Cache) so each counter can be calibrated
@@ -147,11 +147,11 @@ def summarize(fetch_db, write_db, workload):
         json.dump(out, f, indent=1)
     w = workload
     lines = [f"# rocprofv3 HBM traffic per launch: {w}", "", "Commands (repository root, one MI355X):", "", "```",
-             f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch_{w} -o run -- "
+             f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {fetch_db} -o run -- "
              f"python3 -m flecsale_amd.profiling run --workload {w} --steps 3",
-             f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write_{w} -o run -- "
+             f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {write_db} -o run -- "
              f"python3 -m flecsale_amd.profiling run --workload {w} --steps 3",
-             f"python3 -m flecsale_amd.profiling summarize gpurun_out/pmc_fetch_{w} gpurun_out/pmc_write_{w} "
+             f"python3 -m flecsale_amd.profiling summarize {fetch_db} {write_db} "
              f"--workload {w}", "```", "",
              "FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what they read for the runtime's "
              "512 MiB device-to-device copy (scale factors in the first line).", "",
