@@ -273,10 +273,11 @@ struct MaireFields {
   // 2D: the same as two planes of pairs, (rho a, p) and (v_x, v_y) [cap_c]
   // (the coordinates are DevMesh::xy)
   double2 *cs2[2] = {};
-  // 2D driver steps: the coordinates at the start of the step (x0 again, as
-  // pairs), saved by the step's first nodal solve; the predictor and
-  // corrector updates move the vertices from these and rebuild each cell's
-  // geometry from the same values
+  // 2D driver steps: the coordinates at the start of the step, as pairs,
+  // saved by the step's first nodal solve (x0 is written from them at the
+  // end of a batch of steps, k_maire_settle); the predictor and corrector
+  // updates move the vertices from these and rebuild each cell's geometry
+  // from the same values
   double2 *x0p = nullptr;
 };
 
