@@ -188,6 +188,7 @@ int sync_tiles(fsale_ctx *ctx) {
   ctx->h_ctiles.clear();
   ctx->h_vtiles.clear();
   ctx->h_btiles.clear();
+  ctx->h_bbtiles.clear();
   for (int p = 0; p < (int)ctx->probs.size(); ++p) {
     const Problem &P = ctx->probs[p];
     if (!P.live) continue;
@@ -200,6 +201,9 @@ int sync_tiles(fsale_ctx *ctx) {
       ctx->h_vtiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nv - i * kBlock), P.vt0 + i});
     for (int i = 0; i * kBlock < P.nb; ++i)
       ctx->h_btiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nb - i * kBlock), P.vt0 + i});
+    for (int i = 0; i * kBlock < P.nb - P.nb2; ++i)
+      ctx->h_bbtiles.push_back(
+          Tile{p, P.v0 + P.nb2 + i * kBlock, std::min(kBlock, P.nb - P.nb2 - i * kBlock), P.vt0 + i});
   }
   std::vector<int> plist;
   ctx->lds = ctx->lds_all = ctx->m.lnb != nullptr;
@@ -222,8 +226,9 @@ int sync_tiles(fsale_ctx *ctx) {
   if (!ctx->lds) ctx->lds_all = false;
   if (!ctx->s3) ctx->s3_all = false;
   if (plist.empty()) ctx->brick_all = false;
-  const int counts[4] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size(),
-                         (int)ctx->h_btiles.size()};
+  const int counts[5] = {(int)ctx->h_ctiles.size(), (int)ctx->h_vtiles.size(), (int)plist.size(),
+                         (int)ctx->h_btiles.size(), (int)ctx->h_bbtiles.size()};
+  ctx->n_bbtiles = counts[4];
   if (!plist.empty())
     FSALE_HIP(hipMemcpyAsync(ctx->d_plist, plist.data(), sizeof(int) * plist.size(), hipMemcpyHostToDevice,
                              ctx->stream));
@@ -249,6 +254,9 @@ int sync_tiles(fsale_ctx *ctx) {
                              ctx->stream));
     if (!ctx->h_btiles.empty())
       FSALE_HIP(hipMemcpyAsync(ctx->d_btiles, ctx->h_btiles.data(), sizeof(Tile) * ctx->h_btiles.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->h_bbtiles.empty())
+      FSALE_HIP(hipMemcpyAsync(ctx->d_bbtiles, ctx->h_bbtiles.data(), sizeof(Tile) * ctx->h_bbtiles.size(),
                                hipMemcpyHostToDevice, ctx->stream));
     ctx->n_bdirect = direct_grid(counts[3], ctx->ra_vt.capacity);
     build_direct(ctx->h_btiles, ctx->n_bdirect, db);
@@ -559,9 +567,10 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if (solver == FSALE_SOLVER_MAIRE) {
     if ((rc = dev_alloc_n(ctx, &ctx->d_vdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
     if ((rc = dev_alloc_n(ctx, &ctx->d_btiles, cap_vt))) return rc;
+    if ((rc = dev_alloc_n(ctx, &ctx->d_bbtiles, cap_vt))) return rc;
     if ((rc = dev_alloc_n(ctx, &ctx->d_bdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
   }
-  if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 4))) return rc;
+  if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 5))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_plist, max_problems))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->d_params, max_problems))) return rc;
   if ((rc = dev_alloc_n(ctx, &ctx->s, max_problems))) return rc;
@@ -574,7 +583,7 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   std::vector<ProbParams> zero(max_problems);
   std::memset(zero.data(), 0, sizeof(ProbParams) * max_problems);
   FSALE_HIP(hipMemcpy(ctx->d_params, zero.data(), sizeof(ProbParams) * max_problems, hipMemcpyHostToDevice));
-  const int counts[4] = {0, 0, 0, 0};
+  const int counts[5] = {0, 0, 0, 0, 0};
   FSALE_HIP(hipMemcpy(ctx->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice));
   ctx->probs.assign(max_problems, Problem{});
   if (solver == FSALE_SOLVER_MAIRE) return maire_alloc(ctx);
@@ -860,13 +869,17 @@ int place_mesh(fsale_ctx *ctx, const fsale_mesh_desc &mesh, int *out_prob, Stage
     int32_t *bvl = st.slot<int32_t>(nb > 0 ? nb : 1);
     if (!bvl) return FSALE_ERR_ARG;
     nb = 0;
-    for (int ns = 0; ns <= 3; ++ns)
+    int nb2 = 0;
+    for (int ns = 0; ns <= 3; ++ns) {
+      if (ns == 2) nb2 = nb;
       for (int sp = 0; sp < nv; ++sp) {
         const int vt = vt_popcount(vtags[vidx[sp]]);
         if (vbnd[sp] && vt == ns) bvl[nb++] = sp + v0;
       }
+    }
     if ((rc = st.put(m.bv + v0, bvl, nb))) return rc;
     ctx->probs[prob].nb = nb;
+    ctx->probs[prob].nb2 = nb2;
   }
   // the deferred rows read this function's locals (P, the row lambdas): fill
   // and send them before returning
@@ -1298,6 +1311,7 @@ int run_steps(fsale_ctx *ctx, int nsteps, int use_graph) {
   std::vector<double> key = {(double)ctx->solver, (double)chunk};
   for (int k = 0; k < K_COUNT; ++k) key.push_back((double)persistent_grid(ctx, k));
   key.push_back((double)ctx->pgrid);
+  key.push_back((double)ctx->n_bbtiles);
   key.push_back((double)ctx->lds + 2.0 * ctx->lds_all + 4.0 * ctx->s3 + 8.0 * ctx->s3_all + 16.0 * ctx->fp3 +
                 32.0 * ctx->brick_all);
   GraphCache &gc = ctx->graph;

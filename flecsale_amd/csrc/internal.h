@@ -337,6 +337,7 @@ struct Problem {
   int c0 = 0, nc = 0, f0 = 0, nf = 0, v0 = 0, nv = 0, n0 = 0, nn = 0;   // ranges
   int ct0 = 0, nct = 0, vt0 = 0, nvt = 0;                               // tile ranges
   int nb = 0;                                                           // boundary vertices (Maire)
+  int nb2 = 0;   // of these, the ones on fewer than two tagged faces (listed first)
   std::shared_ptr<const HostTopology> topo;
   ProbParams params{};
   bool faces_on_device = true;   // face connectivity and geometry uploaded
@@ -390,14 +391,20 @@ struct fsale_ctx {
   int n_cdirect = 0, n_vdirect = 0;
   fsale::Tile *d_btiles = nullptr, *d_bdirect = nullptr;    // boundary-vertex tiles (Maire)
   int n_bdirect = 0;
-  int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles, [2] problems, [3] boundary tiles
+  int *d_counts = nullptr;             // [0] cell tiles, [1] vertex tiles, [2] problems, [3] boundary tiles,
+                                       // [4] boundary tiles of vertices on two or more tagged faces
+  // 3D Maire: the boundary vertices on two or three tagged faces (each
+  // problem's tail of DevMesh::bv from nb2), the only ones whose symmetry
+  // solve can have two or three rows
+  fsale::Tile *d_bbtiles = nullptr;
+  int n_bbtiles = 0;
   int *d_plist = nullptr;              // problems with tiles in the lists
   int pgrid = 1;                       // blocks of the per-problem kernels
   hipStream_t upload = nullptr;        // admissions: uploads, geometry, initial state
   fsale::ProbParams *d_params = nullptr;
   fsale::DevScalars *s = nullptr;      // [max_problems] device
   fsale::DevScalars *h_s = nullptr;    // pinned host mirror
-  std::vector<fsale::Tile> h_ctiles, h_vtiles, h_btiles;
+  std::vector<fsale::Tile> h_ctiles, h_vtiles, h_btiles, h_bbtiles;
   bool tiles_dirty = true;
   bool lds = false;                    // every live problem has LDS staging plans (sync_tiles)
   bool lds_all = false;                // ... for every one of its tiles

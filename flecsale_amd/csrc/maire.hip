@@ -1457,7 +1457,16 @@ int launch_nodal_n(fsale_ctx *ctx) {
     const int gb = persistent_grid(ctx, K_MAIRE_NODAL_BND);
     hipLaunchKernelGGL((k_maire_nodal<true, 3>), dim3(gb), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, bb);
     FSALE_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_maire3_nodal_bnd_big, dim3(gb), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, bb);
+    // the solves with two or three symmetry rows: only the vertices on two
+    // or three tagged faces can have them, a short tail of each problem's
+    // list, one block per tile
+    Batch bg = bb;
+    bg.bt.tiles = ctx->d_bbtiles;
+    bg.bt.n = ctx->d_counts + 4;
+    bg.bt.direct = nullptr;
+    bg.bt.ndirect = 0;
+    const int gg = std::max(8, (ctx->n_bbtiles + 7) / 8 * 8);
+    hipLaunchKernelGGL(k_maire3_nodal_bnd_big, dim3(gg), dim3(kBlock), 0, ctx->stream, ctx->m, ctx->ma, bg);
     FSALE_HIP(hipGetLastError());
     return FSALE_OK;
   }
