@@ -201,9 +201,11 @@ int sync_tiles(fsale_ctx *ctx) {
       ctx->h_vtiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nv - i * kBlock), P.vt0 + i});
     for (int i = 0; i * kBlock < P.nb; ++i)
       ctx->h_btiles.push_back(Tile{p, P.v0 + i * kBlock, std::min(kBlock, P.nb - i * kBlock), P.vt0 + i});
-    for (int i = 0; i * kBlock < P.nb - P.nb2; ++i)
-      ctx->h_bbtiles.push_back(
-          Tile{p, P.v0 + P.nb2 + i * kBlock, std::min(kBlock, P.nb - P.nb2 - i * kBlock), P.vt0 + i});
+    // one pass of a block each (kBlock / kMaxCornersPerVertex vertices):
+    // their solves are long serial chains, so spread over as many blocks
+    constexpr int kBig = kBlock / kMaxCornersPerVertex;
+    for (int i = 0; i * kBig < P.nb - P.nb2; ++i)
+      ctx->h_bbtiles.push_back(Tile{p, P.v0 + P.nb2 + i * kBig, std::min(kBig, P.nb - P.nb2 - i * kBig), P.vt0});
   }
   std::vector<int> plist;
   ctx->lds = ctx->lds_all = ctx->m.lnb != nullptr;
@@ -567,7 +569,10 @@ int alloc_batch(fsale_ctx *ctx, int solver, int dim, int vpc, int max_problems, 
   if (solver == FSALE_SOLVER_MAIRE) {
     if ((rc = dev_alloc_n(ctx, &ctx->d_vdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
     if ((rc = dev_alloc_n(ctx, &ctx->d_btiles, cap_vt))) return rc;
-    if ((rc = dev_alloc_n(ctx, &ctx->d_bbtiles, cap_vt))) return rc;
+    // tiles of kBlock / kMaxCornersPerVertex boundary vertices (sync_tiles)
+    const size_t cap_bb = ((size_t)cap_v + 31) / 32 + (size_t)max_problems;
+    static_assert(kBlock / kMaxCornersPerVertex == 32, "one block pass of 3D vertices");
+    if ((rc = dev_alloc_n(ctx, &ctx->d_bbtiles, cap_bb))) return rc;
     if ((rc = dev_alloc_n(ctx, &ctx->d_bdirect, ((size_t)cap_vt + 7) / 8 * 8))) return rc;
   }
   if ((rc = dev_alloc_n(ctx, &ctx->d_counts, 5))) return rc;

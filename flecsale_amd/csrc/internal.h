@@ -395,7 +395,7 @@ struct fsale_ctx {
                                        // [4] boundary tiles of vertices on two or more tagged faces
   // 3D Maire: the boundary vertices on two or three tagged faces (each
   // problem's tail of DevMesh::bv from nb2), the only ones whose symmetry
-  // solve can have two or three rows
+  // solve can have two or three rows, in tiles of one block pass each
   fsale::Tile *d_bbtiles = nullptr;
   int n_bbtiles = 0;
   int *d_plist = nullptr;              // problems with tiles in the lists
