@@ -589,11 +589,13 @@ k_eul_grad_lds(DevMesh m, EulerFields u, Batch b) {
   constexpr int NP = Rec<2>::S / 2;
   __shared__ double2 sh[NP * kLdsW];
   for_each_tile(b.ct, [&](const Tile &tile) {
+    // the tile's plan and the problem's order are read ahead of the FIN
+    // checks, so that the loads overlap (discarded when the tile is skipped)
+    const int h = m.halo_cnt[tile.slot];
+    const int first_order = b.pp[tile.prob].first_order;
     // FIN: as k_eul_grad
     if (FIN && tile.first == b.pp[tile.prob].cell_first) finalize_problem<kBlock>(b, u.partials, 1, tile.prob);
     if (FIN && !b.s[tile.prob].packed) return;   // uniform across the block
-    const int first_order = b.pp[tile.prob].first_order;
-    const int h = m.halo_cnt[tile.slot];
     if (!ALL && h < 0) {
       if (!FIN && !wave_live(b.s, tile.prob, 0)) return;
       if ((int)threadIdx.x < tile.count) grad_cell<2, VPC, 1>(m, u, tile.first + threadIdx.x, first_order);
