@@ -647,11 +647,14 @@ k_eul_grad_brick(DevMesh m, EulerFields u, Batch b) {
   __shared__ double2 sh[NP * kB4];
   for_each_tile(b.ct, [&](const Tile &tile) {
     const ProbParams &P = b.pp[tile.prob];
+    // the brick and the grid's shape are read ahead of the checks below, so
+    // that the loads overlap (discarded when the tile is skipped)
+    const int base = m.brick[tile.slot];
+    const int nx = P.s_nx, np = P.s_np, ncells = P.cell_count;
     if (FIN && tile.first == P.cell_first) finalize_problem<kBlock>(b, u.partials, 1, tile.prob);
     if (FIN && !b.s[tile.prob].packed) return;
     if (!FIN && !wave_live(b.s, tile.prob, 0)) return;
-    const int nx = P.s_nx, np = P.s_np, ny = np / nx, nz = P.cell_count / np;
-    const int base = m.brick[tile.slot];
+    const int ny = np / nx, nz = ncells / np;
     const int lb = base - P.cell_first;
     const int z0 = lb / np, y0 = (lb - z0 * np) / nx, x0 = lb - z0 * np - y0 * nx;
     const bool interior = x0 > 0 && x0 + kBrickX < nx && y0 > 0 && y0 + kBrickY < ny && z0 > 0 && z0 + kBrickZ < nz;
