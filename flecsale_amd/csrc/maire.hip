@@ -709,9 +709,11 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
   if constexpr (N == 3) {
     nodal3_tile<BOUNDARY ? 1 : 0>(m, u, b, tile);
   } else {
-    if (!step_live<PEND>(b, tile.prob)) return;
-    if ((int)threadIdx.x >= tile.count) return;
-    int vt = tile.first + threadIdx.x;
+    if (tile.count <= 0) return;
+    // the vertex, its first corners and its boundary flag are read ahead of
+    // the step check, so that the loads overlap (discarded when the tile
+    // does not step; lanes past the tile read its last vertex)
+    int vt = tile.first + min((int)threadIdx.x, tile.count - 1);
     // interior: a boundary vertex (the boundary path's) computes like the
     // others with a corner count of 0, so that nothing it stores and no
     // gather waits on the boundary flag (a branch on it would)
@@ -724,6 +726,8 @@ __device__ inline void nodal_tile(const DevMesh &m, const MaireFields &u, const 
       for (int q = 0; q < kNodalKeep; ++q) cn0[q] = m.vck[(size_t)q * m.cap_v + vt];
       own = !m.vbnd[vt];
     }
+    if (!step_live<PEND>(b, tile.prob)) return;
+    if ((int)threadIdx.x >= tile.count) return;
     if (SAVE && own) u.x0p[vt] = m.xy[vt];   // u.x0 follows in k_maire_settle
     const int cnt = m.vc_cnt[vt] & -(int)own;
     constexpr int KEEP = BOUNDARY ? kNodalKeepBnd : kNodalKeep;
