@@ -874,7 +874,7 @@ k_maire_nodal_both(DevMesh m, MaireFields u, Batch b, int nb) {
 // evaluate_residual of cell c (compute_update lagrange_eqns.h:374-389): its
 // corners' normals and forces against its vertices' velocities, in local
 // vertex order; vt / uv return the vertex ids and velocities it read.
-template <int N, int V>
+template <int N, int V, bool PRE = false>
 __device__ __forceinline__ void cell_residual(const DevMesh &m, const MaireFields &u, int c, double r[N + 2],
                                               int vt[V], double uv[V][N]) {
 #pragma unroll
@@ -882,7 +882,7 @@ __device__ __forceinline__ void cell_residual(const DevMesh &m, const MaireField
 #pragma unroll
   for (int k = 0; k < V; ++k) {
     const size_t ci = (size_t)k * m.cap_c + c;
-    vt[k] = m.cv[ci];
+    if (!PRE) vt[k] = m.cv[ci];   // PRE: the caller read them
     double F[N], n[N];
 #pragma unroll
     for (int d = 0; d < N; ++d) {
@@ -910,6 +910,15 @@ __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireField
   constexpr int V = N == 2 ? 4 : 8;
   static_assert(!TILE || DT, "tile partials are time-step partials");
   for_each_tile(b.ct, [&](const Tile &tile) {
+    // the cell's vertex ids are read ahead of the step check, so that the
+    // loads overlap (discarded when the tile does not step; lanes past the
+    // tile read its last cell's)
+    int vt[V];
+    {
+      const int cp = tile.first + max(min((int)threadIdx.x, tile.count - 1), 0);
+#pragma unroll
+      for (int k = 0; k < V; ++k) vt[k] = m.cv[(size_t)k * m.cap_c + cp];
+    }
     if (TILE && threadIdx.x == 0 && tile.first == b.pp[tile.prob].cell_first) {
       DevScalars &S = b.s[tile.prob];
       const ProbParams &P = b.pp[tile.prob];
@@ -933,8 +942,7 @@ __global__ void __launch_bounds__(kBlock) k_maire_residual(DevMesh m, MaireField
     if ((int)threadIdx.x < tile.count) {
     const int c = tile.first + threadIdx.x;
     double r[N + 2], uv[V][N];
-    int vt[V];
-    cell_residual<N, V>(m, u, c, r, vt, uv);
+    cell_residual<N, V, true>(m, u, c, r, vt, uv);
 #pragma unroll
     for (int i = 0; i < N + 2; ++i) u.dudt[i][c] = r[i];
     if (DT) {
